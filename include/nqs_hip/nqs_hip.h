@@ -1,0 +1,131 @@
+/*
+ * nqs_hip/nqs_hip.h — engine extensions of libnqs_hip.so that have no
+ * counterpart in the reference API (include/nqs has no chain count, no
+ * move kind, no periodic square lattice and no precision switch; see
+ * include/nqs/nqs_config.h:82-120).
+ *
+ * A sampler created with nqs_hip_sampler_create_ex carries these options;
+ * every reference entry point that takes that sampler (nqs_sampler_*,
+ * nqs_sr_step*, nqs_sr_run*) honours them.  nqs_sampler_create uses the
+ * defaults, which reproduce the reference exactly (one chain, single-site
+ * flips, uniform random start, open square lattices, FP64), except that the
+ * environment variable NQS_HIP_CHAINS=<n> selects n chains.
+ *
+ * Multi-chain semantics (also restated by the CPU oracle):
+ *   chain 0 uses the reference seed (rng_seed, or 0x9E3779B97F4A7C15 when
+ *   zero); chain c > 0 uses splitmix64(seed0 ^ (c * 0xD1B54A32D192ED03)),
+ *   replaced by 1 if zero.  nqs_sampler_batch(s, n, out) runs rounds; in
+ *   each round every chain makes max(num_decorrelate, 1) moves, then the
+ *   configuration of chain c is stored as sample r*num_chains + c when that
+ *   index is below n.  thermalize makes num_thermalize moves per chain.
+ *   The acceptance ratio counts all chains.
+ */
+#ifndef NQS_HIP_H
+#define NQS_HIP_H
+
+#include "../nqs/nqs_config.h"
+#include "../nqs/nqs_ansatz.h"
+#include "../nqs/nqs_sampler.h"
+#include "../nqs/nqs_optimizer.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    NQS_HIP_MOVE_FLIP     = 0, /* reference single-site flip                  */
+    NQS_HIP_MOVE_EXCHANGE = 1, /* swap a random up site with a random down    */
+    NQS_HIP_MOVE_LEARNED  = 2  /* autoregressive proposal, MH-corrected       */
+} nqs_hip_move_t;
+
+typedef enum {
+    NQS_HIP_FP64 = 0,          /* all arithmetic FP64 (parity mode)           */
+    NQS_HIP_FP32 = 1           /* amplitudes, E_loc, moves and O in FP32;
+                                  reductions, Gram, solve and update FP64     */
+} nqs_hip_precision_t;
+
+typedef enum {
+    NQS_HIP_GRAM_DIRECT   = 0, /* T = O_c O_c^T over the materialised O       */
+    NQS_HIP_GRAM_FACTORED = 1  /* RBM / complex RBM: T from (S S^T, Th Th^T)  */
+} nqs_hip_gram_t;
+
+typedef struct {
+    int num_chains;      /* >= 1                                               */
+    int move;            /* nqs_hip_move_t                                     */
+    int sz_zero_start;   /* 1: each chain starts at a random Sz=0 state        */
+    int periodic;        /* 1: periodic square lattice for TFIM, Heisenberg,
+                            XXZ, J1-J2 (dimensions of length >= 3 wrap)        */
+    int precision;       /* nqs_hip_precision_t                                */
+    int gram;            /* nqs_hip_gram_t (MinSR)                             */
+    /* learned proposal (move == NQS_HIP_MOVE_LEARNED) */
+    int ar_hidden;       /* hidden width of the masked MLP (default 64)        */
+    int ar_train_every;  /* retrain every k batches (default 10)               */
+    int ar_train_window; /* samples kept for training (default 16384)          */
+    int ar_epochs;       /* passes over the window per retrain (default 4)     */
+    int ar_batch;        /* minibatch size (default 256)                       */
+    double ar_lr;        /* Adam learning rate (default 1e-3)                  */
+    double ar_mix;       /* probability of a learned move vs exchange (0.5)    */
+} nqs_hip_options_t;
+
+/* Defaults: one chain, flips, random start, open boundaries, FP64, direct
+ * Gram, learned-proposal defaults as listed above. */
+nqs_hip_options_t nqs_hip_options_defaults(void);
+
+/* nqs_sampler_create with options.  NULL on bad args or unsupported
+ * combinations (message on stderr). */
+nqs_sampler_t *nqs_hip_sampler_create_ex(int num_sites, const nqs_config_t *cfg,
+                                         const nqs_hip_options_t *opt,
+                                         nqs_log_amp_fn_t log_amp,
+                                         void *log_amp_user);
+
+/* Copy of the options a sampler carries (0 on success). */
+int nqs_hip_sampler_get_options(const nqs_sampler_t *s, nqs_hip_options_t *out);
+
+/* Per-chain energy series of the last SR step: E_loc of sample r*C + c is
+ * element [c][r] of out (num_chains x rounds, row-major).  Returns the
+ * number of rounds, or -1.  Used for tau_int / ESS. */
+int nqs_hip_sampler_last_energies(const nqs_sampler_t *s, double *out, int capacity);
+
+/* Number of samples per chain in the last SR step (rounds). */
+int nqs_hip_sampler_rounds(const nqs_sampler_t *s);
+
+/* Local energies with options (periodic boundaries; FP32 arithmetic). */
+int nqs_hip_local_energy_batch_ex(const nqs_config_t *cfg, int size_x, int size_y,
+                                  const nqs_hip_options_t *opt,
+                                  const int *spins_batch, int batch_size,
+                                  nqs_log_amp_fn_t log_amp, void *log_amp_user,
+                                  double *out_re, double *out_im);
+
+/* Batched amplitudes: log|psi| and arg psi of B configurations. */
+int nqs_hip_log_amp_batch(nqs_ansatz_t *a, const int *spins_batch, int batch_size,
+                          double *out_log_abs, double *out_arg);
+
+/* Batched O matrix, row-major [B][P]: mode 0 real-projected (out_re only),
+ * mode 1 holomorphic (out_re, out_im). */
+int nqs_hip_logpsi_gradient_batch(nqs_ansatz_t *a, const int *spins_batch,
+                                  int batch_size, int mode,
+                                  double *out_re, double *out_im);
+
+/* Integrated autocorrelation time of a series (Sokal window, c = 5). */
+double nqs_hip_tau_int(const double *x, int n);
+
+/* Stage timings of the last SR step on the sampler's stream, milliseconds:
+ * [0] sampling, [1] forward, [2] local energy, [3] O / reductions,
+ * [4] solve, [5] update, [6] total.  Returns 0 when available. */
+int nqs_hip_last_stage_ms(const nqs_sampler_t *s, double out[7]);
+
+/* Diagnostics of the device path: how many kernel launches the library has
+ * made, and how many host-callback adaptor calls it has made (a callback
+ * that is not nqs_ansatz_log_amp or nqs_marshall_log_amp over it runs on
+ * the host). */
+long long nqs_hip_device_launches(void);
+long long nqs_hip_host_callback_calls(void);
+
+/* Library version string. */
+const char *nqs_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NQS_HIP_H */

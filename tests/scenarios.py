@@ -1,0 +1,297 @@
+"""Parity scenarios, runnable against any library exporting the reference
+API under a prefix: the compiled reference (``nqs_``, only where it was
+built; writes tests/golden), the CPU oracle (``orc_``) and libnqs_hip.so
+(``nqs_``).  Each scenario is data; ``run(lib, prefix, sc)`` returns a dict
+of numpy arrays with the same keys whichever library ran it.
+
+Large gradient rows are compared through fixed random sketches
+(O @ V, V ~ N(0,1), seed per scenario) so fixtures stay small.
+"""
+import ctypes
+
+import numpy as np
+
+from spin_based_neural_network_amd.abi import (MarshallWrapper, SrInfo,
+                                               config_defaults, fnptr, ptr)
+
+RBM, CRBM, MF = 5, 6, 0
+TFIM, HEIS, J1J2, XXZ, KAGOME = 0, 1, 2, 4, 6
+N_SKETCH = 4
+
+
+def spins_random(n, N, seed, sz0=False):
+    rng = np.random.default_rng(seed)
+    if sz0:
+        base = np.array([1] * (N // 2) + [-1] * (N - N // 2), dtype=np.int32)
+        out = np.stack([rng.permutation(base) for _ in range(n)])
+    else:
+        out = rng.choice(np.array([-1, 1], dtype=np.int32), size=(n, N))
+    return np.ascontiguousarray(out.astype(np.int32))
+
+
+def sketch_matrix(P, seed):
+    return np.random.default_rng(seed + 7919).standard_normal((P, N_SKETCH))
+
+
+# ---------------------------------------------------------------------------
+INIT = [
+    dict(name="mf_n9", cfg=dict(), N=9),
+    dict(name="rbm_n16", cfg=dict(ansatz=RBM), N=16),
+    dict(name="rbm_n8_m4_seed0", cfg=dict(ansatz=RBM, rbm_hidden_units=4, rng_seed=0), N=8),
+    dict(name="rbm_n10_m7_s005", cfg=dict(ansatz=RBM, rbm_hidden_units=7, rbm_init_scale=0.05, rng_seed=11), N=10),
+    dict(name="crbm_n24_m96", cfg=dict(ansatz=CRBM, rbm_hidden_units=96), N=24),
+    dict(name="crbm_n36_m144", cfg=dict(ansatz=CRBM, rbm_hidden_units=144), N=36),
+    dict(name="rbm_n100_m800", cfg=dict(ansatz=RBM, rbm_hidden_units=800), N=100, sketch_only=True),
+]
+
+AMP = [
+    dict(name="mf_n9", cfg=dict(), N=9, n=16),
+    dict(name="rbm_n16", cfg=dict(ansatz=RBM, rbm_init_scale=0.3), N=16, n=16),
+    dict(name="crbm_n24", cfg=dict(ansatz=CRBM, rbm_hidden_units=96, rbm_init_scale=0.3), N=24, n=12, sz0=True),
+    dict(name="crbm_n36", cfg=dict(ansatz=CRBM, rbm_hidden_units=144, rbm_init_scale=0.2), N=36, n=8, sz0=True),
+    dict(name="rbm_n100_m800", cfg=dict(ansatz=RBM, rbm_hidden_units=800, rbm_init_scale=0.2), N=100, n=4, sz0=True),
+]
+
+ELOC = [
+    dict(name="tfim_chain16_obc_rbm", cfg=dict(ansatz=RBM, rbm_init_scale=0.3, hamiltonian=TFIM), Lx=16, Ly=1, n=16),
+    dict(name="tfim_4x4_obc_crbm", cfg=dict(ansatz=CRBM, rbm_hidden_units=16, rbm_init_scale=0.3, hamiltonian=TFIM, transverse_field=0.7), Lx=4, Ly=4, n=12),
+    dict(name="tfim_3x3_mf", cfg=dict(hamiltonian=TFIM, transverse_field=0.5), Lx=3, Ly=3, n=8),
+    dict(name="heis_6x6_obc_marshall_crbm", cfg=dict(ansatz=CRBM, rbm_hidden_units=144, rbm_init_scale=0.2, hamiltonian=HEIS), Lx=6, Ly=6, n=6, sz0=True, marshall=True),
+    dict(name="heis_4x4_marshall_rbm", cfg=dict(ansatz=RBM, rbm_hidden_units=16, rbm_init_scale=0.3, hamiltonian=HEIS), Lx=4, Ly=4, n=12, sz0=True, marshall=True),
+    dict(name="xxz_4x3_rbm", cfg=dict(ansatz=RBM, rbm_hidden_units=12, rbm_init_scale=0.3, hamiltonian=XXZ, j_coupling=0.8, j_z_coupling=0.4), Lx=4, Ly=3, n=12),
+    dict(name="j1j2_10x10_obc_rbm_m800", cfg=dict(ansatz=RBM, rbm_hidden_units=800, rbm_init_scale=0.2, hamiltonian=J1J2, j2_coupling=0.5), Lx=10, Ly=10, n=3, sz0=True),
+    dict(name="j1j2_4x4_crbm", cfg=dict(ansatz=CRBM, rbm_hidden_units=16, rbm_init_scale=0.3, hamiltonian=J1J2, j2_coupling=0.5), Lx=4, Ly=4, n=12, sz0=True),
+    dict(name="kagome_2x2_pbc_crbm", cfg=dict(ansatz=CRBM, rbm_hidden_units=24, rbm_init_scale=0.3, hamiltonian=KAGOME), Lx=2, Ly=2, n=16),
+    dict(name="kagome_2x4_pbc_crbm_m96", cfg=dict(ansatz=CRBM, rbm_hidden_units=96, rbm_init_scale=0.3, hamiltonian=KAGOME), Lx=2, Ly=4, n=8, sz0=True),
+    dict(name="kagome_2x2_obc_rbm", cfg=dict(ansatz=RBM, rbm_hidden_units=12, rbm_init_scale=0.3, hamiltonian=KAGOME, kagome_pbc=0), Lx=2, Ly=2, n=8),
+    dict(name="kagome_1x1_pbc_mf", cfg=dict(hamiltonian=KAGOME), Lx=1, Ly=1, n=8),
+]
+
+SAMPLER = [
+    dict(name="tfim_chain16_rbm", cfg=dict(ansatz=RBM, rbm_init_scale=0.5, num_thermalize=32, num_decorrelate=4), N=16, n=48),
+    dict(name="kagome_2x2_crbm", cfg=dict(ansatz=CRBM, rbm_hidden_units=24, rbm_init_scale=0.5, num_thermalize=16, num_decorrelate=2, rng_seed=99), N=12, n=32),
+    dict(name="heis_4x4_marshall_rbm", cfg=dict(ansatz=RBM, rbm_hidden_units=16, rbm_init_scale=0.5, num_thermalize=16, num_decorrelate=3, rng_seed=5), N=16, n=32, marshall=(4, 4)),
+    dict(name="mf_n10_seed3", cfg=dict(num_thermalize=128, rng_seed=3), N=10, n=16),
+    dict(name="rbm_n8_seed0", cfg=dict(ansatz=RBM, rbm_init_scale=0.5, rng_seed=0, num_thermalize=8, num_decorrelate=0), N=8, n=24),
+]
+
+SR = [
+    dict(name="cg_tfim_2x2_rbm", method="cg", cfg=dict(ansatz=RBM, num_samples=64, num_thermalize=64, num_decorrelate=1, sr_diag_shift=1e-2, sr_cg_max_iters=10), Lx=2, Ly=2, steps=3),
+    dict(name="cg_tfim_chain16_rbm", method="cg", cfg=dict(ansatz=RBM, num_samples=256, num_thermalize=64, num_decorrelate=2), Lx=16, Ly=1, steps=2),
+    dict(name="cg_tfim_3x3_mf", method="cg", cfg=dict(num_samples=64, num_thermalize=32, num_decorrelate=1, transverse_field=0.5), Lx=3, Ly=3, steps=3),
+    dict(name="cg_heis_2x2_marshall_rbm", method="cg", cfg=dict(ansatz=RBM, hamiltonian=HEIS, num_samples=64, num_thermalize=32, num_decorrelate=2, sr_diag_shift=1e-2, rbm_init_scale=0.3, learning_rate=0.05), Lx=2, Ly=2, steps=3, marshall=True),
+    dict(name="holo_kagome_2x2_crbm", method="holo", cfg=dict(ansatz=CRBM, rbm_hidden_units=12, hamiltonian=KAGOME, num_samples=128, num_thermalize=32, num_decorrelate=2, sr_diag_shift=1e-2, rbm_init_scale=0.3, learning_rate=0.02), Lx=2, Ly=2, steps=2),
+    dict(name="holo_tfim_2x2_rbm", method="holo", cfg=dict(ansatz=RBM, num_samples=64, num_thermalize=32, num_decorrelate=1, sr_diag_shift=1e-2), Lx=2, Ly=2, steps=2),
+    dict(name="minsr_tfim_4x1_rbm", method="minsr", cfg=dict(ansatz=RBM, num_samples=64, num_thermalize=32, num_decorrelate=2, rbm_init_scale=0.3), Lx=4, Ly=1, steps=2),
+    dict(name="minsr_heis_2x2_marshall_crbm", method="minsr", cfg=dict(ansatz=CRBM, rbm_hidden_units=8, hamiltonian=HEIS, num_samples=64, num_thermalize=32, num_decorrelate=2, rbm_init_scale=0.3, sr_diag_shift=1e-3, learning_rate=0.05), Lx=2, Ly=2, steps=2, marshall=True),
+    dict(name="minsr_kagome_2x2_crbm", method="minsr", cfg=dict(ansatz=CRBM, rbm_hidden_units=12, hamiltonian=KAGOME, num_samples=128, num_thermalize=32, num_decorrelate=2, rbm_init_scale=0.3, sr_diag_shift=1e-4), Lx=2, Ly=2, steps=2),
+    dict(name="minsr_retry_tfim_4x1_rbm", method="minsr", cfg=dict(ansatz=RBM, rbm_hidden_units=4, num_samples=64, num_thermalize=32, num_decorrelate=2, rbm_init_scale=0.3, sr_diag_shift=0.0), Lx=4, Ly=1, steps=1),
+]
+
+RUN = [
+    dict(name="run_cg_tfim_2x2_rbm", method="cg", cfg=dict(ansatz=RBM, num_samples=64, num_thermalize=64, num_decorrelate=1, num_iterations=5, sr_diag_shift=1e-2, learning_rate=0.05), Lx=2, Ly=2),
+    dict(name="run_minsr_tfim_2x2_rbm", method="minsr", cfg=dict(ansatz=RBM, num_samples=64, num_thermalize=64, num_decorrelate=1, num_iterations=5, learning_rate=0.05), Lx=2, Ly=2),
+    dict(name="run_holo_kagome_2x2_crbm", method="holo", cfg=dict(ansatz=CRBM, rbm_hidden_units=12, hamiltonian=KAGOME, num_samples=64, num_thermalize=32, num_decorrelate=2, num_iterations=4, sr_diag_shift=1e-2, learning_rate=0.02), Lx=2, Ly=2),
+]
+
+
+def _cfg(d):
+    return config_defaults(**d)
+
+
+def _n_sites(cfg, Lx, Ly):
+    return 3 * Lx * Ly if cfg.hamiltonian == KAGOME else Lx * Ly
+
+
+class Runner:
+    """Calls the reference API of ``lib`` under ``prefix``."""
+
+    def __init__(self, lib, prefix):
+        self.lib = lib
+        self.p = prefix
+        self.f = lambda name: getattr(lib, prefix + name)
+        self.amp_fn = fnptr(lib, prefix + "ansatz_log_amp")
+        self.marshall_fn = fnptr(lib, prefix + "marshall_log_amp")
+        self._keep = []
+
+    def ansatz(self, cfg, N):
+        h = self.f("ansatz_create")(ctypes.byref(cfg), N)
+        assert h, "ansatz_create returned NULL"
+        return h
+
+    def params(self, h):
+        P = self.f("ansatz_num_params")(h)
+        raw = self.f("ansatz_params_raw")(h)
+        return np.ctypeslib.as_array(ctypes.cast(raw, ctypes.POINTER(ctypes.c_double)), (P,)).copy()
+
+    def callback(self, h, marshall=None):
+        """(fn, user) for the ansatz, optionally Marshall-wrapped."""
+        if not marshall:
+            return self.amp_fn, h
+        w = MarshallWrapper(self.amp_fn, h, marshall[0], marshall[1])
+        self._keep.append(w)
+        return self.marshall_fn, ctypes.addressof(w)
+
+    def log_amp(self, fn, user, spins):
+        B, N = spins.shape
+        la = np.zeros(B)
+        ar = np.zeros(B)
+        cf = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                              ctypes.c_void_p, ctypes.c_void_p)(fn)
+        for b in range(B):
+            row = np.ascontiguousarray(spins[b])
+            cf(ptr(row), N, user, ptr(la[b:b + 1]), ptr(ar[b:b + 1]))
+        return la, ar
+
+
+def run_init(r, sc):
+    cfg = _cfg(sc["cfg"])
+    h = r.ansatz(cfg, sc["N"])
+    p = r.params(h)
+    r.f("ansatz_free")(h)
+    out = {"P": np.array([p.size]), "sum": np.array([p.sum()]),
+           "sumsq": np.array([(p * p).sum()]),
+           "sketch": p @ np.random.default_rng(1).standard_normal(p.size)}
+    if sc.get("sketch_only"):
+        out["strided"] = p[::97].copy()
+    else:
+        out["params"] = p
+    return out
+
+
+def run_amp(r, sc):
+    cfg = _cfg(sc["cfg"])
+    N = sc["N"]
+    h = r.ansatz(cfg, N)
+    s = spins_random(sc["n"], N, 1000 + N, sc.get("sz0", False))
+    la, ar = r.log_amp(r.amp_fn, h, s)
+    P = r.f("ansatz_num_params")(h)
+    V = sketch_matrix(P, N)
+    g = np.zeros(P)
+    gr = np.zeros(P)
+    gi = np.zeros(P)
+    sk, skr, ski = [], [], []
+    first = {}
+    for b in range(s.shape[0]):
+        row = np.ascontiguousarray(s[b])
+        assert r.f("ansatz_logpsi_gradient")(h, ptr(row), N, ptr(g)) == 0
+        assert r.f("ansatz_logpsi_gradient_complex")(h, ptr(row), N, ptr(gr), ptr(gi)) == 0
+        sk.append(g @ V)
+        skr.append(gr @ V)
+        ski.append(gi @ V)
+        if b == 0 and P <= 12000:
+            first = {"grad0": g.copy(), "grad0_re": gr.copy(), "grad0_im": gi.copy()}
+    r.f("ansatz_free")(h)
+    out = {"spins": s, "log_abs": la, "arg": ar, "grad_sketch": np.array(sk),
+           "gradc_re_sketch": np.array(skr), "gradc_im_sketch": np.array(ski)}
+    out.update(first)
+    return out
+
+
+def run_eloc(r, sc):
+    cfg = _cfg(sc["cfg"])
+    Lx, Ly = sc["Lx"], sc["Ly"]
+    N = _n_sites(cfg, Lx, Ly)
+    h = r.ansatz(cfg, N)
+    fn, user = r.callback(h, (Lx, Ly) if sc.get("marshall") else None)
+    s = spins_random(sc["n"], N, 2000 + N, sc.get("sz0", False))
+    B = s.shape[0]
+    e = np.zeros(B)
+    er = np.zeros(B)
+    ei = np.zeros(B)
+    r.f("local_energy_batch")(ctypes.byref(cfg), Lx, Ly, ptr(s), B, fn, user, ptr(e))
+    r.f("local_energy_batch_complex")(ctypes.byref(cfg), Lx, Ly, ptr(s), B, fn, user, ptr(er), ptr(ei))
+    e0 = r.f("local_energy")(ctypes.byref(cfg), Lx, Ly, ptr(np.ascontiguousarray(s[0])), fn, user)
+    r.f("ansatz_free")(h)
+    return {"spins": s, "e": e, "e_re": er, "e_im": ei, "e_single": np.array([e0])}
+
+
+def run_sampler(r, sc):
+    cfg = _cfg(sc["cfg"])
+    N = sc["N"]
+    h = r.ansatz(cfg, N)
+    fn, user = r.callback(h, sc.get("marshall"))
+    smp = r.f("sampler_create")(N, ctypes.byref(cfg), fn, user)
+    assert smp
+    r.f("sampler_thermalize")(smp)
+    acc_therm = r.f("sampler_acceptance_ratio")(smp)
+    n = sc["n"]
+    out = np.zeros((n, N), dtype=np.int32)
+    assert r.f("sampler_batch")(smp, n, ptr(out)) == 0
+    nxt = r.f("sampler_next")(smp)
+    nxt = np.ctypeslib.as_array(ctypes.cast(nxt, ctypes.POINTER(ctypes.c_int)), (N,)).copy()
+    cur = r.f("sampler_current")(smp)
+    cur = np.ctypeslib.as_array(ctypes.cast(cur, ctypes.POINTER(ctypes.c_int)), (N,)).copy()
+    acc = r.f("sampler_acceptance_ratio")(smp)
+    ns = r.f("sampler_num_sites")(smp)
+    r.f("sampler_free")(smp)
+    r.f("ansatz_free")(h)
+    return {"samples": out, "next": nxt, "current": cur,
+            "acceptance": np.array([acc_therm, acc]), "num_sites": np.array([ns])}
+
+
+def _step_fn(r, method):
+    return {"cg": "sr_step_custom_full", "holo": "sr_step_holomorphic_full",
+            "minsr": "sr_step_minsr_full"}[method]
+
+
+def run_sr(r, sc):
+    cfg = _cfg(sc["cfg"])
+    Lx, Ly = sc["Lx"], sc["Ly"]
+    N = _n_sites(cfg, Lx, Ly)
+    h = r.ansatz(cfg, N)
+    fn, user = r.callback(h, (Lx, Ly) if sc.get("marshall") else None)
+    smp = r.f("sampler_create")(N, ctypes.byref(cfg), fn, user)
+    r.f("sampler_thermalize")(smp)
+    params = [r.params(h)]
+    infos = []
+    step = r.f(_step_fn(r, sc["method"]))
+    for _ in range(sc["steps"]):
+        info = SrInfo()
+        rc = step(ctypes.byref(cfg), Lx, Ly, h, smp, fn, user, None, None, ctypes.byref(info))
+        assert rc == 0
+        params.append(r.params(h))
+        infos.append([info.mean_energy, info.variance_energy, info.update_norm,
+                      info.acceptance_ratio, info.cg_iterations, info.converged])
+    r.f("sampler_free")(smp)
+    r.f("ansatz_free")(h)
+    return {"params": np.array(params), "info": np.array(infos)}
+
+
+def run_run(r, sc):
+    cfg = _cfg(sc["cfg"])
+    Lx, Ly = sc["Lx"], sc["Ly"]
+    N = _n_sites(cfg, Lx, Ly)
+    h = r.ansatz(cfg, N)
+    fn, user = r.callback(h, None)
+    smp = r.f("sampler_create")(N, ctypes.byref(cfg), fn, user)
+    tr = np.zeros(cfg.num_iterations)
+    m = sc["method"]
+    if m == "cg":
+        rc = r.f("sr_run")(ctypes.byref(cfg), Lx, Ly, h, smp, ptr(tr))
+    elif m == "minsr":
+        rc = r.f("sr_run_minsr")(ctypes.byref(cfg), Lx, Ly, h, smp, None, None, ptr(tr))
+    else:
+        rc = r.f("sr_run_holomorphic")(ctypes.byref(cfg), Lx, Ly, h, smp, None, None, ptr(tr))
+    assert rc == 0
+    p = r.params(h)
+    r.f("sampler_free")(smp)
+    r.f("ansatz_free")(h)
+    return {"trace": tr, "params": p}
+
+
+KINDS = {"init": (INIT, run_init), "amp": (AMP, run_amp), "eloc": (ELOC, run_eloc),
+         "sampler": (SAMPLER, run_sampler), "sr": (SR, run_sr), "run": (RUN, run_run)}
+
+
+def run_all(lib, prefix, kinds=None, names=None):
+    r = Runner(lib, prefix)
+    res = {}
+    for kind, (cases, fn) in KINDS.items():
+        if kinds and kind not in kinds:
+            continue
+        for sc in cases:
+            if names and sc["name"] not in names:
+                continue
+            res[f"{kind}/{sc['name']}"] = fn(r, sc)
+    return res

@@ -1,0 +1,356 @@
+"""Pins the CPU oracle (oracle/nqs_oracle.c) before it is trusted.
+
+1. Golden vectors from the compiled reference (tests/golden/golden.npz):
+   init stream, amplitudes, gradients, local energies, sample sequences,
+   SR / holomorphic SR / MinSR updates and traces, Lanczos anchors.
+2. Checks that do not come from the reference: closed forms (TFIM free
+   fermions), the reference tests' uniform / linear-ansatz known answers,
+   finite-difference gradients, <E_loc>_{|psi|^2} == <psi|H|psi> by
+   enumeration, MinSR == CG == dense (S + eps I)^-1 F, exchange-move Sz
+   conservation and the exact stationary distribution, tau_int on AR(1).
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import scenarios
+from conftest import golden_case
+from spin_based_neural_network_amd.abi import (HipOptions, config_defaults,
+                                               fnptr, options_defaults, ptr)
+
+RTOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def orc_results(orc):
+    return scenarios.run_all(orc, "orc_")
+
+
+def _close(a, b, tol=RTOL, scale=None):
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    s = scale if scale is not None else max(1.0, float(np.max(np.abs(b))) if b.size else 1.0)
+    err = float(np.max(np.abs(a - b))) if a.size else 0.0
+    assert err <= tol * s, f"max err {err:.3e} > {tol * s:.3e}"
+
+
+@pytest.mark.parametrize("sc", scenarios.INIT, ids=lambda s: s["name"])
+def test_init_matches_reference_bit_exact(orc_results, golden, sc):
+    got = orc_results[f"init/{sc['name']}"]
+    ref = golden_case(golden, f"init/{sc['name']}")
+    for k in ref:
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+
+
+@pytest.mark.parametrize("sc", scenarios.AMP, ids=lambda s: s["name"])
+def test_amplitudes_and_gradients(orc_results, golden, sc):
+    got = orc_results[f"amp/{sc['name']}"]
+    ref = golden_case(golden, f"amp/{sc['name']}")
+    np.testing.assert_array_equal(got["spins"], ref["spins"])
+    for k in ref:
+        if k != "spins":
+            _close(got[k], ref[k], 1e-12)
+
+
+@pytest.mark.parametrize("sc", scenarios.ELOC, ids=lambda s: s["name"])
+def test_local_energies(orc_results, golden, sc):
+    got = orc_results[f"eloc/{sc['name']}"]
+    ref = golden_case(golden, f"eloc/{sc['name']}")
+    for k in ("e", "e_re", "e_im", "e_single"):
+        _close(got[k], ref[k], 1e-12)
+
+
+@pytest.mark.parametrize("sc", scenarios.SAMPLER, ids=lambda s: s["name"])
+def test_sampler_sequence_bit_exact(orc_results, golden, sc):
+    got = orc_results[f"sampler/{sc['name']}"]
+    ref = golden_case(golden, f"sampler/{sc['name']}")
+    for k in ref:
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+
+
+@pytest.mark.parametrize("sc", scenarios.SR, ids=lambda s: s["name"])
+def test_sr_steps(orc_results, golden, sc):
+    got = orc_results[f"sr/{sc['name']}"]
+    ref = golden_case(golden, f"sr/{sc['name']}")
+    _close(got["params"], ref["params"], 1e-11)
+    _close(got["info"][:, :4], ref["info"][:, :4], 1e-10)
+    np.testing.assert_array_equal(got["info"][:, 4:], ref["info"][:, 4:])
+
+
+@pytest.mark.parametrize("sc", scenarios.RUN, ids=lambda s: s["name"])
+def test_sr_runs(orc_results, golden, sc):
+    got = orc_results[f"run/{sc['name']}"]
+    ref = golden_case(golden, f"run/{sc['name']}")
+    _close(got["trace"], ref["trace"], 1e-10)
+    _close(got["params"], ref["params"], 1e-10)
+
+
+def test_minsr_breakdown_zero_update(golden):
+    ref = golden_case(golden, "sr/minsr_retry_tfim_4x1_rbm")
+    # the reference reports the failed factorisation and applies no update
+    assert ref["info"][0, 5] == 0
+    np.testing.assert_array_equal(ref["params"][0], ref["params"][1])
+
+
+# ---- exact diagonalisation -------------------------------------------------
+def _ed(orc, cfg, Lx, Ly, per=0, n_down=-1):
+    return orc.orc_ed_ground_energy(ctypes.byref(cfg), Lx, Ly, per, n_down, 400, 1e-13)
+
+
+@pytest.mark.parametrize("key", ["tfim_2x2_J1.0_G1.0", "tfim_4x1_J1.0_G1.0", "tfim_3x3_J1.0_G0.5",
+                                 "tfim_8x1_J1.0_G1.0", "heis_4x1_J1.0_Jz1.0", "heis_8x1_J1.0_Jz1.0",
+                                 "heis_4x2_J1.0_Jz1.0", "heis_3x3_J1.0_Jz0.5", "kagome_2x2_pbc1",
+                                 "kagome_2x2_pbc0", "kagome_1x1_pbc1"])
+def test_ed_matches_reference_lanczos(orc, golden, key):
+    ref = float(golden[f"ed/{key}"][0])
+    parts = key.split("_")
+    kind = parts[0]
+    Lx, Ly = map(int, parts[1].split("x"))
+    if kind == "tfim":
+        J, G = float(parts[2][1:]), float(parts[3][1:])
+        cfg = config_defaults(hamiltonian=0, j_coupling=J, transverse_field=G)
+    elif kind == "heis":
+        J, Jz = float(parts[2][1:]), float(parts[3][2:])
+        cfg = config_defaults(hamiltonian=4, j_coupling=J, j_z_coupling=Jz)
+    else:
+        cfg = config_defaults(hamiltonian=6, kagome_pbc=int(parts[2][3:]))
+    assert abs(_ed(orc, cfg, Lx, Ly) - ref) < 1e-8
+
+
+def tfim_pbc_closed_form(N, J, h):
+    k = (2 * np.arange(N) + 1) * np.pi / N
+    return -np.sum(np.sqrt(J * J + h * h - 2 * J * h * np.cos(k)))
+
+
+@pytest.mark.parametrize("N,h", [(16, 1.0), (10, 0.7), (12, 1.3)])
+def test_ed_tfim_pbc_free_fermion_closed_form(orc, N, h):
+    cfg = config_defaults(hamiltonian=0, transverse_field=h)
+    e = _ed(orc, cfg, N, 1, per=1)
+    assert abs(e - tfim_pbc_closed_form(N, 1.0, h)) < 1e-9
+    if N == 16 and h == 1.0:
+        assert abs(e - (-2.0 / math.sin(math.pi / 32))) < 1e-9
+
+
+def test_ed_known_anchors(orc):
+    # Heisenberg 4-site OBC chain (reference tests/test_nqs_lanczos.c:163)
+    assert abs(_ed(orc, config_defaults(hamiltonian=1), 4, 1) - (-1.616025)) < 1e-6
+    # kagome 2x2 PBC (reference docs/nqs.md:487; tests/test_nqs_sector_lanczos.c:34)
+    e = _ed(orc, config_defaults(hamiltonian=6), 2, 2, n_down=6)
+    assert abs(e - (-5.4448752170)) < 1e-8
+
+
+def test_ed_sector_matches_full_space_and_dense(orc):
+    cfg = config_defaults(hamiltonian=2, j2_coupling=0.5)
+    full = _ed(orc, cfg, 3, 3, per=0)
+    secs = [_ed(orc, cfg, 3, 3, per=0, n_down=k) for k in range(10)]
+    assert abs(min(secs) - full) < 1e-9
+
+
+# ---- reference-test known answers with callback ansaetze -------------------
+CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                      ctypes.c_void_p, ctypes.c_void_p)
+
+
+@CB
+def _uniform(s, n, u, la, ar):
+    ctypes.cast(la, ctypes.POINTER(ctypes.c_double))[0] = 0.0
+    ctypes.cast(ar, ctypes.POINTER(ctypes.c_double))[0] = 0.0
+
+
+def _le(orc, cfg, Lx, Ly, spins, fn=None, user=None):
+    s = np.ascontiguousarray(np.array(spins, dtype=np.int32))
+    fn = fn or ctypes.cast(_uniform, ctypes.c_void_p).value
+    return orc.orc_local_energy(ctypes.byref(cfg), Lx, Ly, ptr(s), fn, user)
+
+
+def test_reference_known_answers(orc):
+    # tests/test_nqs.c:110-121, 133-156
+    assert abs(_le(orc, config_defaults(transverse_field=0.0), 3, 3, [1] * 9) + 12.0) < 1e-12
+    neel = [1, -1, 1, -1, 1, -1, 1, -1, 1]
+    assert abs(_le(orc, config_defaults(hamiltonian=1), 3, 3, neel) - 3.0) < 1e-12
+    assert abs(_le(orc, config_defaults(hamiltonian=2, j2_coupling=0.0), 3, 3, neel) - 3.0) < 1e-12
+    # tests/test_nqs_kagome.c:52-141
+    k = config_defaults(hamiltonian=6)
+    assert abs(_le(orc, k, 2, 2, [1] * 12) - 6.0) < 1e-12
+    assert abs(_le(orc, config_defaults(hamiltonian=6, j_coupling=2.0), 2, 2, [1] * 12) - 12.0) < 1e-12
+    s3 = [1, -1, -1] * 4
+    assert abs(_le(orc, k, 2, 2, s3) - 6.0) < 1e-12
+    assert abs(_le(orc, config_defaults(hamiltonian=6, kagome_pbc=0), 2, 2, [1] * 12) - 3.75) < 1e-12
+    assert abs(_le(orc, k, 1, 1, [1, 1, 1]) - 1.5) < 1e-12
+
+
+def test_linear_ansatz_known_answer(orc):
+    # tests/test_nqs_kagome.c:170-201: the linear ansatz is the mean-field one
+    cfg = config_defaults(hamiltonian=6)
+    a = orc.orc_ansatz_create(ctypes.byref(cfg), 3)
+    p = np.ctypeslib.as_array(ctypes.cast(orc.orc_ansatz_params_raw(a), ctypes.POINTER(ctypes.c_double)), (3,))
+    p[:] = [0.1, 0.2, 0.3]
+    e = _le(orc, cfg, 1, 1, [1, -1, 1], fnptr(orc, "orc_ansatz_log_amp"), a)
+    assert abs(e - (-0.5 + 2.0 * math.cosh(0.2))) < 1e-12
+    orc.orc_ansatz_free(a)
+
+
+# ---- finite differences ----------------------------------------------------
+@pytest.mark.parametrize("kind,M", [(5, 6), (6, 5)])
+def test_gradients_finite_difference(orc, kind, M):
+    N = 5
+    cfg = config_defaults(ansatz=kind, rbm_hidden_units=M, rbm_init_scale=0.6, rng_seed=17)
+    a = orc.orc_ansatz_create(ctypes.byref(cfg), N)
+    P = orc.orc_ansatz_num_params(a)
+    p = np.ctypeslib.as_array(ctypes.cast(orc.orc_ansatz_params_raw(a), ctypes.POINTER(ctypes.c_double)), (P,))
+    s = np.ascontiguousarray(np.array([1, -1, -1, 1, 1], dtype=np.int32))
+    g = np.zeros(P)
+    gr = np.zeros(P)
+    gi = np.zeros(P)
+    orc.orc_ansatz_logpsi_gradient(a, ptr(s), N, ptr(g))
+    orc.orc_ansatz_logpsi_gradient_complex(a, ptr(s), N, ptr(gr), ptr(gi))
+    la, ar = np.zeros(1), np.zeros(1)
+
+    def f():
+        orc.orc_ansatz_log_amp(ptr(s), N, a, ptr(la), ptr(ar))
+        return la[0], ar[0]
+
+    h = 1e-6
+    for k in range(P):
+        p[k] += h
+        l1, a1 = f()
+        p[k] -= 2 * h
+        l2, a2 = f()
+        p[k] += h
+        dl, da = (l1 - l2) / (2 * h), (a1 - a2) / (2 * h)
+        assert abs(dl - g[k]) < 1e-6
+        assert abs(dl - gr[k]) < 1e-6      # Re d log psi = d log|psi|
+        assert abs(da - gi[k]) < 1e-6      # Im d log psi = d arg psi
+    orc.orc_ansatz_free(a)
+
+
+# ---- E_loc consistency with the exact energy -------------------------------
+@pytest.mark.parametrize("ham,Lx,Ly,per,kind", [(0, 3, 3, 0, 5), (1, 4, 2, 0, 6), (2, 3, 3, 1, 6),
+                                                (6, 1, 3, 1, 6), (0, 9, 1, 1, 5)])
+def test_mean_local_energy_equals_exact_energy(orc, ham, Lx, Ly, per, kind):
+    cfg = config_defaults(ansatz=kind, hamiltonian=ham, rbm_hidden_units=7, rbm_init_scale=0.5,
+                          j2_coupling=0.4, transverse_field=0.8)
+    N = 3 * Lx * Ly if ham == 6 else Lx * Ly
+    a = orc.orc_ansatz_create(ctypes.byref(cfg), N)
+    fn = fnptr(orc, "orc_ansatz_log_amp")
+    dim = 1 << N
+    st = np.arange(dim)
+    spins = np.ascontiguousarray(np.where((st[:, None] >> np.arange(N)) & 1, -1, 1).astype(np.int32))
+    re = np.zeros(dim)
+    im = np.zeros(dim)
+    orc.orc_local_energy_batch_ex(ctypes.byref(cfg), Lx, Ly, per, ptr(spins), dim, fn, a, ptr(re), ptr(im))
+    la = np.zeros(dim)
+    ar = np.zeros(dim)
+    for b in range(dim):
+        orc.orc_ansatz_log_amp(ptr(spins[b]), N, a, ptr(la[b:]), ptr(ar[b:]))
+    w = np.exp(2 * (la - la.max()))
+    w /= w.sum()
+    exact = orc.orc_exact_energy(ctypes.byref(cfg), Lx, Ly, per, fn, a)
+    assert abs(np.sum(w * re) - exact) < 1e-10
+    assert abs(np.sum(w * im)) < 1e-10     # Hermitian H: imaginary part averages out
+    orc.orc_ansatz_free(a)
+
+
+# ---- solvers --------------------------------------------------------------
+def test_minsr_equals_cg_equals_dense(orc):
+    rng = np.random.default_rng(3)
+    Ns, P = 40, 25
+    O = rng.standard_normal((Ns, P))
+    E = rng.standard_normal(Ns)
+    eps = 1e-2
+    Oc = O - O.mean(0)
+    S = Oc.T @ Oc / Ns
+    F = Oc.T @ (E - E.mean()) / Ns
+    dense = np.linalg.solve(S + eps * np.eye(P), F)
+    d1 = np.zeros(P)
+    conv = ctypes.c_int(0)
+    Om = np.ascontiguousarray(O.mean(0))
+    it = orc.orc_cg_solve(ptr(np.ascontiguousarray(O)), Ns, P, ptr(Om), eps, ptr(np.ascontiguousarray(F)),
+                          500, 1e-14, ptr(d1), ctypes.byref(conv))
+    assert conv.value == 1 and it > 0
+    np.testing.assert_allclose(d1, dense, rtol=1e-8, atol=1e-10)
+    d2 = np.zeros(P)
+    T = np.zeros((Ns, Ns))
+    y = np.zeros(Ns)
+    Ow = np.ascontiguousarray(O.copy())
+    ok = orc.orc_minsr_solve(ptr(Ow), Ns, P, ptr(np.ascontiguousarray(E)), eps, ptr(d2), ptr(T), ptr(y))
+    assert ok == 1
+    np.testing.assert_allclose(T, Oc @ Oc.T / Ns + eps * np.eye(Ns), atol=1e-12)
+    np.testing.assert_allclose(d2, dense, rtol=1e-9, atol=1e-11)
+
+
+def test_cholesky_residual(orc):
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((30, 30))
+    A = A @ A.T + 30 * np.eye(30)
+    L = np.ascontiguousarray(A.copy())
+    assert orc.orc_cholesky_lower(ptr(L), 30) == 0
+    np.testing.assert_allclose(L @ L.T, A, atol=1e-11)
+    assert np.all(np.triu(L, 1) == 0)
+
+
+# ---- sampler extensions ----------------------------------------------------
+def _mc(orc, N, cfg, opt, n):
+    a = orc.orc_ansatz_create(ctypes.byref(cfg), N)
+    smp = orc.orc_sampler_create_ex(N, ctypes.byref(cfg), ctypes.byref(opt), fnptr(orc, "orc_ansatz_log_amp"), a)
+    orc.orc_sampler_thermalize(smp)
+    out = np.zeros((n, N), dtype=np.int32)
+    assert orc.orc_sampler_batch(smp, n, ptr(out)) == 0
+    acc = orc.orc_sampler_acceptance_ratio(smp)
+    orc.orc_sampler_free(smp)
+    return a, out, acc
+
+
+def test_chain0_is_the_reference_chain(orc):
+    cfg = config_defaults(ansatz=5, rbm_init_scale=0.5, num_thermalize=10, num_decorrelate=3)
+    a1, one, _ = _mc(orc, 8, cfg, options_defaults(), 20)
+    a2, many, _ = _mc(orc, 8, cfg, options_defaults(num_chains=4), 80)
+    np.testing.assert_array_equal(many[0::4], one)
+    orc.orc_ansatz_free(a1)
+    orc.orc_ansatz_free(a2)
+
+
+def test_exchange_conserves_sz(orc):
+    cfg = config_defaults(ansatz=6, rbm_hidden_units=6, rbm_init_scale=0.5, num_thermalize=20)
+    a, out, acc = _mc(orc, 10, cfg, options_defaults(num_chains=3, move=1, sz_zero_start=1), 90)
+    assert np.all(out.sum(1) == 0)
+    assert 0.0 < acc < 1.0
+    orc.orc_ansatz_free(a)
+
+
+@pytest.mark.parametrize("move,sz0", [(0, 0), (1, 1)])
+def test_stationary_distribution_by_enumeration(orc, move, sz0):
+    N = 6
+    cfg = config_defaults(ansatz=5, rbm_hidden_units=4, rbm_init_scale=1.5, num_thermalize=50,
+                          num_decorrelate=2, rng_seed=77)
+    a, out, _ = _mc(orc, N, cfg, options_defaults(num_chains=16, move=move, sz_zero_start=sz0), 64000)
+    st = np.arange(1 << N)
+    sp = np.ascontiguousarray(np.where((st[:, None] >> np.arange(N)) & 1, -1, 1).astype(np.int32))
+    la = np.zeros(len(st))
+    ar = np.zeros(len(st))
+    for b in range(len(st)):
+        orc.orc_ansatz_log_amp(ptr(sp[b]), N, a, ptr(la[b:]), ptr(ar[b:]))
+    mask = (sp.sum(1) == 0) if sz0 else np.ones(len(st), bool)
+    p = np.exp(2 * (la - la.max())) * mask
+    p /= p.sum()
+    idx = ((out < 0) * (1 << np.arange(N))).sum(1)
+    freq = np.bincount(idx, minlength=len(st)) / len(idx)
+    # correlated samples: compare with a generous multiple of the iid error
+    err = np.abs(freq - p)
+    assert np.all(err <= 6 * np.sqrt(p * (1 - p) / len(idx)) * 4 + 1e-4)
+    orc.orc_ansatz_free(a)
+
+
+def test_tau_int_ar1(orc):
+    rng = np.random.default_rng(9)
+    for rho in (0.0, 0.5, 0.9):
+        n = 200000
+        x = np.zeros(n)
+        e = rng.standard_normal(n)
+        for t in range(1, n):
+            x[t] = rho * x[t - 1] + e[t]
+        tau = orc.orc_tau_int(ptr(x), n)
+        exact = (1 + rho) / (2 * (1 - rho))
+        assert abs(tau - exact) / exact < 0.08
