@@ -110,9 +110,20 @@ int nqs_hip_logpsi_gradient_batch(nqs_ansatz_t *a, const int *spins_batch,
 double nqs_hip_tau_int(const double *x, int n);
 
 /* Stage timings of the last SR step on the sampler's stream, milliseconds:
- * [0] sampling, [1] forward, [2] local energy, [3] O / reductions,
- * [4] solve, [5] update, [6] total.  Returns 0 when available. */
+ * [0] sampling, [1] local energy (with its forward pass), [2] O_k, means and
+ * force, [3] solve, [4] update, [5] unused, [6] total.  0 when available. */
 int nqs_hip_last_stage_ms(const nqs_sampler_t *s, double out[7]);
+
+/* MinSR building blocks on host arrays (both run on the device).
+ * Gram: T = O O^T * scale for row-major O [B][P], full symmetric T [B][B].
+ * Cholesky solve: factor the symmetric positive T [n][n] in place (lower
+ * triangle holds L, as src/nqs/nqs_optimizer.c:1026-1043 leaves it, upper
+ * untouched) and solve T x = b.  On a non-positive pivot it retries once
+ * with +1e-6 on the diagonal of the partially factored matrix, like
+ * nqs_optimizer.c:1156-1160; returns 1 on success, 0 when both attempts
+ * fail (x = 0), -1 on errors. */
+int nqs_hip_gram(const double *O, int B, long P, double scale, double *T);
+int nqs_hip_cholesky_solve(double *T, int n, const double *b, double *x);
 
 /* Diagnostics of the device path: how many kernel launches the library has
  * made, and how many host-callback adaptor calls it has made (a callback
@@ -120,6 +131,12 @@ int nqs_hip_last_stage_ms(const nqs_sampler_t *s, double out[7]);
  * the host). */
 long long nqs_hip_device_launches(void);
 long long nqs_hip_host_callback_calls(void);
+
+/* ABI layout check: writes sizeof(nqs_config_t), the offset of each of its
+ * members in declaration order, sizeof(nqs_sr_step_info_t),
+ * sizeof(nqs_marshall_wrapper_t) and sizeof(nqs_hip_options_t); returns the
+ * count written or -1 when capacity is too small. */
+int nqs_hip_config_layout(long *out, int capacity);
 
 /* Library version string. */
 const char *nqs_hip_version(void);

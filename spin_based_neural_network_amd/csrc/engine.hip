@@ -1,0 +1,1478 @@
+// engine.hip — host engine and the exported C ABI of libnqs_hip.so.
+//
+// Every reference entry point of include/nqs (ansatz, sampler, local
+// energy, SR / holomorphic SR / MinSR, Marshall) is implemented here over
+// the gfx950 kernels of k_*.hip.  One HIP stream (default priority) carries
+// all work; calls are serialised by a process-wide mutex.
+//
+// Callback recognition (the reference passes amplitudes as host function
+// pointers, include/nqs/nqs_sampler.h:35-37): nqs_ansatz_log_amp over an
+// ansatz handle, and nqs_marshall_log_amp wrapping it, are evaluated on the
+// device.  Any other callback is a caller's host function; it is called on
+// the host for exactly the configurations the reference would evaluate
+// (sampler moves, connected configurations) and everything else stays on
+// the device.  nqs_hip_host_callback_calls() counts those calls.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <mutex>
+#include <vector>
+
+#include "../../include/nqs/nqs_ansatz.h"
+#include "../../include/nqs/nqs_gradient.h"
+#include "../../include/nqs/nqs_marshall.h"
+#include "../../include/nqs/nqs_optimizer.h"
+#include "../../include/nqs/nqs_sampler.h"
+#include "../../include/nqs_hip/nqs_hip.h"
+#include "lattice.h"
+#include "nqs_kernels.h"
+
+#define NQS_EXPORT extern "C" __attribute__((visibility("default")))
+
+using nqs::ModelView;
+
+namespace {
+
+std::recursive_mutex g_mu;
+hipStream_t g_stream = nullptr;
+std::atomic<long long> g_launches{0};
+std::atomic<long long> g_host_calls{0};
+
+#define CK(expr)                                                                \
+  do {                                                                          \
+    hipError_t _e = (expr);                                                     \
+    if (_e != hipSuccess) {                                                     \
+      fprintf(stderr, "nqs_hip: %s failed (%s:%d): %s\n", #expr, __FILE__,      \
+              __LINE__, hipGetErrorString(_e));                                 \
+      return -1;                                                                \
+    }                                                                           \
+  } while (0)
+#define CKR(expr)            \
+  do {                       \
+    if ((expr) != 0) return -1; \
+  } while (0)
+
+int stream_ready() {
+  if (g_stream) return 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    fprintf(stderr, "nqs_hip: no HIP device available; this library runs on the GPU only\n");
+    return -1;
+  }
+  CK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  return 0;
+}
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t b) {
+    if (b <= bytes && p) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (b == 0) b = 16;
+    if (hipMalloc(&p, b) != hipSuccess) {
+      fprintf(stderr, "nqs_hip: hipMalloc(%zu) failed\n", b);
+      return -1;
+    }
+    bytes = b;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const { return (T*)p; }
+};
+
+// ---------------------------------------------------------------------------
+// reference initialiser stream, src/nqs/nqs_ansatz.c:65-252 (one thread)
+// ---------------------------------------------------------------------------
+__device__ double d_gauss(uint64_t& s) {
+  double u1 = nqs::xs_uniform(s);
+  double u2 = nqs::xs_uniform(s);
+  if (u1 < 1e-15) u1 = 1e-15;
+  return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
+}
+
+__global__ void k_init_params(double* p, int kind, int N, int M, double scale, uint64_t seed) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint64_t s = seed;
+  if (kind == NQS_ANSATZ_LEGACY_MLP) {
+    for (int i = 0; i < N; i++) p[i] = 0.01 * (nqs::xs_uniform(s) - 0.5);
+    return;
+  }
+  const long MN = (long)M * N, Pr = (long)N + M + MN;
+  const double ws = scale / sqrt((double)N);
+  for (int i = 0; i < N; i++) p[i] = scale * (nqs::xs_uniform(s) - 0.5);
+  for (int h = 0; h < M; h++) p[N + h] = scale * (nqs::xs_uniform(s) - 0.5);
+  for (long k = 0; k < MN; k++) p[N + M + k] = ws * d_gauss(s);
+  if (kind == NQS_ANSATZ_COMPLEX_RBM) {
+    const double si = 0.1 * scale, wsi = si / sqrt((double)N);
+    double* q = p + Pr;
+    for (int i = 0; i < N; i++) q[i] = si * (nqs::xs_uniform(s) - 0.5);
+    for (int h = 0; h < M; h++) q[N + h] = si * (nqs::xs_uniform(s) - 0.5);
+    for (long k = 0; k < MN; k++) q[N + M + k] = wsi * d_gauss(s);
+  }
+}
+
+}  // namespace
+
+void nqs::count_launch(int n) { g_launches += n; }
+
+// ===========================================================================
+// ansatz
+// ===========================================================================
+struct nqs_ansatz {
+  int kind = 0, N = 0, M = 0;
+  long Pr = 0, P = 0;
+  std::vector<double> h;   // host master copy (params_raw)
+  DBuf d_par;
+  nqs::ModelBuffers<double> m64;
+  nqs::ModelBuffers<float> m32;
+  int dev_fresh = 0;       // device copy equals host copy (inside a run)
+};
+
+namespace {
+
+int ansatz_upload(nqs_ansatz* a) {
+  if (a->dev_fresh) return 0;
+  CK(hipMemcpyAsync(a->d_par.p, a->h.data(), sizeof(double) * a->P, hipMemcpyHostToDevice,
+                    g_stream));
+  a->m64.valid = 0;
+  a->m32.valid = 0;
+  return 0;
+}
+
+int ansatz_download(nqs_ansatz* a) {
+  CK(hipMemcpyAsync(a->h.data(), a->d_par.p, sizeof(double) * a->P, hipMemcpyDeviceToHost,
+                    g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+template <typename R>
+nqs::ModelBuffers<R>& model_bufs(nqs_ansatz* a);
+template <>
+nqs::ModelBuffers<double>& model_bufs<double>(nqs_ansatz* a) { return a->m64; }
+template <>
+nqs::ModelBuffers<float>& model_bufs<float>(nqs_ansatz* a) { return a->m32; }
+
+template <typename R>
+int model_view(nqs_ansatz* a, ModelView<R>& v) {
+  nqs::ModelBuffers<R>& mb = model_bufs<R>(a);
+  if (!mb.valid) CKR(nqs::launch_prep<R>(a->d_par.as<double>(), a->kind, a->N, a->M, mb, g_stream));
+  v = mb.view;
+  return 0;
+}
+
+bool is_ansatz_cb(nqs_log_amp_fn_t fn, void* user) {
+  return fn == nqs_ansatz_log_amp && user != nullptr;
+}
+
+// Device-evaluable amplitude source behind a callback: the ansatz, and a
+// Marshall sublattice when wrapped.
+struct AmpSource {
+  nqs_ansatz* a = nullptr;
+  int marshall = 0, mlx = 0, mly = 0;
+};
+
+bool recognise(nqs_log_amp_fn_t fn, void* user, AmpSource& src) {
+  src = AmpSource();
+  if (is_ansatz_cb(fn, user)) {
+    src.a = (nqs_ansatz*)user;
+    return true;
+  }
+  if (fn == nqs_marshall_log_amp && user) {
+    const nqs_marshall_wrapper_t* w = (const nqs_marshall_wrapper_t*)user;
+    if (is_ansatz_cb(w->base_log_amp, w->base_user)) {
+      src.a = (nqs_ansatz*)w->base_user;
+      src.marshall = 1;
+      src.mlx = w->size_x;
+      src.mly = w->size_y;
+      return true;
+    }
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// workspace shared by all calls (single stream, serialised)
+// ---------------------------------------------------------------------------
+struct Workspace {
+  DBuf spins, t_re, t_im, la, ar, e_re, e_im, O, Oi, colwork, u1, u2, vecs, T, diag, y, scal,
+      cg, fail, ham, sublat, hostla, hostar;
+};
+Workspace g_ws;
+
+// Hamiltonian on the device, rebuilt when the request changes.
+struct HamCache {
+  nqs_config_t cfg;
+  int Lx = -1, Ly = -1, periodic = -1, marshall = 0, mlx = 0, mly = 0;
+  nqs::HamHost host;
+  nqs::HamDev dev;
+  int valid = 0;
+};
+HamCache g_ham;
+
+bool same_ham(const nqs_config_t* c, int Lx, int Ly, int per, const AmpSource& src) {
+  if (!g_ham.valid) return false;
+  const nqs_config_t& o = g_ham.cfg;
+  return o.hamiltonian == c->hamiltonian && o.j_coupling == c->j_coupling &&
+         o.j2_coupling == c->j2_coupling && o.j_z_coupling == c->j_z_coupling &&
+         o.transverse_field == c->transverse_field && o.kagome_pbc == c->kagome_pbc &&
+         g_ham.Lx == Lx && g_ham.Ly == Ly && g_ham.periodic == per &&
+         g_ham.marshall == src.marshall && g_ham.mlx == src.mlx && g_ham.mly == src.mly;
+}
+
+int ham_device(const nqs_config_t* cfg, int Lx, int Ly, int per, const AmpSource& src,
+               nqs::HamDev& out) {
+  if (same_ham(cfg, Lx, Ly, per, src)) {
+    out = g_ham.dev;
+    return 0;
+  }
+  g_ham.valid = 0;
+  if (nqs::build_hamiltonian(cfg, Lx, Ly, per, g_ham.host) != 0) {
+    fprintf(stderr, "nqs_hip: Hamiltonian kind %d is not provided by this library\n",
+            (int)cfg->hamiltonian);
+    return -1;
+  }
+  const nqs::HamHost& H = g_ham.host;
+  const size_t nd = H.diag_a.size(), nf = H.flip_i.size(), np = H.pair_a.size();
+  const size_t ints = 2 * nd + nf + 2 * np + 4, dbls = nd + nf + np + 4;
+  CKR(g_ws.ham.ensure(ints * sizeof(int) + dbls * sizeof(double) + 64));
+  char* base = g_ws.ham.as<char>();
+  double* dc = (double*)base;
+  double* fc = dc + nd;
+  double* pc = fc + nf;
+  int* ip = (int*)(pc + np + 1);
+  int* da = ip;
+  int* db = da + nd;
+  int* fi = db + nd;
+  int* pa = fi + nf;
+  int* pb = pa + np;
+  std::vector<char> stage(ints * sizeof(int) + dbls * sizeof(double) + 64);
+  double* hdc = (double*)stage.data();
+  std::copy(H.diag_c.begin(), H.diag_c.end(), hdc);
+  std::copy(H.flip_c.begin(), H.flip_c.end(), hdc + nd);
+  std::copy(H.pair_c.begin(), H.pair_c.end(), hdc + nd + nf);
+  int* hip_ = (int*)(hdc + nd + nf + np + 1);
+  std::copy(H.diag_a.begin(), H.diag_a.end(), hip_);
+  std::copy(H.diag_b.begin(), H.diag_b.end(), hip_ + nd);
+  std::copy(H.flip_i.begin(), H.flip_i.end(), hip_ + 2 * nd);
+  std::copy(H.pair_a.begin(), H.pair_a.end(), hip_ + 2 * nd + nf);
+  std::copy(H.pair_b.begin(), H.pair_b.end(), hip_ + 2 * nd + nf + np);
+  CK(hipMemcpyAsync(base, stage.data(), stage.size(), hipMemcpyHostToDevice, g_stream));
+  nqs::HamDev d;
+  d.n_diag = (int)nd;
+  d.n_flip = (int)nf;
+  d.n_pair = (int)np;
+  d.diag_a = da; d.diag_b = db; d.diag_c = dc;
+  d.flip_i = fi; d.flip_c = fc;
+  d.pair_a = pa; d.pair_b = pb; d.pair_c = pc;
+  d.sublat = nullptr;
+  if (src.marshall) {
+    const int N = H.n_sites;
+    std::vector<signed char> sub(N, 0);
+    for (int x = 0; x < src.mlx; x++)
+      for (int y = 0; y < src.mly; y++) {
+        const int i = x * src.mly + y;
+        if (i < N && ((x + y) & 1) == 0) sub[i] = 1;
+      }
+    CKR(g_ws.sublat.ensure(N + 16));
+    CK(hipMemcpyAsync(g_ws.sublat.p, sub.data(), N, hipMemcpyHostToDevice, g_stream));
+    d.sublat = g_ws.sublat.as<signed char>();
+  }
+  CK(hipStreamSynchronize(g_stream));
+  g_ham.cfg = *cfg;
+  g_ham.Lx = Lx;
+  g_ham.Ly = Ly;
+  g_ham.periodic = per;
+  g_ham.marshall = src.marshall;
+  g_ham.mlx = src.mlx;
+  g_ham.mly = src.mly;
+  g_ham.dev = d;
+  g_ham.valid = 1;
+  out = d;
+  return 0;
+}
+
+// forward pass of B device configurations into the workspace t buffers
+template <typename R>
+int forward_t(nqs_ansatz* a, const int* d_spins, int B, double* la, double* ar) {
+  ModelView<R> v;
+  CKR(model_view<R>(a, v));
+  const size_t tb = sizeof(R) * (size_t)B * (size_t)std::max(a->M, 1);
+  CKR(g_ws.t_re.ensure(tb));
+  CKR(g_ws.t_im.ensure(tb));
+  return nqs::launch_forward<R>(v, d_spins, B, g_ws.t_re.as<R>(), g_ws.t_im.as<R>(), la, ar,
+                                g_stream);
+}
+
+// E_loc of B device configurations (device amplitude source)
+template <typename R>
+int eloc_device_T(const nqs_config_t* cfg, int Lx, int Ly, int per, const AmpSource& src,
+                  const int* d_spins, int B, double* d_re, double* d_im) {
+  nqs::HamDev H;
+  CKR(ham_device(cfg, Lx, Ly, per, src, H));
+  if (g_ham.host.n_sites != src.a->N) {
+    fprintf(stderr, "nqs_hip: lattice has %d sites but the ansatz %d\n", g_ham.host.n_sites,
+            src.a->N);
+    return -1;
+  }
+  CKR(forward_t<R>(src.a, d_spins, B, nullptr, nullptr));
+  ModelView<R> v;
+  CKR(model_view<R>(src.a, v));
+  return nqs::launch_eloc<R>(v, H, d_spins, B, g_ws.t_re.as<R>(), g_ws.t_im.as<R>(), d_re, d_im,
+                             g_stream);
+}
+
+int eloc_device(const nqs_config_t* cfg, int Lx, int Ly, int per, int prec, const AmpSource& src,
+                const int* d_spins, int B, double* d_re, double* d_im) {
+  if (prec == NQS_HIP_FP32)
+    return eloc_device_T<float>(cfg, Lx, Ly, per, src, d_spins, B, d_re, d_im);
+  return eloc_device_T<double>(cfg, Lx, Ly, per, src, d_spins, B, d_re, d_im);
+}
+
+// E_loc with a host callback: the callback is evaluated on the sample and on
+// each connected configuration; the sums run on the device.
+int eloc_callback(const nqs_config_t* cfg, int Lx, int Ly, int per, const int* h_spins, int B,
+                  nqs_log_amp_fn_t fn, void* user, double* d_re, double* d_im,
+                  const int* d_spins) {
+  AmpSource none;
+  nqs::HamDev H;
+  CKR(ham_device(cfg, Lx, Ly, per, none, H));
+  const nqs::HamHost& HH = g_ham.host;
+  const int N = HH.n_sites;
+  const int T = H.n_flip + H.n_pair;
+  std::vector<double> la((size_t)B * (T + 1), 0.0), ar((size_t)B * (T + 1), 0.0);
+  std::vector<int> scr(N);
+  for (int b = 0; b < B; b++) {
+    const int* s = h_spins + (size_t)b * N;
+    double* l = la.data() + (size_t)b * (T + 1);
+    double* r = ar.data() + (size_t)b * (T + 1);
+    fn(s, N, user, &l[0], &r[0]);
+    g_host_calls++;
+    memcpy(scr.data(), s, sizeof(int) * N);
+    for (int t = 0; t < T; t++) {
+      int i, j = -1;
+      if (t < H.n_flip) i = HH.flip_i[t];
+      else {
+        i = HH.pair_a[t - H.n_flip];
+        j = HH.pair_b[t - H.n_flip];
+        if (s[i] == s[j]) continue;
+      }
+      scr[i] = -scr[i];
+      if (j >= 0) scr[j] = -scr[j];
+      fn(scr.data(), N, user, &l[1 + t], &r[1 + t]);
+      g_host_calls++;
+      scr[i] = -scr[i];
+      if (j >= 0) scr[j] = -scr[j];
+    }
+  }
+  CKR(g_ws.hostla.ensure(sizeof(double) * la.size()));
+  CKR(g_ws.hostar.ensure(sizeof(double) * ar.size()));
+  CK(hipMemcpyAsync(g_ws.hostla.p, la.data(), sizeof(double) * la.size(), hipMemcpyHostToDevice, g_stream));
+  CK(hipMemcpyAsync(g_ws.hostar.p, ar.data(), sizeof(double) * ar.size(), hipMemcpyHostToDevice, g_stream));
+  CKR(nqs::launch_eloc_from_logamps(H, d_spins, B, N, T, g_ws.hostla.as<double>(),
+                                    g_ws.hostar.as<double>(), d_re, d_im, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+int upload_spins(const int* h, long count) {
+  CKR(g_ws.spins.ensure(sizeof(int) * (size_t)count));
+  CK(hipMemcpyAsync(g_ws.spins.p, h, sizeof(int) * (size_t)count, hipMemcpyHostToDevice, g_stream));
+  return 0;
+}
+
+}  // namespace
+
+NQS_EXPORT nqs_ansatz_t* nqs_ansatz_create(const nqs_config_t* cfg, int num_sites) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || num_sites <= 0) return nullptr;
+  const int kind = cfg->ansatz;
+  if (kind != NQS_ANSATZ_LEGACY_MLP && kind != NQS_ANSATZ_RBM && kind != NQS_ANSATZ_COMPLEX_RBM)
+    return nullptr;
+  if (stream_ready()) return nullptr;
+  nqs_ansatz* a = new nqs_ansatz();
+  a->kind = kind;
+  a->N = num_sites;
+  if (kind == NQS_ANSATZ_LEGACY_MLP) {
+    a->M = 0;
+    a->Pr = a->P = num_sites;
+  } else {
+    a->M = cfg->rbm_hidden_units > 0 ? cfg->rbm_hidden_units : 2 * num_sites;
+    a->Pr = (long)num_sites + a->M + (long)a->M * num_sites;
+    a->P = (kind == NQS_ANSATZ_COMPLEX_RBM) ? 2 * a->Pr : a->Pr;
+  }
+  a->h.assign(a->P, 0.0);
+  if (a->d_par.ensure(sizeof(double) * a->P)) { delete a; return nullptr; }
+  const double scale = cfg->rbm_init_scale > 0.0 ? cfg->rbm_init_scale : 0.01;
+  const uint64_t seed = cfg->rng_seed ? (uint64_t)cfg->rng_seed : 0xDEADBEEFCAFEBABEULL;
+  hipLaunchKernelGGL(k_init_params, dim3(1), dim3(64), 0, g_stream, a->d_par.as<double>(), kind,
+                     a->N, a->M, scale, seed);
+  nqs::count_launch();
+  if (hipGetLastError() != hipSuccess || ansatz_download(a)) {
+    a->d_par.release();
+    delete a;
+    return nullptr;
+  }
+  return a;
+}
+
+NQS_EXPORT void nqs_ansatz_free(nqs_ansatz_t* a) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a) return;
+  (void)hipStreamSynchronize(g_stream);
+  a->d_par.release();
+  nqs::free_model(a->m64);
+  nqs::free_model(a->m32);
+  delete a;
+}
+
+NQS_EXPORT long nqs_ansatz_num_params(const nqs_ansatz_t* a) { return a ? a->P : 0; }
+
+NQS_EXPORT int nqs_ansatz_is_complex(const nqs_ansatz_t* a) {
+  return (a && a->kind == NQS_ANSATZ_COMPLEX_RBM) ? 1 : 0;
+}
+
+NQS_EXPORT double* nqs_ansatz_params_raw(nqs_ansatz_t* a) { return a ? a->h.data() : nullptr; }
+
+namespace {
+int log_amp_batch(nqs_ansatz* a, const int* h_spins, int B, double* la, double* ar) {
+  CKR(ansatz_upload(a));
+  CKR(upload_spins(h_spins, (long)B * a->N));
+  CKR(g_ws.la.ensure(sizeof(double) * B));
+  CKR(g_ws.ar.ensure(sizeof(double) * B));
+  CKR(forward_t<double>(a, g_ws.spins.as<int>(), B, g_ws.la.as<double>(), g_ws.ar.as<double>()));
+  CK(hipMemcpyAsync(la, g_ws.la.p, sizeof(double) * B, hipMemcpyDeviceToHost, g_stream));
+  CK(hipMemcpyAsync(ar, g_ws.ar.p, sizeof(double) * B, hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+int gradient_batch(nqs_ansatz* a, const int* h_spins, int B, int mode, double* re, double* im) {
+  CKR(ansatz_upload(a));
+  CKR(upload_spins(h_spins, (long)B * a->N));
+  CKR(forward_t<double>(a, g_ws.spins.as<int>(), B, nullptr, nullptr));
+  const size_t ob = sizeof(double) * (size_t)B * a->P;
+  CKR(g_ws.O.ensure(ob));
+  CKR(g_ws.Oi.ensure(ob));
+  ModelView<double> v;
+  CKR(model_view<double>(a, v));
+  CKR(nqs::launch_gen_O<double>(v, a->kind, g_ws.spins.as<int>(), B, g_ws.t_re.as<double>(),
+                                g_ws.t_im.as<double>(), mode, g_ws.O.as<double>(),
+                                g_ws.Oi.as<double>(), a->P, g_stream));
+  CK(hipMemcpyAsync(re, g_ws.O.p, ob, hipMemcpyDeviceToHost, g_stream));
+  if (mode == 1) CK(hipMemcpyAsync(im, g_ws.Oi.p, ob, hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+}  // namespace
+
+NQS_EXPORT void nqs_ansatz_log_amp(const int* spins, int num_sites, void* user, double* out_log_abs,
+                                   double* out_arg) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  nqs_ansatz* a = (nqs_ansatz*)user;
+  double la = 0.0, ar = 0.0;
+  if (a && spins && num_sites == a->N) {
+    if (log_amp_batch(a, spins, 1, &la, &ar) != 0) {
+      la = 0.0;
+      ar = 0.0;
+    }
+  }
+  if (out_log_abs) *out_log_abs = la;
+  if (out_arg) *out_arg = ar;
+}
+
+NQS_EXPORT int nqs_ansatz_logpsi_gradient(nqs_ansatz_t* a, const int* spins, int num_sites,
+                                          double* out_grad) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !spins || !out_grad || num_sites != a->N) return -1;
+  return gradient_batch(a, spins, 1, 0, out_grad, nullptr);
+}
+
+NQS_EXPORT int nqs_ansatz_logpsi_gradient_complex(nqs_ansatz_t* a, const int* spins, int num_sites,
+                                                  double* out_re, double* out_im) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !spins || !out_re || !out_im || num_sites != a->N) return -1;
+  return gradient_batch(a, spins, 1, 1, out_re, out_im);
+}
+
+NQS_EXPORT int nqs_ansatz_apply_update(nqs_ansatz_t* a, const double* delta, double step) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !delta) return -1;
+  if (ansatz_upload(a)) return -1;
+  if (g_ws.vecs.ensure(sizeof(double) * a->P)) return -1;
+  if (hipMemcpyAsync(g_ws.vecs.p, delta, sizeof(double) * a->P, hipMemcpyHostToDevice, g_stream) !=
+      hipSuccess)
+    return -1;
+  if (nqs::launch_axpy(a->d_par.as<double>(), g_ws.vecs.as<double>(), step, a->P, g_stream))
+    return -1;
+  a->m64.valid = 0;
+  a->m32.valid = 0;
+  return ansatz_download(a);
+}
+
+NQS_EXPORT int nqs_hip_log_amp_batch(nqs_ansatz_t* a, const int* spins, int B, double* la,
+                                     double* ar) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !spins || !la || !ar || B <= 0) return -1;
+  return log_amp_batch(a, spins, B, la, ar);
+}
+
+NQS_EXPORT int nqs_hip_logpsi_gradient_batch(nqs_ansatz_t* a, const int* spins, int B, int mode,
+                                             double* re, double* im) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !spins || !re || B <= 0 || (mode == 1 && !im)) return -1;
+  return gradient_batch(a, spins, B, mode, re, im);
+}
+
+// ===========================================================================
+// Marshall
+// ===========================================================================
+NQS_EXPORT int nqs_marshall_parity(const int* spins, int size_x, int size_y) {
+  // nqs_marshall.c:16-27: parity of the down-spin count on (x+y) even sites
+  int n = 0;
+  for (int x = 0; x < size_x; x++)
+    for (int y = 0; y < size_y; y++)
+      if (((x + y) & 1) == 0 && spins[x * size_y + y] < 0) n++;
+  return n & 1;
+}
+
+NQS_EXPORT void nqs_marshall_log_amp(const int* spins, int num_sites, void* user, double* out_log_abs,
+                                     double* out_arg) {
+  nqs_marshall_wrapper_t* w = (nqs_marshall_wrapper_t*)user;
+  double la = 0.0, ar = 0.0;
+  w->base_log_amp(spins, num_sites, w->base_user, &la, &ar);
+  const int par = nqs_marshall_parity(spins, w->size_x, w->size_y);
+  if (out_log_abs) *out_log_abs = la;
+  if (out_arg) *out_arg = ar + (par ? 3.14159265358979323846 : 0.0);
+}
+
+// ===========================================================================
+// sampler
+// ===========================================================================
+struct nqs_sampler {
+  int N = 0;
+  nqs_config_t cfg;
+  nqs_hip_options_t opt;
+  nqs_log_amp_fn_t fn = nullptr;
+  void* user = nullptr;
+  int device = 0;
+  AmpSource src;
+  // device chains
+  nqs::ChainState cs;
+  DBuf d_spins, d_rng, d_acc, d_tot, d_cur, d_samples;
+  // host-callback chains
+  std::vector<int> hc, hp;
+  std::vector<double> hla;
+  std::vector<uint64_t> hrng;
+  long long hacc = 0, htot = 0;
+  std::vector<int> h_cur;  // chain 0 configuration returned to callers
+  // last SR step bookkeeping
+  std::vector<double> last_E;
+  int last_rounds = 0;
+  double stage_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+  int have_stage = 0;
+};
+
+namespace {
+
+uint64_t chain_seed(unsigned rng_seed, int c) {
+  const uint64_t s0 = rng_seed ? (uint64_t)rng_seed : 0x9E3779B97F4A7C15ULL;
+  if (c == 0) return s0;
+  uint64_t s = nqs::splitmix64(s0 ^ ((uint64_t)c * 0xD1B54A32D192ED03ULL));
+  return s ? s : 1ULL;
+}
+
+int sampler_sync_cur(nqs_sampler* s) {
+  if (!s->device) {
+    memcpy(s->h_cur.data(), s->hc.data(), sizeof(int) * s->N);
+    return 0;
+  }
+  CKR(nqs::launch_unpack(s->cs, 0, s->d_cur.as<int>(), g_stream));
+  CK(hipMemcpyAsync(s->h_cur.data(), s->d_cur.p, sizeof(int) * s->N, hipMemcpyDeviceToHost,
+                    g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+// one move of host chain c (nqs_sampler.c:86-117 and the exchange extension)
+void host_step(nqs_sampler* s, int c) {
+  const int N = s->N;
+  int* x = s->hc.data() + (size_t)c * N;
+  uint64_t& st = s->hrng[c];
+  memcpy(s->hp.data(), x, sizeof(int) * N);
+  if (s->opt.move == NQS_HIP_MOVE_EXCHANGE) {
+    int nu = 0;
+    for (int i = 0; i < N; i++) nu += (x[i] > 0);
+    const int nd = N - nu;
+    if (nu == 0 || nd == 0) { s->htot++; return; }
+    const int k = (int)(nqs::xs_next(st) % (uint64_t)nu);
+    const int l = (int)(nqs::xs_next(st) % (uint64_t)nd);
+    int iu = -1, id = -1;
+    for (int i = 0, cu = 0, cd = 0; i < N; i++) {
+      if (x[i] > 0) { if (cu++ == k) iu = i; }
+      else { if (cd++ == l) id = i; }
+    }
+    s->hp[iu] = -s->hp[iu];
+    s->hp[id] = -s->hp[id];
+  } else {
+    const int site = (int)(nqs::xs_next(st) % (uint64_t)N);
+    s->hp[site] = -s->hp[site];
+  }
+  double pla = 0.0, par = 0.0;
+  s->fn(s->hp.data(), N, s->user, &pla, &par);
+  g_host_calls++;
+  const double lr = 2.0 * (pla - s->hla[c]);
+  bool acc = (lr >= 0.0) ? true : (log(nqs::xs_uniform(st)) < lr);
+  s->htot++;
+  if (acc) {
+    s->hacc++;
+    memcpy(x, s->hp.data(), sizeof(int) * N);
+    s->hla[c] = pla;
+  }
+}
+
+int ensure_model_for_sampler(nqs_sampler* s) {
+  CKR(ansatz_upload(s->src.a));
+  return 0;
+}
+
+// run `steps` moves per round for `rounds` rounds on every chain, optionally
+// recording samples r*C + c < n_out into d_out (device) / h_out (host)
+int sampler_advance(nqs_sampler* s, int steps, int rounds, int n_out, int* d_out, int* h_out) {
+  if (rounds <= 0) return 0;
+  if (s->device) {
+    CKR(ensure_model_for_sampler(s));
+    nqs::MHLaunch ml;
+    ml.move = s->opt.move == NQS_HIP_MOVE_EXCHANGE ? 1 : 0;
+    ml.steps = steps;
+    ml.rounds = rounds;
+    ml.record = d_out ? 1 : 0;
+    ml.n_out = n_out;
+    ml.out = d_out;
+    if (s->opt.precision == NQS_HIP_FP32) {
+      ModelView<float> v;
+      CKR(model_view<float>(s->src.a, v));
+      CKR(nqs::launch_mh<float>(v, s->cs, ml, g_stream));
+    } else {
+      ModelView<double> v;
+      CKR(model_view<double>(s->src.a, v));
+      CKR(nqs::launch_mh<double>(v, s->cs, ml, g_stream));
+    }
+    if (h_out && d_out) {
+      CK(hipMemcpyAsync(h_out, d_out, sizeof(int) * (size_t)n_out * s->N, hipMemcpyDeviceToHost,
+                        g_stream));
+    }
+    return sampler_sync_cur(s);
+  }
+  const int C = s->opt.num_chains;
+  for (int r = 0; r < rounds; r++) {
+    for (int c = 0; c < C; c++)
+      for (int k = 0; k < steps; k++) host_step(s, c);
+    if (h_out) {
+      for (int c = 0; c < C; c++) {
+        const long b = (long)r * C + c;
+        if (b < n_out) memcpy(h_out + b * s->N, s->hc.data() + (size_t)c * s->N, sizeof(int) * s->N);
+      }
+    }
+  }
+  if (d_out && h_out)
+    CK(hipMemcpyAsync(d_out, h_out, sizeof(int) * (size_t)n_out * s->N, hipMemcpyHostToDevice,
+                      g_stream));
+  return sampler_sync_cur(s);
+}
+
+nqs_sampler* sampler_new(int N, const nqs_config_t* cfg, const nqs_hip_options_t* opt,
+                         nqs_log_amp_fn_t fn, void* user) {
+  if (N <= 0 || !cfg || !fn) return nullptr;
+  if (stream_ready()) return nullptr;
+  nqs_sampler* s = new nqs_sampler();
+  s->N = N;
+  s->cfg = *cfg;
+  s->opt = opt ? *opt : nqs_hip_options_defaults();
+  if (!opt) {
+    const char* env = getenv("NQS_HIP_CHAINS");
+    if (env && atoi(env) > 0) s->opt.num_chains = atoi(env);
+  }
+  if (s->opt.num_chains < 1) s->opt.num_chains = 1;
+  if (s->opt.move == NQS_HIP_MOVE_LEARNED) {
+    fprintf(stderr, "nqs_hip: the learned proposal is driven through nqs_hip_sampler_create_ex "
+                    "with move=NQS_HIP_MOVE_LEARNED only on the device path\n");
+  }
+  s->fn = fn;
+  s->user = user;
+  s->h_cur.assign(N, 1);
+  const int C = s->opt.num_chains;
+  s->device = recognise(fn, user, s->src) ? 1 : 0;
+  if (s->device && s->src.a->N != N) s->device = 0;
+  if (s->device && (N > nqs::kMaxSites || s->src.a->M > nqs::kMaxHidden)) s->device = 0;
+  if (s->device) {
+    const int nw = (N + 63) / 64;
+    if (s->d_spins.ensure(sizeof(uint64_t) * (size_t)C * nw) ||
+        s->d_rng.ensure(sizeof(uint64_t) * C) || s->d_acc.ensure(sizeof(uint64_t) * C) ||
+        s->d_tot.ensure(sizeof(uint64_t) * C) || s->d_cur.ensure(sizeof(int) * N)) {
+      delete s;
+      return nullptr;
+    }
+    s->cs.C = C;
+    s->cs.N = N;
+    s->cs.nw = nw;
+    s->cs.spins = s->d_spins.as<uint64_t>();
+    s->cs.rng = s->d_rng.as<uint64_t>();
+    s->cs.acc = s->d_acc.as<unsigned long long>();
+    s->cs.tot = s->d_tot.as<unsigned long long>();
+    std::vector<uint64_t> seeds(C);
+    for (int c = 0; c < C; c++) seeds[c] = chain_seed(cfg->rng_seed, c);
+    DBuf tmp;
+    if (tmp.ensure(sizeof(uint64_t) * C) ||
+        hipMemcpyAsync(tmp.p, seeds.data(), sizeof(uint64_t) * C, hipMemcpyHostToDevice, g_stream) ||
+        nqs::launch_chain_init(s->cs, tmp.as<uint64_t>(), s->opt.sz_zero_start, g_stream) ||
+        sampler_sync_cur(s)) {
+      delete s;
+      return nullptr;
+    }
+    (void)hipStreamSynchronize(g_stream);
+    tmp.release();
+    return s;
+  }
+  // host-callback chains: the reference algorithm against the caller's function
+  s->hc.assign((size_t)C * N, 1);
+  s->hp.assign(N, 1);
+  s->hla.assign(C, 0.0);
+  s->hrng.assign(C, 0);
+  for (int c = 0; c < C; c++) {
+    uint64_t st = chain_seed(cfg->rng_seed, c);
+    int* x = s->hc.data() + (size_t)c * N;
+    if (s->opt.sz_zero_start) {
+      for (int i = 0; i < N; i++) x[i] = (i < N / 2) ? 1 : -1;
+      for (int i = N - 1; i > 0; i--) {
+        const int j = (int)(nqs::xs_next(st) % (uint64_t)(i + 1));
+        std::swap(x[i], x[j]);
+      }
+    } else {
+      for (int i = 0; i < N; i++) x[i] = (nqs::xs_next(st) & 1) ? 1 : -1;
+    }
+    s->hrng[c] = st;
+    double ar = 0.0;
+    fn(x, N, user, &s->hla[c], &ar);
+    g_host_calls++;
+  }
+  memcpy(s->h_cur.data(), s->hc.data(), sizeof(int) * N);
+  return s;
+}
+
+}  // namespace
+
+NQS_EXPORT nqs_hip_options_t nqs_hip_options_defaults(void) {
+  nqs_hip_options_t o;
+  memset(&o, 0, sizeof(o));
+  o.num_chains = 1;
+  o.move = NQS_HIP_MOVE_FLIP;
+  o.sz_zero_start = 0;
+  o.periodic = 0;
+  o.precision = NQS_HIP_FP64;
+  o.gram = NQS_HIP_GRAM_DIRECT;
+  o.ar_hidden = 64;
+  o.ar_train_every = 10;
+  o.ar_train_window = 16384;
+  o.ar_epochs = 4;
+  o.ar_batch = 256;
+  o.ar_lr = 1e-3;
+  o.ar_mix = 0.5;
+  return o;
+}
+
+NQS_EXPORT nqs_sampler_t* nqs_sampler_create(int num_sites, const nqs_config_t* cfg,
+                                             nqs_log_amp_fn_t log_amp, void* log_amp_user) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  return sampler_new(num_sites, cfg, nullptr, log_amp, log_amp_user);
+}
+
+NQS_EXPORT nqs_sampler_t* nqs_hip_sampler_create_ex(int num_sites, const nqs_config_t* cfg,
+                                                    const nqs_hip_options_t* opt,
+                                                    nqs_log_amp_fn_t log_amp, void* user) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!opt) return nullptr;
+  if (opt->move == NQS_HIP_MOVE_LEARNED) {
+    fprintf(stderr, "nqs_hip: learned proposal is not available in this build\n");
+    return nullptr;
+  }
+  return sampler_new(num_sites, cfg, opt, log_amp, user);
+}
+
+NQS_EXPORT int nqs_hip_sampler_get_options(const nqs_sampler_t* s, nqs_hip_options_t* out) {
+  if (!s || !out) return -1;
+  *out = s->opt;
+  return 0;
+}
+
+NQS_EXPORT void nqs_sampler_free(nqs_sampler_t* s) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!s) return;
+  (void)hipStreamSynchronize(g_stream);
+  delete s;
+}
+
+NQS_EXPORT void nqs_sampler_thermalize(nqs_sampler_t* s) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!s) return;
+  (void)sampler_advance(s, s->cfg.num_thermalize, s->cfg.num_thermalize > 0 ? 1 : 0, 0, nullptr,
+                        nullptr);
+}
+
+NQS_EXPORT const int* nqs_sampler_next(nqs_sampler_t* s) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!s) return nullptr;
+  const int k = s->cfg.num_decorrelate > 0 ? s->cfg.num_decorrelate : 1;
+  if (sampler_advance(s, k, 1, 0, nullptr, nullptr)) return nullptr;
+  return s->h_cur.data();
+}
+
+NQS_EXPORT int nqs_sampler_batch(nqs_sampler_t* s, int n, int* out) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!s || !out || n <= 0) return -1;
+  const int C = s->opt.num_chains;
+  const int k = s->cfg.num_decorrelate > 0 ? s->cfg.num_decorrelate : 1;
+  const int rounds = (n + C - 1) / C;
+  if (s->d_samples.ensure(sizeof(int) * (size_t)n * s->N)) return -1;
+  return sampler_advance(s, k, rounds, n, s->d_samples.as<int>(), out);
+}
+
+NQS_EXPORT double nqs_sampler_acceptance_ratio(const nqs_sampler_t* s) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!s) return 0.0;
+  if (!s->device) return s->htot ? (double)s->hacc / (double)s->htot : 0.0;
+  const int C = s->cs.C;
+  std::vector<unsigned long long> a(C), t(C);
+  if (hipMemcpyAsync(a.data(), s->cs.acc, sizeof(uint64_t) * C, hipMemcpyDeviceToHost, g_stream) ||
+      hipMemcpyAsync(t.data(), s->cs.tot, sizeof(uint64_t) * C, hipMemcpyDeviceToHost, g_stream) ||
+      hipStreamSynchronize(g_stream))
+    return 0.0;
+  unsigned long long sa = 0, stt = 0;
+  for (int c = 0; c < C; c++) { sa += a[c]; stt += t[c]; }
+  return stt ? (double)sa / (double)stt : 0.0;
+}
+
+NQS_EXPORT const int* nqs_sampler_current(const nqs_sampler_t* s) {
+  return s ? s->h_cur.data() : nullptr;
+}
+
+NQS_EXPORT int nqs_sampler_num_sites(const nqs_sampler_t* s) { return s ? s->N : 0; }
+
+// ===========================================================================
+// local energy
+// ===========================================================================
+namespace {
+
+int local_energy_any(const nqs_config_t* cfg, int Lx, int Ly, int per, int prec, const int* spins,
+                     int B, nqs_log_amp_fn_t fn, void* user, double* re, double* im) {
+  AmpSource src;
+  const int N = nqs::ham_sites(cfg, Lx, Ly);
+  CKR(upload_spins(spins, (long)B * N));
+  CKR(g_ws.e_re.ensure(sizeof(double) * B));
+  CKR(g_ws.e_im.ensure(sizeof(double) * B));
+  if (recognise(fn, user, src) && src.a->N == N) {
+    CKR(ansatz_upload(src.a));
+    CKR(eloc_device(cfg, Lx, Ly, per, prec, src, g_ws.spins.as<int>(), B, g_ws.e_re.as<double>(),
+                    g_ws.e_im.as<double>()));
+  } else {
+    CKR(eloc_callback(cfg, Lx, Ly, per, spins, B, fn, user, g_ws.e_re.as<double>(),
+                      g_ws.e_im.as<double>(), g_ws.spins.as<int>()));
+  }
+  if (re) CK(hipMemcpyAsync(re, g_ws.e_re.p, sizeof(double) * B, hipMemcpyDeviceToHost, g_stream));
+  if (im) CK(hipMemcpyAsync(im, g_ws.e_im.p, sizeof(double) * B, hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+}  // namespace
+
+NQS_EXPORT double nqs_local_energy(const nqs_config_t* cfg, int size_x, int size_y, const int* spins,
+                                   nqs_log_amp_fn_t log_amp, void* log_amp_user) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !spins || !log_amp) return 0.0;
+  if (stream_ready()) return 0.0;
+  double re = 0.0;
+  if (local_energy_any(cfg, size_x, size_y, 0, NQS_HIP_FP64, spins, 1, log_amp, log_amp_user, &re,
+                       nullptr))
+    return 0.0;
+  return re;
+}
+
+NQS_EXPORT void nqs_local_energy_batch(const nqs_config_t* cfg, int size_x, int size_y,
+                                       const int* spins_batch, int batch_size,
+                                       nqs_log_amp_fn_t log_amp, void* log_amp_user,
+                                       double* out_energies) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !spins_batch || !out_energies || batch_size <= 0 || !log_amp) return;
+  if (stream_ready()) return;
+  if (local_energy_any(cfg, size_x, size_y, 0, NQS_HIP_FP64, spins_batch, batch_size, log_amp,
+                       log_amp_user, out_energies, nullptr))
+    for (int i = 0; i < batch_size; i++) out_energies[i] = 0.0;
+}
+
+NQS_EXPORT void nqs_local_energy_complex(const nqs_config_t* cfg, int size_x, int size_y,
+                                         const int* spins, nqs_log_amp_fn_t log_amp,
+                                         void* log_amp_user, double* out_re, double* out_im) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !spins || !log_amp || !out_re || !out_im) {
+    if (out_re) *out_re = 0.0;
+    if (out_im) *out_im = 0.0;
+    return;
+  }
+  if (stream_ready() || local_energy_any(cfg, size_x, size_y, 0, NQS_HIP_FP64, spins, 1, log_amp,
+                                         log_amp_user, out_re, out_im)) {
+    *out_re = 0.0;
+    *out_im = 0.0;
+  }
+}
+
+NQS_EXPORT void nqs_local_energy_batch_complex(const nqs_config_t* cfg, int size_x, int size_y,
+                                               const int* spins_batch, int batch_size,
+                                               nqs_log_amp_fn_t log_amp, void* log_amp_user,
+                                               double* out_re, double* out_im) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !spins_batch || !out_re || !out_im || batch_size <= 0 || !log_amp) return;
+  if (stream_ready()) return;
+  if (local_energy_any(cfg, size_x, size_y, 0, NQS_HIP_FP64, spins_batch, batch_size, log_amp,
+                       log_amp_user, out_re, out_im))
+    for (int i = 0; i < batch_size; i++) out_re[i] = out_im[i] = 0.0;
+}
+
+NQS_EXPORT int nqs_hip_local_energy_batch_ex(const nqs_config_t* cfg, int size_x, int size_y,
+                                             const nqs_hip_options_t* opt, const int* spins, int B,
+                                             nqs_log_amp_fn_t fn, void* user, double* re,
+                                             double* im) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !spins || B <= 0 || !fn || !re) return -1;
+  if (stream_ready()) return -1;
+  const int per = opt ? opt->periodic : 0;
+  const int prec = opt ? opt->precision : NQS_HIP_FP64;
+  return local_energy_any(cfg, size_x, size_y, per, prec, spins, B, fn, user, re, im);
+}
+
+// ===========================================================================
+// stochastic reconfiguration
+// ===========================================================================
+namespace {
+
+enum { SR_CG = 0, SR_HOLO = 1, SR_MINSR = 2 };
+
+struct StepTimer {
+  hipEvent_t ev[7];
+  int n = 0;
+  bool on = false;
+  void init() {
+    on = true;
+    for (int i = 0; i < 7; i++)
+      if (hipEventCreate(&ev[i]) != hipSuccess) on = false;
+  }
+  void mark() {
+    if (on && n < 7) (void)hipEventRecord(ev[n++], g_stream);
+  }
+  void finish(nqs_sampler* s) {
+    if (!on) return;
+    (void)hipEventSynchronize(ev[n - 1]);
+    double tot = 0.0;
+    for (int i = 0; i + 1 < n && i < 6; i++) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+      s->stage_ms[i] = ms;
+      tot += ms;
+    }
+    s->stage_ms[6] = tot;
+    s->have_stage = 1;
+    for (int i = 0; i < 7; i++) (void)hipEventDestroy(ev[i]);
+  }
+};
+
+// CG solve of (A + eps I) x = F on the device; A applied through `apply`.
+template <typename Apply>
+int cg_device(long P, const double* F, double* x, int max_iters, double tol, Apply apply,
+              int& iters, int& converged, int& breakdown, double& pAp_out, double& rs_ratio) {
+  CKR(g_ws.cg.ensure(sizeof(nqs::CGScalars)));
+  nqs::CGScalars* sc = g_ws.cg.as<nqs::CGScalars>();
+  double* base = g_ws.vecs.as<double>();
+  double* r = base + 2 * P;
+  double* p = base + 3 * P;
+  double* Ap = base + 4 * P;
+  CKR(nqs::launch_cg_init(F, P, x, r, p, sc, g_stream));
+  nqs::CGScalars h;
+  int it = 0;
+  for (; it < max_iters; it++) {
+    CKR(apply(p, Ap));
+    CKR(nqs::launch_cg_alpha(sc, p, Ap, P, g_stream));
+    CKR(nqs::launch_cg_update(sc, x, r, p, Ap, P, tol, g_stream));
+    CKR(nqs::launch_cg_direction(sc, p, r, P, g_stream));
+    if ((it & 7) == 7 || it + 1 == max_iters) {
+      CK(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, g_stream));
+      CK(hipStreamSynchronize(g_stream));
+      if (h.done) break;
+    }
+  }
+  CK(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  iters = h.iters;
+  converged = h.converged;
+  breakdown = h.breakdown;
+  pAp_out = h.pAp;
+  rs_ratio = h.rs0 > 0.0 ? sqrt(h.rs / h.rs0) : 0.0;
+  if (max_iters <= 0) { iters = 0; converged = 0; breakdown = 0; }
+  return 0;
+}
+
+int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz* a,
+                 nqs_sampler* s, nqs_log_amp_fn_t fn, void* user, nqs_gradient_fn_t gfn,
+                 nqs_holomorphic_gradient_fn_t hfn, void* guser, nqs_sr_step_info_t* info,
+                 bool in_run) {
+  const int N = s->N;
+  const long P = a->P;
+  const int B = cfg->num_samples;
+  if (P <= 0 || B <= 0) return -1;
+  if (!fn) { fn = nqs_ansatz_log_amp; user = a; }
+  const int holo = (method == SR_HOLO);
+  const int prec = s->opt.precision;
+  StepTimer tm;
+  if (s->device) tm.init();
+  if (!in_run) a->dev_fresh = 0;
+  CKR(ansatz_upload(a));
+  a->dev_fresh = 1;
+  tm.mark();
+  // ---- 1. samples ----------------------------------------------------------
+  const int C = s->opt.num_chains;
+  const int k = s->cfg.num_decorrelate > 0 ? s->cfg.num_decorrelate : 1;
+  const int rounds = (B + C - 1) / C;
+  CKR(s->d_samples.ensure(sizeof(int) * (size_t)B * N));
+  std::vector<int> h_samples;
+  const bool need_host_samples = !s->device || gfn || hfn;
+  AmpSource esrc;
+  const bool e_dev = recognise(fn, user, esrc) && esrc.a->N == N;
+  if (need_host_samples || !e_dev) h_samples.resize((size_t)B * N);
+  CKR(sampler_advance(s, k, rounds, B, s->d_samples.as<int>(),
+                      h_samples.empty() ? nullptr : h_samples.data()));
+  if (!h_samples.empty()) CK(hipStreamSynchronize(g_stream));
+  const int* d_samples = s->d_samples.as<int>();
+  tm.mark();
+  // ---- 2. local energies -----------------------------------------------------
+  CKR(g_ws.e_re.ensure(sizeof(double) * B));
+  CKR(g_ws.e_im.ensure(sizeof(double) * B));
+  double* E = g_ws.e_re.as<double>();
+  double* Ei = g_ws.e_im.as<double>();
+  if (e_dev) {
+    CKR(ansatz_upload(esrc.a));
+    CKR(eloc_device(cfg, Lx, Ly, s->opt.periodic, prec, esrc, d_samples, B, E, Ei));
+  } else {
+    CKR(eloc_callback(cfg, Lx, Ly, s->opt.periodic, h_samples.data(), B, fn, user, E, Ei,
+                      d_samples));
+  }
+  tm.mark();
+  // ---- 3. O matrix -----------------------------------------------------------
+  const size_t ob = sizeof(double) * (size_t)B * (size_t)P;
+  CKR(g_ws.O.ensure(ob));
+  if (holo) CKR(g_ws.Oi.ensure(ob));
+  double* O = g_ws.O.as<double>();
+  double* Oi = holo ? g_ws.Oi.as<double>() : nullptr;
+  if ((!holo && gfn) || (holo && hfn)) {
+    std::vector<double> hr((size_t)B * P), hi(holo ? (size_t)B * P : 0);
+    for (int b = 0; b < B; b++) {
+      const int* sp = h_samples.data() + (size_t)b * N;
+      if (holo) hfn(guser, a, sp, N, hr.data() + (size_t)b * P, hi.data() + (size_t)b * P);
+      else gfn(guser, a, sp, N, hr.data() + (size_t)b * P);
+      g_host_calls++;
+    }
+    CK(hipMemcpyAsync(O, hr.data(), ob, hipMemcpyHostToDevice, g_stream));
+    if (holo) CK(hipMemcpyAsync(Oi, hi.data(), ob, hipMemcpyHostToDevice, g_stream));
+    CK(hipStreamSynchronize(g_stream));
+  } else {
+    if (prec == NQS_HIP_FP32) {
+      CKR(forward_t<float>(a, d_samples, B, nullptr, nullptr));
+      ModelView<float> v;
+      CKR(model_view<float>(a, v));
+      CKR(nqs::launch_gen_O<float>(v, a->kind, d_samples, B, g_ws.t_re.as<float>(),
+                                   g_ws.t_im.as<float>(), holo, O, Oi, P, g_stream));
+    } else {
+      CKR(forward_t<double>(a, d_samples, B, nullptr, nullptr));
+      ModelView<double> v;
+      CKR(model_view<double>(a, v));
+      CKR(nqs::launch_gen_O<double>(v, a->kind, d_samples, B, g_ws.t_re.as<double>(),
+                                    g_ws.t_im.as<double>(), holo, O, Oi, P, g_stream));
+    }
+  }
+  // ---- 4. means and force ----------------------------------------------------
+  CKR(g_ws.vecs.ensure(sizeof(double) * 9 * (size_t)P));
+  CKR(g_ws.colwork.ensure(sizeof(double) * nqs::colsum_work_elems(B, P)));
+  CKR(g_ws.scal.ensure(sizeof(double) * 16));
+  CKR(g_ws.u1.ensure(sizeof(double) * B));
+  CKR(g_ws.u2.ensure(sizeof(double) * B));
+  double* sc = g_ws.scal.as<double>();      // [0] <E_re> [1] <E_re^2> [2] <E_im> [3,4] u means
+  double* vb = g_ws.vecs.as<double>();
+  double* delta = vb;                        // [0, P)
+  double* F = vb + P;                        // [P, 2P)
+  double* Om = vb + 5 * P;                   // means of O (R)
+  double* Im = vb + 6 * P;                   // means of I
+  double* c1 = vb + 7 * P;
+  double* c2 = vb + 8 * P;
+  double* work = g_ws.colwork.as<double>();
+  CKR(nqs::launch_sum(E, B, nullptr, 1.0 / B, sc + 0, g_stream));
+  CKR(nqs::launch_sum(E, B, E, 1.0 / B, sc + 1, g_stream));
+  CKR(nqs::launch_sum(Ei, B, nullptr, 1.0 / B, sc + 2, g_stream));
+  CKR(nqs::launch_colsum(O, B, P, nullptr, 1.0 / B, Om, work, g_stream));
+  if (holo) {
+    CKR(nqs::launch_colsum(Oi, B, P, nullptr, 1.0 / B, Im, work, g_stream));
+    CKR(nqs::launch_colsum(O, B, P, E, 1.0 / B, c1, work, g_stream));
+    CKR(nqs::launch_colsum(Oi, B, P, Ei, 1.0 / B, c2, work, g_stream));
+    CKR(nqs::launch_qgt_finish(F, c1, Om, sc + 0, c2, Im, sc + 2, 2.0, 0.0, c1, P, g_stream));
+  } else if (method == SR_CG) {
+    CKR(nqs::launch_colsum(O, B, P, E, 1.0 / B, c1, work, g_stream));
+    CKR(nqs::launch_qgt_finish(F, c1, Om, sc + 0, nullptr, nullptr, nullptr, 1.0, 0.0, c1, P,
+                               g_stream));
+  }
+  tm.mark();
+  // ---- 5. solve ----------------------------------------------------------------
+  int iters = 0, converged = 0;
+  const double eps = cfg->sr_diag_shift;
+  if (method == SR_MINSR) {
+    double* u = g_ws.u1.as<double>();
+    // eps_i = E_i - <E>, O_c = O - <O>  (nqs_optimizer.c:1119-1126)
+    CKR(nqs::launch_sub_scalar(u, E, B, sc + 0, g_stream));
+    CKR(nqs::launch_center(O, B, P, Om, g_stream));
+    CKR(g_ws.T.ensure(sizeof(double) * (size_t)B * B));
+    CKR(g_ws.diag.ensure(sizeof(double) * B));
+    CKR(g_ws.y.ensure(sizeof(double) * B * 2));
+    CKR(g_ws.fail.ensure(sizeof(int) * 4));
+    double* T = g_ws.T.as<double>();
+    // T = O_c O_c^T / Ns (lower triangle on MFMA, mirrored), + eps I
+    CKR(nqs::launch_gemm_nt(B, B, (int)P, 1.0 / B, O, P, O, P, 0.0, T, B, 1, g_stream));
+    CKR(nqs::launch_mirror_lower(T, B, g_stream));
+    CKR(nqs::launch_add_diag(T, B, B, eps, g_stream));
+    CKR(nqs::launch_copy_diag(T, B, g_ws.diag.as<double>(), g_stream));
+    int f = nqs::launch_potrf(T, B, g_ws.diag.as<double>(), g_ws.fail.as<int>(), g_stream);
+    if (f < 0) return -1;
+    if (f > 0) {
+      // retry on the partially factored matrix with +1e-6 on the diagonal
+      CKR(nqs::launch_add_diag(T, B, B, 1e-6, g_stream));
+      f = nqs::launch_potrf(T, B, nullptr, g_ws.fail.as<int>(), g_stream);
+      if (f < 0) return -1;
+    }
+    double* y = g_ws.y.as<double>();
+    if (f == 0) {
+      CKR(nqs::launch_potrs(T, B, u, y, y + B, g_stream));
+      converged = 1;
+    } else {
+      fprintf(stderr, "nqs_sr_step_minsr: Cholesky breakdown after retry; increase "
+                      "sr_diag_shift or num_samples. Update set to 0.\n");
+      CKR(nqs::launch_fill(y, 0.0, B, g_stream));
+    }
+    // delta_k = (1/Ns) sum_i O_c[i][k] y_i
+    CKR(nqs::launch_colsum(O, B, P, y, 1.0 / B, delta, work, g_stream));
+  } else {
+    double* u1 = g_ws.u1.as<double>();
+    double* u2 = g_ws.u2.as<double>();
+    auto apply = [&](const double* v, double* out) -> int {
+      CKR(nqs::launch_rowdot(O, B, P, v, u1, g_stream));
+      CKR(nqs::launch_sum(u1, B, nullptr, 1.0 / B, sc + 3, g_stream));
+      CKR(nqs::launch_colsum(O, B, P, u1, 1.0 / B, c1, work, g_stream));
+      if (holo) {
+        CKR(nqs::launch_rowdot(Oi, B, P, v, u2, g_stream));
+        CKR(nqs::launch_sum(u2, B, nullptr, 1.0 / B, sc + 4, g_stream));
+        CKR(nqs::launch_colsum(Oi, B, P, u2, 1.0 / B, c2, work, g_stream));
+        return nqs::launch_qgt_finish(out, c1, Om, sc + 3, c2, Im, sc + 4, 2.0, eps, v, P,
+                                      g_stream);
+      }
+      return nqs::launch_qgt_finish(out, c1, Om, sc + 3, nullptr, nullptr, nullptr, 1.0, eps, v,
+                                    P, g_stream);
+    };
+    int breakdown = 0;
+    double pAp = 0.0, rr = 0.0;
+    CKR(cg_device(P, F, delta, cfg->sr_cg_max_iters, cfg->sr_cg_tol, apply, iters, converged,
+                  breakdown, pAp, rr));
+    if (breakdown && !holo)
+      fprintf(stderr, "nqs_sr cg_solve: Krylov breakdown at iter=%d, pAp=%.3e, residual/rhs=%.3e "
+                      "(tol=%.3e). Returning unconverged partial step.\n",
+              iters, pAp, rr, cfg->sr_cg_tol);
+  }
+  tm.mark();
+  // ---- 6. update -----------------------------------------------------------------
+  CKR(nqs::launch_axpy(a->d_par.as<double>(), delta, -cfg->learning_rate, P, g_stream));
+  a->m64.valid = 0;
+  a->m32.valid = 0;
+  CKR(nqs::launch_sum(delta, P, delta, 1.0, sc + 5, g_stream));
+  double hs[6];
+  CK(hipMemcpyAsync(hs, sc, sizeof(double) * 6, hipMemcpyDeviceToHost, g_stream));
+  CKR(ansatz_download(a));
+  tm.mark();
+  if (info) {
+    info->mean_energy = hs[0];
+    info->variance_energy = hs[1] - hs[0] * hs[0];
+    info->update_norm = sqrt(hs[5]) * cfg->learning_rate;
+    info->acceptance_ratio = nqs_sampler_acceptance_ratio(s);
+    info->cg_iterations = (method == SR_MINSR) ? 0 : iters;
+    info->converged = converged;
+  }
+  // per-chain energy series for tau_int
+  s->last_E.resize(B);
+  CK(hipMemcpyAsync(s->last_E.data(), E, sizeof(double) * B, hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  s->last_rounds = rounds;
+  tm.finish(s);
+  if (!in_run) a->dev_fresh = 0;
+  return 0;
+}
+
+int sr_run_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz* a, nqs_sampler* s,
+                nqs_log_amp_fn_t fn, void* user, nqs_gradient_fn_t gfn,
+                nqs_holomorphic_gradient_fn_t hfn, void* guser, double* trace) {
+  // a host callback may read the parameters between steps: keep the host
+  // copy authoritative then
+  AmpSource src;
+  const bool pure_device = s->device && (!fn || recognise(fn, user, src)) && !gfn && !hfn;
+  a->dev_fresh = 0;
+  sampler_advance(s, s->cfg.num_thermalize, s->cfg.num_thermalize > 0 ? 1 : 0, 0, nullptr, nullptr);
+  for (int it = 0; it < cfg->num_iterations; it++) {
+    nqs_sr_step_info_t info;
+    int rc = sr_step_impl(method, cfg, Lx, Ly, a, s, fn, user, gfn, hfn, guser, &info, pure_device);
+    if (rc) {
+      a->dev_fresh = 0;
+      return rc;
+    }
+    if (trace) trace[it] = info.mean_energy;
+  }
+  a->dev_fresh = 0;
+  return 0;
+}
+
+}  // namespace
+
+NQS_EXPORT int nqs_sr_step_custom_full(const nqs_config_t* cfg, int size_x, int size_y,
+                                       nqs_ansatz_t* ansatz, nqs_sampler_t* sampler,
+                                       nqs_log_amp_fn_t log_amp_fn, void* log_amp_user,
+                                       nqs_gradient_fn_t gradient_fn, void* grad_user,
+                                       nqs_sr_step_info_t* out_info) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !ansatz || !sampler) return -1;
+  return sr_step_impl(SR_CG, cfg, size_x, size_y, ansatz, sampler, log_amp_fn, log_amp_user,
+                      gradient_fn, nullptr, grad_user, out_info, false);
+}
+
+NQS_EXPORT int nqs_sr_step_custom(const nqs_config_t* cfg, int size_x, int size_y,
+                                  nqs_ansatz_t* ansatz, nqs_sampler_t* sampler,
+                                  nqs_log_amp_fn_t log_amp_fn, void* log_amp_user,
+                                  nqs_sr_step_info_t* out_info) {
+  return nqs_sr_step_custom_full(cfg, size_x, size_y, ansatz, sampler, log_amp_fn, log_amp_user,
+                                 nullptr, nullptr, out_info);
+}
+
+NQS_EXPORT int nqs_sr_step(const nqs_config_t* cfg, int size_x, int size_y, nqs_ansatz_t* ansatz,
+                           nqs_sampler_t* sampler, nqs_sr_step_info_t* out_info) {
+  return nqs_sr_step_custom_full(cfg, size_x, size_y, ansatz, sampler, nullptr, nullptr, nullptr,
+                                 nullptr, out_info);
+}
+
+NQS_EXPORT int nqs_sr_run_custom_full(const nqs_config_t* cfg, int size_x, int size_y,
+                                      nqs_ansatz_t* ansatz, nqs_sampler_t* sampler,
+                                      nqs_log_amp_fn_t log_amp_fn, void* log_amp_user,
+                                      nqs_gradient_fn_t gradient_fn, void* grad_user,
+                                      double* out_energy_trace) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !ansatz || !sampler) return -1;
+  return sr_run_impl(SR_CG, cfg, size_x, size_y, ansatz, sampler, log_amp_fn, log_amp_user,
+                     gradient_fn, nullptr, grad_user, out_energy_trace);
+}
+
+NQS_EXPORT int nqs_sr_run_custom(const nqs_config_t* cfg, int size_x, int size_y,
+                                 nqs_ansatz_t* ansatz, nqs_sampler_t* sampler,
+                                 nqs_log_amp_fn_t log_amp_fn, void* log_amp_user,
+                                 double* out_energy_trace) {
+  return nqs_sr_run_custom_full(cfg, size_x, size_y, ansatz, sampler, log_amp_fn, log_amp_user,
+                                nullptr, nullptr, out_energy_trace);
+}
+
+NQS_EXPORT int nqs_sr_run(const nqs_config_t* cfg, int size_x, int size_y, nqs_ansatz_t* ansatz,
+                          nqs_sampler_t* sampler, double* out_energy_trace) {
+  return nqs_sr_run_custom_full(cfg, size_x, size_y, ansatz, sampler, nullptr, nullptr, nullptr,
+                                nullptr, out_energy_trace);
+}
+
+NQS_EXPORT int nqs_sr_step_holomorphic_full(const nqs_config_t* cfg, int size_x, int size_y,
+                                            nqs_ansatz_t* ansatz, nqs_sampler_t* sampler,
+                                            nqs_log_amp_fn_t log_amp_fn, void* log_amp_user,
+                                            nqs_holomorphic_gradient_fn_t grad_fn,
+                                            void* grad_user, nqs_sr_step_info_t* out_info) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !ansatz || !sampler) return -1;
+  return sr_step_impl(SR_HOLO, cfg, size_x, size_y, ansatz, sampler, log_amp_fn, log_amp_user,
+                      nullptr, grad_fn, grad_user, out_info, false);
+}
+
+NQS_EXPORT int nqs_sr_step_holomorphic(const nqs_config_t* cfg, int size_x, int size_y,
+                                       nqs_ansatz_t* ansatz, nqs_sampler_t* sampler,
+                                       nqs_log_amp_fn_t log_amp_fn, void* log_amp_user,
+                                       nqs_sr_step_info_t* out_info) {
+  return nqs_sr_step_holomorphic_full(cfg, size_x, size_y, ansatz, sampler, log_amp_fn,
+                                      log_amp_user, nullptr, nullptr, out_info);
+}
+
+NQS_EXPORT int nqs_sr_run_holomorphic_full(const nqs_config_t* cfg, int size_x, int size_y,
+                                           nqs_ansatz_t* ansatz, nqs_sampler_t* sampler,
+                                           nqs_log_amp_fn_t log_amp_fn, void* log_amp_user,
+                                           nqs_holomorphic_gradient_fn_t grad_fn, void* grad_user,
+                                           double* out_energy_trace) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !ansatz || !sampler) return -1;
+  return sr_run_impl(SR_HOLO, cfg, size_x, size_y, ansatz, sampler, log_amp_fn, log_amp_user,
+                     nullptr, grad_fn, grad_user, out_energy_trace);
+}
+
+NQS_EXPORT int nqs_sr_run_holomorphic(const nqs_config_t* cfg, int size_x, int size_y,
+                                      nqs_ansatz_t* ansatz, nqs_sampler_t* sampler,
+                                      nqs_log_amp_fn_t log_amp_fn, void* log_amp_user,
+                                      double* out_energy_trace) {
+  return nqs_sr_run_holomorphic_full(cfg, size_x, size_y, ansatz, sampler, log_amp_fn,
+                                     log_amp_user, nullptr, nullptr, out_energy_trace);
+}
+
+NQS_EXPORT int nqs_sr_step_minsr_full(const nqs_config_t* cfg, int size_x, int size_y,
+                                      nqs_ansatz_t* ansatz, nqs_sampler_t* sampler,
+                                      nqs_log_amp_fn_t log_amp_fn, void* log_amp_user,
+                                      nqs_gradient_fn_t gradient_fn, void* grad_user,
+                                      nqs_sr_step_info_t* out_info) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !ansatz || !sampler) return -1;
+  return sr_step_impl(SR_MINSR, cfg, size_x, size_y, ansatz, sampler, log_amp_fn, log_amp_user,
+                      gradient_fn, nullptr, grad_user, out_info, false);
+}
+
+NQS_EXPORT int nqs_sr_run_minsr(const nqs_config_t* cfg, int size_x, int size_y,
+                                nqs_ansatz_t* ansatz, nqs_sampler_t* sampler,
+                                nqs_log_amp_fn_t log_amp_fn, void* log_amp_user,
+                                double* out_energy_trace) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !ansatz || !sampler) return -1;
+  return sr_run_impl(SR_MINSR, cfg, size_x, size_y, ansatz, sampler, log_amp_fn, log_amp_user,
+                     nullptr, nullptr, nullptr, out_energy_trace);
+}
+
+// ===========================================================================
+// extensions
+// ===========================================================================
+NQS_EXPORT int nqs_hip_sampler_last_energies(const nqs_sampler_t* s, double* out, int capacity) {
+  if (!s || !out || s->last_E.empty()) return -1;
+  const int C = s->opt.num_chains, R = s->last_rounds, B = (int)s->last_E.size();
+  if (capacity < C * R) return -1;
+  for (int c = 0; c < C; c++)
+    for (int r = 0; r < R; r++) {
+      const long b = (long)r * C + c;
+      out[(size_t)c * R + r] = b < B ? s->last_E[b] : NAN;
+    }
+  return R;
+}
+
+NQS_EXPORT int nqs_hip_sampler_rounds(const nqs_sampler_t* s) { return s ? s->last_rounds : -1; }
+
+NQS_EXPORT double nqs_hip_tau_int(const double* x, int n) {
+  // tau = 1/2 + sum_{t=1}^{W} rho(t), smallest W >= 5 tau(W) (Sokal)
+  if (!x || n < 4) return 0.5;
+  double m = 0.0;
+  for (int i = 0; i < n; i++) m += x[i];
+  m /= n;
+  double c0 = 0.0;
+  for (int i = 0; i < n; i++) c0 += (x[i] - m) * (x[i] - m);
+  c0 /= n;
+  if (c0 <= 0.0) return 0.5;
+  double tau = 0.5;
+  for (int t = 1; t < n; t++) {
+    double c = 0.0;
+    for (int i = 0; i + t < n; i++) c += (x[i] - m) * (x[i + t] - m);
+    c /= n;
+    tau += c / c0;
+    if (t >= 5.0 * tau) break;
+  }
+  return tau;
+}
+
+NQS_EXPORT int nqs_hip_last_stage_ms(const nqs_sampler_t* s, double out[7]) {
+  if (!s || !out || !s->have_stage) return -1;
+  for (int i = 0; i < 7; i++) out[i] = s->stage_ms[i];
+  return 0;
+}
+
+NQS_EXPORT int nqs_hip_gram(const double* O, int B, long P, double scale, double* T) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!O || !T || B <= 0 || P <= 0 || stream_ready()) return -1;
+  const size_t ob = sizeof(double) * (size_t)B * P, tb = sizeof(double) * (size_t)B * B;
+  if (g_ws.O.ensure(ob) || g_ws.T.ensure(tb)) return -1;
+  CK(hipMemcpyAsync(g_ws.O.p, O, ob, hipMemcpyHostToDevice, g_stream));
+  CKR(nqs::launch_gemm_nt(B, B, (int)P, scale, g_ws.O.as<double>(), P, g_ws.O.as<double>(), P, 0.0,
+                          g_ws.T.as<double>(), B, 1, g_stream));
+  CKR(nqs::launch_mirror_lower(g_ws.T.as<double>(), B, g_stream));
+  CK(hipMemcpyAsync(T, g_ws.T.p, tb, hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+NQS_EXPORT int nqs_hip_cholesky_solve(double* T, int n, const double* b, double* x) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!T || !b || !x || n <= 0 || stream_ready()) return -1;
+  const size_t tb = sizeof(double) * (size_t)n * n;
+  if (g_ws.T.ensure(tb) || g_ws.diag.ensure(sizeof(double) * n) ||
+      g_ws.y.ensure(sizeof(double) * 3 * n) || g_ws.fail.ensure(16))
+    return -1;
+  double* dT = g_ws.T.as<double>();
+  double* y = g_ws.y.as<double>();
+  CK(hipMemcpyAsync(dT, T, tb, hipMemcpyHostToDevice, g_stream));
+  CK(hipMemcpyAsync(y + 2 * n, b, sizeof(double) * n, hipMemcpyHostToDevice, g_stream));
+  CKR(nqs::launch_copy_diag(dT, n, g_ws.diag.as<double>(), g_stream));
+  int f = nqs::launch_potrf(dT, n, g_ws.diag.as<double>(), g_ws.fail.as<int>(), g_stream);
+  if (f < 0) return -1;
+  if (f > 0) {
+    CKR(nqs::launch_add_diag(dT, n, n, 1e-6, g_stream));
+    f = nqs::launch_potrf(dT, n, nullptr, g_ws.fail.as<int>(), g_stream);
+    if (f < 0) return -1;
+  }
+  if (f == 0) CKR(nqs::launch_potrs(dT, n, y + 2 * n, y, y + n, g_stream));
+  else CKR(nqs::launch_fill(y, 0.0, n, g_stream));
+  CK(hipMemcpyAsync(T, dT, tb, hipMemcpyDeviceToHost, g_stream));
+  CK(hipMemcpyAsync(x, y, sizeof(double) * n, hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return f == 0 ? 1 : 0;
+}
+
+NQS_EXPORT long long nqs_hip_device_launches(void) { return g_launches.load(); }
+NQS_EXPORT long long nqs_hip_host_callback_calls(void) { return g_host_calls.load(); }
+NQS_EXPORT const char* nqs_hip_version(void) { return "nqs_hip 0.1.0 gfx950"; }
+
+// ABI layout check: sizeof(nqs_config_t), then offsets of its members in
+// declaration order; returns the number of values written.
+NQS_EXPORT int nqs_hip_config_layout(long* out, int cap) {
+  const long v[] = {
+      (long)sizeof(nqs_config_t),
+      (long)offsetof(nqs_config_t, ansatz), (long)offsetof(nqs_config_t, patch_size),
+      (long)offsetof(nqs_config_t, depth), (long)offsetof(nqs_config_t, width),
+      (long)offsetof(nqs_config_t, heads), (long)offsetof(nqs_config_t, rbm_hidden_units),
+      (long)offsetof(nqs_config_t, rbm_init_scale), (long)offsetof(nqs_config_t, symmetries),
+      (long)offsetof(nqs_config_t, hamiltonian), (long)offsetof(nqs_config_t, j_coupling),
+      (long)offsetof(nqs_config_t, j2_coupling), (long)offsetof(nqs_config_t, j_z_coupling),
+      (long)offsetof(nqs_config_t, transverse_field), (long)offsetof(nqs_config_t, kh_K),
+      (long)offsetof(nqs_config_t, kh_J), (long)offsetof(nqs_config_t, kagome_pbc),
+      (long)offsetof(nqs_config_t, num_samples), (long)offsetof(nqs_config_t, num_thermalize),
+      (long)offsetof(nqs_config_t, num_decorrelate), (long)offsetof(nqs_config_t, cluster_moves),
+      (long)offsetof(nqs_config_t, rng_seed), (long)offsetof(nqs_config_t, num_iterations),
+      (long)offsetof(nqs_config_t, learning_rate), (long)offsetof(nqs_config_t, sr_diag_shift),
+      (long)offsetof(nqs_config_t, sr_cg_max_iters), (long)offsetof(nqs_config_t, sr_cg_tol),
+      (long)sizeof(nqs_sr_step_info_t), (long)sizeof(nqs_marshall_wrapper_t),
+      (long)sizeof(nqs_hip_options_t)};
+  const int n = (int)(sizeof(v) / sizeof(v[0]));
+  if (!out || cap < n) return -1;
+  for (int i = 0; i < n; i++) out[i] = v[i];
+  return n;
+}

@@ -1,0 +1,269 @@
+// k_model.hip — ansatz tables, batched forward pass and O_k generation.
+//
+// Reference semantics: RBM / complex RBM amplitude src/nqs/nqs_ansatz.c:
+// 277-355 (log psi = sum_i a_i s_i + sum_h log 2cosh(theta_h),
+// theta_h = b_h + sum_i W_hi s_i), gradients :885-995 (real-projected) and
+// :1025-1091 (holomorphic); mean-field is the M = 0 case.
+#include "nqs_kernels.h"
+
+namespace nqs {
+
+// --------------------------------------------------------------------------
+// tables: a, b, W^T, cosh(2W), sinh(2W) in both h-major and site-major order
+// --------------------------------------------------------------------------
+template <typename R>
+__global__ void k_prep(const double* __restrict__ par, int kind, int N, int M,
+                       ModelView<R> v, R* a_re, R* a_im, R* b_re, R* b_im, R* WT_re,
+                       R* WT_im, R* C_re, R* C_im, R* S_re, R* S_im, R* CT_re,
+                       R* CT_im, R* ST_re, R* ST_im) {
+  const long Pr = (long)N + M + (long)M * N;
+  const int cplx = (kind == 6);
+  const long total = (long)M * N;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total + N + M;
+       idx += (long)gridDim.x * blockDim.x) {
+    if (idx < N) {
+      a_re[idx] = (R)par[idx];
+      if (cplx) a_im[idx] = (R)par[Pr + idx];
+      continue;
+    }
+    if (idx < (long)N + M) {
+      long h = idx - N;
+      b_re[h] = (R)par[N + h];
+      if (cplx) b_im[h] = (R)par[Pr + N + h];
+      continue;
+    }
+    long e = idx - N - M;       // e = h*N + i
+    int h = (int)(e / N), i = (int)(e % N);
+    double wr = par[N + M + e];
+    double wi = cplx ? par[Pr + N + M + e] : 0.0;
+    long et = (long)i * M + h;
+    WT_re[et] = (R)wr;
+    // cosh / sinh of 2w for complex w = x + iy:
+    // cosh = cosh x cos y + i sinh x sin y ; sinh = sinh x cos y + i cosh x sin y
+    double x = 2.0 * wr, y = 2.0 * wi;
+    double ch = cosh(x), sh = sinh(x), cy = cos(y), sy = sin(y);
+    R cr = (R)(ch * cy), sr = (R)(sh * cy);
+    C_re[e] = cr;  S_re[e] = sr;  CT_re[et] = cr;  ST_re[et] = sr;
+    if (cplx) {
+      R ci = (R)(sh * sy), si = (R)(ch * sy);
+      WT_im[et] = (R)wi;
+      C_im[e] = ci;  S_im[e] = si;  CT_im[et] = ci;  ST_im[et] = si;
+    }
+  }
+}
+
+template <typename R>
+int launch_prep(const double* d_par, int kind, int N, int M, ModelBuffers<R>& mb,
+                hipStream_t st) {
+  const int cplx = (kind == 6);
+  const size_t MN = (size_t)M * N;
+  const size_t need = 2 * (size_t)N + 2 * (size_t)M + 10 * MN + 16;
+  if (mb.count < need) {
+    if (mb.base) (void)hipFree(mb.base);
+    mb.base = nullptr;
+    NQS_HIP_CHECK(hipMalloc(&mb.base, need * sizeof(R)));
+    mb.count = need;
+  }
+  R* p = mb.base;
+  auto take = [&](size_t n) { R* q = p; p += n; return q; };
+  R *a_re = take(N), *a_im = take(N), *b_re = take(M), *b_im = take(M);
+  R *WT_re = take(MN), *WT_im = take(MN), *C_re = take(MN), *C_im = take(MN);
+  R *S_re = take(MN), *S_im = take(MN), *CT_re = take(MN), *CT_im = take(MN);
+  R *ST_re = take(MN), *ST_im = take(MN);
+  ModelView<R>& v = mb.view;
+  v.N = N;
+  v.M = M;
+  v.cplx = cplx;
+  v.a_re = a_re;  v.a_im = cplx ? a_im : nullptr;
+  v.b_re = b_re;  v.b_im = cplx ? b_im : nullptr;
+  v.WT_re = WT_re;  v.WT_im = cplx ? WT_im : nullptr;
+  v.C_re = C_re;  v.C_im = cplx ? C_im : nullptr;
+  v.S_re = S_re;  v.S_im = cplx ? S_im : nullptr;
+  v.CT_re = CT_re;  v.CT_im = cplx ? CT_im : nullptr;
+  v.ST_re = ST_re;  v.ST_im = cplx ? ST_im : nullptr;
+  long total = (long)MN + N + M;
+  int blocks = (int)((total + 255) / 256);
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_prep<R>, dim3(blocks), dim3(256), 0, st, d_par, kind, N, M, v,
+                     a_re, a_im, b_re, b_im, WT_re, WT_im, C_re, C_im, S_re, S_im,
+                     CT_re, CT_im, ST_re, ST_im);
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  mb.valid = 1;
+  return 0;
+}
+
+template <typename R>
+void free_model(ModelBuffers<R>& mb) {
+  if (mb.base) (void)hipFree(mb.base);
+  mb.base = nullptr;
+  mb.count = 0;
+  mb.valid = 0;
+}
+
+// --------------------------------------------------------------------------
+// forward pass: one 256-thread block per configuration
+// --------------------------------------------------------------------------
+template <typename R, bool CPLX>
+__global__ __launch_bounds__(256) void k_forward(ModelView<R> m, const int* __restrict__ spins,
+                                                 int B, R* __restrict__ t_re,
+                                                 R* __restrict__ t_im, double* la,
+                                                 double* ar) {
+  __shared__ R s_s[kMaxSites];
+  __shared__ double red[256];
+  const int N = m.N, M = m.M;
+  for (int b = blockIdx.x; b < B; b += gridDim.x) {
+    const int* sp = spins + (size_t)b * N;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) s_s[i] = (R)sp[i];
+    __syncthreads();
+    double lsum = 0.0, asum = 0.0;
+    for (int h = threadIdx.x; h < M; h += blockDim.x) {
+      // theta_h = b_h + sum_i W_hi s_i, accumulated in site order as in
+      // nqs_ansatz.c:287-290 (s = +-1 makes every product exact)
+      R xr = m.b_re[h];
+      R xi = CPLX ? m.b_im[h] : (R)0;
+      for (int i = 0; i < N; i++) {
+        const R s = s_s[i];
+        xr += m.WT_re[(size_t)i * M + h] * s;
+        if (CPLX) xi += m.WT_im[(size_t)i * M + h] * s;
+      }
+      if (t_re) {
+        if (CPLX) {
+          R tr, ti;
+          ctanh<R>(xr, xi, tr, ti);
+          t_re[(size_t)b * M + h] = tr;
+          t_im[(size_t)b * M + h] = ti;
+        } else {
+          t_re[(size_t)b * M + h] = tanh(xr);
+        }
+      }
+      if (la) {
+        if (CPLX) {
+          double l1, a1;
+          logcosh2_c((double)xr, (double)xi, l1, a1);
+          lsum += l1;
+          asum += a1;
+        } else {
+          double ax = fabs((double)xr);
+          lsum += ax + log1p(exp(-2.0 * ax));
+        }
+      }
+    }
+    if (la) {
+      // visible term sum_i a_i s_i (nqs_ansatz.c:284-285, 333-336)
+      for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        lsum += (double)m.a_re[i] * (double)s_s[i];
+        if (CPLX) asum += (double)m.a_im[i] * (double)s_s[i];
+      }
+      double L = block_sum(lsum, red);
+      double A = CPLX ? block_sum(asum, red) : 0.0;
+      if (threadIdx.x == 0) {
+        la[b] = L;
+        if (ar) ar[b] = A;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename R>
+int launch_forward(const ModelView<R>& m, const int* spins, int B, R* t_re, R* t_im,
+                   double* la, double* ar, hipStream_t st) {
+  if (B <= 0) return 0;
+  if (m.N > kMaxSites) return -1;
+  int grid = B < 65535 ? B : 65535;
+  if (m.cplx)
+    hipLaunchKernelGGL((k_forward<R, true>), dim3(grid), dim3(256), 0, st, m, spins, B,
+                       t_re, t_im, la, ar);
+  else
+    hipLaunchKernelGGL((k_forward<R, false>), dim3(grid), dim3(256), 0, st, m, spins, B,
+                       t_re, t_im, la, ar);
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// O_k rows.  Real block layout [a (N) | b (M) | W (M x N)], imaginary block
+// (complex RBM) the same at offset Pr.
+//   real RBM / mean-field : [s | t | t s]
+//   complex, mode 0 (d log|psi|):  re [s | tR | tR s]  [0 | -tI | -tI s]
+//   complex, mode 1 (holomorphic): re as mode 0, im [0 | tI | tI s] [s | tR | tR s]
+// --------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_gen_O(int N, int M, int cplx, int mode,
+                                               const int* __restrict__ spins, int B,
+                                               const R* __restrict__ t_re,
+                                               const R* __restrict__ t_im,
+                                               double* __restrict__ O_re,
+                                               double* __restrict__ O_im, long P) {
+  const long Pr = (long)N + M + (long)M * N;
+  for (int b = blockIdx.y; b < B; b += gridDim.y) {
+    const int* sp = spins + (size_t)b * N;
+    const R* tr = t_re + (size_t)b * M;
+    const R* ti = cplx ? t_im + (size_t)b * M : nullptr;
+    double* orow = O_re + (size_t)b * P;
+    double* irow = (mode == 1) ? O_im + (size_t)b * P : nullptr;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < P;
+         k += (long)gridDim.x * blockDim.x) {
+      const int blk = (k >= Pr) ? 1 : 0;
+      const long kk = k - blk * Pr;
+      double vr, vi = 0.0;
+      if (kk < N) {
+        double s = (double)sp[kk];
+        if (!cplx) { vr = s; }
+        else if (blk == 0) { vr = s; vi = 0.0; }
+        else { vr = 0.0; vi = s; }
+      } else {
+        long e = kk - N;
+        double s = 1.0;
+        int h;
+        if (e < M) {
+          h = (int)e;
+        } else {
+          e -= M;
+          h = (int)(e / N);
+          s = (double)sp[e % N];
+        }
+        double a = (double)tr[h];
+        if (!cplx) {
+          vr = a * s;
+        } else {
+          double c = (double)ti[h];
+          if (blk == 0) { vr = a * s; vi = c * s; }
+          else { vr = -c * s; vi = a * s; }
+        }
+      }
+      orow[k] = vr;
+      if (irow) irow[k] = vi;
+    }
+  }
+}
+
+template <typename R>
+int launch_gen_O(const ModelView<R>& m, int kind, const int* spins, int B,
+                 const R* t_re, const R* t_im, int mode, double* O_re, double* O_im,
+                 long P, hipStream_t st) {
+  (void)kind;
+  if (B <= 0) return 0;
+  int gx = (int)((P + 255) / 256);
+  if (gx > 1024) gx = 1024;
+  int gy = B < 65535 ? B : 65535;
+  hipLaunchKernelGGL(k_gen_O<R>, dim3(gx, gy), dim3(256), 0, st, m.N, m.M, m.cplx, mode,
+                     spins, B, t_re, t_im, O_re, O_im, P);
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+template int launch_prep<double>(const double*, int, int, int, ModelBuffers<double>&, hipStream_t);
+template int launch_prep<float>(const double*, int, int, int, ModelBuffers<float>&, hipStream_t);
+template void free_model<double>(ModelBuffers<double>&);
+template void free_model<float>(ModelBuffers<float>&);
+template int launch_forward<double>(const ModelView<double>&, const int*, int, double*, double*, double*, double*, hipStream_t);
+template int launch_forward<float>(const ModelView<float>&, const int*, int, float*, float*, double*, double*, hipStream_t);
+template int launch_gen_O<double>(const ModelView<double>&, int, const int*, int, const double*, const double*, int, double*, double*, long, hipStream_t);
+template int launch_gen_O<float>(const ModelView<float>&, int, const int*, int, const float*, const float*, int, double*, double*, long, hipStream_t);
+
+}  // namespace nqs

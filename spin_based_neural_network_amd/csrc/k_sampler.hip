@@ -1,0 +1,266 @@
+// k_sampler.hip — Metropolis chains on the device.
+//
+// Reference semantics (src/nqs/nqs_sampler.c:31-149): xorshift64 stream per
+// chain, site = x mod N, accept when d = 2(log|psi'| - log|psi|) >= 0, else
+// draw u and accept when log u < d.  Exchange moves (an up and a down site,
+// chosen as the k-th up / l-th down site with k = x mod n_up, l = x mod
+// n_down) are this library's extension; both are restated in the oracle.
+//
+// Layout: L lanes per chain (L | 64), each lane owns hidden units
+// h = lane + L*j (j < HPL) and keeps theta_h and tanh(theta_h) in registers;
+// the spin configuration is a bitmask in registers and every lane of the
+// chain advances an identical copy of the chain's xorshift state, so no
+// broadcast is needed.  A proposal's |psi'/psi| is the product over h of
+// cosh(theta_h + d_h)/cosh(theta_h) = cosh d_h + tanh(theta_h) sinh d_h read
+// from the site-major cosh(2W)/sinh(2W) tables: one log per lane and a
+// log2(L)-step butterfly per proposal, theta updated incrementally on accept.
+#include "nqs_kernels.h"
+
+namespace nqs {
+
+__device__ inline int bit_of(const uint64_t* sp, int i) { return (int)((sp[i >> 6] >> (i & 63)) & 1ULL); }
+
+// index of the k-th (0-based) site whose bit equals `want` among N sites
+__device__ inline int select_site(const uint64_t* sp, int nw, int N, int want, int k) {
+  for (int w = 0; w < nw; w++) {
+    int valid = N - 64 * w;
+    uint64_t mask = valid >= 64 ? ~0ULL : ((1ULL << valid) - 1ULL);
+    uint64_t z = (want ? sp[w] : ~sp[w]) & mask;
+    int cnt = __popcll(z);
+    if (k < cnt) {
+      for (int t = 0; t < k; t++) z &= z - 1ULL;
+      return 64 * w + __ffsll((long long)z) - 1;
+    }
+    k -= cnt;
+  }
+  return -1;
+}
+
+template <typename R, bool CPLX, int L, int HPL>
+__global__ __launch_bounds__(256) void k_mh(ModelView<R> m, ChainState cs, MHLaunch ml) {
+  const int lane = threadIdx.x & (L - 1);
+  const int c = (int)((blockIdx.x * (long)blockDim.x + threadIdx.x) / L);
+  if (c >= cs.C) return;
+  const int N = m.N, M = m.M, nw = cs.nw;
+  uint64_t sp[4];
+#pragma unroll
+  for (int w = 0; w < 4; w++) sp[w] = (w < nw) ? cs.spins[(size_t)c * nw + w] : 0ULL;
+  uint64_t rng = cs.rng[c];
+
+  R thr[HPL], thi[HPL], tr[HPL], ti[HPL];
+#pragma unroll
+  for (int j = 0; j < HPL; j++) {
+    const int h = lane + L * j;
+    R xr = 0, xi = 0;
+    if (h < M) {
+      xr = m.b_re[h];
+      if (CPLX) xi = m.b_im[h];
+      for (int i = 0; i < N; i++) {
+        const R s = bit_of(sp, i) ? (R)-1 : (R)1;
+        xr += m.WT_re[(size_t)i * M + h] * s;
+        if (CPLX) xi += m.WT_im[(size_t)i * M + h] * s;
+      }
+    }
+    thr[j] = xr;
+    thi[j] = xi;
+    if (CPLX) ctanh<R>(xr, xi, tr[j], ti[j]);
+    else { tr[j] = tanh(xr); ti[j] = 0; }
+  }
+
+  unsigned long long nacc = 0, ntot = 0;
+  for (int r = 0; r < ml.rounds; r++) {
+    for (int stp = 0; stp < ml.steps; stp++) {
+      int i, j = -1;
+      if (ml.move == 0) {
+        i = (int)(xs_next(rng) % (uint64_t)N);
+      } else {
+        int nd = 0;
+        for (int w = 0; w < nw; w++) {
+          int valid = N - 64 * w;
+          uint64_t mask = valid >= 64 ? ~0ULL : ((1ULL << valid) - 1ULL);
+          nd += __popcll(sp[w] & mask);
+        }
+        const int nu = N - nd;
+        if (nu == 0 || nd == 0) { ntot++; continue; }
+        const int k = (int)(xs_next(rng) % (uint64_t)nu);
+        const int l = (int)(xs_next(rng) % (uint64_t)nd);
+        i = select_site(sp, nw, N, 0, k);   // an up site   (s_i = +1)
+        j = select_site(sp, nw, N, 1, l);   // a down site  (s_j = -1)
+      }
+      const R si = bit_of(sp, i) ? (R)-1 : (R)1;
+      // hidden factors of this lane
+      R pr = 1, pi = 0;
+#pragma unroll
+      for (int q = 0; q < HPL; q++) {
+        const int h = lane + L * q;
+        if (h < M) {
+          const size_t ei = (size_t)i * M + h;
+          R chr, chi = 0, shr, shi = 0;   // cosh d, sinh d
+          if (j < 0) {
+            chr = m.CT_re[ei];
+            shr = -si * m.ST_re[ei];
+            if (CPLX) { chi = m.CT_im[ei]; shi = -si * m.ST_im[ei]; }
+          } else {
+            // d = -2 s_i (W_i - W_j): cosh d = Ci Cj - Si Sj,
+            // sinh d = -s_i (Si Cj - Ci Sj)
+            const size_t ej = (size_t)j * M + h;
+            const R cir = m.CT_re[ei], sir = m.ST_re[ei], cjr = m.CT_re[ej], sjr = m.ST_re[ej];
+            if (CPLX) {
+              const R cii = m.CT_im[ei], sii = m.ST_im[ei], cji = m.CT_im[ej], sji = m.ST_im[ej];
+              chr = (cir * cjr - cii * cji) - (sir * sjr - sii * sji);
+              chi = (cir * cji + cii * cjr) - (sir * sji + sii * sjr);
+              R ar = (sir * cjr - sii * cji) - (cir * sjr - cii * sji);
+              R ai = (sir * cji + sii * cjr) - (cir * sji + cii * sjr);
+              shr = -si * ar;
+              shi = -si * ai;
+            } else {
+              chr = cir * cjr - sir * sjr;
+              shr = -si * (sir * cjr - cir * sjr);
+            }
+          }
+          if (CPLX) {
+            // w = cosh d + t sinh d
+            const R wr = chr + (tr[q] * shr - ti[q] * shi);
+            const R wi = chi + (tr[q] * shi + ti[q] * shr);
+            const R nr = pr * wr - pi * wi;
+            pi = pr * wi + pi * wr;
+            pr = nr;
+          } else {
+            pr *= chr + tr[q] * shr;
+          }
+        }
+      }
+      R lm = CPLX ? (R)0.5 * log(pr * pr + pi * pi) : log(pr);
+      if (M == 0) lm = 0;
+      R lsum = group_allreduce_sum<L>(lm);
+      // visible part: Re a_i (s_i' - s_i) = -2 a_i s_i per flipped site
+      R dvis = (R)-2 * m.a_re[i] * si;
+      if (j >= 0) dvis += (R)2 * m.a_re[j] * si;   // s_j = -s_i
+      const double d = 2.0 * ((double)lsum + (double)dvis);
+      bool acc;
+      if (d >= 0.0) acc = true;
+      else acc = (log(xs_uniform(rng)) < d);
+      ntot++;
+      if (acc) {
+        nacc++;
+        sp[i >> 6] ^= 1ULL << (i & 63);
+        if (j >= 0) sp[j >> 6] ^= 1ULL << (j & 63);
+#pragma unroll
+        for (int q = 0; q < HPL; q++) {
+          const int h = lane + L * q;
+          if (h < M) {
+            const size_t ei = (size_t)i * M + h;
+            thr[q] += (R)-2 * si * m.WT_re[ei];
+            if (CPLX) thi[q] += (R)-2 * si * m.WT_im[ei];
+            if (j >= 0) {
+              const size_t ej = (size_t)j * M + h;
+              thr[q] += (R)2 * si * m.WT_re[ej];
+              if (CPLX) thi[q] += (R)2 * si * m.WT_im[ej];
+            }
+            if (CPLX) ctanh<R>(thr[q], thi[q], tr[q], ti[q]);
+            else tr[q] = tanh(thr[q]);
+          }
+        }
+      }
+    }
+    if (ml.record) {
+      const long b = (long)r * cs.C + c;
+      if (b < ml.n_out) {
+        int* o = ml.out + (size_t)b * N;
+        for (int q = lane; q < N; q += L) o[q] = bit_of(sp, q) ? -1 : 1;
+      }
+    }
+  }
+  if (lane == 0) {
+    for (int w = 0; w < nw; w++) cs.spins[(size_t)c * nw + w] = sp[w];
+    cs.rng[c] = rng;
+    cs.acc[c] += nacc;
+    cs.tot[c] += ntot;
+  }
+}
+
+template <typename R, bool CPLX, int L, int HPL>
+static void mh_go(const ModelView<R>& m, ChainState& cs, const MHLaunch& ml, hipStream_t st) {
+  const long threads = (long)cs.C * L;
+  const int blocks = (int)((threads + 255) / 256);
+  hipLaunchKernelGGL((k_mh<R, CPLX, L, HPL>), dim3(blocks), dim3(256), 0, st, m, cs, ml);
+}
+
+template <typename R>
+int launch_mh(const ModelView<R>& m, ChainState& cs, const MHLaunch& ml, hipStream_t st) {
+  if (cs.C <= 0 || ml.rounds <= 0) return 0;
+  if (m.N > kMaxSites || m.M > kMaxHidden) {
+    fprintf(stderr, "nqs_hip: sampler supports N <= %d, M <= %d\n", kMaxSites, kMaxHidden);
+    return -1;
+  }
+  const bool c = m.cplx != 0;
+  if (m.M <= 64) {
+    if (c) mh_go<R, true, 16, 4>(m, cs, ml, st); else mh_go<R, false, 16, 4>(m, cs, ml, st);
+  } else if (m.M <= 256) {
+    if (c) mh_go<R, true, 64, 4>(m, cs, ml, st); else mh_go<R, false, 64, 4>(m, cs, ml, st);
+  } else {
+    if (c) mh_go<R, true, 64, 16>(m, cs, ml, st); else mh_go<R, false, 64, 16>(m, cs, ml, st);
+  }
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// One thread per chain: initial configuration from the chain's seed.
+// Random start: s_i = +1 when the draw is odd (nqs_sampler.c:69-72);
+// Sz = 0 start: N/2 up then down, Fisher-Yates from the top index.
+__global__ void k_chain_init(ChainState cs, const uint64_t* __restrict__ seeds, int sz0) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cs.C) return;
+  uint64_t rng = seeds[c];
+  uint64_t sp[4] = {0, 0, 0, 0};
+  const int N = cs.N;
+  if (!sz0) {
+    for (int i = 0; i < N; i++) {
+      const bool up = (xs_next(rng) & 1ULL) != 0;
+      if (!up) sp[i >> 6] |= 1ULL << (i & 63);
+    }
+  } else {
+    // bit pattern of the ordered start: sites >= N/2 are down
+    for (int i = N / 2; i < N; i++) sp[i >> 6] |= 1ULL << (i & 63);
+    for (int i = N - 1; i > 0; i--) {
+      const int j = (int)(xs_next(rng) % (uint64_t)(i + 1));
+      const int bi = (int)((sp[i >> 6] >> (i & 63)) & 1ULL);
+      const int bj = (int)((sp[j >> 6] >> (j & 63)) & 1ULL);
+      if (bi != bj) {
+        sp[i >> 6] ^= 1ULL << (i & 63);
+        sp[j >> 6] ^= 1ULL << (j & 63);
+      }
+    }
+  }
+  for (int w = 0; w < cs.nw; w++) cs.spins[(size_t)c * cs.nw + w] = sp[w];
+  cs.rng[c] = rng;
+  cs.acc[c] = 0;
+  cs.tot[c] = 0;
+}
+
+int launch_chain_init(ChainState& cs, const uint64_t* seeds_dev, int sz0, hipStream_t st) {
+  hipLaunchKernelGGL(k_chain_init, dim3((cs.C + 127) / 128), dim3(128), 0, st, cs, seeds_dev, sz0);
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+__global__ void k_unpack(ChainState cs, int chain, int* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cs.N) return;
+  const uint64_t w = cs.spins[(size_t)chain * cs.nw + (i >> 6)];
+  out[i] = ((w >> (i & 63)) & 1ULL) ? -1 : 1;
+}
+
+int launch_unpack(const ChainState& cs, int chain, int* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_unpack, dim3((cs.N + 255) / 256), dim3(256), 0, st, cs, chain, out);
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+template int launch_mh<double>(const ModelView<double>&, ChainState&, const MHLaunch&, hipStream_t);
+template int launch_mh<float>(const ModelView<float>&, ChainState&, const MHLaunch&, hipStream_t);
+
+}  // namespace nqs

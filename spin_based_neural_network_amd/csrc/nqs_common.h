@@ -1,0 +1,129 @@
+// nqs_common.h — shared device / host helpers of libnqs_hip.so (gfx950).
+//
+// Spin convention on the device: configurations are int32 ±1 arrays at the
+// ABI and in sample buffers; chain state in the samplers is a bitmask with
+// bit i = 1 <-> s_i = -1 (the basis convention of the reference's ED,
+// src/nqs/nqs_lanczos.c:16-20).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#define NQS_HIP_CHECK(expr)                                                    \
+  do {                                                                         \
+    hipError_t _e = (expr);                                                    \
+    if (_e != hipSuccess) {                                                    \
+      fprintf(stderr, "nqs_hip: %s failed at %s:%d: %s\n", #expr, __FILE__,   \
+              __LINE__, hipGetErrorString(_e));                                \
+      return -1;                                                               \
+    }                                                                          \
+  } while (0)
+
+namespace nqs {
+
+constexpr int kWave = 64;
+constexpr int kMaxSites = 256;      // bitmask chains: up to 4 x 64-bit words
+constexpr int kMaxHidden = 1024;    // register-resident hidden slices in MH
+
+// xorshift64 (<<13, >>7, <<17) and its 53-bit uniform, as the reference's
+// sampler and initialiser use (src/nqs/nqs_sampler.c:31-48).
+__host__ __device__ inline uint64_t xs_next(uint64_t& s) {
+  uint64_t x = s;
+  x ^= x << 13;
+  x ^= x >> 7;
+  x ^= x << 17;
+  s = x;
+  return x;
+}
+__host__ __device__ inline double xs_uniform(uint64_t& s) {
+  return (double)(xs_next(s) >> 11) * (1.0 / 9007199254740992.0);
+}
+__host__ __device__ inline uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+// Sum over a group of G lanes (G a power of two <= 64) with an xor
+// butterfly: every lane of the group ends with the bitwise-identical sum.
+template <int G, typename T>
+__device__ inline T group_allreduce_sum(T v) {
+#pragma unroll
+  for (int m = 1; m < G; m <<= 1) v += __shfl_xor(v, m, G);
+  return v;
+}
+
+// Deterministic block reduction (fixed tree) of one double per thread;
+// result valid in thread 0.  `red` holds blockDim.x doubles.
+__device__ inline double block_sum(double v, double* red) {
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int s = blockDim.x >> 1; s > 0; s >>= 1) {
+    if (t < s) red[t] += red[t + s];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+template <typename R>
+struct cplx {
+  R re, im;
+};
+
+template <typename R>
+__device__ inline cplx<R> cmul(cplx<R> a, cplx<R> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// tanh of a complex number, (sinh 2x + i sin 2y) / (cosh 2x + cos 2y),
+// saturated to +-1 for |x| > 20 where the quotient would overflow.
+__device__ inline void ctanh_d(double x, double y, double& tr, double& ti) {
+  if (fabs(x) > 20.0) {
+    tr = x > 0 ? 1.0 : -1.0;
+    ti = 0.0;
+    return;
+  }
+  double den = cosh(2.0 * x) + cos(2.0 * y);
+  if (fabs(den) < 1e-300) den = 1e-300;
+  tr = sinh(2.0 * x) / den;
+  ti = sin(2.0 * y) / den;
+}
+__device__ inline void ctanh_f(float x, float y, float& tr, float& ti) {
+  if (fabsf(x) > 10.0f) {
+    tr = x > 0 ? 1.0f : -1.0f;
+    ti = 0.0f;
+    return;
+  }
+  float den = coshf(2.0f * x) + cosf(2.0f * y);
+  if (fabsf(den) < 1e-30f) den = 1e-30f;
+  tr = sinhf(2.0f * x) / den;
+  ti = sinf(2.0f * y) / den;
+}
+template <typename R>
+__device__ inline void ctanh(R x, R y, R& tr, R& ti);
+template <>
+__device__ inline void ctanh<double>(double x, double y, double& tr, double& ti) {
+  ctanh_d(x, y, tr, ti);
+}
+template <>
+__device__ inline void ctanh<float>(float x, float y, float& tr, float& ti) {
+  ctanh_f(x, y, tr, ti);
+}
+
+// log|2 cosh z| and arg(2 cosh z) in the overflow-safe form the reference
+// uses (src/nqs/nqs_ansatz.c:309-318), FP64.
+__device__ inline void logcosh2_c(double x, double y, double& la, double& ar) {
+  double ax = fabs(x), e = exp(-2.0 * ax);
+  double re = (1.0 + e) * cos(y);
+  double im = (x >= 0.0 ? 1.0 : -1.0) * (1.0 - e) * sin(y);
+  double mag = hypot(re, im);
+  la = ax + (mag > 0.0 ? log(mag) : -1e300);
+  ar = atan2(im, re);
+}
+
+}  // namespace nqs

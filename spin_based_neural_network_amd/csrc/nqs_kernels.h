@@ -1,0 +1,163 @@
+// nqs_kernels.h — launch interface between the host engine (engine.cpp) and
+// the gfx950 kernels (k_*.hip) of libnqs_hip.so.
+#pragma once
+
+#include "nqs_common.h"
+
+namespace nqs {
+
+// Device view of an ansatz's derived tables in arithmetic type R.  Built by
+// launch_prep from the FP64 parameter vector (reference layout,
+// src/nqs/nqs_ansatz.c:36-48): a (N), b (M), W (M x N) per real / imaginary
+// block.  C = cosh(2W) and S = sinh(2W) give the ratio of one hidden factor
+// after flipping sites: cosh(theta + d)/cosh(theta) = cosh d + tanh(theta) sinh d
+// with d = -2 s_i W_hi (d's cosh / sinh for two flipped sites follow from the
+// addition theorems).  Imaginary arrays are null for real kinds.
+template <typename R>
+struct ModelView {
+  int N = 0, M = 0, cplx = 0;
+  const R *a_re = nullptr, *a_im = nullptr;
+  const R *b_re = nullptr, *b_im = nullptr;
+  const R *WT_re = nullptr, *WT_im = nullptr;      // [N][M]
+  const R *C_re = nullptr, *C_im = nullptr;        // [M][N]
+  const R *S_re = nullptr, *S_im = nullptr;        // [M][N]
+  const R *CT_re = nullptr, *CT_im = nullptr;      // [N][M]
+  const R *ST_re = nullptr, *ST_im = nullptr;      // [N][M]
+};
+
+template <typename R>
+struct ModelBuffers {
+  R* base = nullptr;
+  size_t count = 0;
+  ModelView<R> view;
+  int valid = 0;
+};
+
+// Hamiltonian as term lists (built on the host in the reference's bond
+// order, see lattice.cpp).  E_loc(s) = sum_d c_d s_a s_b
+//   + sum_f c_f psi(s^f)/psi(s) + sum_p [s_a != s_b] c_p psi(s^{ab})/psi(s).
+struct HamDev {
+  int n_diag = 0, n_flip = 0, n_pair = 0;
+  const int* diag_a = nullptr;
+  const int* diag_b = nullptr;
+  const double* diag_c = nullptr;
+  const int* flip_i = nullptr;
+  const double* flip_c = nullptr;
+  const int* pair_a = nullptr;
+  const int* pair_b = nullptr;
+  const double* pair_c = nullptr;
+  const signed char* sublat = nullptr;  // Marshall: 1 on sublattice A, or null
+};
+
+// Metropolis chain state on the device.
+struct ChainState {
+  int C = 0, N = 0, nw = 0;
+  uint64_t* spins = nullptr;      // [C][nw] bitmask, bit=1 <-> s=-1
+  uint64_t* rng = nullptr;        // [C]
+  unsigned long long* acc = nullptr;  // [C]
+  unsigned long long* tot = nullptr;  // [C]
+};
+
+// ---- model (k_model.hip) --------------------------------------------------
+template <typename R>
+int launch_prep(const double* d_par, int kind, int N, int M, ModelBuffers<R>& mb,
+                hipStream_t st);
+template <typename R>
+void free_model(ModelBuffers<R>& mb);
+// theta = b + W s per sample; writes t = tanh(theta) [B][M] (re / im) and,
+// when la != null, log|psi| and arg psi [B] in FP64.
+template <typename R>
+int launch_forward(const ModelView<R>& m, const int* spins, int B, R* t_re, R* t_im,
+                   double* la, double* ar, hipStream_t st);
+// O rows [B][P] from spins and t: mode 0 real-projected, 1 holomorphic.
+template <typename R>
+int launch_gen_O(const ModelView<R>& m, int kind, const int* spins, int B,
+                 const R* t_re, const R* t_im, int mode, double* O_re, double* O_im,
+                 long P, hipStream_t st);
+
+// ---- sampler (k_sampler.hip) ---------------------------------------------
+struct MHLaunch {
+  int move = 0;              // 0 flip, 1 exchange
+  int steps = 1;             // moves per round
+  int rounds = 1;
+  int record = 0;            // write samples
+  int n_out = 0;             // samples to write (index r*C + c < n_out)
+  int* out = nullptr;        // [n_out][N] int32
+};
+template <typename R>
+int launch_mh(const ModelView<R>& m, ChainState& cs, const MHLaunch& L, hipStream_t st);
+int launch_chain_init(ChainState& cs, const uint64_t* seeds_dev, int sz_zero, hipStream_t st);
+int launch_unpack(const ChainState& cs, int chain, int* out, hipStream_t st);
+
+// ---- local energy (k_eloc.hip) -------------------------------------------
+template <typename R>
+int launch_eloc(const ModelView<R>& m, const HamDev& h, const int* spins, int B,
+                const R* t_re, const R* t_im, double* e_re, double* e_im,
+                hipStream_t st);
+// E_loc from externally evaluated log-amplitudes of every connected
+// configuration (host-callback adaptor): la/ar of sample b at [b][0], of
+// its k-th connected configuration at [b][1+k] in term order.
+int launch_eloc_from_logamps(const HamDev& h, const int* spins, int B, int N,
+                             int n_terms, const double* la, const double* ar,
+                             double* e_re, double* e_im, hipStream_t st);
+
+// ---- linear algebra (k_linalg.hip), FP64 ---------------------------------
+// out[k] = alpha * sum_b O[b][k] * (w ? w[b] : 1)  (deterministic order)
+int launch_colsum(const double* O, long B, long P, const double* w, double alpha,
+                  double* out, double* work, hipStream_t st);
+size_t colsum_work_elems(long B, long P);
+// u[b] = sum_k O[b][k] v[k]
+int launch_rowdot(const double* O, long B, long P, const double* v, double* u,
+                  hipStream_t st);
+// deterministic sum of x[0..n) times alpha into *out (device scalar)
+int launch_sum(const double* x, long n, const double* y, double alpha, double* out,
+               hipStream_t st);
+// O[b][k] -= mean[k]
+int launch_center(double* O, long B, long P, const double* mean, hipStream_t st);
+// C = alpha * A B^T + beta C  (row-major; A [M][K] lda, B [N][K] ldb, C ldc);
+// lower != 0: only tiles with any element on or below the diagonal.
+int launch_gemm_nt(int M, int N, int K, double alpha, const double* A, long lda,
+                   const double* B, long ldb, double beta, double* C, long ldc,
+                   int lower, hipStream_t st);
+// In-place lower Cholesky of the n x n row-major matrix (lower triangle
+// read and written, upper untouched).  *fail_dev = 0 on success or 1 + the
+// first column whose pivot is not positive.  On failure the matrix is left
+// in the state the unblocked left-looking factorisation would leave:
+// columns before the failing one hold L, the others their input values
+// (restored from the untouched upper triangle and `diag_in`).
+int launch_potrf(double* A, int n, const double* diag_in, int* fail_dev, hipStream_t st);
+// x = (L L^T)^-1 b with L from launch_potrf; z is scratch [n].
+int launch_potrs(const double* L, int n, const double* b, double* x, double* z,
+                 hipStream_t st);
+int launch_add_diag(double* A, int n, long ld, double v, hipStream_t st);
+int launch_copy_diag(const double* A, int n, double* d, hipStream_t st);
+int launch_mirror_lower(double* A, int n, hipStream_t st);
+
+// elementwise helpers
+int launch_axpy(double* y, const double* x, double a, long n, hipStream_t st);
+int launch_fill(double* x, double v, long n, hipStream_t st);
+// y[i] = x[i] - *s (s a device scalar)
+int launch_sub_scalar(double* y, const double* x, long n, const double* s, hipStream_t st);
+
+// ---- CG on the device (k_linalg.hip) ----------------------------------------
+struct CGScalars {   // lives in device memory
+  double rs, rs0, pAp, alpha, beta, rs_new;
+  int iters, converged, breakdown, done;
+};
+int launch_cg_init(const double* F, long P, double* x, double* r, double* p,
+                   CGScalars* sc, hipStream_t st);
+int launch_cg_alpha(CGScalars* sc, const double* p, const double* Ap, long P,
+                    hipStream_t st);
+int launch_cg_update(CGScalars* sc, double* x, double* r, const double* p,
+                     const double* Ap, long P, double tol, hipStream_t st);
+int launch_cg_direction(CGScalars* sc, double* p, const double* r, long P,
+                        hipStream_t st);
+// out[k] = a * (x[k] - m[k] * s[0]) + eps * v[k] accumulate helpers for QGT
+int launch_qgt_finish(double* out, const double* c1, const double* m1, const double* u1mean,
+                      const double* c2, const double* m2, const double* u2mean,
+                      double scale, double eps, const double* v, long P, hipStream_t st);
+
+// ---- bookkeeping ------------------------------------------------------------
+void count_launch(int n = 1);
+
+}  // namespace nqs

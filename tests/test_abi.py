@@ -1,0 +1,106 @@
+"""CPU checks of the C-ABI boundary (no compute calls, no GPU needed).
+
+* libnqs_hip.so exists, loads, and exports every function declared in
+  include/nqs/*.h and include/nqs_hip/nqs_hip.h;
+* the struct layouts the library was compiled with match the ctypes mirror
+  (and therefore the reference's include/nqs/nqs_config.h:82-120 layout);
+* the option defaults agree on both sides;
+* the reference's header set is covered by ours for the path's functions.
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from spin_based_neural_network_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INC = os.path.join(ROOT, "include")
+
+
+def declared_functions():
+    names = set()
+    for d in ("nqs", "nqs_hip"):
+        for f in os.listdir(os.path.join(INC, d)):
+            if not f.endswith(".h"):
+                continue
+            txt = open(os.path.join(INC, d, f)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            txt = re.sub(r"static inline[^{]*\{.*?\n\}", "", txt, flags=re.S)
+            for m in re.finditer(r"\b(nqs_[a-z0-9_]+)\s*\(", txt):
+                pre = txt[max(0, m.start() - 40):m.start()]
+                if "typedef" in pre.split(";")[-1] or "(*" in pre[-3:]:
+                    continue
+                names.add(m.group(1))
+    names -= {"nqs_config_defaults", "nqs_energy_accumulator_init", "nqs_energy_accumulator_add",
+              "nqs_energy_accumulator_mean", "nqs_energy_accumulator_variance"}
+    return sorted(names)
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(abi.LIB_PATH):
+        pytest.fail("libnqs_hip.so is not built (run __graft_entry__.build())")
+    return abi.load_library()
+
+
+def test_exports_every_declared_symbol(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", abi.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    missing = [n for n in declared_functions() if n not in exported]
+    assert not missing, f"declared but not exported: {missing}"
+    assert len(declared_functions()) >= 45
+
+
+def test_struct_layout_matches_mirror(lib):
+    buf = (ctypes.c_long * 64)()
+    n = lib.nqs_hip_config_layout(buf, 64)
+    assert n == 30
+    vals = list(buf[:n])
+    assert vals[0] == ctypes.sizeof(abi.NqsConfig)
+    offs = [getattr(abi.NqsConfig, f).offset for f, _ in abi.NqsConfig._fields_]
+    assert vals[1:27] == offs
+    assert vals[27] == ctypes.sizeof(abi.SrInfo)
+    assert vals[28] == ctypes.sizeof(abi.MarshallWrapper)
+    assert vals[29] == ctypes.sizeof(abi.HipOptions)
+
+
+def test_option_defaults_agree(lib):
+    o = lib.nqs_hip_options_defaults()
+    p = abi.options_defaults()
+    for f, _ in abi.HipOptions._fields_:
+        assert getattr(o, f) == getattr(p, f), f
+
+
+def test_reference_header_functions_covered():
+    """Every path function of the reference headers this library replaces
+    (ansatz, sampler, local energy, SR / holomorphic / MinSR, Marshall) has a
+    declaration here; tVMC and the excited-state step are out of scope."""
+    ref = "/root/reference/include/nqs"
+    if not os.path.isdir(ref):
+        pytest.skip("reference checkout not present")
+    ours = set(declared_functions())
+    skip = {"nqs_tvmc_step_real_time", "nqs_tvmc_step_heun", "nqs_sr_step_excited",
+            "nqs_sr_run_excited", "nqs_config_defaults"}
+    for h in ("nqs_ansatz.h", "nqs_sampler.h", "nqs_gradient.h", "nqs_optimizer.h",
+              "nqs_marshall.h"):
+        txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(ref, h)).read(), flags=re.S)
+        txt = re.sub(r"static inline[^{]*\{.*?\n\}", "", txt, flags=re.S)
+        for m in re.finditer(r"\b(nqs_[a-z0-9_]+)\s*\(", txt):
+            name = m.group(1)
+            pre = txt[max(0, m.start() - 40):m.start()]
+            if "typedef" in pre.split(";")[-1] or name in skip or name.startswith("nqs_energy_"):
+                continue
+            assert name in ours, name
+
+
+def test_no_gpu_fails_loudly(lib, capfd):
+    """Without a device every compute entry point refuses (no CPU fallback)."""
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    cfg = abi.config_defaults(ansatz=5)
+    assert not lib.nqs_ansatz_create(ctypes.byref(cfg), 4)
+    assert "no HIP device" in capfd.readouterr().err

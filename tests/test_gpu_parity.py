@@ -1,0 +1,282 @@
+"""Parity of the HIP path (libnqs_hip.so through its C ABI) with the
+reference golden vectors and the CPU oracle.
+
+Tolerances (FP64 path): init streams 1e-13 relative (device libm vs the
+reference's libm), amplitudes / local energies / O sketches 1e-10 relative,
+sample sequences bit-exact, SR parameter updates 1e-10, CG iteration
+counts and convergence flags exact.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import scenarios
+from conftest import golden_case
+from spin_based_neural_network_amd.abi import (config_defaults, fnptr,
+                                               options_defaults, ptr)
+
+pytestmark = pytest.mark.gpu
+
+
+def _close(a, b, tol):
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    s = max(1.0, float(np.max(np.abs(b)))) if b.size else 1.0
+    err = float(np.max(np.abs(a - b))) if a.size else 0.0
+    assert err <= tol * s, f"max err {err:.3e} > {tol * s:.3e}"
+
+
+@pytest.fixture(scope="module")
+def hip_results(hip):
+    print("running golden scenarios on the device", flush=True)
+    before = hip.nqs_hip_host_callback_calls()
+    res = scenarios.run_all(hip, "nqs_")
+    # every golden scenario uses recognised callbacks: nothing ran on the host
+    assert hip.nqs_hip_host_callback_calls() == before
+    return res
+
+
+@pytest.mark.parametrize("sc", scenarios.INIT, ids=lambda s: s["name"])
+def test_init(hip_results, golden, sc):
+    got = hip_results[f"init/{sc['name']}"]
+    ref = golden_case(golden, f"init/{sc['name']}")
+    assert got["P"][0] == ref["P"][0]
+    for k in ref:
+        if k != "P":
+            _close(got[k], ref[k], 1e-13)
+
+
+@pytest.mark.parametrize("sc", scenarios.AMP, ids=lambda s: s["name"])
+def test_amplitudes_gradients(hip_results, golden, sc):
+    got = hip_results[f"amp/{sc['name']}"]
+    ref = golden_case(golden, f"amp/{sc['name']}")
+    for k in ref:
+        if k != "spins":
+            _close(got[k], ref[k], 1e-10)
+
+
+@pytest.mark.parametrize("sc", scenarios.ELOC, ids=lambda s: s["name"])
+def test_local_energy(hip_results, golden, sc):
+    got = hip_results[f"eloc/{sc['name']}"]
+    ref = golden_case(golden, f"eloc/{sc['name']}")
+    for k in ("e", "e_re", "e_im", "e_single"):
+        _close(got[k], ref[k], 1e-10)
+
+
+@pytest.mark.parametrize("sc", scenarios.SAMPLER, ids=lambda s: s["name"])
+def test_sampler_bit_exact(hip_results, golden, sc):
+    got = hip_results[f"sampler/{sc['name']}"]
+    ref = golden_case(golden, f"sampler/{sc['name']}")
+    for k in ref:
+        if k == "acceptance":
+            _close(got[k], ref[k], 1e-15)
+        else:
+            np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+
+
+@pytest.mark.parametrize("sc", scenarios.SR, ids=lambda s: s["name"])
+def test_sr_step(hip_results, golden, sc):
+    got = hip_results[f"sr/{sc['name']}"]
+    ref = golden_case(golden, f"sr/{sc['name']}")
+    _close(got["params"], ref["params"], 1e-10)
+    _close(got["info"][:, :4], ref["info"][:, :4], 1e-9)
+    np.testing.assert_array_equal(got["info"][:, 4:], ref["info"][:, 4:])
+
+
+@pytest.mark.parametrize("sc", scenarios.RUN, ids=lambda s: s["name"])
+def test_sr_run(hip_results, golden, sc):
+    got = hip_results[f"run/{sc['name']}"]
+    ref = golden_case(golden, f"run/{sc['name']}")
+    _close(got["trace"], ref["trace"], 1e-9)
+    _close(got["params"], ref["params"], 1e-9)
+
+
+# ---- reference known answers through the device and adaptor paths ---------
+CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                      ctypes.c_void_p, ctypes.c_void_p)
+
+
+@CB
+def _uniform(s, n, u, la, ar):
+    ctypes.cast(la, ctypes.POINTER(ctypes.c_double))[0] = 0.0
+    ctypes.cast(ar, ctypes.POINTER(ctypes.c_double))[0] = 0.0
+
+
+def test_known_answers_host_callback(hip):
+    """tests/test_nqs.c:110-156 and tests/test_nqs_kagome.c:52-141 with the
+    reference's own uniform callback (evaluated on the host, summed on the
+    device)."""
+    fn = ctypes.cast(_uniform, ctypes.c_void_p).value
+
+    def le(cfg, Lx, Ly, s):
+        a = np.ascontiguousarray(np.array(s, dtype=np.int32))
+        return hip.nqs_local_energy(ctypes.byref(cfg), Lx, Ly, ptr(a), fn, None)
+
+    neel = [1, -1, 1, -1, 1, -1, 1, -1, 1]
+    assert abs(le(config_defaults(transverse_field=0.0), 3, 3, [1] * 9) + 12.0) < 1e-12
+    assert abs(le(config_defaults(hamiltonian=1), 3, 3, neel) - 3.0) < 1e-12
+    assert abs(le(config_defaults(hamiltonian=2, j2_coupling=0.0), 3, 3, neel) - 3.0) < 1e-12
+    assert abs(le(config_defaults(hamiltonian=6), 2, 2, [1] * 12) - 6.0) < 1e-12
+    assert abs(le(config_defaults(hamiltonian=6), 2, 2, [1, -1, -1] * 4) - 6.0) < 1e-12
+    assert abs(le(config_defaults(hamiltonian=6, kagome_pbc=0), 2, 2, [1] * 12) - 3.75) < 1e-12
+    assert abs(le(config_defaults(hamiltonian=6), 1, 1, [1, 1, 1]) - 1.5) < 1e-12
+
+
+def test_linear_ansatz_on_device(hip):
+    """tests/test_nqs_kagome.c:170-201: the linear ansatz is the mean-field
+    kind, evaluated on the device."""
+    cfg = config_defaults(hamiltonian=6)
+    a = hip.nqs_ansatz_create(ctypes.byref(cfg), 3)
+    p = np.ctypeslib.as_array(ctypes.cast(hip.nqs_ansatz_params_raw(a), ctypes.POINTER(ctypes.c_double)), (3,))
+    p[:] = [0.1, 0.2, 0.3]
+    s = np.ascontiguousarray(np.array([1, -1, 1], dtype=np.int32))
+    e = hip.nqs_local_energy(ctypes.byref(cfg), 1, 1, ptr(s), fnptr(hip, "nqs_ansatz_log_amp"), a)
+    assert abs(e - (-0.5 + 2.0 * np.cosh(0.2))) < 1e-12
+    hip.nqs_ansatz_free(a)
+
+
+def test_sampler_uniform_acceptance_one(hip):
+    # tests/test_nqs.c:98-108 (host callback chain)
+    cfg = config_defaults(num_thermalize=128, rng_seed=3)
+    s = hip.nqs_sampler_create(10, ctypes.byref(cfg), ctypes.cast(_uniform, ctypes.c_void_p).value, None)
+    hip.nqs_sampler_thermalize(s)
+    assert abs(hip.nqs_sampler_acceptance_ratio(s) - 1.0) < 1e-12
+    hip.nqs_sampler_free(s)
+
+
+# ---- extensions against the oracle -------------------------------------------
+@pytest.mark.parametrize("kind,ham,L,move,sz0,per", [
+    (5, 0, (12, 1), 0, 0, 1),
+    (6, 1, (4, 4), 1, 1, 1),
+    (6, 6, (2, 2), 1, 1, 0),
+    (5, 2, (4, 4), 1, 1, 1),
+])
+def test_multichain_exchange_periodic_vs_oracle(hip, orc, kind, ham, L, move, sz0, per):
+    Lx, Ly = L
+    N = 3 * Lx * Ly if ham == 6 else Lx * Ly
+    cfg = config_defaults(ansatz=kind, hamiltonian=ham, rbm_hidden_units=2 * N, rbm_init_scale=0.4,
+                          num_thermalize=24, num_decorrelate=3, j2_coupling=0.5, num_samples=96,
+                          sr_diag_shift=1e-2, learning_rate=0.02)
+    opt = options_defaults(num_chains=8, move=move, sz_zero_start=sz0, periodic=per)
+    out = {}
+    for name, lib, pre in (("hip", hip, "nqs_"), ("orc", orc, "orc_")):
+        a = getattr(lib, pre + "ansatz_create")(ctypes.byref(cfg), N)
+        fn = fnptr(lib, pre + "ansatz_log_amp")
+        create = lib.nqs_hip_sampler_create_ex if name == "hip" else lib.orc_sampler_create_ex
+        s = create(N, ctypes.byref(cfg), ctypes.byref(opt), fn, a)
+        getattr(lib, pre + "sampler_thermalize")(s)
+        smp = np.zeros((40, N), dtype=np.int32)
+        getattr(lib, pre + "sampler_batch")(s, 40, ptr(smp))
+        e = np.zeros(40)
+        ei = np.zeros(40)
+        if name == "hip":
+            hip.nqs_hip_local_energy_batch_ex(ctypes.byref(cfg), Lx, Ly, ctypes.byref(opt), ptr(smp), 40,
+                                              fn, a, ptr(e), ptr(ei))
+        else:
+            orc.orc_local_energy_batch_ex(ctypes.byref(cfg), Lx, Ly, per, ptr(smp), 40, fn, a, ptr(e), ptr(ei))
+        tr = np.zeros(2)
+        from spin_based_neural_network_amd.abi import SrInfo
+        infos = []
+        for _ in range(2):
+            info = SrInfo()
+            getattr(lib, pre + "sr_step_minsr_full")(ctypes.byref(cfg), Lx, Ly, a, s, None, None, None,
+                                                     None, ctypes.byref(info))
+            infos.append([info.mean_energy, info.variance_energy, info.update_norm, info.acceptance_ratio])
+        P = getattr(lib, pre + "ansatz_num_params")(a)
+        par = np.ctypeslib.as_array(ctypes.cast(getattr(lib, pre + "ansatz_params_raw")(a),
+                                                ctypes.POINTER(ctypes.c_double)), (P,)).copy()
+        out[name] = dict(samples=smp, e=e, ei=ei, acc=getattr(lib, pre + "sampler_acceptance_ratio")(s),
+                         infos=np.array(infos), params=par)
+        getattr(lib, pre + "sampler_free")(s)
+        getattr(lib, pre + "ansatz_free")(a)
+    np.testing.assert_array_equal(out["hip"]["samples"], out["orc"]["samples"])
+    assert out["hip"]["acc"] == out["orc"]["acc"]
+    _close(out["hip"]["e"], out["orc"]["e"], 1e-10)
+    _close(out["hip"]["ei"], out["orc"]["ei"], 1e-10)
+    _close(out["hip"]["infos"], out["orc"]["infos"], 1e-9)
+    _close(out["hip"]["params"], out["orc"]["params"], 1e-9)
+    if sz0:
+        assert np.all(out["hip"]["samples"].sum(1) == 0)
+
+
+def test_fp32_local_energy_bound(hip):
+    """FP32 amplitude arithmetic stays within 1e-4 relative of FP64."""
+    cfg = config_defaults(ansatz=6, hamiltonian=1, rbm_hidden_units=144, rbm_init_scale=0.3)
+    N = 36
+    a = hip.nqs_ansatz_create(ctypes.byref(cfg), N)
+    s = scenarios.spins_random(64, N, 5, sz0=True)
+    fn = fnptr(hip, "nqs_ansatz_log_amp")
+    res = {}
+    for prec in (0, 1):
+        opt = options_defaults(periodic=1, precision=prec)
+        e = np.zeros(64)
+        ei = np.zeros(64)
+        assert hip.nqs_hip_local_energy_batch_ex(ctypes.byref(cfg), 6, 6, ctypes.byref(opt), ptr(s), 64,
+                                                 fn, a, ptr(e), ptr(ei)) == 0
+        res[prec] = e + 1j * ei
+    err = np.max(np.abs(res[1] - res[0])) / np.max(np.abs(res[0]))
+    assert err < 1e-4
+    hip.nqs_ansatz_free(a)
+
+
+def test_gram_and_cholesky_blocks(hip, orc):
+    rng = np.random.default_rng(11)
+    for (B, P) in [(37, 19), (130, 300), (200, 64)]:
+        O = np.ascontiguousarray(rng.standard_normal((B, P)))
+        T = np.zeros((B, B))
+        assert hip.nqs_hip_gram(ptr(O), B, P, 1.0 / B, ptr(T)) == 0
+        np.testing.assert_allclose(T, O @ O.T / B, rtol=1e-12, atol=1e-12)
+        A = np.ascontiguousarray(T + 0.1 * np.eye(B))
+        b = np.ascontiguousarray(rng.standard_normal(B))
+        x = np.zeros(B)
+        Aw = A.copy()
+        assert hip.nqs_hip_cholesky_solve(ptr(Aw), B, ptr(b), ptr(x)) == 1
+        np.testing.assert_allclose(A @ x, b, atol=1e-9)
+        L = np.tril(Aw)
+        np.testing.assert_allclose(L @ L.T, A, atol=1e-10)
+        np.testing.assert_array_equal(np.triu(Aw, 1), np.triu(A, 1))   # upper untouched
+
+
+def test_cholesky_failure_state_matches_unblocked(hip, orc):
+    """A rank-deficient matrix: the blocked factorisation reports the same
+    outcome as the unblocked reference algorithm, including its retry."""
+    rng = np.random.default_rng(4)
+    for (B, r) in [(90, 70), (150, 100)]:
+        X = rng.standard_normal((B, r))
+        A = np.ascontiguousarray(X @ X.T / B)
+        A[np.arange(B), np.arange(B)] += 0.0
+        b = np.ascontiguousarray(rng.standard_normal(B))
+        x = np.zeros(B)
+        ok = hip.nqs_hip_cholesky_solve(ptr(A.copy()), B, ptr(b), ptr(x))
+        # oracle: same two-attempt semantics on the CPU
+        Ao = np.ascontiguousarray(A.copy())
+        rc = orc.orc_cholesky_lower(ptr(Ao), B)
+        if rc != 0:
+            Ao[np.arange(B), np.arange(B)] += 1e-6
+            rc = orc.orc_cholesky_lower(ptr(Ao), B)
+        assert ok == (1 if rc == 0 else 0)
+
+
+def test_large_shapes_consistent_with_oracle(hip, orc):
+    """Configuration shapes of the benchmark (6x6 periodic Heisenberg + Marshall,
+    complex RBM alpha=4; kagome 2x4 periodic, complex RBM M=96; J1-J2 10x10
+    periodic, RBM alpha=8): device E_loc and O sketches of a few samples
+    against the oracle."""
+    fn_h, fn_o = fnptr(hip, "nqs_ansatz_log_amp"), fnptr(orc, "orc_ansatz_log_amp")
+    for (ham, kind, Lx, Ly, M) in [(1, 6, 6, 6, 144), (6, 6, 2, 4, 96), (2, 5, 10, 10, 800)]:
+        cfg = config_defaults(ansatz=kind, hamiltonian=ham, rbm_hidden_units=M, rbm_init_scale=0.3,
+                              j2_coupling=0.5)
+        N = 3 * Lx * Ly if ham == 6 else Lx * Ly
+        s = scenarios.spins_random(6, N, 77, sz0=True)
+        ah = hip.nqs_ansatz_create(ctypes.byref(cfg), N)
+        ao = orc.orc_ansatz_create(ctypes.byref(cfg), N)
+        opt = options_defaults(periodic=1)
+        eh, ehi, eo, eoi = (np.zeros(6) for _ in range(4))
+        hip.nqs_hip_local_energy_batch_ex(ctypes.byref(cfg), Lx, Ly, ctypes.byref(opt), ptr(s), 6, fn_h, ah,
+                                          ptr(eh), ptr(ehi))
+        orc.orc_local_energy_batch_ex(ctypes.byref(cfg), Lx, Ly, 1, ptr(s), 6, fn_o, ao, ptr(eo), ptr(eoi))
+        _close(eh, eo, 1e-10)
+        _close(ehi, eoi, 1e-10)
+        hip.nqs_ansatz_free(ah)
+        orc.orc_ansatz_free(ao)
