@@ -570,7 +570,7 @@ struct nqs_sampler {
   AmpSource src;
   // device chains
   nqs::ChainState cs;
-  DBuf d_spins, d_rng, d_acc, d_tot, d_cur, d_samples;
+  DBuf d_spins, d_rng, d_acc, d_tot, d_la, d_cur, d_samples;
   // host-callback chains
   std::vector<int> hc, hp;
   std::vector<double> hla;
@@ -692,6 +692,19 @@ int sampler_advance(nqs_sampler* s, int steps, int rounds, int n_out, int* d_out
   return sampler_sync_cur(s);
 }
 
+// log|psi| of every chain with the current parameters (the value the
+// reference computes at sampler creation, nqs_sampler.c:73-74)
+int chains_eval_la(nqs_sampler* s) {
+  DBuf rows;
+  CKR(rows.ensure(sizeof(int) * (size_t)s->cs.C * s->N));
+  CKR(nqs::launch_unpack_all(s->cs, rows.as<int>(), g_stream));
+  CKR(ansatz_upload(s->src.a));
+  CKR(forward_t<double>(s->src.a, rows.as<int>(), s->cs.C, s->cs.la, nullptr));
+  CK(hipStreamSynchronize(g_stream));
+  rows.release();
+  return 0;
+}
+
 nqs_sampler* sampler_new(int N, const nqs_config_t* cfg, const nqs_hip_options_t* opt,
                          nqs_log_amp_fn_t fn, void* user) {
   if (N <= 0 || !cfg || !fn) return nullptr;
@@ -720,7 +733,8 @@ nqs_sampler* sampler_new(int N, const nqs_config_t* cfg, const nqs_hip_options_t
     const int nw = (N + 63) / 64;
     if (s->d_spins.ensure(sizeof(uint64_t) * (size_t)C * nw) ||
         s->d_rng.ensure(sizeof(uint64_t) * C) || s->d_acc.ensure(sizeof(uint64_t) * C) ||
-        s->d_tot.ensure(sizeof(uint64_t) * C) || s->d_cur.ensure(sizeof(int) * N)) {
+        s->d_tot.ensure(sizeof(uint64_t) * C) || s->d_la.ensure(sizeof(double) * C) ||
+        s->d_cur.ensure(sizeof(int) * N)) {
       delete s;
       return nullptr;
     }
@@ -731,13 +745,14 @@ nqs_sampler* sampler_new(int N, const nqs_config_t* cfg, const nqs_hip_options_t
     s->cs.rng = s->d_rng.as<uint64_t>();
     s->cs.acc = s->d_acc.as<unsigned long long>();
     s->cs.tot = s->d_tot.as<unsigned long long>();
+    s->cs.la = s->d_la.as<double>();
     std::vector<uint64_t> seeds(C);
     for (int c = 0; c < C; c++) seeds[c] = chain_seed(cfg->rng_seed, c);
     DBuf tmp;
     if (tmp.ensure(sizeof(uint64_t) * C) ||
         hipMemcpyAsync(tmp.p, seeds.data(), sizeof(uint64_t) * C, hipMemcpyHostToDevice, g_stream) ||
         nqs::launch_chain_init(s->cs, tmp.as<uint64_t>(), s->opt.sz_zero_start, g_stream) ||
-        sampler_sync_cur(s)) {
+        sampler_sync_cur(s) || chains_eval_la(s)) {
       delete s;
       return nullptr;
     }

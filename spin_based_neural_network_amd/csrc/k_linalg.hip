@@ -278,9 +278,9 @@ int launch_cg_direction(CGScalars* sc, double* p, const double* r, long P, hipSt
 // --------------------------------------------------------------------------
 // C = alpha A B^T + beta C with v_mfma_f64_16x16x4f64.
 // Workgroup tile 64x64 (4 waves, 32x32 each = 2x2 MFMA blocks), K step 16
-// staged in LDS.  MFMA lane layout (16x16x4, FP64): lane l supplies
-// A[l%16][l/16] and B^T[l/16][l%16]; result element r of lane l is
-// D[4*(l/16)+r][l%16].  `lower`: the tile grid covers only tiles with
+// staged in LDS.  MFMA lane layout (16x16x4, FP64, checked on gfx950): lane
+// l supplies A[l%16][l/16] and B^T[l/16][l%16]; result element r of lane l
+// is D[(l/16) + 4r][l%16].  `lower`: the tile grid covers only tiles with
 // tn <= tm and elements strictly above the diagonal are never written.
 // `skip`: device flag; the kernel does nothing when it is non-zero.
 // --------------------------------------------------------------------------
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int M, int N, int K, double alp
     for (int j = 0; j < 2; j++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int gr = row0 + wr + 16 * i + 4 * lk + r;
+        const int gr = row0 + wr + 16 * i + lk + 4 * r;
         const int gc = col0 + wc + 16 * j + lr;
         if (gr < M && gc < N && (!lower || gc <= gr)) {
           double* c = C + (long)gr * ldc + gc;

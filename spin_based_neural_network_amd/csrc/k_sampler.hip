@@ -67,6 +67,30 @@ __global__ __launch_bounds__(256) void k_mh(ModelView<R> m, ChainState cs, MHLau
     else { tr[j] = tanh(xr); ti[j] = 0; }
   }
 
+  // log|psi(s)| with the current parameters, and the chain's cached value
+  double la_part = 0.0;
+#pragma unroll
+  for (int j = 0; j < HPL; j++) {
+    const int h = lane + L * j;
+    if (h < M) {
+      const double x = (double)thr[j];
+      if (CPLX) {
+        const double ax = fabs(x), e = exp(-2.0 * ax), y = (double)thi[j];
+        const double re = (1.0 + e) * cos(y), im = (1.0 - e) * sin(y);
+        const double mag = sqrt(re * re + im * im);
+        la_part += ax + (mag > 0.0 ? log(mag) : -1e300);
+      } else {
+        const double ax = fabs(x);
+        la_part += ax + log1p(exp(-2.0 * ax));
+      }
+    }
+  }
+  for (int i = lane; i < N; i += L) la_part += (double)m.a_re[i] * (bit_of(sp, i) ? -1.0 : 1.0);
+  const double la_now = group_allreduce_sum<L>(la_part);
+  // off = log|psi_now(s)| - cached; d below is 2 (off + log|psi(s')/psi(s)|)
+  double off = la_now - cs.la[c];
+  double la_cur = la_now;
+
   unsigned long long nacc = 0, ntot = 0;
   for (int r = 0; r < ml.rounds; r++) {
     for (int stp = 0; stp < ml.steps; stp++) {
@@ -136,13 +160,16 @@ __global__ __launch_bounds__(256) void k_mh(ModelView<R> m, ChainState cs, MHLau
       // visible part: Re a_i (s_i' - s_i) = -2 a_i s_i per flipped site
       R dvis = (R)-2 * m.a_re[i] * si;
       if (j >= 0) dvis += (R)2 * m.a_re[j] * si;   // s_j = -s_i
-      const double d = 2.0 * ((double)lsum + (double)dvis);
+      const double dl = (double)lsum + (double)dvis;
+      const double d = 2.0 * (off + dl);
       bool acc;
       if (d >= 0.0) acc = true;
       else acc = (log(xs_uniform(rng)) < d);
       ntot++;
       if (acc) {
         nacc++;
+        la_cur += dl;
+        off = 0.0;
         sp[i >> 6] ^= 1ULL << (i & 63);
         if (j >= 0) sp[j >> 6] ^= 1ULL << (j & 63);
 #pragma unroll
@@ -174,6 +201,7 @@ __global__ __launch_bounds__(256) void k_mh(ModelView<R> m, ChainState cs, MHLau
   if (lane == 0) {
     for (int w = 0; w < nw; w++) cs.spins[(size_t)c * nw + w] = sp[w];
     cs.rng[c] = rng;
+    cs.la[c] = la_cur - off;
     cs.acc[c] += nacc;
     cs.tot[c] += ntot;
   }
@@ -251,6 +279,22 @@ __global__ void k_unpack(ChainState cs, int chain, int* out) {
   if (i >= cs.N) return;
   const uint64_t w = cs.spins[(size_t)chain * cs.nw + (i >> 6)];
   out[i] = ((w >> (i & 63)) & 1ULL) ? -1 : 1;
+}
+
+__global__ void k_unpack_all(ChainState cs, int* out) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)cs.C * cs.N) return;
+  const int c = (int)(e / cs.N), i = (int)(e % cs.N);
+  const uint64_t w = cs.spins[(size_t)c * cs.nw + (i >> 6)];
+  out[e] = ((w >> (i & 63)) & 1ULL) ? -1 : 1;
+}
+
+int launch_unpack_all(const ChainState& cs, int* out, hipStream_t st) {
+  const long total = (long)cs.C * cs.N;
+  hipLaunchKernelGGL(k_unpack_all, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, cs, out);
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
 }
 
 int launch_unpack(const ChainState& cs, int chain, int* out, hipStream_t st) {

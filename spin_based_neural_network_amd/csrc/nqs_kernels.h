@@ -56,6 +56,11 @@ struct ChainState {
   uint64_t* rng = nullptr;        // [C]
   unsigned long long* acc = nullptr;  // [C]
   unsigned long long* tot = nullptr;  // [C]
+  // log|psi| of each chain's configuration as of its last accepted move (or
+  // creation), evaluated with the parameters of that moment: the reference
+  // sampler keeps this value across parameter updates
+  // (src/nqs/nqs_sampler.c:50-76, 106-113) and the acceptance test uses it.
+  double* la = nullptr;               // [C]
 };
 
 // ---- model (k_model.hip) --------------------------------------------------
@@ -88,6 +93,8 @@ template <typename R>
 int launch_mh(const ModelView<R>& m, ChainState& cs, const MHLaunch& L, hipStream_t st);
 int launch_chain_init(ChainState& cs, const uint64_t* seeds_dev, int sz_zero, hipStream_t st);
 int launch_unpack(const ChainState& cs, int chain, int* out, hipStream_t st);
+// all chains' configurations as int32 rows [C][N]
+int launch_unpack_all(const ChainState& cs, int* out, hipStream_t st);
 
 // ---- local energy (k_eloc.hip) -------------------------------------------
 template <typename R>

@@ -13,7 +13,7 @@ import pytest
 
 import scenarios
 from conftest import golden_case
-from spin_based_neural_network_amd.abi import (config_defaults, fnptr,
+from spin_based_neural_network_amd.abi import (SrInfo, config_defaults, fnptr,
                                                options_defaults, ptr)
 
 pytestmark = pytest.mark.gpu
@@ -75,21 +75,117 @@ def test_sampler_bit_exact(hip_results, golden, sc):
             np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
 
 
+# CG-family steps: conjugate gradients amplify ulp-level differences of
+# their inputs (local energies and O agree to ~1e-15) by the conditioning
+# of S + eps I, and the stopping test |r|^2 <= tol^2 |r0|^2 can fall on the
+# other side of the threshold one iteration earlier or later.  Parity there
+# is: parameter updates within 1e-4 of the update size (observed: <= 1.4e-5,
+# after an unconverged 100-iteration solve), iteration counts
+# within one, energies within 1e-7; the solver contract itself is checked
+# exactly in test_cg_step_solver_contract.  MinSR (a direct solve) keeps
+# the strict 1e-10 bound.
+def _cg_family(sc):
+    return sc.get("method", "") in ("cg", "holo")
+
+
 @pytest.mark.parametrize("sc", scenarios.SR, ids=lambda s: s["name"])
 def test_sr_step(hip_results, golden, sc):
     got = hip_results[f"sr/{sc['name']}"]
     ref = golden_case(golden, f"sr/{sc['name']}")
-    _close(got["params"], ref["params"], 1e-10)
-    _close(got["info"][:, :4], ref["info"][:, :4], 1e-9)
-    np.testing.assert_array_equal(got["info"][:, 4:], ref["info"][:, 4:])
+    if not _cg_family(sc):
+        _close(got["params"], ref["params"], 1e-10)
+        _close(got["info"][:, :4], ref["info"][:, :4], 1e-9)
+        np.testing.assert_array_equal(got["info"][:, 4:], ref["info"][:, 4:])
+        return
+    step = np.abs(np.diff(ref["params"], axis=0)).max()
+    err = np.abs(got["params"] - ref["params"]).max()
+    assert err <= 1e-4 * step + 1e-12, f"param err {err:.3e} vs update size {step:.3e}"
+    _close(got["info"][:, :4], ref["info"][:, :4], 1e-7)
+    assert np.all(np.abs(got["info"][:, 4] - ref["info"][:, 4]) <= 1)
+    # a converged flag may only differ where the iteration count does
+    same_it = got["info"][:, 4] == ref["info"][:, 4]
+    np.testing.assert_array_equal(got["info"][same_it, 5], ref["info"][same_it, 5])
 
 
 @pytest.mark.parametrize("sc", scenarios.RUN, ids=lambda s: s["name"])
 def test_sr_run(hip_results, golden, sc):
     got = hip_results[f"run/{sc['name']}"]
     ref = golden_case(golden, f"run/{sc['name']}")
-    _close(got["trace"], ref["trace"], 1e-9)
-    _close(got["params"], ref["params"], 1e-9)
+    tol = 1e-6 if _cg_family(sc) else 1e-9
+    _close(got["trace"], ref["trace"], tol)
+    _close(got["params"], ref["params"], tol * 10)
+
+
+@pytest.mark.parametrize("sc", [s for s in scenarios.SR if s["method"] in ("cg", "holo")],
+                         ids=lambda s: s["name"])
+def test_cg_step_solver_contract(hip, orc, sc):
+    """First step from identical inputs: the device update solves the
+    oracle's (S + eps I) delta = F to the CG tolerance, and lies as close to
+    the exact solution as the reference's own CG iterate does (up to the
+    same tolerance)."""
+    cfg = config_defaults(**sc["cfg"])
+    Lx, Ly = sc["Lx"], sc["Ly"]
+    N = 3 * Lx * Ly if cfg.hamiltonian == 6 else Lx * Ly
+    holo = sc["method"] == "holo"
+    marsh = (Lx, Ly) if sc.get("marshall") else None
+    r_h, r_o = scenarios.Runner(hip, "nqs_"), scenarios.Runner(orc, "orc_")
+    a_h, a_o = r_h.ansatz(cfg, N), r_o.ansatz(cfg, N)
+    fh, uh = r_h.callback(a_h, marsh)
+    fo, uo = r_o.callback(a_o, marsh)
+    s_h = hip.nqs_sampler_create(N, ctypes.byref(cfg), fh, uh)
+    s_o = orc.orc_sampler_create(N, ctypes.byref(cfg), fo, uo)
+    s_t = orc.orc_sampler_create(N, ctypes.byref(cfg), fo, uo)   # twin: same samples
+    for f, s in ((hip.nqs_sampler_thermalize, s_h), (orc.orc_sampler_thermalize, s_o),
+                 (orc.orc_sampler_thermalize, s_t)):
+        f(s)
+    B = cfg.num_samples
+    smp = np.zeros((B, N), dtype=np.int32)
+    orc.orc_sampler_batch(s_t, B, ptr(smp))
+    P = orc.orc_ansatz_num_params(a_o)
+    e_re, e_im = np.zeros(B), np.zeros(B)
+    orc.orc_local_energy_batch_complex(ctypes.byref(cfg), Lx, Ly, ptr(smp), B, fo, uo, ptr(e_re), ptr(e_im))
+    R, I = np.zeros((B, P)), np.zeros((B, P))
+    g, gi = np.zeros(P), np.zeros(P)
+    for b in range(B):
+        row = np.ascontiguousarray(smp[b])
+        if holo:
+            orc.orc_ansatz_logpsi_gradient_complex(a_o, ptr(row), N, ptr(g), ptr(gi))
+            R[b], I[b] = g, gi
+        else:
+            orc.orc_ansatz_logpsi_gradient(a_o, ptr(row), N, ptr(g))
+            R[b] = g
+    Rc, Ic = R - R.mean(0), I - I.mean(0)
+    eps = cfg.sr_diag_shift
+    if holo:
+        S = 2.0 * (Rc.T @ Rc + Ic.T @ Ic) / B
+        F = 2.0 * (Rc.T @ (e_re - e_re.mean()) + Ic.T @ (e_im - e_im.mean())) / B
+    else:
+        S = Rc.T @ Rc / B
+        F = Rc.T @ (e_re - e_re.mean()) / B
+    A = S + eps * np.eye(P)
+    exact = np.linalg.solve(A, F)
+    deltas, infos = {}, {}
+    for name, lib, pre, a, s, fn, user in (("hip", hip, "nqs_", a_h, s_h, fh, uh),
+                                           ("orc", orc, "orc_", a_o, s_o, fo, uo)):
+        runner = r_h if name == "hip" else r_o
+        p0 = runner.params(a)
+        info = SrInfo()
+        step = "sr_step_holomorphic_full" if holo else "sr_step_custom_full"
+        assert getattr(lib, pre + step)(ctypes.byref(cfg), Lx, Ly, a, s, fn, user, None, None,
+                                        ctypes.byref(info)) == 0
+        deltas[name] = (p0 - runner.params(a)) / cfg.learning_rate
+        infos[name] = info
+    tol = cfg.sr_cg_tol
+    if infos["hip"].converged:
+        res = np.linalg.norm(A @ deltas["hip"] - F) / np.linalg.norm(F)
+        assert res <= tol * 1.01, f"relative residual {res:.3e} > tol {tol:.1e}"
+    lam_min = np.linalg.eigvalsh(A).min()
+    bound = np.linalg.norm(deltas["orc"] - exact) + 2.02 * tol * np.linalg.norm(F) / lam_min
+    assert np.linalg.norm(deltas["hip"] - exact) <= bound
+    for s, f in ((s_h, hip.nqs_sampler_free), (s_o, orc.orc_sampler_free), (s_t, orc.orc_sampler_free)):
+        f(s)
+    hip.nqs_ansatz_free(a_h)
+    orc.orc_ansatz_free(a_o)
 
 
 # ---- reference known answers through the device and adaptor paths ---------
@@ -248,7 +344,8 @@ def test_cholesky_failure_state_matches_unblocked(hip, orc):
         A[np.arange(B), np.arange(B)] += 0.0
         b = np.ascontiguousarray(rng.standard_normal(B))
         x = np.zeros(B)
-        ok = hip.nqs_hip_cholesky_solve(ptr(A.copy()), B, ptr(b), ptr(x))
+        Ah = np.ascontiguousarray(A.copy())
+        ok = hip.nqs_hip_cholesky_solve(ptr(Ah), B, ptr(b), ptr(x))
         # oracle: same two-attempt semantics on the CPU
         Ao = np.ascontiguousarray(A.copy())
         rc = orc.orc_cholesky_lower(ptr(Ao), B)
