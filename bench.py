@@ -1,0 +1,349 @@
+"""Benchmark of the NQS VMC hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
+                    [--precision fp64|fp32] [--gram direct|factored] [--no-cpu]
+
+A step is one full SR iteration (Metropolis sampling of every chain, local
+energies, O_k, MinSR / CG-SR solve, parameter update) through the library's
+C ABI.  Default workload (the 6x6 Heisenberg configuration of the metric):
+periodic 6x6 Heisenberg AFM with the Marshall sign, Sz = 0, complex RBM
+alpha = 4 (M = 144, Re/Im ~ N(0, 0.01)), 1024 chains x 8 samples = 8192
+samples per iteration, exchange moves with one sweep (N proposals) between
+samples, MinSR with shift 1e-4.  Prints one JSON line.
+
+value = effective samples per second through whole SR iterations:
+sum over chains of n_c / (2 tau_int) with tau_int from the pooled per-chain
+autocorrelation of E_loc (Sokal window, c = 5), divided by the wall time of
+the timed iterations.  ms_per_step is the SR-iteration wall time.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from spin_based_neural_network_amd import abi  # noqa: E402
+
+METRIC = "effective samples/s & SR-iter wall time, learned vs plain MH, same ΔE vs ED"
+PEAK_FP64 = 78.6e12          # MI355X dense FP64 (vector / MFMA), FLOP/s
+PEAK_HBM = 8.0e12            # MI355X HBM3E, B/s
+
+WORKLOADS = {
+    # 6x6 periodic Heisenberg + Marshall, complex RBM alpha=4, MinSR
+    "c2": dict(desc="Heisenberg AFM 6x6 PBC, Marshall sign, Sz=0; complex RBM alpha=4 (M=144, "
+                    "P=10728 real); 1024 chains x 8 samples = 8192/iter; exchange moves, "
+                    "1 sweep between samples; MinSR shift 1e-4",
+               ham=1, Lx=6, Ly=6, kind=6, M=144, chains=1024, per_chain=8, move=1, sz0=1,
+               periodic=1, marshall=True, method="minsr", shift=1e-4, lr=1e-2, ed=None),
+    # 1D TFIM N=16 PBC, RBM alpha=2, CG-SR
+    "c1": dict(desc="TFIM chain N=16 PBC J=h=1; RBM alpha=2 (M=32, P=560); 64 chains x 16 = "
+                    "1024/iter; single flips; CG-SR shift 1e-3 lr 0.01",
+               ham=0, Lx=16, Ly=1, kind=5, M=32, chains=64, per_chain=16, move=0, sz0=0,
+               periodic=1, marshall=False, method="cg", shift=1e-3, lr=1e-2,
+               ed=-2.0 / math.sin(math.pi / 32)),
+    # J1-J2 10x10 PBC J2=0.5, RBM alpha=8, MinSR 16384^2 Gram
+    "c3": dict(desc="J1-J2 10x10 PBC J2/J1=0.5, Sz=0; RBM alpha=8 (M=800, P=80900); 4096 "
+                    "chains x 4 = 16384/iter; exchange moves; MinSR 16384^2 Gram",
+               ham=2, Lx=10, Ly=10, kind=5, M=800, chains=4096, per_chain=4, move=1, sz0=1,
+               periodic=1, marshall=False, method="minsr", shift=1e-4, lr=1e-2, ed=None),
+    # kagome 2x4 PBC (N=24), complex RBM M=96, MinSR
+    "c5": dict(desc="kagome Heisenberg 2x4 cells PBC (N=24), Sz=0; complex RBM M=96; 2048 "
+                    "chains x 8 = 16384/iter; exchange moves; MinSR",
+               ham=6, Lx=2, Ly=4, kind=6, M=96, chains=2048, per_chain=8, move=1, sz0=1,
+               periodic=1, marshall=False, method="minsr", shift=1e-4, lr=1e-2,
+               ed=-10.759897248084),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pooled_tau(series):
+    """tau_int from the chain-averaged autocorrelation (Sokal window c=5).
+
+    series: [C][n] per-chain E_loc series."""
+    x = np.asarray(series, dtype=float)
+    x = x - x.mean(axis=1, keepdims=True)
+    C, n = x.shape
+    var = np.mean(x * x)
+    if var <= 0 or n < 4:
+        return 0.5
+    tau = 0.5
+    for t in range(1, n):
+        rho = np.mean(x[:, :-t] * x[:, t:]) * n / (n - t) / var if t < n else 0.0
+        tau += rho
+        if t >= 5.0 * tau:
+            break
+    return max(tau, 0.5)
+
+
+def setup(lib, w, args):
+    N = 3 * w["Lx"] * w["Ly"] if w["ham"] == 6 else w["Lx"] * w["Ly"]
+    cfg = abi.config_defaults(ansatz=w["kind"], hamiltonian=w["ham"], rbm_hidden_units=w["M"],
+                              j_coupling=1.0, j2_coupling=0.5, transverse_field=1.0, kagome_pbc=1,
+                              num_samples=w["chains"] * w["per_chain"], num_thermalize=4 * N,
+                              num_decorrelate=N, rng_seed=20260,
+                              learning_rate=w["lr"], sr_diag_shift=w["shift"],
+                              sr_cg_max_iters=100, sr_cg_tol=1e-6)
+    a = lib.nqs_ansatz_create(ctypes.byref(cfg), N)
+    if not a:
+        raise RuntimeError("nqs_ansatz_create failed")
+    P = lib.nqs_ansatz_num_params(a)
+    par = np.ctypeslib.as_array(ctypes.cast(lib.nqs_ansatz_params_raw(a), ctypes.POINTER(ctypes.c_double)), (P,))
+    par[:] = np.random.default_rng(12345).normal(0.0, 0.01, P)
+    fn, user, keep = abi.fnptr(lib, "nqs_ansatz_log_amp"), a, None
+    if w["marshall"]:
+        keep = abi.MarshallWrapper(fn, a, w["Lx"], w["Ly"])
+        fn, user = abi.fnptr(lib, "nqs_marshall_log_amp"), ctypes.addressof(keep)
+    opt = abi.options_defaults(num_chains=w["chains"], move=w["move"], sz_zero_start=w["sz0"],
+                               periodic=w["periodic"],
+                               precision=abi.FP32 if args.precision == "fp32" else abi.FP64,
+                               gram=abi.GRAM_FACTORED if args.gram == "factored" else abi.GRAM_DIRECT)
+    s = lib.nqs_hip_sampler_create_ex(N, ctypes.byref(cfg), ctypes.byref(opt), fn, user)
+    if not s:
+        raise RuntimeError("nqs_hip_sampler_create_ex failed")
+    return dict(cfg=cfg, a=a, s=s, fn=fn, user=user, keep=keep, N=N, P=P, opt=opt)
+
+
+def step(lib, w, st):
+    info = abi.SrInfo()
+    fname = {"minsr": "nqs_sr_step_minsr_full", "cg": "nqs_sr_step_custom_full",
+             "holo": "nqs_sr_step_holomorphic_full"}[w["method"]]
+    rc = getattr(lib, fname)(ctypes.byref(st["cfg"]), w["Lx"], w["Ly"], st["a"], st["s"], st["fn"],
+                             st["user"], None, None, ctypes.byref(info))
+    if rc != 0:
+        raise RuntimeError(f"{fname} returned {rc}")
+    return info
+
+
+TAGS = ["mh", "forward", "eloc", "gen_O", "colsum", "gemm", "gemm_potrf", "potrf_diag", "trsm"]
+
+
+def kernel_work(tag, w, st, launches):
+    """Algorithmic work of one launch of `tag`: (bound, amount, unit)."""
+    B = w["chains"] * w["per_chain"]
+    P, N, M = st["P"], st["N"], w["M"]
+    if tag == "gemm":
+        # lower triangle of T = O_c O_c^T: B(B+1)/2 dot products of length P
+        return "mfma", B * (B + 1) * P, "FLOP"
+    if tag == "gemm_potrf":
+        nb, tot, k0 = 64, 0, 0
+        while k0 < B:
+            r = B - min(k0 + nb, B)
+            tot += r * (r + 1) * min(nb, B - k0)
+            k0 += nb
+        return "mfma", tot / max(launches, 1), "FLOP"
+    if tag == "gen_O":
+        return "hbm", B * P * 8 + B * N * 4 + B * M * 8 * (2 if w["kind"] == 6 else 1), "B"
+    if tag == "colsum":
+        return "hbm", B * P * 8, "B"
+    return "hbm", B * N * 4, "B"
+
+
+def gpu_run(args, w):
+    lib = abi.load_library()
+    st = setup(lib, w, args)
+    log(f"[bench] workload {args.workload}: N={st['N']} P={st['P']} samples/iter="
+        f"{w['chains'] * w['per_chain']}")
+    lib.nqs_sampler_thermalize(st["s"])
+    # one profiling step: which kernel dominates the iteration
+    lib.nqs_hip_kernel_timing(",".join(TAGS).encode())
+    step(lib, w, st)
+    per = {}
+    for t in TAGS:
+        c, ms = ctypes.c_longlong(0), ctypes.c_double(0)
+        if lib.nqs_hip_kernel_stats(t.encode(), ctypes.byref(c), ctypes.byref(ms)) == 0 and c.value:
+            per[t] = (c.value, ms.value)
+    lib.nqs_hip_kernel_timing(None)
+    dom = max(per, key=lambda t: per[t][1]) if per else "gemm"
+    log("[bench] kernel split of one iteration (launches, ms): " +
+        ", ".join(f"{t}={c}/{ms:.2f}" for t, (c, ms) in sorted(per.items(), key=lambda kv: -kv[1][1])))
+    for i in range(args.warmup):
+        step(lib, w, st)
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    C, R = w["chains"], w["per_chain"]
+    series = np.zeros((C, 0))
+    buf = np.zeros(C * R)
+    lib.nqs_hip_kernel_timing(dom.encode())
+    energies, infos, stages = [], [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        info = step(lib, w, st)
+        n = lib.nqs_hip_sampler_last_energies(st["s"], abi.ptr(buf), C * R)
+        series = np.concatenate([series, buf[: C * n].reshape(C, n)], axis=1)
+        energies.append(info.mean_energy)
+        infos.append(info.as_dict())
+        sm = np.zeros(7)
+        if lib.nqs_hip_last_stage_ms(st["s"], abi.ptr(sm)) == 0:
+            stages.append(sm.copy())
+    elapsed = time.perf_counter() - t0
+    c, ms = ctypes.c_longlong(0), ctypes.c_double(0)
+    lib.nqs_hip_kernel_stats(dom.encode(), ctypes.byref(c), ctypes.byref(ms))
+    lib.nqs_hip_kernel_timing(None)
+    log(f"[bench] timed {args.steps} iterations in {elapsed:.3f} s")
+    tau = pooled_tau(series)
+    n_total = series.shape[0] * series.shape[1]
+    ess = n_total / (2.0 * tau)
+    avg_ms = ms.value / max(c.value, 1)
+    bound, work, unit = kernel_work(dom, w, st, c.value)
+    if bound == "mfma":
+        achieved, peak, runit = work / (avg_ms * 1e-3) / 1e12, PEAK_FP64 / 1e12, "TFLOP/s"
+    else:
+        achieved, peak, runit = work / (avg_ms * 1e-3) / 1e9, PEAK_HBM / 1e9, "GB/s"
+    res = dict(
+        ess_per_s=ess / elapsed, ms_per_step=1e3 * elapsed / args.steps, tau=tau,
+        energies=energies, acceptance=infos[-1]["acceptance_ratio"],
+        stage_ms=np.mean(stages, axis=0).tolist() if stages else None,
+        roofline=dict(kernel=dom, bound=bound, achieved=achieved, peak=peak, unit=runit,
+                      frac=achieved / peak, traffic=None, launches=c.value, avg_ms=avg_ms,
+                      work_per_launch=work, work_unit=unit),
+        split={t: dict(launches=cc, ms=mm) for t, (cc, mm) in per.items()}, N=st["N"], P=st["P"])
+    lib.nqs_sampler_free(st["s"])
+    lib.nqs_ansatz_free(st["a"])
+    return res
+
+
+def cpu_baseline(w, gpu):
+    """Reference CPU path on a bounded sample of the same workload, 1 core.
+
+    Per-sample cost (MH moves, E_loc, O row) and the MinSR linear-algebra
+    MAC rate are measured on the compiled reference (oracle/_ref) when it is
+    present, else on the oracle port; the full-size iteration is
+    extrapolated from exact operation counts."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import load_oracle, load_reference
+    lib, pre, kind = load_reference(), "nqs_", "reference"
+    if lib is None:
+        lib, pre, kind = load_oracle(), "orc_", "port"
+    f = lambda n: getattr(lib, pre + n)
+    # the reference has open boundaries only: time the open twin and scale
+    # E_loc by the ratio of off-diagonal bond counts
+    N, Lx, Ly = gpu["N"], w["Lx"], w["Ly"]
+    B_full = w["chains"] * w["per_chain"]
+    cfg = abi.config_defaults(ansatz=w["kind"], hamiltonian=w["ham"], rbm_hidden_units=w["M"],
+                              j2_coupling=0.5, num_thermalize=0, num_decorrelate=N, rng_seed=7,
+                              rbm_init_scale=0.06)
+    a = f("ansatz_create")(ctypes.byref(cfg), N)
+    P = f("ansatz_num_params")(a)
+    fn = abi.fnptr(lib, pre + "ansatz_log_amp")
+    s = f("sampler_create")(N, ctypes.byref(cfg), fn, a)
+    nb = 32
+    smp = np.zeros((nb, N), dtype=np.int32)
+    t0 = time.perf_counter()
+    f("sampler_batch")(s, nb, abi.ptr(smp))
+    t_mh = (time.perf_counter() - t0) / nb                       # N moves per sample
+    if w["sz0"]:
+        rng = np.random.default_rng(3)
+        base = np.array([1] * (N // 2) + [-1] * (N - N // 2), dtype=np.int32)
+        smp = np.ascontiguousarray(np.stack([rng.permutation(base) for _ in range(nb)]))
+    e = np.zeros(nb)
+    t0 = time.perf_counter()
+    f("local_energy_batch")(ctypes.byref(cfg), Lx, Ly, abi.ptr(smp), nb, fn, a, abi.ptr(e))
+    t_el_open = (time.perf_counter() - t0) / nb
+    bonds_open = {0: (Lx - 1) * Ly + Lx * (Ly - 1) + N, 1: (Lx - 1) * Ly + Lx * (Ly - 1),
+                  2: (Lx - 1) * Ly + Lx * (Ly - 1) + 2 * (Lx - 1) * (Ly - 1), 6: 6 * Lx * Ly}
+    bonds_per = {0: 2 * N, 1: 2 * N, 2: 4 * N, 6: 6 * Lx * Ly}
+    if w["Ly"] == 1:
+        bonds_open[0], bonds_per[0] = (Lx - 1) + N, Lx + N
+    t_el = t_el_open * bonds_per[w["ham"]] / bonds_open[w["ham"]]
+    g = np.zeros(P)
+    t0 = time.perf_counter()
+    for b in range(nb):
+        f("ansatz_logpsi_gradient")(a, abi.ptr(np.ascontiguousarray(smp[b])), N, abi.ptr(g))
+    t_o = (time.perf_counter() - t0) / nb
+    per_sample = t_mh + t_el + t_o
+    # MinSR linear algebra rate: one full reference MinSR step at Ns = 768
+    # (384 for the largest parameter counts)
+    ns = 768 if P < 20000 else 384
+    cfg2 = abi.config_defaults(ansatz=w["kind"], hamiltonian=w["ham"], rbm_hidden_units=w["M"],
+                               j2_coupling=0.5, num_samples=ns, num_decorrelate=1, rng_seed=9,
+                               rbm_init_scale=0.06, sr_diag_shift=1e-3)
+    info = abi.SrInfo()
+    f("sampler_thermalize")(s)
+    t0 = time.perf_counter()
+    f("sr_step_minsr_full")(ctypes.byref(cfg2), Lx, Ly, a, s, None, None, None, None, ctypes.byref(info))
+    t_step = time.perf_counter() - t0
+    per_sample_small = t_mh + t_el_open + t_o
+    macs_small = ns * (ns + 1) / 2 * P + ns ** 3 / 6 + 2 * ns * P
+    rate = macs_small / max(t_step - ns * per_sample_small, 1e-6)
+    if w["method"] == "minsr":
+        macs_full = B_full * (B_full + 1) / 2 * P + B_full ** 3 / 6 + 2 * B_full * P
+    else:   # CG-SR: 100 iterations x 2 passes over O, plus means and force
+        macs_full = 100 * 2 * B_full * P + 2 * B_full * P
+    t_iter = B_full * per_sample + macs_full / rate
+    f("sampler_free")(s)
+    f("ansatz_free")(a)
+    ess_cpu = (B_full / (2.0 * gpu["tau"])) / t_iter
+    sample = (f"{kind} CPU path, 1 core: {nb} samples of MH ({N} moves each), E_loc (open twin, "
+              f"scaled to the periodic bond count) and O rows, plus one full MinSR step at "
+              f"Ns={ns} for the linear-algebra MAC rate ({rate / 1e9:.2f} GMAC/s); the "
+              f"{B_full}-sample iteration ({t_iter:.1f} s) is extrapolated from exact "
+              f"operation counts with the GPU run's tau_int")
+    return dict(value=ess_cpu, unit="effective samples/s", cores=1, kind=kind, sample=sample,
+                sr_iter_s=t_iter)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=12)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--gram", default="direct", choices=["direct", "factored"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of "
+              f"this path is a design, not a build", file=sys.stderr)
+        sys.exit(2)
+    w = WORKLOADS[args.workload]
+    gpu = gpu_run(args, w)
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] timing the reference CPU path on a bounded sample")
+        cpu = cpu_baseline(w, gpu)
+    out = {
+        "metric": METRIC,
+        "value": gpu["ess_per_s"],
+        "unit": "effective samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": gpu["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic (random-init parameters N(0,0.01), chains started at random Sz=0 states)",
+        "config": {"workload": args.workload + ": " + w["desc"], "N": gpu["N"], "P": gpu["P"],
+                   "samples_per_iter": w["chains"] * w["per_chain"], "chains": w["chains"],
+                   "method": w["method"], "gram": args.gram, "precision": args.precision},
+        "sr_iter_ms": gpu["ms_per_step"],
+        "tau_int": gpu["tau"],
+        "acceptance": gpu["acceptance"],
+        "energy_last": gpu["energies"][-1],
+        "stage_ms": dict(zip(["sample", "eloc", "O_reduce", "solve", "update"], gpu["stage_ms"][:5]))
+        if gpu["stage_ms"] else None,
+        "kernel_split_ms": {k: v["ms"] for k, v in gpu["split"].items()},
+        "roofline": {k: gpu["roofline"][k] for k in ("bound", "achieved", "peak", "unit", "frac",
+                                                     "traffic", "kernel", "avg_ms")},
+        "cpu_baseline": cpu,
+    }
+    if w["ed"] is not None:
+        out["delta_e_vs_ed"] = (gpu["energies"][-1] - w["ed"]) / abs(w["ed"])
+    if cpu:
+        out["speedup_vs_cpu_iter"] = cpu["sr_iter_s"] / (gpu["ms_per_step"] * 1e-3)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

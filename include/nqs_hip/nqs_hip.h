@@ -125,6 +125,14 @@ int nqs_hip_last_stage_ms(const nqs_sampler_t *s, double out[7]);
 int nqs_hip_gram(const double *O, int B, long P, double scale, double *T);
 int nqs_hip_cholesky_solve(double *T, int n, const double *b, double *x);
 
+/* Live kernel timing with HIP events on the library's stream.
+ * nqs_hip_kernel_timing("eloc,gemm") starts timing the listed kernel tags
+ * (forward, gen_O, mh, eloc, colsum, gemm, gemm_potrf, potrf_diag, trsm;
+ * NULL or "" stops) and resets the counts; nqs_hip_kernel_stats returns the
+ * number of timed launches of a tag and their total duration. */
+int nqs_hip_kernel_timing(const char *names);
+int nqs_hip_kernel_stats(const char *name, long long *count, double *total_ms);
+
 /* Diagnostics of the device path: how many kernel launches the library has
  * made, and how many host-callback adaptor calls it has made (a callback
  * that is not nqs_ansatz_log_amp or nqs_marshall_log_amp over it runs on

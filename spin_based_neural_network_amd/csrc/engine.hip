@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <atomic>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/nqs/nqs_ansatz.h"
@@ -126,6 +127,92 @@ __global__ void k_init_params(double* p, int kind, int N, int M, double scale, u
 
 void nqs::count_launch(int n) { g_launches += n; }
 
+namespace {
+struct KStat {
+  std::string name;
+  long long count = 0;
+  double total_ms = 0.0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+std::mutex g_kt_mu;
+std::vector<KStat> g_kstats;
+std::atomic<int> g_kt_on{0};
+
+void kt_resolve() {
+  for (KStat& k : g_kstats) {
+    for (auto& pr : k.pending) {
+      float ms = 0.f;
+      if (hipEventSynchronize(pr.second) == hipSuccess &&
+          hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+        k.total_ms += ms;
+        k.count++;
+      }
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+    k.pending.clear();
+  }
+}
+}  // namespace
+
+nqs::KScope::KScope(const char* name, hipStream_t stream) {
+  if (!g_kt_on.load()) return;
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  for (size_t i = 0; i < g_kstats.size(); i++)
+    if (g_kstats[i].name == name) slot = (int)i;
+  if (slot < 0) return;
+  st = stream;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    slot = -1;
+    return;
+  }
+  (void)hipEventRecord(e0, st);
+}
+
+nqs::KScope::~KScope() {
+  if (slot < 0) return;
+  (void)hipEventRecord(e1, st);
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  g_kstats[slot].pending.push_back({e0, e1});
+}
+
+// names: comma-separated kernel tags to time (NULL or "" disables);
+// resets the statistics.
+NQS_EXPORT int nqs_hip_kernel_timing(const char* names) {
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  kt_resolve();
+  g_kstats.clear();
+  g_kt_on = 0;
+  if (!names || !*names) return 0;
+  std::string all(names);
+  size_t pos = 0;
+  while (pos <= all.size()) {
+    size_t q = all.find(',', pos);
+    if (q == std::string::npos) q = all.size();
+    if (q > pos) {
+      KStat k;
+      k.name = all.substr(pos, q - pos);
+      g_kstats.push_back(k);
+    }
+    pos = q + 1;
+  }
+  g_kt_on = 1;
+  return (int)g_kstats.size();
+}
+
+// launches and total milliseconds of one timed kernel tag (0 on success)
+NQS_EXPORT int nqs_hip_kernel_stats(const char* name, long long* count, double* total_ms) {
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  kt_resolve();
+  for (KStat& k : g_kstats)
+    if (name && k.name == name) {
+      if (count) *count = k.count;
+      if (total_ms) *total_ms = k.total_ms;
+      return 0;
+    }
+  return -1;
+}
+
 // ===========================================================================
 // ansatz
 // ===========================================================================
@@ -207,7 +294,7 @@ bool recognise(nqs_log_amp_fn_t fn, void* user, AmpSource& src) {
 // ---------------------------------------------------------------------------
 struct Workspace {
   DBuf spins, t_re, t_im, la, ar, e_re, e_im, O, Oi, colwork, u1, u2, vecs, T, diag, y, scal,
-      cg, fail, ham, sublat, hostla, hostar;
+      cg, fail, ham, sublat, hostla, hostar, linv, T2;
 };
 Workspace g_ws;
 
@@ -1166,22 +1253,29 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
     CKR(g_ws.y.ensure(sizeof(double) * B * 2));
     CKR(g_ws.fail.ensure(sizeof(int) * 4));
     double* T = g_ws.T.as<double>();
-    // T = O_c O_c^T / Ns (lower triangle on MFMA, mirrored), + eps I
+    // T = O_c O_c^T / Ns + eps I (lower triangle, FP64 MFMA)
     CKR(nqs::launch_gemm_nt(B, B, (int)P, 1.0 / B, O, P, O, P, 0.0, T, B, 1, g_stream));
-    CKR(nqs::launch_mirror_lower(T, B, g_stream));
     CKR(nqs::launch_add_diag(T, B, B, eps, g_stream));
-    CKR(nqs::launch_copy_diag(T, B, g_ws.diag.as<double>(), g_stream));
-    int f = nqs::launch_potrf(T, B, g_ws.diag.as<double>(), g_ws.fail.as<int>(), g_stream);
+    CKR(g_ws.linv.ensure(sizeof(double) * nqs::potrf_linv_elems(B)));
+    double* Linv = g_ws.linv.as<double>();
+    int f = nqs::launch_potrf(T, B, Linv, g_ws.fail.as<int>(), g_stream);
     if (f < 0) return -1;
     if (f > 0) {
-      // retry on the partially factored matrix with +1e-6 on the diagonal
+      // the reference retries on its partially factored matrix with +1e-6 on
+      // the diagonal (nqs_optimizer.c:1156-1160): rebuild that matrix (input
+      // columns from a second Gram evaluation) and factor again
+      CKR(g_ws.T2.ensure(sizeof(double) * (size_t)B * B));
+      double* T2 = g_ws.T2.as<double>();
+      CKR(nqs::launch_gemm_nt(B, B, (int)P, 1.0 / B, O, P, O, P, 0.0, T2, B, 1, g_stream));
+      CKR(nqs::launch_add_diag(T2, B, B, eps, g_stream));
+      CKR(nqs::launch_potrf_repair(T, B, f - 1, T2, g_stream));
       CKR(nqs::launch_add_diag(T, B, B, 1e-6, g_stream));
-      f = nqs::launch_potrf(T, B, nullptr, g_ws.fail.as<int>(), g_stream);
+      f = nqs::launch_potrf(T, B, Linv, g_ws.fail.as<int>(), g_stream);
       if (f < 0) return -1;
     }
     double* y = g_ws.y.as<double>();
     if (f == 0) {
-      CKR(nqs::launch_potrs(T, B, u, y, y + B, g_stream));
+      CKR(nqs::launch_potrs(T, Linv, B, u, y, y + B, g_stream));
       converged = 1;
     } else {
       fprintf(stderr, "nqs_sr_step_minsr: Cholesky breakdown after retry; increase "
@@ -1439,22 +1533,24 @@ NQS_EXPORT int nqs_hip_cholesky_solve(double* T, int n, const double* b, double*
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (!T || !b || !x || n <= 0 || stream_ready()) return -1;
   const size_t tb = sizeof(double) * (size_t)n * n;
-  if (g_ws.T.ensure(tb) || g_ws.diag.ensure(sizeof(double) * n) ||
-      g_ws.y.ensure(sizeof(double) * 3 * n) || g_ws.fail.ensure(16))
+  if (g_ws.T.ensure(tb) || g_ws.T2.ensure(tb) || g_ws.y.ensure(sizeof(double) * 3 * n) ||
+      g_ws.fail.ensure(16) || g_ws.linv.ensure(sizeof(double) * nqs::potrf_linv_elems(n)))
     return -1;
   double* dT = g_ws.T.as<double>();
   double* y = g_ws.y.as<double>();
+  double* Linv = g_ws.linv.as<double>();
   CK(hipMemcpyAsync(dT, T, tb, hipMemcpyHostToDevice, g_stream));
+  CK(hipMemcpyAsync(g_ws.T2.p, dT, tb, hipMemcpyDeviceToDevice, g_stream));
   CK(hipMemcpyAsync(y + 2 * n, b, sizeof(double) * n, hipMemcpyHostToDevice, g_stream));
-  CKR(nqs::launch_copy_diag(dT, n, g_ws.diag.as<double>(), g_stream));
-  int f = nqs::launch_potrf(dT, n, g_ws.diag.as<double>(), g_ws.fail.as<int>(), g_stream);
+  int f = nqs::launch_potrf(dT, n, Linv, g_ws.fail.as<int>(), g_stream);
   if (f < 0) return -1;
   if (f > 0) {
+    CKR(nqs::launch_potrf_repair(dT, n, f - 1, g_ws.T2.as<double>(), g_stream));
     CKR(nqs::launch_add_diag(dT, n, n, 1e-6, g_stream));
-    f = nqs::launch_potrf(dT, n, nullptr, g_ws.fail.as<int>(), g_stream);
+    f = nqs::launch_potrf(dT, n, Linv, g_ws.fail.as<int>(), g_stream);
     if (f < 0) return -1;
   }
-  if (f == 0) CKR(nqs::launch_potrs(dT, n, y + 2 * n, y, y + n, g_stream));
+  if (f == 0) CKR(nqs::launch_potrs(dT, Linv, n, y + 2 * n, y, y + n, g_stream));
   else CKR(nqs::launch_fill(y, 0.0, n, g_stream));
   CK(hipMemcpyAsync(T, dT, tb, hipMemcpyDeviceToHost, g_stream));
   CK(hipMemcpyAsync(x, y, sizeof(double) * n, hipMemcpyDeviceToHost, g_stream));

@@ -264,6 +264,7 @@ static int eloc_go(const ModelView<R>& m, const HamDev& h, const int* spins, int
   }
   int grid = (B + SB - 1) / SB;
   if (grid > 65535) grid = 65535;
+  KScope ks("eloc", st);
   hipLaunchKernelGGL((k_eloc<R, CPLX, IPT>), dim3(grid), dim3(256), bytes, st, m, h, spins, B,
                      t_re, t_im, e_re, e_im, SB, HT);
   count_launch();

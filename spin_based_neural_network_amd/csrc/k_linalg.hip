@@ -53,6 +53,7 @@ int launch_colsum(const double* O, long B, long P, const double* w, double alpha
   const long nch = (B + kColChunk - 1) / kColChunk;
   if (nch > 65535) return -1;
   dim3 g1((unsigned)((P + 255) / 256), (unsigned)nch);
+  KScope ks("colsum", st);
   hipLaunchKernelGGL(k_colsum_partial, g1, dim3(256), 0, st, O, B, P, w, work);
   hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, work,
                      nch, P, alpha, out);
@@ -295,9 +296,9 @@ __device__ inline void lower_tile(int t, int& tm, int& tn) {
 }
 
 __global__ __launch_bounds__(256) void k_gemm_nt(int M, int N, int K, double alpha,
-                                                 const double* __restrict__ A, long lda,
+                                                 const double* A, long lda,
                                                  const double* __restrict__ Bm, long ldb,
-                                                 double beta, double* __restrict__ C, long ldc,
+                                                 double beta, double* C, long ldc,
                                                  int lower, int tiles_n, const int* skip) {
   if (skip && *skip) return;
   __shared__ double As[GT][GKS + 1];
@@ -360,6 +361,7 @@ static int gemm_nt_impl(int M, int N, int K, double alpha, const double* A, long
   const int tm = (M + GT - 1) / GT, tn = (N + GT - 1) / GT;
   long tiles = lower ? (long)tm * (tm + 1) / 2 : (long)tm * tn;
   if (lower && M != N) return -1;
+  KScope ks(skip ? (lower ? "gemm_potrf" : "gemm_trsm") : "gemm", st);
   hipLaunchKernelGGL(k_gemm_nt, dim3((unsigned)tiles), dim3(256), 0, st, M, N, K, alpha, A, lda,
                      B, ldb, beta, C, ldc, lower, tn, skip);
   count_launch();
@@ -373,15 +375,22 @@ int launch_gemm_nt(int M, int N, int K, double alpha, const double* A, long lda,
 }
 
 // --------------------------------------------------------------------------
-// Cholesky, right-looking with 64-wide panels.  The diagonal block is
-// factored column by column (left-looking inside the block, as the
-// reference's loop), the panel below by a row-wise triangular solve, and the
-// trailing matrix by the MFMA GEMM on its lower triangle.
+// Cholesky, right-looking with 64-wide panels.  The panel kernel factors the
+// diagonal block in LDS (right-looking, one column per step) and also
+// inverts it; the rows below are then A21 <- A21 Linv^T on the MFMA GEMM and
+// the trailing matrix is updated on its lower triangle by the same GEMM.
+// The inverted diagonal blocks are kept for the triangular solves.
 // --------------------------------------------------------------------------
 constexpr int NB = 64;
 
-__global__ __launch_bounds__(256) void k_potrf_diag(double* A, int n, int k0, int nb, int* fail) {
+size_t potrf_linv_elems(int n) { return (size_t)((n + NB - 1) / NB) * NB * NB; }
+
+__global__ __launch_bounds__(256) void k_potrf_panel(double* A, int n, int k0, int nb,
+                                                     double* Linv, int* fail) {
   if (*fail) return;
+  // lower triangle (incl. diagonal) holds L; the strict upper triangle holds
+  // the inverse transposed (X[i][c] at L[c][i], c < i); column NB holds the
+  // inverse's diagonal
   __shared__ double L[NB][NB + 1];
   __shared__ int bad;
   const int tid = threadIdx.x;
@@ -394,34 +403,58 @@ __global__ __launch_bounds__(256) void k_potrf_diag(double* A, int n, int k0, in
   int jdone = nb;
   for (int j = 0; j < nb; j++) {
     if (tid == 0) {
-      double d = L[j][j];
-      for (int k = 0; k < j; k++) d -= L[j][k] * L[j][k];
+      const double d = L[j][j];
       if (!(d > 0.0)) bad = j;
       else L[j][j] = sqrt(d);
     }
     __syncthreads();
     if (bad >= 0) { jdone = bad; break; }
     const double inv = 1.0 / L[j][j];
-    for (int i = j + 1 + tid; i < nb; i += blockDim.x) {
-      double s = L[i][j];
-      for (int k = 0; k < j; k++) s -= L[i][k] * L[j][k];
-      L[i][j] = s * inv;
+    for (int i = j + 1 + tid; i < nb; i += blockDim.x) L[i][j] *= inv;
+    __syncthreads();
+    const int m = nb - 1 - j;
+    for (int e = tid; e < m * m; e += blockDim.x) {
+      const int i = j + 1 + e / m, c = j + 1 + e % m;
+      if (c <= i) L[i][c] -= L[i][j] * L[c][j];
     }
     __syncthreads();
   }
-  // write back the factored columns (all of them on success)
   for (int e = tid; e < nb * nb; e += blockDim.x) {
     const int i = e / nb, j = e % nb;
     if (j <= i && j < jdone) A[(long)(k0 + i) * n + k0 + j] = L[i][j];
   }
-  if (tid == 0 && jdone < nb) *fail = k0 + jdone + 1;
+  if (jdone < nb) {
+    if (tid == 0) *fail = k0 + jdone + 1;
+    return;
+  }
+  // invert the diagonal block: thread c solves L x = e_c
+  if (tid < nb) {
+    const int c = tid;
+    L[c][NB] = 1.0 / L[c][c];
+    for (int i = c + 1; i < nb; i++) {
+      double s = 0.0;
+      for (int k = c; k < i; k++) s += L[i][k] * (k == c ? L[c][NB] : L[c][k]);
+      L[c][i] = -s / L[i][i];
+    }
+  }
+  __syncthreads();
+  double* X = Linv + (size_t)(k0 / NB) * NB * NB;
+  for (int e = tid; e < NB * NB; e += blockDim.x) {
+    const int i = e / NB, c = e % NB;
+    double v = 0.0;
+    if (i < nb && c < nb) {
+      if (c == i) v = L[i][NB];
+      else if (c < i) v = L[c][i];
+    }
+    X[e] = v;
+  }
 }
 
 // rows i >= r0: A[i][k0 + j] for j < ncols solved against the diagonal block
-// (x L^T = a, forward over j); thread per row.
-__global__ __launch_bounds__(256) void k_trsm_rows(double* A, int n, int k0, int ncols, int r0,
-                                                   const int* fail, int check) {
-  if (check && *fail) return;
+// (x L^T = a, forward over j); thread per row.  Used to finish the failing
+// panel's factored columns when bringing a failed factorisation to the
+// unblocked algorithm's state.
+__global__ __launch_bounds__(256) void k_trsm_rows(double* A, int n, int k0, int ncols, int r0) {
   __shared__ double L[NB][NB + 1];
   for (int e = threadIdx.x; e < ncols * ncols; e += blockDim.x) {
     const int i = e / ncols, j = e % ncols;
@@ -449,35 +482,41 @@ __global__ __launch_bounds__(256) void k_trsm_rows(double* A, int n, int k0, int
     if (j < ncols) row[j] = x[j];
 }
 
-// restore columns >= jf (lower triangle) from the upper triangle and the
-// saved diagonal; used after a failed factorisation
-__global__ void k_restore_from_upper(double* A, int n, int jf, const double* diag) {
+// lower-triangle columns >= jf of A restored from the lower triangle of orig
+__global__ void k_restore_cols(double* A, const double* orig, int n, int jf) {
   const long total = (long)n * n;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
        e += (long)gridDim.x * blockDim.x) {
     const int i = (int)(e / n), j = (int)(e % n);
-    if (j < jf || j > i) continue;
-    A[e] = (i == j) ? diag[i] : A[(long)j * n + i];
+    if (j >= jf && j <= i) A[e] = orig[e];
   }
 }
 
 __global__ void k_zero_int(int* p) { *p = 0; }
 
-int launch_potrf(double* A, int n, const double* diag_in, int* fail_dev, hipStream_t st) {
+int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, fail_dev);
   count_launch();
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nb = (n - k0) < NB ? (n - k0) : NB;
-    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, st, A, n, k0, nb, fail_dev);
+    {
+      KScope ks("potrf_diag", st);
+      hipLaunchKernelGGL(k_potrf_panel, dim3(1), dim3(256), 0, st, A, n, k0, nb, Linv, fail_dev);
+    }
     count_launch();
     const int r0 = k0 + nb;
     if (r0 < n) {
       const int rows = n - r0;
-      hipLaunchKernelGGL(k_trsm_rows, dim3((rows + 255) / 256), dim3(256), 0, st, A, n, k0, nb,
-                         r0, fail_dev, 1);
-      count_launch();
-      if (gemm_nt_impl(rows, rows, nb, -1.0, A + (long)r0 * n + k0, n, A + (long)r0 * n + k0, n,
-                       1.0, A + (long)r0 * n + r0, n, 1, fail_dev, st))
+      double* panel = A + (long)r0 * n + k0;
+      // A21 <- A21 Linv^T (in place: each workgroup reads its rows before writing)
+      {
+        KScope ks("trsm", st);
+        if (gemm_nt_impl(rows, nb, nb, 1.0, panel, n, Linv + (size_t)(k0 / NB) * NB * NB, NB, 0.0,
+                         panel, n, 0, fail_dev, st))
+          return -1;
+      }
+      if (gemm_nt_impl(rows, rows, nb, -1.0, panel, n, panel, n, 1.0, A + (long)r0 * n + r0, n, 1,
+                       fail_dev, st))
         return -1;
     }
   }
@@ -485,100 +524,96 @@ int launch_potrf(double* A, int n, const double* diag_in, int* fail_dev, hipStre
   int fail = 0;
   NQS_HIP_CHECK(hipMemcpyAsync(&fail, fail_dev, sizeof(int), hipMemcpyDeviceToHost, st));
   NQS_HIP_CHECK(hipStreamSynchronize(st));
-  if (fail && diag_in) {
-    // bring the matrix to the unblocked algorithm's failure state: finish the
-    // failing panel's already factored columns below the block, restore the
-    // rest from the upper triangle
-    const int jf = fail - 1;
-    const int k0 = (jf / NB) * NB;
-    const int ncols = jf - k0;
-    const int nb = (n - k0) < NB ? (n - k0) : NB;
-    const int r0 = k0 + nb;
-    if (ncols > 0 && r0 < n) {
-      hipLaunchKernelGGL(k_trsm_rows, dim3((n - r0 + 255) / 256), dim3(256), 0, st, A, n, k0,
-                         ncols, r0, fail_dev, 0);
-      count_launch();
-    }
-    long blocks = ((long)n * n + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_restore_from_upper, dim3((unsigned)blocks), dim3(256), 0, st, A, n, jf,
-                       diag_in);
-    count_launch();
-    NQS_HIP_CHECK(hipGetLastError());
-  }
   return fail;
 }
 
-// --------------------------------------------------------------------------
-// triangular solves with L (blocked; right-looking updates)
-// --------------------------------------------------------------------------
-__global__ void k_trsv_fwd_block(const double* L, int n, int k0, int nb, double* w) {
-  // one wave: w[k0..k0+nb) = L_kk^-1 w[k0..k0+nb)
-  __shared__ double xs[NB];
-  const int lane = threadIdx.x;
-  if (lane < nb) xs[lane] = w[k0 + lane];
-  __syncthreads();
-  for (int j = 0; j < nb; j++) {
-    double part = (lane < j) ? L[(long)(k0 + j) * n + k0 + lane] * xs[lane] : 0.0;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) part += __shfl_xor(part, m, 64);
-    if (lane == 0) xs[j] = (xs[j] - part) / L[(long)(k0 + j) * n + k0 + j];
-    __syncthreads();
+int launch_potrf_repair(double* A, int n, int jf, const double* orig, hipStream_t st) {
+  // finish the failing panel's factored columns below the block, then
+  // restore the untouched columns from the input
+  const int k0 = (jf / NB) * NB;
+  const int ncols = jf - k0;
+  const int nb = (n - k0) < NB ? (n - k0) : NB;
+  const int r0 = k0 + nb;
+  if (ncols > 0 && r0 < n) {
+    hipLaunchKernelGGL(k_trsm_rows, dim3((n - r0 + 255) / 256), dim3(256), 0, st, A, n, k0, ncols, r0);
+    count_launch();
   }
-  if (lane < nb) w[k0 + lane] = xs[lane];
-}
-__global__ void k_trsv_fwd_update(const double* L, int n, int k0, int nb, double* w) {
-  const long i = (long)k0 + nb + blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double* row = L + i * n + k0;
-  double s = 0.0;
-  for (int k = 0; k < nb; k++) s += row[k] * w[k0 + k];
-  w[i] -= s;
-}
-__global__ void k_trsv_bwd_block(const double* L, int n, int k0, int nb, double* w) {
-  // w[k0..k0+nb) = L_kk^-T w[k0..k0+nb)
-  __shared__ double xs[NB];
-  const int lane = threadIdx.x;
-  if (lane < nb) xs[lane] = w[k0 + lane];
-  __syncthreads();
-  for (int j = nb - 1; j >= 0; j--) {
-    double part = (lane > j && lane < nb) ? L[(long)(k0 + lane) * n + k0 + j] * xs[lane] : 0.0;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) part += __shfl_xor(part, m, 64);
-    if (lane == 0) xs[j] = (xs[j] - part) / L[(long)(k0 + j) * n + k0 + j];
-    __syncthreads();
-  }
-  if (lane < nb) w[k0 + lane] = xs[lane];
-}
-__global__ void k_trsv_bwd_update(const double* L, int n, int k0, int nb, double* w) {
-  // w[i] -= sum_{k in block} L[k][i] x_k  for i < k0
-  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i >= k0) return;
-  double s = 0.0;
-  for (int k = 0; k < nb; k++) s += L[(long)(k0 + k) * n + i] * w[k0 + k];
-  w[i] -= s;
+  long blocks = ((long)n * n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_restore_cols, dim3((unsigned)blocks), dim3(256), 0, st, A, orig, n, jf);
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
 }
 
-int launch_potrs(const double* L, int n, const double* b, double* x, double* z, hipStream_t st) {
-  (void)z;
-  NQS_HIP_CHECK(hipMemcpyAsync(x, b, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st));
+// --------------------------------------------------------------------------
+// triangular solves L L^T x = b with the inverted diagonal blocks: one
+// launch per block and direction; every workgroup recomputes the block's
+// solution (64x64 product) and workgroup 0 publishes it
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_trsv_fwd(const double* L, const double* Linv, int n,
+                                                  int k0, int nb, double* w, double* x) {
+  __shared__ double wb[NB], xb[NB];
+  const int tid = threadIdx.x;
+  if (tid < nb) wb[tid] = w[k0 + tid];
+  __syncthreads();
+  const double* X = Linv + (size_t)(k0 / NB) * NB * NB;
+  if (tid < nb) {
+    double s = 0.0;
+    for (int c = 0; c <= tid; c++) s += X[tid * NB + c] * wb[c];
+    xb[tid] = s;
+    if (blockIdx.x == 0) x[k0 + tid] = s;
+  }
+  __syncthreads();
+  const long i = (long)k0 + nb + blockIdx.x * (long)blockDim.x + tid;
+  if (i < n) {
+    const double* row = L + i * n + k0;
+    double s = 0.0;
+    for (int c = 0; c < nb; c++) s += row[c] * xb[c];
+    w[i] -= s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_trsv_bwd(const double* L, const double* Linv, int n,
+                                                  int k0, int nb, double* z, double* x) {
+  __shared__ double zb[NB], xb[NB];
+  const int tid = threadIdx.x;
+  if (tid < nb) zb[tid] = z[k0 + tid];
+  __syncthreads();
+  const double* X = Linv + (size_t)(k0 / NB) * NB * NB;
+  if (tid < nb) {
+    double s = 0.0;
+    for (int i = tid; i < nb; i++) s += X[i * NB + tid] * zb[i];
+    xb[tid] = s;
+    if (blockIdx.x == 0) x[k0 + tid] = s;
+  }
+  __syncthreads();
+  const long j = blockIdx.x * (long)blockDim.x + tid;
+  if (j < k0) {
+    double s = 0.0;
+    for (int r = 0; r < nb; r++) s += L[(long)(k0 + r) * n + j] * xb[r];
+    z[j] -= s;
+  }
+}
+
+int launch_potrs(const double* L, const double* Linv, int n, const double* b, double* x,
+                 double* w, hipStream_t st) {
+  NQS_HIP_CHECK(hipMemcpyAsync(w, b, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st));
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nb = (n - k0) < NB ? (n - k0) : NB;
-    hipLaunchKernelGGL(k_trsv_fwd_block, dim3(1), dim3(64), 0, st, L, n, k0, nb, x);
     const int rows = n - k0 - nb;
-    if (rows > 0)
-      hipLaunchKernelGGL(k_trsv_fwd_update, dim3((rows + 255) / 256), dim3(256), 0, st, L, n, k0,
-                         nb, x);
-    count_launch(2);
+    const int grid = rows > 0 ? (rows + 255) / 256 : 1;
+    hipLaunchKernelGGL(k_trsv_fwd, dim3(grid), dim3(256), 0, st, L, Linv, n, k0, nb, w, x);
+    count_launch();
   }
+  // x now holds z = L^-1 b; back substitution in place on w = z
+  NQS_HIP_CHECK(hipMemcpyAsync(w, x, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st));
   const int last = ((n - 1) / NB) * NB;
   for (int k0 = last; k0 >= 0; k0 -= NB) {
     const int nb = (n - k0) < NB ? (n - k0) : NB;
-    hipLaunchKernelGGL(k_trsv_bwd_block, dim3(1), dim3(64), 0, st, L, n, k0, nb, x);
-    if (k0 > 0)
-      hipLaunchKernelGGL(k_trsv_bwd_update, dim3((k0 + 255) / 256), dim3(256), 0, st, L, n, k0,
-                         nb, x);
-    count_launch(2);
+    const int grid = k0 > 0 ? (k0 + 255) / 256 : 1;
+    hipLaunchKernelGGL(k_trsv_bwd, dim3(grid), dim3(256), 0, st, L, Linv, n, k0, nb, w, x);
+    count_launch();
   }
   NQS_HIP_CHECK(hipGetLastError());
   return 0;

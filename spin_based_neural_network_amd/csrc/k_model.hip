@@ -173,6 +173,7 @@ int launch_forward(const ModelView<R>& m, const int* spins, int B, R* t_re, R* t
   if (B <= 0) return 0;
   if (m.N > kMaxSites) return -1;
   int grid = B < 65535 ? B : 65535;
+  KScope ks("forward", st);
   if (m.cplx)
     hipLaunchKernelGGL((k_forward<R, true>), dim3(grid), dim3(256), 0, st, m, spins, B,
                        t_re, t_im, la, ar);
@@ -250,6 +251,7 @@ int launch_gen_O(const ModelView<R>& m, int kind, const int* spins, int B,
   int gx = (int)((P + 255) / 256);
   if (gx > 1024) gx = 1024;
   int gy = B < 65535 ? B : 65535;
+  KScope ks("gen_O", st);
   hipLaunchKernelGGL(k_gen_O<R>, dim3(gx, gy), dim3(256), 0, st, m.N, m.M, m.cplx, mode,
                      spins, B, t_re, t_im, O_re, O_im, P);
   count_launch();

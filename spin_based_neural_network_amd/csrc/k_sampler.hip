@@ -222,6 +222,7 @@ int launch_mh(const ModelView<R>& m, ChainState& cs, const MHLaunch& ml, hipStre
     return -1;
   }
   const bool c = m.cplx != 0;
+  KScope ks("mh", st);
   if (m.M <= 64) {
     if (c) mh_go<R, true, 16, 4>(m, cs, ml, st); else mh_go<R, false, 16, 4>(m, cs, ml, st);
   } else if (m.M <= 256) {

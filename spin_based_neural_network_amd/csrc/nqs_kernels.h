@@ -126,16 +126,19 @@ int launch_center(double* O, long B, long P, const double* mean, hipStream_t st)
 int launch_gemm_nt(int M, int N, int K, double alpha, const double* A, long lda,
                    const double* B, long ldb, double beta, double* C, long ldc,
                    int lower, hipStream_t st);
-// In-place lower Cholesky of the n x n row-major matrix (lower triangle
-// read and written, upper untouched).  *fail_dev = 0 on success or 1 + the
-// first column whose pivot is not positive.  On failure the matrix is left
-// in the state the unblocked left-looking factorisation would leave:
-// columns before the failing one hold L, the others their input values
-// (restored from the untouched upper triangle and `diag_in`).
-int launch_potrf(double* A, int n, const double* diag_in, int* fail_dev, hipStream_t st);
-// x = (L L^T)^-1 b with L from launch_potrf; z is scratch [n].
-int launch_potrs(const double* L, int n, const double* b, double* x, double* z,
-                 hipStream_t st);
+// In-place lower Cholesky of the n x n row-major matrix (lower triangle read
+// and written, upper untouched).  Returns 0 on success or 1 + the first
+// column whose pivot is not positive, -1 on errors.  Linv receives the
+// inverses of the 64 x 64 diagonal blocks (potrf_linv_elems(n) doubles).
+size_t potrf_linv_elems(int n);
+int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st);
+// After a failure at column jf: bring A to the state the unblocked
+// left-looking factorisation leaves (columns < jf hold L, columns >= jf
+// their input values, taken from the lower triangle of orig).
+int launch_potrf_repair(double* A, int n, int jf, const double* orig, hipStream_t st);
+// x = (L L^T)^-1 b with L, Linv from launch_potrf; w is scratch [n].
+int launch_potrs(const double* L, const double* Linv, int n, const double* b, double* x,
+                 double* w, hipStream_t st);
 int launch_add_diag(double* A, int n, long ld, double v, hipStream_t st);
 int launch_copy_diag(const double* A, int n, double* d, hipStream_t st);
 int launch_mirror_lower(double* A, int n, hipStream_t st);
@@ -166,5 +169,15 @@ int launch_qgt_finish(double* out, const double* c1, const double* m1, const dou
 
 // ---- bookkeeping ------------------------------------------------------------
 void count_launch(int n = 1);
+
+// Live per-kernel timing with HIP events on the launch stream, enabled by
+// name through nqs_hip_kernel_timing(); a no-op otherwise.
+struct KScope {
+  int slot = -1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipStream_t st = nullptr;
+  KScope(const char* name, hipStream_t stream);
+  ~KScope();
+};
 
 }  // namespace nqs
