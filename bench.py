@@ -124,7 +124,7 @@ def step(lib, w, st):
     return info
 
 
-TAGS = ["mh", "forward", "eloc", "gen_O", "colsum", "gemm", "gemm_potrf", "potrf_diag", "trsm"]
+TAGS = ["mh", "forward", "eloc", "gen_O", "colsum", "gemm", "gemm_potrf", "potrf_diag", "trsm", "trsv", "gram_f"]
 
 
 def kernel_work(tag, w, st, launches):
@@ -298,13 +298,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--gram", default="direct", choices=["direct", "factored"])
+    ap.add_argument("--gram", default="factored", choices=["direct", "factored"])
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the HBM copy probe on MIB mebibytes (counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of "
               f"this path is a design, not a build", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        lib = abi.load_library()
+        nbytes = args.copy_probe << 20
+        ms = lib.nqs_hip_copy_probe(nbytes, 10)
+        print(json.dumps({"copy_probe_bytes": nbytes, "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}))
+        return
     w = WORKLOADS[args.workload]
     gpu = gpu_run(args, w)
     cpu = None

@@ -133,6 +133,11 @@ int nqs_hip_cholesky_solve(double *T, int n, const double *b, double *x);
 int nqs_hip_kernel_timing(const char *names);
 int nqs_hip_kernel_stats(const char *name, long long *count, double *total_ms);
 
+/* HBM probe: streaming copy of `bytes` bytes between two device buffers,
+ * `reps` times; average milliseconds per copy (the reference point for
+ * hardware byte counters), < 0 on error. */
+double nqs_hip_copy_probe(long bytes, int reps);
+
 /* Diagnostics of the device path: how many kernel launches the library has
  * made, and how many host-callback adaptor calls it has made (a callback
  * that is not nqs_ansatz_log_amp or nqs_marshall_log_amp over it runs on

@@ -164,6 +164,7 @@ HIP_API = {
     "nqs_hip_config_layout": (c_int, [_P, c_int]),
     "nqs_hip_gram": (c_int, [_P, c_int, c_long, c_double, _P]),
     "nqs_hip_kernel_timing": (c_int, [c_char_p]),
+    "nqs_hip_copy_probe": (c_double, [c_long, c_int]),
     "nqs_hip_kernel_stats": (c_int, [c_char_p, POINTER(c_longlong), POINTER(c_double)]),
     "nqs_hip_cholesky_solve": (c_int, [_P, c_int, _P, _P]),
 }

@@ -294,7 +294,7 @@ bool recognise(nqs_log_amp_fn_t fn, void* user, AmpSource& src) {
 // ---------------------------------------------------------------------------
 struct Workspace {
   DBuf spins, t_re, t_im, la, ar, e_re, e_im, O, Oi, colwork, u1, u2, vecs, T, diag, y, scal,
-      cg, fail, ham, sublat, hostla, hostar, linv, T2;
+      cg, fail, ham, sublat, hostla, hostar, linv, T2, Sd;
 };
 Workspace g_ws;
 
@@ -1253,9 +1253,22 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
     CKR(g_ws.y.ensure(sizeof(double) * B * 2));
     CKR(g_ws.fail.ensure(sizeof(int) * 4));
     double* T = g_ws.T.as<double>();
-    // T = O_c O_c^T / Ns + eps I (lower triangle, FP64 MFMA)
-    CKR(nqs::launch_gemm_nt(B, B, (int)P, 1.0 / B, O, P, O, P, 0.0, T, B, 1, g_stream));
-    CKR(nqs::launch_add_diag(T, B, B, eps, g_stream));
+    // T = O_c O_c^T / Ns + eps I (lower triangle, FP64 MFMA): directly over
+    // O_c, or from the factored form of RBM-type rows
+    const bool factored = s->opt.gram == NQS_HIP_GRAM_FACTORED && !gfn;
+    if (factored) {
+      if (prec == NQS_HIP_FP32) CKR(forward_t<double>(a, d_samples, B, nullptr, nullptr));
+      CKR(g_ws.T2.ensure(sizeof(double) * (size_t)B * B));
+      CKR(g_ws.Sd.ensure(sizeof(double) * (size_t)B * N));
+      const int cpl = a->kind == NQS_ANSATZ_COMPLEX_RBM;
+      CKR(nqs::launch_gram_factored(d_samples, B, N, g_ws.t_re.as<double>(),
+                                    cpl ? g_ws.t_im.as<double>() : nullptr, a->M, eps, T,
+                                    g_ws.T2.as<double>(), g_ws.Sd.as<double>(),
+                                    g_ws.u2.as<double>(), sc + 6, g_stream));
+    } else {
+      CKR(nqs::launch_gemm_nt(B, B, (int)P, 1.0 / B, O, P, O, P, 0.0, T, B, 1, g_stream));
+      CKR(nqs::launch_add_diag(T, B, B, eps, g_stream));
+    }
     CKR(g_ws.linv.ensure(sizeof(double) * nqs::potrf_linv_elems(B)));
     double* Linv = g_ws.linv.as<double>();
     int f = nqs::launch_potrf(T, B, Linv, g_ws.fail.as<int>(), g_stream);
@@ -1556,6 +1569,31 @@ NQS_EXPORT int nqs_hip_cholesky_solve(double* T, int n, const double* b, double*
   CK(hipMemcpyAsync(x, y, sizeof(double) * n, hipMemcpyDeviceToHost, g_stream));
   CK(hipStreamSynchronize(g_stream));
   return f == 0 ? 1 : 0;
+}
+
+// Streaming copy of `bytes` bytes between two device buffers, `reps` times;
+// returns the average time per copy in milliseconds (HIP events) or < 0.
+NQS_EXPORT double nqs_hip_copy_probe(long bytes, int reps) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (bytes <= 0 || reps <= 0 || stream_ready()) return -1.0;
+  const long n = (bytes / 32) * 4;
+  DBuf a, b;
+  if (a.ensure(sizeof(double) * n) || b.ensure(sizeof(double) * n)) return -1.0;
+  if (nqs::launch_fill(a.as<double>(), 1.0, n, g_stream)) return -1.0;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) || hipEventCreate(&e1)) return -1.0;
+  (void)nqs::launch_copy_probe(a.as<double>(), b.as<double>(), n, g_stream);
+  (void)hipEventRecord(e0, g_stream);
+  for (int r = 0; r < reps; r++) (void)nqs::launch_copy_probe(a.as<double>(), b.as<double>(), n, g_stream);
+  (void)hipEventRecord(e1, g_stream);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  a.release();
+  b.release();
+  return ms / reps;
 }
 
 NQS_EXPORT long long nqs_hip_device_launches(void) { return g_launches.load(); }

@@ -6,6 +6,10 @@
 // triangular solves (:1046-1061), MinSR linear algebra (:1101-1177).  Every
 // reduction here runs in a fixed order, so results are reproducible run to
 // run; they differ from the reference's sequential sums only by rounding.
+#include <stdlib.h>
+
+#include <vector>
+
 #include "nqs_kernels.h"
 
 namespace nqs {
@@ -299,7 +303,8 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int M, int N, int K, double alp
                                                  const double* A, long lda,
                                                  const double* __restrict__ Bm, long ldb,
                                                  double beta, double* C, long ldc,
-                                                 int lower, int tiles_n, const int* skip) {
+                                                 int lower, int mask, int tiles_n,
+                                                 const int* skip) {
   if (skip && *skip) return;
   __shared__ double As[GT][GKS + 1];
   __shared__ double Bs[GT][GKS + 1];
@@ -347,7 +352,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int M, int N, int K, double alp
       for (int r = 0; r < 4; r++) {
         const int gr = row0 + wr + 16 * i + lk + 4 * r;
         const int gc = col0 + wc + 16 * j + lr;
-        if (gr < M && gc < N && (!lower || gc <= gr)) {
+        if (gr < M && gc < N && (!mask || gc <= gr)) {
           double* c = C + (long)gr * ldc + gc;
           *c = (beta == 0.0) ? alpha * acc[i][j][r] : alpha * acc[i][j][r] + beta * (*c);
         }
@@ -356,14 +361,15 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int M, int N, int K, double alp
 
 static int gemm_nt_impl(int M, int N, int K, double alpha, const double* A, long lda,
                         const double* B, long ldb, double beta, double* C, long ldc, int lower,
-                        const int* skip, hipStream_t st) {
+                        const int* skip, hipStream_t st, int mask = -1) {
   if (M <= 0 || N <= 0) return 0;
   const int tm = (M + GT - 1) / GT, tn = (N + GT - 1) / GT;
   long tiles = lower ? (long)tm * (tm + 1) / 2 : (long)tm * tn;
   if (lower && M != N) return -1;
-  KScope ks(skip ? (lower ? "gemm_potrf" : "gemm_trsm") : "gemm", st);
+  if (mask < 0) mask = lower;
+  KScope ks(skip ? (lower ? "gemm_potrf" : (mask ? "gemm_potrf" : "gemm_trsm")) : "gemm", st);
   hipLaunchKernelGGL(k_gemm_nt, dim3((unsigned)tiles), dim3(256), 0, st, M, N, K, alpha, A, lda,
-                     B, ldb, beta, C, ldc, lower, tn, skip);
+                     B, ldb, beta, C, ldc, lower, mask, tn, skip);
   count_launch();
   NQS_HIP_CHECK(hipGetLastError());
   return 0;
@@ -385,69 +391,84 @@ constexpr int NB = 64;
 
 size_t potrf_linv_elems(int n) { return (size_t)((n + NB - 1) / NB) * NB * NB; }
 
-__global__ __launch_bounds__(256) void k_potrf_panel(double* A, int n, int k0, int nb,
-                                                     double* Linv, int* fail) {
+// value of a double held by lane `lane` (a compile-time constant after
+// unrolling), read into scalar registers: a wave-uniform broadcast with no
+// LDS traffic
+__device__ inline double lane_value(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+__global__ __launch_bounds__(64) void k_potrf_panel(double* A, int n, int k0, int nb,
+                                                    double* Linv, int* fail) {
+  // One wave.  The block is staged through LDS with coalesced row loads;
+  // lane i then keeps row i in registers.  Right-looking, one column per
+  // step: L[c][j] for the rank-1 update is read from lane c's registers
+  // with v_readlane (uniform), so the whole factorisation and inversion run
+  // out of registers.  The update is applied to every c > j without
+  // predication: entries c > i of a lane's row are never read back.
   if (*fail) return;
-  // lower triangle (incl. diagonal) holds L; the strict upper triangle holds
-  // the inverse transposed (X[i][c] at L[c][i], c < i); column NB holds the
-  // inverse's diagonal
-  __shared__ double L[NB][NB + 1];
-  __shared__ int bad;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < nb * nb; e += blockDim.x) {
-    const int i = e / nb, j = e % nb;
-    if (j <= i) L[i][j] = A[(long)(k0 + i) * n + k0 + j];
+  __shared__ double Ls[NB][NB + 1];
+  const int i = threadIdx.x;
+  {
+    // all 64 coalesced row loads in flight before the first wait: lane i
+    // holds column i, transposed through LDS
+    double v[NB];
+#pragma unroll
+    for (int t = 0; t < NB; t++)
+      v[t] = (t < nb && i < nb && i <= t) ? A[(long)(k0 + t) * n + k0 + i] : (t == i ? 1.0 : 0.0);
+#pragma unroll
+    for (int t = 0; t < NB; t++) Ls[t][i] = v[t];
   }
-  if (tid == 0) bad = -1;
   __syncthreads();
+  double r[NB];
+#pragma unroll
+  for (int c = 0; c < NB; c++) r[c] = Ls[i][c];
   int jdone = nb;
-  for (int j = 0; j < nb; j++) {
-    if (tid == 0) {
-      const double d = L[j][j];
-      if (!(d > 0.0)) bad = j;
-      else L[j][j] = sqrt(d);
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    if (j < nb && jdone == nb) {
+      const double d = lane_value(r[j], j);
+      if (!(d > 0.0)) {
+        jdone = j;
+      } else {
+        const double Ljj = sqrt(d);
+        const double inv = 1.0 / Ljj;
+        r[j] = (i == j) ? Ljj : r[j] * inv;
+        const double lij = r[j];
+#pragma unroll
+        for (int c = j + 1; c < NB; c++) r[c] -= lij * lane_value(r[j], c);
+      }
     }
-    __syncthreads();
-    if (bad >= 0) { jdone = bad; break; }
-    const double inv = 1.0 / L[j][j];
-    for (int i = j + 1 + tid; i < nb; i += blockDim.x) L[i][j] *= inv;
-    __syncthreads();
-    const int m = nb - 1 - j;
-    for (int e = tid; e < m * m; e += blockDim.x) {
-      const int i = j + 1 + e / m, c = j + 1 + e % m;
-      if (c <= i) L[i][c] -= L[i][j] * L[c][j];
-    }
-    __syncthreads();
   }
-  for (int e = tid; e < nb * nb; e += blockDim.x) {
-    const int i = e / nb, j = e % nb;
-    if (j <= i && j < jdone) A[(long)(k0 + i) * n + k0 + j] = L[i][j];
-  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NB; c++) Ls[i][c] = r[c];
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < NB; t++)
+    if (t < nb && i <= t && i < jdone) A[(long)(k0 + t) * n + k0 + i] = Ls[t][i];
   if (jdone < nb) {
-    if (tid == 0) *fail = k0 + jdone + 1;
+    if (i == 0) *fail = k0 + jdone + 1;
     return;
   }
-  // invert the diagonal block: thread c solves L x = e_c
-  if (tid < nb) {
-    const int c = tid;
-    L[c][NB] = 1.0 / L[c][c];
-    for (int i = c + 1; i < nb; i++) {
-      double s = 0.0;
-      for (int k = c; k < i; k++) s += L[i][k] * (k == c ? L[c][NB] : L[c][k]);
-      L[c][i] = -s / L[i][i];
-    }
+  // inverse of the diagonal block, lane c solving L x = e_c:
+  // x_q = (delta_qc - sum_{k<q} L_qk x_k) / L_qq, with L_qk from lane q
+  // (x_q = 0 for q < c follows).  Rows q >= nb hold the identity.
+  const int c = i;
+  double x[NB];
+#pragma unroll
+  for (int q = 0; q < NB; q++) {
+    const double dq = 1.0 / lane_value(r[q], q);
+    double sum = (q == c) ? -1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < q; k++) sum += lane_value(r[k], q) * x[k];
+    x[q] = -sum * dq;
   }
-  __syncthreads();
   double* X = Linv + (size_t)(k0 / NB) * NB * NB;
-  for (int e = tid; e < NB * NB; e += blockDim.x) {
-    const int i = e / NB, c = e % NB;
-    double v = 0.0;
-    if (i < nb && c < nb) {
-      if (c == i) v = L[i][NB];
-      else if (c < i) v = L[c][i];
-    }
-    X[e] = v;
-  }
+#pragma unroll
+  for (int q = 0; q < NB; q++) X[q * NB + c] = (q < nb && c < nb && q >= c) ? x[q] : 0.0;
 }
 
 // rows i >= r0: A[i][k0 + j] for j < ncols solved against the diagonal block
@@ -494,31 +515,71 @@ __global__ void k_restore_cols(double* A, const double* orig, int n, int jf) {
 
 __global__ void k_zero_int(int* p) { *p = 0; }
 
+static hipStream_t g_look = nullptr;
+static std::vector<hipEvent_t> g_ev;
+
+static int panel_and_trsm(double* A, int n, int k0, double* Linv, int* fail_dev, hipStream_t st) {
+  const int nb = (n - k0) < NB ? (n - k0) : NB;
+  {
+    KScope ks("potrf_diag", st);
+    hipLaunchKernelGGL(k_potrf_panel, dim3(1), dim3(64), 0, st, A, n, k0, nb, Linv, fail_dev);
+  }
+  count_launch();
+  const int r0 = k0 + nb;
+  if (r0 < n) {
+    double* panel = A + (long)r0 * n + k0;
+    // A21 <- A21 Linv^T (in place: each workgroup reads its rows before writing)
+    KScope ks("trsm", st);
+    if (gemm_nt_impl(n - r0, nb, nb, 1.0, panel, n, Linv + (size_t)(k0 / NB) * NB * NB, NB, 0.0,
+                     panel, n, 0, fail_dev, st))
+      return -1;
+  }
+  return 0;
+}
+
+// Right-looking blocked Cholesky with lookahead one: after panel k, the
+// next panel's column block is updated first and factored on a second
+// stream while the rest of the trailing matrix is updated on the main one.
 int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) {
+  if (!g_look) NQS_HIP_CHECK(hipStreamCreateWithFlags(&g_look, hipStreamNonBlocking));
+  const int np = (n + NB - 1) / NB;
+  while ((int)g_ev.size() < 2 * np) {
+    hipEvent_t e;
+    NQS_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    g_ev.push_back(e);
+  }
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, fail_dev);
   count_launch();
-  for (int k0 = 0; k0 < n; k0 += NB) {
+  if (panel_and_trsm(A, n, 0, Linv, fail_dev, st)) return -1;
+  // NQS_HIP_POTRF_LOOKAHEAD=0: the same schedule on one stream
+  static const int lookahead = [] {
+    const char* e = getenv("NQS_HIP_POTRF_LOOKAHEAD");
+    return (e && e[0] == '0') ? 0 : 1;
+  }();
+  hipStream_t side = lookahead ? g_look : st;
+  for (int k = 0; k < np; k++) {
+    const int k0 = k * NB;
     const int nb = (n - k0) < NB ? (n - k0) : NB;
-    {
-      KScope ks("potrf_diag", st);
-      hipLaunchKernelGGL(k_potrf_panel, dim3(1), dim3(256), 0, st, A, n, k0, nb, Linv, fail_dev);
-    }
-    count_launch();
     const int r0 = k0 + nb;
-    if (r0 < n) {
-      const int rows = n - r0;
-      double* panel = A + (long)r0 * n + k0;
-      // A21 <- A21 Linv^T (in place: each workgroup reads its rows before writing)
-      {
-        KScope ks("trsm", st);
-        if (gemm_nt_impl(rows, nb, nb, 1.0, panel, n, Linv + (size_t)(k0 / NB) * NB * NB, NB, 0.0,
-                         panel, n, 0, fail_dev, st))
-          return -1;
-      }
-      if (gemm_nt_impl(rows, rows, nb, -1.0, panel, n, panel, n, 1.0, A + (long)r0 * n + r0, n, 1,
-                       fail_dev, st))
+    if (r0 >= n) break;
+    const int nb1 = (n - r0) < NB ? (n - r0) : NB;
+    const double* L21 = A + (long)r0 * n + k0;
+    // next panel's columns [r0, r0 + nb1), rows >= r0 (lower part only)
+    if (gemm_nt_impl(n - r0, nb1, nb, -1.0, L21, n, L21, n, 1.0, A + (long)r0 * n + r0, n, 0,
+                     fail_dev, st, 1))
+      return -1;
+    NQS_HIP_CHECK(hipEventRecord(g_ev[2 * k], st));
+    NQS_HIP_CHECK(hipStreamWaitEvent(side, g_ev[2 * k], 0));
+    if (panel_and_trsm(A, n, r0, Linv, fail_dev, side)) return -1;
+    NQS_HIP_CHECK(hipEventRecord(g_ev[2 * k + 1], side));
+    // the rest of the trailing matrix: rows and columns >= r0 + nb1
+    const int r1 = r0 + nb1;
+    if (r1 < n) {
+      if (gemm_nt_impl(n - r1, n - r1, nb, -1.0, A + (long)r1 * n + k0, n, A + (long)r1 * n + k0, n,
+                       1.0, A + (long)r1 * n + r1, n, 1, fail_dev, st))
         return -1;
     }
+    NQS_HIP_CHECK(hipStreamWaitEvent(st, g_ev[2 * k + 1], 0));
   }
   NQS_HIP_CHECK(hipGetLastError());
   int fail = 0;
@@ -553,45 +614,71 @@ int launch_potrf_repair(double* A, int n, int jf, const double* orig, hipStream_
 // --------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_trsv_fwd(const double* L, const double* Linv, int n,
                                                   int k0, int nb, double* w, double* x) {
+  __shared__ double X[NB][NB + 1];
   __shared__ double wb[NB], xb[NB];
   const int tid = threadIdx.x;
-  if (tid < nb) wb[tid] = w[k0 + tid];
+  const double* Xg = Linv + (size_t)(k0 / NB) * NB * NB;
+  {
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = Xg[tid + 256 * q];
+#pragma unroll
+    for (int q = 0; q < 16; q++) X[(tid + 256 * q) / NB][(tid + 256 * q) % NB] = v[q];
+  }
+  if (tid < NB) wb[tid] = tid < nb ? w[k0 + tid] : 0.0;
   __syncthreads();
-  const double* X = Linv + (size_t)(k0 / NB) * NB * NB;
-  if (tid < nb) {
+  if (tid < NB) {
     double s = 0.0;
-    for (int c = 0; c <= tid; c++) s += X[tid * NB + c] * wb[c];
+#pragma unroll
+    for (int c = 0; c < NB; c++) s += X[tid][c] * wb[c];
     xb[tid] = s;
-    if (blockIdx.x == 0) x[k0 + tid] = s;
+    if (blockIdx.x == 0 && tid < nb) x[k0 + tid] = s;
   }
   __syncthreads();
   const long i = (long)k0 + nb + blockIdx.x * (long)blockDim.x + tid;
   if (i < n) {
     const double* row = L + i * n + k0;
+    double v[NB];
+#pragma unroll
+    for (int c = 0; c < NB; c++) v[c] = c < nb ? row[c] : 0.0;
     double s = 0.0;
-    for (int c = 0; c < nb; c++) s += row[c] * xb[c];
+#pragma unroll
+    for (int c = 0; c < NB; c++) s += v[c] * xb[c];
     w[i] -= s;
   }
 }
 
 __global__ __launch_bounds__(256) void k_trsv_bwd(const double* L, const double* Linv, int n,
                                                   int k0, int nb, double* z, double* x) {
+  __shared__ double X[NB][NB + 1];
   __shared__ double zb[NB], xb[NB];
   const int tid = threadIdx.x;
-  if (tid < nb) zb[tid] = z[k0 + tid];
+  const double* Xg = Linv + (size_t)(k0 / NB) * NB * NB;
+  {
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = Xg[tid + 256 * q];
+#pragma unroll
+    for (int q = 0; q < 16; q++) X[(tid + 256 * q) / NB][(tid + 256 * q) % NB] = v[q];
+  }
+  if (tid < NB) zb[tid] = tid < nb ? z[k0 + tid] : 0.0;
   __syncthreads();
-  const double* X = Linv + (size_t)(k0 / NB) * NB * NB;
-  if (tid < nb) {
+  if (tid < NB) {
     double s = 0.0;
-    for (int i = tid; i < nb; i++) s += X[i * NB + tid] * zb[i];
+#pragma unroll
+    for (int r = 0; r < NB; r++) s += X[r][tid] * zb[r];
     xb[tid] = s;
-    if (blockIdx.x == 0) x[k0 + tid] = s;
+    if (blockIdx.x == 0 && tid < nb) x[k0 + tid] = s;
   }
   __syncthreads();
   const long j = blockIdx.x * (long)blockDim.x + tid;
   if (j < k0) {
+    double v[NB];
+#pragma unroll
+    for (int r = 0; r < NB; r++) v[r] = r < nb ? L[(long)(k0 + r) * n + j] : 0.0;
     double s = 0.0;
-    for (int r = 0; r < nb; r++) s += L[(long)(k0 + r) * n + j] * xb[r];
+#pragma unroll
+    for (int r = 0; r < NB; r++) s += v[r] * xb[r];
     z[j] -= s;
   }
 }
@@ -603,6 +690,7 @@ int launch_potrs(const double* L, const double* Linv, int n, const double* b, do
     const int nb = (n - k0) < NB ? (n - k0) : NB;
     const int rows = n - k0 - nb;
     const int grid = rows > 0 ? (rows + 255) / 256 : 1;
+    KScope ks("trsv", st);
     hipLaunchKernelGGL(k_trsv_fwd, dim3(grid), dim3(256), 0, st, L, Linv, n, k0, nb, w, x);
     count_launch();
   }
@@ -612,9 +700,96 @@ int launch_potrs(const double* L, const double* Linv, int n, const double* b, do
   for (int k0 = last; k0 >= 0; k0 -= NB) {
     const int nb = (n - k0) < NB ? (n - k0) : NB;
     const int grid = k0 > 0 ? (k0 + 255) / 256 : 1;
+    KScope ks("trsv", st);
     hipLaunchKernelGGL(k_trsv_bwd, dim3(grid), dim3(256), 0, st, L, Linv, n, k0, nb, w, x);
     count_launch();
   }
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// factored MinSR Gram for RBM-type O rows (s, t, t (x) s): with
+// SS = S S^T and UU = U U^T (U = t, or [Re t, Im t] for the real-projected
+// complex RBM), O_i . O_j = (1 + SS_ij)(1 + UU_ij) - 1 =: G_ij, and
+// T = C G C / Ns + eps I with C = I - 1/Ns: T_ij = (G_ij - r_i - r_j + g)/Ns
+// where r_i = mean_j G_ij and g = mean_i r_i.  Lower triangles only.
+// --------------------------------------------------------------------------
+__global__ void k_spins_to_f64(const int* __restrict__ s, double* __restrict__ out, long n) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    out[e] = (double)s[e];
+}
+__global__ void k_gram_combine(double* T, const double* __restrict__ UU, int B) {
+  const long total = (long)B * B;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e / B), j = (int)(e % B);
+    if (j <= i) T[e] = (1.0 + T[e]) * (1.0 + UU[e]) - 1.0;
+  }
+}
+// r_i = (sum_{j<=i} G_ij + sum_{j>i} G_ji) / B, thread per row
+__global__ void k_sym_rowmean(const double* __restrict__ G, int B, double* __restrict__ r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B) return;
+  double s = 0.0;
+  const double* row = G + (long)i * B;
+  for (int j = 0; j <= i; j++) s += row[j];
+  for (int j = i + 1; j < B; j++) s += G[(long)j * B + i];
+  r[i] = s / B;
+}
+__global__ void k_gram_center(double* T, const double* __restrict__ r, const double* g, int B,
+                              double eps) {
+  const long total = (long)B * B;
+  const double gg = *g, inv = 1.0 / B;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e / B), j = (int)(e % B);
+    if (j <= i) T[e] = (T[e] - r[i] - r[j] + gg) * inv + (i == j ? eps : 0.0);
+  }
+}
+
+int launch_gram_factored(const int* spins, int B, int N, const double* U1, const double* U2, int Ku,
+                         double eps, double* T, double* UU, double* Sd, double* r, double* g,
+                         hipStream_t st) {
+  const long ns = (long)B * N;
+  long blocks = (ns + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_spins_to_f64, dim3((unsigned)blocks), dim3(256), 0, st, spins, Sd, ns);
+  count_launch();
+  {
+    KScope ks("gram_f", st);
+    if (gemm_nt_impl(B, B, N, 1.0, Sd, N, Sd, N, 0.0, T, B, 1, nullptr, st)) return -1;
+    if (Ku > 0) {
+      if (gemm_nt_impl(B, B, Ku, 1.0, U1, Ku, U1, Ku, 0.0, UU, B, 1, nullptr, st)) return -1;
+      if (U2 && gemm_nt_impl(B, B, Ku, 1.0, U2, Ku, U2, Ku, 1.0, UU, B, 1, nullptr, st)) return -1;
+    }
+  }
+  long tb = ((long)B * B + 255) / 256;
+  if (tb > 65536) tb = 65536;
+  if (Ku > 0) {
+    hipLaunchKernelGGL(k_gram_combine, dim3((unsigned)tb), dim3(256), 0, st, T, UU, B);
+    count_launch();
+  }
+  hipLaunchKernelGGL(k_sym_rowmean, dim3((B + 255) / 256), dim3(256), 0, st, T, B, r);
+  count_launch();
+  if (launch_sum(r, B, nullptr, 1.0 / B, g, st)) return -1;
+  hipLaunchKernelGGL(k_gram_center, dim3((unsigned)tb), dim3(256), 0, st, T, r, g, B, eps);
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// plain streaming copy (HBM probe used to calibrate byte counters)
+__global__ __launch_bounds__(256) void k_copy_probe(const double4* __restrict__ src,
+                                                    double4* __restrict__ dst, long n4) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+int launch_copy_probe(const double* src, double* dst, long n, hipStream_t st) {
+  const long n4 = n / 4;
+  KScope ks("copy_probe", st);
+  hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, st, (const double4*)src, (double4*)dst, n4);
+  count_launch();
   NQS_HIP_CHECK(hipGetLastError());
   return 0;
 }

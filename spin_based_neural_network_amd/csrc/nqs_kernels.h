@@ -139,9 +139,19 @@ int launch_potrf_repair(double* A, int n, int jf, const double* orig, hipStream_
 // x = (L L^T)^-1 b with L, Linv from launch_potrf; w is scratch [n].
 int launch_potrs(const double* L, const double* Linv, int n, const double* b, double* x,
                  double* w, hipStream_t st);
+// Factored MinSR Gram T = (O_c O_c^T)/B + eps I (lower triangle) for O rows
+// (s, u, u (x) s); u . u' = U1 U1^T (+ U2 U2^T when U2 is given), Ku columns
+// each (Ku = 0 for mean-field).  UU: B*B scratch, Sd: B*N scratch, r: B
+// scratch, g: device scalar.
+int launch_gram_factored(const int* spins, int B, int N, const double* U1, const double* U2, int Ku,
+                         double eps, double* T, double* UU, double* Sd, double* r, double* g,
+                         hipStream_t st);
 int launch_add_diag(double* A, int n, long ld, double v, hipStream_t st);
 int launch_copy_diag(const double* A, int n, double* d, hipStream_t st);
 int launch_mirror_lower(double* A, int n, hipStream_t st);
+
+// streaming copy of n doubles (n a multiple of 4)
+int launch_copy_probe(const double* src, double* dst, long n, hipStream_t st);
 
 // elementwise helpers
 int launch_axpy(double* y, const double* x, double a, long n, hipStream_t st);

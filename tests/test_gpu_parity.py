@@ -378,3 +378,38 @@ def test_large_shapes_consistent_with_oracle(hip, orc):
         _close(ehi, eoi, 1e-10)
         hip.nqs_ansatz_free(ah)
         orc.orc_ansatz_free(ao)
+
+
+@pytest.mark.parametrize("kind,ham,L,M,marsh", [(6, 1, (4, 4), 16, True), (5, 0, (8, 1), 16, False),
+                                                 (0, 0, (3, 3), 0, False), (6, 6, (2, 2), 24, False)])
+def test_factored_gram_equals_direct(hip, kind, ham, L, M, marsh):
+    """MinSR with the Gram from the factored RBM form (S S^T, t t^T) gives the
+    same update as the Gram over the materialised O_c."""
+    Lx, Ly = L
+    N = 3 * Lx * Ly if ham == 6 else Lx * Ly
+    cfg = config_defaults(ansatz=kind, hamiltonian=ham, rbm_hidden_units=M, rbm_init_scale=0.3,
+                          num_samples=128, num_thermalize=32, num_decorrelate=2, sr_diag_shift=1e-3,
+                          learning_rate=0.05)
+    res = {}
+    for gram in (0, 1):
+        opt = options_defaults(num_chains=16, move=1 if ham in (1, 6) else 0, sz_zero_start=1 if ham in (1, 6) else 0,
+                               periodic=1, gram=gram)
+        r = scenarios.Runner(hip, "nqs_")
+        a = r.ansatz(cfg, N)
+        fn, user = r.callback(a, (Lx, Ly) if marsh else None)
+        s = hip.nqs_hip_sampler_create_ex(N, ctypes.byref(cfg), ctypes.byref(opt), fn, user)
+        hip.nqs_sampler_thermalize(s)
+        ps, es = [r.params(a)], []
+        for _ in range(2):
+            info = SrInfo()
+            assert hip.nqs_sr_step_minsr_full(ctypes.byref(cfg), Lx, Ly, a, s, fn, user, None, None,
+                                              ctypes.byref(info)) == 0
+            assert info.converged == 1
+            ps.append(r.params(a))
+            es.append(info.mean_energy)
+        res[gram] = (np.array(ps), np.array(es))
+        hip.nqs_sampler_free(s)
+        hip.nqs_ansatz_free(a)
+    step = np.abs(np.diff(res[0][0], axis=0)).max()
+    assert np.abs(res[0][0] - res[1][0]).max() <= 1e-9 * step + 1e-13
+    _close(res[0][1], res[1][1], 1e-10)
