@@ -124,6 +124,25 @@ def step(lib, w, st):
     return info
 
 
+# kernel symbol (as rocprofv3 names it) behind each timing tag
+SYMBOL = {"gemm_potrf": "nqs::k_gemm_nt<1>", "gemm": "nqs::k_gemm_nt<0>", "trsm": "nqs::k_gemm_nt<2>",
+          "potrf_diag": "nqs::k_potrf_panel", "eloc": "nqs::k_eloc<double, true, 1>",
+          "gen_O": "nqs::k_gen_O<double>", "colsum": "nqs::k_colsum_partial",
+          "mh": "nqs::k_mh<double, true, 64, 4>", "trsv": "nqs::k_trsv_fwd"}
+
+
+def traffic_bytes(workload, tag):
+    """HBM bytes per dispatch of the kernel behind `tag`, from the committed
+    rocprofv3 --pmc summary of this workload (counter values scaled by the
+    copy-probe calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    d = json.load(open(path)).get("bytes_per_dispatch", {})
+    v = d.get(SYMBOL.get(tag, ""))
+    return None if v is None else v["fetch"] + v["write"]
+
+
 TAGS = ["mh", "forward", "eloc", "gen_O", "colsum", "gemm", "gemm_potrf", "potrf_diag", "trsm", "trsv", "gram_f"]
 
 
@@ -202,8 +221,9 @@ def gpu_run(args, w):
         ess_per_s=ess / elapsed, ms_per_step=1e3 * elapsed / args.steps, tau=tau,
         energies=energies, acceptance=infos[-1]["acceptance_ratio"],
         stage_ms=np.mean(stages, axis=0).tolist() if stages else None,
-        roofline=dict(kernel=dom, bound=bound, achieved=achieved, peak=peak, unit=runit,
-                      frac=achieved / peak, traffic=None, launches=c.value, avg_ms=avg_ms,
+        roofline=dict(kernel=dom, symbol=SYMBOL.get(dom), bound=bound, achieved=achieved, peak=peak,
+                      unit=runit, frac=achieved / peak, traffic=traffic_bytes(args.workload, dom),
+                      launches=c.value, avg_ms=avg_ms,
                       work_per_launch=work, work_unit=unit),
         split={t: dict(launches=cc, ms=mm) for t, (cc, mm) in per.items()}, N=st["N"], P=st["P"])
     lib.nqs_sampler_free(st["s"])
@@ -343,7 +363,7 @@ def main():
         if gpu["stage_ms"] else None,
         "kernel_split_ms": {k: v["ms"] for k, v in gpu["split"].items()},
         "roofline": {k: gpu["roofline"][k] for k in ("bound", "achieved", "peak", "unit", "frac",
-                                                     "traffic", "kernel", "avg_ms")},
+                                                     "traffic", "kernel", "symbol", "avg_ms")},
         "cpu_baseline": cpu,
     }
     if w["ed"] is not None:

@@ -299,6 +299,7 @@ __device__ inline void lower_tile(int t, int& tm, int& tn) {
   tn = t - m * (m + 1) / 2;
 }
 
+template <int USE>   // 0: Gram / general, 1: Cholesky trailing update, 2: panel solve
 __global__ __launch_bounds__(256) void k_gemm_nt(int M, int N, int K, double alpha,
                                                  const double* A, long lda,
                                                  const double* __restrict__ Bm, long ldb,
@@ -367,9 +368,17 @@ static int gemm_nt_impl(int M, int N, int K, double alpha, const double* A, long
   long tiles = lower ? (long)tm * (tm + 1) / 2 : (long)tm * tn;
   if (lower && M != N) return -1;
   if (mask < 0) mask = lower;
-  KScope ks(skip ? (lower ? "gemm_potrf" : (mask ? "gemm_potrf" : "gemm_trsm")) : "gemm", st);
-  hipLaunchKernelGGL(k_gemm_nt, dim3((unsigned)tiles), dim3(256), 0, st, M, N, K, alpha, A, lda,
-                     B, ldb, beta, C, ldc, lower, mask, tn, skip);
+  const int use = !skip ? 0 : (mask ? 1 : 2);
+  KScope ks(use == 0 ? "gemm" : (use == 1 ? "gemm_potrf" : "gemm_trsm"), st);
+  if (use == 0)
+    hipLaunchKernelGGL(k_gemm_nt<0>, dim3((unsigned)tiles), dim3(256), 0, st, M, N, K, alpha, A,
+                       lda, B, ldb, beta, C, ldc, lower, mask, tn, skip);
+  else if (use == 1)
+    hipLaunchKernelGGL(k_gemm_nt<1>, dim3((unsigned)tiles), dim3(256), 0, st, M, N, K, alpha, A,
+                       lda, B, ldb, beta, C, ldc, lower, mask, tn, skip);
+  else
+    hipLaunchKernelGGL(k_gemm_nt<2>, dim3((unsigned)tiles), dim3(256), 0, st, M, N, K, alpha, A,
+                       lda, B, ldb, beta, C, ldc, lower, mask, tn, skip);
   count_launch();
   NQS_HIP_CHECK(hipGetLastError());
   return 0;
@@ -727,16 +736,41 @@ __global__ void k_gram_combine(double* T, const double* __restrict__ UU, int B) 
     if (j <= i) T[e] = (1.0 + T[e]) * (1.0 + UU[e]) - 1.0;
   }
 }
-// r_i = (sum_{j<=i} G_ij + sum_{j>i} G_ji) / B, thread per row
-__global__ void k_sym_rowmean(const double* __restrict__ G, int B, double* __restrict__ r) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// r_i = (sum_{j<=i} G_ij + sum_{j>i} G_ji) / B from the lower triangle, in
+// fixed order: row part (one wave per row, coalesced), column part (two
+// stages over 128-row chunks, coalesced across columns)
+__global__ __launch_bounds__(256) void k_sym_rowpart(const double* __restrict__ G, int B,
+                                                     double* __restrict__ part) {
+  const long i = blockIdx.x * 4L + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
   if (i >= B) return;
+  const double* row = G + i * B;
   double s = 0.0;
-  const double* row = G + (long)i * B;
-  for (int j = 0; j <= i; j++) s += row[j];
-  for (int j = i + 1; j < B; j++) s += G[(long)j * B + i];
-  r[i] = s / B;
+  for (long j = lane; j <= i; j += 64) s += row[j];
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+  if (lane == 0) part[i] = s;
 }
+__global__ __launch_bounds__(256) void k_sym_colpart(const double* __restrict__ G, int B,
+                                                     double* __restrict__ work) {
+  const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long c = blockIdx.y;
+  if (j >= B) return;
+  const long i0 = c * 128, i1 = (i0 + 128 < B) ? i0 + 128 : B;
+  double s = 0.0;
+  for (long i = (i0 > j + 1 ? i0 : j + 1); i < i1; i++) s += G[i * B + j];
+  work[c * B + j] = s;
+}
+__global__ __launch_bounds__(256) void k_sym_rowmean(const double* __restrict__ part,
+                                                     const double* __restrict__ work, long nch,
+                                                     int B, double* __restrict__ r) {
+  const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (j >= B) return;
+  double s = part[j];
+  for (long c = 0; c < nch; c++) s += work[c * B + j];
+  r[j] = s / B;
+}
+
 __global__ void k_gram_center(double* T, const double* __restrict__ r, const double* g, int B,
                               double eps) {
   const long total = (long)B * B;
@@ -770,8 +804,16 @@ int launch_gram_factored(const int* spins, int B, int N, const double* U1, const
     hipLaunchKernelGGL(k_gram_combine, dim3((unsigned)tb), dim3(256), 0, st, T, UU, B);
     count_launch();
   }
-  hipLaunchKernelGGL(k_sym_rowmean, dim3((B + 255) / 256), dim3(256), 0, st, T, B, r);
-  count_launch();
+  {
+    const long nch = (B + 127) / 128;
+    double* work = UU;   // UU has been consumed by k_gram_combine
+    double* part = Sd;   // B doubles of the B*N scratch
+    hipLaunchKernelGGL(k_sym_rowpart, dim3((B + 3) / 4), dim3(256), 0, st, T, B, part);
+    hipLaunchKernelGGL(k_sym_colpart, dim3((B + 255) / 256, (unsigned)nch), dim3(256), 0, st, T, B,
+                       work);
+    hipLaunchKernelGGL(k_sym_rowmean, dim3((B + 255) / 256), dim3(256), 0, st, part, work, nch, B, r);
+    count_launch(3);
+  }
   if (launch_sum(r, B, nullptr, 1.0 / B, g, st)) return -1;
   hipLaunchKernelGGL(k_gram_center, dim3((unsigned)tb), dim3(256), 0, st, T, r, g, B, eps);
   count_launch();
