@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from spin_based_neural_network_amd import abi  # noqa: E402
+from spin_based_neural_network_amd.stats import pooled_tau  # noqa: E402
 
 METRIC = "effective samples/s & SR-iter wall time, learned vs plain MH, same ΔE vs ED"
 PEAK_FP64 = 78.6e12          # MI355X dense FP64 (vector / MFMA), FLOP/s
@@ -64,25 +65,6 @@ WORKLOADS = {
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def pooled_tau(series):
-    """tau_int from the chain-averaged autocorrelation (Sokal window c=5).
-
-    series: [C][n] per-chain E_loc series."""
-    x = np.asarray(series, dtype=float)
-    x = x - x.mean(axis=1, keepdims=True)
-    C, n = x.shape
-    var = np.mean(x * x)
-    if var <= 0 or n < 4:
-        return 0.5
-    tau = 0.5
-    for t in range(1, n):
-        rho = np.mean(x[:, :-t] * x[:, t:]) * n / (n - t) / var if t < n else 0.0
-        tau += rho
-        if t >= 5.0 * tau:
-            break
-    return max(tau, 0.5)
 
 
 def setup(lib, w, args):

@@ -58,17 +58,33 @@ typedef struct {
     int precision;       /* nqs_hip_precision_t                                */
     int gram;            /* nqs_hip_gram_t (MinSR)                             */
     /* learned proposal (move == NQS_HIP_MOVE_LEARNED) */
-    int ar_hidden;       /* hidden width of the masked MLP (default 64)        */
+    int ar_hidden;       /* hidden width of the masked MLP (64; 0 = 64)        */
     int ar_train_every;  /* retrain every k batches (default 10)               */
     int ar_train_window; /* samples kept for training (default 16384)          */
     int ar_epochs;       /* passes over the window per retrain (default 4)     */
     int ar_batch;        /* minibatch size (default 256)                       */
     double ar_lr;        /* Adam learning rate (default 1e-3)                  */
-    double ar_mix;       /* probability of a learned move vs exchange (0.5)    */
+    int ar_local;        /* exchange moves before each learned move (0)        */
+    unsigned ar_seed;    /* network initialisation seed (0: chain 0's seed)    */
 } nqs_hip_options_t;
 
 /* Defaults: one chain, flips, random start, open boundaries, FP64, direct
- * Gram, learned-proposal defaults as listed above. */
+ * Gram, learned-proposal defaults as listed above.
+ *
+ * Learned proposal (move = NQS_HIP_MOVE_LEARNED, device amplitudes only,
+ * N <= 64).  A masked autoregressive network q (N -> 64 -> 64 -> N, ReLU,
+ * BF16 weights and activations, FP32 accumulation) proposes whole
+ * configurations in the chain's own magnetisation sector; each proposal is
+ * accepted with min(1, |psi(s')|^2 q(s) / (|psi(s)|^2 q(s'))), so the chains
+ * sample |psi|^2 exactly whatever the network has learned.  Thermalisation
+ * and nqs_sampler_next use exchange moves.  Every recorded batch
+ * (nqs_sampler_batch, the sampling stage of an SR step) appends its samples
+ * to a window of ar_train_window configurations; before a batch the network
+ * is trained by maximum likelihood on the window (ar_epochs passes, Adam,
+ * minibatch ar_batch, rate ar_lr): first as soon as the window holds
+ * samples, then every ar_train_every batches.  A batch before the first
+ * training uses exchange moves (num_decorrelate per round); afterwards each
+ * round is ar_local exchange moves followed by one learned move. */
 nqs_hip_options_t nqs_hip_options_defaults(void);
 
 /* nqs_sampler_create with options.  NULL on bad args or unsupported
@@ -105,6 +121,36 @@ int nqs_hip_log_amp_batch(nqs_ansatz_t *a, const int *spins_batch, int batch_siz
 int nqs_hip_logpsi_gradient_batch(nqs_ansatz_t *a, const int *spins_batch,
                                   int batch_size, int mode,
                                   double *out_re, double *out_im);
+
+/* Learned-proposal network on its own (the sampler's, or a standalone one
+ * for tests and offline training).  Parameters are FP32, layout
+ * [W1 (64 x N) | b1 (64) | W2 (64 x 64) | b2 (64) | W3 (N x 64) | b3 (N)],
+ * with MADE masks (hidden degree 1 + k mod (N-1)); masked entries stay 0.
+ * `constrained` restricts a configuration to the magnetisation of its
+ * reference (sampling: ref_spins; evaluation and training: itself). */
+typedef struct nqs_hip_ar nqs_hip_ar_t;
+nqs_hip_ar_t *nqs_hip_ar_create(int num_sites, unsigned long long seed, double init_scale);
+void nqs_hip_ar_free(nqs_hip_ar_t *ar);
+long nqs_hip_ar_num_params(const nqs_hip_ar_t *ar);
+int nqs_hip_ar_get_params(nqs_hip_ar_t *ar, float *out);
+int nqs_hip_ar_set_params(nqs_hip_ar_t *ar, const float *in);
+/* log q of B configurations [B][N] (the sampling walk in evaluation mode) */
+int nqs_hip_ar_log_prob(nqs_hip_ar_t *ar, const int *spins, int batch, int constrained,
+                        double *out_logq);
+/* B draws; row b uses xorshift64 state seeds[b] (nonzero) */
+int nqs_hip_ar_sample(nqs_hip_ar_t *ar, const int *ref_spins, int batch, int constrained,
+                      const unsigned long long *seeds, int *out_spins, double *out_logq);
+/* gradient of the mean negative log-likelihood over B configurations */
+int nqs_hip_ar_gradient(nqs_hip_ar_t *ar, const int *spins, int batch, int constrained,
+                        float *grad, double *mean_nll);
+/* Adam training on n configurations; mean NLL of the last epoch */
+int nqs_hip_ar_train(nqs_hip_ar_t *ar, const int *spins, int n, int constrained, int epochs,
+                     int minibatch, double lr, double *mean_nll);
+/* the network of a learned-proposal sampler (NULL otherwise) */
+nqs_hip_ar_t *nqs_hip_sampler_ar(nqs_sampler_t *s);
+/* learned moves: acceptance ratio, proposals made, trainings, last mean NLL */
+int nqs_hip_sampler_learned_stats(const nqs_sampler_t *s, double *acceptance,
+                                  long long *proposals, int *trainings, double *last_nll);
 
 /* Integrated autocorrelation time of a series (Sokal window, c = 5). */
 double nqs_hip_tau_int(const double *x, int n);

@@ -1,7 +1,8 @@
 """CPU oracle bindings -- TEST INFRASTRUCTURE ONLY.
 
 Loads ``oracle/liboracle.so`` (the plain-C FP64 restatement of the reference
-algorithm, oracle/nqs_oracle.c) and, where it was built,
+algorithm, oracle/nqs_oracle.c, and of the learned-proposal network,
+oracle/ar_oracle.c) and, where it was built,
 ``oracle/_ref/libnqs_ref.so`` (the reference's own sources compiled by
 oracle/Makefile).  Only tests/, ``__graft_entry__.smoke()`` and the
 cpu_baseline leg of bench.py import this package; the product never does.
@@ -32,6 +33,11 @@ _EXTRA = {
     "orc_ed_ground_energy": (c_double, [POINTER(NqsConfig), c_int, c_int, c_int, c_int, c_int, c_double]),
     "orc_exact_energy": (c_double, [POINTER(NqsConfig), c_int, c_int, c_int, c_void_p, c_void_p]),
     "orc_tau_int": (c_double, [c_void_p, c_int]),
+    "orc_ar_num_params": (ctypes.c_long, [c_int]),
+    "orc_ar_init": (None, [c_int, c_ulonglong, c_double, c_void_p]),
+    "orc_ar_log_prob": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "orc_ar_gradient": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "orc_ar_adam_step": (None, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_long, c_double]),
 }
 
 _orc = None

@@ -184,6 +184,16 @@ double orc_exact_energy(const nqs_config_t *cfg, int size_x, int size_y,
 /* ---- statistics ----------------------------------------------------- */
 double orc_tau_int(const double *x, int n);
 
+/* ---- learned proposal network (ar_oracle.c) ----------------------------- */
+long orc_ar_num_params(int N);
+void orc_ar_init(int N, unsigned long long seed, double scale, float *w);
+int  orc_ar_log_prob(int N, const float *w, const int *spins, int B, int constrained,
+                     int quantize, double *out);
+int  orc_ar_gradient(int N, const float *w, const int *spins, int B, int constrained,
+                     int quantize, double *grad, double *mean_nll);
+void orc_ar_adam_step(int N, float *w, double *m, double *v, const double *grad, long t,
+                      double lr);
+
 #ifdef __cplusplus
 }
 #endif

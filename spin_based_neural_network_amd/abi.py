@@ -15,7 +15,7 @@ Loading this module loads nothing; ``load_library()`` opens the in-tree
 import ctypes
 import os
 from ctypes import (POINTER, Structure, c_double, c_int, c_long, c_longlong,
-                    c_char_p, c_uint, c_void_p)
+                    c_char_p, c_uint, c_ulonglong, c_void_p)
 
 import numpy as np
 
@@ -87,7 +87,8 @@ class HipOptions(Structure):
                 ("precision", c_int), ("gram", c_int),
                 ("ar_hidden", c_int), ("ar_train_every", c_int),
                 ("ar_train_window", c_int), ("ar_epochs", c_int),
-                ("ar_batch", c_int), ("ar_lr", c_double), ("ar_mix", c_double)]
+                ("ar_batch", c_int), ("ar_lr", c_double), ("ar_local", c_int),
+                ("ar_seed", c_uint)]
 
 
 def options_defaults(**overrides):
@@ -95,7 +96,7 @@ def options_defaults(**overrides):
     o = HipOptions(num_chains=1, move=MOVE_FLIP, sz_zero_start=0, periodic=0,
                    precision=FP64, gram=GRAM_DIRECT, ar_hidden=64,
                    ar_train_every=10, ar_train_window=16384, ar_epochs=4,
-                   ar_batch=256, ar_lr=1e-3, ar_mix=0.5)
+                   ar_batch=256, ar_lr=1e-3, ar_local=0, ar_seed=0)
     for k, v in overrides.items():
         if not hasattr(o, k):
             raise AttributeError(f"nqs_hip_options_t has no field {k!r}")
@@ -167,6 +168,17 @@ HIP_API = {
     "nqs_hip_copy_probe": (c_double, [c_long, c_int]),
     "nqs_hip_kernel_stats": (c_int, [c_char_p, POINTER(c_longlong), POINTER(c_double)]),
     "nqs_hip_cholesky_solve": (c_int, [_P, c_int, _P, _P]),
+    "nqs_hip_ar_create": (c_void_p, [c_int, c_ulonglong, c_double]),
+    "nqs_hip_ar_free": (None, [_P]),
+    "nqs_hip_ar_num_params": (c_long, [_P]),
+    "nqs_hip_ar_get_params": (c_int, [_P, _P]),
+    "nqs_hip_ar_set_params": (c_int, [_P, _P]),
+    "nqs_hip_ar_log_prob": (c_int, [_P, _P, c_int, c_int, _P]),
+    "nqs_hip_ar_sample": (c_int, [_P, _P, c_int, c_int, _P, _P, _P]),
+    "nqs_hip_ar_gradient": (c_int, [_P, _P, c_int, c_int, _P, _P]),
+    "nqs_hip_ar_train": (c_int, [_P, _P, c_int, c_int, c_int, c_int, c_double, _P]),
+    "nqs_hip_sampler_ar": (c_void_p, [_P]),
+    "nqs_hip_sampler_learned_stats": (c_int, [_P, _P, _P, _P, _P]),
 }
 
 

@@ -28,6 +28,7 @@
 #include "../../include/nqs/nqs_optimizer.h"
 #include "../../include/nqs/nqs_sampler.h"
 #include "../../include/nqs_hip/nqs_hip.h"
+#include "ar_layout.h"
 #include "lattice.h"
 #include "nqs_kernels.h"
 
@@ -647,6 +648,24 @@ NQS_EXPORT void nqs_marshall_log_amp(const int* spins, int num_sites, void* user
 // ===========================================================================
 // sampler
 // ===========================================================================
+// ---------------------------------------------------------------------------
+// learned proposal network (kernels k_ar.hip, layout ar_layout.h)
+// ---------------------------------------------------------------------------
+struct nqs_hip_ar {
+  int N = 0;
+  nqs::ar::Dims d;
+  DBuf w, m, v, op1f, op2f, op2b, op3f, op3b, w1t, b1, b2, b3;
+  DBuf partial, loss, bits, lq, rng, spins, grad;   // scratch
+  nqs::ArDev dev;
+  long t = 0;                                        // Adam steps taken
+  uint64_t shuffle = 0x243F6A8885A308D3ULL;          // epoch offsets
+  ~nqs_hip_ar() {
+    DBuf* all[] = {&w, &m, &v, &op1f, &op2f, &op2b, &op3f, &op3b, &w1t, &b1, &b2, &b3,
+                   &partial, &loss, &bits, &lq, &rng, &spins, &grad};
+    for (DBuf* b : all) b->release();
+  }
+};
+
 struct nqs_sampler {
   int N = 0;
   nqs_config_t cfg;
@@ -669,6 +688,20 @@ struct nqs_sampler {
   int last_rounds = 0;
   double stage_ms[7] = {0, 0, 0, 0, 0, 0, 0};
   int have_stage = 0;
+  // learned proposal: network, training window (bit rows, ring), proposal
+  // buffers and counters
+  nqs_hip_ar* ar = nullptr;
+  DBuf win, prop_bits, prop_spins, lq, la_prop, lacc;
+  long win_pos = 0, win_n = 0;
+  int since_train = 0, trained = 0, n_train = 0;
+  long long ar_props = 0;
+  double ar_loss = 0.0;
+  ~nqs_sampler() {
+    DBuf* all[] = {&d_spins, &d_rng, &d_acc, &d_tot, &d_la, &d_cur, &d_samples,
+                   &win, &prop_bits, &prop_spins, &lq, &la_prop, &lacc};
+    for (DBuf* b : all) b->release();
+    delete ar;
+  }
 };
 
 namespace {
@@ -678,6 +711,147 @@ uint64_t chain_seed(unsigned rng_seed, int c) {
   if (c == 0) return s0;
   uint64_t s = nqs::splitmix64(s0 ^ ((uint64_t)c * 0xD1B54A32D192ED03ULL));
   return s ? s : 1ULL;
+}
+
+// ---- learned proposal: host side ------------------------------------------
+// Initial weights: one xorshift64 stream over the parameter vector in layout
+// order (a draw per entry); W1 entries scale (2u - 1) / sqrt(N), W2 and W3
+// entries scale (2u - 1) / sqrt(64), biases 0, masked entries 0.
+void ar_init_host(int N, uint64_t seed, double scale, std::vector<float>& w) {
+  const nqs::ar::Dims d = nqs::ar::dims(N);
+  w.assign(d.P, 0.f);
+  uint64_t st = nqs::splitmix64(seed ^ 0xA17E5EEDULL);
+  if (!st) st = 1;
+  for (long p = 0; p < d.P; p++) {
+    const double u = nqs::xs_uniform(st);
+    double x = 0.0;
+    if (p < d.ob1) x = scale * (2.0 * u - 1.0) / sqrt((double)N);
+    else if ((p >= d.oW2 && p < d.ob2) || (p >= d.oW3 && p < d.ob3))
+      x = scale * (2.0 * u - 1.0) / 8.0;
+    w[p] = nqs::ar::free_param(d, p) ? (float)x : 0.f;
+  }
+}
+
+int ar_set(nqs_hip_ar* a, const float* host_w) {
+  CK(hipMemcpyAsync(a->w.p, host_w, sizeof(float) * a->d.P, hipMemcpyHostToDevice, g_stream));
+  return nqs::launch_ar_pack(a->dev, g_stream);
+}
+
+nqs_hip_ar* ar_new(int N, uint64_t seed, double scale) {
+  if (N < 2 || N > nqs::ar::kMaxN) {
+    fprintf(stderr, "nqs_hip: the learned proposal supports 2 <= N <= %d sites\n", nqs::ar::kMaxN);
+    return nullptr;
+  }
+  if (stream_ready()) return nullptr;
+  nqs_hip_ar* a = new nqs_hip_ar();
+  a->N = N;
+  a->d = nqs::ar::dims(N);
+  const nqs::ar::Dims& d = a->d;
+  struct Z { DBuf* b; size_t bytes; } z[] = {
+      {&a->w, 4 * (size_t)d.P},       {&a->m, 4 * (size_t)d.P},      {&a->v, 4 * (size_t)d.P},
+      {&a->op1f, 2 * (size_t)d.n_op1f}, {&a->op2f, 2 * (size_t)d.n_op2}, {&a->op2b, 2 * (size_t)d.n_op2},
+      {&a->op3f, 2 * (size_t)d.n_op3f}, {&a->op3b, 2 * (size_t)d.n_op3b},
+      {&a->w1t, 4 * (size_t)N * nqs::ar::H}, {&a->b1, 4 * (size_t)nqs::ar::H},
+      {&a->b2, 4 * (size_t)nqs::ar::H}, {&a->b3, 4 * (size_t)16 * d.nt}};
+  for (const Z& e : z) {
+    if (e.b->ensure(e.bytes) || hipMemsetAsync(e.b->p, 0, e.bytes, g_stream) != hipSuccess) {
+      delete a;
+      return nullptr;
+    }
+  }
+  nqs::ArDev& v = a->dev;
+  v.N = N;
+  v.w = a->w.as<float>();
+  v.m = a->m.as<float>();
+  v.v = a->v.as<float>();
+  v.op1f = a->op1f.as<uint16_t>();
+  v.op2f = a->op2f.as<uint16_t>();
+  v.op2b = a->op2b.as<uint16_t>();
+  v.op3f = a->op3f.as<uint16_t>();
+  v.op3b = a->op3b.as<uint16_t>();
+  v.w1t = a->w1t.as<float>();
+  v.b1 = a->b1.as<float>();
+  v.b2 = a->b2.as<float>();
+  v.b3 = a->b3.as<float>();
+  a->shuffle ^= seed;
+  if (!a->shuffle) a->shuffle = 1;
+  std::vector<float> hw;
+  ar_init_host(N, seed, scale, hw);
+  if (ar_set(a, hw.data()) || hipStreamSynchronize(g_stream) != hipSuccess) {
+    delete a;
+    return nullptr;
+  }
+  return a;
+}
+
+// Adam epochs over n_avail device bit rows; mean loss of the last epoch.
+int ar_train(nqs_hip_ar* a, const uint64_t* rows, long n_avail, int constrained, int epochs,
+             int batch, double lr, double* mean_loss) {
+  if (n_avail <= 0 || epochs <= 0) return 0;
+  if (batch < 1) batch = 1;
+  const int nb = (batch + 63) / 64;
+  const long steps = (n_avail + batch - 1) / batch;
+  CKR(a->partial.ensure(sizeof(float) * (size_t)nb * a->d.P));
+  CKR(a->loss.ensure(sizeof(double) * (size_t)nb * steps));
+  for (int e = 0; e < epochs; e++) {
+    if (e == epochs - 1) CKR(nqs::launch_fill(a->loss.as<double>(), 0.0, (long)nb * steps, g_stream));
+    const long off = (long)(nqs::xs_next(a->shuffle) % (uint64_t)n_avail);
+    for (long st = 0; st < steps; st++) {
+      const int count = (int)std::min<long>(batch, n_avail - st * batch);
+      nqs::ArGrad g;
+      g.rows = rows;
+      g.n_avail = n_avail;
+      g.start = (off + st * batch) % n_avail;
+      g.count = count;
+      g.constrained = constrained;
+      g.scale = 1.0f / (float)count;
+      g.partial = a->partial.as<float>();
+      g.loss = a->loss.as<double>() + st * nb;
+      CKR(nqs::launch_ar_grad(a->dev, g, g_stream));
+      a->t++;
+      nqs::ArAdam ad;
+      ad.partial = a->partial.as<float>();
+      ad.nblk = (count + 63) / 64;
+      ad.lr = (float)lr;
+      ad.bc1 = (float)(1.0 - pow(0.9, (double)a->t));
+      ad.bc2 = (float)(1.0 - pow(0.999, (double)a->t));
+      CKR(nqs::launch_ar_adam(a->dev, ad, g_stream));
+    }
+  }
+  std::vector<double> l((size_t)nb * steps);
+  CK(hipMemcpyAsync(l.data(), a->loss.p, sizeof(double) * l.size(), hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  double tot = 0.0;
+  for (double x : l) tot += x;
+  if (mean_loss) *mean_loss = tot / (double)n_avail;
+  return 0;
+}
+
+// retrain before a batch when due: first once the window holds samples,
+// then every ar_train_every batches
+int ar_maybe_train(nqs_sampler* s) {
+  const int every = std::max(1, s->opt.ar_train_every);
+  const bool due = s->trained ? (s->since_train >= every) : (s->win_n > 0);
+  if (!due) return 0;
+  double loss = 0.0;
+  CKR(ar_train(s->ar, s->win.as<uint64_t>(), s->win_n, 1, s->opt.ar_epochs, s->opt.ar_batch,
+               s->opt.ar_lr, &loss));
+  s->ar_loss = loss;
+  s->trained = 1;
+  s->since_train = 0;
+  s->n_train++;
+  return 0;
+}
+
+int ar_window_append(nqs_sampler* s, int n_out, const int* d_out) {
+  const long cap = std::max(1, s->opt.ar_train_window);
+  const long k = std::min<long>(n_out, cap);
+  CKR(nqs::launch_bits_append(d_out + (size_t)(n_out - k) * s->N, k, s->N, s->win.as<uint64_t>(),
+                              cap, s->win_pos, g_stream));
+  s->win_pos = (s->win_pos + k) % cap;
+  s->win_n = std::min(cap, s->win_n + k);
+  s->since_train++;
+  return 0;
 }
 
 int sampler_sync_cur(nqs_sampler* s) {
@@ -734,28 +908,71 @@ int ensure_model_for_sampler(nqs_sampler* s) {
   return 0;
 }
 
+// rounds of the learned proposal: opt.ar_local exchange moves, then one
+// global move per chain (walk -> log|psi| of the proposals -> accept), then
+// the chains' configurations are recorded as samples r*C + c < n_out
+template <typename R>
+int learned_rounds(nqs_sampler* s, int rounds, int n_out, int* d_out) {
+  const int C = s->cs.C;
+  ModelView<R> mv;
+  CKR(model_view<R>(s->src.a, mv));
+  nqs::MHLaunch ml;
+  ml.move = 1;
+  ml.steps = s->opt.ar_local;
+  ml.rounds = 1;
+  nqs::ArWalk w;
+  w.rows = 2 * C;
+  w.n_sample = C;
+  w.constrained = 1;
+  w.bits = s->cs.spins;
+  w.rng = s->cs.rng;
+  w.out_bits = s->prop_bits.as<uint64_t>();
+  w.out_spins = s->prop_spins.as<int>();
+  w.lq = s->lq.as<double>();
+  for (int r = 0; r < rounds; r++) {
+    if (ml.steps > 0) CKR(nqs::launch_mh<R>(mv, s->cs, ml, g_stream));
+    CKR(nqs::launch_ar_walk(s->ar->dev, w, g_stream));
+    CKR(forward_t<R>(s->src.a, s->prop_spins.as<int>(), C, s->la_prop.as<double>(), nullptr));
+    CKR(nqs::launch_ar_accept(s->cs, s->prop_bits.as<uint64_t>(), s->la_prop.as<double>(),
+                              s->lq.as<double>(), s->lacc.as<unsigned long long>(), g_stream));
+    if ((long)r * C < n_out) CKR(nqs::launch_unpack_rows(s->cs, (long)r * C, n_out, d_out, g_stream));
+  }
+  s->ar_props += (long long)rounds * C;
+  return 0;
+}
+
 // run `steps` moves per round for `rounds` rounds on every chain, optionally
-// recording samples r*C + c < n_out into d_out (device) / h_out (host)
+// recording samples r*C + c < n_out into d_out (device) / h_out (host).
+// A learned-proposal sampler trains when due before a recorded batch, runs
+// learned rounds once trained, and appends the batch to its window.
 int sampler_advance(nqs_sampler* s, int steps, int rounds, int n_out, int* d_out, int* h_out) {
   if (rounds <= 0) return 0;
   if (s->device) {
     CKR(ensure_model_for_sampler(s));
-    nqs::MHLaunch ml;
-    ml.move = s->opt.move == NQS_HIP_MOVE_EXCHANGE ? 1 : 0;
-    ml.steps = steps;
-    ml.rounds = rounds;
-    ml.record = d_out ? 1 : 0;
-    ml.n_out = n_out;
-    ml.out = d_out;
-    if (s->opt.precision == NQS_HIP_FP32) {
-      ModelView<float> v;
-      CKR(model_view<float>(s->src.a, v));
-      CKR(nqs::launch_mh<float>(v, s->cs, ml, g_stream));
+    const bool learned_batch = s->ar && d_out;
+    if (learned_batch) CKR(ar_maybe_train(s));
+    if (learned_batch && s->trained) {
+      if (s->opt.precision == NQS_HIP_FP32) CKR(learned_rounds<float>(s, rounds, n_out, d_out));
+      else CKR(learned_rounds<double>(s, rounds, n_out, d_out));
     } else {
-      ModelView<double> v;
-      CKR(model_view<double>(s->src.a, v));
-      CKR(nqs::launch_mh<double>(v, s->cs, ml, g_stream));
+      nqs::MHLaunch ml;
+      ml.move = s->opt.move == NQS_HIP_MOVE_FLIP ? 0 : 1;
+      ml.steps = steps;
+      ml.rounds = rounds;
+      ml.record = d_out ? 1 : 0;
+      ml.n_out = n_out;
+      ml.out = d_out;
+      if (s->opt.precision == NQS_HIP_FP32) {
+        ModelView<float> v;
+        CKR(model_view<float>(s->src.a, v));
+        CKR(nqs::launch_mh<float>(v, s->cs, ml, g_stream));
+      } else {
+        ModelView<double> v;
+        CKR(model_view<double>(s->src.a, v));
+        CKR(nqs::launch_mh<double>(v, s->cs, ml, g_stream));
+      }
     }
+    if (learned_batch) CKR(ar_window_append(s, n_out, d_out));
     if (h_out && d_out) {
       CK(hipMemcpyAsync(h_out, d_out, sizeof(int) * (size_t)n_out * s->N, hipMemcpyDeviceToHost,
                         g_stream));
@@ -805,10 +1022,6 @@ nqs_sampler* sampler_new(int N, const nqs_config_t* cfg, const nqs_hip_options_t
     if (env && atoi(env) > 0) s->opt.num_chains = atoi(env);
   }
   if (s->opt.num_chains < 1) s->opt.num_chains = 1;
-  if (s->opt.move == NQS_HIP_MOVE_LEARNED) {
-    fprintf(stderr, "nqs_hip: the learned proposal is driven through nqs_hip_sampler_create_ex "
-                    "with move=NQS_HIP_MOVE_LEARNED only on the device path\n");
-  }
   s->fn = fn;
   s->user = user;
   s->h_cur.assign(N, 1);
@@ -845,7 +1058,27 @@ nqs_sampler* sampler_new(int N, const nqs_config_t* cfg, const nqs_hip_options_t
     }
     (void)hipStreamSynchronize(g_stream);
     tmp.release();
+    if (s->opt.move == NQS_HIP_MOVE_LEARNED) {
+      const uint64_t seed = s->opt.ar_seed ? (uint64_t)s->opt.ar_seed : chain_seed(cfg->rng_seed, 0);
+      s->ar = ar_new(N, seed, 1.0);
+      const size_t cap = (size_t)std::max(1, s->opt.ar_train_window);
+      if (!s->ar || s->win.ensure(sizeof(uint64_t) * cap) ||
+          s->prop_bits.ensure(sizeof(uint64_t) * C) || s->prop_spins.ensure(sizeof(int) * (size_t)C * N) ||
+          s->lq.ensure(sizeof(double) * 2 * C) || s->la_prop.ensure(sizeof(double) * C) ||
+          s->lacc.ensure(sizeof(uint64_t) * C) ||
+          hipMemsetAsync(s->lacc.p, 0, sizeof(uint64_t) * C, g_stream) != hipSuccess ||
+          hipStreamSynchronize(g_stream) != hipSuccess) {
+        delete s;
+        return nullptr;
+      }
+    }
     return s;
+  }
+  if (s->opt.move == NQS_HIP_MOVE_LEARNED) {
+    fprintf(stderr, "nqs_hip: the learned proposal needs a device amplitude (nqs_ansatz_log_amp "
+                    "or nqs_marshall_log_amp over it)\n");
+    delete s;
+    return nullptr;
   }
   // host-callback chains: the reference algorithm against the caller's function
   s->hc.assign((size_t)C * N, 1);
@@ -890,7 +1123,8 @@ NQS_EXPORT nqs_hip_options_t nqs_hip_options_defaults(void) {
   o.ar_epochs = 4;
   o.ar_batch = 256;
   o.ar_lr = 1e-3;
-  o.ar_mix = 0.5;
+  o.ar_local = 0;
+  o.ar_seed = 0;
   return o;
 }
 
@@ -906,8 +1140,12 @@ NQS_EXPORT nqs_sampler_t* nqs_hip_sampler_create_ex(int num_sites, const nqs_con
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (!opt) return nullptr;
   if (opt->move == NQS_HIP_MOVE_LEARNED) {
-    fprintf(stderr, "nqs_hip: learned proposal is not available in this build\n");
-    return nullptr;
+    if (num_sites < 2 || num_sites > nqs::ar::kMaxN ||
+        (opt->ar_hidden != 0 && opt->ar_hidden != nqs::ar::H) || opt->ar_local < 0) {
+      fprintf(stderr, "nqs_hip: learned proposal needs 2 <= N <= %d, ar_hidden = %d and "
+                      "ar_local >= 0\n", nqs::ar::kMaxN, nqs::ar::H);
+      return nullptr;
+    }
   }
   return sampler_new(num_sites, cfg, opt, log_amp, user);
 }
@@ -1594,6 +1832,161 @@ NQS_EXPORT double nqs_hip_copy_probe(long bytes, int reps) {
   a.release();
   b.release();
   return ms / reps;
+}
+
+// ---- learned-proposal network API -------------------------------------------
+namespace {
+int ar_rows_to_device(nqs_hip_ar* a, const int* spins, int B, uint64_t*& d_bits) {
+  CKR(a->spins.ensure(sizeof(int) * (size_t)B * a->N));
+  CKR(a->bits.ensure(sizeof(uint64_t) * (size_t)B));
+  CK(hipMemcpyAsync(a->spins.p, spins, sizeof(int) * (size_t)B * a->N, hipMemcpyHostToDevice,
+                    g_stream));
+  CKR(nqs::launch_spins_to_bits(a->spins.as<int>(), B, a->N, a->bits.as<uint64_t>(), g_stream));
+  d_bits = a->bits.as<uint64_t>();
+  return 0;
+}
+}  // namespace
+
+NQS_EXPORT nqs_hip_ar_t* nqs_hip_ar_create(int num_sites, unsigned long long seed, double init_scale) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  return ar_new(num_sites, seed, init_scale);
+}
+
+NQS_EXPORT void nqs_hip_ar_free(nqs_hip_ar_t* a) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a) return;
+  (void)hipStreamSynchronize(g_stream);
+  delete a;
+}
+
+NQS_EXPORT long nqs_hip_ar_num_params(const nqs_hip_ar_t* a) { return a ? a->d.P : 0; }
+
+NQS_EXPORT int nqs_hip_ar_get_params(nqs_hip_ar_t* a, float* out) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !out) return -1;
+  CK(hipMemcpyAsync(out, a->w.p, sizeof(float) * a->d.P, hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+NQS_EXPORT int nqs_hip_ar_set_params(nqs_hip_ar_t* a, const float* in) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !in) return -1;
+  CKR(ar_set(a, in));
+  CK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+NQS_EXPORT int nqs_hip_ar_log_prob(nqs_hip_ar_t* a, const int* spins, int B, int constrained,
+                                   double* out) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !spins || !out || B <= 0) return -1;
+  uint64_t* bits = nullptr;
+  CKR(ar_rows_to_device(a, spins, B, bits));
+  CKR(a->lq.ensure(sizeof(double) * B));
+  nqs::ArWalk w;
+  w.rows = B;
+  w.n_sample = 0;
+  w.constrained = constrained ? 1 : 0;
+  w.bits = bits;
+  w.lq = a->lq.as<double>();
+  CKR(nqs::launch_ar_walk(a->dev, w, g_stream));
+  CK(hipMemcpyAsync(out, a->lq.p, sizeof(double) * B, hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+NQS_EXPORT int nqs_hip_ar_sample(nqs_hip_ar_t* a, const int* ref_spins, int B, int constrained,
+                                 const unsigned long long* seeds, int* out_spins, double* out_logq) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !ref_spins || !seeds || !out_spins || B <= 0) return -1;
+  std::vector<int> ref((size_t)B * a->N);
+  for (int b = 0; b < B; b++) memcpy(ref.data() + (size_t)b * a->N, ref_spins, sizeof(int) * a->N);
+  uint64_t* bits = nullptr;
+  CKR(ar_rows_to_device(a, ref.data(), B, bits));
+  std::vector<uint64_t> st(B);
+  for (int b = 0; b < B; b++) st[b] = seeds[b] ? seeds[b] : 1ULL;
+  CKR(a->rng.ensure(sizeof(uint64_t) * B));
+  CKR(a->lq.ensure(sizeof(double) * B));
+  CKR(a->grad.ensure(sizeof(int) * (size_t)B * a->N));
+  CK(hipMemcpyAsync(a->rng.p, st.data(), sizeof(uint64_t) * B, hipMemcpyHostToDevice, g_stream));
+  nqs::ArWalk w;
+  w.rows = B;
+  w.n_sample = B;
+  w.constrained = constrained ? 1 : 0;
+  w.bits = bits;
+  w.rng = a->rng.as<uint64_t>();
+  w.out_spins = a->grad.as<int>();
+  w.lq = a->lq.as<double>();
+  CKR(nqs::launch_ar_walk(a->dev, w, g_stream));
+  CK(hipMemcpyAsync(out_spins, a->grad.p, sizeof(int) * (size_t)B * a->N, hipMemcpyDeviceToHost,
+                    g_stream));
+  if (out_logq)
+    CK(hipMemcpyAsync(out_logq, a->lq.p, sizeof(double) * B, hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+NQS_EXPORT int nqs_hip_ar_gradient(nqs_hip_ar_t* a, const int* spins, int B, int constrained,
+                                   float* grad, double* mean_nll) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !spins || !grad || B <= 0) return -1;
+  uint64_t* bits = nullptr;
+  CKR(ar_rows_to_device(a, spins, B, bits));
+  const int nb = (B + 63) / 64;
+  CKR(a->partial.ensure(sizeof(float) * (size_t)nb * a->d.P));
+  CKR(a->loss.ensure(sizeof(double) * nb));
+  CKR(a->grad.ensure(sizeof(float) * a->d.P));
+  nqs::ArGrad g;
+  g.rows = bits;
+  g.n_avail = B;
+  g.start = 0;
+  g.count = B;
+  g.constrained = constrained ? 1 : 0;
+  g.scale = 1.0f / (float)B;
+  g.partial = a->partial.as<float>();
+  g.loss = a->loss.as<double>();
+  CKR(nqs::launch_ar_grad(a->dev, g, g_stream));
+  CKR(nqs::launch_ar_sum_grad(a->dev, a->partial.as<float>(), nb, a->grad.as<float>(), g_stream));
+  std::vector<double> l(nb);
+  CK(hipMemcpyAsync(grad, a->grad.p, sizeof(float) * a->d.P, hipMemcpyDeviceToHost, g_stream));
+  CK(hipMemcpyAsync(l.data(), a->loss.p, sizeof(double) * nb, hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  if (mean_nll) {
+    double t = 0.0;
+    for (double x : l) t += x;
+    *mean_nll = t / (double)B;
+  }
+  return 0;
+}
+
+NQS_EXPORT int nqs_hip_ar_train(nqs_hip_ar_t* a, const int* spins, int n, int constrained,
+                                int epochs, int minibatch, double lr, double* mean_nll) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !spins || n <= 0 || epochs <= 0 || minibatch <= 0) return -1;
+  uint64_t* bits = nullptr;
+  CKR(ar_rows_to_device(a, spins, n, bits));
+  return ar_train(a, bits, n, constrained ? 1 : 0, epochs, minibatch, lr, mean_nll);
+}
+
+NQS_EXPORT nqs_hip_ar_t* nqs_hip_sampler_ar(nqs_sampler_t* s) { return s ? s->ar : nullptr; }
+
+NQS_EXPORT int nqs_hip_sampler_learned_stats(const nqs_sampler_t* s, double* acceptance,
+                                             long long* proposals, int* trainings,
+                                             double* last_nll) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!s || !s->ar) return -1;
+  const int C = s->cs.C;
+  std::vector<unsigned long long> acc(C);
+  CK(hipMemcpyAsync(acc.data(), s->lacc.p, sizeof(uint64_t) * C, hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  unsigned long long t = 0;
+  for (unsigned long long x : acc) t += x;
+  if (acceptance) *acceptance = s->ar_props ? (double)t / (double)s->ar_props : 0.0;
+  if (proposals) *proposals = s->ar_props;
+  if (trainings) *trainings = s->n_train;
+  if (last_nll) *last_nll = s->ar_loss;
+  return 0;
 }
 
 NQS_EXPORT long long nqs_hip_device_launches(void) { return g_launches.load(); }

@@ -96,6 +96,60 @@ int launch_unpack(const ChainState& cs, int chain, int* out, hipStream_t st);
 // all chains' configurations as int32 rows [C][N]
 int launch_unpack_all(const ChainState& cs, int* out, hipStream_t st);
 
+// ---- learned proposal (k_ar.hip; network layout in ar_layout.h) -----------
+// Device copy of the masked autoregressive network: FP32 master weights and
+// Adam moments [P], BF16 MFMA operand copies, FP32 biases, and W1m
+// transposed [N][64] (BF16-rounded values) for the walk's per-site update.
+struct ArDev {
+  int N = 0;
+  float *w = nullptr, *m = nullptr, *v = nullptr;
+  uint16_t *op1f = nullptr, *op2f = nullptr, *op2b = nullptr, *op3f = nullptr, *op3b = nullptr;
+  float *w1t = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr;
+};
+// Rows [0, n_sample) draw a configuration with xorshift state rng[r] and
+// the magnetisation of bits[r]; rows [n_sample, rows) evaluate the
+// configuration bits[r - n_sample].  lq[r] = log q of the row's result.
+struct ArWalk {
+  int rows = 0, n_sample = 0, constrained = 1;
+  const uint64_t* bits = nullptr;
+  uint64_t* rng = nullptr;
+  uint64_t* out_bits = nullptr;   // [n_sample], optional
+  int* out_spins = nullptr;       // [n_sample][N], optional
+  double* lq = nullptr;           // [rows]
+};
+// Gradient of the summed negative log-likelihood times `scale` over rows
+// (start + b) mod n_avail, b < count; partial[blk][P] and loss[blk] per
+// 64-row block.
+struct ArGrad {
+  const uint64_t* rows = nullptr;
+  long n_avail = 0, start = 0;
+  int count = 0, constrained = 1;
+  float scale = 1.f;
+  float* partial = nullptr;
+  double* loss = nullptr;
+};
+struct ArAdam {
+  const float* partial = nullptr;
+  int nblk = 0;
+  float lr = 1e-3f, bc1 = 1.f, bc2 = 1.f, beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+};
+int launch_ar_walk(const ArDev& v, const ArWalk& w, hipStream_t st);
+int launch_ar_grad(const ArDev& v, const ArGrad& g, hipStream_t st);
+int launch_ar_adam(const ArDev& v, const ArAdam& a, hipStream_t st);
+int launch_ar_pack(const ArDev& v, hipStream_t st);
+int launch_ar_sum_grad(const ArDev& v, const float* partial, int nblk, float* out, hipStream_t st);
+// one global move per chain (nw == 1): proposals prop[C], log|psi| of the
+// proposals la_prop[C], lq[2C] = (log q of proposals, log q of current);
+// lacc[c] counts accepted learned moves.
+int launch_ar_accept(ChainState& cs, const uint64_t* prop, const double* la_prop, const double* lq,
+                     unsigned long long* lacc, hipStream_t st);
+// sample rows first + c of every chain c (rows < n_out) as int32 +-1
+int launch_unpack_rows(const ChainState& cs, long first, long n_out, int* out, hipStream_t st);
+// int32 rows -> bitmasks (N <= 64): ring positions (pos + b) mod cap
+int launch_bits_append(const int* spins, long n, int N, uint64_t* win, long cap, long pos,
+                       hipStream_t st);
+int launch_spins_to_bits(const int* spins, long n, int N, uint64_t* out, hipStream_t st);
+
 // ---- local energy (k_eloc.hip) -------------------------------------------
 template <typename R>
 int launch_eloc(const ModelView<R>& m, const HamDev& h, const int* spins, int B,

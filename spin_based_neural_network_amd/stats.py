@@ -1,0 +1,29 @@
+"""Sampling statistics of the metric: integrated autocorrelation time and
+effective sample size of per-chain local-energy series."""
+import numpy as np
+
+
+def pooled_tau(series):
+    """tau_int from the chain-averaged autocorrelation (Sokal window, c = 5).
+
+    series: [C][n] per-chain series (each chain's mean removed).  Returns at
+    least 0.5 (independent samples)."""
+    x = np.asarray(series, dtype=float)
+    x = x - x.mean(axis=1, keepdims=True)
+    C, n = x.shape
+    var = np.mean(x * x)
+    if var <= 0 or n < 4:
+        return 0.5
+    tau = 0.5
+    for t in range(1, n):
+        rho = np.mean(x[:, :-t] * x[:, t:]) * n / (n - t) / var
+        tau += rho
+        if t >= 5.0 * tau:
+            break
+    return max(tau, 0.5)
+
+
+def ess(series):
+    """Effective sample size sum_c n_c / (2 tau) with the pooled tau."""
+    x = np.asarray(series, dtype=float)
+    return x.size / (2.0 * pooled_tau(x))

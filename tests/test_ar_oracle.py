@@ -1,0 +1,150 @@
+"""CPU checks of the learned-proposal network oracle (oracle/ar_oracle.c).
+
+The reference has no learned proposal, so the oracle is pinned by what the
+mathematics fixes:
+  * an autoregressive density sums to one over its support -- over all 2^N
+    configurations unconstrained, over the magnetisation sector when
+    constrained -- and only if every mask keeps l_j independent of s_>=j;
+  * the backward pass equals central finite differences of the FP64 loss;
+  * the BF16-quantised forward stays within rounding of the FP64 one;
+  * the Adam step follows its closed form at step 1 (-lr g / (|g| + eps)).
+"""
+import ctypes
+import itertools
+
+import numpy as np
+import pytest
+
+from spin_based_neural_network_amd.abi import ptr
+
+
+def _all_configs(N, n_down=None):
+    rows = [c for c in itertools.product([1, -1], repeat=N)
+            if n_down is None or sum(1 for x in c if x < 0) == n_down]
+    return np.array(rows, dtype=np.int32)
+
+
+def _params(orc, N, seed=7, scale=1.0):
+    P = orc.orc_ar_num_params(N)
+    w = np.zeros(P, dtype=np.float32)
+    orc.orc_ar_init(N, seed, scale, ptr(w))
+    return w
+
+
+def _logq(orc, N, w, sp, constrained, quantize):
+    out = np.zeros(len(sp))
+    assert orc.orc_ar_log_prob(N, ptr(w), ptr(np.ascontiguousarray(sp)), len(sp), constrained,
+                               quantize, ptr(out)) == 0
+    return out
+
+
+def test_num_params(orc):
+    # 24 -> 64 -> 64 -> 24: 2*64*24 + 64*64 + 2*64 + 24 = 7320
+    assert orc.orc_ar_num_params(24) == 7320
+    assert orc.orc_ar_num_params(8) == 2 * 64 * 8 + 64 * 64 + 128 + 8
+
+
+def test_masks_leave_free_weights(orc):
+    N = 10
+    w = _params(orc, N)
+    # W1 row k has exactly m(k) = 1 + k mod (N-1) free inputs
+    W1 = w[:64 * N].reshape(64, N)
+    for k in range(64):
+        m = 1 + k % (N - 1)
+        assert np.all(W1[k, m:] == 0) and np.count_nonzero(W1[k, :m]) == m
+    # the first output has no hidden input: its weights are all zero
+    W3 = w[64 * N + 64 + 64 * 64 + 64:64 * N + 64 + 64 * 64 + 64 + N * 64].reshape(N, 64)
+    assert np.all(W3[0] == 0)
+
+
+@pytest.mark.parametrize("N", [4, 7, 10])
+@pytest.mark.parametrize("quantize", [0, 1])
+def test_normalised_over_support(orc, N, quantize):
+    w = _params(orc, N, seed=N, scale=2.0)
+    full = _all_configs(N)
+    assert abs(np.exp(_logq(orc, N, w, full, 0, quantize)).sum() - 1.0) < 1e-12
+    for nd in (N // 2, 1, N - 1):
+        sec = _all_configs(N, nd)
+        assert abs(np.exp(_logq(orc, N, w, sec, 1, quantize)).sum() - 1.0) < 1e-12
+
+
+def test_normalisation_detects_a_broken_mask(orc):
+    # plant a mistake: give output 0 a dependence on site 0 through a hidden
+    # unit of degree 1 (mask3 excludes it); the density no longer sums to 1
+    N = 6
+    w = _params(orc, N, scale=2.0)
+    full = _all_configs(N)
+    good = np.exp(_logq(orc, N, w, full, 0, 0)).sum()
+    # the oracle applies masks itself, so emulate the broken network by
+    # evaluating l_0 with the true s_0 folded into b3[0]: q(s) changes by a
+    # factor that is not a conditional of a normalised distribution
+    ob3 = len(w) - N
+    lq = []
+    for c in full:
+        wb = w.copy()
+        wb[ob3] += 1.5 * c[0]
+        lq.append(_logq(orc, N, wb, c[None, :], 0, 0)[0])
+    bad = np.exp(np.array(lq)).sum()
+    assert abs(good - 1.0) < 1e-12 and abs(bad - 1.0) > 1e-3
+
+
+def test_gradient_matches_finite_differences(orc):
+    N, B = 8, 6
+    rng = np.random.default_rng(0)
+    w = _params(orc, N, seed=3, scale=1.0)
+    sp = np.ones((B, N), dtype=np.int32)
+    for b in range(B):
+        sp[b, rng.permutation(N)[:N // 2]] = -1
+    for constrained in (0, 1):
+        g = np.zeros(len(w))
+        nll = ctypes.c_double()
+        assert orc.orc_ar_gradient(N, ptr(w), ptr(sp), B, constrained, 0, ptr(g),
+                                   ctypes.byref(nll)) == 0
+        assert abs(nll.value + _logq(orc, N, w, sp, constrained, 0).mean()) < 1e-12
+
+        def loss(wv):
+            return -_logq(orc, N, wv, sp, constrained, 0).mean()
+
+        idx = rng.choice(np.nonzero(g)[0], 40, replace=False)
+        for p in idx:
+            h = np.float32(1e-2 * max(1e-3, abs(float(w[p]))))
+            wp, wm = w.copy(), w.copy()
+            wp[p] += h
+            wm[p] -= h
+            fd = (loss(wp) - loss(wm)) / (float(wp[p]) - float(wm[p]))
+            assert abs(fd - g[p]) <= 1e-6 * max(1e-3, abs(g[p])), (constrained, p, fd, g[p])
+        # masked parameters get no gradient
+        free = w != 0
+        assert np.all(g[~free & (np.arange(len(w)) < 64 * N)] == 0)
+
+
+def test_quantised_forward_close_to_fp64(orc):
+    N = 12
+    w = _params(orc, N, seed=5, scale=1.0)
+    sp = _all_configs(N, N // 2)[::7]
+    a = _logq(orc, N, w, sp, 1, 0)
+    b = _logq(orc, N, w, sp, 1, 1)
+    # BF16 weights and activations: relative rounding 2^-9 per element
+    assert np.max(np.abs(a - b)) < 0.05
+    assert np.max(np.abs(a - b)) > 0.0
+
+
+def test_adam_first_step_closed_form(orc):
+    N = 6
+    w = _params(orc, N)
+    P = len(w)
+    g = np.random.default_rng(1).normal(0, 1e-2, P)
+    m, v = np.zeros(P), np.zeros(P)
+    w1 = w.copy()
+    orc.orc_ar_adam_step(N, ptr(w1), ptr(m), ptr(v), ptr(g), 1, 1e-3)
+    free = np.zeros(P, dtype=bool)
+    free[w != 0] = True
+    # biases are free as well
+    ob1, oW2 = 64 * N, 64 * N + 64
+    ob2, oW3 = oW2 + 64 * 64, oW2 + 64 * 64 + 64
+    free[ob1:oW2] = True
+    free[ob2:oW3] = True
+    free[P - N:] = True
+    expect = w.astype(np.float64) - 1e-3 * g / (np.abs(g) + 1e-8)
+    assert np.allclose(w1[free], expect[free].astype(np.float32), rtol=0, atol=1e-7)
+    assert np.all(w1[~free] == 0)
