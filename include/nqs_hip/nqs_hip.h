@@ -125,7 +125,8 @@ int nqs_hip_logpsi_gradient_batch(nqs_ansatz_t *a, const int *spins_batch,
 /* Learned-proposal network on its own (the sampler's, or a standalone one
  * for tests and offline training).  Parameters are FP32, layout
  * [W1 (64 x N) | b1 (64) | W2 (64 x 64) | b2 (64) | W3 (N x 64) | b3 (N)],
- * with MADE masks (hidden degree 1 + k mod (N-1)); masked entries stay 0.
+ * with MADE masks (hidden unit k has degree 1 + floor(k (N-1) / 64)); masked
+ * entries stay 0.
  * `constrained` restricts a configuration to the magnetisation of its
  * reference (sampling: ref_spins; evaluation and training: itself). */
 typedef struct nqs_hip_ar nqs_hip_ar_t;

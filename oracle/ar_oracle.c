@@ -6,10 +6,12 @@
  * The reference has no learned proposal (its autoregressive ansatz kind is
  * rejected by nqs_ansatz_create, src/nqs/nqs_ansatz.c:250-251), so this file
  * restates this library's own definition of the network: N -> 64 -> 64 -> N
- * masked autoregressive MLP (MADE degrees 1 + k mod (N-1)), ReLU, logit l_j
+ * masked autoregressive MLP (MADE degrees 1 + floor(k (N-1) / 64)), ReLU, logit l_j
  * of p(s_j = -1 | s_<j), parameters [W1 | b1 | W2 | b2 | W3 | b3] in FP32.
  * With quantize = 1 the weights and the activations a1, a2 are rounded to
- * BF16 (round to nearest even) where the device rounds them; sums are FP64.
+ * BF16 (round to nearest even) where the device rounds them, sums are FP64,
+ * and log q uses the device's conditional: p = sigmoid(l) evaluated in FP32
+ * with l clamped to [-16, 16], log p or log(1 - p) in FP64.
  * With quantize = 0 everything is FP64 (used for finite-difference checks of
  * the backward pass).  A magnetisation constraint forces a site once the
  * remaining sites must all be up or all down; forced sites contribute
@@ -63,7 +65,7 @@ static double qa(double x, int quantize) {
     return quantize ? (double)bf2f(f2bf((float)r)) : r;
 }
 
-static int degree(int k, int N) { return 1 + k % (N > 1 ? N - 1 : 1); }
+static int degree(int k, int N) { return N > 1 ? 1 + (k * (N - 1)) / ARH : 1; }
 static int mask1(int k, int i, int N) { return degree(k, N) >= i + 1; }
 static int mask2(int kp, int k, int N) { return degree(kp, N) >= degree(k, N); }
 static int mask3(int j, int kp, int N) { return j + 1 > degree(kp, N); }
@@ -157,8 +159,16 @@ int orc_ar_log_prob(int N, const float *w, const int *spins, int B, int constrai
         double lq = 0.0;
         for (int j = 0; j < N; j++) {
             if (!fr[j]) continue;
-            const double ls = log_sigmoid(l[j]);
-            lq += s[j] < 0 ? ls : ls - l[j];
+            if (quantize) {
+                /* the device conditional: FP32 sigmoid of l clamped to
+                 * [-16, 16]; probabilities p and 1 - p in FP64 */
+                const float lc = fminf(fmaxf((float)l[j], -16.0f), 16.0f);
+                const double p = (double)(1.0f / (1.0f + expf(-lc)));
+                lq += s[j] < 0 ? log(p) : log(1.0 - p);
+            } else {
+                const double ls = log_sigmoid(l[j]);
+                lq += s[j] < 0 ? ls : ls - l[j];
+            }
         }
         out[b] = lq;
     }

@@ -8,7 +8,9 @@
 //   z2 = b2 + W2m a1,  a1 = bf16(relu(z1))                        [H]
 //   l  = b3 + W3m a2,  a2 = bf16(relu(z2))                        [N]
 //   p(s_j = -1 | s_<j) = sigmoid(l_j)
-// Hidden unit k has degree m(k) = 1 + k mod (N-1); site i has degree i+1.
+// Hidden unit k has degree m(k) = 1 + floor(k (N-1) / H), non-decreasing in k,
+// so 16-unit tiles and 32-unit chunks cover contiguous degree ranges; site i
+// has degree i+1.
 // mask1[k][i] = m(k) >= i+1, mask2[k'][k] = m(k') >= m(k),
 // mask3[j][k'] = j+1 > m(k'), so l_j depends on s_0..s_{j-1} only.
 // Parameters (FP32 master), P = 2HN + H^2 + 2H + N:
@@ -66,7 +68,7 @@ NQS_AR_HD inline Dims dims(int N) {
   return d;
 }
 
-NQS_AR_HD inline int degree(int k, int N) { return 1 + k % (N > 1 ? N - 1 : 1); }
+NQS_AR_HD inline int degree(int k, int N) { return N > 1 ? 1 + (k * (N - 1)) / H : 1; }
 NQS_AR_HD inline bool mask1(int k, int i, int N) { return degree(k, N) >= i + 1; }
 NQS_AR_HD inline bool mask2(int kp, int k, int N) { return degree(kp, N) >= degree(k, N); }
 NQS_AR_HD inline bool mask3(int j, int kp, int N) { return j + 1 > degree(kp, N); }

@@ -691,14 +691,14 @@ struct nqs_sampler {
   // learned proposal: network, training window (bit rows, ring), proposal
   // buffers and counters
   nqs_hip_ar* ar = nullptr;
-  DBuf win, prop_bits, prop_spins, lq, la_prop, lacc;
+  DBuf win, lacc;
   long win_pos = 0, win_n = 0;
   int since_train = 0, trained = 0, n_train = 0;
   long long ar_props = 0;
   double ar_loss = 0.0;
   ~nqs_sampler() {
     DBuf* all[] = {&d_spins, &d_rng, &d_acc, &d_tot, &d_la, &d_cur, &d_samples,
-                   &win, &prop_bits, &prop_spins, &lq, &la_prop, &lacc};
+                   &win, &lacc};
     for (DBuf* b : all) b->release();
     delete ar;
   }
@@ -908,34 +908,35 @@ int ensure_model_for_sampler(nqs_sampler* s) {
   return 0;
 }
 
-// rounds of the learned proposal: opt.ar_local exchange moves, then one
-// global move per chain (walk -> log|psi| of the proposals -> accept), then
-// the chains' configurations are recorded as samples r*C + c < n_out
+// rounds of the learned proposal: without local moves all rounds run in
+// one launch; with opt.ar_local > 0 each round is ar_local exchange moves
+// followed by a one-round learned launch
 template <typename R>
 int learned_rounds(nqs_sampler* s, int rounds, int n_out, int* d_out) {
   const int C = s->cs.C;
   ModelView<R> mv;
   CKR(model_view<R>(s->src.a, mv));
-  nqs::MHLaunch ml;
-  ml.move = 1;
-  ml.steps = s->opt.ar_local;
-  ml.rounds = 1;
-  nqs::ArWalk w;
-  w.rows = 2 * C;
-  w.n_sample = C;
-  w.constrained = 1;
-  w.bits = s->cs.spins;
-  w.rng = s->cs.rng;
-  w.out_bits = s->prop_bits.as<uint64_t>();
-  w.out_spins = s->prop_spins.as<int>();
-  w.lq = s->lq.as<double>();
-  for (int r = 0; r < rounds; r++) {
-    if (ml.steps > 0) CKR(nqs::launch_mh<R>(mv, s->cs, ml, g_stream));
-    CKR(nqs::launch_ar_walk(s->ar->dev, w, g_stream));
-    CKR(forward_t<R>(s->src.a, s->prop_spins.as<int>(), C, s->la_prop.as<double>(), nullptr));
-    CKR(nqs::launch_ar_accept(s->cs, s->prop_bits.as<uint64_t>(), s->la_prop.as<double>(),
-                              s->lq.as<double>(), s->lacc.as<unsigned long long>(), g_stream));
-    if ((long)r * C < n_out) CKR(nqs::launch_unpack_rows(s->cs, (long)r * C, n_out, d_out, g_stream));
+  nqs::ArRounds ar;
+  ar.constrained = 1;
+  ar.lacc = s->lacc.as<unsigned long long>();
+  if (s->opt.ar_local <= 0) {
+    ar.rounds = rounds;
+    ar.n_out = n_out;
+    ar.out = d_out;
+    CKR(nqs::launch_ar_mh<R>(s->ar->dev, mv, s->cs, ar, g_stream));
+  } else {
+    nqs::MHLaunch ml;
+    ml.move = 1;
+    ml.steps = s->opt.ar_local;
+    ml.rounds = 1;
+    ar.rounds = 1;
+    for (int r = 0; r < rounds; r++) {
+      CKR(nqs::launch_mh<R>(mv, s->cs, ml, g_stream));
+      const long first = (long)r * C;
+      ar.out = first < n_out ? d_out + first * s->N : nullptr;
+      ar.n_out = std::max(0L, (long)n_out - first);
+      CKR(nqs::launch_ar_mh<R>(s->ar->dev, mv, s->cs, ar, g_stream));
+    }
   }
   s->ar_props += (long long)rounds * C;
   return 0;
@@ -1062,10 +1063,7 @@ nqs_sampler* sampler_new(int N, const nqs_config_t* cfg, const nqs_hip_options_t
       const uint64_t seed = s->opt.ar_seed ? (uint64_t)s->opt.ar_seed : chain_seed(cfg->rng_seed, 0);
       s->ar = ar_new(N, seed, 1.0);
       const size_t cap = (size_t)std::max(1, s->opt.ar_train_window);
-      if (!s->ar || s->win.ensure(sizeof(uint64_t) * cap) ||
-          s->prop_bits.ensure(sizeof(uint64_t) * C) || s->prop_spins.ensure(sizeof(int) * (size_t)C * N) ||
-          s->lq.ensure(sizeof(double) * 2 * C) || s->la_prop.ensure(sizeof(double) * C) ||
-          s->lacc.ensure(sizeof(uint64_t) * C) ||
+      if (!s->ar || s->win.ensure(sizeof(uint64_t) * cap) || s->lacc.ensure(sizeof(uint64_t) * C) ||
           hipMemsetAsync(s->lacc.p, 0, sizeof(uint64_t) * C, g_stream) != hipSuccess ||
           hipStreamSynchronize(g_stream) != hipSuccess) {
         delete s;

@@ -12,7 +12,7 @@
 //             ratio q(s)/q(s') below is exact.  With `constrained`, a site
 //             is forced once the remaining sites must all be up or all down,
 //             which keeps every proposal in the chain's magnetisation sector.
-// k_ar_accept accept / reject of one global move per chain:
+// k_ar_mh     learned Metropolis-Hastings rounds of 16 chains per wave:
 //             d = 2(log|psi(s')| - log|psi(s)|) + log q(s) - log q(s');
 //             accept when d >= 0 or log u < d (u from the chain's xorshift).
 // k_ar_grad   forward + backward of the mean negative log-likelihood over a
@@ -39,22 +39,25 @@ __device__ inline f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 __device__ inline float relu(float v) { return v > 0.f ? v : 0.f; }
+// BF16 round-to-nearest-even (v_cvt_pk_bf16_f32; the same rounding as
+// ar::f2bf for finite values)
+__device__ inline uint16_t to_bf(float v) { return __builtin_bit_cast(uint16_t, (__bf16)v); }
 // chunk c operand from per-tile values z[t][r] (feature 16t + 4q + r)
 __device__ inline bf16x8 relu_op(const f32x4* z, int c) {
-  u16x8 u;
+  bf16x8 r;
 #pragma unroll
-  for (int e = 0; e < 8; e++) u[e] = ar::f2bf(relu(z[2 * c + (e >> 2)][e & 3]));
-  return __builtin_bit_cast(bf16x8, u);
+  for (int e = 0; e < 8; e++) r[e] = (__bf16)relu(z[2 * c + (e >> 2)][e & 3]);
+  return r;
 }
 template <int T>
 __device__ inline bf16x8 plain_op(const f32x4* z, int c) {
-  u16x8 u;
+  bf16x8 r;
 #pragma unroll
   for (int e = 0; e < 8; e++) {
     const int t = 2 * c + (e >> 2);
-    u[e] = t < T ? ar::f2bf(z[t][e & 3]) : (unsigned short)0;
+    r[e] = (__bf16)(t < T ? z[t][e & 3] : 0.f);
   }
-  return __builtin_bit_cast(bf16x8, u);
+  return r;
 }
 __device__ inline double log_sigmoid(double x) {
   return x >= 0.0 ? -log1p(exp(-x)) : x - log1p(exp(x));
@@ -63,79 +66,136 @@ __device__ inline f32x4 zero4() { return f32x4{0.f, 0.f, 0.f, 0.f}; }
 __device__ inline uint64_t site_mask(int N) { return N >= 64 ? ~0ULL : ((1ULL << N) - 1ULL); }
 
 // ---------------------------------------------------------------------------
+// The walk over one 16-row tile, shared by every kernel that samples or
+// evaluates q (so that all of them compute the same function).
 template <int NT>
-__global__ __launch_bounds__(64) void k_ar_walk(ArDev v, ArWalk w) {
-  __shared__ float s_w1t[ar::kMaxN * ar::H];
-  const int N = v.N, lane = threadIdx.x, rl = lane & 15, q = lane >> 4;
-  for (int e = lane; e < N * ar::H; e += 64) s_w1t[e] = v.w1t[e];
-  const int row = blockIdx.x * 16 + rl;
-  const bool live = row < w.rows;
-  const bool samp = row < w.n_sample;
-
+struct WalkRegs {
   bf16x8 A2[4][2], A3[NT][2];
+  f32x4 B2[4], B3[NT], Z1[4];
+};
+
+template <int NT>
+__device__ inline void walk_load(const ArDev& v, float* s_w1t, WalkRegs<NT>& W) {
+  const int N = v.N, lane = threadIdx.x, q = lane >> 4;
+  for (int e = lane; e < N * ar::H; e += 64) s_w1t[e] = v.w1t[e];
 #pragma unroll
   for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-    for (int c = 0; c < 2; c++) A2[mt][c] = ld_op(v.op2f, ((long)(mt * 2 + c) * 64 + lane) * 8);
+    for (int c = 0; c < 2; c++) W.A2[mt][c] = ld_op(v.op2f, ((long)(mt * 2 + c) * 64 + lane) * 8);
 #pragma unroll
   for (int mt = 0; mt < NT; mt++)
 #pragma unroll
-    for (int c = 0; c < 2; c++) A3[mt][c] = ld_op(v.op3f, ((long)(mt * 2 + c) * 64 + lane) * 8);
-  f32x4 B2[4], B3[NT], z1[4];
+    for (int c = 0; c < 2; c++) W.A3[mt][c] = ld_op(v.op3f, ((long)(mt * 2 + c) * 64 + lane) * 8);
 #pragma unroll
   for (int t = 0; t < 4; t++)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      B2[t][r] = v.b2[16 * t + 4 * q + r];
-      z1[t][r] = v.b1[16 * t + 4 * q + r];
+      W.B2[t][r] = v.b2[16 * t + 4 * q + r];
+      W.Z1[t][r] = v.b1[16 * t + 4 * q + r];
     }
 #pragma unroll
   for (int t = 0; t < NT; t++)
 #pragma unroll
-    for (int r = 0; r < 4; r++) B3[t][r] = v.b3[16 * t + 4 * q + r];
+    for (int r = 0; r < 4; r++) W.B3[t][r] = v.b3[16 * t + 4 * q + r];
+}
 
-  uint64_t bits = 0;
-  if (live) bits = w.bits[samp ? row : row - w.n_sample];
-  const int target = __popcll(bits & site_mask(N));
-  uint64_t rng = (live && samp) ? w.rng[row] : 0ULL;
-  uint64_t ob = 0;
+// Lane (row rl = lane%16, group q = lane/16).  samp: draw each site with
+// the row's xorshift state `rng` (one draw per site); otherwise read the
+// sites from `bits`.  `target`: down spins of the row's sector.  Returns the
+// configuration in `ob` and log q in `lq`, identical in the row's 4 lanes.
+// The conditional is p = sigmoid(l) in FP32 with l clamped to [-16, 16]
+// (so 0 < p < 1 in FP32); site j is down when u < p, which happens with
+// probability p, and up with probability 1 - p (FP64), so q is the product
+// of exactly the probabilities the draws use.  The product is kept in FP64
+// with a logarithm taken whenever it falls below 1e-280 and at the end.
+template <int NT>
+__device__ inline void walk_tile(const WalkRegs<NT>& W, const float* s_w1t, int N, bool samp,
+                                 int constrained, uint64_t bits, int target, uint64_t& rng,
+                                 uint64_t& ob, double& lq) {
+  const int lane = threadIdx.x, rl = lane & 15, q = lane >> 4;
+  f32x4 z1[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) z1[t] = W.Z1[t];
+  ob = 0;
+  lq = 0.0;
+  double pr = 1.0;
   int downs = 0;
-  double lq = 0.0;
-  __syncthreads();
-
   for (int j = 0; j < N; j++) {
     const bf16x8 a0 = relu_op(z1, 0), a1 = relu_op(z1, 1);
+    // Degrees are non-decreasing in the unit index, so whole MFMA tiles are
+    // masked out: tile mt of z2 enters l_j only if its first unit has degree
+    // <= j, and a1 chunk 1 enters tile mt only if its first unit's degree is
+    // <= the tile's last.  Skipped products are sums of exact zeros.
     f32x4 z2[4];
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++) z2[mt] = mfma(A2[mt][1], a1, mfma(A2[mt][0], a0, B2[mt]));
+    for (int mt = 0; mt < 4; mt++) {
+      if (ar::degree(16 * mt, N) <= j) {
+        f32x4 acc = mfma(W.A2[mt][0], a0, W.B2[mt]);
+        if (ar::degree(32, N) <= ar::degree(16 * mt + 15, N)) acc = mfma(W.A2[mt][1], a1, acc);
+        z2[mt] = acc;
+      } else {
+        z2[mt] = zero4();
+      }
+    }
     const bf16x8 c0 = relu_op(z2, 0), c1 = relu_op(z2, 1);
     const int mj = j >> 4;
+    const bool need1 = ar::degree(32, N) <= j;
     f32x4 lg = zero4();
 #pragma unroll
     for (int t = 0; t < NT; t++)
-      if (t == mj) lg = mfma(A3[t][1], c1, mfma(A3[t][0], c0, B3[t]));
+      if (t == mj) {
+        lg = mfma(W.A3[t][0], c0, W.B3[t]);
+        if (need1) lg = mfma(W.A3[t][1], c1, lg);
+      }
     const int rj = j & 3, qj = (j >> 2) & 3;
     const float mine = rj == 0 ? lg[0] : rj == 1 ? lg[1] : rj == 2 ? lg[2] : lg[3];
     const float l = __shfl(mine, rl + 16 * qj, 64);
 
     const int rem_d = target - downs, rem = N - j;
-    const bool forced = w.constrained && (rem_d <= 0 || rem_d >= rem);
+    const bool forced = constrained && (rem_d <= 0 || rem_d >= rem);
     const double u = samp ? xs_uniform(rng) : 0.0;
-    const double ld = log_sigmoid((double)l);   // log p(down)
+    const float lc = fminf(fmaxf(l, -16.f), 16.f);
+    const double p = (double)(1.f / (1.f + __expf(-lc)));   // p(down)
     int bit;
     if (forced) bit = rem_d > 0 ? 1 : 0;
-    else if (samp) bit = u < exp(ld) ? 1 : 0;
+    else if (samp) bit = u < p ? 1 : 0;
     else bit = (int)((bits >> j) & 1ULL);
-    if (!forced) lq += bit ? ld : ld - (double)l;
+    if (!forced) {
+      pr *= bit ? p : 1.0 - p;
+      if (pr < 1e-280) {
+        lq += log(pr);
+        pr = 1.0;
+      }
+    }
     downs += bit;
     ob |= (uint64_t)bit << j;
-    const float s = bit ? -1.f : 1.f;
+    const float sg = bit ? -1.f : 1.f;
     const float* wr = s_w1t + j * ar::H + 4 * q;
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) z1[t][r] = fmaf(wr[16 * t + r], s, z1[t][r]);
+      for (int r = 0; r < 4; r++) z1[t][r] = fmaf(wr[16 * t + r], sg, z1[t][r]);
   }
+  lq += log(pr);
+}
+
+template <int NT>
+__global__ __launch_bounds__(64) void k_ar_walk(ArDev v, ArWalk w) {
+  __shared__ float s_w1t[ar::kMaxN * ar::H];
+  WalkRegs<NT> W;
+  walk_load<NT>(v, s_w1t, W);
+  const int N = v.N, lane = threadIdx.x, rl = lane & 15, q = lane >> 4;
+  const int row = blockIdx.x * 16 + rl;
+  const bool live = row < w.rows;
+  const bool samp = row < w.n_sample;
+  uint64_t bits = 0;
+  if (live) bits = w.bits[samp ? row : row - w.n_sample];
+  const int target = __popcll(bits & site_mask(N));
+  uint64_t rng = (live && samp) ? w.rng[row] : 0ULL;
+  __syncthreads();
+  uint64_t ob;
+  double lq;
+  walk_tile<NT>(W, s_w1t, N, samp, w.constrained, bits, target, rng, ob, lq);
   if (!live) return;
   if (q == 0) w.lq[row] = lq;
   if (samp) {
@@ -147,32 +207,97 @@ __global__ __launch_bounds__(64) void k_ar_walk(ArDev v, ArWalk w) {
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_ar_accept(ChainState cs, const uint64_t* __restrict__ prop,
-                            const double* __restrict__ la_prop, const double* __restrict__ lq,
-                            unsigned long long* __restrict__ lacc) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cs.C) return;
-  uint64_t rng = cs.rng[c];
-  const double d = 2.0 * (la_prop[c] - cs.la[c]) + (lq[cs.C + c] - lq[c]);
-  const bool acc = d >= 0.0 ? true : (log(xs_uniform(rng)) < d);
-  cs.rng[c] = rng;
-  cs.tot[c] += 1;
-  if (acc) {
-    cs.spins[c] = prop[c];
-    cs.la[c] = la_prop[c];
-    cs.acc[c] += 1;
-    lacc[c] += 1;
+// Learned Metropolis-Hastings rounds, CPW chains per wave (rows 0..CPW-1 of
+// the walk tile), all rounds in one launch.  The chains' log q is evaluated
+// once at entry (the network or the states may have changed since the last
+// launch) and afterwards taken from the accepted proposal.  Per round: walk
+// a proposal s' in the chain's sector; log|psi(s')| from scratch, every lane
+// taking hidden units h = sub, sub + 64/CPW, ... of chain lane % CPW (W^T
+// from LDS when it fits), summed over the chain's lanes by a butterfly;
+//   d = 2(log|psi(s')| - cached log|psi(s)|) + log q(s) - log q(s');
+// accept when d >= 0 or log u < d; record the chain as sample r*C + c.
+template <int NT, typename R, bool CPLX, int CPW>
+__global__ __launch_bounds__(64) void k_ar_mh(ArDev v, ModelView<R> m, ChainState cs, ArRounds ar) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char dyn_lds[];
+  __shared__ float s_w1t[ar::kMaxN * ar::H];
+  constexpr int LPC = 64 / CPW;
+  WalkRegs<NT> W;
+  walk_load<NT>(v, s_w1t, W);
+  const int N = v.N, M = m.M, lane = threadIdx.x, rl = lane & 15, q = lane >> 4;
+  const int sub = lane / CPW, own = lane % CPW;
+  const R* Wr = m.WT_re;
+  const R* Wi = m.WT_im;
+  if (ar.w_lds) {
+    R* sr = (R*)dyn_lds;
+    R* si = sr + (size_t)N * M;
+    for (int e = lane; e < N * M; e += 64) {
+      sr[e] = m.WT_re[e];
+      if (CPLX) si[e] = m.WT_im[e];
+    }
+    Wr = sr;
+    Wi = si;
   }
-}
-
-// rows first + c (c < C, first + c < n_out) of the sample buffer
-__global__ void k_unpack_rows(ChainState cs, long first, long n_out, int* out) {
-  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (e >= (long)cs.C * cs.N) return;
-  const int c = (int)(e / cs.N), i = (int)(e % cs.N);
-  if (first + c >= n_out) return;
-  const uint64_t wd = cs.spins[(size_t)c * cs.nw + (i >> 6)];
-  out[(first + c) * cs.N + i] = ((wd >> (i & 63)) & 1ULL) ? -1 : 1;
+  const int c = blockIdx.x * CPW + rl;
+  const bool live = rl < CPW && c < cs.C;
+  uint64_t bits = live ? cs.spins[c] : 0ULL;
+  uint64_t rng = live ? cs.rng[c] : 0ULL;
+  double la = live ? cs.la[c] : 0.0;
+  const int target = __popcll(bits & site_mask(N));
+  __syncthreads();
+  uint64_t ob, none = 0;
+  double lq_cur;
+  walk_tile<NT>(W, s_w1t, N, false, ar.constrained, bits, target, none, ob, lq_cur);
+  unsigned long long nacc = 0;
+  for (int r = 0; r < ar.rounds; r++) {
+    uint64_t prop;
+    double lq_p;
+    walk_tile<NT>(W, s_w1t, N, true, ar.constrained, bits, target, rng, prop, lq_p);
+    const uint64_t pp = __shfl((unsigned long long)prop, own, 64);
+    double lp = 0.0;
+    for (int h = sub; h < M; h += LPC) {
+      R xr = m.b_re[h], xi = CPLX ? m.b_im[h] : (R)0;
+#pragma unroll 8
+      for (int i = 0; i < N; i++) {
+        const R sg = ((pp >> i) & 1ULL) ? (R)-1 : (R)1;
+        xr += Wr[(size_t)i * M + h] * sg;
+        if (CPLX) xi += Wi[(size_t)i * M + h] * sg;
+      }
+      // log|2 cosh(x + iy)| = |x| + log((1 - e)^2 + 4 e cos^2 y) / 2 with
+      // e = exp(-2|x|): both terms are non-negative, no cancellation
+      const double ax = fabs((double)xr), e = exp(-2.0 * ax);
+      if (CPLX) {
+        const double cy = cos((double)xi);
+        lp += ax + 0.5 * log((1.0 - e) * (1.0 - e) + 4.0 * e * cy * cy);
+      } else {
+        lp += ax + log1p(e);
+      }
+    }
+    for (int i = sub; i < N; i += LPC) lp += (double)m.a_re[i] * (((pp >> i) & 1ULL) ? -1.0 : 1.0);
+#pragma unroll
+    for (int msk = CPW; msk < 64; msk <<= 1) lp += __shfl_xor(lp, msk, 64);
+    // row lanes (rl < CPW) hold their own chain's sum: lane % CPW == rl
+    const double d = 2.0 * (lp - la) + (lq_cur - lq_p);
+    const bool acc = d >= 0.0 ? true : (log(xs_uniform(rng)) < d);
+    if (acc) {
+      bits = prop;
+      la = lp;
+      lq_cur = lq_p;
+      nacc++;
+    }
+    if (live && ar.out) {
+      const long b = (long)r * cs.C + c;
+      if (b < ar.n_out)
+        for (int i = q; i < N; i += 4) ar.out[b * N + i] = ((bits >> i) & 1ULL) ? -1 : 1;
+    }
+  }
+  if (live && q == 0) {
+    cs.spins[c] = bits;
+    cs.rng[c] = rng;
+    cs.la[c] = la;
+    cs.acc[c] += nacc;
+    cs.tot[c] += (unsigned long long)ar.rounds;
+    ar.lacc[c] += nacc;
+  }
 }
 
 __global__ void k_bits_append(const int* __restrict__ spins, long n, int N, uint64_t* win,
@@ -327,15 +452,15 @@ __global__ __launch_bounds__(64) void k_ar_grad(ArDev v, ArGrad g) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const int f = 16 * t + 4 * q + r;
-          sA1[f][r32] = ar::f2bf(relu(z1[t][r]));
-          sA2[f][r32] = ar::f2bf(relu(z2[t][r]));
-          sG1[f][r32] = ar::f2bf(g1[t][r]);
-          sG2[f][r32] = ar::f2bf(g2[t][r]);
+          sA1[f][r32] = to_bf(relu(z1[t][r]));
+          sA2[f][r32] = to_bf(relu(z2[t][r]));
+          sG1[f][r32] = to_bf(g1[t][r]);
+          sG2[f][r32] = to_bf(g2[t][r]);
         }
 #pragma unroll
       for (int t = 0; t < 4; t++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) sG3[16 * t + 4 * q + r][r32] = t < NT ? ar::f2bf(g3[t < NT ? t : 0][r]) : (uint16_t)0;
+        for (int r = 0; r < 4; r++) sG3[16 * t + 4 * q + r][r32] = t < NT ? to_bf(g3[t < NT ? t : 0][r]) : (uint16_t)0;
     }
     __syncthreads();
     // weight gradients: sums over the 32 staged rows (K = rows 8q + e)
@@ -470,24 +595,46 @@ int launch_ar_walk(const ArDev& v, const ArWalk& w, hipStream_t st) {
   return 0;
 }
 
-int launch_ar_accept(ChainState& cs, const uint64_t* prop, const double* la_prop, const double* lq,
-                     unsigned long long* lacc, hipStream_t st) {
-  if (cs.nw != 1) return -1;
-  hipLaunchKernelGGL(k_ar_accept, dim3((cs.C + 255) / 256), dim3(256), 0, st, cs, prop, la_prop, lq,
-                     lacc);
-  count_launch();
-  NQS_HIP_CHECK(hipGetLastError());
-  return 0;
+template <typename R, int CPW>
+static void ar_mh_go(int nt, const ArDev& v, const ModelView<R>& m, ChainState& cs,
+                     const ArRounds& ar, size_t lds, hipStream_t st) {
+  const dim3 grid((cs.C + CPW - 1) / CPW), blk(64);
+#define NQS_AR_MH(NTV)                                                                         \
+  if (m.cplx) hipLaunchKernelGGL((k_ar_mh<NTV, R, true, CPW>), grid, blk, lds, st, v, m, cs, ar); \
+  else hipLaunchKernelGGL((k_ar_mh<NTV, R, false, CPW>), grid, blk, lds, st, v, m, cs, ar);
+  switch (nt) {
+    case 1: NQS_AR_MH(1) break;
+    case 2: NQS_AR_MH(2) break;
+    case 3: NQS_AR_MH(3) break;
+    default: NQS_AR_MH(4) break;
+  }
+#undef NQS_AR_MH
 }
 
-int launch_unpack_rows(const ChainState& cs, long first, long n_out, int* out, hipStream_t st) {
-  const long total = (long)cs.C * cs.N;
-  hipLaunchKernelGGL(k_unpack_rows, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, cs,
-                     first, n_out, out);
+// chains per wave: enough waves for the 256 CUs (16 when C >= 4096, 8 when
+// C >= 2048, else 4); W^T in LDS when it takes at most 48 KB
+template <typename R>
+int launch_ar_mh(const ArDev& v, const ModelView<R>& m, ChainState& cs, const ArRounds& ar_in,
+                 hipStream_t st) {
+  if (cs.C <= 0 || ar_in.rounds <= 0) return 0;
+  if (cs.nw != 1 || v.N != cs.N || m.N != cs.N) return -1;
+  ArRounds ar = ar_in;
+  const size_t wbytes = (size_t)m.N * m.M * sizeof(R) * (m.cplx ? 2 : 1);
+  ar.w_lds = wbytes <= 48 * 1024 ? 1 : 0;
+  const size_t lds = ar.w_lds ? wbytes : 0;
+  const int nt = (v.N + 15) / 16;
+  KScope ks("ar_mh", st);
+  if (cs.C >= 4096) ar_mh_go<R, 16>(nt, v, m, cs, ar, lds, st);
+  else if (cs.C >= 2048) ar_mh_go<R, 8>(nt, v, m, cs, ar, lds, st);
+  else ar_mh_go<R, 4>(nt, v, m, cs, ar, lds, st);
   count_launch();
   NQS_HIP_CHECK(hipGetLastError());
   return 0;
 }
+template int launch_ar_mh<double>(const ArDev&, const ModelView<double>&, ChainState&, const ArRounds&,
+                                  hipStream_t);
+template int launch_ar_mh<float>(const ArDev&, const ModelView<float>&, ChainState&, const ArRounds&,
+                                 hipStream_t);
 
 int launch_bits_append(const int* spins, long n, int N, uint64_t* win, long cap, long pos,
                        hipStream_t st) {

@@ -138,13 +138,20 @@ int launch_ar_grad(const ArDev& v, const ArGrad& g, hipStream_t st);
 int launch_ar_adam(const ArDev& v, const ArAdam& a, hipStream_t st);
 int launch_ar_pack(const ArDev& v, hipStream_t st);
 int launch_ar_sum_grad(const ArDev& v, const float* partial, int nblk, float* out, hipStream_t st);
-// one global move per chain (nw == 1): proposals prop[C], log|psi| of the
-// proposals la_prop[C], lq[2C] = (log q of proposals, log q of current);
-// lacc[c] counts accepted learned moves.
-int launch_ar_accept(ChainState& cs, const uint64_t* prop, const double* la_prop, const double* lq,
-                     unsigned long long* lacc, hipStream_t st);
-// sample rows first + c of every chain c (rows < n_out) as int32 +-1
-int launch_unpack_rows(const ChainState& cs, long first, long n_out, int* out, hipStream_t st);
+// Learned Metropolis-Hastings rounds (nw == 1): rounds global moves per
+// chain; chain c's state after round r is written as sample r*C + c of out
+// when that index is below n_out (out may be null).  lacc[c] counts the
+// chain's accepted learned moves.
+struct ArRounds {
+  int rounds = 0, constrained = 1;
+  int w_lds = 0;                 // set by the launcher: W^T staged in LDS
+  long n_out = 0;
+  int* out = nullptr;
+  unsigned long long* lacc = nullptr;
+};
+template <typename R>
+int launch_ar_mh(const ArDev& v, const ModelView<R>& m, ChainState& cs, const ArRounds& ar,
+                 hipStream_t st);
 // int32 rows -> bitmasks (N <= 64): ring positions (pos + b) mod cap
 int launch_bits_append(const int* spins, long n, int N, uint64_t* win, long cap, long pos,
                        hipStream_t st);

@@ -47,10 +47,11 @@ def test_num_params(orc):
 def test_masks_leave_free_weights(orc):
     N = 10
     w = _params(orc, N)
-    # W1 row k has exactly m(k) = 1 + k mod (N-1) free inputs
+    # W1 row k has exactly m(k) = 1 + floor(k (N-1) / 64) free inputs, and
+    # the degrees never decrease with k
     W1 = w[:64 * N].reshape(64, N)
     for k in range(64):
-        m = 1 + k % (N - 1)
+        m = 1 + (k * (N - 1)) // 64
         assert np.all(W1[k, m:] == 0) and np.count_nonzero(W1[k, :m]) == m
     # the first output has no hidden input: its weights are all zero
     W3 = w[64 * N + 64 + 64 * 64 + 64:64 * N + 64 + 64 * 64 + 64 + N * 64].reshape(N, 64)
