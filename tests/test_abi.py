@@ -104,3 +104,29 @@ def test_no_gpu_fails_loudly(lib, capfd):
     cfg = abi.config_defaults(ansatz=5)
     assert not lib.nqs_ansatz_create(ctypes.byref(cfg), 4)
     assert "no HIP device" in capfd.readouterr().err
+
+
+REF_INC = "/root/reference/include"
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF_INC, "nqs")),
+                    reason="reference checkout not present (layout pinned by the ctypes mirror)")
+def test_layout_matches_reference_headers(tmp_path):
+    """sizeof / offsetof of nqs_config_t and the info / Marshall structs,
+    compiled by gcc once against our include/ and once against the
+    reference's include/ (read in place, never copied)."""
+    fields = [f for f, _ in abi.NqsConfig._fields_]
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "nqs/nqs_config.h"',
+           '#include "nqs/nqs_optimizer.h"', '#include "nqs/nqs_marshall.h"', 'int main(void) {',
+           'printf("%zu\\n", sizeof(nqs_config_t));']
+    src += [f'printf("%zu\\n", offsetof(nqs_config_t, {f}));' for f in fields]
+    src += ['printf("%zu\\n", sizeof(nqs_sr_step_info_t));',
+            'printf("%zu\\n", sizeof(nqs_marshall_wrapper_t));', 'return 0; }']
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    outs = []
+    for inc in (INC, REF_INC):
+        exe = tmp_path / ("ours" if inc == INC else "ref")
+        subprocess.run(["gcc", "-std=c11", "-I", inc, str(c), "-o", str(exe)], check=True)
+        outs.append(subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout)
+    assert outs[0] == outs[1]
