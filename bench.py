@@ -11,10 +11,18 @@ alpha = 4 (M = 144, Re/Im ~ N(0, 0.01)), 1024 chains x 8 samples = 8192
 samples per iteration, exchange moves with one sweep (N proposals) between
 samples, MinSR with shift 1e-4.  Prints one JSON line.
 
-value = effective samples per second through whole SR iterations:
-sum over chains of n_c / (2 tau_int) with tau_int from the pooled per-chain
-autocorrelation of E_loc (Sokal window, c = 5), divided by the wall time of
-the timed iterations.  ms_per_step is the SR-iteration wall time.
+value = effective samples per second of the sampling phase: sum over chains
+of n_c / (2 tau_int) with tau_int from the pooled per-chain autocorrelation
+of E_loc (Sokal window, c = 5), divided by the sampling-stage time of the
+timed iterations (HIP events on the library's stream).  ess_per_s_iteration
+divides by the whole SR-iteration wall time instead; ms_per_step is the
+SR-iteration wall time.
+
+    --move exchange|learned|flip   override the workload's move kind
+    --converge K                   after the timed steps, run K more SR
+                                   iterations and report the energy trace's
+                                   tail (last 50) and its error vs exact
+                                   diagonalisation where it is known
 """
 import argparse
 import ctypes
@@ -85,14 +93,24 @@ def setup(lib, w, args):
     if w["marshall"]:
         keep = abi.MarshallWrapper(fn, a, w["Lx"], w["Ly"])
         fn, user = abi.fnptr(lib, "nqs_marshall_log_amp"), ctypes.addressof(keep)
-    opt = abi.options_defaults(num_chains=w["chains"], move=w["move"], sz_zero_start=w["sz0"],
+    move = {"flip": abi.MOVE_FLIP, "exchange": abi.MOVE_EXCHANGE,
+            "learned": abi.MOVE_LEARNED}.get(args.move, w["move"])
+    opt = abi.options_defaults(num_chains=w["chains"], move=move, sz_zero_start=w["sz0"],
                                periodic=w["periodic"],
                                precision=abi.FP32 if args.precision == "fp32" else abi.FP64,
-                               gram=abi.GRAM_FACTORED if args.gram == "factored" else abi.GRAM_DIRECT)
+                               gram={"direct": abi.GRAM_DIRECT, "factored": abi.GRAM_FACTORED,
+                                     "auto": abi.GRAM_AUTO}[args.gram])
     s = lib.nqs_hip_sampler_create_ex(N, ctypes.byref(cfg), ctypes.byref(opt), fn, user)
     if not s:
         raise RuntimeError("nqs_hip_sampler_create_ex failed")
-    return dict(cfg=cfg, a=a, s=s, fn=fn, user=user, keep=keep, N=N, P=P, opt=opt)
+    # which MinSR form the engine uses (the rule of NQS_HIP_GRAM_AUTO)
+    B = w["chains"] * w["per_chain"]
+    factored = args.gram in ("factored", "auto") and w["kind"] in (5, 6)
+    kdim = (2 * w["M"] if w["kind"] == 6 else w["M"])
+    gram_b = B * B * (N + kdim) if factored else B * B * P
+    dual = args.gram == "auto" and B * P * P + P ** 3 / 3 < gram_b + B ** 3 / 3
+    return dict(cfg=cfg, a=a, s=s, fn=fn, user=user, keep=keep, N=N, P=P, opt=opt,
+                factored=factored and not dual, dual=dual)
 
 
 def step(lib, w, st):
@@ -107,10 +125,11 @@ def step(lib, w, st):
 
 
 # kernel symbol (as rocprofv3 names it) behind each timing tag
-SYMBOL = {"gemm_potrf": "nqs::k_gemm_nt<1>", "gemm": "nqs::k_gemm_nt<0>", "trsm": "nqs::k_gemm_nt<2>",
-          "potrf_diag": "nqs::k_potrf_panel", "eloc": "nqs::k_eloc<double, true, 1>",
-          "gen_O": "nqs::k_gen_O<double>", "colsum": "nqs::k_colsum_partial",
-          "mh": "nqs::k_mh<double, true, 64, 4>", "trsv": "nqs::k_trsv_fwd"}
+SYMBOL = {"gemm_potrf": "nqs::k_gemm_big<1>", "gemm": "nqs::k_gemm_big<0>",
+          "potrf_panel": "nqs::k_panel",
+          "eloc": "nqs::k_eloc<double, true, 1>", "gen_O": "nqs::k_gen_O<double>",
+          "colsum": "nqs::k_colsum_partial", "mh": "nqs::k_mh<double, true, 64, 4>",
+          "trsv": "nqs::k_trsv_fwd"}
 
 
 def traffic_bytes(workload, tag):
@@ -125,23 +144,50 @@ def traffic_bytes(workload, tag):
     return None if v is None else v["fetch"] + v["write"]
 
 
-TAGS = ["mh", "forward", "eloc", "gen_O", "colsum", "gemm", "gemm_potrf", "potrf_diag", "trsm", "trsv", "gram_f"]
+TAGS = ["mh", "ar_mh", "forward", "eloc", "gen_O", "colsum", "gemm", "gemm_potrf", "potrf_panel",
+        "trsv", "gram_f"]
 
 
-def kernel_work(tag, w, st, launches):
-    """Algorithmic work of one launch of `tag`: (bound, amount, unit)."""
+def kernel_work(tag, w, st, launches, iters):
+    """Algorithmic work of one launch of `tag`: (bound, amount, unit).
+    `launches` were counted over `iters` SR iterations."""
     B = w["chains"] * w["per_chain"]
     P, N, M = st["P"], st["N"], w["M"]
+    per_iter = max(launches / max(iters, 1), 1.0)
     if tag == "gemm":
+        kdim = 2 * M if w["kind"] == 6 else M
+        if st["dual"]:
+            # parameter-space MinSR: lower triangle of S = O_c^T O_c, P(P+1)/2
+            # dot products of length B
+            return "mfma", P * (P + 1) * B / per_iter, "FLOP"
+        if st["factored"]:
+            # factored Gram: lower triangle of U U^T, K = M (2M complex)
+            return "mfma", B * (B + 1) * kdim / per_iter, "FLOP"
         # lower triangle of T = O_c O_c^T: B(B+1)/2 dot products of length P
-        return "mfma", B * (B + 1) * P, "FLOP"
+        return "mfma", B * (B + 1) * P / per_iter, "FLOP"
     if tag == "gemm_potrf":
-        nb, tot, k0 = 64, 0, 0
-        while k0 < B:
-            r = B - min(k0 + nb, B)
-            tot += r * (r + 1) * min(nb, B - k0)
-            k0 += nb
-        return "mfma", tot / max(launches, 1), "FLOP"
+        # rank-128 updates of the blocked Cholesky: the next outer panel's
+        # columns (lower part) and the rest of the trailing lower triangle
+        tot, k0 = 0, 0
+        while True:
+            wd = min(128, B - k0)
+            r0 = k0 + wd
+            if r0 >= B:
+                break
+            w1 = min(128, B - r0)
+            nxt = w1 * (w1 + 1) // 2 + (B - r0 - w1) * w1
+            m = B - r0 - w1
+            tot += 2 * wd * (nxt + m * (m + 1) // 2)
+            k0 += 128
+        return "mfma", tot / per_iter, "FLOP"
+    if tag == "potrf_panel":
+        # per inner panel: factor + inverse of the 64x64 block, and the rows
+        # below solved by a 64-wide product (and the preceding panel's
+        # rank-64 update for every second panel), averaged over the panels
+        npan = (B + 63) // 64
+        tot = sum(2 * 64 ** 3 / 3 + 2 * (B - 64 * (k + 1)) * 64 * 64 * (2 if k % 2 else 1)
+                  for k in range(npan))
+        return "mfma", tot / npan, "FLOP"
     if tag == "gen_O":
         return "hbm", B * P * 8 + B * N * 4 + B * M * 8 * (2 if w["kind"] == 6 else 1), "B"
     if tag == "colsum":
@@ -193,14 +239,46 @@ def gpu_run(args, w):
     tau = pooled_tau(series)
     n_total = series.shape[0] * series.shape[1]
     ess = n_total / (2.0 * tau)
+    sample_s = sum(sm[0] for sm in stages) * 1e-3 if len(stages) == args.steps else None
+    learned = None
+    if args.move == "learned":
+        acc, props, ntr, nll = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_int(), ctypes.c_double()
+        lib.nqs_hip_sampler_learned_stats(st["s"], ctypes.byref(acc), ctypes.byref(props),
+                                          ctypes.byref(ntr), ctypes.byref(nll))
+        learned = dict(acceptance=acc.value, proposals=props.value, trainings=ntr.value,
+                       last_nll=nll.value)
+    conv = None
+    if args.converge > 0:
+        trace = list(energies)
+        tc = time.perf_counter()
+        for i in range(args.converge):
+            info = step(lib, w, st)
+            trace.append(info.mean_energy)
+            if (i + 1) % 50 == 0:
+                log(f"[bench] converge {i + 1}/{args.converge}: E = {info.mean_energy:.6f}")
+        tail = np.array(trace[-50:])
+        nbin = 5
+        bins = tail[: len(tail) // nbin * nbin].reshape(nbin, -1).mean(axis=1)
+        conv = dict(iterations=len(trace), wall_s=time.perf_counter() - tc,
+                    tail_mean=float(tail.mean()), tail_err=float(bins.std(ddof=1) / np.sqrt(nbin)),
+                    trace_every_10=[float(x) for x in trace[::10]])
+        if w["ed"] is not None:
+            conv["e_ed"] = w["ed"]
+            conv["delta_e_rel"] = (conv["tail_mean"] - w["ed"]) / abs(w["ed"])
+        if args.move == "learned":
+            acc = ctypes.c_double()
+            lib.nqs_hip_sampler_learned_stats(st["s"], ctypes.byref(acc), None, None, None)
+            conv["learned_acceptance"] = acc.value
     avg_ms = ms.value / max(c.value, 1)
-    bound, work, unit = kernel_work(dom, w, st, c.value)
+    bound, work, unit = kernel_work(dom, w, st, c.value, args.steps)
     if bound == "mfma":
         achieved, peak, runit = work / (avg_ms * 1e-3) / 1e12, PEAK_FP64 / 1e12, "TFLOP/s"
     else:
         achieved, peak, runit = work / (avg_ms * 1e-3) / 1e9, PEAK_HBM / 1e9, "GB/s"
     res = dict(
-        ess_per_s=ess / elapsed, ms_per_step=1e3 * elapsed / args.steps, tau=tau,
+        ess_per_s=ess / sample_s if sample_s else ess / elapsed, ess_per_s_iteration=ess / elapsed,
+        sample_ms=1e3 * sample_s / args.steps if sample_s else None, learned=learned,
+        convergence=conv, ms_per_step=1e3 * elapsed / args.steps, tau=tau,
         energies=energies, acceptance=infos[-1]["acceptance_ratio"],
         stage_ms=np.mean(stages, axis=0).tolist() if stages else None,
         roofline=dict(kernel=dom, symbol=SYMBOL.get(dom), bound=bound, achieved=achieved, peak=peak,
@@ -283,14 +361,17 @@ def cpu_baseline(w, gpu):
     t_iter = B_full * per_sample + macs_full / rate
     f("sampler_free")(s)
     f("ansatz_free")(a)
-    ess_cpu = (B_full / (2.0 * gpu["tau"])) / t_iter
+    # the same definitions as the GPU line: effective samples per second of
+    # the sampling phase (value) and of the whole iteration
+    ess_cpu = (B_full / (2.0 * gpu["tau"])) / (B_full * t_mh)
+    ess_cpu_iter = (B_full / (2.0 * gpu["tau"])) / t_iter
     sample = (f"{kind} CPU path, 1 core: {nb} samples of MH ({N} moves each), E_loc (open twin, "
               f"scaled to the periodic bond count) and O rows, plus one full MinSR step at "
               f"Ns={ns} for the linear-algebra MAC rate ({rate / 1e9:.2f} GMAC/s); the "
               f"{B_full}-sample iteration ({t_iter:.1f} s) is extrapolated from exact "
               f"operation counts with the GPU run's tau_int")
     return dict(value=ess_cpu, unit="effective samples/s", cores=1, kind=kind, sample=sample,
-                sr_iter_s=t_iter)
+                sr_iter_s=t_iter, ess_per_s_iteration=ess_cpu_iter)
 
 
 def main():
@@ -300,10 +381,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--gram", default="factored", choices=["direct", "factored"])
+    ap.add_argument("--gram", default="auto", choices=["direct", "factored", "auto"])
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the HBM copy probe on MIB mebibytes (counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--move", default=None, choices=["flip", "exchange", "learned"])
+    ap.add_argument("--converge", type=int, default=0, metavar="K")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of "
@@ -336,8 +419,11 @@ def main():
         "data": "synthetic (random-init parameters N(0,0.01), chains started at random Sz=0 states)",
         "config": {"workload": args.workload + ": " + w["desc"], "N": gpu["N"], "P": gpu["P"],
                    "samples_per_iter": w["chains"] * w["per_chain"], "chains": w["chains"],
-                   "method": w["method"], "gram": args.gram, "precision": args.precision},
+                   "method": w["method"], "gram": args.gram, "precision": args.precision,
+                   "move": args.move or {0: "flip", 1: "exchange", 2: "learned"}[w["move"]]},
         "sr_iter_ms": gpu["ms_per_step"],
+        "sample_ms": gpu["sample_ms"],
+        "ess_per_s_iteration": gpu["ess_per_s_iteration"],
         "tau_int": gpu["tau"],
         "acceptance": gpu["acceptance"],
         "energy_last": gpu["energies"][-1],
@@ -348,8 +434,10 @@ def main():
                                                      "traffic", "kernel", "symbol", "avg_ms")},
         "cpu_baseline": cpu,
     }
-    if w["ed"] is not None:
-        out["delta_e_vs_ed"] = (gpu["energies"][-1] - w["ed"]) / abs(w["ed"])
+    if gpu["learned"]:
+        out["learned"] = gpu["learned"]
+    if gpu["convergence"]:
+        out["convergence"] = gpu["convergence"]
     if cpu:
         out["speedup_vs_cpu_iter"] = cpu["sr_iter_s"] / (gpu["ms_per_step"] * 1e-3)
     print(json.dumps(out))

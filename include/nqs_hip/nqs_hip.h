@@ -46,7 +46,13 @@ typedef enum {
 
 typedef enum {
     NQS_HIP_GRAM_DIRECT   = 0, /* T = O_c O_c^T over the materialised O       */
-    NQS_HIP_GRAM_FACTORED = 1  /* RBM / complex RBM: T from (S S^T, Th Th^T)  */
+    NQS_HIP_GRAM_FACTORED = 1, /* RBM / complex RBM: T from (S S^T, Th Th^T)  */
+    NQS_HIP_GRAM_AUTO     = 2  /* the cheaper of the sample-space solve
+                                  (factored Gram where the ansatz allows it)
+                                  and the parameter-space solve
+                                  (O_c^T O_c / Ns + eps I) delta = O_c^T eps / Ns,
+                                  which gives the same delta (push-through
+                                  identity), by operation count              */
 } nqs_hip_gram_t;
 
 typedef struct {

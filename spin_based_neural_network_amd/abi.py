@@ -29,7 +29,7 @@ HAM_TFIM, HAM_HEIS, HAM_J1J2, HAM_KITAEV, HAM_XXZ, HAM_KH, HAM_KAGOME = range(7)
 # nqs_hip_move_t / nqs_hip_precision_t / nqs_hip_gram_t
 MOVE_FLIP, MOVE_EXCHANGE, MOVE_LEARNED = 0, 1, 2
 FP64, FP32 = 0, 1
-GRAM_DIRECT, GRAM_FACTORED = 0, 1
+GRAM_DIRECT, GRAM_FACTORED, GRAM_AUTO = 0, 1, 2
 
 
 class NqsConfig(Structure):

@@ -1484,15 +1484,65 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
     // eps_i = E_i - <E>, O_c = O - <O>  (nqs_optimizer.c:1119-1126)
     CKR(nqs::launch_sub_scalar(u, E, B, sc + 0, g_stream));
     CKR(nqs::launch_center(O, B, P, Om, g_stream));
-    CKR(g_ws.T.ensure(sizeof(double) * (size_t)B * B));
     CKR(g_ws.diag.ensure(sizeof(double) * B));
     CKR(g_ws.y.ensure(sizeof(double) * B * 2));
     CKR(g_ws.fail.ensure(sizeof(int) * 4));
-    double* T = g_ws.T.as<double>();
-    // T = O_c O_c^T / Ns + eps I (lower triangle, FP64 MFMA): directly over
-    // O_c, or from the factored form of RBM-type rows
-    const bool factored = s->opt.gram == NQS_HIP_GRAM_FACTORED && !gfn;
-    if (factored) {
+    double* T = nullptr;
+    // sample space (T = O_c O_c^T / Ns + eps I, Ns x Ns) or, with
+    // NQS_HIP_GRAM_AUTO when it costs less, parameter space
+    // (S = O_c^T O_c / Ns + eps I, P x P): O_c^T (O_c O_c^T/Ns + eps)^-1 =
+    // (O_c^T O_c/Ns + eps)^-1 O_c^T, so both give the same delta
+    const bool rbm_like = (a->kind == NQS_ANSATZ_RBM || a->kind == NQS_ANSATZ_COMPLEX_RBM) && !gfn;
+    const bool factored = (s->opt.gram == NQS_HIP_GRAM_FACTORED ||
+                           s->opt.gram == NQS_HIP_GRAM_AUTO) && rbm_like;
+    bool dual = false;
+    if (s->opt.gram == NQS_HIP_GRAM_AUTO) {
+      const double b = (double)B, p = (double)P;
+      const double gram_b = factored ? b * b * (N + 2.0 * a->M) : b * b * p;
+      const double primal = gram_b + b * b * b / 3.0;
+      const double dualc = b * p * p + p * p * p / 3.0;
+      dual = dualc < primal;
+    }
+    if (dual) {
+      // parameter-space MinSR: delta = S^-1 (O_c^T eps / Ns)
+      CKR(g_ws.T.ensure(sizeof(double) * (size_t)P * P));
+      CKR(g_ws.T2.ensure(sizeof(double) * (size_t)P * B));
+      T = g_ws.T.as<double>();
+      double* OT = g_ws.T2.as<double>();
+      double* rhs = c1;
+      CKR(nqs::launch_transpose(O, B, P, OT, g_stream));
+      CKR(nqs::launch_gemm_nt((int)P, (int)P, B, 1.0 / B, OT, B, OT, B, 0.0, T, P, 1, g_stream));
+      CKR(nqs::launch_add_diag(T, (int)P, P, eps, g_stream));
+      CKR(nqs::launch_colsum(O, B, P, u, 1.0 / B, rhs, work, g_stream));
+      CKR(g_ws.linv.ensure(sizeof(double) * nqs::potrf_linv_elems((int)P)));
+      CKR(g_ws.y.ensure(sizeof(double) * 2 * (size_t)P));
+      double* Linv = g_ws.linv.as<double>();
+      int f = nqs::launch_potrf(T, (int)P, Linv, g_ws.fail.as<int>(), g_stream);
+      if (f < 0) return -1;
+      if (f > 0) {
+        // as in the sample-space solve: the partially factored matrix with
+        // +1e-6 on its diagonal is factored once more (input columns from a
+        // second evaluation of S)
+        CKR(g_ws.Oi.ensure(sizeof(double) * (size_t)P * P));
+        double* S2 = g_ws.Oi.as<double>();
+        CKR(nqs::launch_gemm_nt((int)P, (int)P, B, 1.0 / B, OT, B, OT, B, 0.0, S2, P, 1, g_stream));
+        CKR(nqs::launch_add_diag(S2, (int)P, P, eps, g_stream));
+        CKR(nqs::launch_potrf_repair(T, (int)P, f - 1, S2, g_stream));
+        CKR(nqs::launch_add_diag(T, (int)P, P, 1e-6, g_stream));
+        f = nqs::launch_potrf(T, (int)P, Linv, g_ws.fail.as<int>(), g_stream);
+        if (f < 0) return -1;
+      }
+      if (f == 0) {
+        CKR(nqs::launch_potrs(T, Linv, (int)P, rhs, delta, g_ws.y.as<double>(), g_stream));
+        converged = 1;
+      } else {
+        fprintf(stderr, "nqs_sr_step_minsr: Cholesky breakdown after retry; increase "
+                        "sr_diag_shift or num_samples. Update set to 0.\n");
+        CKR(nqs::launch_fill(delta, 0.0, P, g_stream));
+      }
+    } else if (factored) {
+      CKR(g_ws.T.ensure(sizeof(double) * (size_t)B * B));
+      T = g_ws.T.as<double>();
       if (prec == NQS_HIP_FP32) CKR(forward_t<double>(a, d_samples, B, nullptr, nullptr));
       CKR(g_ws.T2.ensure(sizeof(double) * (size_t)B * B));
       CKR(g_ws.Sd.ensure(sizeof(double) * (size_t)B * N));
@@ -1502,9 +1552,12 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
                                     g_ws.T2.as<double>(), g_ws.Sd.as<double>(),
                                     g_ws.u2.as<double>(), sc + 6, g_stream));
     } else {
+      CKR(g_ws.T.ensure(sizeof(double) * (size_t)B * B));
+      T = g_ws.T.as<double>();
       CKR(nqs::launch_gemm_nt(B, B, (int)P, 1.0 / B, O, P, O, P, 0.0, T, B, 1, g_stream));
       CKR(nqs::launch_add_diag(T, B, B, eps, g_stream));
     }
+    if (!dual) {
     CKR(g_ws.linv.ensure(sizeof(double) * nqs::potrf_linv_elems(B)));
     double* Linv = g_ws.linv.as<double>();
     int f = nqs::launch_potrf(T, B, Linv, g_ws.fail.as<int>(), g_stream);
@@ -1533,6 +1586,7 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
     }
     // delta_k = (1/Ns) sum_i O_c[i][k] y_i
     CKR(nqs::launch_colsum(O, B, P, y, 1.0 / B, delta, work, g_stream));
+    }
   } else {
     double* u1 = g_ws.u1.as<double>();
     double* u2 = g_ws.u2.as<double>();

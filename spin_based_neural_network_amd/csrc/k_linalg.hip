@@ -369,7 +369,7 @@ static int gemm_nt_impl(int M, int N, int K, double alpha, const double* A, long
   if (lower && M != N) return -1;
   if (mask < 0) mask = lower;
   const int use = !skip ? 0 : (mask ? 1 : 2);
-  KScope ks(use == 0 ? "gemm" : (use == 1 ? "gemm_potrf" : "gemm_trsm"), st);
+  KScope ks(use == 0 ? "gemm" : (use == 1 ? "gemm_inner" : "gemm_trsm"), st);
   if (use == 0)
     hipLaunchKernelGGL(k_gemm_nt<0>, dim3((unsigned)tiles), dim3(256), 0, st, M, N, K, alpha, A,
                        lda, B, ldb, beta, C, ldc, lower, mask, tn, skip);
@@ -384,100 +384,338 @@ static int gemm_nt_impl(int M, int N, int K, double alpha, const double* A, long
   return 0;
 }
 
+
+
+// --------------------------------------------------------------------------
+// Large-tile NT GEMM for the Cholesky trailing updates and the direct Gram:
+// workgroup tile 128x128, 4 waves of 64x64 (4x4 MFMA blocks), K in chunks of
+// 32 double-buffered through LDS (global loads of chunk c+1 in flight while
+// chunk c feeds the MFMAs).  Each load instruction of a wave reads two rows'
+// 32 consecutive doubles (coalesced).  Same lower / mask / skip conventions
+// as k_gemm_nt.
+// --------------------------------------------------------------------------
+constexpr int BT = 128, BKC = 32;
+
+template <int USE>
+__global__ __launch_bounds__(512) void k_gemm_big(int M, int N, int K, double alpha,
+                                                  const double* A, long lda,
+                                                  const double* __restrict__ Bm, long ldb,
+                                                  double beta, double* C, long ldc, int lower,
+                                                  int mask, int tiles_n, const int* skip) {
+  // 8 waves, 2 per SIMD: wave (wm, wn) computes rows wm*64 + [0, 64) and
+  // columns wn*32 + [0, 32) of the tile as 4x2 MFMA blocks; the operands of
+  // step kk + 4 are read from LDS while the MFMAs of step kk issue.
+  if (skip && *skip) return;
+  __shared__ double As[2][BT][BKC + 1];
+  __shared__ double Bs[2][BT][BKC + 1];
+  int tm, tn;
+  if (lower) lower_tile(blockIdx.x, tm, tn);
+  else { tm = blockIdx.x / tiles_n; tn = blockIdx.x % tiles_n; }
+  const int row0 = tm * BT, col0 = tn * BT;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wr = (wave >> 2) * 64, wc = (wave & 3) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  v4d acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+  double ra[8], rb[8];
+  const int nch = (K + BKC - 1) / BKC;
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int e = tid + 512 * q, r = e >> 5, kk = e & 31, gk = k0 + kk;
+      const int ga = row0 + r, gb = col0 + r;
+      ra[q] = (ga < M && gk < K) ? A[(long)ga * lda + gk] : 0.0;
+      rb[q] = (gb < N && gk < K) ? Bm[(long)gb * ldb + gk] : 0.0;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int e = tid + 512 * q, r = e >> 5, kk = e & 31;
+      As[buf][r][kk] = ra[q];
+      Bs[buf][r][kk] = rb[q];
+    }
+  };
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int c = 0; c < nch; c++) {
+    const int buf = c & 1;
+    if (c + 1 < nch) gload((c + 1) * BKC);
+    double a[2][4], b[2][2];
+#pragma unroll
+    for (int i = 0; i < 4; i++) a[0][i] = As[buf][wr + 16 * i + lr][lk];
+#pragma unroll
+    for (int j = 0; j < 2; j++) b[0][j] = Bs[buf][wc + 16 * j + lr][lk];
+#pragma unroll
+    for (int s = 0; s < BKC / 4; s++) {
+      const int cur = s & 1;
+      if (s + 1 < BKC / 4) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) a[cur ^ 1][i] = As[buf][wr + 16 * i + lr][4 * (s + 1) + lk];
+#pragma unroll
+        for (int j = 0; j < 2; j++) b[cur ^ 1][j] = Bs[buf][wc + 16 * j + lr][4 * (s + 1) + lk];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[cur][i], b[cur][j], acc[i][j], 0, 0, 0);
+    }
+    if (c + 1 < nch) sstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gr = row0 + wr + 16 * i + lk + 4 * r;
+        const int gc = col0 + wc + 16 * j + lr;
+        if (gr < M && gc < N && (!mask || gc <= gr)) {
+          double* cp = C + (long)gr * ldc + gc;
+          *cp = (beta == 0.0) ? alpha * acc[i][j][r] : alpha * acc[i][j][r] + beta * (*cp);
+        }
+      }
+}
+
+static int gemm_big_impl(int M, int N, int K, double alpha, const double* A, long lda,
+                         const double* B, long ldb, double beta, double* C, long ldc, int lower,
+                         int mask, const int* skip, const char* tag, hipStream_t st) {
+  if (M <= 0 || N <= 0) return 0;
+  const int tm = (M + BT - 1) / BT, tn = (N + BT - 1) / BT;
+  if (lower && M != N) return -1;
+  const long tiles = lower ? (long)tm * (tm + 1) / 2 : (long)tm * tn;
+  KScope ks(tag, st);
+  if (skip)
+    hipLaunchKernelGGL(k_gemm_big<1>, dim3((unsigned)tiles), dim3(512), 0, st, M, N, K, alpha, A,
+                       lda, B, ldb, beta, C, ldc, lower, mask, tn, skip);
+  else
+    hipLaunchKernelGGL(k_gemm_big<0>, dim3((unsigned)tiles), dim3(512), 0, st, M, N, K, alpha, A,
+                       lda, B, ldb, beta, C, ldc, lower, mask, tn, skip);
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
 int launch_gemm_nt(int M, int N, int K, double alpha, const double* A, long lda, const double* B,
                    long ldb, double beta, double* C, long ldc, int lower, hipStream_t st) {
+  if (M >= 2 * BT && N >= 2 * BT && K >= 64)
+    return gemm_big_impl(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower, lower, nullptr, "gemm",
+                         st);
   return gemm_nt_impl(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower, nullptr, st);
 }
 
 // --------------------------------------------------------------------------
-// Cholesky, right-looking with 64-wide panels.  The panel kernel factors the
-// diagonal block in LDS (right-looking, one column per step) and also
-// inverts it; the rows below are then A21 <- A21 Linv^T on the MFMA GEMM and
-// the trailing matrix is updated on its lower triangle by the same GEMM.
-// The inverted diagonal blocks are kept for the triangular solves.
+// Cholesky, right-looking.  Outer panels of 128 columns, each made of two
+// 64-column inner panels; one launch (k_panel) per inner panel factors the
+// diagonal block, inverts it, and solves every row below it (the second
+// inner panel first takes the first's rank-64 update inside the same
+// launch).  The trailing matrix takes one rank-128 update per outer panel on
+// the large-tile MFMA GEMM.  With lookahead, the next outer panel's columns
+// are updated first and that panel is factored on a second stream while the
+// rest of the trailing matrix is updated.  The inverses of the 64x64
+// diagonal blocks are kept for the triangular solves.
 // --------------------------------------------------------------------------
-constexpr int NB = 64;
+constexpr int NB = 64, NBO = 128;
 
-size_t potrf_linv_elems(int n) { return (size_t)((n + NB - 1) / NB) * NB * NB; }
+// Linv workspace: the inverses of the diagonal blocks, then the blocks' L
+// (written by the panel kernel and copied into A once every panel has read
+// its input)
+size_t potrf_linv_elems(int n) { return 2 * (size_t)((n + NB - 1) / NB) * NB * NB; }
 
-// value of a double held by lane `lane` (a compile-time constant after
-// unrolling), read into scalar registers: a wave-uniform broadcast with no
-// LDS traffic
-__device__ inline double lane_value(double v, int lane) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
-}
-
-__global__ __launch_bounds__(64) void k_potrf_panel(double* A, int n, int k0, int nb,
-                                                    double* Linv, int* fail) {
-  // One wave.  The block is staged through LDS with coalesced row loads;
-  // lane i then keeps row i in registers.  Right-looking, one column per
-  // step: L[c][j] for the rank-1 update is read from lane c's registers
-  // with v_readlane (uniform), so the whole factorisation and inversion run
-  // out of registers.  The update is applied to every c > j without
-  // predication: entries c > i of a lane's row are never read back.
+// One inner panel, columns [k0, k0 + nb), in one launch.  Workgroup b
+// (4 waves) owns rows rowbase = r0 + 64 b + [0, 64) below the diagonal block
+// (r0 = k0 + nb); workgroup 0 also writes the block's L and inverse.
+// kp >= 0: columns [kp, kp + 64) left of the panel are final L whose rank-64
+// update the panel has not yet received; it is applied here.
+//   1. D = A_kk - Lp_d Lp_d^T and R = A_rows - Lp_r Lp_d^T   (MFMA)
+//   2. D = L L^T: wave q keeps columns 16q..16q+15 of row `lane` in
+//      registers; per column one barrier: the owning wave publishes the
+//      column through LDS (double-buffered), every wave reads the pivot and
+//      applies the rank-1 update to its columns
+//   3. Dinv = L^-1 (wave 0, lane c solves L x = e_c)
+//   4. rows = R Dinv^T                                           (MFMA)
+// A non-positive pivot at column j: the factored columns < j of the block
+// and the updated, unsolved rows are stored, and k0 + j + 1 is reported.
+__global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb, int kp,
+                                               double* Linv, int* fail) {
   if (*fail) return;
-  __shared__ double Ls[NB][NB + 1];
-  const int i = threadIdx.x;
-  {
-    // all 64 coalesced row loads in flight before the first wait: lane i
-    // holds column i, transposed through LDS
-    double v[NB];
-#pragma unroll
-    for (int t = 0; t < NB; t++)
-      v[t] = (t < nb && i < nb && i <= t) ? A[(long)(k0 + t) * n + k0 + i] : (t == i ? 1.0 : 0.0);
-#pragma unroll
-    for (int t = 0; t < NB; t++) Ls[t][i] = v[t];
+  __shared__ double D[NB][NB + 1];
+  __shared__ double X[NB][NB + 1];     // Lp_d in step 1, L^-1 from step 3
+  __shared__ double Rs[NB][NB + 1];
+  __shared__ double col[2][NB];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int r0 = k0 + nb;
+  const long rowbase = (long)r0 + (long)blockIdx.x * NB;
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    D[r][c] = (r < nb) ? ((c <= r) ? A[(long)(k0 + r) * n + k0 + c] : 0.0) : (r == c ? 1.0 : 0.0);
+    X[r][c] = (kp >= 0 && r < nb) ? A[(long)(k0 + r) * n + kp + c] : 0.0;
   }
   __syncthreads();
-  double r[NB];
+  // step 1: wave w computes row block w of the rows and of D
+  {
+    v4d accR[4], accD[4];
 #pragma unroll
-  for (int c = 0; c < NB; c++) r[c] = Ls[i][c];
+    for (int bj = 0; bj < 4; bj++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const long row = rowbase + 16 * wave + lk + 4 * r;
+        const int cc = 16 * bj + lr;
+        accR[bj][r] = (row < n && cc < nb) ? A[row * n + k0 + cc] : 0.0;
+      }
+      accD[bj] = (v4d){0.0, 0.0, 0.0, 0.0};
+    }
+    if (kp >= 0) {
+      const long arow = rowbase + 16 * wave + lr;
+#pragma unroll 4
+      for (int kk = 0; kk < NB; kk += 4) {
+        const double ar = arow < n ? -A[arow * n + kp + kk + lk] : 0.0;
+        const double ad = -X[16 * wave + lr][kk + lk];
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++) {
+          const double b = X[16 * bj + lr][kk + lk];
+          accR[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b, accR[bj], 0, 0, 0);
+          accD[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, b, accD[bj], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * wave + lk + 4 * r, c = 16 * bj + lr;
+        Rs[i][c] = accR[bj][r];
+        if (kp >= 0 && c <= i && i < nb) D[i][c] += accD[bj][r];
+      }
+  }
+  __syncthreads();
+  // step 2: factor D
+  double rr[16];
+#pragma unroll
+  for (int t = 0; t < 16; t++) rr[t] = D[lane][16 * wave + t];
   int jdone = nb;
 #pragma unroll
   for (int j = 0; j < NB; j++) {
     if (j < nb && jdone == nb) {
-      const double d = lane_value(r[j], j);
+      const int jb = j >> 4, jj = j & 15;
+      if (wave == jb) col[j & 1][lane] = rr[jj];
+      __syncthreads();
+      const double d = col[j & 1][j];
       if (!(d > 0.0)) {
         jdone = j;
       } else {
-        const double Ljj = sqrt(d);
-        const double inv = 1.0 / Ljj;
-        r[j] = (i == j) ? Ljj : r[j] * inv;
-        const double lij = r[j];
+        // only 1/d is on the column-to-column chain: the update is
+        // a_ic -= (a_ij / d) a_cj; L_ij = a_ij / sqrt(d) is formed after it
+        const double ci = col[j & 1][lane];
+        const double rd = 1.0 / d;
+        const double si = (lane > j) ? ci * rd : 0.0;
+        if (16 * wave + 15 > j) {
 #pragma unroll
-        for (int c = j + 1; c < NB; c++) r[c] -= lij * lane_value(r[j], c);
+          for (int t = 0; t < 16; t++) {
+            const int c = 16 * wave + t;
+            const double lc = (c > j) ? col[j & 1][c] : 0.0;
+            rr[t] -= si * lc;
+          }
+        }
+        if (wave == jb) {
+          const double Ljj = sqrt(d);
+          rr[jj] = (lane == j) ? Ljj : ((lane > j) ? ci / Ljj : rr[jj]);
+        }
       }
     }
   }
-  __syncthreads();
 #pragma unroll
-  for (int c = 0; c < NB; c++) Ls[i][c] = r[c];
+  for (int t = 0; t < 16; t++) D[lane][16 * wave + t] = rr[t];
   __syncthreads();
-#pragma unroll
-  for (int t = 0; t < NB; t++)
-    if (t < nb && i <= t && i < jdone) A[(long)(k0 + t) * n + k0 + i] = Ls[t][i];
+  // the block's L goes to the workspace (other workgroups may still be
+  // reading the block from A); k_diag_copy moves it into A at the end
+  double* Lg = Linv + ((size_t)((n + NB - 1) / NB) + k0 / NB) * NB * NB;
   if (jdone < nb) {
-    if (i == 0) *fail = k0 + jdone + 1;
+    if (blockIdx.x == 0) {
+      for (int e = tid; e < NB * NB; e += 256) Lg[e] = D[e >> 6][e & 63];
+      if (tid == 0) *fail = k0 + jdone + 1;
+    }
+    for (int e = tid; e < NB * NB; e += 256) {
+      const int r = e >> 6, c = e & 63;
+      const long row = rowbase + r;
+      if (row < n && c < nb) A[row * n + k0 + c] = Rs[r][c];
+    }
     return;
   }
-  // inverse of the diagonal block, lane c solving L x = e_c:
-  // x_q = (delta_qc - sum_{k<q} L_qk x_k) / L_qq, with L_qk from lane q
-  // (x_q = 0 for q < c follows).  Rows q >= nb hold the identity.
-  const int c = i;
-  double x[NB];
+  // step 3: inverse of L (wave 0, reciprocal pivots precomputed, each dot
+  // product split over 8 accumulators); identity rows beyond nb give identity
+  if (tid < NB) col[0][tid] = 1.0 / D[tid][tid];
+  __syncthreads();
+  if (wave == 0) {
+    double x[NB];
 #pragma unroll
-  for (int q = 0; q < NB; q++) {
-    const double dq = 1.0 / lane_value(r[q], q);
-    double sum = (q == c) ? -1.0 : 0.0;
+    for (int q = 0; q < NB; q++) {
+      double p[8];
 #pragma unroll
-    for (int k = 0; k < q; k++) sum += lane_value(r[k], q) * x[k];
-    x[q] = -sum * dq;
+      for (int u = 0; u < 8; u++) p[u] = 0.0;
+#pragma unroll
+      for (int k = 0; k < q; k++) p[k & 7] += D[q][k] * x[k];
+      const double sum = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7])) +
+                         ((q == lane) ? -1.0 : 0.0);
+      x[q] = -sum * col[0][q];
+    }
+#pragma unroll
+    for (int q = 0; q < NB; q++) X[q][lane] = (q >= lane) ? x[q] : 0.0;
   }
-  double* X = Linv + (size_t)(k0 / NB) * NB * NB;
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    double* Xg = Linv + (size_t)(k0 / NB) * NB * NB;
+    for (int e = tid; e < NB * NB; e += 256) {
+      const int r = e >> 6, c = e & 63;
+      Lg[e] = D[r][c];
+      Xg[e] = (r < nb && c < nb) ? X[r][c] : 0.0;
+    }
+  }
+  // step 4: rows = R Dinv^T
+  v4d acc[4];
 #pragma unroll
-  for (int q = 0; q < NB; q++) X[q * NB + c] = (q < nb && c < nb && q >= c) ? x[q] : 0.0;
+  for (int bj = 0; bj < 4; bj++) acc[bj] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int kk = 0; kk < NB; kk += 4) {
+    const double a = Rs[16 * wave + lr][kk + lk];
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+      acc[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, X[16 * bj + lr][kk + lk], acc[bj], 0, 0, 0);
+  }
+#pragma unroll
+  for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const long row = rowbase + 16 * wave + lk + 4 * r;
+      const int cc = 16 * bj + lr;
+      if (row < n && cc < nb) A[row * n + k0 + cc] = acc[bj][r];
+    }
+}
+
+// Diagonal blocks' L from the workspace into A (lower triangles): every
+// block before a failure, and the factored columns of the failing block.
+__global__ __launch_bounds__(256) void k_diag_copy(double* A, int n, const double* Linv,
+                                                   const int* fail) {
+  const int k0 = blockIdx.x * NB, nb = (n - k0) < NB ? (n - k0) : NB;
+  const int f = *fail;
+  int lim = nb;
+  if (f) {
+    const int jf = f - 1;
+    if (k0 > jf) return;
+    if (jf < k0 + nb) lim = jf - k0;
+  }
+  const double* Lg = Linv + ((size_t)((n + NB - 1) / NB) + blockIdx.x) * NB * NB;
+  for (int e = threadIdx.x; e < NB * NB; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    if (r < nb && c <= r && c < lim) A[(long)(k0 + r) * n + k0 + c] = Lg[e];
+  }
 }
 
 // rows i >= r0: A[i][k0 + j] for j < ncols solved against the diagonal block
@@ -527,31 +765,30 @@ __global__ void k_zero_int(int* p) { *p = 0; }
 static hipStream_t g_look = nullptr;
 static std::vector<hipEvent_t> g_ev;
 
-static int panel_and_trsm(double* A, int n, int k0, double* Linv, int* fail_dev, hipStream_t st) {
+// inner panel at k0 (kp: the preceding inner panel whose update is due)
+static int inner_panel(double* A, int n, int k0, int kp, double* Linv, int* fail_dev,
+                       hipStream_t st) {
   const int nb = (n - k0) < NB ? (n - k0) : NB;
-  {
-    KScope ks("potrf_diag", st);
-    hipLaunchKernelGGL(k_potrf_panel, dim3(1), dim3(64), 0, st, A, n, k0, nb, Linv, fail_dev);
-  }
+  const int rows = n - k0 - nb;
+  const int grid = rows > 0 ? (rows + NB - 1) / NB : 1;
+  KScope ks("potrf_panel", st);
+  hipLaunchKernelGGL(k_panel, dim3((unsigned)grid), dim3(256), 0, st, A, n, k0, nb, kp, Linv,
+                     fail_dev);
   count_launch();
-  const int r0 = k0 + nb;
-  if (r0 < n) {
-    double* panel = A + (long)r0 * n + k0;
-    // A21 <- A21 Linv^T (in place: each workgroup reads its rows before writing)
-    KScope ks("trsm", st);
-    if (gemm_nt_impl(n - r0, nb, nb, 1.0, panel, n, Linv + (size_t)(k0 / NB) * NB * NB, NB, 0.0,
-                     panel, n, 0, fail_dev, st))
-      return -1;
-  }
+  NQS_HIP_CHECK(hipGetLastError());
   return 0;
 }
 
-// Right-looking blocked Cholesky with lookahead one: after panel k, the
-// next panel's column block is updated first and factored on a second
-// stream while the rest of the trailing matrix is updated on the main one.
+// outer panel at K0: its two inner panels
+static int outer_panel(double* A, int n, int K0, double* Linv, int* fail_dev, hipStream_t st) {
+  if (inner_panel(A, n, K0, -1, Linv, fail_dev, st)) return -1;
+  if (n - K0 > NB && inner_panel(A, n, K0 + NB, K0, Linv, fail_dev, st)) return -1;
+  return 0;
+}
+
 int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) {
   if (!g_look) NQS_HIP_CHECK(hipStreamCreateWithFlags(&g_look, hipStreamNonBlocking));
-  const int np = (n + NB - 1) / NB;
+  const int np = (n + NBO - 1) / NBO;
   while ((int)g_ev.size() < 2 * np) {
     hipEvent_t e;
     NQS_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -559,7 +796,7 @@ int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) 
   }
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, fail_dev);
   count_launch();
-  if (panel_and_trsm(A, n, 0, Linv, fail_dev, st)) return -1;
+  if (outer_panel(A, n, 0, Linv, fail_dev, st)) return -1;
   // NQS_HIP_POTRF_LOOKAHEAD=0: the same schedule on one stream
   static const int lookahead = [] {
     const char* e = getenv("NQS_HIP_POTRF_LOOKAHEAD");
@@ -567,29 +804,33 @@ int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) 
   }();
   hipStream_t side = lookahead ? g_look : st;
   for (int k = 0; k < np; k++) {
-    const int k0 = k * NB;
-    const int nb = (n - k0) < NB ? (n - k0) : NB;
-    const int r0 = k0 + nb;
-    if (r0 >= n) break;
-    const int nb1 = (n - r0) < NB ? (n - r0) : NB;
-    const double* L21 = A + (long)r0 * n + k0;
-    // next panel's columns [r0, r0 + nb1), rows >= r0 (lower part only)
-    if (gemm_nt_impl(n - r0, nb1, nb, -1.0, L21, n, L21, n, 1.0, A + (long)r0 * n + r0, n, 0,
-                     fail_dev, st, 1))
+    const int K0 = k * NBO;
+    const int w = (n - K0) < NBO ? (n - K0) : NBO;
+    const int R0 = K0 + w;
+    if (R0 >= n) break;
+    const int w1 = (n - R0) < NBO ? (n - R0) : NBO;
+    const double* L = A + (long)R0 * n + K0;
+    // next outer panel's columns [R0, R0 + w1), rows >= R0 (lower part)
+    if (gemm_big_impl(n - R0, w1, w, -1.0, L, n, L, n, 1.0, A + (long)R0 * n + R0, n, 0, 1,
+                      fail_dev, "gemm_potrf", st))
       return -1;
     NQS_HIP_CHECK(hipEventRecord(g_ev[2 * k], st));
     NQS_HIP_CHECK(hipStreamWaitEvent(side, g_ev[2 * k], 0));
-    if (panel_and_trsm(A, n, r0, Linv, fail_dev, side)) return -1;
+    if (outer_panel(A, n, R0, Linv, fail_dev, side)) return -1;
     NQS_HIP_CHECK(hipEventRecord(g_ev[2 * k + 1], side));
-    // the rest of the trailing matrix: rows and columns >= r0 + nb1
-    const int r1 = r0 + nb1;
-    if (r1 < n) {
-      if (gemm_nt_impl(n - r1, n - r1, nb, -1.0, A + (long)r1 * n + k0, n, A + (long)r1 * n + k0, n,
-                       1.0, A + (long)r1 * n + r1, n, 1, fail_dev, st))
+    // the rest of the trailing matrix: rows and columns >= R0 + w1
+    const int R1 = R0 + w1;
+    if (R1 < n) {
+      const double* L1 = A + (long)R1 * n + K0;
+      if (gemm_big_impl(n - R1, n - R1, w, -1.0, L1, n, L1, n, 1.0, A + (long)R1 * n + R1, n, 1, 1,
+                        fail_dev, "gemm_potrf", st))
         return -1;
     }
     NQS_HIP_CHECK(hipStreamWaitEvent(st, g_ev[2 * k + 1], 0));
   }
+  hipLaunchKernelGGL(k_diag_copy, dim3((unsigned)((n + NB - 1) / NB)), dim3(256), 0, st, A, n, Linv,
+                     fail_dev);
+  count_launch();
   NQS_HIP_CHECK(hipGetLastError());
   int fail = 0;
   NQS_HIP_CHECK(hipMemcpyAsync(&fail, fail_dev, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -831,6 +1072,32 @@ int launch_copy_probe(const double* src, double* dst, long n, hipStream_t st) {
   const long n4 = n / 4;
   KScope ks("copy_probe", st);
   hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, st, (const double4*)src, (double4*)dst, n4);
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// out[c][r] = in[r][c] for an R x C row-major matrix (64 x 64 tiles through LDS)
+__global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ in, long R, long C,
+                                                   double* __restrict__ out) {
+  __shared__ double t[64][65];
+  const long r0 = (long)blockIdx.y * 64, c0 = (long)blockIdx.x * 64;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    t[r][c] = (r0 + r < R && c0 + c < C) ? in[(r0 + r) * C + c0 + c] : 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int c = e >> 6, r = e & 63;
+    if (c0 + c < C && r0 + r < R) out[(c0 + c) * R + r0 + r] = t[r][c];
+  }
+}
+
+int launch_transpose(const double* in, long R, long C, double* out, hipStream_t st) {
+  if (R <= 0 || C <= 0) return 0;
+  if ((R + 63) / 64 > 65535) return -1;
+  hipLaunchKernelGGL(k_transpose, dim3((unsigned)((C + 63) / 64), (unsigned)((R + 63) / 64)),
+                     dim3(256), 0, st, in, R, C, out);
   count_launch();
   NQS_HIP_CHECK(hipGetLastError());
   return 0;

@@ -208,6 +208,8 @@ int launch_gram_factored(const int* spins, int B, int N, const double* U1, const
                          double eps, double* T, double* UU, double* Sd, double* r, double* g,
                          hipStream_t st);
 int launch_add_diag(double* A, int n, long ld, double v, hipStream_t st);
+// out [C][R] = in [R][C]^T
+int launch_transpose(const double* in, long R, long C, double* out, hipStream_t st);
 int launch_copy_diag(const double* A, int n, double* d, hipStream_t st);
 int launch_mirror_lower(double* A, int n, hipStream_t st);
 

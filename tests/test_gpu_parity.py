@@ -413,3 +413,41 @@ def test_factored_gram_equals_direct(hip, kind, ham, L, M, marsh):
     step = np.abs(np.diff(res[0][0], axis=0)).max()
     assert np.abs(res[0][0] - res[1][0]).max() <= 1e-9 * step + 1e-13
     _close(res[0][1], res[1][1], 1e-10)
+
+
+@pytest.mark.parametrize("kind,ham,L,M", [(5, 0, (8, 1), 8), (6, 6, (2, 2), 6), (6, 1, (4, 2), 8)])
+def test_parameter_space_minsr_equals_sample_space(hip, kind, ham, L, M):
+    """With more samples than parameters, NQS_HIP_GRAM_AUTO solves MinSR in
+    parameter space, (O_c^T O_c / Ns + eps I) delta = O_c^T eps / Ns; by the
+    push-through identity this is the sample-space update
+    O_c^T (O_c O_c^T / Ns + eps I)^-1 eps / Ns, checked on the same samples
+    within the conditioning of the two solves."""
+    Lx, Ly = L
+    N = 3 * Lx * Ly if ham == 6 else Lx * Ly
+    cfg = config_defaults(ansatz=kind, hamiltonian=ham, rbm_hidden_units=M, rbm_init_scale=0.3,
+                          num_samples=1024, num_thermalize=64, num_decorrelate=2, sr_diag_shift=1e-3,
+                          learning_rate=0.05)
+    res = {}
+    for gram in (1, 2):
+        opt = options_defaults(num_chains=64, move=1 if ham in (1, 6) else 0,
+                               sz_zero_start=1 if ham in (1, 6) else 0, periodic=1, gram=gram)
+        r = scenarios.Runner(hip, "nqs_")
+        a = r.ansatz(cfg, N)
+        P = hip.nqs_ansatz_num_params(a)
+        assert P < 1024
+        fn, user = r.callback(a, None)
+        s = hip.nqs_hip_sampler_create_ex(N, ctypes.byref(cfg), ctypes.byref(opt), fn, user)
+        hip.nqs_sampler_thermalize(s)
+        p0 = r.params(a)
+        info = SrInfo()
+        assert hip.nqs_sr_step_minsr_full(ctypes.byref(cfg), Lx, Ly, a, s, fn, user, None, None,
+                                          ctypes.byref(info)) == 0
+        assert info.converged == 1
+        res[gram] = (p0, r.params(a), info.mean_energy)
+        hip.nqs_sampler_free(s)
+        hip.nqs_ansatz_free(a)
+    d1 = res[1][1] - res[1][0]
+    d2 = res[2][1] - res[2][0]
+    assert np.array_equal(res[1][0], res[2][0])
+    assert res[1][2] == res[2][2]
+    assert np.linalg.norm(d1 - d2) <= 1e-7 * np.linalg.norm(d1)
