@@ -68,6 +68,14 @@ WORKLOADS = {
                ham=6, Lx=2, Ly=4, kind=6, M=96, chains=2048, per_chain=8, move=1, sz0=1,
                periodic=1, marshall=False, method="minsr", shift=1e-4, lr=1e-2,
                ed=-10.759897248084),
+    # the same system optimised as the reference's kagome runs do (holomorphic
+    # SR, lr 0.02, shift 1e-3: scripts/research_kagome_N12_convergence.c:68-69);
+    # the real-projected MinSR of c5 cannot move the phases of psi
+    "c5h": dict(desc="kagome Heisenberg 2x4 cells PBC (N=24), Sz=0; complex RBM M=96; 2048 "
+                     "chains x 8 = 16384/iter; exchange moves; holomorphic CG-SR lr 0.02 shift 1e-3",
+                ham=6, Lx=2, Ly=4, kind=6, M=96, chains=2048, per_chain=8, move=1, sz0=1,
+                periodic=1, marshall=False, method="holo", shift=1e-3, lr=2e-2,
+                ed=-10.759897248084),
 }
 
 
@@ -106,8 +114,7 @@ def setup(lib, w, args):
     # which MinSR form the engine uses (the rule of NQS_HIP_GRAM_AUTO)
     B = w["chains"] * w["per_chain"]
     factored = args.gram in ("factored", "auto") and w["kind"] in (5, 6)
-    kdim = (2 * w["M"] if w["kind"] == 6 else w["M"])
-    gram_b = B * B * (N + kdim) if factored else B * B * P
+    gram_b = B * B * (N + 2 * w["M"]) if factored else B * B * P
     dual = args.gram == "auto" and B * P * P + P ** 3 / 3 < gram_b + B ** 3 / 3
     return dict(cfg=cfg, a=a, s=s, fn=fn, user=user, keep=keep, N=N, P=P, opt=opt,
                 factored=factored and not dual, dual=dual)
@@ -256,10 +263,26 @@ def gpu_run(args, w):
             trace.append(info.mean_energy)
             if (i + 1) % 50 == 0:
                 log(f"[bench] converge {i + 1}/{args.converge}: E = {info.mean_energy:.6f}")
+        # effective samples per second at the optimised psi: the sampler's
+        # mixing matters most once |psi|^2 has structure
+        ser = np.zeros((C, 0))
+        smp_ms = []
+        for i in range(max(args.steps, 4)):
+            info = step(lib, w, st)
+            trace.append(info.mean_energy)
+            n = lib.nqs_hip_sampler_last_energies(st["s"], abi.ptr(buf), C * R)
+            ser = np.concatenate([ser, buf[: C * n].reshape(C, n)], axis=1)
+            sm = np.zeros(7)
+            lib.nqs_hip_last_stage_ms(st["s"], abi.ptr(sm))
+            smp_ms.append(sm[0])
+        tau_end = pooled_tau(ser)
+        ess_end = ser.size / (2.0 * tau_end) / (sum(smp_ms) * 1e-3)
         tail = np.array(trace[-50:])
         nbin = 5
         bins = tail[: len(tail) // nbin * nbin].reshape(nbin, -1).mean(axis=1)
         conv = dict(iterations=len(trace), wall_s=time.perf_counter() - tc,
+                    tau_int_end=tau_end, ess_per_s_end=ess_end,
+                    sample_ms_end=float(np.mean(smp_ms)),
                     tail_mean=float(tail.mean()), tail_err=float(bins.std(ddof=1) / np.sqrt(nbin)),
                     trace_every_10=[float(x) for x in trace[::10]])
         if w["ed"] is not None:
