@@ -163,7 +163,7 @@ int orc_ar_log_prob(int N, const float *w, const int *spins, int B, int constrai
                 /* the device conditional: FP32 sigmoid of l clamped to
                  * [-16, 16]; probabilities p and 1 - p in FP64 */
                 const float lc = fminf(fmaxf((float)l[j], -16.0f), 16.0f);
-                const double p = (double)(1.0f / (1.0f + expf(-lc)));
+                const double p = (double)(1.0f / (1.0f + expf(-lc)));   /* device: v_exp_f32, v_rcp_f32 */
                 lq += s[j] < 0 ? log(p) : log(1.0 - p);
             } else {
                 const double ls = log_sigmoid(l[j]);

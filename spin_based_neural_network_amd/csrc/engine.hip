@@ -691,14 +691,14 @@ struct nqs_sampler {
   // learned proposal: network, training window (bit rows, ring), proposal
   // buffers and counters
   nqs_hip_ar* ar = nullptr;
-  DBuf win, lacc;
+  DBuf win, lacc, prop_bits, prop_spins, lq, la_prop;
   long win_pos = 0, win_n = 0;
   int since_train = 0, trained = 0, n_train = 0;
   long long ar_props = 0;
   double ar_loss = 0.0;
   ~nqs_sampler() {
     DBuf* all[] = {&d_spins, &d_rng, &d_acc, &d_tot, &d_la, &d_cur, &d_samples,
-                   &win, &lacc};
+                   &win, &lacc, &prop_bits, &prop_spins, &lq, &la_prop};
     for (DBuf* b : all) b->release();
     delete ar;
   }
@@ -920,10 +920,25 @@ int learned_rounds(nqs_sampler* s, int rounds, int n_out, int* d_out) {
   ar.constrained = 1;
   ar.lacc = s->lacc.as<unsigned long long>();
   if (s->opt.ar_local <= 0) {
-    ar.rounds = rounds;
-    ar.n_out = n_out;
-    ar.out = d_out;
-    CKR(nqs::launch_ar_mh<R>(s->ar->dev, mv, s->cs, ar, g_stream));
+    // proposals of every round at once, their amplitudes in one forward
+    // pass, then the per-chain accept / reject rounds
+    const size_t nprop = (size_t)rounds * C;
+    CKR(s->prop_bits.ensure(sizeof(uint64_t) * nprop));
+    CKR(s->prop_spins.ensure(sizeof(int) * nprop * s->N));
+    CKR(s->lq.ensure(sizeof(double) * (nprop + C)));
+    CKR(s->la_prop.ensure(sizeof(double) * nprop));
+    nqs::ArBulk b;
+    b.rounds = rounds;
+    b.prop = s->prop_bits.as<uint64_t>();
+    b.spins = s->prop_spins.as<int>();
+    b.lq = s->lq.as<double>();
+    b.lp = s->la_prop.as<double>();
+    b.n_out = n_out;
+    b.out = d_out;
+    b.lacc = s->lacc.as<unsigned long long>();
+    CKR(nqs::launch_ar_propose(s->ar->dev, s->cs, b, g_stream));
+    CKR(forward_t<R>(s->src.a, b.spins, (int)nprop, b.lp, nullptr));
+    CKR(nqs::launch_ar_accept_rounds(s->cs, b, g_stream));
   } else {
     nqs::MHLaunch ml;
     ml.move = 1;

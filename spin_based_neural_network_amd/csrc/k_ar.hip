@@ -107,7 +107,8 @@ __device__ inline void walk_load(const ArDev& v, float* s_w1t, WalkRegs<NT>& W) 
 // (so 0 < p < 1 in FP32); site j is down when u < p, which happens with
 // probability p, and up with probability 1 - p (FP64), so q is the product
 // of exactly the probabilities the draws use.  The product is kept in FP64
-// with a logarithm taken whenever it falls below 1e-280 and at the end.
+// as a mantissa and a power of two (exact frexp renormalisation), and its
+// logarithm is taken once at the end.
 template <int NT>
 __device__ inline void walk_tile(const WalkRegs<NT>& W, const float* s_w1t, int N, bool samp,
                                  int constrained, uint64_t bits, int target, uint64_t& rng,
@@ -118,55 +119,49 @@ __device__ inline void walk_tile(const WalkRegs<NT>& W, const float* s_w1t, int 
   for (int t = 0; t < 4; t++) z1[t] = W.Z1[t];
   ob = 0;
   lq = 0.0;
-  double pr = 1.0;
+  double pr = 1.0;   // product of the conditionals as pr * 2^pe
+  int pe = 0;
   int downs = 0;
   for (int j = 0; j < N; j++) {
     const bf16x8 a0 = relu_op(z1, 0), a1 = relu_op(z1, 1);
-    // Degrees are non-decreasing in the unit index, so whole MFMA tiles are
-    // masked out: tile mt of z2 enters l_j only if its first unit has degree
-    // <= j, and a1 chunk 1 enters tile mt only if its first unit's degree is
-    // <= the tile's last.  Skipped products are sums of exact zeros.
-    f32x4 z2[4];
+    // all eight products issue back to back (four independent chains) so
+    // their latencies overlap; masked weights contribute exact zeros
+    // the two K chunks of each product are independent MFMAs added on the
+    // VALU, so a layer costs one MFMA latency
+    f32x4 z2[4], z2b[4];
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) {
-      if (ar::degree(16 * mt, N) <= j) {
-        f32x4 acc = mfma(W.A2[mt][0], a0, W.B2[mt]);
-        if (ar::degree(32, N) <= ar::degree(16 * mt + 15, N)) acc = mfma(W.A2[mt][1], a1, acc);
-        z2[mt] = acc;
-      } else {
-        z2[mt] = zero4();
-      }
+      z2[mt] = mfma(W.A2[mt][0], a0, W.B2[mt]);
+      z2b[mt] = mfma(W.A2[mt][1], a1, zero4());
     }
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) z2[mt] += z2b[mt];
     const bf16x8 c0 = relu_op(z2, 0), c1 = relu_op(z2, 1);
     const int mj = j >> 4;
-    const bool need1 = ar::degree(32, N) <= j;
     f32x4 lg = zero4();
 #pragma unroll
     for (int t = 0; t < NT; t++)
-      if (t == mj) {
-        lg = mfma(W.A3[t][0], c0, W.B3[t]);
-        if (need1) lg = mfma(W.A3[t][1], c1, lg);
-      }
+      if (t == mj) lg = mfma(W.A3[t][0], c0, W.B3[t]) + mfma(W.A3[t][1], c1, zero4());
     const int rj = j & 3, qj = (j >> 2) & 3;
     const float mine = rj == 0 ? lg[0] : rj == 1 ? lg[1] : rj == 2 ? lg[2] : lg[3];
     const float l = __shfl(mine, rl + 16 * qj, 64);
 
     const int rem_d = target - downs, rem = N - j;
     const bool forced = constrained && (rem_d <= 0 || rem_d >= rem);
-    const double u = samp ? xs_uniform(rng) : 0.0;
+    // the 53-bit uniform u = k 2^-53 of the row's stream; p is an FP32
+    // value >= 2^-24, so u < p is the exact integer test k < p 2^53
+    const uint64_t k53 = samp ? (xs_next(rng) >> 11) : 0ULL;
     const float lc = fminf(fmaxf(l, -16.f), 16.f);
-    const double p = (double)(1.f / (1.f + __expf(-lc)));   // p(down)
-    int bit;
-    if (forced) bit = rem_d > 0 ? 1 : 0;
-    else if (samp) bit = u < p ? 1 : 0;
-    else bit = (int)((bits >> j) & 1ULL);
-    if (!forced) {
-      pr *= bit ? p : 1.0 - p;
-      if (pr < 1e-280) {
-        lq += log(pr);
-        pr = 1.0;
-      }
-    }
+    const float pf = __builtin_amdgcn_rcpf(1.f + __expf(-lc));   // p(down), v_rcp_f32
+    const double p = (double)pf;
+    // p 2^53 as an integer from the bits of pf (normal, 2^-24 <= pf < 1)
+    const uint32_t fb = __float_as_uint(pf);
+    const uint64_t thr = (uint64_t)((fb & 0x7FFFFFu) | 0x800000u) << (((fb >> 23) & 0xFF) - 127 - 23 + 53);
+    const int drawn = samp ? (k53 < thr ? 1 : 0) : (int)((bits >> j) & 1ULL);
+    const int bit = forced ? (rem_d > 0 ? 1 : 0) : drawn;
+    int e2;
+    pr = frexp(pr * (forced ? 1.0 : (bit ? p : 1.0 - p)), &e2);
+    pe += e2;
     downs += bit;
     ob |= (uint64_t)bit << j;
     const float sg = bit ? -1.f : 1.f;
@@ -176,7 +171,7 @@ __device__ inline void walk_tile(const WalkRegs<NT>& W, const float* s_w1t, int 
 #pragma unroll
       for (int r = 0; r < 4; r++) z1[t][r] = fmaf(wr[16 * t + r], sg, z1[t][r]);
   }
-  lq += log(pr);
+  lq = log(pr) + (double)pe * 0.69314718055994530942;
 }
 
 template <int NT>
@@ -253,7 +248,11 @@ __global__ __launch_bounds__(64) void k_ar_mh(ArDev v, ModelView<R> m, ChainStat
     double lq_p;
     walk_tile<NT>(W, s_w1t, N, true, ar.constrained, bits, target, rng, prop, lq_p);
     const uint64_t pp = __shfl((unsigned long long)prop, own, 64);
-    double lp = 0.0;
+    // log|2 cosh(x + iy)| = |x| + log((1 - e)^2 + 4 e cos^2 y) / 2 with
+    // e = exp(-2|x|) (both terms non-negative: no cancellation); the factors
+    // of a lane's hidden units are multiplied (each lies in (0, 4]) and one
+    // logarithm is taken per lane, or earlier if the product gets small
+    double lp = 0.0, prod = 1.0;
     for (int h = sub; h < M; h += LPC) {
       R xr = m.b_re[h], xi = CPLX ? m.b_im[h] : (R)0;
 #pragma unroll 8
@@ -262,16 +261,20 @@ __global__ __launch_bounds__(64) void k_ar_mh(ArDev v, ModelView<R> m, ChainStat
         xr += Wr[(size_t)i * M + h] * sg;
         if (CPLX) xi += Wi[(size_t)i * M + h] * sg;
       }
-      // log|2 cosh(x + iy)| = |x| + log((1 - e)^2 + 4 e cos^2 y) / 2 with
-      // e = exp(-2|x|): both terms are non-negative, no cancellation
       const double ax = fabs((double)xr), e = exp(-2.0 * ax);
+      lp += ax;
       if (CPLX) {
         const double cy = cos((double)xi);
-        lp += ax + 0.5 * log((1.0 - e) * (1.0 - e) + 4.0 * e * cy * cy);
+        prod *= (1.0 - e) * (1.0 - e) + 4.0 * e * cy * cy;
       } else {
-        lp += ax + log1p(e);
+        prod *= 1.0 + e;
+      }
+      if (prod < 1e-200) {
+        lp += (CPLX ? 0.5 : 1.0) * log(prod);
+        prod = 1.0;
       }
     }
+    lp += (CPLX ? 0.5 : 1.0) * log(prod);
     for (int i = sub; i < N; i += LPC) lp += (double)m.a_re[i] * (((pp >> i) & 1ULL) ? -1.0 : 1.0);
 #pragma unroll
     for (int msk = CPW; msk < 64; msk <<= 1) lp += __shfl_xor(lp, msk, 64);
@@ -298,6 +301,83 @@ __global__ __launch_bounds__(64) void k_ar_mh(ArDev v, ModelView<R> m, ChainStat
     cs.tot[c] += (unsigned long long)ar.rounds;
     ar.lacc[c] += nacc;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Learned rounds in bulk.  The proposal distribution q does not depend on the
+// chain's state, so the proposals of all R rounds of a batch are drawn at once
+// (k_ar_propose: R*C sampling rows plus C rows evaluating the chains' current
+// states), their amplitudes are evaluated at once, and only the accept /
+// reject decisions run round by round per chain (k_ar_accept_rounds).  This
+// is the same Markov chain as drawing each proposal when its round comes.
+// Row r*C + c draws with the xorshift state splitmix64(rng_c + (r+1) G) (1 if
+// zero), G = 0x9E3779B97F4A7C15, rng_c the chain's state at the batch start.
+template <int NT>
+__global__ __launch_bounds__(64) void k_ar_propose(ArDev v, ChainState cs, ArBulk b) {
+  __shared__ float s_w1t[ar::kMaxN * ar::H];
+  WalkRegs<NT> W;
+  walk_load<NT>(v, s_w1t, W);
+  const int N = v.N, lane = threadIdx.x, rl = lane & 15, q = lane >> 4;
+  const long C = cs.C, nsamp = (long)b.rounds * C;
+  const long row = (long)blockIdx.x * 16 + rl;
+  const bool live = row < nsamp + C;
+  const bool samp = row < nsamp;
+  const int c = live ? (int)(row % C) : 0;
+  const uint64_t bits = live ? cs.spins[c] : 0ULL;
+  const int target = __popcll(bits & site_mask(N));
+  uint64_t rng = 0ULL;
+  if (live && samp) {
+    rng = splitmix64(cs.rng[c] + (uint64_t)(row / C + 1) * 0x9E3779B97F4A7C15ULL);
+    if (!rng) rng = 1ULL;
+  }
+  __syncthreads();
+  uint64_t ob;
+  double lq;
+  walk_tile<NT>(W, s_w1t, N, samp, b.constrained, bits, target, rng, ob, lq);
+  if (!live) return;
+  if (q == 0) b.lq[row] = lq;
+  if (samp) {
+    if (q == 0) b.prop[row] = ob;
+    for (int i = q; i < N; i += 4) b.spins[row * N + i] = ((ob >> i) & 1ULL) ? -1 : 1;
+  }
+}
+
+// One thread per chain: rounds of accept / reject against the pre-drawn
+// proposals (log|psi| in b.lp, log q in b.lq; the chains' own log q at
+// b.lq[R*C + c]); d = 2(log|psi(s')| - cached log|psi(s)|) + log q(s) -
+// log q(s'), accept when d >= 0 or log u < d (u from the chain's xorshift).
+// The chain's state after round r is sample r*C + c of b.out when that index
+// is below b.n_out.
+__global__ __launch_bounds__(256) void k_ar_accept_rounds(ChainState cs, ArBulk b) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cs.C) return;
+  const long C = cs.C;
+  const int N = cs.N;
+  uint64_t bits = cs.spins[c], rng = cs.rng[c];
+  double la = cs.la[c], lqc = b.lq[(long)b.rounds * C + c];
+  unsigned long long nacc = 0;
+  for (int r = 0; r < b.rounds; r++) {
+    const long i = (long)r * C + c;
+    const double d = 2.0 * (b.lp[i] - la) + (lqc - b.lq[i]);
+    // one draw per round whatever d is, so that the chain's stream (which
+    // seeds the next batch's proposals) advances every batch
+    const double u = xs_uniform(rng);
+    const bool acc = d >= 0.0 ? true : (log(u) < d);
+    if (acc) {
+      bits = b.prop[i];
+      la = b.lp[i];
+      lqc = b.lq[i];
+      nacc++;
+    }
+    if (b.out && i < b.n_out)
+      for (int k = 0; k < N; k++) b.out[i * N + k] = ((bits >> k) & 1ULL) ? -1 : 1;
+  }
+  cs.spins[c] = bits;
+  cs.rng[c] = rng;
+  cs.la[c] = la;
+  cs.acc[c] += nacc;
+  cs.tot[c] += (unsigned long long)b.rounds;
+  b.lacc[c] += nacc;
 }
 
 __global__ void k_bits_append(const int* __restrict__ spins, long n, int N, uint64_t* win,
@@ -635,6 +715,33 @@ template int launch_ar_mh<double>(const ArDev&, const ModelView<double>&, ChainS
                                   hipStream_t);
 template int launch_ar_mh<float>(const ArDev&, const ModelView<float>&, ChainState&, const ArRounds&,
                                  hipStream_t);
+
+int launch_ar_propose(const ArDev& v, ChainState& cs, const ArBulk& b, hipStream_t st) {
+  if (cs.C <= 0 || b.rounds <= 0) return 0;
+  if (cs.nw != 1 || v.N != cs.N) return -1;
+  const long rows = ((long)b.rounds + 1) * cs.C;
+  const int nt = (v.N + 15) / 16;
+  const dim3 grid((unsigned)((rows + 15) / 16)), blk(64);
+  KScope ks("ar_walk", st);
+  switch (nt) {
+    case 1: hipLaunchKernelGGL(k_ar_propose<1>, grid, blk, 0, st, v, cs, b); break;
+    case 2: hipLaunchKernelGGL(k_ar_propose<2>, grid, blk, 0, st, v, cs, b); break;
+    case 3: hipLaunchKernelGGL(k_ar_propose<3>, grid, blk, 0, st, v, cs, b); break;
+    default: hipLaunchKernelGGL(k_ar_propose<4>, grid, blk, 0, st, v, cs, b); break;
+  }
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_ar_accept_rounds(ChainState& cs, const ArBulk& b, hipStream_t st) {
+  if (cs.C <= 0 || b.rounds <= 0) return 0;
+  KScope ks("ar_accept", st);
+  hipLaunchKernelGGL(k_ar_accept_rounds, dim3((cs.C + 255) / 256), dim3(256), 0, st, cs, b);
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
 
 int launch_bits_append(const int* spins, long n, int N, uint64_t* win, long cap, long pos,
                        hipStream_t st) {

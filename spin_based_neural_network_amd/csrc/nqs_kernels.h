@@ -152,6 +152,22 @@ struct ArRounds {
 template <typename R>
 int launch_ar_mh(const ArDev& v, const ModelView<R>& m, ChainState& cs, const ArRounds& ar,
                  hipStream_t st);
+// Learned rounds in bulk (independence proposals drawn for all rounds at
+// once; see k_ar.hip): prop [rounds*C], spins [rounds*C][N], lq
+// [(rounds+1)*C] (the last C: the chains' current states), lp [rounds*C]
+// (log|psi| of the proposals, filled between the two launches).
+struct ArBulk {
+  int rounds = 0, constrained = 1;
+  uint64_t* prop = nullptr;
+  int* spins = nullptr;
+  double* lq = nullptr;
+  double* lp = nullptr;
+  long n_out = 0;
+  int* out = nullptr;
+  unsigned long long* lacc = nullptr;
+};
+int launch_ar_propose(const ArDev& v, ChainState& cs, const ArBulk& b, hipStream_t st);
+int launch_ar_accept_rounds(ChainState& cs, const ArBulk& b, hipStream_t st);
 // int32 rows -> bitmasks (N <= 64): ring positions (pos + b) mod cap
 int launch_bits_append(const int* spins, long n, int N, uint64_t* win, long cap, long pos,
                        hipStream_t st);
