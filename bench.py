@@ -76,6 +76,13 @@ WORKLOADS = {
                 ham=6, Lx=2, Ly=4, kind=6, M=96, chains=2048, per_chain=8, move=1, sz0=1,
                 periodic=1, marshall=False, method="holo", shift=1e-3, lr=2e-2,
                 ed=-10.759897248084),
+    # 32 independent TFIM chains of uneven size, one batch (nqs_hip_multi_*)
+    "c4": dict(desc="32 TFIM chains PBC, J=1, N_i in {16,24,32,48,64,96,128} and h_i ~ U[0.5,1.5] "
+                    "(seed 0xC4); RBM alpha=2 each; 256 chains x 8 = 2048 samples per instance; "
+                    "single flips, 1 sweep between samples; CG-SR shift 1e-3 lr 0.01 (100 / 1e-6); "
+                    "all instances in one launch sequence, one hipGraph per iteration",
+               multi=True, n=32, sizes=(16, 24, 32, 48, 64, 96, 128), chains=256, per_chain=8,
+               shift=1e-3, lr=1e-2, seed=0xC4, method="cg", kind=5, ham=0),
 }
 
 
@@ -314,6 +321,212 @@ def gpu_run(args, w):
     return res
 
 
+MULTI_TAGS = ["mh", "forward", "eloc", "multi_prep", "multi_gemm_z", "multi_rows", "multi_gemm_g",
+              "multi_vec"]
+
+
+def multi_instances(w):
+    rng = np.random.default_rng(w["seed"])
+    Ns = rng.choice(np.array(w["sizes"]), w["n"])
+    hs = rng.uniform(0.5, 1.5, w["n"])
+    return [(int(N), float(h)) for N, h in zip(Ns, hs)]
+
+
+def tfim_pbc_closed_form(N, h):
+    k = (2 * np.arange(N) + 1) * np.pi / N
+    return float(-np.sum(np.sqrt(1.0 + h * h - 2 * h * np.cos(k))))
+
+
+def multi_gemm_flops(tag, inst, B, iters):
+    """FLOPs of all launches of a batched GEMM tag over one step: the force
+    product (two weight sets) and one product per CG iteration taken."""
+    tot = 0.0
+    for (N, _), it in zip(inst, iters):
+        M1, N1 = 2 * N + 1, N + 1
+        per = 2.0 * M1 * N1 * B
+        tot += per * (2 + it) if tag == "multi_gemm_g" else per * it
+    return tot
+
+
+def gpu_run_multi(args, w):
+    lib = abi.load_library()
+    inst = multi_instances(w)
+    C, R = w["chains"], w["per_chain"]
+    B = C * R
+    log(f"[bench] workload {args.workload}: {len(inst)} instances, N = "
+        f"{sorted(set(N for N, _ in inst))}, {B} samples each")
+    probs, keep = [], []
+    fn = abi.fnptr(lib, "nqs_ansatz_log_amp")
+    for i, (N, h) in enumerate(inst):
+        cfg = abi.config_defaults(ansatz=w["kind"], hamiltonian=w["ham"], transverse_field=h,
+                                  rbm_hidden_units=2 * N, num_samples=B, num_thermalize=4 * N,
+                                  num_decorrelate=N, rng_seed=1000 + i, learning_rate=w["lr"],
+                                  sr_diag_shift=w["shift"], sr_cg_max_iters=100, sr_cg_tol=1e-6)
+        a = lib.nqs_ansatz_create(ctypes.byref(cfg), N)
+        P = lib.nqs_ansatz_num_params(a)
+        par = np.ctypeslib.as_array(ctypes.cast(lib.nqs_ansatz_params_raw(a),
+                                                ctypes.POINTER(ctypes.c_double)), (P,))
+        par[:] = np.random.default_rng(1 + i).normal(0.0, 0.01, P)
+        opt = abi.options_defaults(num_chains=C, periodic=1)
+        s = lib.nqs_hip_sampler_create_ex(N, ctypes.byref(cfg), ctypes.byref(opt), fn, a)
+        if not a or not s:
+            raise RuntimeError("instance setup failed")
+        probs.append(dict(cfg=cfg, a=a, s=s, N=N, h=h, P=P))
+    n = len(probs)
+    cfgs = (abi.NqsConfig * n)(*[p["cfg"] for p in probs])
+    lx = (ctypes.c_int * n)(*[p["N"] for p in probs])
+    ly = (ctypes.c_int * n)(*([1] * n))
+    av = (ctypes.c_void_p * n)(*[p["a"] for p in probs])
+    sv = (ctypes.c_void_p * n)(*[p["s"] for p in probs])
+    m = lib.nqs_hip_multi_create(n, cfgs, lx, ly, av, sv)
+    if not m:
+        raise RuntimeError("nqs_hip_multi_create failed")
+    for p in probs:
+        lib.nqs_sampler_thermalize(p["s"])
+    infos = (abi.SrInfo * n)()
+    lib.nqs_hip_multi_step(m, infos)          # first step: autotune
+    tg, us = (ctypes.c_int * 8)(), (ctypes.c_double * 8)()
+    chosen = lib.nqs_hip_multi_tuning(m, tg, us, 8)
+    log("[bench] autotune (G tiles x splits target: CG product us): " +
+        ", ".join(f"{tg[i]}: {us[i]:.1f}" for i in range(6)) + f" -> {chosen}")
+    # one step launch by launch for the per-kernel split
+    lib.nqs_hip_multi_use_graph(m, 0)
+    lib.nqs_hip_kernel_timing(",".join(MULTI_TAGS).encode())
+    lib.nqs_hip_multi_step(m, infos)
+    per = {}
+    for t in MULTI_TAGS:
+        c, ms = ctypes.c_longlong(0), ctypes.c_double(0)
+        if lib.nqs_hip_kernel_stats(t.encode(), ctypes.byref(c), ctypes.byref(ms)) == 0 and c.value:
+            per[t] = (c.value, ms.value)
+    lib.nqs_hip_kernel_timing(None)
+    lib.nqs_hip_multi_use_graph(m, 1)
+    iters_prof = [infos[i].cg_iterations for i in range(n)]
+    log("[bench] kernel split of one step (launches, ms): " +
+        ", ".join(f"{t}={c}/{ms:.2f}" for t, (c, ms) in sorted(per.items(), key=lambda kv: -kv[1][1])))
+    for i in range(args.warmup):
+        lib.nqs_hip_multi_step(m, infos)
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    series = [np.zeros((C, 0)) for _ in range(n)]
+    buf = np.zeros(B)
+    t_total = 0.0
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        lib.nqs_hip_multi_step(m, infos)
+        t_total += time.perf_counter() - t0
+        for i in range(n):
+            lib.nqs_hip_multi_last_energies(m, i, abi.ptr(buf), B)
+            series[i] = np.concatenate([series[i], buf.reshape(R, C).T], axis=1)
+    ms_step = 1e3 * t_total / args.steps
+    log(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms per step (all {n} instances)")
+    taus = [pooled_tau(x) for x in series]
+    ess_total = sum(x.size / (2.0 * t) for x, t in zip(series, taus))
+    iters = [infos[i].cg_iterations for i in range(n)]
+    energies = [infos[i].mean_energy for i in range(n)]
+    conv = None
+    if args.converge > 0:
+        trace = np.zeros((n, args.converge))
+        for k in range(args.converge):
+            lib.nqs_hip_multi_step(m, infos)
+            trace[:, k] = [infos[i].mean_energy for i in range(n)]
+            if (k + 1) % 50 == 0:
+                log(f"[bench] converge {k + 1}/{args.converge}")
+        tail = trace[:, -20:].mean(axis=1)
+        e0 = np.array([tfim_pbc_closed_form(N, h) for N, h in inst])
+        rel = (tail - e0) / np.abs(e0)
+        conv = dict(iterations=args.converge, delta_e_rel=[float(x) for x in rel],
+                    delta_e_rel_max=float(rel.max()), delta_e_rel_median=float(np.median(rel)))
+    dom = max(per, key=lambda t: per[t][1])
+    c, tot_ms = per[dom]
+    avg_ms = tot_ms / c
+    if dom in ("multi_gemm_g", "multi_gemm_z"):
+        work = multi_gemm_flops(dom, inst, B, iters_prof) / c
+        achieved, peak, runit, bound = work / (avg_ms * 1e-3) / 1e12, PEAK_FP64 / 1e12, "TFLOP/s", "mfma"
+    else:
+        # bytes the kernel must move: spins in, samples / energies out
+        work = sum(B * N * 4 for N, _ in inst) / c
+        achieved, peak, runit, bound = work / (avg_ms * 1e-3) / 1e9, PEAK_HBM / 1e9, "GB/s", "hbm"
+    lib.nqs_hip_multi_free(m)
+    for p in probs:
+        lib.nqs_sampler_free(p["s"])
+        lib.nqs_ansatz_free(p["a"])
+    return dict(ess_per_s=ess_total / (t_total / args.steps), ms_per_step=ms_step, taus=taus,
+                iters=iters, energies=energies, inst=inst, convergence=conv, chosen=chosen,
+                tuning={int(tg[i]): float(us[i]) for i in range(6)},
+                roofline=dict(kernel=dom, symbol=None, bound=bound, achieved=achieved, peak=peak,
+                              unit=runit, frac=achieved / peak, traffic=None, avg_ms=avg_ms),
+                split={t: dict(launches=cc, ms=mm) for t, (cc, mm) in per.items()})
+
+
+def cpu_baseline_multi(w, gpu):
+    """Reference CPU path (1 core) on a bounded sample of each instance size:
+    per-sample MH / E_loc / O cost on 8 samples per distinct N, and the CG
+    MAC rate from one reference CG-SR step (N=32, Ns=256); the full step of
+    all instances is extrapolated with the GPU run's CG iteration counts."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import load_oracle, load_reference
+    lib, pre, kind = load_reference(), "nqs_", "reference"
+    if lib is None:
+        lib, pre, kind = load_oracle(), "orc_", "port"
+    f = lambda nm: getattr(lib, pre + nm)
+    fn = abi.fnptr(lib, pre + "ansatz_log_amp")
+    B = w["chains"] * w["per_chain"]
+    per_sample = {}
+    nb = 8
+    for N in sorted(set(N for N, _ in gpu["inst"])):
+        cfg = abi.config_defaults(ansatz=w["kind"], hamiltonian=0, rbm_hidden_units=2 * N,
+                                  num_thermalize=0, num_decorrelate=N, rng_seed=7,
+                                  rbm_init_scale=0.06)
+        a = f("ansatz_create")(ctypes.byref(cfg), N)
+        P = f("ansatz_num_params")(a)
+        s = f("sampler_create")(N, ctypes.byref(cfg), fn, a)
+        smp = np.zeros((nb, N), dtype=np.int32)
+        t0 = time.perf_counter()
+        f("sampler_batch")(s, nb, abi.ptr(smp))
+        t_mh = (time.perf_counter() - t0) / nb
+        e = np.zeros(nb)
+        t0 = time.perf_counter()
+        f("local_energy_batch")(ctypes.byref(cfg), N, 1, abi.ptr(smp), nb, fn, a, abi.ptr(e))
+        # open chain: N - 1 bonds and N flips; periodic: N and N
+        t_el = (time.perf_counter() - t0) / nb * (2 * N) / (2 * N - 1)
+        g = np.zeros(P)
+        t0 = time.perf_counter()
+        for b in range(nb):
+            f("ansatz_logpsi_gradient")(a, abi.ptr(np.ascontiguousarray(smp[b])), N, abi.ptr(g))
+        t_o = (time.perf_counter() - t0) / nb
+        per_sample[N] = t_mh + t_el + t_o
+        f("sampler_free")(s)
+        f("ansatz_free")(a)
+    # CG MAC rate: one reference CG-SR step, N = 32, Ns = 256
+    N, ns = 32, 256
+    cfg = abi.config_defaults(ansatz=w["kind"], hamiltonian=0, rbm_hidden_units=2 * N,
+                              num_samples=ns, num_thermalize=0, num_decorrelate=N, rng_seed=9,
+                              rbm_init_scale=0.06, sr_diag_shift=w["shift"], sr_cg_max_iters=100,
+                              sr_cg_tol=1e-6)
+    a = f("ansatz_create")(ctypes.byref(cfg), N)
+    P = f("ansatz_num_params")(a)
+    s = f("sampler_create")(N, ctypes.byref(cfg), fn, a)
+    info = abi.SrInfo()
+    t0 = time.perf_counter()
+    f("sr_step")(ctypes.byref(cfg), N, 1, a, s, ctypes.byref(info))
+    t_step = time.perf_counter() - t0
+    macs = (info.cg_iterations * 2 + 2) * ns * P
+    rate = macs / max(t_step - ns * per_sample[32] if 32 in per_sample else t_step * 0.5, 1e-6)
+    f("sampler_free")(s)
+    f("ansatz_free")(a)
+    t_iter = 0.0
+    for (N, _), it in zip(gpu["inst"], gpu["iters"]):
+        Pn = 2 * N * N + 3 * N
+        t_iter += B * per_sample[N] + (2 * it + 2) * B * Pn / rate
+    ess_cpu = sum(B / (2.0 * t) for t in gpu["taus"]) / t_iter
+    sample = (f"{kind} CPU path, 1 core: {nb} samples per instance size of MH (N moves each), "
+              f"E_loc (open twin scaled to the periodic bond count) and O rows, plus one "
+              f"reference CG-SR step at N=32, Ns={ns} for the CG MAC rate "
+              f"({rate / 1e9:.2f} GMAC/s); the step of all {len(gpu['inst'])} instances "
+              f"({t_iter:.1f} s) is extrapolated with the GPU run's CG iteration counts and tau_int")
+    return dict(value=ess_cpu, unit="effective samples/s", cores=1, kind=kind, sample=sample,
+                sr_iter_s=t_iter)
+
+
 def cpu_baseline(w, gpu):
     """Reference CPU path on a bounded sample of the same workload, 1 core.
 
@@ -422,6 +635,8 @@ def main():
         print(json.dumps({"copy_probe_bytes": nbytes, "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}))
         return
     w = WORKLOADS[args.workload]
+    if w.get("multi"):
+        return main_multi(args, w)
     gpu = gpu_run(args, w)
     cpu = None
     if not args.no_cpu:
@@ -459,6 +674,43 @@ def main():
     }
     if gpu["learned"]:
         out["learned"] = gpu["learned"]
+    if gpu["convergence"]:
+        out["convergence"] = gpu["convergence"]
+    if cpu:
+        out["speedup_vs_cpu_iter"] = cpu["sr_iter_s"] / (gpu["ms_per_step"] * 1e-3)
+    print(json.dumps(out))
+
+
+def main_multi(args, w):
+    gpu = gpu_run_multi(args, w)
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] timing the reference CPU path on a bounded sample")
+        cpu = cpu_baseline_multi(w, gpu)
+    out = {
+        "metric": METRIC,
+        "value": gpu["ess_per_s"],
+        "unit": "effective samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": gpu["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (random-init parameters N(0,0.01), chains started at random states)",
+        "config": {"workload": args.workload + ": " + w["desc"], "instances": len(gpu["inst"]),
+                   "samples_per_instance": w["chains"] * w["per_chain"], "method": "cg",
+                   "split_target": gpu["chosen"]},
+        "sr_iter_ms": gpu["ms_per_step"],
+        "ess_per_s_iteration": gpu["ess_per_s"],
+        "cg_iterations": gpu["iters"],
+        "autotune_us": gpu["tuning"],
+        "kernel_split_ms": {k: v["ms"] for k, v in gpu["split"].items()},
+        "roofline": gpu["roofline"],
+        "cpu_baseline": cpu,
+    }
     if gpu["convergence"]:
         out["convergence"] = gpu["convergence"]
     if cpu:

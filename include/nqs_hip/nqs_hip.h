@@ -159,6 +159,40 @@ nqs_hip_ar_t *nqs_hip_sampler_ar(nqs_sampler_t *s);
 int nqs_hip_sampler_learned_stats(const nqs_sampler_t *s, double *acceptance,
                                   long long *proposals, int *trainings, double *last_nll);
 
+/* Batched independent problems.  n (ansatz, sampler) pairs, each as
+ * nqs_sr_step(&cfgs[i], size_x[i], size_y[i], ansatz[i], samplers[i], ...)
+ * would step it (CG-SR, src/nqs/nqs_optimizer.c:255-262), stepped together:
+ * every problem's sampling and local energies, then one launch sequence for
+ * all problems' force, CG iterations and updates; from the second step on,
+ * the whole step is one replayed hipGraph.  Requirements per problem: a real
+ * RBM ansatz (NQS_ANSATZ_RBM) and a device sampler of that same ansatz
+ * (nqs_ansatz_log_amp, FP64, flip or exchange moves); each ansatz and
+ * sampler appears once.  The batch owns its problems' device state from
+ * create to free: parameters are updated on the device, and
+ * nqs_hip_multi_sync copies them (and each sampler's current
+ * configuration) back to the host.  CG results equal nqs_sr_step's up to
+ * the order of floating-point sums (the QGT products run through the RBM
+ * factors instead of an explicit O matrix).
+ * run: thermalizes each sampler once, then num_iterations steps;
+ * out_traces [n][num_iterations] receives the mean energies (may be NULL).
+ * tuning: the split-factor targets timed at the first step and their CG
+ * product times (microseconds); returns the chosen target.
+ * use_graph(m, 0): later steps are issued launch by launch instead of as
+ * the captured graph (same kernels and results; for per-kernel timing).
+ * last_energies: the local energies of problem i's last batch, sample
+ * r*C + c in slot r*C + c (chain c, round r); returns the count. */
+typedef struct nqs_hip_multi nqs_hip_multi_t;
+nqs_hip_multi_t *nqs_hip_multi_create(int n, const nqs_config_t *cfgs, const int *size_x,
+                                      const int *size_y, nqs_ansatz_t *const *ansatz,
+                                      nqs_sampler_t *const *samplers);
+int nqs_hip_multi_step(nqs_hip_multi_t *m, nqs_sr_step_info_t *out_infos);
+int nqs_hip_multi_run(nqs_hip_multi_t *m, int num_iterations, double *out_traces);
+int nqs_hip_multi_sync(nqs_hip_multi_t *m);
+int nqs_hip_multi_tuning(const nqs_hip_multi_t *m, int *targets, double *matvec_us, int capacity);
+int nqs_hip_multi_use_graph(nqs_hip_multi_t *m, int on);
+int nqs_hip_multi_last_energies(const nqs_hip_multi_t *m, int i, double *out, int capacity);
+void nqs_hip_multi_free(nqs_hip_multi_t *m);
+
 /* Integrated autocorrelation time of a series (Sokal window, c = 5). */
 double nqs_hip_tau_int(const double *x, int n);
 

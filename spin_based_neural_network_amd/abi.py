@@ -179,6 +179,14 @@ HIP_API = {
     "nqs_hip_ar_train": (c_int, [_P, _P, c_int, c_int, c_int, c_int, c_double, _P]),
     "nqs_hip_sampler_ar": (c_void_p, [_P]),
     "nqs_hip_sampler_learned_stats": (c_int, [_P, _P, _P, _P, _P]),
+    "nqs_hip_multi_create": (c_void_p, [c_int, _P, _P, _P, _P, _P]),
+    "nqs_hip_multi_step": (c_int, [_P, _P]),
+    "nqs_hip_multi_run": (c_int, [_P, c_int, _P]),
+    "nqs_hip_multi_sync": (c_int, [_P]),
+    "nqs_hip_multi_tuning": (c_int, [_P, _P, _P, c_int]),
+    "nqs_hip_multi_free": (None, [_P]),
+    "nqs_hip_multi_use_graph": (c_int, [_P, c_int]),
+    "nqs_hip_multi_last_energies": (c_int, [_P, c_int, _P, c_int]),
 }
 
 

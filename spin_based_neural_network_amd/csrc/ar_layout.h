@@ -74,18 +74,19 @@ NQS_AR_HD inline bool mask2(int kp, int k, int N) { return degree(kp, N) >= degr
 NQS_AR_HD inline bool mask3(int j, int kp, int N) { return j + 1 > degree(kp, N); }
 
 // whether parameter p is a free (unmasked) parameter
+// (P < 2^15, so the index arithmetic is 32-bit)
 NQS_AR_HD inline bool free_param(const Dims& d, long p) {
-  const int N = d.N;
-  if (p < d.ob1) return mask1((int)(p / N), (int)(p % N), N);
-  if (p < d.oW2) return true;
-  if (p < d.ob2) {
-    const long q = p - d.oW2;
-    return mask2((int)(q / H), (int)(q % H), N);
+  const int N = d.N, x = (int)p;
+  if (x < (int)d.ob1) return mask1(x / N, x % N, N);
+  if (x < (int)d.oW2) return true;
+  if (x < (int)d.ob2) {
+    const int q = x - (int)d.oW2;
+    return mask2(q / H, q % H, N);
   }
-  if (p < d.oW3) return true;
-  if (p < d.ob3) {
-    const long q = p - d.oW3;
-    return mask3((int)(q / H), (int)(q % H), N);
+  if (x < (int)d.oW3) return true;
+  if (x < (int)d.ob3) {
+    const int q = x - (int)d.oW3;
+    return mask3(q / H, q % H, N);
   }
   return true;
 }

@@ -319,6 +319,9 @@ bool same_ham(const nqs_config_t* c, int Lx, int Ly, int per, const AmpSource& s
          g_ham.marshall == src.marshall && g_ham.mlx == src.mlx && g_ham.mly == src.mly;
 }
 
+int ham_upload(const nqs::HamHost& H, const AmpSource& src, DBuf& store, DBuf& sub_store,
+               nqs::HamDev& d);
+
 int ham_device(const nqs_config_t* cfg, int Lx, int Ly, int per, const AmpSource& src,
                nqs::HamDev& out) {
   if (same_ham(cfg, Lx, Ly, per, src)) {
@@ -331,11 +334,28 @@ int ham_device(const nqs_config_t* cfg, int Lx, int Ly, int per, const AmpSource
             (int)cfg->hamiltonian);
     return -1;
   }
-  const nqs::HamHost& H = g_ham.host;
+  nqs::HamDev d;
+  CKR(ham_upload(g_ham.host, src, g_ws.ham, g_ws.sublat, d));
+  g_ham.cfg = *cfg;
+  g_ham.Lx = Lx;
+  g_ham.Ly = Ly;
+  g_ham.periodic = per;
+  g_ham.marshall = src.marshall;
+  g_ham.mlx = src.mlx;
+  g_ham.mly = src.mly;
+  g_ham.dev = d;
+  g_ham.valid = 1;
+  out = d;
+  return 0;
+}
+
+// term lists of H (and the Marshall sublattice of src) into device buffers
+int ham_upload(const nqs::HamHost& H, const AmpSource& src, DBuf& store, DBuf& sub_store,
+               nqs::HamDev& d) {
   const size_t nd = H.diag_a.size(), nf = H.flip_i.size(), np = H.pair_a.size();
   const size_t ints = 2 * nd + nf + 2 * np + 4, dbls = nd + nf + np + 4;
-  CKR(g_ws.ham.ensure(ints * sizeof(int) + dbls * sizeof(double) + 64));
-  char* base = g_ws.ham.as<char>();
+  CKR(store.ensure(ints * sizeof(int) + dbls * sizeof(double) + 64));
+  char* base = store.as<char>();
   double* dc = (double*)base;
   double* fc = dc + nd;
   double* pc = fc + nf;
@@ -357,7 +377,7 @@ int ham_device(const nqs_config_t* cfg, int Lx, int Ly, int per, const AmpSource
   std::copy(H.pair_a.begin(), H.pair_a.end(), hip_ + 2 * nd + nf);
   std::copy(H.pair_b.begin(), H.pair_b.end(), hip_ + 2 * nd + nf + np);
   CK(hipMemcpyAsync(base, stage.data(), stage.size(), hipMemcpyHostToDevice, g_stream));
-  nqs::HamDev d;
+  d = nqs::HamDev();
   d.n_diag = (int)nd;
   d.n_flip = (int)nf;
   d.n_pair = (int)np;
@@ -373,21 +393,11 @@ int ham_device(const nqs_config_t* cfg, int Lx, int Ly, int per, const AmpSource
         const int i = x * src.mly + y;
         if (i < N && ((x + y) & 1) == 0) sub[i] = 1;
       }
-    CKR(g_ws.sublat.ensure(N + 16));
-    CK(hipMemcpyAsync(g_ws.sublat.p, sub.data(), N, hipMemcpyHostToDevice, g_stream));
-    d.sublat = g_ws.sublat.as<signed char>();
+    CKR(sub_store.ensure(N + 16));
+    CK(hipMemcpyAsync(sub_store.p, sub.data(), N, hipMemcpyHostToDevice, g_stream));
+    d.sublat = sub_store.as<signed char>();
   }
   CK(hipStreamSynchronize(g_stream));
-  g_ham.cfg = *cfg;
-  g_ham.Lx = Lx;
-  g_ham.Ly = Ly;
-  g_ham.periodic = per;
-  g_ham.marshall = src.marshall;
-  g_ham.mlx = src.mlx;
-  g_ham.mly = src.mly;
-  g_ham.dev = d;
-  g_ham.valid = 1;
-  out = d;
   return 0;
 }
 
@@ -789,7 +799,7 @@ int ar_train(nqs_hip_ar* a, const uint64_t* rows, long n_avail, int constrained,
              int batch, double lr, double* mean_loss) {
   if (n_avail <= 0 || epochs <= 0) return 0;
   if (batch < 1) batch = 1;
-  const int nb = (batch + 63) / 64;
+  const int nb = nqs::ar_grad_blocks(batch);
   const long steps = (n_avail + batch - 1) / batch;
   CKR(a->partial.ensure(sizeof(float) * (size_t)nb * a->d.P));
   CKR(a->loss.ensure(sizeof(double) * (size_t)nb * steps));
@@ -811,7 +821,7 @@ int ar_train(nqs_hip_ar* a, const uint64_t* rows, long n_avail, int constrained,
       a->t++;
       nqs::ArAdam ad;
       ad.partial = a->partial.as<float>();
-      ad.nblk = (count + 63) / 64;
+      ad.nblk = nqs::ar_grad_blocks(count);
       ad.lr = (float)lr;
       ad.bc1 = (float)(1.0 - pow(0.9, (double)a->t));
       ad.bc2 = (float)(1.0 - pow(0.999, (double)a->t));
@@ -2000,7 +2010,7 @@ NQS_EXPORT int nqs_hip_ar_gradient(nqs_hip_ar_t* a, const int* spins, int B, int
   if (!a || !spins || !grad || B <= 0) return -1;
   uint64_t* bits = nullptr;
   CKR(ar_rows_to_device(a, spins, B, bits));
-  const int nb = (B + 63) / 64;
+  const int nb = nqs::ar_grad_blocks(B);
   CKR(a->partial.ensure(sizeof(float) * (size_t)nb * a->d.P));
   CKR(a->loss.ensure(sizeof(double) * nb));
   CKR(a->grad.ensure(sizeof(float) * a->d.P));
@@ -2084,4 +2094,410 @@ NQS_EXPORT int nqs_hip_config_layout(long* out, int cap) {
   if (!out || cap < n) return -1;
   for (int i = 0; i < n; i++) out[i] = v[i];
   return n;
+}
+
+// ===========================================================================
+// batched independent problems: n (real RBM ansatz, device sampler) pairs,
+// one CG-SR iteration each per nqs_hip_multi_step (nqs_sr_step semantics,
+// nqs_optimizer.c:255-262).  Per problem, on one of kMultiSide streams:
+// tables, Metropolis batch, forward, E_loc, energy sums; then for all
+// problems together (k_multi.hip): operands, force, a fixed number of CG
+// iterations (finished problems skip theirs), update and a gather of the
+// step's scalars.  The first step runs eagerly, times the split-K factor
+// candidates of the CG product on that step's data, and captures the whole
+// step into one hipGraph that every later step replays.
+// ===========================================================================
+struct nqs_hip_multi {
+  static constexpr int kSide = 4;
+  struct Prob {
+    nqs_ansatz* a = nullptr;
+    nqs_sampler* s = nullptr;
+    nqs_config_t cfg;
+    nqs::HamDev H;
+    DBuf ham, sublat, t, E, S1, S1t, T1t, Zt, TWt, Gp, vec, part, ticket, sc, stats;
+    int B = 0, rounds = 0, steps = 1, max_split = 1;
+  };
+  std::vector<Prob> p;
+  std::vector<nqs::MultiProb> mp;
+  DBuf d_mp, d_out;
+  nqs::MultiBatch mb;
+  int max_iters = 0;
+  int target = 0;                       // autotuned: G tiles x splits per problem
+  std::vector<std::pair<int, double>> tuning;   // (target, matvec us)
+  hipStream_t side[kSide] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[kSide] = {};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double* h_out = nullptr;              // pinned, kMultiOut per problem
+  long long steps_done = 0;
+  int tuned = 0, use_graph = 1;
+  ~nqs_hip_multi() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (int k = 0; k < kSide; k++) {
+      if (side[k]) (void)hipStreamDestroy(side[k]);
+      if (ev_join[k]) (void)hipEventDestroy(ev_join[k]);
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (h_out) (void)hipHostFree(h_out);
+    for (Prob& q : p) {
+      DBuf* all[] = {&q.ham, &q.sublat, &q.t, &q.E, &q.S1, &q.S1t, &q.T1t, &q.Zt, &q.TWt,
+                     &q.Gp, &q.vec, &q.part, &q.ticket, &q.sc, &q.stats};
+      for (DBuf* b : all) b->release();
+    }
+    d_mp.release();
+    d_out.release();
+  }
+};
+
+namespace {
+
+// split factors from the target count of G tiles x splits per problem
+int multi_set_split(nqs_hip_multi* m, int target) {
+  // the previous descriptor copy reads m->mp
+  CK(hipStreamSynchronize(g_stream));
+  for (size_t i = 0; i < m->p.size(); i++) {
+    const nqs_hip_multi::Prob& q = m->p[i];
+    const int M1 = q.a->M + 1, N1 = q.a->N + 1;
+    const int tiles = ((M1 + 63) / 64) * ((N1 + 63) / 64);
+    int ns = 1;
+    while (ns * 2 <= q.max_split && tiles * ns * 2 <= target) ns *= 2;
+    m->mp[i].nsplit = ns;
+  }
+  nqs::multi_layout(m->mp.data(), (int)m->mp.size(), m->mb);
+  CK(hipMemcpyAsync(m->d_mp.p, m->mp.data(), sizeof(nqs::MultiProb) * m->mp.size(),
+                    hipMemcpyHostToDevice, g_stream));
+  m->mb.probs = m->d_mp.as<nqs::MultiProb>();
+  m->target = target;
+  return 0;
+}
+
+// per-problem sampling and energies on the side streams, joined into g_stream
+int multi_issue_front(nqs_hip_multi* m) {
+  CK(hipEventRecord(m->ev_fork, g_stream));
+  for (int k = 0; k < nqs_hip_multi::kSide; k++) CK(hipStreamWaitEvent(m->side[k], m->ev_fork, 0));
+  for (size_t i = 0; i < m->p.size(); i++) {
+    nqs_hip_multi::Prob& q = m->p[i];
+    hipStream_t st = m->side[i % nqs_hip_multi::kSide];
+    nqs_ansatz* a = q.a;
+    nqs_sampler* s = q.s;
+    CKR(nqs::launch_prep<double>(a->d_par.as<double>(), a->kind, a->N, a->M, a->m64, st));
+    const ModelView<double> v = a->m64.view;
+    nqs::MHLaunch ml;
+    ml.move = s->opt.move == NQS_HIP_MOVE_FLIP ? 0 : 1;
+    ml.steps = q.steps;
+    ml.rounds = q.rounds;
+    ml.record = 1;
+    ml.n_out = q.B;
+    ml.out = s->d_samples.as<int>();
+    CKR(nqs::launch_mh<double>(v, s->cs, ml, st));
+    const size_t tb = (size_t)q.B * a->M;
+    double* t = q.t.as<double>();
+    CKR(nqs::launch_forward<double>(v, ml.out, q.B, t, t + tb, nullptr, nullptr, st));
+    double* E = q.E.as<double>();
+    CKR(nqs::launch_eloc<double>(v, q.H, ml.out, q.B, t, t + tb, E, E + q.B, st));
+    double* stats = q.stats.as<double>();
+    CKR(nqs::launch_sum(E, q.B, nullptr, 1.0 / q.B, stats + 0, st));
+    CKR(nqs::launch_sum(E, q.B, E, 1.0 / q.B, stats + 1, st));
+  }
+  for (int k = 0; k < nqs_hip_multi::kSide; k++) {
+    CK(hipEventRecord(m->ev_join[k], m->side[k]));
+    CK(hipStreamWaitEvent(g_stream, m->ev_join[k], 0));
+  }
+  return 0;
+}
+
+int multi_issue_back(nqs_hip_multi* m) {
+  for (int it = 0; it < m->max_iters; it++) CKR(nqs::launch_multi_cg_iter(m->mb, 0, g_stream));
+  CKR(nqs::launch_multi_apply(m->mb, g_stream));
+  CKR(nqs::launch_multi_gather(m->mb, m->d_out.as<double>(), g_stream));
+  CK(hipMemcpyAsync(m->h_out, m->d_out.p, sizeof(double) * nqs::kMultiOut * m->p.size(),
+                    hipMemcpyDeviceToHost, g_stream));
+  return 0;
+}
+
+int multi_issue_step(nqs_hip_multi* m) {
+  CKR(multi_issue_front(m));
+  CKR(nqs::launch_multi_prep(m->mb, g_stream));
+  CKR(nqs::launch_multi_force(m->mb, g_stream));
+  return multi_issue_back(m);
+}
+
+// time the CG product (matvec and alpha, idempotent at the CG start) for
+// each target and keep the fastest
+int multi_autotune(nqs_hip_multi* m) {
+  hipEvent_t e0, e1;
+  CK(hipEventCreate(&e0));
+  CK(hipEventCreate(&e1));
+  const int targets[] = {8, 16, 32, 64, 128, 256};
+  double best = 1e300;
+  int best_t = 32;
+  m->tuning.clear();
+  for (int t : targets) {
+    CKR(multi_set_split(m, t));
+    CKR(nqs::launch_multi_cg_iter(m->mb, 1, g_stream));   // warm
+    CK(hipEventRecord(e0, g_stream));
+    const int reps = 5;
+    for (int r = 0; r < reps; r++) CKR(nqs::launch_multi_cg_iter(m->mb, 1, g_stream));
+    CK(hipEventRecord(e1, g_stream));
+    CK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CK(hipEventElapsedTime(&ms, e0, e1));
+    const double us = 1e3 * ms / reps;
+    m->tuning.push_back({t, us});
+    if (us < best) {
+      best = us;
+      best_t = t;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return multi_set_split(m, best_t);
+}
+
+void multi_fill_info(nqs_hip_multi* m, nqs_sr_step_info_t* infos) {
+  for (size_t i = 0; i < m->p.size(); i++) {
+    const double* o = m->h_out + i * nqs::kMultiOut;
+    const nqs_config_t& c = m->p[i].cfg;
+    if (o[5] != 0.0)
+      fprintf(stderr, "nqs_sr cg_solve: Krylov breakdown at iter=%d, pAp=%.3e, residual/rhs=%.3e "
+                      "(tol=%.3e). Returning unconverged partial step.\n",
+              (int)o[3], o[6], o[8] > 0.0 ? sqrt(o[7] / o[8]) : 0.0, c.sr_cg_tol);
+    if (!infos) continue;
+    nqs_sr_step_info_t& f = infos[i];
+    f.mean_energy = o[0];
+    f.variance_energy = o[1] - o[0] * o[0];
+    f.update_norm = sqrt(o[2]) * c.learning_rate;
+    f.acceptance_ratio = o[9];
+    f.cg_iterations = (int)o[3];
+    f.converged = (int)o[4];
+  }
+}
+
+int multi_capture(nqs_hip_multi* m) {
+  const int kt = g_kt_on.exchange(0);
+  hipError_t e = hipStreamBeginCapture(g_stream, hipStreamCaptureModeRelaxed);
+  int rc = e == hipSuccess ? multi_issue_step(m) : -1;
+  hipGraph_t g = nullptr;
+  hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(g_stream, &g) : e;
+  g_kt_on = kt;
+  if (rc || e2 != hipSuccess || !g) {
+    fprintf(stderr, "nqs_hip: graph capture of the batched step failed (%s)\n",
+            hipGetErrorString(e2));
+    if (g) (void)hipGraphDestroy(g);
+    return -1;
+  }
+  m->graph = g;
+  CK(hipGraphInstantiate(&m->exec, m->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+int multi_step(nqs_hip_multi* m, nqs_sr_step_info_t* infos) {
+  if (!m->tuned) {
+    // first step: eager, with the split factors tuned on its own data
+    CKR(multi_issue_front(m));
+    CKR(multi_set_split(m, 32));
+    CKR(nqs::launch_multi_prep(m->mb, g_stream));
+    CKR(nqs::launch_multi_force(m->mb, g_stream));
+    CKR(multi_autotune(m));
+    CKR(multi_issue_back(m));
+    m->tuned = 1;
+  } else if (!m->use_graph) {
+    CKR(multi_issue_step(m));
+  } else {
+    if (!m->exec) CKR(multi_capture(m));
+    CK(hipGraphLaunch(m->exec, g_stream));
+  }
+  CK(hipStreamSynchronize(g_stream));
+  multi_fill_info(m, infos);
+  for (nqs_hip_multi::Prob& q : m->p) q.a->m64.valid = 0;
+  m->steps_done++;
+  return 0;
+}
+
+}  // namespace
+
+NQS_EXPORT nqs_hip_multi_t* nqs_hip_multi_create(int n, const nqs_config_t* cfgs, const int* size_x,
+                                                 const int* size_y, nqs_ansatz_t* const* ansatz,
+                                                 nqs_sampler_t* const* samplers) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (n <= 0 || !cfgs || !size_x || !size_y || !ansatz || !samplers) return nullptr;
+  if (stream_ready()) return nullptr;
+  nqs_hip_multi* m = new nqs_hip_multi();
+  m->p.resize(n);
+  m->mp.resize(n);
+  auto fail = [&](int i, const char* why) -> nqs_hip_multi_t* {
+    fprintf(stderr, "nqs_hip_multi_create: problem %d: %s\n", i, why);
+    delete m;
+    return nullptr;
+  };
+  for (int i = 0; i < n; i++) {
+    nqs_hip_multi::Prob& q = m->p[i];
+    q.a = ansatz[i];
+    q.s = samplers[i];
+    q.cfg = cfgs[i];
+    nqs_ansatz* a = q.a;
+    nqs_sampler* s = q.s;
+    if (!a || !s) return fail(i, "null ansatz or sampler");
+    for (int j = 0; j < i; j++)
+      if (m->p[j].a == a || m->p[j].s == s) return fail(i, "ansatz or sampler used twice");
+    if (a->kind != NQS_ANSATZ_RBM || a->M < 1) return fail(i, "the ansatz is not a real RBM");
+    if (!s->device || s->src.a != a || s->src.marshall)
+      return fail(i, "the sampler does not sample this ansatz on the device");
+    if (s->opt.precision != NQS_HIP_FP64 || s->opt.move == NQS_HIP_MOVE_LEARNED)
+      return fail(i, "the sampler needs FP64 and flip or exchange moves");
+    const int N = a->N, M = a->M, N1 = N + 1, M1 = M + 1;
+    if ((M1 * N1 + nqs::kMultiVecBlock - 1) / nqs::kMultiVecBlock > nqs::kMultiMaxPart)
+      return fail(i, "problem too large for the batched CG");
+    q.B = q.cfg.num_samples;
+    if (q.B <= 0) return fail(i, "num_samples must be positive");
+    const int C = s->cs.C;
+    q.rounds = (q.B + C - 1) / C;
+    q.steps = s->cfg.num_decorrelate > 0 ? s->cfg.num_decorrelate : 1;
+    nqs::HamHost hh;
+    if (nqs::build_hamiltonian(&q.cfg, size_x[i], size_y[i], s->opt.periodic, hh) != 0 ||
+        hh.n_sites != N)
+      return fail(i, "Hamiltonian unavailable or of another size");
+    if (ham_upload(hh, s->src, q.ham, q.sublat, q.H)) return fail(i, "device memory");
+    q.max_split = 1;
+    while (q.max_split * 2 <= 32 && q.max_split * 2 * 64 <= q.B) q.max_split *= 2;
+    const size_t L = (size_t)M1 * N1, B = (size_t)q.B, d = sizeof(double);
+    if (q.t.ensure(d * 2 * B * M) || q.E.ensure(d * 2 * B) || q.S1.ensure(d * B * N1) ||
+        q.S1t.ensure(d * B * N1) || q.T1t.ensure(d * B * M1) || q.Zt.ensure(d * B * M1) ||
+        q.TWt.ensure(d * B * M1) || q.Gp.ensure(d * 2 * q.max_split * L) ||
+        q.vec.ensure(d * 6 * L) || q.part.ensure(d * nqs::kMultiMaxPart) ||
+        q.ticket.ensure(64) || q.sc.ensure(sizeof(nqs::CGScalars)) || q.stats.ensure(d * 8) ||
+        s->d_samples.ensure(sizeof(int) * B * N))
+      return fail(i, "device memory");
+    if (hipMemsetAsync(q.ticket.p, 0, 64, g_stream) != hipSuccess ||
+        hipMemsetAsync(q.sc.p, 0, sizeof(nqs::CGScalars), g_stream) != hipSuccess)
+      return fail(i, "device memory");
+    a->dev_fresh = 0;
+    if (ansatz_upload(a)) return fail(i, "parameter upload");
+    a->dev_fresh = 1;
+    nqs::MultiProb& mp = m->mp[i];
+    mp.B = q.B;
+    mp.N = N;
+    mp.M = M;
+    mp.nsplit = 1;
+    mp.max_it = q.cfg.sr_cg_max_iters;
+    mp.C = C;
+    mp.eps = q.cfg.sr_diag_shift;
+    mp.tol = q.cfg.sr_cg_tol;
+    mp.lr = q.cfg.learning_rate;
+    mp.acc = s->cs.acc;
+    mp.tot = s->cs.tot;
+    mp.spins = s->d_samples.as<int>();
+    mp.t = q.t.as<double>();
+    mp.E = q.E.as<double>();
+    mp.S1 = q.S1.as<double>();
+    mp.S1t = q.S1t.as<double>();
+    mp.T1t = q.T1t.as<double>();
+    mp.Zt = q.Zt.as<double>();
+    mp.TWt = q.TWt.as<double>();
+    mp.Gp = q.Gp.as<double>();
+    double* v = q.vec.as<double>();
+    mp.x = v;
+    mp.r = v + L;
+    mp.p = v + 2 * L;
+    mp.Ap = v + 3 * L;
+    mp.Om = v + 4 * L;
+    mp.F = v + 5 * L;
+    mp.part = q.part.as<double>();
+    mp.ticket = q.ticket.as<unsigned>();
+    mp.sc = q.sc.as<nqs::CGScalars>();
+    mp.stats = q.stats.as<double>();
+    mp.norm2 = q.stats.as<double>() + 5;
+    mp.par = a->d_par.as<double>();
+    m->max_iters = std::max(m->max_iters, q.cfg.sr_cg_max_iters);
+  }
+  if (m->d_mp.ensure(sizeof(nqs::MultiProb) * n) || m->d_out.ensure(sizeof(double) * nqs::kMultiOut * n) ||
+      hipHostMalloc((void**)&m->h_out, sizeof(double) * nqs::kMultiOut * n, hipHostMallocDefault) !=
+          hipSuccess)
+    return fail(0, "memory");
+  if (hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming) != hipSuccess)
+    return fail(0, "event");
+  for (int k = 0; k < nqs_hip_multi::kSide; k++)
+    if (hipStreamCreateWithFlags(&m->side[k], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&m->ev_join[k], hipEventDisableTiming) != hipSuccess)
+      return fail(0, "stream");
+  if (multi_set_split(m, 32) || hipStreamSynchronize(g_stream) != hipSuccess)
+    return fail(0, "setup");
+  return m;
+}
+
+NQS_EXPORT int nqs_hip_multi_step(nqs_hip_multi_t* m, nqs_sr_step_info_t* out_infos) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!m) return -1;
+  return multi_step(m, out_infos);
+}
+
+NQS_EXPORT int nqs_hip_multi_run(nqs_hip_multi_t* m, int num_iterations, double* out_traces) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!m || num_iterations < 0) return -1;
+  // nqs_sr_run: thermalize once, then the iterations (nqs_optimizer.c:264-305)
+  for (nqs_hip_multi::Prob& q : m->p)
+    CKR(sampler_advance(q.s, q.s->cfg.num_thermalize, q.s->cfg.num_thermalize > 0 ? 1 : 0, 0,
+                        nullptr, nullptr));
+  std::vector<nqs_sr_step_info_t> info(m->p.size());
+  for (int it = 0; it < num_iterations; it++) {
+    CKR(multi_step(m, info.data()));
+    if (out_traces)
+      for (size_t i = 0; i < m->p.size(); i++)
+        out_traces[i * (size_t)num_iterations + it] = info[i].mean_energy;
+  }
+  return 0;
+}
+
+NQS_EXPORT int nqs_hip_multi_sync(nqs_hip_multi_t* m) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!m) return -1;
+  for (nqs_hip_multi::Prob& q : m->p) {
+    CKR(ansatz_download(q.a));
+    CKR(sampler_sync_cur(q.s));
+  }
+  return 0;
+}
+
+NQS_EXPORT int nqs_hip_multi_tuning(const nqs_hip_multi_t* m, int* targets, double* matvec_us,
+                                    int capacity) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!m) return -1;
+  const int n = (int)m->tuning.size();
+  for (int i = 0; i < n && i < capacity; i++) {
+    if (targets) targets[i] = m->tuning[i].first;
+    if (matvec_us) matvec_us[i] = m->tuning[i].second;
+  }
+  return m->target;
+}
+
+NQS_EXPORT int nqs_hip_multi_use_graph(nqs_hip_multi_t* m, int on) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!m) return -1;
+  m->use_graph = on ? 1 : 0;
+  return 0;
+}
+
+NQS_EXPORT int nqs_hip_multi_last_energies(const nqs_hip_multi_t* m, int i, double* out,
+                                           int capacity) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!m || i < 0 || i >= (int)m->p.size() || !out || !m->steps_done) return -1;
+  const nqs_hip_multi::Prob& q = m->p[i];
+  const int n = std::min(capacity, q.B);
+  CK(hipMemcpyAsync(out, q.E.p, sizeof(double) * n, hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return n;
+}
+
+NQS_EXPORT void nqs_hip_multi_free(nqs_hip_multi_t* m) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!m) return;
+  (void)hipStreamSynchronize(g_stream);
+  for (nqs_hip_multi::Prob& q : m->p) {
+    (void)ansatz_download(q.a);
+    (void)sampler_sync_cur(q.s);
+    q.a->dev_fresh = 0;
+    q.a->m64.valid = 0;
+  }
+  delete m;
 }

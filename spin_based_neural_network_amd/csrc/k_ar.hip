@@ -15,9 +15,14 @@
 // k_ar_mh     learned Metropolis-Hastings rounds of 16 chains per wave:
 //             d = 2(log|psi(s')| - log|psi(s)|) + log q(s) - log q(s');
 //             accept when d >= 0 or log u < d (u from the chain's xorshift).
+// k_ar_propose / k_ar_accept_rounds  the batched form of k_ar_mh: the
+//             proposal does not depend on the chain state, so every round's
+//             proposals are drawn in one launch, evaluated by one batched
+//             amplitude pass, and accepted round by round per chain.
 // k_ar_grad   forward + backward of the mean negative log-likelihood over a
-//             minibatch; 64 rows per single-wave block, weight gradients as
-//             MFMA products over the rows, partial sums per block.
+//             minibatch; kArGradRows rows per single-wave block, dL/dl in
+//             FP32, weight gradients as MFMA products over the rows, partial
+//             sums per block.
 // k_ar_adam   sums the partials in block order (deterministic), Adam on the
 //             FP32 master weights, and writes the BF16 operand copies.
 #include "ar_layout.h"
@@ -59,8 +64,12 @@ __device__ inline bf16x8 plain_op(const f32x4* z, int c) {
   }
   return r;
 }
-__device__ inline double log_sigmoid(double x) {
-  return x >= 0.0 ? -log1p(exp(-x)) : x - log1p(exp(x));
+// log sigmoid(x) and sigmoid(x) in FP32 from one exponential of -|x|
+__device__ inline float log_sigmoid_f(float x, float& sig) {
+  const float e = __expf(-fabsf(x));
+  const float r = __frcp_rn(1.f + e);
+  sig = x >= 0.f ? r : e * r;
+  return fminf(x, 0.f) - __logf(1.f + e);
 }
 __device__ inline f32x4 zero4() { return f32x4{0.f, 0.f, 0.f, 0.f}; }
 __device__ inline uint64_t site_mask(int N) { return N >= 64 ? ~0ULL : ((1ULL << N) - 1ULL); }
@@ -420,14 +429,24 @@ __global__ __launch_bounds__(64) void k_ar_grad(ArDev v, ArGrad g) {
   const int N = v.N, lane = threadIdx.x, rl = lane & 15, q = lane >> 4;
   const ar::Dims d = ar::dims(N);
   const int c1 = d.c1, ns = (N + 15) / 16;
-  for (long p = lane; p < d.P; p += 64) acc[p] = 0.f;
+  for (int p = lane; p < (int)d.P; p += 64) acc[p] = 0.f;
 
-  bf16x8 A2[4][2], A3[NT][2];
+  // every operand of the forward and backward products stays in registers
+  // for the whole block (one wave per block, so no other wave hides loads)
+  bf16x8 A1[4][2], A2[4][2], A3[NT][2], A2b[4][2], A3b[4][C3];
   f32x4 B1[4], B2[4], B3[NT];
 #pragma unroll
   for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-    for (int c = 0; c < 2; c++) A2[mt][c] = ld_op(v.op2f, ((long)(mt * 2 + c) * 64 + lane) * 8);
+    for (int c = 0; c < 2; c++) {
+      A1[mt][c] = c < c1 ? ld_op(v.op1f, ((long)(mt * c1 + c) * 64 + lane) * 8) : bf16x8{};
+      A2[mt][c] = ld_op(v.op2f, ((long)(mt * 2 + c) * 64 + lane) * 8);
+      A2b[mt][c] = ld_op(v.op2b, ((long)(mt * 2 + c) * 64 + lane) * 8);
+    }
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int c = 0; c < C3; c++) A3b[mt][c] = ld_op(v.op3b, ((long)(mt * C3 + c) * 64 + lane) * 8);
 #pragma unroll
   for (int mt = 0; mt < NT; mt++)
 #pragma unroll
@@ -447,13 +466,20 @@ __global__ __launch_bounds__(64) void k_ar_grad(ArDev v, ArGrad g) {
   double loss = 0.0;
   __syncthreads();
 
-  for (int h = 0; h < 2; h++) {
+  for (int h = 0; h < kArGradRows / 32; h++) {
+    uint64_t rbits[2];
+    bool rvalid[2];
+#pragma unroll
+    for (int tile = 0; tile < 2; tile++) {
+      const long mrow = (long)blockIdx.x * kArGradRows + h * 32 + tile * 16 + rl;
+      rvalid[tile] = mrow < g.count;
+      rbits[tile] = rvalid[tile] ? g.rows[(g.start + mrow) % g.n_avail] : 0ULL;
+    }
 #pragma unroll 1
     for (int tile = 0; tile < 2; tile++) {
       const int r32 = tile * 16 + rl;
-      const long mrow = (long)blockIdx.x * 64 + h * 32 + r32;
-      const bool valid = mrow < g.count;
-      const uint64_t bits = valid ? g.rows[(g.start + mrow) % g.n_avail] : 0ULL;
+      const bool valid = rvalid[tile];
+      const uint64_t bits = rbits[tile];
       // inputs, K slot 32c + 8q + e
       bf16x8 X[2];
 #pragma unroll
@@ -471,9 +497,8 @@ __global__ __launch_bounds__(64) void k_ar_grad(ArDev v, ArGrad g) {
       f32x4 z1[4], z2[4], lg[NT];
 #pragma unroll
       for (int mt = 0; mt < 4; mt++) {
-        f32x4 a = B1[mt];
-        for (int c = 0; c < c1; c++)
-          a = mfma(ld_op(v.op1f, ((long)(mt * c1 + c) * 64 + lane) * 8), X[c], a);
+        f32x4 a = mfma(A1[mt][0], X[0], B1[mt]);
+        if (c1 > 1) a = mfma(A1[mt][1], X[1], a);
         z1[mt] = a;
       }
       const bf16x8 a0 = relu_op(z1, 0), a1 = relu_op(z1, 1);
@@ -497,10 +522,11 @@ __global__ __launch_bounds__(64) void k_ar_grad(ArDev v, ArGrad g) {
             const bool forced = g.constrained && (rem_d <= 0 || rem_d >= rem);
             if (!forced) {
               const int x = (int)((bits >> o) & 1ULL);
-              const double l = (double)lg[mt][r];
-              const double ls = log_sigmoid(l);
-              loss -= x ? ls : ls - l;
-              gv = (float)((exp(ls) - (double)x) * (double)g.scale);
+              const float l = lg[mt][r];
+              float sig;
+              const float ls = log_sigmoid_f(l, sig);
+              loss -= (double)(x ? ls : ls - l);
+              gv = (sig - (float)x) * g.scale;
             }
           }
           g3[mt][r] = gv;
@@ -513,16 +539,14 @@ __global__ __launch_bounds__(64) void k_ar_grad(ArDev v, ArGrad g) {
       for (int mt = 0; mt < 4; mt++) {
         f32x4 a = zero4();
 #pragma unroll
-        for (int c = 0; c < C3; c++)
-          a = mfma(ld_op(v.op3b, ((long)(mt * C3 + c) * 64 + lane) * 8), G3[c], a);
+        for (int c = 0; c < C3; c++) a = mfma(A3b[mt][c], G3[c], a);
 #pragma unroll
         for (int r = 0; r < 4; r++) g2[mt][r] = z2[mt][r] > 0.f ? a[r] : 0.f;
       }
       const bf16x8 G20 = plain_op<4>(g2, 0), G21 = plain_op<4>(g2, 1);
 #pragma unroll
       for (int mt = 0; mt < 4; mt++) {
-        f32x4 a = mfma(ld_op(v.op2b, ((long)(mt * 2 + 1) * 64 + lane) * 8), G21,
-                       mfma(ld_op(v.op2b, ((long)(mt * 2) * 64 + lane) * 8), G20, zero4()));
+        f32x4 a = mfma(A2b[mt][1], G21, mfma(A2b[mt][0], G20, zero4()));
 #pragma unroll
         for (int r = 0; r < 4; r++) g1[mt][r] = z1[mt][r] > 0.f ? a[r] : 0.f;
       }
@@ -588,8 +612,9 @@ __global__ __launch_bounds__(64) void k_ar_grad(ArDev v, ArGrad g) {
     }
     __syncthreads();
   }
-  for (long p = lane; p < d.P; p += 64)
-    g.partial[(long)blockIdx.x * d.P + p] = ar::free_param(d, p) ? acc[p] : 0.f;
+  // masked entries are written as computed; the consumers of the partials
+  // (k_ar_adam, k_ar_sum_grad) skip or zero them
+  for (int p = lane; p < (int)d.P; p += 64) g.partial[(long)blockIdx.x * d.P + p] = acc[p];
   loss = group_allreduce_sum<64>(loss);
   if (lane == 0) g.loss[blockIdx.x] = loss;
 }
@@ -652,7 +677,7 @@ __global__ void k_ar_sum_grad(ArDev v, const float* __restrict__ partial, int nb
   if (p >= d.P) return;
   float gs = 0.f;
   for (int b = 0; b < nblk; b++) gs += partial[(long)b * d.P + p];
-  out[p] = gs;
+  out[p] = ar::free_param(d, p) ? gs : 0.f;
 }
 
 }  // namespace
@@ -768,7 +793,7 @@ int launch_ar_grad(const ArDev& v, const ArGrad& g, hipStream_t st) {
   if (g.count <= 0 || g.n_avail <= 0) return 0;
   if (v.N < 1 || v.N > ar::kMaxN) return -1;
   const int nt = (v.N + 15) / 16;
-  const dim3 grid((unsigned)((g.count + 63) / 64)), blk(64);
+  const dim3 grid((unsigned)ar_grad_blocks(g.count)), blk(64);
   KScope ks("ar_grad", st);
   switch (nt) {
     case 1: hipLaunchKernelGGL(k_ar_grad<1>, grid, blk, 0, st, v, g); break;

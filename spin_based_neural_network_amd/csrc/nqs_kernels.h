@@ -128,6 +128,10 @@ struct ArGrad {
   float* partial = nullptr;
   double* loss = nullptr;
 };
+// rows of a training batch per k_ar_grad block (a multiple of 32); each
+// block writes one partial gradient of P floats
+constexpr int kArGradRows = 32;
+inline int ar_grad_blocks(long count) { return (int)((count + kArGradRows - 1) / kArGradRows); }
 struct ArAdam {
   const float* partial = nullptr;
   int nblk = 0;
@@ -255,6 +259,63 @@ int launch_cg_direction(CGScalars* sc, double* p, const double* r, long P,
 int launch_qgt_finish(double* out, const double* c1, const double* m1, const double* u1mean,
                       const double* c2, const double* m2, const double* u2mean,
                       double scale, double eps, const double* v, long P, hipStream_t st);
+
+// ---- batched independent CG-SR problems (k_multi.hip) -------------------------
+// One real-RBM problem of a batch.  Vectors over the parameters live in a
+// padded (M+1) x (N+1) row-major layout: W[h][i] at (h, i), b[h] at (h, N),
+// a[i] at (M, i), and (M, N) is always 0.  In that layout the O row of
+// sample b is the outer product [t_b, 1] (x) [s_b, 1] (t = tanh theta), so
+// O v = rowsum([t, 1] * ([s, 1] V^T)) and O^T w = ([t, 1] * w)^T [s, 1]:
+// two small GEMMs through the spins and t instead of two passes over O.
+constexpr int kMultiVecBlock = 256;      // threads of the vector kernels
+constexpr int kMultiMaxPart = 512;       // block partials per problem
+struct MultiProb {
+  int B = 0, N = 0, M = 0, nsplit = 1, max_it = 0, C = 0;
+  double eps = 0.0, tol = 0.0, lr = 0.0;
+  const unsigned long long* acc = nullptr;   // chains' accepted / proposed moves [C]
+  const unsigned long long* tot = nullptr;
+  const int* spins = nullptr;    // [B][N]
+  const double* t = nullptr;     // [B][M]
+  const double* E = nullptr;     // [B]
+  double* S1 = nullptr;          // [B][N+1]   [s_b, 1]
+  double* S1t = nullptr;         // [N+1][B]
+  double* T1t = nullptr;         // [M+1][B]   [t_b, 1]^T
+  double* Zt = nullptr;          // [M+1][B]   V [s_b, 1]^T
+  double* TWt = nullptr;         // [2][M+1][B] weighted [t_b, 1]^T
+  double* Gp = nullptr;          // [2][nsplit][M+1][N+1] split-K partials
+  double *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr, *Om = nullptr,
+         *F = nullptr;           // padded vectors
+  double* part = nullptr;        // [kMultiMaxPart] block partials
+  unsigned* ticket = nullptr;    // arrival counter of the last-block reductions
+  CGScalars* sc = nullptr;
+  const double* stats = nullptr; // [0] <E> (reference order), read by the force
+  double* norm2 = nullptr;       // |delta|^2
+  double* par = nullptr;         // [P] parameters, reference layout
+  // first block of this problem in each batched launch
+  // (blk_g: one G product per problem, CG; blk_g2: two, the force)
+  int blk_prep = 0, blk_z = 0, blk_rows = 0, blk_g = 0, blk_g2 = 0, blk_vec = 0;
+};
+struct MultiBatch {
+  int n = 0;
+  const MultiProb* probs = nullptr;   // device copy
+  int nblk_prep = 0, nblk_z = 0, nblk_rows = 0, nblk_g = 0, nblk_g2 = 0, nblk_vec = 0;
+};
+// fills the problems' first-block offsets of every launch and the totals
+void multi_layout(MultiProb* probs, int n, MultiBatch& mb);
+// [s, 1], [t, 1] operands and the two weight sets (1, E) of the force
+int launch_multi_prep(const MultiBatch& mb, hipStream_t st);
+// Om, F, CG start (x = 0, r = p = F, rs = |F|^2)
+int launch_multi_force(const MultiBatch& mb, hipStream_t st);
+// one CG iteration (A p, alpha, x / r update, beta, p) on every problem
+// not yet done; `matvec_only` stops after A p and alpha (idempotent at the
+// CG start, used to time the split factors)
+int launch_multi_cg_iter(const MultiBatch& mb, int matvec_only, hipStream_t st);
+// par -= lr * x (reference layout), |x|^2
+int launch_multi_apply(const MultiBatch& mb, hipStream_t st);
+// per problem kMultiOut doubles: <E>, <E^2>, |delta|^2, CG iterations,
+// converged, breakdown, pAp, rs_new, rs0, accepted / proposed moves
+constexpr int kMultiOut = 10;
+int launch_multi_gather(const MultiBatch& mb, double* out, hipStream_t st);
 
 // ---- bookkeeping ------------------------------------------------------------
 void count_launch(int n = 1);

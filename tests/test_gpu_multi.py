@@ -1,0 +1,190 @@
+"""Batched independent CG-SR problems (nqs_hip_multi_*), through the C ABI.
+
+* Against the reference: the golden CG-SR steps of the real-RBM scenarios
+  (single chain, reference sample sequence) run as one ragged batch and
+  meet the same CG-family tolerances as nqs_sr_step does.
+* Against the library's own single-problem path on identical clones: the
+  same samples (mean energies and acceptance bit-equal) and updates within
+  the CG conditioning bound, for a batch of TFIM chains of different sizes;
+  graph replays (steps >= 2) keep agreeing.
+* TFIM chains of several sizes and fields approach the free-fermion ground
+  state energy -sum_k sqrt(1 + h^2 - 2 h cos k), k = (2n+1) pi / N.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import scenarios
+from conftest import golden_case
+from spin_based_neural_network_amd.abi import SrInfo, config_defaults, options_defaults, ptr
+
+pytestmark = pytest.mark.gpu
+
+RBM = 5
+
+
+def tfim_pbc_closed_form(N, h):
+    k = (2 * np.arange(N) + 1) * np.pi / N
+    return -np.sum(np.sqrt(1.0 + h * h - 2 * h * np.cos(k)))
+
+
+def _make_batch(hip, problems):
+    n = len(problems)
+    cfgs = (type(problems[0]["cfg"]) * n)(*[p["cfg"] for p in problems])
+    lx = (ctypes.c_int * n)(*[p["Lx"] for p in problems])
+    ly = (ctypes.c_int * n)(*[p["Ly"] for p in problems])
+    av = (ctypes.c_void_p * n)(*[p["a"] for p in problems])
+    sv = (ctypes.c_void_p * n)(*[p["s"] for p in problems])
+    m = hip.nqs_hip_multi_create(n, cfgs, lx, ly, av, sv)
+    assert m, "nqs_hip_multi_create returned NULL"
+    return m
+
+
+def _infos(n):
+    return (SrInfo * n)()
+
+
+def _info_row(i):
+    return [i.mean_energy, i.variance_energy, i.update_norm, i.acceptance_ratio,
+            i.cg_iterations, i.converged]
+
+
+@pytest.mark.parametrize("names", [["cg_tfim_2x2_rbm", "cg_tfim_chain16_rbm"]])
+def test_multi_matches_reference_golden(hip, golden, names):
+    print("multi vs golden:", names, flush=True)
+    r = scenarios.Runner(hip, "nqs_")
+    scs = [next(s for s in scenarios.SR if s["name"] == nm) for nm in names]
+    probs = []
+    for sc in scs:
+        cfg = config_defaults(**sc["cfg"])
+        N = sc["Lx"] * sc["Ly"]
+        a = r.ansatz(cfg, N)
+        fn, user = r.callback(a, None)
+        s = hip.nqs_sampler_create(N, ctypes.byref(cfg), fn, user)
+        hip.nqs_sampler_thermalize(s)
+        probs.append(dict(cfg=cfg, Lx=sc["Lx"], Ly=sc["Ly"], a=a, s=s, p0=r.params(a)))
+    m = _make_batch(hip, probs)
+    steps = max(sc["steps"] for sc in scs)
+    params = [[p["p0"]] for p in probs]
+    infos = [[] for _ in probs]
+    for _ in range(steps):
+        out = _infos(len(probs))
+        assert hip.nqs_hip_multi_step(m, out) == 0
+        assert hip.nqs_hip_multi_sync(m) == 0
+        for i, p in enumerate(probs):
+            params[i].append(r.params(p["a"]))
+            infos[i].append(_info_row(out[i]))
+    hip.nqs_hip_multi_free(m)
+    for i, sc in enumerate(scs):
+        ref = golden_case(golden, f"sr/{sc['name']}")
+        k = sc["steps"]
+        got_p = np.array(params[i][:k + 1])
+        got_i = np.array(infos[i][:k])
+        step = np.abs(np.diff(ref["params"], axis=0)).max()
+        err = np.abs(got_p - ref["params"]).max()
+        assert err <= 1e-4 * step + 1e-12, (sc["name"], err, step)
+        s = max(1.0, float(np.abs(ref["info"][:, :4]).max()))
+        assert np.abs(got_i[:, :4] - ref["info"][:, :4]).max() <= 1e-7 * s, sc["name"]
+        assert np.all(np.abs(got_i[:, 4] - ref["info"][:, 4]) <= 1)
+        same = got_i[:, 4] == ref["info"][:, 4]
+        np.testing.assert_array_equal(got_i[same, 5], ref["info"][same, 5])
+    for p in probs:
+        hip.nqs_sampler_free(p["s"])
+        hip.nqs_ansatz_free(p["a"])
+
+
+def _tfim_problem(hip, r, N, h, chains, samples, seed, shift=1e-3, lr=1e-2, cg_iters=100):
+    cfg = config_defaults(ansatz=RBM, hamiltonian=0, transverse_field=h, rbm_hidden_units=2 * N,
+                          num_samples=samples, num_thermalize=2 * N, num_decorrelate=N,
+                          rng_seed=seed, learning_rate=lr, sr_diag_shift=shift,
+                          sr_cg_max_iters=cg_iters, sr_cg_tol=1e-6)
+    a = r.ansatz(cfg, N)
+    fn, user = r.callback(a, None)
+    opt = options_defaults(num_chains=chains, periodic=1)
+    s = hip.nqs_hip_sampler_create_ex(N, ctypes.byref(cfg), ctypes.byref(opt), fn, user)
+    assert s
+    return dict(cfg=cfg, Lx=N, Ly=1, a=a, s=s, N=N, h=h)
+
+
+def test_multi_equals_single_path_ragged(hip):
+    sizes = [(8, 0.6), (12, 1.0), (16, 1.4), (24, 0.9), (32, 1.2)]
+    print("multi vs single path:", sizes, flush=True)
+    r = scenarios.Runner(hip, "nqs_")
+    single = [_tfim_problem(hip, r, N, h, 64, 512, 11 + N) for N, h in sizes]
+    batch = [_tfim_problem(hip, r, N, h, 64, 512, 11 + N) for N, h in sizes]
+    for p in single + batch:
+        hip.nqs_sampler_thermalize(p["s"])
+        p["p0"] = r.params(p["a"])
+    for p, q in zip(single, batch):
+        assert np.array_equal(p["p0"], q["p0"])
+    m = _make_batch(hip, batch)
+    for step in range(3):
+        out = _infos(len(batch))
+        assert hip.nqs_hip_multi_step(m, out) == 0
+        assert hip.nqs_hip_multi_sync(m) == 0
+        for i, (p, q) in enumerate(zip(single, batch)):
+            info = SrInfo()
+            pre = r.params(p["a"])
+            assert hip.nqs_sr_step(ctypes.byref(p["cfg"]), p["Lx"], p["Ly"], p["a"], p["s"],
+                                   ctypes.byref(info)) == 0
+            d_single = r.params(p["a"]) - pre
+            d_multi = r.params(q["a"]) - pre
+            if step == 0:
+                # identical inputs: identical samples and energies
+                assert out[i].mean_energy == info.mean_energy, (step, i)
+                assert out[i].acceptance_ratio == info.acceptance_ratio
+                assert abs(out[i].cg_iterations - info.cg_iterations) <= 1
+            else:
+                assert abs(out[i].mean_energy - info.mean_energy) <= 1e-9 * abs(info.mean_energy)
+            assert np.linalg.norm(d_multi - d_single) <= 1e-4 * np.linalg.norm(d_single), (step, i)
+            assert abs(out[i].update_norm - info.update_norm) <= 1e-4 * info.update_norm
+            # keep the clones on the same parameters
+            raw = hip.nqs_ansatz_params_raw(q["a"])
+            arr = np.ctypeslib.as_array(ctypes.cast(raw, ctypes.POINTER(ctypes.c_double)),
+                                        (len(pre),))
+            assert np.max(np.abs(arr - r.params(p["a"]))) <= 1e-4 * np.abs(d_single).max() + 1e-15
+    targets = (ctypes.c_int * 8)()
+    us = (ctypes.c_double * 8)()
+    chosen = hip.nqs_hip_multi_tuning(m, targets, us, 8)
+    assert chosen in list(targets)[:6] and all(x > 0 for x in list(us)[:6])
+    hip.nqs_hip_multi_free(m)
+    for p in single + batch:
+        hip.nqs_sampler_free(p["s"])
+        hip.nqs_ansatz_free(p["a"])
+
+
+def test_multi_rejects_unsupported(hip):
+    r = scenarios.Runner(hip, "nqs_")
+    cfg = config_defaults(ansatz=6, hamiltonian=0, num_samples=64)
+    a = r.ansatz(cfg, 8)
+    fn, user = r.callback(a, None)
+    s = hip.nqs_sampler_create(8, ctypes.byref(cfg), fn, user)
+    cfgs = (type(cfg) * 1)(cfg)
+    lx, ly = (ctypes.c_int * 1)(8), (ctypes.c_int * 1)(1)
+    av, sv = (ctypes.c_void_p * 1)(a), (ctypes.c_void_p * 1)(s)
+    assert not hip.nqs_hip_multi_create(1, cfgs, lx, ly, av, sv)   # complex RBM
+    hip.nqs_sampler_free(s)
+    hip.nqs_ansatz_free(a)
+
+
+def test_multi_tfim_chains_reach_closed_form(hip):
+    rng = np.random.default_rng(0xC4)
+    sizes = [16, 24, 32, 48, 64]
+    hs = rng.uniform(0.5, 1.5, len(sizes))
+    print("multi convergence:", list(zip(sizes, np.round(hs, 3))), flush=True)
+    r = scenarios.Runner(hip, "nqs_")
+    probs = [_tfim_problem(hip, r, N, h, 256, 2048, 100 + N, lr=0.02) for N, h in zip(sizes, hs)]
+    m = _make_batch(hip, probs)
+    iters = 150
+    tr = np.zeros((len(probs), iters))
+    assert hip.nqs_hip_multi_run(m, iters, ptr(tr)) == 0
+    hip.nqs_hip_multi_free(m)
+    for i, p in enumerate(probs):
+        e0 = tfim_pbc_closed_form(p["N"], p["h"])
+        tail = tr[i, -20:].mean()
+        rel = (tail - e0) / abs(e0)
+        print(f"N={p['N']} h={p['h']:.3f}: E_tail={tail:.5f} E0={e0:.5f} rel={rel:.2e}", flush=True)
+        assert -2e-3 < rel < 1e-2, (p["N"], p["h"], tail, e0)
+        hip.nqs_sampler_free(p["s"])
+        hip.nqs_ansatz_free(p["a"])
