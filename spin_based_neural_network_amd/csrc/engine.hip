@@ -947,7 +947,8 @@ int learned_rounds(nqs_sampler* s, int rounds, int n_out, int* d_out) {
     b.out = d_out;
     b.lacc = s->lacc.as<unsigned long long>();
     CKR(nqs::launch_ar_propose(s->ar->dev, s->cs, b, g_stream));
-    CKR(forward_t<R>(s->src.a, b.spins, (int)nprop, b.lp, nullptr));
+    // log|psi| only (no tanh theta)
+    CKR(nqs::launch_forward<R>(mv, b.spins, (int)nprop, nullptr, nullptr, b.lp, nullptr, g_stream));
     CKR(nqs::launch_ar_accept_rounds(s->cs, b, g_stream));
   } else {
     nqs::MHLaunch ml;

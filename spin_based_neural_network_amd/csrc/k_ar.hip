@@ -20,7 +20,7 @@
 //             proposals are drawn in one launch, evaluated by one batched
 //             amplitude pass, and accepted round by round per chain.
 // k_ar_grad   forward + backward of the mean negative log-likelihood over a
-//             minibatch; kArGradRows rows per single-wave block, dL/dl in
+//             minibatch; ar_grad_rows() rows per single-wave block, dL/dl in
 //             FP32, weight gradients as MFMA products over the rows, partial
 //             sums per block.
 // k_ar_adam   sums the partials in block order (deterministic), Adam on the
@@ -466,12 +466,12 @@ __global__ __launch_bounds__(64) void k_ar_grad(ArDev v, ArGrad g) {
   double loss = 0.0;
   __syncthreads();
 
-  for (int h = 0; h < kArGradRows / 32; h++) {
+  for (int h = 0; h < g.rows_per_block / 32; h++) {
     uint64_t rbits[2];
     bool rvalid[2];
 #pragma unroll
     for (int tile = 0; tile < 2; tile++) {
-      const long mrow = (long)blockIdx.x * kArGradRows + h * 32 + tile * 16 + rl;
+      const long mrow = (long)blockIdx.x * g.rows_per_block + h * 32 + tile * 16 + rl;
       rvalid[tile] = mrow < g.count;
       rbits[tile] = rvalid[tile] ? g.rows[(g.start + mrow) % g.n_avail] : 0ULL;
     }
@@ -789,9 +789,11 @@ int launch_spins_to_bits(const int* spins, long n, int N, uint64_t* out, hipStre
   return 0;
 }
 
-int launch_ar_grad(const ArDev& v, const ArGrad& g, hipStream_t st) {
-  if (g.count <= 0 || g.n_avail <= 0) return 0;
+int launch_ar_grad(const ArDev& v, const ArGrad& g0, hipStream_t st) {
+  if (g0.count <= 0 || g0.n_avail <= 0) return 0;
   if (v.N < 1 || v.N > ar::kMaxN) return -1;
+  ArGrad g = g0;
+  g.rows_per_block = ar_grad_rows(g.count);
   const int nt = (v.N + 15) / 16;
   const dim3 grid((unsigned)ar_grad_blocks(g.count)), blk(64);
   KScope ks("ar_grad", st);

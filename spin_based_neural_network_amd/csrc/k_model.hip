@@ -103,28 +103,34 @@ void free_model(ModelBuffers<R>& mb) {
 }
 
 // --------------------------------------------------------------------------
-// forward pass: one 256-thread block per configuration
+// forward pass: one wave per configuration (4 per 256-thread block), lane h
+// owns hidden units h, h + 64, ...; the sums over hidden units are xor
+// butterflies inside the wave, so there is no block barrier.
 // --------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
 template <typename R, bool CPLX>
 __global__ __launch_bounds__(256) void k_forward(ModelView<R> m, const int* __restrict__ spins,
                                                  int B, R* __restrict__ t_re,
                                                  R* __restrict__ t_im, double* la,
                                                  double* ar) {
-  __shared__ R s_s[kMaxSites];
-  __shared__ double red[256];
   const int N = m.N, M = m.M;
-  for (int b = blockIdx.x; b < B; b += gridDim.x) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int b = blockIdx.x * 4 + wave; b < B; b += gridDim.x * 4) {
     const int* sp = spins + (size_t)b * N;
-    for (int i = threadIdx.x; i < N; i += blockDim.x) s_s[i] = (R)sp[i];
-    __syncthreads();
     double lsum = 0.0, asum = 0.0;
-    for (int h = threadIdx.x; h < M; h += blockDim.x) {
+    for (int h = lane; h < M; h += 64) {
       // theta_h = b_h + sum_i W_hi s_i, accumulated in site order as in
       // nqs_ansatz.c:287-290 (s = +-1 makes every product exact)
       R xr = m.b_re[h];
       R xi = CPLX ? m.b_im[h] : (R)0;
       for (int i = 0; i < N; i++) {
-        const R s = s_s[i];
+        const R s = (R)sp[i];
         xr += m.WT_re[(size_t)i * M + h] * s;
         if (CPLX) xi += m.WT_im[(size_t)i * M + h] * s;
       }
@@ -152,18 +158,17 @@ __global__ __launch_bounds__(256) void k_forward(ModelView<R> m, const int* __re
     }
     if (la) {
       // visible term sum_i a_i s_i (nqs_ansatz.c:284-285, 333-336)
-      for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        lsum += (double)m.a_re[i] * (double)s_s[i];
-        if (CPLX) asum += (double)m.a_im[i] * (double)s_s[i];
+      for (int i = lane; i < N; i += 64) {
+        lsum += (double)m.a_re[i] * (double)sp[i];
+        if (CPLX) asum += (double)m.a_im[i] * (double)sp[i];
       }
-      double L = block_sum(lsum, red);
-      double A = CPLX ? block_sum(asum, red) : 0.0;
-      if (threadIdx.x == 0) {
+      const double L = wave_sum(lsum);
+      const double A = CPLX ? wave_sum(asum) : 0.0;
+      if (lane == 0) {
         la[b] = L;
         if (ar) ar[b] = A;
       }
     }
-    __syncthreads();
   }
 }
 
@@ -172,7 +177,7 @@ int launch_forward(const ModelView<R>& m, const int* spins, int B, R* t_re, R* t
                    double* la, double* ar, hipStream_t st) {
   if (B <= 0) return 0;
   if (m.N > kMaxSites) return -1;
-  int grid = B < 65535 ? B : 65535;
+  const int grid = (B + 3) / 4 < 65535 ? (B + 3) / 4 : 65535;
   KScope ks("forward", st);
   if (m.cplx)
     hipLaunchKernelGGL((k_forward<R, true>), dim3(grid), dim3(256), 0, st, m, spins, B,

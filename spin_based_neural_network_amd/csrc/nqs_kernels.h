@@ -124,14 +124,23 @@ struct ArGrad {
   const uint64_t* rows = nullptr;
   long n_avail = 0, start = 0;
   int count = 0, constrained = 1;
+  int rows_per_block = 32;   // set by launch_ar_grad
   float scale = 1.f;
   float* partial = nullptr;
   double* loss = nullptr;
 };
-// rows of a training batch per k_ar_grad block (a multiple of 32); each
-// block writes one partial gradient of P floats
-constexpr int kArGradRows = 32;
-inline int ar_grad_blocks(long count) { return (int)((count + kArGradRows - 1) / kArGradRows); }
+// k_ar_grad blocks: 32 rows per block up to kArGradMaxBlocks blocks, then
+// more rows per block (a multiple of 32); each block writes one partial
+// gradient of P floats, which the Adam kernel sums
+constexpr int kArGradMaxBlocks = 32;
+inline int ar_grad_rows(long count) {
+  const long per = (count + kArGradMaxBlocks - 1) / kArGradMaxBlocks;
+  return (int)(per <= 32 ? 32 : (per + 31) / 32 * 32);
+}
+inline int ar_grad_blocks(long count) {
+  const int r = ar_grad_rows(count);
+  return (int)((count + r - 1) / r);
+}
 struct ArAdam {
   const float* partial = nullptr;
   int nblk = 0;
