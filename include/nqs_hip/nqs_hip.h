@@ -67,9 +67,9 @@ typedef struct {
     int ar_hidden;       /* hidden width of the masked MLP (64; 0 = 64)        */
     int ar_train_every;  /* retrain every k batches (default 10)               */
     int ar_train_window; /* samples kept for training (default 16384)          */
-    int ar_epochs;       /* passes over the window per retrain (default 4)     */
-    int ar_batch;        /* minibatch size (default 256)                       */
-    double ar_lr;        /* Adam learning rate (default 1e-3)                  */
+    int ar_epochs;       /* passes over the window per retrain (default 2)     */
+    int ar_batch;        /* minibatch size (default 2048)                      */
+    double ar_lr;        /* Adam learning rate (default 3e-3)                  */
     int ar_local;        /* exchange moves before each learned move (0)        */
     unsigned ar_seed;    /* network initialisation seed (0: chain 0's seed)    */
 } nqs_hip_options_t;

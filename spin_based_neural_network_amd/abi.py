@@ -95,8 +95,8 @@ def options_defaults(**overrides):
     """nqs_hip_options_defaults() + overrides (same values as the C side)."""
     o = HipOptions(num_chains=1, move=MOVE_FLIP, sz_zero_start=0, periodic=0,
                    precision=FP64, gram=GRAM_DIRECT, ar_hidden=64,
-                   ar_train_every=10, ar_train_window=16384, ar_epochs=4,
-                   ar_batch=256, ar_lr=1e-3, ar_local=0, ar_seed=0)
+                   ar_train_every=10, ar_train_window=16384, ar_epochs=2,
+                   ar_batch=2048, ar_lr=3e-3, ar_local=0, ar_seed=0)
     for k, v in overrides.items():
         if not hasattr(o, k):
             raise AttributeError(f"nqs_hip_options_t has no field {k!r}")

@@ -1144,9 +1144,9 @@ NQS_EXPORT nqs_hip_options_t nqs_hip_options_defaults(void) {
   o.ar_hidden = 64;
   o.ar_train_every = 10;
   o.ar_train_window = 16384;
-  o.ar_epochs = 4;
-  o.ar_batch = 256;
-  o.ar_lr = 1e-3;
+  o.ar_epochs = 2;
+  o.ar_batch = 2048;
+  o.ar_lr = 3e-3;
   o.ar_local = 0;
   o.ar_seed = 0;
   return o;

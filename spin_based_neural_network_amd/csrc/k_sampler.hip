@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void k_mh(ModelView<R> m, ChainState cs, MHLau
     for (int stp = 0; stp < ml.steps; stp++) {
       int i, j = -1;
       if (ml.move == 0) {
-        i = (int)(xs_next(rng) % (uint64_t)N);
+        i = mod_small(xs_next(rng), (uint32_t)N);
       } else {
         int nd = 0;
         for (int w = 0; w < nw; w++) {
@@ -106,8 +106,8 @@ __global__ __launch_bounds__(256) void k_mh(ModelView<R> m, ChainState cs, MHLau
         }
         const int nu = N - nd;
         if (nu == 0 || nd == 0) { ntot++; continue; }
-        const int k = (int)(xs_next(rng) % (uint64_t)nu);
-        const int l = (int)(xs_next(rng) % (uint64_t)nd);
+        const int k = mod_small(xs_next(rng), (uint32_t)nu);
+        const int l = mod_small(xs_next(rng), (uint32_t)nd);
         i = select_site(sp, nw, N, 0, k);   // an up site   (s_i = +1)
         j = select_site(sp, nw, N, 1, l);   // a down site  (s_j = -1)
       }
@@ -253,7 +253,7 @@ __global__ void k_chain_init(ChainState cs, const uint64_t* __restrict__ seeds, 
     // bit pattern of the ordered start: sites >= N/2 are down
     for (int i = N / 2; i < N; i++) sp[i >> 6] |= 1ULL << (i & 63);
     for (int i = N - 1; i > 0; i--) {
-      const int j = (int)(xs_next(rng) % (uint64_t)(i + 1));
+      const int j = mod_small(xs_next(rng), (uint32_t)(i + 1));
       const int bi = (int)((sp[i >> 6] >> (i & 63)) & 1ULL);
       const int bj = (int)((sp[j >> 6] >> (j & 63)) & 1ULL);
       if (bi != bj) {

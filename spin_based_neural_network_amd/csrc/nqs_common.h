@@ -36,6 +36,17 @@ __host__ __device__ inline uint64_t xs_next(uint64_t& s) {
   s = x;
   return x;
 }
+// x mod n for 0 < n < 2^21, exact: (x_hi mod n) 2^32 + x_lo < 2^53 is
+// reduced in double precision and corrected by one step
+__host__ __device__ inline int mod_small(uint64_t x, uint32_t n) {
+  const uint32_t rh = (uint32_t)(x >> 32) % n;
+  const uint64_t t = ((uint64_t)rh << 32) | (uint32_t)x;
+  const int64_t q = (int64_t)((double)t / (double)n);
+  int64_t r = (int64_t)t - q * (int64_t)n;
+  if (r < 0) r += n;
+  else if (r >= (int64_t)n) r -= n;
+  return (int)r;
+}
 __host__ __device__ inline double xs_uniform(uint64_t& s) {
   return (double)(xs_next(s) >> 11) * (1.0 / 9007199254740992.0);
 }
@@ -81,17 +92,25 @@ __device__ inline cplx<R> cmul(cplx<R> a, cplx<R> b) {
 }
 
 // tanh of a complex number, (sinh 2x + i sin 2y) / (cosh 2x + cos 2y),
-// saturated to +-1 for |x| > 20 where the quotient would overflow.
+// saturated to +-1 for |x| > 20.  Numerator and denominator are scaled by
+// 2e, e = exp(-2|x|) <= 1: tanh = (sgn(x)(1 - e^2) + 2i e sin 2y) /
+// (1 + e^2 + 2e cos 2y), with 1 - e^2 = -m(2 + m), m = expm1(-2|x|), exact
+// for small |x| (one exponential and one sincos instead of cosh, sinh, cos
+// and sin).
 __device__ inline void ctanh_d(double x, double y, double& tr, double& ti) {
   if (fabs(x) > 20.0) {
     tr = x > 0 ? 1.0 : -1.0;
     ti = 0.0;
     return;
   }
-  double den = cosh(2.0 * x) + cos(2.0 * y);
+  const double m = expm1(-2.0 * fabs(x)), e = 1.0 + m;
+  double s2, c2;
+  sincos(2.0 * y, &s2, &c2);
+  double den = 1.0 + e * e + 2.0 * e * c2;
   if (fabs(den) < 1e-300) den = 1e-300;
-  tr = sinh(2.0 * x) / den;
-  ti = sin(2.0 * y) / den;
+  const double r = 1.0 / den;
+  tr = (x >= 0.0 ? -m : m) * (2.0 + m) * r;
+  ti = 2.0 * e * s2 * r;
 }
 __device__ inline void ctanh_f(float x, float y, float& tr, float& ti) {
   if (fabsf(x) > 10.0f) {
