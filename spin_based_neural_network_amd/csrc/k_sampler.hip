@@ -112,10 +112,15 @@ __global__ __launch_bounds__(256) void k_mh(ModelView<R> m, ChainState cs, MHLau
         j = select_site(sp, nw, N, 1, l);   // a down site  (s_j = -1)
       }
       const R si = bit_of(sp, i) ? (R)-1 : (R)1;
-      // hidden factors of this lane
+      // hidden factors of this lane; cosh d, sinh d and w = cosh d + t sinh d
+      // of each unit are kept for the update of t on acceptance
       R pr = 1, pi = 0;
+      R kchr[HPL], kchi[HPL], kshr[HPL], kshi[HPL], kwr[HPL], kwi[HPL];
 #pragma unroll
       for (int q = 0; q < HPL; q++) {
+        kchr[q] = 1;
+        kchi[q] = kshr[q] = kshi[q] = kwi[q] = 0;
+        kwr[q] = 1;
         const int h = lane + L * q;
         if (h < M) {
           const size_t ei = (size_t)i * M + h;
@@ -142,6 +147,8 @@ __global__ __launch_bounds__(256) void k_mh(ModelView<R> m, ChainState cs, MHLau
               shr = -si * (sir * cjr - cir * sjr);
             }
           }
+          kchr[q] = chr;
+          kshr[q] = shr;
           if (CPLX) {
             // w = cosh d + t sinh d
             const R wr = chr + (tr[q] * shr - ti[q] * shi);
@@ -149,8 +156,14 @@ __global__ __launch_bounds__(256) void k_mh(ModelView<R> m, ChainState cs, MHLau
             const R nr = pr * wr - pi * wi;
             pi = pr * wi + pi * wr;
             pr = nr;
+            kchi[q] = chi;
+            kshi[q] = shi;
+            kwr[q] = wr;
+            kwi[q] = wi;
           } else {
-            pr *= chr + tr[q] * shr;
+            const R wr = chr + tr[q] * shr;
+            pr *= wr;
+            kwr[q] = wr;
           }
         }
       }
@@ -172,20 +185,22 @@ __global__ __launch_bounds__(256) void k_mh(ModelView<R> m, ChainState cs, MHLau
         off = 0.0;
         sp[i >> 6] ^= 1ULL << (i & 63);
         if (j >= 0) sp[j >> 6] ^= 1ULL << (j & 63);
+        // tanh(theta + d) = (t cosh d + sinh d) / (cosh d + t sinh d): the
+        // denominator is the unit's factor w of the accepted proposal
+        // (w != 0: a zero factor gives d = -inf, never accepted)
 #pragma unroll
         for (int q = 0; q < HPL; q++) {
           const int h = lane + L * q;
           if (h < M) {
-            const size_t ei = (size_t)i * M + h;
-            thr[q] += (R)-2 * si * m.WT_re[ei];
-            if (CPLX) thi[q] += (R)-2 * si * m.WT_im[ei];
-            if (j >= 0) {
-              const size_t ej = (size_t)j * M + h;
-              thr[q] += (R)2 * si * m.WT_re[ej];
-              if (CPLX) thi[q] += (R)2 * si * m.WT_im[ej];
+            if (CPLX) {
+              const R nr = (tr[q] * kchr[q] - ti[q] * kchi[q]) + kshr[q];
+              const R ni = (tr[q] * kchi[q] + ti[q] * kchr[q]) + kshi[q];
+              const R inv = (R)1 / (kwr[q] * kwr[q] + kwi[q] * kwi[q]);
+              tr[q] = (nr * kwr[q] + ni * kwi[q]) * inv;
+              ti[q] = (ni * kwr[q] - nr * kwi[q]) * inv;
+            } else {
+              tr[q] = (tr[q] * kchr[q] + kshr[q]) / kwr[q];
             }
-            if (CPLX) ctanh<R>(thr[q], thi[q], tr[q], ti[q]);
-            else tr[q] = tanh(thr[q]);
           }
         }
       }
