@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <mutex>
 #include <string>
@@ -2132,6 +2133,11 @@ struct nqs_hip_multi {
   double* h_out = nullptr;              // pinned, kMultiOut per problem
   long long steps_done = 0;
   int tuned = 0, use_graph = 1;
+  // Metropolis jobs of all problems, grouped by kernel variant:
+  // (variant, first job, jobs, blocks)
+  std::vector<nqs::MHJob> jobs;
+  std::vector<std::array<int, 4>> groups;
+  DBuf d_jobs;
   ~nqs_hip_multi() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -2148,6 +2154,7 @@ struct nqs_hip_multi {
     }
     d_mp.release();
     d_out.release();
+    d_jobs.release();
   }
 };
 
@@ -2173,39 +2180,87 @@ int multi_set_split(nqs_hip_multi* m, int target) {
   return 0;
 }
 
-// per-problem sampling and energies on the side streams, joined into g_stream
-int multi_issue_front(nqs_hip_multi* m) {
+int multi_fork(nqs_hip_multi* m) {
   CK(hipEventRecord(m->ev_fork, g_stream));
   for (int k = 0; k < nqs_hip_multi::kSide; k++) CK(hipStreamWaitEvent(m->side[k], m->ev_fork, 0));
-  for (size_t i = 0; i < m->p.size(); i++) {
-    nqs_hip_multi::Prob& q = m->p[i];
-    hipStream_t st = m->side[i % nqs_hip_multi::kSide];
-    nqs_ansatz* a = q.a;
-    nqs_sampler* s = q.s;
-    CKR(nqs::launch_prep<double>(a->d_par.as<double>(), a->kind, a->N, a->M, a->m64, st));
-    const ModelView<double> v = a->m64.view;
-    nqs::MHLaunch ml;
-    ml.move = s->opt.move == NQS_HIP_MOVE_FLIP ? 0 : 1;
-    ml.steps = q.steps;
-    ml.rounds = q.rounds;
-    ml.record = 1;
-    ml.n_out = q.B;
-    ml.out = s->d_samples.as<int>();
-    CKR(nqs::launch_mh<double>(v, s->cs, ml, st));
-    const size_t tb = (size_t)q.B * a->M;
-    double* t = q.t.as<double>();
-    CKR(nqs::launch_forward<double>(v, ml.out, q.B, t, t + tb, nullptr, nullptr, st));
-    double* E = q.E.as<double>();
-    CKR(nqs::launch_eloc<double>(v, q.H, ml.out, q.B, t, t + tb, E, E + q.B, st));
-    double* stats = q.stats.as<double>();
-    CKR(nqs::launch_sum(E, q.B, nullptr, 1.0 / q.B, stats + 0, st));
-    CKR(nqs::launch_sum(E, q.B, E, 1.0 / q.B, stats + 1, st));
-  }
+  return 0;
+}
+
+int multi_join(nqs_hip_multi* m) {
   for (int k = 0; k < nqs_hip_multi::kSide; k++) {
     CK(hipEventRecord(m->ev_join[k], m->side[k]));
     CK(hipStreamWaitEvent(g_stream, m->ev_join[k], 0));
   }
   return 0;
+}
+
+// the Metropolis jobs of all problems (the tables' addresses are fixed once
+// the first preparation has allocated them)
+int multi_build_jobs(nqs_hip_multi* m) {
+  std::vector<std::pair<int, int>> order;   // (variant, problem)
+  for (size_t i = 0; i < m->p.size(); i++)
+    order.push_back({nqs::mh_variant(m->p[i].a->m64.view), (int)i});
+  std::stable_sort(order.begin(), order.end());
+  m->jobs.clear();
+  m->groups.clear();
+  for (size_t k = 0; k < order.size(); k++) {
+    const int v = order[k].first;
+    nqs_hip_multi::Prob& q = m->p[order[k].second];
+    nqs_sampler* s = q.s;
+    if (m->groups.empty() || m->groups.back()[0] != v) m->groups.push_back({v, (int)k, 0, 0});
+    std::array<int, 4>& g = m->groups.back();
+    nqs::MHJob jb;
+    jb.m = q.a->m64.view;
+    jb.cs = s->cs;
+    jb.ml.move = s->opt.move == NQS_HIP_MOVE_FLIP ? 0 : 1;
+    jb.ml.steps = q.steps;
+    jb.ml.rounds = q.rounds;
+    jb.ml.record = 1;
+    jb.ml.n_out = q.B;
+    jb.ml.out = s->d_samples.as<int>();
+    jb.blk0 = g[3];
+    g[2]++;
+    g[3] += nqs::mh_blocks(v, s->cs.C);
+    m->jobs.push_back(jb);
+  }
+  CKR(m->d_jobs.ensure(sizeof(nqs::MHJob) * m->jobs.size()));
+  CK(hipMemcpyAsync(m->d_jobs.p, m->jobs.data(), sizeof(nqs::MHJob) * m->jobs.size(),
+                    hipMemcpyHostToDevice, g_stream));
+  return 0;
+}
+
+// per-problem tables on the side streams; every problem's Metropolis batch
+// in one launch per kernel variant; per-problem forward, E_loc and energy
+// sums on the side streams, joined into g_stream
+int multi_issue_front(nqs_hip_multi* m) {
+  CKR(multi_fork(m));
+  for (size_t i = 0; i < m->p.size(); i++) {
+    nqs_ansatz* a = m->p[i].a;
+    CKR(nqs::launch_prep<double>(a->d_par.as<double>(), a->kind, a->N, a->M, a->m64,
+                                 m->side[i % nqs_hip_multi::kSide]));
+  }
+  CKR(multi_join(m));
+  if (m->jobs.empty()) CKR(multi_build_jobs(m));
+  const nqs::MHJob* dj = m->d_jobs.as<nqs::MHJob>();
+  for (const std::array<int, 4>& g : m->groups)
+    CKR(nqs::launch_mh_batch(g[0], dj + g[1], g[2], g[3], g_stream));
+  CKR(multi_fork(m));
+  for (size_t i = 0; i < m->p.size(); i++) {
+    nqs_hip_multi::Prob& q = m->p[i];
+    hipStream_t st = m->side[i % nqs_hip_multi::kSide];
+    nqs_ansatz* a = q.a;
+    const ModelView<double> v = a->m64.view;
+    const int* smp = q.s->d_samples.as<int>();
+    const size_t tb = (size_t)q.B * a->M;
+    double* t = q.t.as<double>();
+    CKR(nqs::launch_forward<double>(v, smp, q.B, t, t + tb, nullptr, nullptr, st));
+    double* E = q.E.as<double>();
+    CKR(nqs::launch_eloc<double>(v, q.H, smp, q.B, t, t + tb, E, E + q.B, st));
+    double* stats = q.stats.as<double>();
+    CKR(nqs::launch_sum(E, q.B, nullptr, 1.0 / q.B, stats + 0, st));
+    CKR(nqs::launch_sum(E, q.B, E, 1.0 / q.B, stats + 1, st));
+  }
+  return multi_join(m);
 }
 
 int multi_issue_back(nqs_hip_multi* m) {

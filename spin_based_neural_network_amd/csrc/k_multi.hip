@@ -165,17 +165,26 @@ __global__ __launch_bounds__(256) void k_mb_gemm_z(MultiBatch mb) {
   tile_nt(M1, q.B, 0, N1, q.p, N1, q.S1, N1, q.Zt, q.B, (t / tiles_n) * MT, (t % tiles_n) * MT);
 }
 
-// u_b = sum_h [t_b, 1]_h Zt[h][b];  TWt[h][b] = [t_b, 1]_h u_b
+// u_b = sum_h [t_b, 1]_h Zt[h][b];  TWt[h][b] = [t_b, 1]_h u_b.
+// Block: 64 samples x 4 groups of hidden rows (h = g, g + 4, ...); the four
+// group sums are added in group order, so every thread of a sample sees the
+// same u_b.
 __global__ __launch_bounds__(256) void k_mb_rows(MultiBatch mb) {
+  __shared__ double red[4][64];
   const int qi = find_prob<F_ROWS>(mb, blockIdx.x);
   const MultiProb& q = mb.probs[qi];
   if (cg_done(q)) return;
-  const int b = (blockIdx.x - q.blk_rows) * 256 + threadIdx.x;
-  if (b >= q.B) return;
+  const int bl = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int b = (blockIdx.x - q.blk_rows) * 64 + bl;
   const long B = q.B;
   double u = 0.0;
-  for (int h = 0; h <= q.M; h++) u += q.T1t[h * B + b] * q.Zt[h * B + b];
-  for (int h = 0; h <= q.M; h++) q.TWt[h * B + b] = q.T1t[h * B + b] * u;
+  if (b < q.B)
+    for (int h = g; h <= q.M; h += 4) u += q.T1t[h * B + b] * q.Zt[h * B + b];
+  red[g][bl] = u;
+  __syncthreads();
+  u = (red[0][bl] + red[1][bl]) + (red[2][bl] + red[3][bl]);
+  if (b < q.B)
+    for (int h = g; h <= q.M; h += 4) q.TWt[h * B + b] = q.T1t[h * B + b] * u;
 }
 
 // Gp[slot][split] = A[:, Kchunk] S1t[:, Kchunk]^T, (M+1) x (N+1).  With two
@@ -386,7 +395,7 @@ void multi_layout(MultiProb* probs, int n, MultiBatch& mb) {
     q.blk_vec = vec;
     prep += (q.B + 255) / 256;
     z += ((M1 + MT - 1) / MT) * ((q.B + MT - 1) / MT);
-    rows += (q.B + 255) / 256;
+    rows += (q.B + 63) / 64;
     const int gt = ((M1 + MT - 1) / MT) * ((N1 + MT - 1) / MT) * q.nsplit;
     g += gt;
     g2 += 2 * gt;

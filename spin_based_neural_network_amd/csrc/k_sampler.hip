@@ -36,11 +36,10 @@ __device__ inline int select_site(const uint64_t* sp, int nw, int N, int want, i
   return -1;
 }
 
+// chain c of one sampler, lane `lane` of its L lanes
 template <typename R, bool CPLX, int L, int HPL>
-__global__ __launch_bounds__(256) void k_mh(ModelView<R> m, ChainState cs, MHLaunch ml) {
-  const int lane = threadIdx.x & (L - 1);
-  const int c = (int)((blockIdx.x * (long)blockDim.x + threadIdx.x) / L);
-  if (c >= cs.C) return;
+__device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, const MHLaunch& ml,
+                                const int c, const int lane) {
   const int N = m.N, M = m.M, nw = cs.nw;
   uint64_t sp[4];
 #pragma unroll
@@ -223,6 +222,44 @@ __global__ __launch_bounds__(256) void k_mh(ModelView<R> m, ChainState cs, MHLau
 }
 
 template <typename R, bool CPLX, int L, int HPL>
+__global__ __launch_bounds__(256) void k_mh(ModelView<R> m, ChainState cs, MHLaunch ml) {
+  const int lane = threadIdx.x & (L - 1);
+  const int c = (int)((blockIdx.x * (long)blockDim.x + threadIdx.x) / L);
+  if (c >= cs.C) return;
+  mh_chain<R, CPLX, L, HPL>(m, cs, ml, c, lane);
+}
+
+// several samplers of one variant in one launch: job j owns the blocks
+// [blk0_j, blk0_{j+1})
+// a block-uniform struct read word by word into scalar registers (so the
+// job's pointers do not occupy vector registers in the chain loop)
+template <typename T>
+__device__ inline T uniform_load(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "word-sized struct");
+  T out;
+  int* o = reinterpret_cast<int*>(&out);
+  const int* q = reinterpret_cast<const int*>(p);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); k++) o[k] = __builtin_amdgcn_readfirstlane(q[k]);
+  return out;
+}
+
+template <bool CPLX, int L, int HPL>
+__global__ __launch_bounds__(256) void k_mh_batch(const MHJob* __restrict__ jobs, int njobs) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (__builtin_amdgcn_readfirstlane(jobs[mid].blk0) <= (int)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const MHJob jb = uniform_load(jobs + lo);
+  const int lane = threadIdx.x & (L - 1);
+  const int c = (int)(((blockIdx.x - jb.blk0) * 256L + threadIdx.x) / L);
+  if (c >= jb.cs.C) return;
+  mh_chain<double, CPLX, L, HPL>(jb.m, jb.cs, jb.ml, c, lane);
+}
+
+template <typename R, bool CPLX, int L, int HPL>
 static void mh_go(const ModelView<R>& m, ChainState& cs, const MHLaunch& ml, hipStream_t st) {
   const long threads = (long)cs.C * L;
   const int blocks = (int)((threads + 255) / 256);
@@ -244,6 +281,33 @@ int launch_mh(const ModelView<R>& m, ChainState& cs, const MHLaunch& ml, hipStre
     if (c) mh_go<R, true, 64, 4>(m, cs, ml, st); else mh_go<R, false, 64, 4>(m, cs, ml, st);
   } else {
     if (c) mh_go<R, true, 64, 16>(m, cs, ml, st); else mh_go<R, false, 64, 16>(m, cs, ml, st);
+  }
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int mh_variant(const ModelView<double>& m) {
+  const int c = m.cplx != 0;
+  return (m.M <= 64 ? 0 : (m.M <= 256 ? 2 : 4)) + c;
+}
+
+int mh_blocks(int variant, int C) {
+  const int L = variant < 2 ? 16 : 64;
+  return (int)(((long)C * L + 255) / 256);
+}
+
+int launch_mh_batch(int variant, const MHJob* jobs, int njobs, int nblocks, hipStream_t st) {
+  if (njobs <= 0 || nblocks <= 0) return 0;
+  KScope ks("mh", st);
+  const dim3 g((unsigned)nblocks), b(256);
+  switch (variant) {
+    case 0: hipLaunchKernelGGL((k_mh_batch<false, 16, 4>), g, b, 0, st, jobs, njobs); break;
+    case 1: hipLaunchKernelGGL((k_mh_batch<true, 16, 4>), g, b, 0, st, jobs, njobs); break;
+    case 2: hipLaunchKernelGGL((k_mh_batch<false, 64, 4>), g, b, 0, st, jobs, njobs); break;
+    case 3: hipLaunchKernelGGL((k_mh_batch<true, 64, 4>), g, b, 0, st, jobs, njobs); break;
+    case 4: hipLaunchKernelGGL((k_mh_batch<false, 64, 16>), g, b, 0, st, jobs, njobs); break;
+    default: hipLaunchKernelGGL((k_mh_batch<true, 64, 16>), g, b, 0, st, jobs, njobs); break;
   }
   count_launch();
   NQS_HIP_CHECK(hipGetLastError());

@@ -91,6 +91,19 @@ struct MHLaunch {
 };
 template <typename R>
 int launch_mh(const ModelView<R>& m, ChainState& cs, const MHLaunch& L, hipStream_t st);
+// Several FP64 samplers in one launch.  A variant (0..5) fixes the lanes per
+// chain and hidden units per lane (by M) and real / complex; every job of a
+// launch has the same variant, jobs[j].blk0 is its first block (jobs in
+// block order, mh_blocks(variant, C) blocks each), `jobs` is device memory.
+struct MHJob {
+  ModelView<double> m;
+  ChainState cs;
+  MHLaunch ml;
+  int blk0 = 0;
+};
+int mh_variant(const ModelView<double>& m);
+int mh_blocks(int variant, int C);
+int launch_mh_batch(int variant, const MHJob* jobs, int njobs, int nblocks, hipStream_t st);
 int launch_chain_init(ChainState& cs, const uint64_t* seeds_dev, int sz_zero, hipStream_t st);
 int launch_unpack(const ChainState& cs, int chain, int* out, hipStream_t st);
 // all chains' configurations as int32 rows [C][N]
