@@ -348,6 +348,35 @@ def multi_gemm_flops(tag, inst, B, iters):
     return tot
 
 
+def multi_nsplit(N, B, target):
+    """Split factor of the batched G product (the rule of the library's
+    autotuned target: tiles x splits <= target, splits <= min(32, B/64))."""
+    M1, N1 = 2 * N + 1, N + 1
+    tiles = -(-M1 // 64) * -(-N1 // 64)
+    max_split = 1
+    while max_split * 2 <= 32 and max_split * 2 * 64 <= B:
+        max_split *= 2
+    ns = 1
+    while ns * 2 <= max_split and tiles * ns * 2 <= target:
+        ns *= 2
+    return ns
+
+
+def multi_bytes(tag, inst, B, target):
+    """Algorithmic HBM bytes of one launch of a batched vector tag over all
+    problems: multi_vec = finish (split partials, Om, p in; Ap out), update
+    (x, p, r, Ap in; x, r out) and direction (r, p in; p out) of one CG
+    iteration; multi_rows = [t, 1], Zt in, weighted [t, 1] out."""
+    tot = 0
+    for N, _ in inst:
+        M1, N1 = 2 * N + 1, N + 1
+        if tag == "multi_vec":
+            tot += (multi_nsplit(N, B, target) + 12) * M1 * N1 * 8
+        else:
+            tot += 3 * M1 * B * 8
+    return tot
+
+
 def gpu_run_multi(args, w):
     lib = abi.load_library()
     inst = multi_instances(w)
@@ -441,6 +470,9 @@ def gpu_run_multi(args, w):
     if dom in ("multi_gemm_g", "multi_gemm_z"):
         work = multi_gemm_flops(dom, inst, B, iters_prof) / c
         achieved, peak, runit, bound = work / (avg_ms * 1e-3) / 1e12, PEAK_FP64 / 1e12, "TFLOP/s", "mfma"
+    elif dom in ("multi_vec", "multi_rows"):
+        work = multi_bytes(dom, inst, B, chosen)
+        achieved, peak, runit, bound = work / (avg_ms * 1e-3) / 1e9, PEAK_HBM / 1e9, "GB/s", "hbm"
     else:
         # bytes the kernel must move: spins in, samples / energies out
         work = sum(B * N * 4 for N, _ in inst) / c

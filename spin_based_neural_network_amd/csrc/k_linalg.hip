@@ -16,6 +16,14 @@ namespace nqs {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// lane l's value of v in every lane (l wave-uniform)
+__device__ inline double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // --------------------------------------------------------------------------
 // column sums  out[k] = alpha * sum_b O[b][k] w[b]   (two fixed-order stages)
 // --------------------------------------------------------------------------
@@ -534,11 +542,12 @@ size_t potrf_linv_elems(int n) { return 2 * (size_t)((n + NB - 1) / NB) * NB * N
 // kp >= 0: columns [kp, kp + 64) left of the panel are final L whose rank-64
 // update the panel has not yet received; it is applied here.
 //   1. D = A_kk - Lp_d Lp_d^T and R = A_rows - Lp_r Lp_d^T   (MFMA)
-//   2. D = L L^T: wave q keeps columns 16q..16q+15 of row `lane` in
-//      registers; per column one barrier: the owning wave publishes the
-//      column through LDS (double-buffered), every wave reads the pivot and
-//      applies the rank-1 update to its columns
-//   3. Dinv = L^-1 (wave 0, lane c solves L x = e_c)
+//   2. D = L L^T by 16-column blocks: wave q keeps columns 16q..16q+15 of
+//      row `lane` in registers; the owning wave factors a block column with
+//      readlane broadcasts, one barrier per block column, later waves apply
+//      the rank-16 update to their columns
+//   3. Dinv = L^-1 by 16 x 16 blocks (diagonal blocks on the four waves,
+//      then the off-diagonal blocks one block diagonal at a time)
 //   4. rows = R Dinv^T                                           (MFMA)
 // A non-positive pivot at column j: the factored columns < j of the block
 // and the updated, unsolved rows are stored, and k0 + j + 1 is reported.
@@ -596,38 +605,58 @@ __global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb,
       }
   }
   __syncthreads();
-  // step 2: factor D
+  // step 2: factor D by blocks of 16 columns.  Wave jb factors its own 16
+  // columns with the rows in its lanes (pivots and column values read with
+  // readlane, no barrier inside the block column; only 1/pivot is on the
+  // column chain, L_ij = a_ij / sqrt(d) is formed after the update), then
+  // publishes them through D, and every later wave subtracts the rank-16
+  // product from its own columns.
+  __shared__ int s_fail;
   double rr[16];
 #pragma unroll
   for (int t = 0; t < 16; t++) rr[t] = D[lane][16 * wave + t];
   int jdone = nb;
+  for (int jb = 0; jb < 4 && 16 * jb < nb; jb++) {
+    if (wave == jb) {
+      int jf = nb;
 #pragma unroll
-  for (int j = 0; j < NB; j++) {
-    if (j < nb && jdone == nb) {
-      const int jb = j >> 4, jj = j & 15;
-      if (wave == jb) col[j & 1][lane] = rr[jj];
-      __syncthreads();
-      const double d = col[j & 1][j];
-      if (!(d > 0.0)) {
-        jdone = j;
-      } else {
-        // only 1/d is on the column-to-column chain: the update is
-        // a_ic -= (a_ij / d) a_cj; L_ij = a_ij / sqrt(d) is formed after it
-        const double ci = col[j & 1][lane];
-        const double rd = 1.0 / d;
-        const double si = (lane > j) ? ci * rd : 0.0;
-        if (16 * wave + 15 > j) {
+      for (int jj = 0; jj < 16; jj++) {
+        const int j = 16 * jb + jj;
+        if (j < nb && jf == nb) {
+          const double d = readlane_d(rr[jj], j);
+          if (!(d > 0.0)) {
+            jf = j;
+          } else {
+            const double ci = rr[jj];
+            const double si = (lane > j) ? ci * (1.0 / d) : 0.0;
 #pragma unroll
-          for (int t = 0; t < 16; t++) {
-            const int c = 16 * wave + t;
-            const double lc = (c > j) ? col[j & 1][c] : 0.0;
-            rr[t] -= si * lc;
+            for (int t = jj + 1; t < 16; t++) rr[t] -= si * readlane_d(ci, 16 * jb + t);
+            const double Ljj = sqrt(d);
+            rr[jj] = (lane == j) ? Ljj : ((lane > j) ? ci / Ljj : ci);
           }
         }
-        if (wave == jb) {
-          const double Ljj = sqrt(d);
-          rr[jj] = (lane == j) ? Ljj : ((lane > j) ? ci / Ljj : rr[jj]);
+      }
+#pragma unroll
+      for (int t = 0; t < 16; t++) D[lane][16 * jb + t] = rr[t];
+      if (lane == 0) s_fail = jf;
+    }
+    __syncthreads();
+    jdone = s_fail;
+    if (jdone < nb) break;
+    if (wave > jb) {
+      double lp[16];
+#pragma unroll
+      for (int jj = 0; jj < 16; jj++) lp[jj] = D[lane][16 * jb + jj];
+#pragma unroll
+      for (int t = 0; t < 16; t++) {
+        const int c = 16 * wave + t;
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < 16; jj += 2) {
+          a0 += lp[jj] * D[c][16 * jb + jj];
+          a1 += lp[jj + 1] * D[c][16 * jb + jj + 1];
         }
+        rr[t] -= a0 + a1;
       }
     }
   }
@@ -649,27 +678,61 @@ __global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb,
     }
     return;
   }
-  // step 3: inverse of L (wave 0, reciprocal pivots precomputed, each dot
-  // product split over 8 accumulators); identity rows beyond nb give identity
+  // step 3: X = L^-1 by 16 x 16 blocks.  (a) wave w inverts diagonal block
+  // w (lane c < 16 solves L_ww x = e_c with reciprocal pivots); (b) one block
+  // diagonal at a time, X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj, the blocks of
+  // a diagonal on separate waves (T_ij staged in LDS).  Rows beyond nb are
+  // identity rows of L, so their part of X is the identity.
+  __shared__ double Tb[3][16][17];
+  for (int e = tid; e < NB * NB; e += 256) X[e >> 6][e & 63] = 0.0;
   if (tid < NB) col[0][tid] = 1.0 / D[tid][tid];
   __syncthreads();
-  if (wave == 0) {
-    double x[NB];
+  if (lane < 16) {
+    const int b0 = 16 * wave, c = lane;
+    double x[16];
 #pragma unroll
-    for (int q = 0; q < NB; q++) {
-      double p[8];
+    for (int q = 0; q < 16; q++) {
+      double p0 = (q == c) ? 1.0 : 0.0, p1 = 0.0;
 #pragma unroll
-      for (int u = 0; u < 8; u++) p[u] = 0.0;
-#pragma unroll
-      for (int k = 0; k < q; k++) p[k & 7] += D[q][k] * x[k];
-      const double sum = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7])) +
-                         ((q == lane) ? -1.0 : 0.0);
-      x[q] = -sum * col[0][q];
+      for (int k = 0; k < q; k += 2) {
+        p0 -= D[b0 + q][b0 + k] * x[k];
+        if (k + 1 < q) p1 -= D[b0 + q][b0 + k + 1] * x[k + 1];
+      }
+      x[q] = (p0 + p1) * col[0][b0 + q];
     }
 #pragma unroll
-    for (int q = 0; q < NB; q++) X[q][lane] = (q >= lane) ? x[q] : 0.0;
+    for (int q = 0; q < 16; q++) X[b0 + q][b0 + c] = x[q];
   }
   __syncthreads();
+  for (int dist = 1; dist < 4; dist++) {
+    const int bi = 16 * (wave + dist), bj = 16 * wave;   // block (wave + dist, wave)
+    const int r = lane >> 2, c0 = 4 * (lane & 3);
+    if (wave + dist < 4) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int m = bj; m < bi; m += 2) {
+          t0 += D[bi + r][m] * X[m][bj + c0 + u];
+          t1 += D[bi + r][m + 1] * X[m + 1][bj + c0 + u];
+        }
+        Tb[wave][r][c0 + u] = t0 + t1;
+      }
+    }
+    __syncthreads();
+    if (wave + dist < 4) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        double x0 = 0.0, x1 = 0.0;
+#pragma unroll
+        for (int m = 0; m < 16; m += 2) {
+          x0 -= X[bi + r][bi + m] * Tb[wave][m][c0 + u];
+          x1 -= X[bi + r][bi + m + 1] * Tb[wave][m + 1][c0 + u];
+        }
+        X[bi + r][bj + c0 + u] = x0 + x1;
+      }
+    }
+    __syncthreads();
+  }
   if (blockIdx.x == 0) {
     double* Xg = Linv + (size_t)(k0 / NB) * NB * NB;
     for (int e = tid; e < NB * NB; e += 256) {
