@@ -670,6 +670,7 @@ struct nqs_hip_ar {
   nqs::ArDev dev;
   long t = 0;                                        // Adam steps taken
   uint64_t shuffle = 0x243F6A8885A308D3ULL;          // epoch offsets
+  long loss_n = 0, loss_avail = 0;                   // last training's loss partials
   ~nqs_hip_ar() {
     DBuf* all[] = {&w, &m, &v, &op1f, &op2f, &op2b, &op3f, &op3b, &w1t, &b1, &b2, &b3,
                    &partial, &loss, &bits, &lq, &rng, &spins, &grad};
@@ -694,6 +695,7 @@ struct nqs_sampler {
   std::vector<uint64_t> hrng;
   long long hacc = 0, htot = 0;
   std::vector<int> h_cur;  // chain 0 configuration returned to callers
+  int cur_dirty = 0;       // h_cur is behind the device chains
   // last SR step bookkeeping
   std::vector<double> last_E;
   int last_rounds = 0;
@@ -706,7 +708,6 @@ struct nqs_sampler {
   long win_pos = 0, win_n = 0;
   int since_train = 0, trained = 0, n_train = 0;
   long long ar_props = 0;
-  double ar_loss = 0.0;
   ~nqs_sampler() {
     DBuf* all[] = {&d_spins, &d_rng, &d_acc, &d_tot, &d_la, &d_cur, &d_samples,
                    &win, &lacc, &prop_bits, &prop_spins, &lq, &la_prop};
@@ -795,7 +796,23 @@ nqs_hip_ar* ar_new(int N, uint64_t seed, double scale) {
   return a;
 }
 
-// Adam epochs over n_avail device bit rows; mean loss of the last epoch.
+// mean loss of the last epoch of the last training (read back on demand)
+int ar_last_loss(nqs_hip_ar* a, double* mean_loss) {
+  if (a->loss_n <= 0) {
+    *mean_loss = 0.0;
+    return 0;
+  }
+  std::vector<double> l((size_t)a->loss_n);
+  CK(hipMemcpyAsync(l.data(), a->loss.p, sizeof(double) * l.size(), hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  double tot = 0.0;
+  for (double x : l) tot += x;
+  *mean_loss = tot / (double)a->loss_avail;
+  return 0;
+}
+
+// Adam epochs over n_avail device bit rows; mean loss of the last epoch
+// into *mean_loss when it is given (a host synchronisation).
 int ar_train(nqs_hip_ar* a, const uint64_t* rows, long n_avail, int constrained, int epochs,
              int batch, double lr, double* mean_loss) {
   if (n_avail <= 0 || epochs <= 0) return 0;
@@ -829,12 +846,9 @@ int ar_train(nqs_hip_ar* a, const uint64_t* rows, long n_avail, int constrained,
       CKR(nqs::launch_ar_adam(a->dev, ad, g_stream));
     }
   }
-  std::vector<double> l((size_t)nb * steps);
-  CK(hipMemcpyAsync(l.data(), a->loss.p, sizeof(double) * l.size(), hipMemcpyDeviceToHost, g_stream));
-  CK(hipStreamSynchronize(g_stream));
-  double tot = 0.0;
-  for (double x : l) tot += x;
-  if (mean_loss) *mean_loss = tot / (double)n_avail;
+  a->loss_n = (long)nb * steps;
+  a->loss_avail = n_avail;
+  if (mean_loss) CKR(ar_last_loss(a, mean_loss));
   return 0;
 }
 
@@ -844,10 +858,8 @@ int ar_maybe_train(nqs_sampler* s) {
   const int every = std::max(1, s->opt.ar_train_every);
   const bool due = s->trained ? (s->since_train >= every) : (s->win_n > 0);
   if (!due) return 0;
-  double loss = 0.0;
   CKR(ar_train(s->ar, s->win.as<uint64_t>(), s->win_n, 1, s->opt.ar_epochs, s->opt.ar_batch,
-               s->opt.ar_lr, &loss));
-  s->ar_loss = loss;
+               s->opt.ar_lr, nullptr));
   s->trained = 1;
   s->since_train = 0;
   s->n_train++;
@@ -866,6 +878,7 @@ int ar_window_append(nqs_sampler* s, int n_out, const int* d_out) {
 }
 
 int sampler_sync_cur(nqs_sampler* s) {
+  s->cur_dirty = 0;
   if (!s->device) {
     memcpy(s->h_cur.data(), s->hc.data(), sizeof(int) * s->N);
     return 0;
@@ -1004,8 +1017,11 @@ int sampler_advance(nqs_sampler* s, int steps, int rounds, int n_out, int* d_out
     if (h_out && d_out) {
       CK(hipMemcpyAsync(h_out, d_out, sizeof(int) * (size_t)n_out * s->N, hipMemcpyDeviceToHost,
                         g_stream));
+      return sampler_sync_cur(s);
     }
-    return sampler_sync_cur(s);
+    // chain 0's configuration is fetched when a caller asks for it
+    s->cur_dirty = 1;
+    return 0;
   }
   const int C = s->opt.num_chains;
   for (int r = 0; r < rounds; r++) {
@@ -1199,7 +1215,7 @@ NQS_EXPORT const int* nqs_sampler_next(nqs_sampler_t* s) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (!s) return nullptr;
   const int k = s->cfg.num_decorrelate > 0 ? s->cfg.num_decorrelate : 1;
-  if (sampler_advance(s, k, 1, 0, nullptr, nullptr)) return nullptr;
+  if (sampler_advance(s, k, 1, 0, nullptr, nullptr) || sampler_sync_cur(s)) return nullptr;
   return s->h_cur.data();
 }
 
@@ -1229,7 +1245,11 @@ NQS_EXPORT double nqs_sampler_acceptance_ratio(const nqs_sampler_t* s) {
 }
 
 NQS_EXPORT const int* nqs_sampler_current(const nqs_sampler_t* s) {
-  return s ? s->h_cur.data() : nullptr;
+  if (!s) return nullptr;
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  nqs_sampler* m = const_cast<nqs_sampler*>(s);
+  if (m->cur_dirty && sampler_sync_cur(m)) return nullptr;
+  return m->h_cur.data();
 }
 
 NQS_EXPORT int nqs_sampler_num_sites(const nqs_sampler_t* s) { return s ? s->N : 0; }
@@ -2064,7 +2084,7 @@ NQS_EXPORT int nqs_hip_sampler_learned_stats(const nqs_sampler_t* s, double* acc
   if (acceptance) *acceptance = s->ar_props ? (double)t / (double)s->ar_props : 0.0;
   if (proposals) *proposals = s->ar_props;
   if (trainings) *trainings = s->n_train;
-  if (last_nll) *last_nll = s->ar_loss;
+  if (last_nll) CKR(ar_last_loss(s->ar, last_nll));
   return 0;
 }
 

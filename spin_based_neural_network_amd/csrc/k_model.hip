@@ -145,11 +145,20 @@ __global__ __launch_bounds__(256) void k_forward(ModelView<R> m, const int* __re
         }
       }
       if (la) {
-        if (CPLX) {
+        if (CPLX && ar) {
           double l1, a1;
           logcosh2_c((double)xr, (double)xi, l1, a1);
           lsum += l1;
           asum += a1;
+        } else if (CPLX) {
+          // log|2 cosh z| only: |x| + log|(1 + e) cos y + i sgn(x) (1 - e) sin y|,
+          // e = exp(-2|x|) (both parts are bounded by 2, no scaling needed)
+          const double ax = fabs((double)xr), e = exp(-2.0 * ax);
+          double sy, cy;
+          sincos((double)xi, &sy, &cy);
+          const double re = (1.0 + e) * cy, im = (1.0 - e) * sy;
+          const double m2 = re * re + im * im;
+          lsum += ax + (m2 > 0.0 ? 0.5 * log(m2) : -1e300);
         } else {
           double ax = fabs((double)xr);
           lsum += ax + log1p(exp(-2.0 * ax));
