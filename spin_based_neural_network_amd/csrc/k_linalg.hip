@@ -430,6 +430,21 @@ __global__ __launch_bounds__(512) void k_gemm_big(int M, int N, int K, double al
     for (int j = 0; j < 2; j++) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
   double ra[8], rb[8];
   const int nch = (K + BKC - 1) / BKC;
+  // C of an accumulating update is loaded before the K loop, so its latency
+  // hides behind the MFMAs
+  double cpre[4][2][4];
+  if (beta != 0.0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int gr = row0 + wr + 16 * i + lk + 4 * r;
+          const int gc = col0 + wc + 16 * j + lr;
+          cpre[i][j][r] = (gr < M && gc < N && (!mask || gc <= gr)) ? C[(long)gr * ldc + gc] : 0.0;
+        }
+  }
   auto gload = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 8; q++) {
@@ -486,7 +501,7 @@ __global__ __launch_bounds__(512) void k_gemm_big(int M, int N, int K, double al
         const int gc = col0 + wc + 16 * j + lr;
         if (gr < M && gc < N && (!mask || gc <= gr)) {
           double* cp = C + (long)gr * ldc + gc;
-          *cp = (beta == 0.0) ? alpha * acc[i][j][r] : alpha * acc[i][j][r] + beta * (*cp);
+          *cp = (beta == 0.0) ? alpha * acc[i][j][r] : alpha * acc[i][j][r] + beta * cpre[i][j][r];
         }
       }
 }
