@@ -940,96 +940,158 @@ int launch_potrf_repair(double* A, int n, int jf, const double* orig, hipStream_
 // launch per block and direction; every workgroup recomputes the block's
 // solution (64x64 product) and workgroup 0 publishes it
 // --------------------------------------------------------------------------
+// triangular solves L L^T x = b with the inverted diagonal blocks: one
+// launch per super-block of kTrsvG diagonal blocks and direction.  Every
+// workgroup solves the super-block block by block (a block's right-hand
+// side takes the earlier blocks' solutions as one dot product, then
+// x_g = Linv_g rhs_g), workgroup 0 publishes x, and each workgroup removes
+// the super-block's contribution from 256 rows below (forward) or columns
+// to the left (backward).
+// --------------------------------------------------------------------------
+constexpr int kTrsvG = 4;
+
 __global__ __launch_bounds__(256) void k_trsv_fwd(const double* L, const double* Linv, int n,
-                                                  int k0, int nb, double* w, double* x) {
+                                                  int k0, int ncol, double* w, double* x) {
   __shared__ double X[NB][NB + 1];
-  __shared__ double wb[NB], xb[NB];
-  const int tid = threadIdx.x;
-  const double* Xg = Linv + (size_t)(k0 / NB) * NB * NB;
-  {
+  __shared__ double xb[NB * kTrsvG], rb[NB], part[4][NB];
+  const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
+  xb[tid] = 0.0;
+  for (int g = 0; g * NB < ncol; g++) {
+    const int b0 = k0 + g * NB, nb = min(NB, ncol - g * NB);
+    const double* Xg = Linv + (size_t)(b0 / NB) * NB * NB;
     double v[16];
 #pragma unroll
-    for (int q = 0; q < 16; q++) v[q] = Xg[tid + 256 * q];
+    for (int e = 0; e < 16; e++) v[e] = Xg[tid + 256 * e];
 #pragma unroll
-    for (int q = 0; q < 16; q++) X[(tid + 256 * q) / NB][(tid + 256 * q) % NB] = v[q];
-  }
-  if (tid < NB) wb[tid] = tid < nb ? w[k0 + tid] : 0.0;
-  __syncthreads();
-  if (tid < NB) {
+    for (int e = 0; e < 16; e++) X[(tid + 256 * e) / NB][(tid + 256 * e) % NB] = v[e];
+    __syncthreads();
+    // earlier blocks of this super-block: 4 partial sums of 16 g terms
     double s = 0.0;
+    if (r < nb) {
+      const double* row = L + (long)(b0 + r) * n + k0;
+      for (int c0 = 16 * g * q; c0 < 16 * g * (q + 1); c0 += 16) {
+        double u[16];
 #pragma unroll
-    for (int c = 0; c < NB; c++) s += X[tid][c] * wb[c];
-    xb[tid] = s;
-    if (blockIdx.x == 0 && tid < nb) x[k0 + tid] = s;
+        for (int e = 0; e < 16; e++) u[e] = row[c0 + e];
+#pragma unroll
+        for (int e = 0; e < 16; e++) s += u[e] * xb[c0 + e];
+      }
+    }
+    part[q][r] = s;
+    __syncthreads();
+    if (tid < NB)
+      rb[tid] = tid < nb ? w[b0 + tid] - ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]))
+                         : 0.0;
+    __syncthreads();
+    if (tid < NB) {
+      double y = 0.0;
+#pragma unroll
+      for (int c = 0; c < NB; c++) y += X[tid][c] * rb[c];
+      xb[g * NB + tid] = y;
+      if (blockIdx.x == 0 && tid < nb) x[b0 + tid] = y;
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  const long i = (long)k0 + nb + blockIdx.x * (long)blockDim.x + tid;
+  const long i = (long)k0 + ncol + blockIdx.x * (long)blockDim.x + tid;
   if (i < n) {
     const double* row = L + i * n + k0;
-    double v[NB];
+    double s0 = 0.0, s1 = 0.0;
+    for (int cb = 0; cb < ncol; cb += NB) {
+      double u[NB];
 #pragma unroll
-    for (int c = 0; c < NB; c++) v[c] = c < nb ? row[c] : 0.0;
-    double s = 0.0;
+      for (int c = 0; c < NB; c++) u[c] = cb + c < ncol ? row[cb + c] : 0.0;
 #pragma unroll
-    for (int c = 0; c < NB; c++) s += v[c] * xb[c];
-    w[i] -= s;
+      for (int c = 0; c < NB; c += 2) {
+        s0 += u[c] * xb[cb + c];
+        s1 += u[c + 1] * xb[cb + c + 1];
+      }
+    }
+    w[i] -= s0 + s1;
   }
 }
 
 __global__ __launch_bounds__(256) void k_trsv_bwd(const double* L, const double* Linv, int n,
-                                                  int k0, int nb, double* z, double* x) {
+                                                  int k0, int ncol, double* z, double* x) {
   __shared__ double X[NB][NB + 1];
-  __shared__ double zb[NB], xb[NB];
-  const int tid = threadIdx.x;
-  const double* Xg = Linv + (size_t)(k0 / NB) * NB * NB;
-  {
+  __shared__ double xb[NB * kTrsvG], rb[NB], part[4][NB];
+  const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
+  const int ng = (ncol + NB - 1) / NB;
+  xb[tid] = 0.0;
+  for (int g = ng - 1; g >= 0; g--) {
+    const int b0 = k0 + g * NB, nb = min(NB, ncol - g * NB);
+    const double* Xg = Linv + (size_t)(b0 / NB) * NB * NB;
     double v[16];
 #pragma unroll
-    for (int q = 0; q < 16; q++) v[q] = Xg[tid + 256 * q];
+    for (int e = 0; e < 16; e++) v[e] = Xg[tid + 256 * e];
 #pragma unroll
-    for (int q = 0; q < 16; q++) X[(tid + 256 * q) / NB][(tid + 256 * q) % NB] = v[q];
-  }
-  if (tid < NB) zb[tid] = tid < nb ? z[k0 + tid] : 0.0;
-  __syncthreads();
-  if (tid < NB) {
+    for (int e = 0; e < 16; e++) X[(tid + 256 * e) / NB][(tid + 256 * e) % NB] = v[e];
+    __syncthreads();
+    // later blocks of this super-block (rows g*64 + 64 .. ncol): 4 partial
+    // sums over their rows, 16 rows at a time
+    const int c0 = (g + 1) * NB, len = ncol - c0, per = (len + 3) / 4;
     double s = 0.0;
+    if (r < nb && len > 0) {
+      const int ca = c0 + per * q, cb = min(ncol, c0 + per * (q + 1));
+      for (int cc = ca; cc < cb; cc += 16) {
+        double u[16];
 #pragma unroll
-    for (int r = 0; r < NB; r++) s += X[r][tid] * zb[r];
-    xb[tid] = s;
-    if (blockIdx.x == 0 && tid < nb) x[k0 + tid] = s;
+        for (int e = 0; e < 16; e++) u[e] = cc + e < cb ? L[(long)(k0 + cc + e) * n + b0 + r] : 0.0;
+#pragma unroll
+        for (int e = 0; e < 16; e++) s += cc + e < cb ? u[e] * xb[cc + e] : 0.0;
+      }
+    }
+    part[q][r] = s;
+    __syncthreads();
+    if (tid < NB)
+      rb[tid] = tid < nb ? z[b0 + tid] - ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]))
+                         : 0.0;
+    __syncthreads();
+    if (tid < NB) {
+      double y = 0.0;
+#pragma unroll
+      for (int c = 0; c < NB; c++) y += X[c][tid] * rb[c];
+      xb[g * NB + tid] = y;
+      if (blockIdx.x == 0 && tid < nb) x[b0 + tid] = y;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   const long j = blockIdx.x * (long)blockDim.x + tid;
   if (j < k0) {
-    double v[NB];
+    double s0 = 0.0, s1 = 0.0;
+    for (int cb = 0; cb < ncol; cb += NB) {
+      double u[NB];
 #pragma unroll
-    for (int r = 0; r < NB; r++) v[r] = r < nb ? L[(long)(k0 + r) * n + j] : 0.0;
-    double s = 0.0;
+      for (int c = 0; c < NB; c++) u[c] = cb + c < ncol ? L[(long)(k0 + cb + c) * n + j] : 0.0;
 #pragma unroll
-    for (int r = 0; r < NB; r++) s += v[r] * xb[r];
-    z[j] -= s;
+      for (int c = 0; c < NB; c += 2) {
+        s0 += u[c] * xb[cb + c];
+        s1 += u[c + 1] * xb[cb + c + 1];
+      }
+    }
+    z[j] -= s0 + s1;
   }
 }
 
 int launch_potrs(const double* L, const double* Linv, int n, const double* b, double* x,
                  double* w, hipStream_t st) {
+  const int SB = NB * kTrsvG;
   NQS_HIP_CHECK(hipMemcpyAsync(w, b, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st));
-  for (int k0 = 0; k0 < n; k0 += NB) {
-    const int nb = (n - k0) < NB ? (n - k0) : NB;
-    const int rows = n - k0 - nb;
+  for (int k0 = 0; k0 < n; k0 += SB) {
+    const int ncol = (n - k0) < SB ? (n - k0) : SB;
+    const int rows = n - k0 - ncol;
     const int grid = rows > 0 ? (rows + 255) / 256 : 1;
     KScope ks("trsv", st);
-    hipLaunchKernelGGL(k_trsv_fwd, dim3(grid), dim3(256), 0, st, L, Linv, n, k0, nb, w, x);
+    hipLaunchKernelGGL(k_trsv_fwd, dim3(grid), dim3(256), 0, st, L, Linv, n, k0, ncol, w, x);
     count_launch();
   }
   // x now holds z = L^-1 b; back substitution in place on w = z
   NQS_HIP_CHECK(hipMemcpyAsync(w, x, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st));
-  const int last = ((n - 1) / NB) * NB;
-  for (int k0 = last; k0 >= 0; k0 -= NB) {
-    const int nb = (n - k0) < NB ? (n - k0) : NB;
+  const int last = ((n - 1) / SB) * SB;
+  for (int k0 = last; k0 >= 0; k0 -= SB) {
+    const int ncol = (n - k0) < SB ? (n - k0) : SB;
     const int grid = k0 > 0 ? (k0 + 255) / 256 : 1;
     KScope ks("trsv", st);
-    hipLaunchKernelGGL(k_trsv_bwd, dim3(grid), dim3(256), 0, st, L, Linv, n, k0, nb, w, x);
+    hipLaunchKernelGGL(k_trsv_bwd, dim3(grid), dim3(256), 0, st, L, Linv, n, k0, ncol, w, x);
     count_launch();
   }
   NQS_HIP_CHECK(hipGetLastError());
