@@ -413,7 +413,7 @@ __global__ __launch_bounds__(512) void k_gemm_big(int M, int N, int K, double al
   // 8 waves, 2 per SIMD: wave (wm, wn) computes rows wm*64 + [0, 64) and
   // columns wn*32 + [0, 32) of the tile as 4x2 MFMA blocks; the operands of
   // step kk + 4 are read from LDS while the MFMAs of step kk issue.
-  if (skip && *skip) return;
+  const int skipped = skip ? *skip : 0;
   __shared__ double As[2][BT][BKC + 1];
   __shared__ double Bs[2][BT][BKC + 1];
   int tm, tn;
@@ -445,6 +445,8 @@ __global__ __launch_bounds__(512) void k_gemm_big(int M, int N, int K, double al
           cpre[i][j][r] = (gr < M && gc < N && (!mask || gc <= gr)) ? C[(long)gr * ldc + gc] : 0.0;
         }
   }
+  // the skip flag is read with the C tile; nothing is written before the test
+  if (skipped) return;
   auto gload = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 8; q++) {
@@ -568,7 +570,9 @@ size_t potrf_linv_elems(int n) { return 2 * (size_t)((n + NB - 1) / NB) * NB * N
 // and the updated, unsolved rows are stored, and k0 + j + 1 is reported.
 __global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb, int kp,
                                                double* Linv, int* fail) {
-  if (*fail) return;
+  // the failure flag is read with the first loads; nothing is written
+  // before it is tested
+  const int failed = *fail;
   __shared__ double D[NB][NB + 1];
   __shared__ double X[NB][NB + 1];     // Lp_d in step 1, L^-1 from step 3
   __shared__ double Rs[NB][NB + 1];
@@ -583,6 +587,7 @@ __global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb,
     X[r][c] = (kp >= 0 && r < nb) ? A[(long)(k0 + r) * n + kp + c] : 0.0;
   }
   __syncthreads();
+  if (failed) return;
   // step 1: wave w computes row block w of the rows and of D
   {
     v4d accR[4], accD[4];
@@ -597,10 +602,15 @@ __global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb,
       accD[bj] = (v4d){0.0, 0.0, 0.0, 0.0};
     }
     if (kp >= 0) {
+      // the row operands of all 16 K steps are loaded up front (one global
+      // round trip instead of one per step)
       const long arow = rowbase + 16 * wave + lr;
-#pragma unroll 4
+      double arv[NB / 4];
+#pragma unroll
+      for (int q = 0; q < NB / 4; q++) arv[q] = arow < n ? -A[arow * n + kp + 4 * q + lk] : 0.0;
+#pragma unroll
       for (int kk = 0; kk < NB; kk += 4) {
-        const double ar = arow < n ? -A[arow * n + kp + kk + lk] : 0.0;
+        const double ar = arv[kk >> 2];
         const double ad = -X[16 * wave + lr][kk + lk];
 #pragma unroll
         for (int bj = 0; bj < 4; bj++) {
