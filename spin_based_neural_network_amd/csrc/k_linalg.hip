@@ -643,23 +643,23 @@ __global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb,
   int jdone = nb;
   for (int jb = 0; jb < 4 && 16 * jb < nb; jb++) {
     if (wave == jb) {
+      // straight-line code (selects instead of branches) so that the
+      // square roots and scalings of column j overlap the chain of j + 1
       int jf = nb;
 #pragma unroll
       for (int jj = 0; jj < 16; jj++) {
         const int j = 16 * jb + jj;
-        if (j < nb && jf == nb) {
-          const double d = readlane_d(rr[jj], j);
-          if (!(d > 0.0)) {
-            jf = j;
-          } else {
-            const double ci = rr[jj];
-            const double si = (lane > j) ? ci * (1.0 / d) : 0.0;
+        const double d = readlane_d(rr[jj], j);
+        const bool live = j < nb && jf == nb;
+        const bool ok = live && d > 0.0;
+        jf = (live && !ok) ? j : jf;
+        const double ci = rr[jj];
+        const double rd = 1.0 / d;
+        const double si = (ok && lane > j) ? ci * rd : 0.0;
 #pragma unroll
-            for (int t = jj + 1; t < 16; t++) rr[t] -= si * readlane_d(ci, 16 * jb + t);
-            const double Ljj = sqrt(d);
-            rr[jj] = (lane == j) ? Ljj : ((lane > j) ? ci / Ljj : ci);
-          }
-        }
+        for (int t = jj + 1; t < 16; t++) rr[t] -= si * readlane_d(ci, 16 * jb + t);
+        const double rs = sqrt(rd);   // 1 / sqrt(d)
+        rr[jj] = !ok ? ci : ((lane == j) ? d * rs : ((lane > j) ? ci * rs : ci));
       }
 #pragma unroll
       for (int t = 0; t < 16; t++) D[lane][16 * jb + t] = rr[t];
