@@ -581,10 +581,21 @@ __global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb,
   const int lr = lane & 15, lk = lane >> 4;
   const int r0 = k0 + nb;
   const long rowbase = (long)r0 + (long)blockIdx.x * NB;
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int r = e >> 6, c = e & 63;
-    D[r][c] = (r < nb) ? ((c <= r) ? A[(long)(k0 + r) * n + k0 + c] : 0.0) : (r == c ? 1.0 : 0.0);
-    X[r][c] = (kp >= 0 && r < nb) ? A[(long)(k0 + r) * n + kp + c] : 0.0;
+  {
+    // all 32 loads of a thread in flight together
+    double dv[NB * NB / 256], xv[NB * NB / 256];
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+      dv[q] = (r < nb) ? ((c <= r) ? A[(long)(k0 + r) * n + k0 + c] : 0.0) : (r == c ? 1.0 : 0.0);
+      xv[q] = (kp >= 0 && r < nb) ? A[(long)(k0 + r) * n + kp + c] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+      D[r][c] = dv[q];
+      X[r][c] = xv[q];
+    }
   }
   __syncthreads();
   if (failed) return;
