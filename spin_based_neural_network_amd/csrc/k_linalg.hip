@@ -648,6 +648,7 @@ __global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb,
   // publishes them through D, and every later wave subtracts the rank-16
   // product from its own columns.
   __shared__ int s_fail;
+  __shared__ double colw[4][NB];
   double rr[16];
 #pragma unroll
   for (int t = 0; t < 16; t++) rr[t] = D[lane][16 * wave + t];
@@ -655,20 +656,26 @@ __global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb,
   for (int jb = 0; jb < 4 && 16 * jb < nb; jb++) {
     if (wave == jb) {
       // straight-line code (selects instead of branches) so that the
-      // square roots and scalings of column j overlap the chain of j + 1
+      // square roots and scalings of column j overlap the chain of j + 1.
+      // Column j is broadcast through the wave's own LDS row: the wave's
+      // LDS accesses complete in issue order, and the wavefront fence keeps
+      // the compiler from moving the reads above the write.
       int jf = nb;
 #pragma unroll
       for (int jj = 0; jj < 16; jj++) {
         const int j = 16 * jb + jj;
-        const double d = readlane_d(rr[jj], j);
+        const double ci = rr[jj];
+        colw[wave][lane] = ci;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const double d = colw[wave][j];
         const bool live = j < nb && jf == nb;
         const bool ok = live && d > 0.0;
         jf = (live && !ok) ? j : jf;
-        const double ci = rr[jj];
         const double rd = 1.0 / d;
         const double si = (ok && lane > j) ? ci * rd : 0.0;
 #pragma unroll
-        for (int t = jj + 1; t < 16; t++) rr[t] -= si * readlane_d(ci, 16 * jb + t);
+        for (int t = jj + 1; t < 16; t++) rr[t] -= si * colw[wave][16 * jb + t];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const double rs = sqrt(rd);   // 1 / sqrt(d)
         rr[jj] = !ok ? ci : ((lane == j) ? d * rs : ((lane > j) ? ci * rs : ci));
       }
