@@ -159,7 +159,7 @@ def traffic_bytes(workload, tag):
 
 
 TAGS = ["mh", "ar_mh", "forward", "eloc", "gen_O", "colsum", "gemm", "gemm_potrf", "potrf_panel",
-        "trsv", "gram_f"]
+        "trsv", "gram_f", "multi_prep", "multi_gemm_z", "multi_rows", "multi_gemm_g", "multi_vec"]
 
 
 def kernel_work(tag, w, st, launches, iters):
@@ -202,6 +202,13 @@ def kernel_work(tag, w, st, launches, iters):
         tot = sum(2 * 64 ** 3 / 3 + 2 * (B - 64 * (k + 1)) * 64 * 64 * (2 if k % 2 else 1)
                   for k in range(npan))
         return "mfma", tot / npan, "FLOP"
+    if tag in ("multi_gemm_g", "multi_gemm_z"):
+        # one problem: one QGT product per CG iteration, 2 (M+1)(N+1) B FLOP
+        return "mfma", 2.0 * (M + 1) * (N + 1) * B, "FLOP"
+    if tag == "multi_vec":
+        return "hbm", 13 * (M + 1) * (N + 1) * 8, "B"
+    if tag == "multi_rows":
+        return "hbm", 3 * (M + 1) * B * 8, "B"
     if tag == "gen_O":
         return "hbm", B * P * 8 + B * N * 4 + B * M * 8 * (2 if w["kind"] == 6 else 1), "B"
     if tag == "colsum":
@@ -280,16 +287,19 @@ def gpu_run(args, w):
             n = lib.nqs_hip_sampler_last_energies(st["s"], abi.ptr(buf), C * R)
             ser = np.concatenate([ser, buf[: C * n].reshape(C, n)], axis=1)
             sm = np.zeros(7)
-            lib.nqs_hip_last_stage_ms(st["s"], abi.ptr(sm))
-            smp_ms.append(sm[0])
+            if lib.nqs_hip_last_stage_ms(st["s"], abi.ptr(sm)) == 0:
+                smp_ms.append(sm[0])
         tau_end = pooled_tau(ser)
-        ess_end = ser.size / (2.0 * tau_end) / (sum(smp_ms) * 1e-3)
+        # sampling-phase time where the library reports stages (a CG-SR step
+        # replayed as one graph has none: then the whole iterations)
+        t_end = sum(smp_ms) * 1e-3 if len(smp_ms) == max(args.steps, 4) else None
+        ess_end = ser.size / (2.0 * tau_end) / t_end if t_end else None
         tail = np.array(trace[-50:])
         nbin = 5
         bins = tail[: len(tail) // nbin * nbin].reshape(nbin, -1).mean(axis=1)
         conv = dict(iterations=len(trace), wall_s=time.perf_counter() - tc,
                     tau_int_end=tau_end, ess_per_s_end=ess_end,
-                    sample_ms_end=float(np.mean(smp_ms)),
+                    sample_ms_end=float(np.mean(smp_ms)) if t_end else None,
                     tail_mean=float(tail.mean()), tail_err=float(bins.std(ddof=1) / np.sqrt(nbin)),
                     trace_every_10=[float(x) for x in trace[::10]])
         if w["ed"] is not None:
@@ -300,7 +310,13 @@ def gpu_run(args, w):
             lib.nqs_hip_sampler_learned_stats(st["s"], ctypes.byref(acc), None, None, None)
             conv["learned_acceptance"] = acc.value
     avg_ms = ms.value / max(c.value, 1)
-    bound, work, unit = kernel_work(dom, w, st, c.value, args.steps)
+    launches, iters = c.value, args.steps
+    if not c.value:
+        # launches replayed in a graph are not timed one by one: use the
+        # launch-by-launch profiling iteration
+        launches, ms_prof = per[dom]
+        avg_ms, iters = ms_prof / launches, 1
+    bound, work, unit = kernel_work(dom, w, st, launches, iters)
     if bound == "mfma":
         achieved, peak, runit = work / (avg_ms * 1e-3) / 1e12, PEAK_FP64 / 1e12, "TFLOP/s"
     else:
@@ -437,11 +453,14 @@ def gpu_run_multi(args, w):
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
     series = [np.zeros((C, 0)) for _ in range(n)]
     buf = np.zeros(B)
-    t_total = 0.0
+    t_total, t_sample = 0.0, 0.0
+    smp = ctypes.c_double()
     for k in range(args.steps):
         t0 = time.perf_counter()
         lib.nqs_hip_multi_step(m, infos)
         t_total += time.perf_counter() - t0
+        if lib.nqs_hip_multi_last_sample_ms(m, ctypes.byref(smp)) == 0:
+            t_sample += smp.value * 1e-3
         for i in range(n):
             lib.nqs_hip_multi_last_energies(m, i, abi.ptr(buf), B)
             series[i] = np.concatenate([series[i], buf.reshape(R, C).T], axis=1)
@@ -481,7 +500,9 @@ def gpu_run_multi(args, w):
     for p in probs:
         lib.nqs_sampler_free(p["s"])
         lib.nqs_ansatz_free(p["a"])
-    return dict(ess_per_s=ess_total / (t_total / args.steps), ms_per_step=ms_step, taus=taus,
+    return dict(ess_per_s=ess_total / t_sample if t_sample > 0 else ess_total / t_total,
+                ess_per_s_iteration=ess_total / t_total, sample_ms=1e3 * t_sample / args.steps,
+                ms_per_step=ms_step, taus=taus,
                 iters=iters, energies=energies, inst=inst, convergence=conv, chosen=chosen,
                 tuning={int(tg[i]): float(us[i]) for i in range(6)},
                 roofline=dict(kernel=dom, symbol=None, bound=bound, achieved=achieved, peak=peak,
@@ -502,7 +523,7 @@ def cpu_baseline_multi(w, gpu):
     f = lambda nm: getattr(lib, pre + nm)
     fn = abi.fnptr(lib, pre + "ansatz_log_amp")
     B = w["chains"] * w["per_chain"]
-    per_sample = {}
+    per_sample, per_sample_mh = {}, {}
     nb = 8
     for N in sorted(set(N for N, _ in gpu["inst"])):
         cfg = abi.config_defaults(ansatz=w["kind"], hamiltonian=0, rbm_hidden_units=2 * N,
@@ -526,6 +547,7 @@ def cpu_baseline_multi(w, gpu):
             f("ansatz_logpsi_gradient")(a, abi.ptr(np.ascontiguousarray(smp[b])), N, abi.ptr(g))
         t_o = (time.perf_counter() - t0) / nb
         per_sample[N] = t_mh + t_el + t_o
+        per_sample_mh[N] = t_mh
         f("sampler_free")(s)
         f("ansatz_free")(a)
     # CG MAC rate: one reference CG-SR step, N = 32, Ns = 256
@@ -549,14 +571,15 @@ def cpu_baseline_multi(w, gpu):
     for (N, _), it in zip(gpu["inst"], gpu["iters"]):
         Pn = 2 * N * N + 3 * N
         t_iter += B * per_sample[N] + (2 * it + 2) * B * Pn / rate
-    ess_cpu = sum(B / (2.0 * t) for t in gpu["taus"]) / t_iter
+    ess_cpu = sum(B / (2.0 * t) for t in gpu["taus"]) / sum(B * per_sample_mh[N] for N, _ in gpu["inst"])
+    ess_cpu_iter = sum(B / (2.0 * t) for t in gpu["taus"]) / t_iter
     sample = (f"{kind} CPU path, 1 core: {nb} samples per instance size of MH (N moves each), "
               f"E_loc (open twin scaled to the periodic bond count) and O rows, plus one "
               f"reference CG-SR step at N=32, Ns={ns} for the CG MAC rate "
               f"({rate / 1e9:.2f} GMAC/s); the step of all {len(gpu['inst'])} instances "
               f"({t_iter:.1f} s) is extrapolated with the GPU run's CG iteration counts and tau_int")
     return dict(value=ess_cpu, unit="effective samples/s", cores=1, kind=kind, sample=sample,
-                sr_iter_s=t_iter)
+                sr_iter_s=t_iter, ess_per_s_iteration=ess_cpu_iter)
 
 
 def cpu_baseline(w, gpu):
@@ -736,7 +759,8 @@ def main_multi(args, w):
                    "samples_per_instance": w["chains"] * w["per_chain"], "method": "cg",
                    "split_target": gpu["chosen"]},
         "sr_iter_ms": gpu["ms_per_step"],
-        "ess_per_s_iteration": gpu["ess_per_s"],
+        "sample_ms": gpu["sample_ms"],
+        "ess_per_s_iteration": gpu["ess_per_s_iteration"],
         "cg_iterations": gpu["iters"],
         "autotune_us": gpu["tuning"],
         "kernel_split_ms": {k: v["ms"] for k, v in gpu["split"].items()},

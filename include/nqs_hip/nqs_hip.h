@@ -190,6 +190,8 @@ int nqs_hip_multi_run(nqs_hip_multi_t *m, int num_iterations, double *out_traces
 int nqs_hip_multi_sync(nqs_hip_multi_t *m);
 int nqs_hip_multi_tuning(const nqs_hip_multi_t *m, int *targets, double *matvec_us, int capacity);
 int nqs_hip_multi_use_graph(nqs_hip_multi_t *m, int on);
+/* device time of the last step's Metropolis launches (all problems), ms */
+int nqs_hip_multi_last_sample_ms(const nqs_hip_multi_t *m, double *ms);
 int nqs_hip_multi_last_energies(const nqs_hip_multi_t *m, int i, double *out, int capacity);
 void nqs_hip_multi_free(nqs_hip_multi_t *m);
 

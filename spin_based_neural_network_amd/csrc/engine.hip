@@ -226,6 +226,7 @@ struct nqs_ansatz {
   nqs::ModelBuffers<double> m64;
   nqs::ModelBuffers<float> m32;
   int dev_fresh = 0;       // device copy equals host copy (inside a run)
+  long long id = 0;        // unique per ansatz created in this process
 };
 
 namespace {
@@ -501,6 +502,8 @@ NQS_EXPORT nqs_ansatz_t* nqs_ansatz_create(const nqs_config_t* cfg, int num_site
     return nullptr;
   if (stream_ready()) return nullptr;
   nqs_ansatz* a = new nqs_ansatz();
+  static long long next_id = 0;
+  a->id = ++next_id;
   a->kind = kind;
   a->N = num_sites;
   if (kind == NQS_ANSATZ_LEGACY_MLP) {
@@ -678,6 +681,9 @@ struct nqs_hip_ar {
   }
 };
 
+struct nqs_hip_multi;
+void multi_delete(nqs_hip_multi* m);
+
 struct nqs_sampler {
   int N = 0;
   nqs_config_t cfg;
@@ -708,7 +714,16 @@ struct nqs_sampler {
   long win_pos = 0, win_n = 0;
   int since_train = 0, trained = 0, n_train = 0;
   long long ar_props = 0;
+  // CG-SR steps of one real RBM run as a batch of one problem (one graph
+  // per step); rebuilt when the ansatz, the step's parameters or the sample
+  // buffer change
+  nqs_hip_multi* solo = nullptr;
+  long long solo_ansatz = 0;
+  nqs_config_t solo_cfg;
+  int solo_Lx = 0, solo_Ly = 0;
+  void* solo_samples = nullptr;
   ~nqs_sampler() {
+    multi_delete(solo);
     DBuf* all[] = {&d_spins, &d_rng, &d_acc, &d_tot, &d_la, &d_cur, &d_samples,
                    &win, &lacc, &prop_bits, &prop_spins, &lq, &la_prop};
     for (DBuf* b : all) b->release();
@@ -1349,6 +1364,9 @@ NQS_EXPORT int nqs_hip_local_energy_batch_ex(const nqs_config_t* cfg, int size_x
 // ===========================================================================
 // stochastic reconfiguration
 // ===========================================================================
+int solo_cg_step(const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz* a, nqs_sampler* s,
+                 nqs_sr_step_info_t* info, bool in_run);
+
 namespace {
 
 enum { SR_CG = 0, SR_HOLO = 1, SR_MINSR = 2 };
@@ -1427,6 +1445,22 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
   if (!fn) { fn = nqs_ansatz_log_amp; user = a; }
   const int holo = (method == SR_HOLO);
   const int prec = s->opt.precision;
+  {
+    // CG-SR of a real RBM whose sampler and local energies use the ansatz
+    // itself: the batched path with one problem (factored QGT products, no
+    // host round trip inside the CG, one graph per step);
+    // NQS_HIP_SOLO_BATCH=0 keeps the general path
+    static const bool solo_on = [] {
+      const char* e = getenv("NQS_HIP_SOLO_BATCH");
+      return !(e && e[0] == '0');
+    }();
+    AmpSource es;
+    if (solo_on && method == SR_CG && !gfn && s->device && a->kind == NQS_ANSATZ_RBM && a->M >= 1 &&
+        s->src.a == a && !s->src.marshall && prec == NQS_HIP_FP64 &&
+        s->opt.move != NQS_HIP_MOVE_LEARNED && recognise(fn, user, es) && es.a == a &&
+        !es.marshall && (a->M + 1) * (a->N + 1) <= nqs::kMultiVecBlock * nqs::kMultiMaxPart)
+      return solo_cg_step(cfg, Lx, Ly, a, s, info, in_run);
+  }
   StepTimer tm;
   if (s->device) tm.init();
   if (!in_run) a->dev_fresh = 0;
@@ -2148,6 +2182,8 @@ struct nqs_hip_multi {
   std::vector<std::pair<int, double>> tuning;   // (target, matvec us)
   hipStream_t side[kSide] = {};
   hipEvent_t ev_fork = nullptr, ev_join[kSide] = {};
+  hipEvent_t ev_s0 = nullptr, ev_s1 = nullptr;   // around the Metropolis launches
+  double sample_ms = -1.0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double* h_out = nullptr;              // pinned, kMultiOut per problem
@@ -2166,6 +2202,8 @@ struct nqs_hip_multi {
       if (ev_join[k]) (void)hipEventDestroy(ev_join[k]);
     }
     if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_s0) (void)hipEventDestroy(ev_s0);
+    if (ev_s1) (void)hipEventDestroy(ev_s1);
     if (h_out) (void)hipHostFree(h_out);
     for (Prob& q : p) {
       DBuf* all[] = {&q.ham, &q.sublat, &q.t, &q.E, &q.S1, &q.S1t, &q.T1t, &q.Zt, &q.TWt,
@@ -2262,8 +2300,10 @@ int multi_issue_front(nqs_hip_multi* m) {
   CKR(multi_join(m));
   if (m->jobs.empty()) CKR(multi_build_jobs(m));
   const nqs::MHJob* dj = m->d_jobs.as<nqs::MHJob>();
+  CK(hipEventRecord(m->ev_s0, g_stream));
   for (const std::array<int, 4>& g : m->groups)
     CKR(nqs::launch_mh_batch(g[0], dj + g[1], g[2], g[3], g_stream));
+  CK(hipEventRecord(m->ev_s1, g_stream));
   CKR(multi_fork(m));
   for (size_t i = 0; i < m->p.size(); i++) {
     nqs_hip_multi::Prob& q = m->p[i];
@@ -2385,6 +2425,8 @@ int multi_step(nqs_hip_multi* m, nqs_sr_step_info_t* infos) {
     CK(hipGraphLaunch(m->exec, g_stream));
   }
   CK(hipStreamSynchronize(g_stream));
+  float sms = 0.f;
+  m->sample_ms = hipEventElapsedTime(&sms, m->ev_s0, m->ev_s1) == hipSuccess ? sms : -1.0;
   multi_fill_info(m, infos);
   for (nqs_hip_multi::Prob& q : m->p) q.a->m64.valid = 0;
   m->steps_done++;
@@ -2393,10 +2435,10 @@ int multi_step(nqs_hip_multi* m, nqs_sr_step_info_t* infos) {
 
 }  // namespace
 
-NQS_EXPORT nqs_hip_multi_t* nqs_hip_multi_create(int n, const nqs_config_t* cfgs, const int* size_x,
-                                                 const int* size_y, nqs_ansatz_t* const* ansatz,
-                                                 nqs_sampler_t* const* samplers) {
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
+void multi_delete(nqs_hip_multi* m) { delete m; }
+
+nqs_hip_multi* multi_new(int n, const nqs_config_t* cfgs, const int* size_x, const int* size_y,
+                         nqs_ansatz* const* ansatz, nqs_sampler* const* samplers) {
   if (n <= 0 || !cfgs || !size_x || !size_y || !ansatz || !samplers) return nullptr;
   if (stream_ready()) return nullptr;
   nqs_hip_multi* m = new nqs_hip_multi();
@@ -2491,7 +2533,8 @@ NQS_EXPORT nqs_hip_multi_t* nqs_hip_multi_create(int n, const nqs_config_t* cfgs
       hipHostMalloc((void**)&m->h_out, sizeof(double) * nqs::kMultiOut * n, hipHostMallocDefault) !=
           hipSuccess)
     return fail(0, "memory");
-  if (hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming) != hipSuccess)
+  if (hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreate(&m->ev_s0) != hipSuccess || hipEventCreate(&m->ev_s1) != hipSuccess)
     return fail(0, "event");
   for (int k = 0; k < nqs_hip_multi::kSide; k++)
     if (hipStreamCreateWithFlags(&m->side[k], hipStreamNonBlocking) != hipSuccess ||
@@ -2500,6 +2543,66 @@ NQS_EXPORT nqs_hip_multi_t* nqs_hip_multi_create(int n, const nqs_config_t* cfgs
   if (multi_set_split(m, 32) || hipStreamSynchronize(g_stream) != hipSuccess)
     return fail(0, "setup");
   return m;
+}
+
+NQS_EXPORT nqs_hip_multi_t* nqs_hip_multi_create(int n, const nqs_config_t* cfgs, const int* size_x,
+                                                 const int* size_y, nqs_ansatz_t* const* ansatz,
+                                                 nqs_sampler_t* const* samplers) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  return multi_new(n, cfgs, size_x, size_y, ansatz, samplers);
+}
+
+// the fields of a configuration that a CG-SR step reads
+static bool same_step_cfg(const nqs_config_t& x, const nqs_config_t& y) {
+  return x.hamiltonian == y.hamiltonian && x.j_coupling == y.j_coupling &&
+         x.j2_coupling == y.j2_coupling && x.j_z_coupling == y.j_z_coupling &&
+         x.transverse_field == y.transverse_field && x.kh_K == y.kh_K && x.kh_J == y.kh_J &&
+         x.kagome_pbc == y.kagome_pbc && x.num_samples == y.num_samples &&
+         x.learning_rate == y.learning_rate && x.sr_diag_shift == y.sr_diag_shift &&
+         x.sr_cg_max_iters == y.sr_cg_max_iters && x.sr_cg_tol == y.sr_cg_tol;
+}
+
+// One CG-SR step of a real RBM through the batched path with a single
+// problem (the sampler keeps the batch and its captured graph between
+// steps).  Host parameters are uploaded before and downloaded after, as in
+// the general path.
+int solo_cg_step(const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz* a, nqs_sampler* s,
+                 nqs_sr_step_info_t* info, bool in_run) {
+  if (!in_run) a->dev_fresh = 0;
+  CKR(ansatz_upload(a));
+  a->dev_fresh = 1;
+  const int B = cfg->num_samples;
+  CKR(s->d_samples.ensure(sizeof(int) * (size_t)B * s->N));
+  const bool same = s->solo && s->solo_ansatz == a->id && s->solo_Lx == Lx && s->solo_Ly == Ly &&
+                    s->solo_samples == s->d_samples.p && same_step_cfg(s->solo_cfg, *cfg);
+  if (!same) {
+    multi_delete(s->solo);
+    s->solo = nullptr;
+    nqs_ansatz* av[1] = {a};
+    nqs_sampler* sv[1] = {s};
+    s->solo = multi_new(1, cfg, &Lx, &Ly, av, sv);
+    if (!s->solo) return -1;
+    s->solo_ansatz = a->id;
+    s->solo_cfg = *cfg;
+    s->solo_Lx = Lx;
+    s->solo_Ly = Ly;
+    s->solo_samples = s->d_samples.p;
+  }
+  CKR(multi_step(s->solo, info));
+  // the step's local energies for the per-chain series
+  s->last_E.resize(B);
+  CK(hipMemcpyAsync(s->last_E.data(), s->solo->p[0].E.p, sizeof(double) * B, hipMemcpyDeviceToHost,
+                    g_stream));
+  CKR(ansatz_download(a));
+  s->last_rounds = (B + s->opt.num_chains - 1) / s->opt.num_chains;
+  // stage timings: the sampling launch only (the rest of the step is one
+  // graph); total = the sampling time as well, other stages 0
+  s->have_stage = s->solo->sample_ms >= 0.0 ? 1 : 0;
+  for (int i = 0; i < 7; i++) s->stage_ms[i] = 0.0;
+  if (s->have_stage) s->stage_ms[0] = s->stage_ms[6] = s->solo->sample_ms;
+  s->cur_dirty = 1;
+  if (!in_run) a->dev_fresh = 0;
+  return 0;
 }
 
 NQS_EXPORT int nqs_hip_multi_step(nqs_hip_multi_t* m, nqs_sr_step_info_t* out_infos) {
@@ -2545,6 +2648,13 @@ NQS_EXPORT int nqs_hip_multi_tuning(const nqs_hip_multi_t* m, int* targets, doub
     if (matvec_us) matvec_us[i] = m->tuning[i].second;
   }
   return m->target;
+}
+
+NQS_EXPORT int nqs_hip_multi_last_sample_ms(const nqs_hip_multi_t* m, double* ms) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!m || !ms || m->sample_ms < 0.0) return -1;
+  *ms = m->sample_ms;
+  return 0;
 }
 
 NQS_EXPORT int nqs_hip_multi_use_graph(nqs_hip_multi_t* m, int on) {

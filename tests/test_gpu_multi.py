@@ -3,10 +3,12 @@
 * Against the reference: the golden CG-SR steps of the real-RBM scenarios
   (single chain, reference sample sequence) run as one ragged batch and
   meet the same CG-family tolerances as nqs_sr_step does.
-* Against the library's own single-problem path on identical clones: the
-  same samples (mean energies and acceptance bit-equal) and updates within
-  the CG conditioning bound, for a batch of TFIM chains of different sizes;
-  graph replays (steps >= 2) keep agreeing.
+* Against nqs_sr_step on identical clones (which runs each real-RBM CG-SR
+  problem as a batch of one): the same samples (mean energies and
+  acceptance bit-equal) and updates within the CG conditioning bound, for a
+  ragged batch of TFIM chains of different sizes (different block offsets
+  and split factors than the batches of one); graph replays (steps >= 2)
+  keep agreeing.
 * TFIM chains of several sizes and fields approach the free-fermion ground
   state energy -sum_k sqrt(1 + h^2 - 2 h cos k), k = (2n+1) pi / N.
 """
