@@ -200,7 +200,8 @@ double nqs_hip_tau_int(const double *x, int n);
 
 /* Stage timings of the last SR step on the sampler's stream, milliseconds:
  * [0] sampling, [1] local energy (with its forward pass), [2] O_k, means and
- * force, [3] solve, [4] update, [5] unused, [6] total.  0 when available. */
+ * force, [3] solve, [4] update, [5] unused, [6] total.  0 when available.
+ * A CG-SR step run as a batch of one problem reports [0] and [6] only. */
 int nqs_hip_last_stage_ms(const nqs_sampler_t *s, double out[7]);
 
 /* MinSR building blocks on host arrays (both run on the device).

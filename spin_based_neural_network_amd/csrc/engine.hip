@@ -2183,7 +2183,8 @@ struct nqs_hip_multi {
   hipStream_t side[kSide] = {};
   hipEvent_t ev_fork = nullptr, ev_join[kSide] = {};
   hipEvent_t ev_s0 = nullptr, ev_s1 = nullptr;   // around the Metropolis launches
-  double sample_ms = -1.0;
+  hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;   // around the whole step
+  double sample_ms = -1.0, step_ms = -1.0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double* h_out = nullptr;              // pinned, kMultiOut per problem
@@ -2202,8 +2203,9 @@ struct nqs_hip_multi {
       if (ev_join[k]) (void)hipEventDestroy(ev_join[k]);
     }
     if (ev_fork) (void)hipEventDestroy(ev_fork);
-    if (ev_s0) (void)hipEventDestroy(ev_s0);
-    if (ev_s1) (void)hipEventDestroy(ev_s1);
+    hipEvent_t evs[] = {ev_s0, ev_s1, ev_t0, ev_t1};
+    for (hipEvent_t e : evs)
+      if (e) (void)hipEventDestroy(e);
     if (h_out) (void)hipHostFree(h_out);
     for (Prob& q : p) {
       DBuf* all[] = {&q.ham, &q.sublat, &q.t, &q.E, &q.S1, &q.S1t, &q.T1t, &q.Zt, &q.TWt,
@@ -2409,6 +2411,7 @@ int multi_capture(nqs_hip_multi* m) {
 }
 
 int multi_step(nqs_hip_multi* m, nqs_sr_step_info_t* infos) {
+  CK(hipEventRecord(m->ev_t0, g_stream));
   if (!m->tuned) {
     // first step: eager, with the split factors tuned on its own data
     CKR(multi_issue_front(m));
@@ -2424,9 +2427,11 @@ int multi_step(nqs_hip_multi* m, nqs_sr_step_info_t* infos) {
     if (!m->exec) CKR(multi_capture(m));
     CK(hipGraphLaunch(m->exec, g_stream));
   }
+  CK(hipEventRecord(m->ev_t1, g_stream));
   CK(hipStreamSynchronize(g_stream));
-  float sms = 0.f;
+  float sms = 0.f, tms = 0.f;
   m->sample_ms = hipEventElapsedTime(&sms, m->ev_s0, m->ev_s1) == hipSuccess ? sms : -1.0;
+  m->step_ms = hipEventElapsedTime(&tms, m->ev_t0, m->ev_t1) == hipSuccess ? tms : -1.0;
   multi_fill_info(m, infos);
   for (nqs_hip_multi::Prob& q : m->p) q.a->m64.valid = 0;
   m->steps_done++;
@@ -2534,7 +2539,8 @@ nqs_hip_multi* multi_new(int n, const nqs_config_t* cfgs, const int* size_x, con
           hipSuccess)
     return fail(0, "memory");
   if (hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreate(&m->ev_s0) != hipSuccess || hipEventCreate(&m->ev_s1) != hipSuccess)
+      hipEventCreate(&m->ev_s0) != hipSuccess || hipEventCreate(&m->ev_s1) != hipSuccess ||
+      hipEventCreate(&m->ev_t0) != hipSuccess || hipEventCreate(&m->ev_t1) != hipSuccess)
     return fail(0, "event");
   for (int k = 0; k < nqs_hip_multi::kSide; k++)
     if (hipStreamCreateWithFlags(&m->side[k], hipStreamNonBlocking) != hipSuccess ||
@@ -2595,11 +2601,14 @@ int solo_cg_step(const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz* a, nqs_sam
                     g_stream));
   CKR(ansatz_download(a));
   s->last_rounds = (B + s->opt.num_chains - 1) / s->opt.num_chains;
-  // stage timings: the sampling launch only (the rest of the step is one
-  // graph); total = the sampling time as well, other stages 0
-  s->have_stage = s->solo->sample_ms >= 0.0 ? 1 : 0;
+  // stage timings: the sampling launches and the whole step (the rest of
+  // the step is one graph; its stages are not timed separately)
+  s->have_stage = s->solo->sample_ms >= 0.0 && s->solo->step_ms >= 0.0 ? 1 : 0;
   for (int i = 0; i < 7; i++) s->stage_ms[i] = 0.0;
-  if (s->have_stage) s->stage_ms[0] = s->stage_ms[6] = s->solo->sample_ms;
+  if (s->have_stage) {
+    s->stage_ms[0] = s->solo->sample_ms;
+    s->stage_ms[6] = s->solo->step_ms;
+  }
   s->cur_dirty = 1;
   if (!in_run) a->dev_fresh = 0;
   return 0;
