@@ -143,7 +143,7 @@ SYMBOL = {"gemm_potrf": "nqs::k_gemm_big<1>", "gemm": "nqs::k_gemm_big<0>",
           "potrf_panel": "nqs::k_panel",
           "eloc": "nqs::k_eloc<double, true, 1>", "gen_O": "nqs::k_gen_O<double>",
           "colsum": "nqs::k_colsum_partial", "mh": "nqs::k_mh<double, true, 64, 4>",
-          "trsv": "nqs::k_trsv_fwd"}
+          "trsv": "nqs::k_sbsv_fwd"}
 
 
 def traffic_bytes(workload, tag):
@@ -159,7 +159,33 @@ def traffic_bytes(workload, tag):
 
 
 TAGS = ["mh", "ar_mh", "forward", "eloc", "gen_O", "colsum", "gemm", "gemm_potrf", "potrf_panel",
-        "trsv", "gram_f", "multi_prep", "multi_gemm_z", "multi_rows", "multi_gemm_g", "multi_vec"]
+        "trsv", "trsv_inv", "gram_f", "multi_prep", "multi_gemm_z", "multi_rows", "multi_gemm_g", "multi_vec"]
+
+
+def potrf_gemm_flops(B):
+    """FLOP of the blocked Cholesky's GEMM updates of a B x B matrix (lower
+    elements only): wide panels of `width` columns made of 128-column outer
+    panels; each wide panel's second outer panel first takes the first's
+    rank-128 update, then the next wide panel's columns and the rest of the
+    trailing triangle take one rank-`width` update."""
+    width = 128 if os.environ.get("NQS_HIP_POTRF_WIDTH") == "128" else 256
+
+    def lower_cols(r0, w):   # elements of columns [r0, r0 + w), rows >= r0, lower part
+        return w * (w + 1) // 2 + (B - r0 - w) * w
+
+    def mid(k0):
+        return sum(2 * 128 * lower_cols(j, min(128, B - j))
+                   for j in range(k0 + 128, min(k0 + width, B), 128))
+
+    tot, k0 = mid(0), 0
+    while k0 + width < B:
+        wd = width
+        r0 = k0 + wd
+        w1 = min(width, B - r0)
+        m = B - r0 - w1
+        tot += 2 * wd * lower_cols(r0, w1) + mid(r0) + 2 * wd * (m * (m + 1) // 2)
+        k0 = r0
+    return tot
 
 
 def kernel_work(tag, w, st, launches, iters):
@@ -180,20 +206,7 @@ def kernel_work(tag, w, st, launches, iters):
         # lower triangle of T = O_c O_c^T: B(B+1)/2 dot products of length P
         return "mfma", B * (B + 1) * P / per_iter, "FLOP"
     if tag == "gemm_potrf":
-        # rank-128 updates of the blocked Cholesky: the next outer panel's
-        # columns (lower part) and the rest of the trailing lower triangle
-        tot, k0 = 0, 0
-        while True:
-            wd = min(128, B - k0)
-            r0 = k0 + wd
-            if r0 >= B:
-                break
-            w1 = min(128, B - r0)
-            nxt = w1 * (w1 + 1) // 2 + (B - r0 - w1) * w1
-            m = B - r0 - w1
-            tot += 2 * wd * (nxt + m * (m + 1) // 2)
-            k0 += 128
-        return "mfma", tot / per_iter, "FLOP"
+        return "mfma", potrf_gemm_flops(B) / per_iter, "FLOP"
     if tag == "potrf_panel":
         # per inner panel: factor + inverse of the 64x64 block, and the rows
         # below solved by a 64-wide product (and the preceding panel's

@@ -2344,6 +2344,13 @@ int multi_issue_step(nqs_hip_multi* m) {
 // time the CG product (matvec and alpha, idempotent at the CG start) for
 // each target and keep the fastest
 int multi_autotune(nqs_hip_multi* m) {
+  // NQS_HIP_MULTI_SPLIT=t: split target t without timing, so that the
+  // summation order (and with it every result) does not depend on timing
+  const char* fixed = getenv("NQS_HIP_MULTI_SPLIT");
+  if (fixed && atoi(fixed) > 0) {
+    m->tuning.clear();
+    return multi_set_split(m, atoi(fixed));
+  }
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));

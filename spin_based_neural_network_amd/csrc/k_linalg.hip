@@ -540,18 +540,20 @@ int launch_gemm_nt(int M, int N, int K, double alpha, const double* A, long lda,
 // 64-column inner panels; one launch (k_panel) per inner panel factors the
 // diagonal block, inverts it, and solves every row below it (the second
 // inner panel first takes the first's rank-64 update inside the same
-// launch).  The trailing matrix takes one rank-128 update per outer panel on
-// the large-tile MFMA GEMM.  With lookahead, the next outer panel's columns
-// are updated first and that panel is factored on a second stream while the
-// rest of the trailing matrix is updated.  The inverses of the 64x64
-// diagonal blocks are kept for the triangular solves.
+// launch).  The trailing matrix takes one rank-256 update per wide panel of
+// two outer panels (the second outer panel takes the first's rank-128
+// update before it is factored) on the large-tile MFMA GEMM.  With
+// lookahead, the next wide panel's columns are updated first and that panel
+// is factored on a second stream while the rest of the trailing matrix is
+// updated.  The inverses of the 64x64 diagonal blocks are kept for the
+// triangular solves.
 // --------------------------------------------------------------------------
 constexpr int NB = 64, NBO = 128;
 
-// Linv workspace: the inverses of the diagonal blocks, then the blocks' L
-// (written by the panel kernel and copied into A once every panel has read
-// its input)
-size_t potrf_linv_elems(int n) { return 2 * (size_t)((n + NB - 1) / NB) * NB * NB; }
+// Linv workspace (potrf_linv_elems): the inverses of the 64 x 64 diagonal
+// blocks, then the blocks' L (written by the panel kernel and copied into A
+// once every panel has read its input), then the triangular solves' 256 x 256
+// super-block inverses (row-major and transposed)
 
 // One inner panel, columns [k0, k0 + nb), in one launch.  Workgroup b
 // (4 waves) owns rows rowbase = r0 + 64 b + [0, 64) below the diagonal block
@@ -892,9 +894,32 @@ static int outer_panel(double* A, int n, int K0, double* Linv, int* fail_dev, hi
   return 0;
 }
 
+// wide panel at K0 of up to `width` columns: 128-column outer panels, each
+// after the rank-128 update of its columns by the outer panel left of it
+static int wide_panel(double* A, int n, int K0, int width, double* Linv, int* fail_dev,
+                      hipStream_t st) {
+  for (int j = K0; j < K0 + width && j < n; j += NBO) {
+    if (j > K0) {
+      const int w = (n - j) < NBO ? (n - j) : NBO;
+      const double* L = A + (long)j * n + j - NBO;
+      if (gemm_big_impl(n - j, w, NBO, -1.0, L, n, L, n, 1.0, A + (long)j * n + j, n, 0, 1,
+                        fail_dev, "gemm_potrf", st))
+        return -1;
+    }
+    if (outer_panel(A, n, j, Linv, fail_dev, st)) return -1;
+  }
+  return 0;
+}
+
 int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) {
   if (!g_look) NQS_HIP_CHECK(hipStreamCreateWithFlags(&g_look, hipStreamNonBlocking));
-  const int np = (n + NBO - 1) / NBO;
+  // trailing updates of rank 256 (wide panels of two outer panels);
+  // NQS_HIP_POTRF_WIDTH=128: rank 128
+  static const int width = [] {
+    const char* e = getenv("NQS_HIP_POTRF_WIDTH");
+    return (e && atoi(e) == NBO) ? NBO : 2 * NBO;
+  }();
+  const int np = (n + width - 1) / width;
   while ((int)g_ev.size() < 2 * np) {
     hipEvent_t e;
     NQS_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -902,7 +927,7 @@ int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) 
   }
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, fail_dev);
   count_launch();
-  if (outer_panel(A, n, 0, Linv, fail_dev, st)) return -1;
+  if (wide_panel(A, n, 0, width, Linv, fail_dev, st)) return -1;
   // NQS_HIP_POTRF_LOOKAHEAD=0: the same schedule on one stream
   static const int lookahead = [] {
     const char* e = getenv("NQS_HIP_POTRF_LOOKAHEAD");
@@ -910,19 +935,19 @@ int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) 
   }();
   hipStream_t side = lookahead ? g_look : st;
   for (int k = 0; k < np; k++) {
-    const int K0 = k * NBO;
-    const int w = (n - K0) < NBO ? (n - K0) : NBO;
+    const int K0 = k * width;
+    const int w = (n - K0) < width ? (n - K0) : width;
     const int R0 = K0 + w;
     if (R0 >= n) break;
-    const int w1 = (n - R0) < NBO ? (n - R0) : NBO;
+    const int w1 = (n - R0) < width ? (n - R0) : width;
     const double* L = A + (long)R0 * n + K0;
-    // next outer panel's columns [R0, R0 + w1), rows >= R0 (lower part)
+    // next wide panel's columns [R0, R0 + w1), rows >= R0 (lower part)
     if (gemm_big_impl(n - R0, w1, w, -1.0, L, n, L, n, 1.0, A + (long)R0 * n + R0, n, 0, 1,
                       fail_dev, "gemm_potrf", st))
       return -1;
     NQS_HIP_CHECK(hipEventRecord(g_ev[2 * k], st));
     NQS_HIP_CHECK(hipStreamWaitEvent(side, g_ev[2 * k], 0));
-    if (outer_panel(A, n, R0, Linv, fail_dev, side)) return -1;
+    if (wide_panel(A, n, R0, width, Linv, fail_dev, side)) return -1;
     NQS_HIP_CHECK(hipEventRecord(g_ev[2 * k + 1], side));
     // the rest of the trailing matrix: rows and columns >= R0 + w1
     const int R1 = R0 + w1;
@@ -964,152 +989,226 @@ int launch_potrf_repair(double* A, int n, int jf, const double* orig, hipStream_
 }
 
 // --------------------------------------------------------------------------
-// triangular solves L L^T x = b with the inverted diagonal blocks: one
-// launch per block and direction; every workgroup recomputes the block's
-// solution (64x64 product) and workgroup 0 publishes it
+// Triangular solves L L^T x = b by super-blocks of 256 rows.  The inverse of
+// every 256 x 256 diagonal block of L is formed once per solve from the
+// 64 x 64 inverses the factorisation kept (X_ii = Linv_i,
+// X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj), stored row-major (X) and
+// transposed (XT).  Each direction is then one launch per super-block: every
+// workgroup multiplies the super-block's right-hand side by the inverse
+// (coalesced over the output index; workgroup 0 publishes the solution) and
+// removes its contribution from 64 rows below (forward) or 64 columns to the
+// left (backward).  The dependent chain is n / 256 launches per direction.
 // --------------------------------------------------------------------------
-// triangular solves L L^T x = b with the inverted diagonal blocks: one
-// launch per super-block of kTrsvG diagonal blocks and direction.  Every
-// workgroup solves the super-block block by block (a block's right-hand
-// side takes the earlier blocks' solutions as one dot product, then
-// x_g = Linv_g rhs_g), workgroup 0 publishes x, and each workgroup removes
-// the super-block's contribution from 256 rows below (forward) or columns
-// to the left (backward).
-// --------------------------------------------------------------------------
-constexpr int kTrsvG = 4;
+constexpr int SB = 256;
 
-__global__ __launch_bounds__(256) void k_trsv_fwd(const double* L, const double* Linv, int n,
-                                                  int k0, int ncol, double* w, double* x) {
-  __shared__ double X[NB][NB + 1];
-  __shared__ double xb[NB * kTrsvG], rb[NB], part[4][NB];
-  const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
-  xb[tid] = 0.0;
-  for (int g = 0; g * NB < ncol; g++) {
-    const int b0 = k0 + g * NB, nb = min(NB, ncol - g * NB);
-    const double* Xg = Linv + (size_t)(b0 / NB) * NB * NB;
-    double v[16];
+static size_t sb_count(int n) { return (size_t)((n + SB - 1) / SB); }
+size_t potrf_linv_elems(int n) {
+  return 2 * (size_t)((n + NB - 1) / NB) * NB * NB + 2 * sb_count(n) * SB * SB;
+}
+
+// workgroup (s, j): block column j of super-block s's inverse.  The
+// operands of every 64 x 64 product are staged in LDS by coalesced loads;
+// each thread owns a 4 x 4 piece of the product.
+__global__ __launch_bounds__(256) void k_sb_inverse(const double* __restrict__ L,
+                                                    const double* __restrict__ Linv, int n,
+                                                    double* __restrict__ X, double* __restrict__ XT) {
+  __shared__ double Xs[3][NB][NB];   // X_jj, X_(j+1)j, X_(j+2)j
+  __shared__ double Ts[NB][NB];      // staged L_ik or Linv_i
+  __shared__ double Ss[NB][NB];      // S = sum_k L_ik X_kj
+  const int s = blockIdx.x, j = blockIdx.y, tid = threadIdx.x;
+  const int k0 = s * SB, ncol = min(SB, n - k0), ng = (ncol + NB - 1) / NB;
+  const int rb = 4 * (tid >> 4), cb = 4 * (tid & 15);
+  double* Xo = X + (size_t)s * SB * SB;
+  double* XTo = XT + (size_t)s * SB * SB;
+  auto put = [&](int i, int r, int c, double v) {
+    Xo[(size_t)(NB * i + r) * SB + NB * j + c] = v;
+    XTo[(size_t)(NB * j + c) * SB + NB * i + r] = v;
+  };
+  // acc (+)= sign * Ts Bm, Bm a 64 x 64 LDS tile
+  auto product = [&](double (&acc)[4][4], const double (*Bm)[NB], double sign) {
+    for (int m = 0; m < NB; m++) {
+      double a[4], b[4];
 #pragma unroll
-    for (int e = 0; e < 16; e++) v[e] = Xg[tid + 256 * e];
+      for (int u = 0; u < 4; u++) a[u] = sign * Ts[rb + u][m];
 #pragma unroll
-    for (int e = 0; e < 16; e++) X[(tid + 256 * e) / NB][(tid + 256 * e) % NB] = v[e];
-    __syncthreads();
-    // earlier blocks of this super-block: 4 partial sums of 16 g terms
-    double s = 0.0;
-    if (r < nb) {
-      const double* row = L + (long)(b0 + r) * n + k0;
-      for (int c0 = 16 * g * q; c0 < 16 * g * (q + 1); c0 += 16) {
-        double u[16];
+      for (int v = 0; v < 4; v++) b[v] = Bm[m][cb + v];
 #pragma unroll
-        for (int e = 0; e < 16; e++) u[e] = row[c0 + e];
+      for (int u = 0; u < 4; u++)
 #pragma unroll
-        for (int e = 0; e < 16; e++) s += u[e] * xb[c0 + e];
+        for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
+    }
+  };
+  for (int i = 0; i < 4; i++) {
+    if (i < j || j >= ng || i >= ng) {
+      for (int e = tid; e < NB * NB; e += 256) put(i, e >> 6, e & 63, 0.0);
+      continue;
+    }
+    double acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc[u][v] = 0.0;
+    if (i == j) {
+      const double* D = Linv + (size_t)(k0 / NB + j) * NB * NB;
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[u][v] = D[(rb + u) * NB + cb + v];
+    } else {
+      // S = sum_{k=j}^{i-1} L_ik X_kj  (rows of block i beyond ncol are zero)
+      for (int k = j; k < i; k++) {
+        const double* Lik = L + (size_t)(k0 + NB * i) * n + k0 + NB * k;
+        for (int e = tid; e < NB * NB; e += 256) {
+          const int r = e >> 6, c = e & 63;
+          Ts[r][c] = (NB * i + r < ncol) ? Lik[(size_t)r * n + c] : 0.0;
+        }
+        __syncthreads();
+        product(acc, Xs[k - j], 1.0);
+        __syncthreads();
       }
-    }
-    part[q][r] = s;
-    __syncthreads();
-    if (tid < NB)
-      rb[tid] = tid < nb ? w[b0 + tid] - ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]))
-                         : 0.0;
-    __syncthreads();
-    if (tid < NB) {
-      double y = 0.0;
 #pragma unroll
-      for (int c = 0; c < NB; c++) y += X[tid][c] * rb[c];
-      xb[g * NB + tid] = y;
-      if (blockIdx.x == 0 && tid < nb) x[b0 + tid] = y;
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          Ss[rb + u][cb + v] = acc[u][v];
+          acc[u][v] = 0.0;
+        }
+      // X_ij = -Linv_i S
+      const double* D = Linv + (size_t)(k0 / NB + i) * NB * NB;
+      for (int e = tid; e < NB * NB; e += 256) Ts[e >> 6][e & 63] = D[e];
+      __syncthreads();
+      product(acc, Ss, -1.0);
     }
+    if (i - j < 3) {
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) Xs[i - j][rb + u][cb + v] = acc[u][v];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) put(i, rb + u, cb + v, acc[u][v]);
     __syncthreads();
-  }
-  const long i = (long)k0 + ncol + blockIdx.x * (long)blockDim.x + tid;
-  if (i < n) {
-    const double* row = L + i * n + k0;
-    double s0 = 0.0, s1 = 0.0;
-    for (int cb = 0; cb < ncol; cb += NB) {
-      double u[NB];
-#pragma unroll
-      for (int c = 0; c < NB; c++) u[c] = cb + c < ncol ? row[cb + c] : 0.0;
-#pragma unroll
-      for (int c = 0; c < NB; c += 2) {
-        s0 += u[c] * xb[cb + c];
-        s1 += u[c + 1] * xb[cb + c + 1];
-      }
-    }
-    w[i] -= s0 + s1;
   }
 }
 
-__global__ __launch_bounds__(256) void k_trsv_bwd(const double* L, const double* Linv, int n,
-                                                  int k0, int ncol, double* z, double* x) {
-  __shared__ double X[NB][NB + 1];
-  __shared__ double xb[NB * kTrsvG], rb[NB], part[4][NB];
-  const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
-  const int ng = (ncol + NB - 1) / NB;
-  xb[tid] = 0.0;
-  for (int g = ng - 1; g >= 0; g--) {
-    const int b0 = k0 + g * NB, nb = min(NB, ncol - g * NB);
-    const double* Xg = Linv + (size_t)(b0 / NB) * NB * NB;
-    double v[16];
+// 16 per-lane partial sums v[0..15] (one per row) -> lane l holds the full
+// sum of row l >> 2 (every lane of its group of 4)
+__device__ inline double reduce16(double (&v)[16], int lane) {
 #pragma unroll
-    for (int e = 0; e < 16; e++) v[e] = Xg[tid + 256 * e];
+  for (int h = 8, m = 32; h >= 1; h >>= 1, m >>= 1) {
+    const bool hi = (lane & m) != 0;
 #pragma unroll
-    for (int e = 0; e < 16; e++) X[(tid + 256 * e) / NB][(tid + 256 * e) % NB] = v[e];
-    __syncthreads();
-    // later blocks of this super-block (rows g*64 + 64 .. ncol): 4 partial
-    // sums over their rows, 16 rows at a time
-    const int c0 = (g + 1) * NB, len = ncol - c0, per = (len + 3) / 4;
-    double s = 0.0;
-    if (r < nb && len > 0) {
-      const int ca = c0 + per * q, cb = min(ncol, c0 + per * (q + 1));
-      for (int cc = ca; cc < cb; cc += 16) {
-        double u[16];
-#pragma unroll
-        for (int e = 0; e < 16; e++) u[e] = cc + e < cb ? L[(long)(k0 + cc + e) * n + b0 + r] : 0.0;
-#pragma unroll
-        for (int e = 0; e < 16; e++) s += cc + e < cb ? u[e] * xb[cc + e] : 0.0;
-      }
+    for (int q = 0; q < h; q++) {
+      const double keep = hi ? v[q + h] : v[q];
+      const double send = hi ? v[q] : v[q + h];
+      v[q] = keep + __shfl_xor(send, m, 64);
     }
-    part[q][r] = s;
-    __syncthreads();
-    if (tid < NB)
-      rb[tid] = tid < nb ? z[b0 + tid] - ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]))
-                         : 0.0;
-    __syncthreads();
-    if (tid < NB) {
-      double y = 0.0;
-#pragma unroll
-      for (int c = 0; c < NB; c++) y += X[c][tid] * rb[c];
-      xb[g * NB + tid] = y;
-      if (blockIdx.x == 0 && tid < nb) x[b0 + tid] = y;
-    }
-    __syncthreads();
   }
-  const long j = blockIdx.x * (long)blockDim.x + tid;
+  double r = v[0];
+  r += __shfl_xor(r, 2, 64);
+  r += __shfl_xor(r, 1, 64);
+  return r;
+}
+
+// forward, super-block at k0 (ncol columns): x_sb = X_sb w_sb, then
+// w[i] -= L[i][k0:k0+ncol] x_sb for the workgroup's 64 rows below
+__global__ __launch_bounds__(256) void k_sbsv_fwd(const double* __restrict__ L,
+                                                  const double* __restrict__ XT, int n, int k0,
+                                                  int ncol, double* __restrict__ w,
+                                                  double* __restrict__ x) {
+  __shared__ double ws[SB], xs[SB];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  ws[tid] = tid < ncol ? w[k0 + tid] : 0.0;
+  __syncthreads();
+  const double* Xs = XT + (size_t)(k0 / SB) * SB * SB;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll 4
+  for (int c = 0; c < SB; c += 4) {
+    a0 += Xs[(size_t)c * SB + tid] * ws[c];
+    a1 += Xs[(size_t)(c + 1) * SB + tid] * ws[c + 1];
+    a2 += Xs[(size_t)(c + 2) * SB + tid] * ws[c + 2];
+    a3 += Xs[(size_t)(c + 3) * SB + tid] * ws[c + 3];
+  }
+  const double y = (a0 + a1) + (a2 + a3);
+  xs[tid] = y;
+  if (blockIdx.x == 0 && tid < ncol) x[k0 + tid] = y;
+  __syncthreads();
+  const long r0 = (long)k0 + ncol + 64L * blockIdx.x + 16 * wave;
+  double v[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const long row = r0 + q;
+    double t = 0.0;
+    if (row < n) {
+      const double* lr = L + row * n + k0;
+#pragma unroll
+      for (int m = 0; m < SB; m += 64)
+        if (m + lane < ncol) t += lr[m + lane] * xs[m + lane];
+    }
+    v[q] = t;
+  }
+  const double sum = reduce16(v, lane);
+  const long row = r0 + (lane >> 2);
+  if ((lane & 3) == 0 && row < n) w[row] -= sum;
+}
+
+// backward, super-block at k0: x_sb = X_sb^T z_sb, then
+// z[j] -= sum_c L[k0 + c][j] x_sb[c] for the workgroup's 64 columns j < k0
+__global__ __launch_bounds__(256) void k_sbsv_bwd(const double* __restrict__ L,
+                                                  const double* __restrict__ X, int n, int k0,
+                                                  int ncol, double* __restrict__ z,
+                                                  double* __restrict__ x) {
+  __shared__ double zs[SB], xs[SB], part[4][64];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  zs[tid] = tid < ncol ? z[k0 + tid] : 0.0;
+  __syncthreads();
+  const double* Xs = X + (size_t)(k0 / SB) * SB * SB;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll 4
+  for (int c = 0; c < SB; c += 4) {
+    a0 += Xs[(size_t)c * SB + tid] * zs[c];
+    a1 += Xs[(size_t)(c + 1) * SB + tid] * zs[c + 1];
+    a2 += Xs[(size_t)(c + 2) * SB + tid] * zs[c + 2];
+    a3 += Xs[(size_t)(c + 3) * SB + tid] * zs[c + 3];
+  }
+  const double y = (a0 + a1) + (a2 + a3);
+  xs[tid] = y;
+  if (blockIdx.x == 0 && tid < ncol) x[k0 + tid] = y;
+  __syncthreads();
+  const long j = 64L * blockIdx.x + lane;
+  double t0 = 0.0, t1 = 0.0;
   if (j < k0) {
-    double s0 = 0.0, s1 = 0.0;
-    for (int cb = 0; cb < ncol; cb += NB) {
-      double u[NB];
-#pragma unroll
-      for (int c = 0; c < NB; c++) u[c] = cb + c < ncol ? L[(long)(k0 + cb + c) * n + j] : 0.0;
-#pragma unroll
-      for (int c = 0; c < NB; c += 2) {
-        s0 += u[c] * xb[cb + c];
-        s1 += u[c + 1] * xb[cb + c + 1];
-      }
+    const int c1 = min(ncol, 64 * (wave + 1));
+#pragma unroll 8
+    for (int c = 64 * wave; c < c1; c += 2) {
+      t0 += L[(long)(k0 + c) * n + j] * xs[c];
+      if (c + 1 < c1) t1 += L[(long)(k0 + c + 1) * n + j] * xs[c + 1];
     }
-    z[j] -= s0 + s1;
   }
+  part[wave][lane] = t0 + t1;
+  __syncthreads();
+  if (wave == 0 && j < k0) z[j] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
 int launch_potrs(const double* L, const double* Linv, int n, const double* b, double* x,
                  double* w, hipStream_t st) {
-  const int SB = NB * kTrsvG;
+  const int nsb = (int)sb_count(n);
+  double* X = const_cast<double*>(Linv) + 2 * (size_t)((n + NB - 1) / NB) * NB * NB;
+  double* XT = X + (size_t)nsb * SB * SB;
+  {
+    KScope ks("trsv_inv", st);
+    hipLaunchKernelGGL(k_sb_inverse, dim3(nsb, 4), dim3(256), 0, st, L, Linv, n, X, XT);
+    count_launch();
+  }
   NQS_HIP_CHECK(hipMemcpyAsync(w, b, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st));
   for (int k0 = 0; k0 < n; k0 += SB) {
     const int ncol = (n - k0) < SB ? (n - k0) : SB;
     const int rows = n - k0 - ncol;
-    const int grid = rows > 0 ? (rows + 255) / 256 : 1;
+    const int grid = rows > 0 ? (rows + 63) / 64 : 1;
     KScope ks("trsv", st);
-    hipLaunchKernelGGL(k_trsv_fwd, dim3(grid), dim3(256), 0, st, L, Linv, n, k0, ncol, w, x);
+    hipLaunchKernelGGL(k_sbsv_fwd, dim3(grid), dim3(256), 0, st, L, XT, n, k0, ncol, w, x);
     count_launch();
   }
   // x now holds z = L^-1 b; back substitution in place on w = z
@@ -1117,9 +1216,9 @@ int launch_potrs(const double* L, const double* Linv, int n, const double* b, do
   const int last = ((n - 1) / SB) * SB;
   for (int k0 = last; k0 >= 0; k0 -= SB) {
     const int ncol = (n - k0) < SB ? (n - k0) : SB;
-    const int grid = k0 > 0 ? (k0 + 255) / 256 : 1;
+    const int grid = k0 > 0 ? (k0 + 63) / 64 : 1;
     KScope ks("trsv", st);
-    hipLaunchKernelGGL(k_trsv_bwd, dim3(grid), dim3(256), 0, st, L, Linv, n, k0, ncol, w, x);
+    hipLaunchKernelGGL(k_sbsv_bwd, dim3(grid), dim3(256), 0, st, L, X, n, k0, ncol, w, x);
     count_launch();
   }
   NQS_HIP_CHECK(hipGetLastError());

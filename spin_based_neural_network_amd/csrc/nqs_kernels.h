@@ -232,7 +232,8 @@ int launch_gemm_nt(int M, int N, int K, double alpha, const double* A, long lda,
 // In-place lower Cholesky of the n x n row-major matrix (lower triangle read
 // and written, upper untouched).  Returns 0 on success or 1 + the first
 // column whose pivot is not positive, -1 on errors.  Linv receives the
-// inverses of the 64 x 64 diagonal blocks (potrf_linv_elems(n) doubles).
+// inverses of the 64 x 64 diagonal blocks; it also holds launch_potrs's
+// super-block inverses (potrf_linv_elems(n) doubles in all).
 size_t potrf_linv_elems(int n);
 int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st);
 // After a failure at column jf: bring A to the state the unblocked

@@ -319,7 +319,7 @@ def test_fp32_local_energy_bound(hip):
 
 def test_gram_and_cholesky_blocks(hip, orc):
     rng = np.random.default_rng(11)
-    for (B, P) in [(37, 19), (130, 300), (200, 64)]:
+    for (B, P) in [(37, 19), (130, 300), (200, 64), (700, 900)]:
         O = np.ascontiguousarray(rng.standard_normal((B, P)))
         T = np.zeros((B, B))
         assert hip.nqs_hip_gram(ptr(O), B, P, 1.0 / B, ptr(T)) == 0
@@ -339,7 +339,7 @@ def test_cholesky_failure_state_matches_unblocked(hip, orc):
     """A rank-deficient matrix: the blocked factorisation reports the same
     outcome as the unblocked reference algorithm, including its retry."""
     rng = np.random.default_rng(4)
-    for (B, r) in [(90, 70), (150, 100)]:
+    for (B, r) in [(90, 70), (150, 100), (600, 450)]:
         X = rng.standard_normal((B, r))
         A = np.ascontiguousarray(X @ X.T / B)
         A[np.arange(B), np.arange(B)] += 0.0
