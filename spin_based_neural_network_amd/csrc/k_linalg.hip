@@ -993,17 +993,35 @@ int launch_potrf_repair(double* A, int n, int jf, const double* orig, hipStream_
 // every 256 x 256 diagonal block of L is formed once per solve from the
 // 64 x 64 inverses the factorisation kept (X_ii = Linv_i,
 // X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj), stored row-major (X) and
-// transposed (XT).  Each direction is then one launch per super-block: every
-// workgroup multiplies the super-block's right-hand side by the inverse
-// (coalesced over the output index; workgroup 0 publishes the solution) and
-// removes its contribution from 64 rows below (forward) or 64 columns to the
-// left (backward).  The dependent chain is n / 256 launches per direction.
+// transposed (XT).  Each direction is then a chain of one launch per
+// super-block: workgroup 0 brings the next block's right-hand side up to
+// date and solves it with one 256 x 256 product (coalesced over the output
+// index), while the other workgroups remove the current block's
+// contribution from the rows below (forward) or the columns to the left
+// (backward) of the next block.
 // --------------------------------------------------------------------------
 constexpr int SB = 256;
 
 static size_t sb_count(int n) { return (size_t)((n + SB - 1) / SB); }
 size_t potrf_linv_elems(int n) {
-  return 2 * (size_t)((n + NB - 1) / NB) * NB * NB + 2 * sb_count(n) * SB * SB;
+  return 2 * (size_t)((n + NB - 1) / NB) * NB * NB + 4 * sb_count(n) * SB * SB;
+}
+
+// acc (+)= sign * A B over 64 x 64 LDS tiles; the thread owns rows
+// rb .. rb + 3 and columns cb .. cb + 3
+__device__ inline void tile_fma(double (&acc)[4][4], const double (*A)[NB], const double (*Bm)[NB],
+                                double sign, int rb, int cb) {
+  for (int m = 0; m < NB; m++) {
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) a[u] = sign * A[rb + u][m];
+#pragma unroll
+    for (int v = 0; v < 4; v++) b[v] = Bm[m][cb + v];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
+  }
 }
 
 // workgroup (s, j): block column j of super-block s's inverse.  The
@@ -1024,19 +1042,8 @@ __global__ __launch_bounds__(256) void k_sb_inverse(const double* __restrict__ L
     Xo[(size_t)(NB * i + r) * SB + NB * j + c] = v;
     XTo[(size_t)(NB * j + c) * SB + NB * i + r] = v;
   };
-  // acc (+)= sign * Ts Bm, Bm a 64 x 64 LDS tile
   auto product = [&](double (&acc)[4][4], const double (*Bm)[NB], double sign) {
-    for (int m = 0; m < NB; m++) {
-      double a[4], b[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) a[u] = sign * Ts[rb + u][m];
-#pragma unroll
-      for (int v = 0; v < 4; v++) b[v] = Bm[m][cb + v];
-#pragma unroll
-      for (int u = 0; u < 4; u++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
-    }
+    tile_fma(acc, Ts, Bm, sign, rb, cb);
   };
   for (int i = 0; i < 4; i++) {
     if (i < j || j >= ng || i >= ng) {
@@ -1093,11 +1100,12 @@ __global__ __launch_bounds__(256) void k_sb_inverse(const double* __restrict__ L
   }
 }
 
-// 16 per-lane partial sums v[0..15] (one per row) -> lane l holds the full
-// sum of row l >> 2 (every lane of its group of 4)
-__device__ inline double reduce16(double (&v)[16], int lane) {
+// R per-lane partial sums v[0..R-1] (one per row, R = 8 or 16) -> lane l
+// holds the full sum of row l >> (6 - log2 R) (every lane of its group)
+template <int R>
+__device__ inline double reduce_rows(double (&v)[R], int lane) {
 #pragma unroll
-  for (int h = 8, m = 32; h >= 1; h >>= 1, m >>= 1) {
+  for (int h = R / 2, m = 32; h >= 1; h >>= 1, m >>= 1) {
     const bool hi = (lane & m) != 0;
 #pragma unroll
     for (int q = 0; q < h; q++) {
@@ -1107,89 +1115,210 @@ __device__ inline double reduce16(double (&v)[16], int lane) {
     }
   }
   double r = v[0];
-  r += __shfl_xor(r, 2, 64);
-  r += __shfl_xor(r, 1, 64);
+#pragma unroll
+  for (int m = 32 / R; m >= 1; m >>= 1) r += __shfl_xor(r, m, 64);
   return r;
 }
 
-// forward, super-block at k0 (ncol columns): x_sb = X_sb w_sb, then
-// w[i] -= L[i][k0:k0+ncol] x_sb for the workgroup's 64 rows below
-__global__ __launch_bounds__(256) void k_sbsv_fwd(const double* __restrict__ L,
-                                                  const double* __restrict__ XT, int n, int k0,
-                                                  int ncol, double* __restrict__ w,
-                                                  double* __restrict__ x) {
-  __shared__ double ws[SB], xs[SB];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  ws[tid] = tid < ncol ? w[k0 + tid] : 0.0;
-  __syncthreads();
-  const double* Xs = XT + (size_t)(k0 / SB) * SB * SB;
+// y[t] = sum_c M[c][t] v[c] over a 256 x 256 super-block matrix M (X or
+// XT), v in LDS; 512 threads, two per output (halves of c), coalesced over
+// t.  Returns y in threads tid < 256.
+__device__ inline double sb_matvec(const double* __restrict__ M, const double* v,
+                                   double (*part)[SB], int tid) {
+  const int t = tid & (SB - 1), h = tid >> 8;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 #pragma unroll 4
-  for (int c = 0; c < SB; c += 4) {
-    a0 += Xs[(size_t)c * SB + tid] * ws[c];
-    a1 += Xs[(size_t)(c + 1) * SB + tid] * ws[c + 1];
-    a2 += Xs[(size_t)(c + 2) * SB + tid] * ws[c + 2];
-    a3 += Xs[(size_t)(c + 3) * SB + tid] * ws[c + 3];
+  for (int c = 128 * h; c < 128 * h + 128; c += 4) {
+    a0 += M[(size_t)c * SB + t] * v[c];
+    a1 += M[(size_t)(c + 1) * SB + t] * v[c + 1];
+    a2 += M[(size_t)(c + 2) * SB + t] * v[c + 2];
+    a3 += M[(size_t)(c + 3) * SB + t] * v[c + 3];
   }
-  const double y = (a0 + a1) + (a2 + a3);
-  xs[tid] = y;
-  if (blockIdx.x == 0 && tid < ncol) x[k0 + tid] = y;
+  part[h][t] = (a0 + a1) + (a2 + a3);
   __syncthreads();
-  const long r0 = (long)k0 + ncol + 64L * blockIdx.x + 16 * wave;
-  double v[16];
-#pragma unroll
-  for (int q = 0; q < 16; q++) {
-    const long row = r0 + q;
-    double t = 0.0;
-    if (row < n) {
-      const double* lr = L + row * n + k0;
-#pragma unroll
-      for (int m = 0; m < SB; m += 64)
-        if (m + lane < ncol) t += lr[m + lane] * xs[m + lane];
-    }
-    v[q] = t;
-  }
-  const double sum = reduce16(v, lane);
-  const long row = r0 + (lane >> 2);
-  if ((lane & 3) == 0 && row < n) w[row] -= sum;
+  return tid < SB ? part[0][t] + part[1][t] : 0.0;
 }
 
-// backward, super-block at k0: x_sb = X_sb^T z_sb, then
-// z[j] -= sum_c L[k0 + c][j] x_sb[c] for the workgroup's 64 columns j < k0
+// L[row][k0 : k0 + 256] xs for the 8 rows r0 .. r0 + 7 of one wave (rows
+// beyond n read row n - 1 and are not used); lane l returns row r0 + (l >> 3).
+// The 32 loads of a lane are independent and issue together.
+__device__ inline double sb_rows8(const double* __restrict__ L, int n, int k0, const double* xs,
+                                  long r0, int lane) {
+  double p[8][4];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const long row = min(r0 + q, (long)n - 1);
+    const double* lr = L + row * n + k0 + lane;
+#pragma unroll
+    for (int m = 0; m < 4; m++) p[q][m] = lr[64 * m];
+  }
+  const double x0 = xs[lane], x1 = xs[64 + lane], x2 = xs[128 + lane], x3 = xs[192 + lane];
+  double v[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) v[q] = (p[q][0] * x0 + p[q][1] * x1) + (p[q][2] * x2 + p[q][3] * x3);
+  return reduce_rows<8>(v, lane);
+}
+
+// first solution block of a direction: out[k0 + t] = sum_c M[c][t] in[k0 + c]
+__global__ __launch_bounds__(512) void k_sbsv_first(const double* __restrict__ M, int k0, int ncol,
+                                                    const double* __restrict__ in,
+                                                    double* __restrict__ out) {
+  __shared__ double vs[SB], part[2][SB];
+  const int tid = threadIdx.x;
+  if (tid < SB) vs[tid] = tid < ncol ? in[k0 + tid] : 0.0;
+  __syncthreads();
+  const double y = sb_matvec(M + (size_t)(k0 / SB) * SB * SB, vs, part, tid);
+  if (tid < ncol) out[k0 + tid] = y;
+}
+
+// Coupling of neighbouring super-blocks a = b - 1 and b (L_ba the 256 x 256
+// block of L below block a's diagonal block): P_b = X_b L_ba, stored
+// transposed, and Q_b = L_ba X_a, row-major.  The next block's solution then
+// needs no update of its right-hand side first:
+//   forward   x_b = X_b w_b - P_b x_a,    backward   x_a = X_a^T z_a - Q_b^T x_b.
+// Workgroup (b - 1, tile, P or Q): one 64 x 64 output tile.
+__global__ __launch_bounds__(256) void k_sb_couple(const double* __restrict__ L,
+                                                   const double* __restrict__ X, int n,
+                                                   double* __restrict__ PT, double* __restrict__ Q) {
+  __shared__ double As[NB][NB], Bs[NB][NB];
+  const int b = blockIdx.x + 1, i = blockIdx.y >> 2, j = blockIdx.y & 3, isq = blockIdx.z;
+  const int tid = threadIdx.x, rb = 4 * (tid >> 4), cb = 4 * (tid & 15);
+  const int kb = b * SB, ka = kb - SB, ncolb = min(SB, n - kb);
+  const double* Xb = X + (size_t)b * SB * SB;
+  const double* Xa = X + (size_t)(b - 1) * SB * SB;
+  double acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) acc[u][v] = 0.0;
+  for (int mt = 0; mt < SB / NB; mt++) {
+    for (int e = tid; e < NB * NB; e += 256) {
+      const int r = e >> 6, c = e & 63;
+      if (!isq) {
+        As[r][c] = Xb[(size_t)(NB * i + r) * SB + NB * mt + c];
+        Bs[r][c] = (NB * mt + r < ncolb) ? L[(size_t)(kb + NB * mt + r) * n + ka + NB * j + c] : 0.0;
+      } else {
+        As[r][c] = (NB * i + r < ncolb) ? L[(size_t)(kb + NB * i + r) * n + ka + NB * mt + c] : 0.0;
+        Bs[r][c] = Xa[(size_t)(NB * mt + r) * SB + NB * j + c];
+      }
+    }
+    __syncthreads();
+    tile_fma(acc, As, Bs, 1.0, rb, cb);
+    __syncthreads();
+  }
+  double* out = (isq ? Q : PT) + (size_t)b * SB * SB;
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int r = NB * i + rb + u, c = NB * j + cb + v;
+      if (isq) out[(size_t)r * SB + c] = acc[u][v];
+      else out[(size_t)c * SB + r] = acc[u][v];
+    }
+}
+
+// Solution block of the next super-block in one pass:
+// y[t] = sum_c M1[c][t] v1[c] - M2[c][t] v2[c] for the workgroup's 16 outputs
+// t0 + (tid & 15); thread slice tid >> 4 covers c in [16 s, 16 s + 16), its
+// 32 loads independent.  Threads tid < 16 return y.
+constexpr int kSbLeads = 16;
+__device__ inline double sb_lead(const double* __restrict__ M1, const double* v1,
+                                 const double* __restrict__ M2, const double* v2,
+                                 double (*part)[16], int t0, int tid) {
+  const int t = t0 + (tid & 15), sl = tid >> 4;
+  double p[16], q[16];
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    p[e] = M1[(size_t)(16 * sl + e) * SB + t];
+    q[e] = M2[(size_t)(16 * sl + e) * SB + t];
+  }
+  double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    a0 += p[e] * v1[16 * sl + e];
+    a1 -= q[e] * v2[16 * sl + e];
+  }
+  part[sl][tid & 15] = a0 + a1;
+  __syncthreads();
+  double y = 0.0;
+  if (tid < 16) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) y += part[k][tid];
+  }
+  return y;
+}
+
+// forward step at super-block k0 (full, 256 rows; x_a already solved).
+// Workgroups 0 .. 15 publish the next block's solution x_b = X_b w_b - P_b x_a
+// (16 entries each).  The others remove block a's contribution from 32 rows
+// each beyond block b.
+__global__ __launch_bounds__(256) void k_sbsv_fwd(const double* __restrict__ L,
+                                                  const double* __restrict__ XT,
+                                                  const double* __restrict__ PT, int n, int k0,
+                                                  double* __restrict__ w, double* __restrict__ x) {
+  __shared__ double xs[SB], ws[SB], part[16][16];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int k1 = k0 + SB, ncol1 = min(SB, n - k1);
+  xs[tid] = x[k0 + tid];
+  if (blockIdx.x < kSbLeads) {
+    ws[tid] = tid < ncol1 ? w[k1 + tid] : 0.0;
+    __syncthreads();
+    const size_t o = (size_t)(k1 / SB) * SB * SB;
+    const double y = sb_lead(XT + o, ws, PT + o, xs, part, 16 * blockIdx.x, tid);
+    const int t = 16 * blockIdx.x + tid;
+    if (tid < 16 && t < ncol1) x[k1 + t] = y;
+    return;
+  }
+  __syncthreads();
+  const long r0 = (long)k1 + SB + 32L * (blockIdx.x - kSbLeads) + 8 * wave;
+  const double sum = sb_rows8(L, n, k0, xs, r0, lane);
+  const long row = r0 + (lane >> 3);
+  if ((lane & 7) == 0 && row < n) w[row] -= sum;
+}
+
+// backward step at super-block k0 > 0 (x_b already solved).  Workgroups
+// 0 .. 15 publish the previous block's solution x_a = X_a^T z_a - Q_b^T x_b.
+// The others remove block b's contribution from 32 columns each left of
+// block a (8 slices of 32 rows; rows beyond n read row n - 1 against a zero
+// x entry).
 __global__ __launch_bounds__(256) void k_sbsv_bwd(const double* __restrict__ L,
-                                                  const double* __restrict__ X, int n, int k0,
+                                                  const double* __restrict__ X,
+                                                  const double* __restrict__ Q, int n, int k0,
                                                   int ncol, double* __restrict__ z,
                                                   double* __restrict__ x) {
-  __shared__ double zs[SB], xs[SB], part[4][64];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  zs[tid] = tid < ncol ? z[k0 + tid] : 0.0;
-  __syncthreads();
-  const double* Xs = X + (size_t)(k0 / SB) * SB * SB;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-#pragma unroll 4
-  for (int c = 0; c < SB; c += 4) {
-    a0 += Xs[(size_t)c * SB + tid] * zs[c];
-    a1 += Xs[(size_t)(c + 1) * SB + tid] * zs[c + 1];
-    a2 += Xs[(size_t)(c + 2) * SB + tid] * zs[c + 2];
-    a3 += Xs[(size_t)(c + 3) * SB + tid] * zs[c + 3];
+  __shared__ double xs[SB], zs[SB], part[16][16], colp[8][32];
+  const int tid = threadIdx.x;
+  const int kp = k0 - SB;
+  xs[tid] = tid < ncol ? x[k0 + tid] : 0.0;
+  if (blockIdx.x < kSbLeads) {
+    zs[tid] = z[kp + tid];
+    __syncthreads();
+    const double y = sb_lead(X + (size_t)(kp / SB) * SB * SB, zs, Q + (size_t)(k0 / SB) * SB * SB,
+                             xs, part, 16 * blockIdx.x, tid);
+    if (tid < 16) x[kp + 16 * blockIdx.x + tid] = y;
+    return;
   }
-  const double y = (a0 + a1) + (a2 + a3);
-  xs[tid] = y;
-  if (blockIdx.x == 0 && tid < ncol) x[k0 + tid] = y;
   __syncthreads();
-  const long j = 64L * blockIdx.x + lane;
+  const int cl = tid & 31, sl = tid >> 5;
+  const long j = min(32L * (blockIdx.x - kSbLeads) + cl, (long)kp - 1);
+  double p[32];
+#pragma unroll
+  for (int e = 0; e < 32; e++) p[e] = L[(long)min(k0 + 32 * sl + e, n - 1) * n + j];
   double t0 = 0.0, t1 = 0.0;
-  if (j < k0) {
-    const int c1 = min(ncol, 64 * (wave + 1));
-#pragma unroll 8
-    for (int c = 64 * wave; c < c1; c += 2) {
-      t0 += L[(long)(k0 + c) * n + j] * xs[c];
-      if (c + 1 < c1) t1 += L[(long)(k0 + c + 1) * n + j] * xs[c + 1];
-    }
+#pragma unroll
+  for (int e = 0; e < 32; e += 2) {
+    t0 += p[e] * xs[32 * sl + e];
+    t1 += p[e + 1] * xs[32 * sl + e + 1];
   }
-  part[wave][lane] = t0 + t1;
+  colp[sl][cl] = t0 + t1;
   __syncthreads();
-  if (wave == 0 && j < k0) z[j] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+  const long jj = 32L * (blockIdx.x - kSbLeads) + tid;
+  if (tid < 32 && jj < kp) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) s += colp[q][tid];
+    z[jj] -= s;
+  }
 }
 
 int launch_potrs(const double* L, const double* Linv, int n, const double* b, double* x,
@@ -1197,28 +1326,45 @@ int launch_potrs(const double* L, const double* Linv, int n, const double* b, do
   const int nsb = (int)sb_count(n);
   double* X = const_cast<double*>(Linv) + 2 * (size_t)((n + NB - 1) / NB) * NB * NB;
   double* XT = X + (size_t)nsb * SB * SB;
+  double* PT = XT + (size_t)nsb * SB * SB;
+  double* Q = PT + (size_t)nsb * SB * SB;
   {
     KScope ks("trsv_inv", st);
     hipLaunchKernelGGL(k_sb_inverse, dim3(nsb, 4), dim3(256), 0, st, L, Linv, n, X, XT);
     count_launch();
+    if (nsb > 1) {
+      hipLaunchKernelGGL(k_sb_couple, dim3(nsb - 1, 16, 2), dim3(256), 0, st, L, X, n, PT, Q);
+      count_launch();
+    }
   }
+  // forward: w = b, solved block by block into x (z = L^-1 b)
   NQS_HIP_CHECK(hipMemcpyAsync(w, b, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st));
-  for (int k0 = 0; k0 < n; k0 += SB) {
-    const int ncol = (n - k0) < SB ? (n - k0) : SB;
-    const int rows = n - k0 - ncol;
-    const int grid = rows > 0 ? (rows + 63) / 64 : 1;
+  {
     KScope ks("trsv", st);
-    hipLaunchKernelGGL(k_sbsv_fwd, dim3(grid), dim3(256), 0, st, L, XT, n, k0, ncol, w, x);
+    hipLaunchKernelGGL(k_sbsv_first, dim3(1), dim3(512), 0, st, XT, 0, min(SB, n), w, x);
     count_launch();
   }
-  // x now holds z = L^-1 b; back substitution in place on w = z
-  NQS_HIP_CHECK(hipMemcpyAsync(w, x, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st));
-  const int last = ((n - 1) / SB) * SB;
-  for (int k0 = last; k0 >= 0; k0 -= SB) {
-    const int ncol = (n - k0) < SB ? (n - k0) : SB;
-    const int grid = k0 > 0 ? (k0 + 63) / 64 : 1;
+  for (int k0 = 0; k0 + SB < n; k0 += SB) {
+    const int beyond = n - (k0 + 2 * SB);
+    const int grid = kSbLeads + (beyond > 0 ? (beyond + 31) / 32 : 0);
     KScope ks("trsv", st);
-    hipLaunchKernelGGL(k_sbsv_bwd, dim3(grid), dim3(256), 0, st, L, X, n, k0, ncol, w, x);
+    hipLaunchKernelGGL(k_sbsv_fwd, dim3(grid), dim3(256), 0, st, L, XT, PT, n, k0, w, x);
+    count_launch();
+  }
+  // backward on w = z, from the last block down
+  NQS_HIP_CHECK(hipMemcpyAsync(w, x, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st));
+  const int last = (nsb - 1) * SB;
+  {
+    KScope ks("trsv", st);
+    hipLaunchKernelGGL(k_sbsv_first, dim3(1), dim3(512), 0, st, X, last, n - last, w, x);
+    count_launch();
+  }
+  for (int k0 = last; k0 > 0; k0 -= SB) {
+    const int ncol = (n - k0) < SB ? (n - k0) : SB;
+    const int left = k0 - SB;
+    const int grid = kSbLeads + (left > 0 ? (left + 31) / 32 : 0);
+    KScope ks("trsv", st);
+    hipLaunchKernelGGL(k_sbsv_bwd, dim3(grid), dim3(256), 0, st, L, X, Q, n, k0, ncol, w, x);
     count_launch();
   }
   NQS_HIP_CHECK(hipGetLastError());
