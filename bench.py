@@ -162,29 +162,52 @@ TAGS = ["mh", "ar_mh", "forward", "eloc", "gen_O", "colsum", "gemm", "gemm_potrf
         "trsv", "trsv_inv", "gram_f", "multi_prep", "multi_gemm_z", "multi_rows", "multi_gemm_g", "multi_vec"]
 
 
+def potrf_widths():
+    """Wide-panel width rules of the Cholesky, as the library reads them from
+    NQS_HIP_POTRF_WIDTHS ("w1:r1,w2:r2,...,wk": w1 while more than r1 columns
+    are left, ...)."""
+    rules = []
+    for part in os.environ.get("NQS_HIP_POTRF_WIDTHS", "512").split(","):
+        if not part:
+            continue
+        w, _, above = part.partition(":")
+        w = int(w)
+        if w >= 128 and w % 128 == 0:
+            rules.append((w, int(above) if above else 0))
+    return rules or [(512, 0)]
+
+
 def potrf_gemm_flops(B):
     """FLOP of the blocked Cholesky's GEMM updates of a B x B matrix (lower
-    elements only): wide panels of `width` columns made of 128-column outer
-    panels; each wide panel's second outer panel first takes the first's
-    rank-128 update, then the next wide panel's columns and the rest of the
-    trailing triangle take one rank-`width` update."""
-    width = 128 if os.environ.get("NQS_HIP_POTRF_WIDTH") == "128" else 256
+    elements only): wide panels made of 128-column outer panels; inside a
+    wide panel each outer panel's rank-128 update reaches the wide panel's
+    later columns, then the next wide panel's columns and the rest of the
+    trailing triangle take one update of the wide panel's rank."""
+    rules = potrf_widths()
+
+    def width(rem):
+        for w, above in rules:
+            if rem > above:
+                return min(w, rem)
+        return min(rules[-1][0], rem)
 
     def lower_cols(r0, w):   # elements of columns [r0, r0 + w), rows >= r0, lower part
         return w * (w + 1) // 2 + (B - r0 - w) * w
 
-    def mid(k0):
-        return sum(2 * 128 * lower_cols(j, min(128, B - j))
-                   for j in range(k0 + 128, min(k0 + width, B), 128))
+    def inside(k0, w):
+        end = min(k0 + w, B)
+        return sum(2 * 128 * lower_cols(j + 128, end - j - 128)
+                   for j in range(k0, end, 128) if j + 128 < end)
 
-    tot, k0 = mid(0), 0
-    while k0 + width < B:
-        wd = width
+    k0 = 0
+    wd = width(B)
+    tot = inside(0, wd)
+    while k0 + wd < B:
         r0 = k0 + wd
-        w1 = min(width, B - r0)
+        w1 = width(B - r0)
         m = B - r0 - w1
-        tot += 2 * wd * lower_cols(r0, w1) + mid(r0) + 2 * wd * (m * (m + 1) // 2)
-        k0 = r0
+        tot += 2 * wd * lower_cols(r0, w1) + inside(r0, w1) + 2 * wd * (m * (m + 1) // 2)
+        k0, wd = r0, w1
     return tot
 
 

@@ -540,13 +540,13 @@ int launch_gemm_nt(int M, int N, int K, double alpha, const double* A, long lda,
 // 64-column inner panels; one launch (k_panel) per inner panel factors the
 // diagonal block, inverts it, and solves every row below it (the second
 // inner panel first takes the first's rank-64 update inside the same
-// launch).  The trailing matrix takes one rank-256 update per wide panel of
-// two outer panels (the second outer panel takes the first's rank-128
-// update before it is factored) on the large-tile MFMA GEMM.  With
-// lookahead, the next wide panel's columns are updated first and that panel
-// is factored on a second stream while the rest of the trailing matrix is
-// updated.  The inverses of the 64x64 diagonal blocks are kept for the
-// triangular solves.
+// launch).  The trailing matrix takes one update per wide panel of outer
+// panels (rank 512 by default; inside a wide panel each outer panel's
+// rank-128 update reaches the wide panel's later columns first) on the
+// large-tile MFMA GEMM.  With lookahead, the next wide panel's columns are
+// updated first and that panel is factored on a second stream while the
+// rest of the trailing matrix is updated.  The inverses of the 64x64
+// diagonal blocks are kept for the triangular solves.
 // --------------------------------------------------------------------------
 constexpr int NB = 64, NBO = 128;
 
@@ -894,52 +894,73 @@ static int outer_panel(double* A, int n, int K0, double* Linv, int* fail_dev, hi
   return 0;
 }
 
-// wide panel at K0 of up to `width` columns: 128-column outer panels, each
-// after the rank-128 update of its columns by the outer panel left of it
+// wide panel at K0 of up to `width` columns (a multiple of 128): each
+// 128-column outer panel is factored, then the wide panel's columns right
+// of it take its rank-128 update
 static int wide_panel(double* A, int n, int K0, int width, double* Linv, int* fail_dev,
                       hipStream_t st) {
-  for (int j = K0; j < K0 + width && j < n; j += NBO) {
-    if (j > K0) {
-      const int w = (n - j) < NBO ? (n - j) : NBO;
-      const double* L = A + (long)j * n + j - NBO;
-      if (gemm_big_impl(n - j, w, NBO, -1.0, L, n, L, n, 1.0, A + (long)j * n + j, n, 0, 1,
-                        fail_dev, "gemm_potrf", st))
+  const int end = (K0 + width) < n ? K0 + width : n;
+  for (int j = K0; j < end; j += NBO) {
+    if (outer_panel(A, n, j, Linv, fail_dev, st)) return -1;
+    const int c0 = j + NBO;
+    if (c0 < end) {
+      const double* L = A + (long)c0 * n + j;
+      if (gemm_big_impl(n - c0, end - c0, NBO, -1.0, L, n, L, n, 1.0, A + (long)c0 * n + c0, n, 0,
+                        1, fail_dev, "gemm_potrf", st))
         return -1;
     }
-    if (outer_panel(A, n, j, Linv, fail_dev, st)) return -1;
   }
   return 0;
 }
 
+// Width of the wide panel that starts with `rem` columns left to factor:
+// NQS_HIP_POTRF_WIDTHS="w1:r1,w2:r2,...,wk" takes w1 while rem > r1, then
+// w2 while rem > r2, ..., then wk (widths multiples of 128).
+static int panel_width(int rem) {
+  static const std::vector<std::pair<int, int>> rules = [] {
+    std::vector<std::pair<int, int>> r;
+    const char* e = getenv("NQS_HIP_POTRF_WIDTHS");
+    const char* spec = e ? e : "512";
+    while (*spec) {
+      const int w = atoi(spec);
+      int above = 0;
+      while (*spec && *spec != ':' && *spec != ',') spec++;
+      if (*spec == ':') above = atoi(++spec);
+      while (*spec && *spec != ',') spec++;
+      if (*spec == ',') spec++;
+      if (w >= NBO && w % NBO == 0) r.push_back({w, above});
+    }
+    if (r.empty()) r.push_back({4 * NBO, 0});
+    return r;
+  }();
+  for (const auto& rule : rules)
+    if (rem > rule.second) return rule.first;
+  return rules.back().first;
+}
+
 int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) {
   if (!g_look) NQS_HIP_CHECK(hipStreamCreateWithFlags(&g_look, hipStreamNonBlocking));
-  // trailing updates of rank 256 (wide panels of two outer panels);
-  // NQS_HIP_POTRF_WIDTH=128: rank 128
-  static const int width = [] {
-    const char* e = getenv("NQS_HIP_POTRF_WIDTH");
-    return (e && atoi(e) == NBO) ? NBO : 2 * NBO;
-  }();
-  const int np = (n + width - 1) / width;
-  while ((int)g_ev.size() < 2 * np) {
-    hipEvent_t e;
-    NQS_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    g_ev.push_back(e);
-  }
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, fail_dev);
   count_launch();
-  if (wide_panel(A, n, 0, width, Linv, fail_dev, st)) return -1;
+  int K0 = 0, w = panel_width(n);
+  if (w > n) w = n;
+  if (wide_panel(A, n, 0, w, Linv, fail_dev, st)) return -1;
   // NQS_HIP_POTRF_LOOKAHEAD=0: the same schedule on one stream
   static const int lookahead = [] {
     const char* e = getenv("NQS_HIP_POTRF_LOOKAHEAD");
     return (e && e[0] == '0') ? 0 : 1;
   }();
   hipStream_t side = lookahead ? g_look : st;
-  for (int k = 0; k < np; k++) {
-    const int K0 = k * width;
-    const int w = (n - K0) < width ? (n - K0) : width;
+  for (int k = 0;; k++) {
     const int R0 = K0 + w;
     if (R0 >= n) break;
-    const int w1 = (n - R0) < width ? (n - R0) : width;
+    int w1 = panel_width(n - R0);
+    if (w1 > n - R0) w1 = n - R0;
+    while ((int)g_ev.size() < 2 * (k + 1)) {
+      hipEvent_t e;
+      NQS_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      g_ev.push_back(e);
+    }
     const double* L = A + (long)R0 * n + K0;
     // next wide panel's columns [R0, R0 + w1), rows >= R0 (lower part)
     if (gemm_big_impl(n - R0, w1, w, -1.0, L, n, L, n, 1.0, A + (long)R0 * n + R0, n, 0, 1,
@@ -947,7 +968,7 @@ int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) 
       return -1;
     NQS_HIP_CHECK(hipEventRecord(g_ev[2 * k], st));
     NQS_HIP_CHECK(hipStreamWaitEvent(side, g_ev[2 * k], 0));
-    if (wide_panel(A, n, R0, width, Linv, fail_dev, side)) return -1;
+    if (wide_panel(A, n, R0, w1, Linv, fail_dev, side)) return -1;
     NQS_HIP_CHECK(hipEventRecord(g_ev[2 * k + 1], side));
     // the rest of the trailing matrix: rows and columns >= R0 + w1
     const int R1 = R0 + w1;
@@ -958,6 +979,8 @@ int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) 
         return -1;
     }
     NQS_HIP_CHECK(hipStreamWaitEvent(st, g_ev[2 * k + 1], 0));
+    K0 = R0;
+    w = w1;
   }
   hipLaunchKernelGGL(k_diag_copy, dim3((unsigned)((n + NB - 1) / NB)), dim3(256), 0, st, A, n, Linv,
                      fail_dev);
