@@ -141,7 +141,7 @@ def step(lib, w, st):
 # kernel symbol (as rocprofv3 names it) behind each timing tag
 SYMBOL = {"gemm_potrf": "nqs::k_gemm_big<1>", "gemm": "nqs::k_gemm_big<0>",
           "potrf_panel": "nqs::k_panel",
-          "eloc": "nqs::k_eloc<double, true, 1>", "gen_O": "nqs::k_gen_O<double>",
+          "eloc": "nqs::k_eloc_terms<double, true>", "gen_O": "nqs::k_gen_O<double>",
           "colsum": "nqs::k_colsum_partial", "mh": "nqs::k_mh<double, true, 64, 4>",
           "trsv": "nqs::k_sbsv_fwd"}
 

@@ -297,7 +297,7 @@ bool recognise(nqs_log_amp_fn_t fn, void* user, AmpSource& src) {
 // ---------------------------------------------------------------------------
 struct Workspace {
   DBuf spins, t_re, t_im, la, ar, e_re, e_im, O, Oi, colwork, u1, u2, vecs, T, diag, y, scal,
-      cg, fail, ham, sublat, hostla, hostar, linv, T2, Sd;
+      cg, fail, ham, sublat, hostla, hostar, linv, T2, Sd, elw;
 };
 Workspace g_ws;
 
@@ -429,8 +429,10 @@ int eloc_device_T(const nqs_config_t* cfg, int Lx, int Ly, int per, const AmpSou
   CKR(forward_t<R>(src.a, d_spins, B, nullptr, nullptr));
   ModelView<R> v;
   CKR(model_view<R>(src.a, v));
+  const size_t wb = nqs::eloc_ws_bytes<R>(v, H, B);
+  CKR(g_ws.elw.ensure(wb));
   return nqs::launch_eloc<R>(v, H, d_spins, B, g_ws.t_re.as<R>(), g_ws.t_im.as<R>(), d_re, d_im,
-                             g_stream);
+                             g_stream, g_ws.elw.p, wb);
 }
 
 int eloc_device(const nqs_config_t* cfg, int Lx, int Ly, int per, int prec, const AmpSource& src,

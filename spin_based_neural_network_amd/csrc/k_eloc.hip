@@ -119,10 +119,27 @@ __global__ __launch_bounds__(256) void k_eloc(ModelView<R> m, HamDev H,
     for (int h0 = 0; h0 < M; h0 += HT) {
       const int hn = (M - h0) < HT ? (M - h0) : HT;
       const size_t toff = (size_t)h0 * N;
-      for (int e = tid; e < hn * N; e += blockDim.x) {
-        cr[e] = m.C_re[toff + e];
-        sr[e] = m.S_re[toff + e];
-        if (CPLX) { ci[e] = m.C_im[toff + e]; si[e] = m.S_im[toff + e]; }
+      // table rows in batches of 8 per thread: every load of a batch is in
+      // flight at once (indices clamped into the tile, stores guarded)
+      const int tn = hn * N;
+      for (int e0 = 0; e0 < tn; e0 += 8 * 256) {
+        R vcr[8], vsr[8], vci[8], vsi[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int e = min(e0 + tid + 256 * u, tn - 1);
+          vcr[u] = m.C_re[toff + e];
+          vsr[u] = m.S_re[toff + e];
+          if (CPLX) { vci[u] = m.C_im[toff + e]; vsi[u] = m.S_im[toff + e]; }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int e = e0 + tid + 256 * u;
+          if (e < tn) {
+            cr[e] = vcr[u];
+            sr[e] = vsr[u];
+            if (CPLX) { ci[e] = vci[u]; si[e] = vsi[u]; }
+          }
+        }
       }
       for (int e = tid; e < nsb * hn; e += blockDim.x) {
         const int s = e / hn, hh = e - s * hn;
@@ -241,6 +258,296 @@ __global__ __launch_bounds__(256) void k_eloc(ModelView<R> m, HamDev H,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Term-major local energies for many samples.  The off-diagonal terms
+// (flips, then pairs, in the Hamiltonian's order) get hidden-unit rows of
+// their own once per call:
+//   flip of a:    Cq = C_a,                 Sq = S_a
+//   pair (a, b):  Cq = C_a C_b - S_a S_b,   Sq = S_a C_b - C_a S_b
+// (C = cosh 2W, S = sinh 2W, complex products for complex RBMs), so that the
+// factor of hidden unit h in psi(s')/psi(s) is Cq + t_h (-s_a) Sq.  A wave
+// then holds 64 samples in its lanes and 8 terms in registers; per tile of
+// hidden units, the samples' t_h and the workgroup's 32 term rows are staged
+// in LDS by coalesced loads (term rows are read at wave-uniform addresses),
+// and the running products are rescaled by a power of two per tile.  Each
+// (term, sample) contribution goes to res[t][b] (zero for inactive pairs);
+// a second kernel adds the diagonal part and the contributions in term
+// order, so E_loc sums in the same order as the per-sample item lists.
+// ---------------------------------------------------------------------------
+template <typename R, bool CPLX>
+__global__ void k_term_tables(ModelView<R> m, HamDev H, int Tp, R* __restrict__ Cr,
+                              R* __restrict__ Ci, R* __restrict__ Sr, R* __restrict__ Si) {
+  const int T = H.n_flip + H.n_pair, M = m.M;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)Tp * M;
+       e += (long)gridDim.x * blockDim.x) {
+    const int t = (int)(e / M), h = (int)(e - (long)t * M);
+    if (t >= T) {
+      // padding rows: factor 1
+      Cr[e] = (R)1;
+      Sr[e] = (R)0;
+      if (CPLX) { Ci[e] = (R)0; Si[e] = (R)0; }
+      continue;
+    }
+    const int a = t < H.n_flip ? H.flip_i[t] : H.pair_a[t - H.n_flip];
+    const int b = t < H.n_flip ? -1 : H.pair_b[t - H.n_flip];
+    const R car = m.CT_re[(size_t)a * M + h], sar = m.ST_re[(size_t)a * M + h];
+    const R cai = CPLX ? m.CT_im[(size_t)a * M + h] : (R)0;
+    const R sai = CPLX ? m.ST_im[(size_t)a * M + h] : (R)0;
+    if (b < 0) {
+      Cr[e] = car;
+      Sr[e] = sar;
+      if (CPLX) { Ci[e] = cai; Si[e] = sai; }
+      continue;
+    }
+    const R cbr = m.CT_re[(size_t)b * M + h], sbr = m.ST_re[(size_t)b * M + h];
+    if (CPLX) {
+      const R cbi = m.CT_im[(size_t)b * M + h], sbi = m.ST_im[(size_t)b * M + h];
+      Cr[e] = (car * cbr - cai * cbi) - (sar * sbr - sai * sbi);
+      Ci[e] = (car * cbi + cai * cbr) - (sar * sbi + sai * sbr);
+      Sr[e] = (sar * cbr - sai * cbi) - (car * sbr - cai * sbi);
+      Si[e] = (sar * cbi + sai * cbr) - (car * sbi + cai * sbr);
+    } else {
+      Cr[e] = car * cbr - sar * sbr;
+      Sr[e] = sar * cbr - car * sbr;
+    }
+  }
+}
+
+constexpr int kEtTerms = 8;    // terms per wave (4 waves per workgroup)
+
+// hidden units per tile (the products are rescaled once per tile)
+template <typename R, bool CPLX>
+struct EtTile { static constexpr int H = (sizeof(R) == 8 && !CPLX) ? 32 : 16; };
+
+template <typename R, bool CPLX>
+__global__ __launch_bounds__(256) void k_eloc_terms(ModelView<R> m, HamDev H,
+                                                    const int* __restrict__ spins, int B,
+                                                    const R* __restrict__ t_re,
+                                                    const R* __restrict__ t_im,
+                                                    const R* __restrict__ Cr,
+                                                    const R* __restrict__ Ci,
+                                                    const R* __restrict__ Sr,
+                                                    const R* __restrict__ Si,
+                                                    double* __restrict__ res_r,
+                                                    double* __restrict__ res_i) {
+  constexpr int HT = EtTile<R, CPLX>::H;
+  constexpr int NTB = 4 * kEtTerms;                 // terms per workgroup
+  __shared__ R tl_r[64][HT + 1];
+  __shared__ R tl_i[CPLX ? 64 : 1][HT + 1];
+  __shared__ R tc_r[NTB][HT], ts_r[NTB][HT];
+  __shared__ R tc_i[CPLX ? NTB : 1][HT], ts_i[CPLX ? NTB : 1][HT];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int T = H.n_flip + H.n_pair, M = m.M, N = m.N;
+  const int b0 = blockIdx.x * 64;
+  const int bl = min(b0 + lane, B - 1);
+  const int tb0 = blockIdx.y * NTB;                 // workgroup's first term
+  const int t0 = tb0 + wave * kEtTerms;             // wave's first term
+  const int nt = max(0, min(kEtTerms, T - t0));
+  R msa[kEtTerms];
+  unsigned active = 0;
+#pragma unroll
+  for (int j = 0; j < kEtTerms; j++) {
+    msa[j] = (R)0;
+    if (j < nt) {
+      const int t = t0 + j;
+      const int a = t < H.n_flip ? H.flip_i[t] : H.pair_a[t - H.n_flip];
+      const int sa = spins[(size_t)bl * N + a];
+      msa[j] = (R)(-sa);
+      const bool act = t < H.n_flip || spins[(size_t)bl * N + H.pair_b[t - H.n_flip]] != sa;
+      active |= act ? (1u << j) : 0u;
+    }
+  }
+  R pr[kEtTerms], pim[kEtTerms];
+  int ex[kEtTerms];
+#pragma unroll
+  for (int j = 0; j < kEtTerms; j++) { pr[j] = (R)1; pim[j] = (R)0; ex[j] = 0; }
+  for (int h0 = 0; h0 < M; h0 += HT) {
+    const int hn = min(HT, M - h0);
+    __syncthreads();
+    {
+      // t of the 64 samples and the workgroup's term rows for this tile; all
+      // loads of a thread in flight together (indices clamped into the tile)
+      constexpr int NQT = 64 * HT / 256, NQC = NTB * HT / 256;
+      R vtr[NQT], vti[NQT], vcr[NQC], vci[NQC], vsr[NQC], vsi[NQC];
+#pragma unroll
+      for (int u = 0; u < NQT; u++) {
+        const int e = tid + 256 * u, sm = e / HT, hh = e - sm * HT;
+        const size_t g = (size_t)min(b0 + sm, B - 1) * M + h0 + min(hh, hn - 1);
+        vtr[u] = t_re[g];
+        if (CPLX) vti[u] = t_im[g];
+      }
+#pragma unroll
+      for (int u = 0; u < NQC; u++) {
+        const int e = tid + 256 * u, r = e / HT, hh = e - r * HT;
+        const size_t g = (size_t)(tb0 + r) * M + h0 + min(hh, hn - 1);
+        vcr[u] = Cr[g];
+        vsr[u] = Sr[g];
+        if (CPLX) { vci[u] = Ci[g]; vsi[u] = Si[g]; }
+      }
+#pragma unroll
+      for (int u = 0; u < NQT; u++) {
+        const int e = tid + 256 * u, sm = e / HT, hh = e - sm * HT;
+        tl_r[sm][hh] = vtr[u];
+        if (CPLX) tl_i[sm][hh] = vti[u];
+      }
+#pragma unroll
+      for (int u = 0; u < NQC; u++) {
+        const int e = tid + 256 * u, r = e / HT, hh = e - r * HT;
+        tc_r[r][hh] = vcr[u];
+        ts_r[r][hh] = vsr[u];
+        if (CPLX) { tc_i[r][hh] = vci[u]; ts_i[r][hh] = vsi[u]; }
+      }
+    }
+    __syncthreads();
+    if (nt == 0) continue;
+    const int rw = wave * kEtTerms;
+    for (int hh = 0; hh < hn; hh++) {
+      const R tr = tl_r[lane][hh];
+      const R ti = CPLX ? tl_i[lane][hh] : (R)0;
+      // term rows are padded to whole workgroups: no test on j
+#pragma unroll
+      for (int j = 0; j < kEtTerms; j++) {
+        if (CPLX) {
+          const R sr = msa[j] * ts_r[rw + j][hh], si = msa[j] * ts_i[rw + j][hh];
+          const R wr = tc_r[rw + j][hh] + (tr * sr - ti * si);
+          const R wi = tc_i[rw + j][hh] + (tr * si + ti * sr);
+          const R nr = pr[j] * wr - pim[j] * wi;
+          pim[j] = pr[j] * wi + pim[j] * wr;
+          pr[j] = nr;
+        } else {
+          pr[j] *= tc_r[rw + j][hh] + tr * msa[j] * ts_r[rw + j][hh];
+        }
+      }
+    }
+    // rescale by a power of two (exact)
+#pragma unroll
+    for (int j = 0; j < kEtTerms; j++) {
+      int e2;
+      if (CPLX) {
+        (void)frexp(fmax(fabs(pr[j]), fabs(pim[j])), &e2);
+        pr[j] = ldexp(pr[j], -e2);
+        pim[j] = ldexp(pim[j], -e2);
+      } else {
+        pr[j] = frexp(pr[j], &e2);
+      }
+      ex[j] += e2;
+    }
+  }
+  if (b0 + lane >= B) return;
+#pragma unroll
+  for (int j = 0; j < kEtTerms; j++) {
+    if (j >= nt) continue;
+    const int t = t0 + j;
+    const size_t o = (size_t)t * B + b0 + lane;
+    if (!(active & (1u << j))) {
+      res_r[o] = 0.0;
+      res_i[o] = 0.0;
+      continue;
+    }
+    const int a = t < H.n_flip ? H.flip_i[t] : H.pair_a[t - H.n_flip];
+    const int bb = t < H.n_flip ? -1 : H.pair_b[t - H.n_flip];
+    const double s_a = -(double)msa[j];
+    double vr = -2.0 * (double)m.a_re[a] * s_a;
+    double vi = CPLX ? -2.0 * (double)m.a_im[a] * s_a : 0.0;
+    if (bb >= 0) {
+      vr += 2.0 * (double)m.a_re[bb] * s_a;
+      if (CPLX) vi += 2.0 * (double)m.a_im[bb] * s_a;
+    }
+    double lm, ph_r = 1.0, ph_i = 0.0;
+    if (CPLX) {
+      const double xr = (double)pr[j], xi = (double)pim[j];
+      const double mg = sqrt(xr * xr + xi * xi);
+      lm = log(mg) + (double)ex[j] * 0.69314718055994530942;
+      ph_r = xr / mg;
+      ph_i = xi / mg;
+      const double c = cos(vi), sn = sin(vi);
+      const double nr = ph_r * c - ph_i * sn;
+      ph_i = ph_r * sn + ph_i * c;
+      ph_r = nr;
+    } else {
+      lm = log((double)pr[j]) + (double)ex[j] * 0.69314718055994530942;
+    }
+    const double mag = exp(lm + vr);
+    double sign = 1.0;
+    if (H.sublat) {
+      const int par = H.sublat[a] + (bb >= 0 ? H.sublat[bb] : 0);
+      if (par & 1) sign = -1.0;
+    }
+    const double coef = t < H.n_flip ? H.flip_c[t] : H.pair_c[t - H.n_flip];
+    res_r[o] = coef * sign * mag * ph_r;
+    res_i[o] = coef * sign * mag * ph_i;
+  }
+}
+
+// E_loc(b) = diagonal part + sum over terms in order of res[t][b]
+__global__ __launch_bounds__(256) void k_eloc_terms_sum(HamDev H, const int* __restrict__ spins,
+                                                        int B, int N,
+                                                        const double* __restrict__ res_r,
+                                                        const double* __restrict__ res_i,
+                                                        double* __restrict__ e_re,
+                                                        double* __restrict__ e_im) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const int* s = spins + (size_t)b * N;
+  double diag = 0.0;
+  for (int d = 0; d < H.n_diag; d++)
+    diag += H.diag_c[d] * (double)(s[H.diag_a[d]] * s[H.diag_b[d]]);
+  const int T = H.n_flip + H.n_pair;
+  double orr = 0.0, oi = 0.0;
+  for (int t = 0; t < T; t++) {
+    orr += res_r[(size_t)t * B + b];
+    oi += res_i[(size_t)t * B + b];
+  }
+  e_re[b] = diag + orr;
+  if (e_im) e_im[b] = oi;
+}
+
+// term rows padded to whole workgroups (4 waves x 16 terms)
+static inline int padded_terms(int T) { return (T + 4 * kEtTerms - 1) / (4 * kEtTerms) * (4 * kEtTerms); }
+
+template <typename R>
+size_t eloc_ws_bytes(const ModelView<R>& m, const HamDev& h, int B) {
+  const size_t T = (size_t)(h.n_flip + h.n_pair);
+  const size_t tab = (size_t)padded_terms((int)T) * (size_t)m.M * sizeof(R) * (m.cplx ? 4 : 2);
+  return ((tab + 255) & ~(size_t)255) + 2 * T * (size_t)B * sizeof(double) + 256;
+}
+
+template <typename R, bool CPLX>
+static int eloc_terms_go(const ModelView<R>& m, const HamDev& h, const int* spins, int B,
+                         const R* t_re, const R* t_im, double* e_re, double* e_im, void* ws,
+                         hipStream_t st) {
+  const int T = h.n_flip + h.n_pair, M = m.M, Tp = padded_terms(T);
+  const size_t TM = (size_t)Tp * M;
+  unsigned char* p = (unsigned char*)ws;
+  R* Cr = (R*)p;
+  R* Sr = Cr + TM;
+  R* Ci = CPLX ? Sr + TM : nullptr;
+  R* Si = CPLX ? Ci + TM : nullptr;
+  const size_t tab = TM * sizeof(R) * (CPLX ? 4 : 2);
+  double* res_r = (double*)(p + ((tab + 255) & ~(size_t)255));
+  double* res_i = res_r + (size_t)T * B;
+  KScope ks("eloc", st);
+  if (T > 0) {
+    if (TM > 0) {
+      long blocks = ((long)TM + 255) / 256;
+      if (blocks > 4096) blocks = 4096;
+      hipLaunchKernelGGL((k_term_tables<R, CPLX>), dim3((unsigned)blocks), dim3(256), 0, st, m, h,
+                         Tp, Cr, Ci, Sr, Si);
+      count_launch();
+    }
+    const dim3 grid((B + 63) / 64, Tp / (4 * kEtTerms));
+    hipLaunchKernelGGL((k_eloc_terms<R, CPLX>), grid, dim3(256), 0, st, m, h, spins, B, t_re, t_im,
+                       Cr, Ci, Sr, Si, res_r, res_i);
+    count_launch();
+  }
+  hipLaunchKernelGGL(k_eloc_terms_sum, dim3((B + 255) / 256), dim3(256), 0, st, h, spins, B, m.N,
+                     res_r, res_i, e_re, e_im);
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
 template <typename R, bool CPLX, int IPT>
 static int eloc_go(const ModelView<R>& m, const HamDev& h, const int* spins, int B,
                    const R* t_re, const R* t_im, double* e_re, double* e_im, hipStream_t st) {
@@ -274,11 +581,15 @@ static int eloc_go(const ModelView<R>& m, const HamDev& h, const int* spins, int
 
 template <typename R>
 int launch_eloc(const ModelView<R>& m, const HamDev& h, const int* spins, int B,
-                const R* t_re, const R* t_im, double* e_re, double* e_im, hipStream_t st) {
+                const R* t_re, const R* t_im, double* e_re, double* e_im, hipStream_t st,
+                void* ws, size_t ws_bytes) {
   if (B <= 0) return 0;
   const int T = h.n_flip + h.n_pair;
   if (m.N > kMaxSites || T >= 65536) return -1;
   const bool c = m.cplx != 0;
+  if (ws && ws_bytes >= eloc_ws_bytes(m, h, B))
+    return c ? eloc_terms_go<R, true>(m, h, spins, B, t_re, t_im, e_re, e_im, ws, st)
+             : eloc_terms_go<R, false>(m, h, spins, B, t_re, t_im, e_re, e_im, ws, st);
   if (T <= 256) {
     return c ? eloc_go<R, true, 1>(m, h, spins, B, t_re, t_im, e_re, e_im, st)
              : eloc_go<R, false, 1>(m, h, spins, B, t_re, t_im, e_re, e_im, st);
@@ -334,7 +645,9 @@ int launch_eloc_from_logamps(const HamDev& h, const int* spins, int B, int N, in
   return 0;
 }
 
-template int launch_eloc<double>(const ModelView<double>&, const HamDev&, const int*, int, const double*, const double*, double*, double*, hipStream_t);
-template int launch_eloc<float>(const ModelView<float>&, const HamDev&, const int*, int, const float*, const float*, double*, double*, hipStream_t);
+template int launch_eloc<double>(const ModelView<double>&, const HamDev&, const int*, int, const double*, const double*, double*, double*, hipStream_t, void*, size_t);
+template int launch_eloc<float>(const ModelView<float>&, const HamDev&, const int*, int, const float*, const float*, double*, double*, hipStream_t, void*, size_t);
+template size_t eloc_ws_bytes<double>(const ModelView<double>&, const HamDev&, int);
+template size_t eloc_ws_bytes<float>(const ModelView<float>&, const HamDev&, int);
 
 }  // namespace nqs

@@ -200,10 +200,15 @@ int launch_bits_append(const int* spins, long n, int N, uint64_t* win, long cap,
 int launch_spins_to_bits(const int* spins, long n, int N, uint64_t* out, hipStream_t st);
 
 // ---- local energy (k_eloc.hip) -------------------------------------------
+// With a workspace of at least eloc_ws_bytes(m, h, B) bytes the term-major
+// kernels run (term tables, 64 samples per wave, fixed-order sums);
+// without one, the per-sample item-list kernel.
+template <typename R>
+size_t eloc_ws_bytes(const ModelView<R>& m, const HamDev& h, int B);
 template <typename R>
 int launch_eloc(const ModelView<R>& m, const HamDev& h, const int* spins, int B,
                 const R* t_re, const R* t_im, double* e_re, double* e_im,
-                hipStream_t st);
+                hipStream_t st, void* ws = nullptr, size_t ws_bytes = 0);
 // E_loc from externally evaluated log-amplitudes of every connected
 // configuration (host-callback adaptor): la/ar of sample b at [b][0], of
 // its k-th connected configuration at [b][1+k] in term order.
