@@ -1627,8 +1627,8 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
       CKR(g_ws.T.ensure(sizeof(double) * (size_t)B * B));
       T = g_ws.T.as<double>();
       if (prec == NQS_HIP_FP32) CKR(forward_t<double>(a, d_samples, B, nullptr, nullptr));
-      CKR(g_ws.T2.ensure(sizeof(double) * (size_t)B * B));
-      CKR(g_ws.Sd.ensure(sizeof(double) * (size_t)B * N));
+      CKR(g_ws.T2.ensure(sizeof(double) * (size_t)((B + 127) / 128) * B));
+      CKR(g_ws.Sd.ensure(sizeof(double) * (size_t)B * (N + 2)));
       const int cpl = a->kind == NQS_ANSATZ_COMPLEX_RBM;
       CKR(nqs::launch_gram_factored(d_samples, B, N, g_ws.t_re.as<double>(),
                                     cpl ? g_ws.t_im.as<double>() : nullptr, a->M, eps, T,

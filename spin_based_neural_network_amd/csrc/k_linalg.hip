@@ -1401,17 +1401,18 @@ int launch_potrs(const double* L, const double* Linv, int n, const double* b, do
 // T = C G C / Ns + eps I with C = I - 1/Ns: T_ij = (G_ij - r_i - r_j + g)/Ns
 // where r_i = mean_j G_ij and g = mean_i r_i.  Lower triangles only.
 // --------------------------------------------------------------------------
-__global__ void k_spins_to_f64(const int* __restrict__ s, double* __restrict__ out, long n) {
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
-    out[e] = (double)s[e];
-}
-__global__ void k_gram_combine(double* T, const double* __restrict__ UU, int B) {
-  const long total = (long)B * B;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
-       e += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(e / B), j = (int)(e % B);
-    if (j <= i) T[e] = (1.0 + T[e]) * (1.0 + UU[e]) - 1.0;
-  }
+// spin rows as bit masks, nw = ceil(N / 64) words per sample, bit 1 <-> s = -1
+__global__ void k_spins_to_mask(const int* __restrict__ s, int B, int N, int nw,
+                                unsigned long long* __restrict__ out) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)B * nw) return;
+  const long b = e / nw;
+  const int w = (int)(e - b * nw);
+  const int i1 = min(N, 64 * (w + 1));
+  unsigned long long m = 0;
+  for (int i = 64 * w; i < i1; i++)
+    if (s[b * N + i] < 0) m |= 1ull << (i - 64 * w);
+  out[e] = m;
 }
 // r_i = (sum_{j<=i} G_ij + sum_{j>i} G_ji) / B from the lower triangle, in
 // fixed order: row part (one wave per row, coalesced), column part (two
@@ -1459,32 +1460,124 @@ __global__ void k_gram_center(double* T, const double* __restrict__ r, const dou
   }
 }
 
+// Factored Gram in one pass: per 128 x 128 lower tile, UU = U1 U1^T
+// (+ U2 U2^T) (K = Ku each) on FP64 MFMA with the same tiling as k_gemm_big;
+// in the epilogue SS_ij = N - 2 popcount(mask_i ^ mask_j) (exact) and
+// G = (1 + SS)(1 + UU) - 1 is written once.
+__global__ __launch_bounds__(512) void k_gram_fused(int B, const unsigned long long* __restrict__ mask,
+                                                    int N, int nw, const double* __restrict__ U1,
+                                                    const double* __restrict__ U2, int Ku,
+                                                    double* __restrict__ T) {
+  __shared__ double As[2][BT][BKC + 1];
+  __shared__ double Bs[2][BT][BKC + 1];
+  int tm, tn;
+  lower_tile(blockIdx.x, tm, tn);
+  const int row0 = tm * BT, col0 = tn * BT;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wr = (wave >> 2) * 64, wc = (wave & 3) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  v4d acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+  // acc += X[rows] X[cols]^T over K columns of the row-major B x K matrix X
+  auto kloop = [&](const double* __restrict__ X, int K) {
+    double ra[8], rb[8];
+    const int nch = (K + BKC - 1) / BKC;
+    auto gload = [&](int k0) {
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int e = tid + 512 * q, r = e >> 5, kk = e & 31, gk = k0 + kk;
+        const int ga = row0 + r, gb = col0 + r;
+        ra[q] = (ga < B && gk < K) ? X[(long)ga * K + gk] : 0.0;
+        rb[q] = (gb < B && gk < K) ? X[(long)gb * K + gk] : 0.0;
+      }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int e = tid + 512 * q, r = e >> 5, kk = e & 31;
+        As[buf][r][kk] = ra[q];
+        Bs[buf][r][kk] = rb[q];
+      }
+    };
+    __syncthreads();   // the previous K loop's last reads of LDS are done
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int c = 0; c < nch; c++) {
+      const int buf = c & 1;
+      if (c + 1 < nch) gload((c + 1) * BKC);
+      double a[2][4], b[2][2];
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[0][i] = As[buf][wr + 16 * i + lr][lk];
+#pragma unroll
+      for (int j = 0; j < 2; j++) b[0][j] = Bs[buf][wc + 16 * j + lr][lk];
+#pragma unroll
+      for (int s = 0; s < BKC / 4; s++) {
+        const int cur = s & 1;
+        if (s + 1 < BKC / 4) {
+#pragma unroll
+          for (int i = 0; i < 4; i++) a[cur ^ 1][i] = As[buf][wr + 16 * i + lr][4 * (s + 1) + lk];
+#pragma unroll
+          for (int j = 0; j < 2; j++) b[cur ^ 1][j] = Bs[buf][wc + 16 * j + lr][4 * (s + 1) + lk];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[cur][i], b[cur][j], acc[i][j], 0, 0, 0);
+      }
+      if (c + 1 < nch) sstore(buf ^ 1);
+      __syncthreads();
+    }
+  };
+  if (Ku > 0) {
+    kloop(U1, Ku);
+    if (U2) kloop(U2, Ku);
+  }
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int gc = col0 + wc + 16 * j + lr;
+    const unsigned long long* mc = mask + (size_t)min(gc, B - 1) * nw;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gr = row0 + wr + 16 * i + lk + 4 * r;
+        if (gr < B && gc <= gr) {
+          const unsigned long long* mr = mask + (size_t)gr * nw;
+          int d = 0;
+          for (int w = 0; w < nw; w++) d += __popcll(mr[w] ^ mc[w]);
+          const double ss = (double)(N - 2 * d);
+          T[(long)gr * B + gc] = (1.0 + ss) * (1.0 + acc[i][j][r]) - 1.0;
+        }
+      }
+  }
+}
+
 int launch_gram_factored(const int* spins, int B, int N, const double* U1, const double* U2, int Ku,
-                         double eps, double* T, double* UU, double* Sd, double* r, double* g,
+                         double eps, double* T, double* work, double* Sd, double* r, double* g,
                          hipStream_t st) {
-  const long ns = (long)B * N;
-  long blocks = (ns + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(k_spins_to_f64, dim3((unsigned)blocks), dim3(256), 0, st, spins, Sd, ns);
+  // the spin masks occupy the first B * nw words of the B * (N + 2) scratch Sd
+  const int nw = (N + 63) / 64;
+  unsigned long long* mask = (unsigned long long*)Sd;
+  hipLaunchKernelGGL(k_spins_to_mask, dim3((unsigned)(((long)B * nw + 255) / 256)), dim3(256), 0, st,
+                     spins, B, N, nw, mask);
   count_launch();
   {
     KScope ks("gram_f", st);
-    if (gemm_nt_impl(B, B, N, 1.0, Sd, N, Sd, N, 0.0, T, B, 1, nullptr, st)) return -1;
-    if (Ku > 0) {
-      if (gemm_nt_impl(B, B, Ku, 1.0, U1, Ku, U1, Ku, 0.0, UU, B, 1, nullptr, st)) return -1;
-      if (U2 && gemm_nt_impl(B, B, Ku, 1.0, U2, Ku, U2, Ku, 1.0, UU, B, 1, nullptr, st)) return -1;
-    }
+    const long tmb = (B + BT - 1) / BT;
+    hipLaunchKernelGGL(k_gram_fused, dim3((unsigned)(tmb * (tmb + 1) / 2)), dim3(512), 0, st, B, mask,
+                       N, nw, U1, U2, Ku, T);
+    count_launch();
   }
   long tb = ((long)B * B + 255) / 256;
   if (tb > 65536) tb = 65536;
-  if (Ku > 0) {
-    hipLaunchKernelGGL(k_gram_combine, dim3((unsigned)tb), dim3(256), 0, st, T, UU, B);
-    count_launch();
-  }
   {
     const long nch = (B + 127) / 128;
-    double* work = UU;   // UU has been consumed by k_gram_combine
-    double* part = Sd;   // B doubles of the B*N scratch
+    double* part = Sd + (size_t)B * nw;   // B doubles after the masks
     hipLaunchKernelGGL(k_sym_rowpart, dim3((B + 3) / 4), dim3(256), 0, st, T, B, part);
     hipLaunchKernelGGL(k_sym_colpart, dim3((B + 255) / 256, (unsigned)nch), dim3(256), 0, st, T, B,
                        work);

@@ -250,10 +250,10 @@ int launch_potrs(const double* L, const double* Linv, int n, const double* b, do
                  double* w, hipStream_t st);
 // Factored MinSR Gram T = (O_c O_c^T)/B + eps I (lower triangle) for O rows
 // (s, u, u (x) s); u . u' = U1 U1^T (+ U2 U2^T when U2 is given), Ku columns
-// each (Ku = 0 for mean-field).  UU: B*B scratch, Sd: B*N scratch, r: B
-// scratch, g: device scalar.
+// each (Ku = 0 for mean-field).  work: ceil(B/128)*B scratch, Sd: B*(N+2)
+// scratch, r: B scratch, g: device scalar.
 int launch_gram_factored(const int* spins, int B, int N, const double* U1, const double* U2, int Ku,
-                         double eps, double* T, double* UU, double* Sd, double* r, double* g,
+                         double eps, double* T, double* work, double* Sd, double* r, double* g,
                          hipStream_t st);
 int launch_add_diag(double* A, int n, long ld, double v, hipStream_t st);
 // out [C][R] = in [R][C]^T
