@@ -1497,13 +1497,36 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
                       d_samples));
   }
   tm.mark();
+  // MinSR path: sample space (T = O_c O_c^T / Ns + eps I, Ns x Ns) or, with
+  // NQS_HIP_GRAM_AUTO when it costs less, parameter space
+  // (S = O_c^T O_c / Ns + eps I, P x P): O_c^T (O_c O_c^T/Ns + eps)^-1 =
+  // (O_c^T O_c/Ns + eps)^-1 O_c^T, so both give the same delta.  A
+  // sample-space solve with the factored Gram never forms O: the update is
+  // taken from the RBM factors as well.
+  const bool rbm_like = (a->kind == NQS_ANSATZ_RBM || a->kind == NQS_ANSATZ_COMPLEX_RBM) && !gfn;
+  const bool factored = method == SR_MINSR && rbm_like &&
+                        (s->opt.gram == NQS_HIP_GRAM_FACTORED || s->opt.gram == NQS_HIP_GRAM_AUTO);
+  bool dual = false;
+  if (method == SR_MINSR && s->opt.gram == NQS_HIP_GRAM_AUTO) {
+    const double b = (double)B, p = (double)P;
+    const double gram_b = factored ? b * b * (N + 2.0 * a->M) : b * b * p;
+    const double primal = gram_b + b * b * b / 3.0;
+    const double dualc = b * p * p + p * p * p / 3.0;
+    dual = dualc < primal;
+  }
+  const bool ofree = factored && !dual;
   // ---- 3. O matrix -----------------------------------------------------------
   const size_t ob = sizeof(double) * (size_t)B * (size_t)P;
-  CKR(g_ws.O.ensure(ob));
+  if (!ofree) CKR(g_ws.O.ensure(ob));
   if (holo) CKR(g_ws.Oi.ensure(ob));
-  double* O = g_ws.O.as<double>();
+  double* O = ofree ? nullptr : g_ws.O.as<double>();
   double* Oi = holo ? g_ws.Oi.as<double>() : nullptr;
-  if ((!holo && gfn) || (holo && hfn)) {
+  if (ofree) {
+    // t in FP64 for the factored Gram and update (the FP64 local energies of
+    // this ansatz have just left it in the workspace)
+    if (!(e_dev && esrc.a == a && prec == NQS_HIP_FP64))
+      CKR(forward_t<double>(a, d_samples, B, nullptr, nullptr));
+  } else if ((!holo && gfn) || (holo && hfn)) {
     std::vector<double> hr((size_t)B * P), hi(holo ? (size_t)B * P : 0);
     for (int b = 0; b < B; b++) {
       const int* sp = h_samples.data() + (size_t)b * N;
@@ -1531,7 +1554,14 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
   }
   // ---- 4. means and force ----------------------------------------------------
   CKR(g_ws.vecs.ensure(sizeof(double) * 9 * (size_t)P));
-  CKR(g_ws.colwork.ensure(sizeof(double) * nqs::colsum_work_elems(B, P)));
+  if (ofree) {
+    const int nblk = a->kind == NQS_ANSATZ_COMPLEX_RBM ? 2 : 1;
+    CKR(g_ws.colwork.ensure(sizeof(double) * (size_t)std::max(
+                                nqs::factored_delta_work_elems(B, N, a->M, nblk),
+                                (B + 127) / 128 * B)));
+  } else {
+    CKR(g_ws.colwork.ensure(sizeof(double) * nqs::colsum_work_elems(B, P)));
+  }
   CKR(g_ws.scal.ensure(sizeof(double) * 16));
   CKR(g_ws.u1.ensure(sizeof(double) * B));
   CKR(g_ws.u2.ensure(sizeof(double) * B));
@@ -1547,7 +1577,7 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
   CKR(nqs::launch_sum(E, B, nullptr, 1.0 / B, sc + 0, g_stream));
   CKR(nqs::launch_sum(E, B, E, 1.0 / B, sc + 1, g_stream));
   CKR(nqs::launch_sum(Ei, B, nullptr, 1.0 / B, sc + 2, g_stream));
-  CKR(nqs::launch_colsum(O, B, P, nullptr, 1.0 / B, Om, work, g_stream));
+  if (!ofree) CKR(nqs::launch_colsum(O, B, P, nullptr, 1.0 / B, Om, work, g_stream));
   if (holo) {
     CKR(nqs::launch_colsum(Oi, B, P, nullptr, 1.0 / B, Im, work, g_stream));
     CKR(nqs::launch_colsum(O, B, P, E, 1.0 / B, c1, work, g_stream));
@@ -1566,26 +1596,19 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
     double* u = g_ws.u1.as<double>();
     // eps_i = E_i - <E>, O_c = O - <O>  (nqs_optimizer.c:1119-1126)
     CKR(nqs::launch_sub_scalar(u, E, B, sc + 0, g_stream));
-    CKR(nqs::launch_center(O, B, P, Om, g_stream));
+    if (!ofree) CKR(nqs::launch_center(O, B, P, Om, g_stream));
     CKR(g_ws.diag.ensure(sizeof(double) * B));
     CKR(g_ws.y.ensure(sizeof(double) * B * 2));
     CKR(g_ws.fail.ensure(sizeof(int) * 4));
     double* T = nullptr;
-    // sample space (T = O_c O_c^T / Ns + eps I, Ns x Ns) or, with
-    // NQS_HIP_GRAM_AUTO when it costs less, parameter space
-    // (S = O_c^T O_c / Ns + eps I, P x P): O_c^T (O_c O_c^T/Ns + eps)^-1 =
-    // (O_c^T O_c/Ns + eps)^-1 O_c^T, so both give the same delta
-    const bool rbm_like = (a->kind == NQS_ANSATZ_RBM || a->kind == NQS_ANSATZ_COMPLEX_RBM) && !gfn;
-    const bool factored = (s->opt.gram == NQS_HIP_GRAM_FACTORED ||
-                           s->opt.gram == NQS_HIP_GRAM_AUTO) && rbm_like;
-    bool dual = false;
-    if (s->opt.gram == NQS_HIP_GRAM_AUTO) {
-      const double b = (double)B, p = (double)P;
-      const double gram_b = factored ? b * b * (N + 2.0 * a->M) : b * b * p;
-      const double primal = gram_b + b * b * b / 3.0;
-      const double dualc = b * p * p + p * p * p / 3.0;
-      dual = dualc < primal;
-    }
+    const int cpl = a->kind == NQS_ANSATZ_COMPLEX_RBM;
+    auto gram_factored = [&](double* out) -> int {
+      CKR(g_ws.Sd.ensure(sizeof(double) * (size_t)B * (N + 2)));
+      return nqs::launch_gram_factored(d_samples, B, N, g_ws.t_re.as<double>(),
+                                       cpl ? g_ws.t_im.as<double>() : nullptr, a->M, eps, out,
+                                       work, g_ws.Sd.as<double>(), g_ws.u2.as<double>(), sc + 6,
+                                       g_stream);
+    };
     if (dual) {
       // parameter-space MinSR: delta = S^-1 (O_c^T eps / Ns)
       CKR(g_ws.T.ensure(sizeof(double) * (size_t)P * P));
@@ -1626,14 +1649,7 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
     } else if (factored) {
       CKR(g_ws.T.ensure(sizeof(double) * (size_t)B * B));
       T = g_ws.T.as<double>();
-      if (prec == NQS_HIP_FP32) CKR(forward_t<double>(a, d_samples, B, nullptr, nullptr));
-      CKR(g_ws.T2.ensure(sizeof(double) * (size_t)((B + 127) / 128) * B));
-      CKR(g_ws.Sd.ensure(sizeof(double) * (size_t)B * (N + 2)));
-      const int cpl = a->kind == NQS_ANSATZ_COMPLEX_RBM;
-      CKR(nqs::launch_gram_factored(d_samples, B, N, g_ws.t_re.as<double>(),
-                                    cpl ? g_ws.t_im.as<double>() : nullptr, a->M, eps, T,
-                                    g_ws.T2.as<double>(), g_ws.Sd.as<double>(),
-                                    g_ws.u2.as<double>(), sc + 6, g_stream));
+      CKR(gram_factored(T));
     } else {
       CKR(g_ws.T.ensure(sizeof(double) * (size_t)B * B));
       T = g_ws.T.as<double>();
@@ -1651,8 +1667,12 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
       // columns from a second Gram evaluation) and factor again
       CKR(g_ws.T2.ensure(sizeof(double) * (size_t)B * B));
       double* T2 = g_ws.T2.as<double>();
-      CKR(nqs::launch_gemm_nt(B, B, (int)P, 1.0 / B, O, P, O, P, 0.0, T2, B, 1, g_stream));
-      CKR(nqs::launch_add_diag(T2, B, B, eps, g_stream));
+      if (factored) {
+        CKR(gram_factored(T2));
+      } else {
+        CKR(nqs::launch_gemm_nt(B, B, (int)P, 1.0 / B, O, P, O, P, 0.0, T2, B, 1, g_stream));
+        CKR(nqs::launch_add_diag(T2, B, B, eps, g_stream));
+      }
       CKR(nqs::launch_potrf_repair(T, B, f - 1, T2, g_stream));
       CKR(nqs::launch_add_diag(T, B, B, 1e-6, g_stream));
       f = nqs::launch_potrf(T, B, Linv, g_ws.fail.as<int>(), g_stream);
@@ -1667,8 +1687,17 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
                       "sr_diag_shift or num_samples. Update set to 0.\n");
       CKR(nqs::launch_fill(y, 0.0, B, g_stream));
     }
-    // delta_k = (1/Ns) sum_i O_c[i][k] y_i
-    CKR(nqs::launch_colsum(O, B, P, y, 1.0 / B, delta, work, g_stream));
+    // delta_k = (1/Ns) sum_i O_c[i][k] y_i = (1/Ns) sum_i O[i][k] (y_i - mean y)
+    if (ofree) {
+      double* yc = y + B;
+      CKR(nqs::launch_sum(y, B, nullptr, 1.0 / B, sc + 7, g_stream));
+      CKR(nqs::launch_sub_scalar(yc, y, B, sc + 7, g_stream));
+      CKR(nqs::launch_factored_delta(d_samples, B, N, a->M, g_ws.t_re.as<double>(),
+                                     cpl ? g_ws.t_im.as<double>() : nullptr, yc, 1.0 / B, work,
+                                     delta, g_stream));
+    } else {
+      CKR(nqs::launch_colsum(O, B, P, y, 1.0 / B, delta, work, g_stream));
+    }
     }
   } else {
     double* u1 = g_ws.u1.as<double>();

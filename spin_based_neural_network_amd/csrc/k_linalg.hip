@@ -1591,6 +1591,107 @@ int launch_gram_factored(const int* spins, int B, int N, const double* U1, const
   return 0;
 }
 
+// --------------------------------------------------------------------------
+// MinSR update of an RBM without the O matrix: with O rows
+// (s, u, u (x) s) per parameter block (u = t for real RBMs; u = Re t, then
+// -Im t with a zero visible part for the real-projected complex RBM),
+// O_c^T y = O^T (y - mean y), and O^T w over one block is
+//   Z = [u, 1]^T diag(w) [s, 1]     ((M+1) x (N+1), K = B samples)
+// whose entries are the W, b and a parts.  Z is formed on FP64 MFMA in
+// 64 x 64 tiles over chunks of 128 samples (partials per chunk), then the
+// chunks are summed in a fixed order and scattered into delta.
+// --------------------------------------------------------------------------
+constexpr int kFdChunk = 128;
+
+__global__ __launch_bounds__(256) void k_factored_ot(const int* __restrict__ spins, int B, int N,
+                                                     int M, const double* __restrict__ U1,
+                                                     const double* __restrict__ U2,
+                                                     const double* __restrict__ w, int tiles_n,
+                                                     int nsplit, double* __restrict__ Zp) {
+  const int M1 = M + 1, N1 = N + 1;
+  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
+  const int split = blockIdx.y, z = blockIdx.z;
+  const double* U = z ? U2 : U1;
+  const double sg = z ? -1.0 : 1.0;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int hb = tm * 64 + (wave >> 1) * 32, ib = tn * 64 + (wave & 1) * 32;
+  const int bs = split * kFdChunk, be = min(B, bs + kFdChunk);
+  v4d acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+  for (int bk = bs; bk < be; bk += 4) {
+    const int b = bk + lk;
+    const bool ok = b < be;
+    const int bc = ok ? b : bs;
+    const double wb = ok ? w[bc] : 0.0;
+    double av[2], bv[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int h = hb + 16 * q + lr, i = ib + 16 * q + lr;
+      const double uh = h < M ? sg * U[(size_t)bc * M + h] : 1.0;
+      av[q] = h < M1 ? wb * uh : 0.0;
+      const double si = i < N ? (double)spins[(size_t)bc * N + i] : 1.0;
+      bv[q] = i < N1 ? si : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+  }
+  double* out = Zp + ((size_t)z * nsplit + split) * (size_t)M1 * N1;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int h = hb + 16 * i + lk + 4 * r, c = ib + 16 * j + lr;
+        if (h < M1 && c < N1) out[(size_t)h * N1 + c] = acc[i][j][r];
+      }
+}
+
+__global__ __launch_bounds__(256) void k_factored_delta(const double* __restrict__ Zp, int N, int M,
+                                                        int nsplit, int nblk, double alpha,
+                                                        double* __restrict__ delta) {
+  const int M1 = M + 1, N1 = N + 1;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)nblk * M1 * N1) return;
+  const int z = (int)(e / ((long)M1 * N1));
+  const long hi = e - (long)z * M1 * N1;
+  const int h = (int)(hi / N1), i = (int)(hi - (long)h * N1);
+  if (h == M && i == N) return;
+  double sum = 0.0;
+  for (int sp = 0; sp < nsplit; sp++) sum += Zp[((size_t)z * nsplit + sp) * (size_t)M1 * N1 + hi];
+  const long base = (long)z * ((long)N + M + (long)M * N);
+  if (h < M && i < N) delta[base + N + M + (long)h * N + i] = alpha * sum;
+  else if (h < M) delta[base + N + h] = alpha * sum;
+  else delta[base + i] = z ? 0.0 : alpha * sum;
+}
+
+int factored_delta_work_elems(int B, int N, int M, int nblk) {
+  return nblk * ((B + kFdChunk - 1) / kFdChunk) * (M + 1) * (N + 1);
+}
+
+int launch_factored_delta(const int* spins, int B, int N, int M, const double* U1,
+                          const double* U2, const double* w, double alpha, double* work,
+                          double* delta, hipStream_t st) {
+  const int nsplit = (B + kFdChunk - 1) / kFdChunk, nblk = U2 ? 2 : 1;
+  const int tiles_m = (M + 1 + 63) / 64, tiles_n = (N + 1 + 63) / 64;
+  KScope ks("colsum", st);
+  hipLaunchKernelGGL(k_factored_ot, dim3(tiles_m * tiles_n, nsplit, nblk), dim3(256), 0, st, spins, B,
+                     N, M, U1, U2, w, tiles_n, nsplit, work);
+  const long ne = (long)nblk * (M + 1) * (N + 1);
+  hipLaunchKernelGGL(k_factored_delta, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, work, N,
+                     M, nsplit, nblk, alpha, delta);
+  count_launch(2);
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
 // plain streaming copy (HBM probe used to calibrate byte counters)
 __global__ __launch_bounds__(256) void k_copy_probe(const double4* __restrict__ src,
                                                     double4* __restrict__ dst, long n4) {

@@ -255,6 +255,14 @@ int launch_potrs(const double* L, const double* Linv, int n, const double* b, do
 int launch_gram_factored(const int* spins, int B, int N, const double* U1, const double* U2, int Ku,
                          double eps, double* T, double* work, double* Sd, double* r, double* g,
                          hipStream_t st);
+// MinSR update of an RBM from its factors: delta = alpha * O^T w over the
+// parameter blocks (O rows (s, u, u (x) s); u = U1, then u = -U2 with a
+// zero visible part when U2 is given), w = centred solution.  work:
+// factored_delta_work_elems(B, N, M, 1 or 2) doubles.
+int factored_delta_work_elems(int B, int N, int M, int nblk);
+int launch_factored_delta(const int* spins, int B, int N, int M, const double* U1,
+                          const double* U2, const double* w, double alpha, double* work,
+                          double* delta, hipStream_t st);
 int launch_add_diag(double* A, int n, long ld, double v, hipStream_t st);
 // out [C][R] = in [R][C]^T
 int launch_transpose(const double* in, long R, long C, double* out, hipStream_t st);
