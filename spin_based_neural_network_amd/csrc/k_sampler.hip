@@ -14,6 +14,8 @@
 // cosh(theta_h + d_h)/cosh(theta_h) = cosh d_h + tanh(theta_h) sinh d_h read
 // from the site-major cosh(2W)/sinh(2W) tables: one log per lane and a
 // log2(L)-step butterfly per proposal, theta updated incrementally on accept.
+#include <type_traits>
+
 #include "nqs_kernels.h"
 
 namespace nqs {
@@ -46,24 +48,38 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
   for (int w = 0; w < 4; w++) sp[w] = (w < nw) ? cs.spins[(size_t)c * nw + w] : 0ULL;
   uint64_t rng = cs.rng[c];
 
+  // theta_h = b_h + sum_i W_hi s_i in site order; the site loop is outer so
+  // that the loads of all of a lane's units for one site are in flight
+  // together (rows clamped into the table, products s * W exact)
   R thr[HPL], thi[HPL], tr[HPL], ti[HPL];
 #pragma unroll
   for (int j = 0; j < HPL; j++) {
     const int h = lane + L * j;
-    R xr = 0, xi = 0;
-    if (h < M) {
-      xr = m.b_re[h];
-      if (CPLX) xi = m.b_im[h];
-      for (int i = 0; i < N; i++) {
-        const R s = bit_of(sp, i) ? (R)-1 : (R)1;
-        xr += m.WT_re[(size_t)i * M + h] * s;
-        if (CPLX) xi += m.WT_im[(size_t)i * M + h] * s;
+    thr[j] = h < M ? m.b_re[h] : (R)0;
+    thi[j] = (CPLX && h < M) ? m.b_im[h] : (R)0;
+  }
+  if (M > 0) {
+#pragma unroll 2
+    for (int i = 0; i < N; i++) {
+      const R s = bit_of(sp, i) ? (R)-1 : (R)1;
+      const size_t row = (size_t)i * M;
+#pragma unroll
+      for (int j = 0; j < HPL; j++) {
+        const int h = lane + L * j;
+        const int hc = h < M ? h : 0;
+        const R wr = m.WT_re[row + hc];
+        const R wi = CPLX ? m.WT_im[row + hc] : (R)0;
+        if (h < M) {
+          thr[j] += wr * s;
+          if (CPLX) thi[j] += wi * s;
+        }
       }
     }
-    thr[j] = xr;
-    thi[j] = xi;
-    if (CPLX) ctanh<R>(xr, xi, tr[j], ti[j]);
-    else { tr[j] = tanh(xr); ti[j] = 0; }
+  }
+#pragma unroll
+  for (int j = 0; j < HPL; j++) {
+    if (CPLX) ctanh<R>(thr[j], thi[j], tr[j], ti[j]);
+    else { tr[j] = tanh(thr[j]); ti[j] = 0; }
   }
 
   // log|psi(s)| with the current parameters, and the chain's cached value
@@ -120,51 +136,83 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
         kchr[q] = 1;
         kchi[q] = kshr[q] = kshi[q] = kwi[q] = 0;
         kwr[q] = 1;
-        const int h = lane + L * q;
-        if (h < M) {
-          const size_t ei = (size_t)i * M + h;
-          R chr, chi = 0, shr, shi = 0;   // cosh d, sinh d
-          if (j < 0) {
-            chr = m.CT_re[ei];
-            shr = -si * m.ST_re[ei];
-            if (CPLX) { chi = m.CT_im[ei]; shi = -si * m.ST_im[ei]; }
-          } else {
-            // d = -2 s_i (W_i - W_j): cosh d = Ci Cj - Si Sj,
-            // sinh d = -s_i (Si Cj - Ci Sj)
-            const size_t ej = (size_t)j * M + h;
-            const R cir = m.CT_re[ei], sir = m.ST_re[ei], cjr = m.CT_re[ej], sjr = m.ST_re[ej];
-            if (CPLX) {
-              const R cii = m.CT_im[ei], sii = m.ST_im[ei], cji = m.CT_im[ej], sji = m.ST_im[ej];
-              chr = (cir * cjr - cii * cji) - (sir * sjr - sii * sji);
-              chi = (cir * cji + cii * cjr) - (sir * sji + sii * sjr);
-              R ar = (sir * cjr - sii * cji) - (cir * sjr - cii * sji);
-              R ai = (sir * cji + sii * cjr) - (cir * sji + cii * sjr);
-              shr = -si * ar;
-              shi = -si * ai;
-            } else {
-              chr = cir * cjr - sir * sjr;
-              shr = -si * (sir * cjr - cir * sjr);
+      }
+      // table rows of the flipped sites for the lane's units, in groups of
+      // QB units whose loads are issued together (rows clamped into the
+      // table; the pair form reads the second site's rows as well)
+      auto factors = [&](auto pair_tag) {
+        constexpr bool PAIR = decltype(pair_tag)::value;
+        constexpr int QB = 4;
+#pragma unroll
+        for (int q0 = 0; q0 < HPL; q0 += QB) {
+          R cir[QB], sir[QB], cii[QB], sii[QB], cjr[QB], sjr[QB], cji[QB], sji[QB];
+#pragma unroll
+          for (int u = 0; u < QB; u++) {
+            const int q = q0 + u;
+            if (q < HPL) {
+              const int h = lane + L * q;
+              const size_t ei = (size_t)i * M + (h < M ? h : 0);
+              cir[u] = m.CT_re[ei];
+              sir[u] = m.ST_re[ei];
+              if (CPLX) { cii[u] = m.CT_im[ei]; sii[u] = m.ST_im[ei]; }
+              if (PAIR) {
+                const size_t ej = (size_t)j * M + (h < M ? h : 0);
+                cjr[u] = m.CT_re[ej];
+                sjr[u] = m.ST_re[ej];
+                if (CPLX) { cji[u] = m.CT_im[ej]; sji[u] = m.ST_im[ej]; }
+              }
             }
           }
-          kchr[q] = chr;
-          kshr[q] = shr;
-          if (CPLX) {
-            // w = cosh d + t sinh d
-            const R wr = chr + (tr[q] * shr - ti[q] * shi);
-            const R wi = chi + (tr[q] * shi + ti[q] * shr);
-            const R nr = pr * wr - pi * wi;
-            pi = pr * wi + pi * wr;
-            pr = nr;
-            kchi[q] = chi;
-            kshi[q] = shi;
-            kwr[q] = wr;
-            kwi[q] = wi;
-          } else {
-            const R wr = chr + tr[q] * shr;
-            pr *= wr;
-            kwr[q] = wr;
+#pragma unroll
+          for (int u = 0; u < QB; u++) {
+            const int q = q0 + u;
+            if (q >= HPL) continue;
+            const int h = lane + L * q;
+            if (h >= M) continue;
+            R chr, chi = 0, shr, shi = 0;   // cosh d, sinh d
+            if (!PAIR) {
+              chr = cir[u];
+              shr = -si * sir[u];
+              if (CPLX) { chi = cii[u]; shi = -si * sii[u]; }
+            } else {
+              // d = -2 s_i (W_i - W_j): cosh d = Ci Cj - Si Sj,
+              // sinh d = -s_i (Si Cj - Ci Sj)
+              if (CPLX) {
+                chr = (cir[u] * cjr[u] - cii[u] * cji[u]) - (sir[u] * sjr[u] - sii[u] * sji[u]);
+                chi = (cir[u] * cji[u] + cii[u] * cjr[u]) - (sir[u] * sji[u] + sii[u] * sjr[u]);
+                R ar = (sir[u] * cjr[u] - sii[u] * cji[u]) - (cir[u] * sjr[u] - cii[u] * sji[u]);
+                R ai = (sir[u] * cji[u] + sii[u] * cjr[u]) - (cir[u] * sji[u] + cii[u] * sjr[u]);
+                shr = -si * ar;
+                shi = -si * ai;
+              } else {
+                chr = cir[u] * cjr[u] - sir[u] * sjr[u];
+                shr = -si * (sir[u] * cjr[u] - cir[u] * sjr[u]);
+              }
+            }
+            kchr[q] = chr;
+            kshr[q] = shr;
+            if (CPLX) {
+              // w = cosh d + t sinh d
+              const R wr = chr + (tr[q] * shr - ti[q] * shi);
+              const R wi = chi + (tr[q] * shi + ti[q] * shr);
+              const R nr = pr * wr - pi * wi;
+              pi = pr * wi + pi * wr;
+              pr = nr;
+              kchi[q] = chi;
+              kshi[q] = shi;
+              kwr[q] = wr;
+              kwi[q] = wi;
+            } else {
+              const R wr = chr + tr[q] * shr;
+              pr *= wr;
+              kwr[q] = wr;
+            }
           }
         }
+      };
+      if (M > 0) {
+        if (j >= 0) factors(std::true_type{});
+        else factors(std::false_type{});
       }
       R lm = CPLX ? (R)0.5 * log(pr * pr + pi * pi) : log(pr);
       if (M == 0) lm = 0;
