@@ -552,8 +552,161 @@ constexpr int NB = 64, NBO = 128;
 
 // Linv workspace (potrf_linv_elems): the inverses of the 64 x 64 diagonal
 // blocks, then the blocks' L (written by the panel kernel and copied into A
-// once every panel has read its input), then the triangular solves' 256 x 256
-// super-block inverses (row-major and transposed)
+// once every panel has read its input), the merged outer panels' L21 blocks,
+// then the triangular solves' 256 x 256 super-block inverses and couplings
+
+// Step 2 of a panel: D = L L^T by blocks of 16 columns.  Wave jb factors
+// its own 16 columns with the rows in its lanes (only 1/pivot is on the
+// column chain, L_ij = a_ij / sqrt(d) is formed after the update; each
+// column is broadcast through the wave's own LDS row), publishes them
+// through D, and every later wave subtracts the rank-16 product from its own
+// columns.  Returns the first column whose pivot is not positive, or nb.
+// Called by all 256 threads.
+__device__ __attribute__((always_inline)) inline int panel_factor(double (*D)[NB + 1], double (*colw)[NB], int* s_fail, int nb,
+                                   int wave, int lane) {
+  double rr[16];
+#pragma unroll
+  for (int t = 0; t < 16; t++) rr[t] = D[lane][16 * wave + t];
+  int jdone = nb;
+  for (int jb = 0; jb < 4 && 16 * jb < nb; jb++) {
+    if (wave == jb) {
+      // straight-line code (selects instead of branches) so that the
+      // square roots and scalings of column j overlap the chain of j + 1.
+      // The wave's LDS accesses complete in issue order, and the wavefront
+      // fence keeps the compiler from moving the reads above the write.
+      int jf = nb;
+#pragma unroll
+      for (int jj = 0; jj < 16; jj++) {
+        const int j = 16 * jb + jj;
+        const double ci = rr[jj];
+        colw[wave][lane] = ci;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const double d = colw[wave][j];
+        const bool live = j < nb && jf == nb;
+        const bool ok = live && d > 0.0;
+        jf = (live && !ok) ? j : jf;
+        const double rd = 1.0 / d;
+        const double si = (ok && lane > j) ? ci * rd : 0.0;
+#pragma unroll
+        for (int t = jj + 1; t < 16; t++) rr[t] -= si * colw[wave][16 * jb + t];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const double rs = sqrt(rd);   // 1 / sqrt(d)
+        rr[jj] = !ok ? ci : ((lane == j) ? d * rs : ((lane > j) ? ci * rs : ci));
+      }
+#pragma unroll
+      for (int t = 0; t < 16; t++) D[lane][16 * jb + t] = rr[t];
+      if (lane == 0) *s_fail = jf;
+    }
+    __syncthreads();
+    jdone = *s_fail;
+    if (jdone < nb) break;
+    if (wave > jb) {
+      double lp[16];
+#pragma unroll
+      for (int jj = 0; jj < 16; jj++) lp[jj] = D[lane][16 * jb + jj];
+#pragma unroll
+      for (int t = 0; t < 16; t++) {
+        const int c = 16 * wave + t;
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < 16; jj += 2) {
+          a0 += lp[jj] * D[c][16 * jb + jj];
+          a1 += lp[jj + 1] * D[c][16 * jb + jj + 1];
+        }
+        rr[t] -= a0 + a1;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 16; t++) D[lane][16 * wave + t] = rr[t];
+  __syncthreads();
+  return jdone;
+}
+
+// Step 3: X = L^-1 by 16 x 16 blocks.  (a) wave w inverts diagonal block w
+// (lane c < 16 solves L_ww x = e_c with reciprocal pivots); (b) one block
+// diagonal at a time, X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj, the blocks of a
+// diagonal on separate waves (T_ij staged in LDS).  Rows beyond nb are
+// identity rows of L, so their part of X is the identity.  All 256 threads.
+__device__ __attribute__((always_inline)) inline void panel_inverse(double (*D)[NB + 1], double (*X)[NB + 1], double* rcp,
+                                     double (*Tb)[16][17], int wave, int lane, int tid) {
+  for (int e = tid; e < NB * NB; e += 256) X[e >> 6][e & 63] = 0.0;
+  if (tid < NB) rcp[tid] = 1.0 / D[tid][tid];
+  __syncthreads();
+  if (lane < 16) {
+    const int b0 = 16 * wave, c = lane;
+    double x[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      double p0 = (q == c) ? 1.0 : 0.0, p1 = 0.0;
+#pragma unroll
+      for (int k = 0; k < q; k += 2) {
+        p0 -= D[b0 + q][b0 + k] * x[k];
+        if (k + 1 < q) p1 -= D[b0 + q][b0 + k + 1] * x[k + 1];
+      }
+      x[q] = (p0 + p1) * rcp[b0 + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) X[b0 + q][b0 + c] = x[q];
+  }
+  __syncthreads();
+  for (int dist = 1; dist < 4; dist++) {
+    const int bi = 16 * (wave + dist), bj = 16 * wave;   // block (wave + dist, wave)
+    const int r = lane >> 2, c0 = 4 * (lane & 3);
+    if (wave + dist < 4) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int m = bj; m < bi; m += 2) {
+          t0 += D[bi + r][m] * X[m][bj + c0 + u];
+          t1 += D[bi + r][m + 1] * X[m + 1][bj + c0 + u];
+        }
+        Tb[wave][r][c0 + u] = t0 + t1;
+      }
+    }
+    __syncthreads();
+    if (wave + dist < 4) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        double x0 = 0.0, x1 = 0.0;
+#pragma unroll
+        for (int m = 0; m < 16; m += 2) {
+          x0 -= X[bi + r][bi + m] * Tb[wave][m][c0 + u];
+          x1 -= X[bi + r][bi + m + 1] * Tb[wave][m + 1][c0 + u];
+        }
+        X[bi + r][bj + c0 + u] = x0 + x1;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// acc = rows(16 rows of wave w) . X^T on MFMA: wave w, column block bj
+__device__ __attribute__((always_inline)) inline void panel_times_xt(const double (*R)[NB + 1], const double (*X)[NB + 1],
+                                      v4d (&acc)[4], int wave, int lr, int lk) {
+#pragma unroll
+  for (int bj = 0; bj < 4; bj++) acc[bj] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int kk = 0; kk < NB; kk += 4) {
+    const double a = R[16 * wave + lr][kk + lk];
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+      acc[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, X[16 * bj + lr][kk + lk], acc[bj], 0, 0, 0);
+  }
+}
+
+// the block's L and inverse to the workspace (rows / columns beyond nb of
+// the inverse zeroed)
+__device__ __attribute__((always_inline)) inline void panel_publish(const double (*D)[NB + 1], const double (*X)[NB + 1], int n,
+                                     int k0, int nb, double* Linv, int tid) {
+  double* Lg = Linv + ((size_t)((n + NB - 1) / NB) + k0 / NB) * NB * NB;
+  double* Xg = Linv + (size_t)(k0 / NB) * NB * NB;
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    Lg[e] = D[r][c];
+    Xg[e] = (r < nb && c < nb) ? X[r][c] : 0.0;
+  }
+}
 
 // One inner panel, columns [k0, k0 + nb), in one launch.  Workgroup b
 // (4 waves) owns rows rowbase = r0 + 64 b + [0, 64) below the diagonal block
@@ -561,12 +714,8 @@ constexpr int NB = 64, NBO = 128;
 // kp >= 0: columns [kp, kp + 64) left of the panel are final L whose rank-64
 // update the panel has not yet received; it is applied here.
 //   1. D = A_kk - Lp_d Lp_d^T and R = A_rows - Lp_r Lp_d^T   (MFMA)
-//   2. D = L L^T by 16-column blocks: wave q keeps columns 16q..16q+15 of
-//      row `lane` in registers; the owning wave factors a block column with
-//      readlane broadcasts, one barrier per block column, later waves apply
-//      the rank-16 update to their columns
-//   3. Dinv = L^-1 by 16 x 16 blocks (diagonal blocks on the four waves,
-//      then the off-diagonal blocks one block diagonal at a time)
+//   2. D = L L^T (panel_factor)
+//   3. Dinv = L^-1 (panel_inverse)
 //   4. rows = R Dinv^T                                           (MFMA)
 // A non-positive pivot at column j: the factored columns < j of the block
 // and the updated, unsolved rows are stored, and k0 + j + 1 is reported.
@@ -578,7 +727,10 @@ __global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb,
   __shared__ double D[NB][NB + 1];
   __shared__ double X[NB][NB + 1];     // Lp_d in step 1, L^-1 from step 3
   __shared__ double Rs[NB][NB + 1];
-  __shared__ double col[2][NB];
+  __shared__ double rcp[NB];
+  __shared__ double colw[4][NB];
+  __shared__ double Tb[3][16][17];
+  __shared__ int s_fail;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lr = lane & 15, lk = lane >> 4;
   const int r0 = k0 + nb;
@@ -643,76 +795,12 @@ __global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb,
       }
   }
   __syncthreads();
-  // step 2: factor D by blocks of 16 columns.  Wave jb factors its own 16
-  // columns with the rows in its lanes (pivots and column values read with
-  // readlane, no barrier inside the block column; only 1/pivot is on the
-  // column chain, L_ij = a_ij / sqrt(d) is formed after the update), then
-  // publishes them through D, and every later wave subtracts the rank-16
-  // product from its own columns.
-  __shared__ int s_fail;
-  __shared__ double colw[4][NB];
-  double rr[16];
-#pragma unroll
-  for (int t = 0; t < 16; t++) rr[t] = D[lane][16 * wave + t];
-  int jdone = nb;
-  for (int jb = 0; jb < 4 && 16 * jb < nb; jb++) {
-    if (wave == jb) {
-      // straight-line code (selects instead of branches) so that the
-      // square roots and scalings of column j overlap the chain of j + 1.
-      // Column j is broadcast through the wave's own LDS row: the wave's
-      // LDS accesses complete in issue order, and the wavefront fence keeps
-      // the compiler from moving the reads above the write.
-      int jf = nb;
-#pragma unroll
-      for (int jj = 0; jj < 16; jj++) {
-        const int j = 16 * jb + jj;
-        const double ci = rr[jj];
-        colw[wave][lane] = ci;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const double d = colw[wave][j];
-        const bool live = j < nb && jf == nb;
-        const bool ok = live && d > 0.0;
-        jf = (live && !ok) ? j : jf;
-        const double rd = 1.0 / d;
-        const double si = (ok && lane > j) ? ci * rd : 0.0;
-#pragma unroll
-        for (int t = jj + 1; t < 16; t++) rr[t] -= si * colw[wave][16 * jb + t];
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const double rs = sqrt(rd);   // 1 / sqrt(d)
-        rr[jj] = !ok ? ci : ((lane == j) ? d * rs : ((lane > j) ? ci * rs : ci));
-      }
-#pragma unroll
-      for (int t = 0; t < 16; t++) D[lane][16 * jb + t] = rr[t];
-      if (lane == 0) s_fail = jf;
-    }
-    __syncthreads();
-    jdone = s_fail;
-    if (jdone < nb) break;
-    if (wave > jb) {
-      double lp[16];
-#pragma unroll
-      for (int jj = 0; jj < 16; jj++) lp[jj] = D[lane][16 * jb + jj];
-#pragma unroll
-      for (int t = 0; t < 16; t++) {
-        const int c = 16 * wave + t;
-        double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-        for (int jj = 0; jj < 16; jj += 2) {
-          a0 += lp[jj] * D[c][16 * jb + jj];
-          a1 += lp[jj + 1] * D[c][16 * jb + jj + 1];
-        }
-        rr[t] -= a0 + a1;
-      }
-    }
-  }
-#pragma unroll
-  for (int t = 0; t < 16; t++) D[lane][16 * wave + t] = rr[t];
-  __syncthreads();
+  const int jdone = panel_factor(D, colw, &s_fail, nb, wave, lane);
   // the block's L goes to the workspace (other workgroups may still be
   // reading the block from A); k_diag_copy moves it into A at the end
-  double* Lg = Linv + ((size_t)((n + NB - 1) / NB) + k0 / NB) * NB * NB;
   if (jdone < nb) {
     if (blockIdx.x == 0) {
+      double* Lg = Linv + ((size_t)((n + NB - 1) / NB) + k0 / NB) * NB * NB;
       for (int e = tid; e < NB * NB; e += 256) Lg[e] = D[e >> 6][e & 63];
       if (tid == 0) *fail = k0 + jdone + 1;
     }
@@ -723,80 +811,11 @@ __global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb,
     }
     return;
   }
-  // step 3: X = L^-1 by 16 x 16 blocks.  (a) wave w inverts diagonal block
-  // w (lane c < 16 solves L_ww x = e_c with reciprocal pivots); (b) one block
-  // diagonal at a time, X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj, the blocks of
-  // a diagonal on separate waves (T_ij staged in LDS).  Rows beyond nb are
-  // identity rows of L, so their part of X is the identity.
-  __shared__ double Tb[3][16][17];
-  for (int e = tid; e < NB * NB; e += 256) X[e >> 6][e & 63] = 0.0;
-  if (tid < NB) col[0][tid] = 1.0 / D[tid][tid];
-  __syncthreads();
-  if (lane < 16) {
-    const int b0 = 16 * wave, c = lane;
-    double x[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-      double p0 = (q == c) ? 1.0 : 0.0, p1 = 0.0;
-#pragma unroll
-      for (int k = 0; k < q; k += 2) {
-        p0 -= D[b0 + q][b0 + k] * x[k];
-        if (k + 1 < q) p1 -= D[b0 + q][b0 + k + 1] * x[k + 1];
-      }
-      x[q] = (p0 + p1) * col[0][b0 + q];
-    }
-#pragma unroll
-    for (int q = 0; q < 16; q++) X[b0 + q][b0 + c] = x[q];
-  }
-  __syncthreads();
-  for (int dist = 1; dist < 4; dist++) {
-    const int bi = 16 * (wave + dist), bj = 16 * wave;   // block (wave + dist, wave)
-    const int r = lane >> 2, c0 = 4 * (lane & 3);
-    if (wave + dist < 4) {
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        double t0 = 0.0, t1 = 0.0;
-        for (int m = bj; m < bi; m += 2) {
-          t0 += D[bi + r][m] * X[m][bj + c0 + u];
-          t1 += D[bi + r][m + 1] * X[m + 1][bj + c0 + u];
-        }
-        Tb[wave][r][c0 + u] = t0 + t1;
-      }
-    }
-    __syncthreads();
-    if (wave + dist < 4) {
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        double x0 = 0.0, x1 = 0.0;
-#pragma unroll
-        for (int m = 0; m < 16; m += 2) {
-          x0 -= X[bi + r][bi + m] * Tb[wave][m][c0 + u];
-          x1 -= X[bi + r][bi + m + 1] * Tb[wave][m + 1][c0 + u];
-        }
-        X[bi + r][bj + c0 + u] = x0 + x1;
-      }
-    }
-    __syncthreads();
-  }
-  if (blockIdx.x == 0) {
-    double* Xg = Linv + (size_t)(k0 / NB) * NB * NB;
-    for (int e = tid; e < NB * NB; e += 256) {
-      const int r = e >> 6, c = e & 63;
-      Lg[e] = D[r][c];
-      Xg[e] = (r < nb && c < nb) ? X[r][c] : 0.0;
-    }
-  }
+  panel_inverse(D, X, rcp, Tb, wave, lane, tid);
+  if (blockIdx.x == 0) panel_publish(D, X, n, k0, nb, Linv, tid);
   // step 4: rows = R Dinv^T
   v4d acc[4];
-#pragma unroll
-  for (int bj = 0; bj < 4; bj++) acc[bj] = (v4d){0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-  for (int kk = 0; kk < NB; kk += 4) {
-    const double a = Rs[16 * wave + lr][kk + lk];
-#pragma unroll
-    for (int bj = 0; bj < 4; bj++)
-      acc[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, X[16 * bj + lr][kk + lk], acc[bj], 0, 0, 0);
-  }
+  panel_times_xt(Rs, X, acc, wave, lr, lk);
 #pragma unroll
   for (int bj = 0; bj < 4; bj++)
 #pragma unroll
@@ -807,15 +826,180 @@ __global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb,
     }
 }
 
+// Both inner panels of an outer panel (columns [k0, k0 + 64) and
+// [k1, k1 + nb2), k1 = k0 + 64, n - k0 > 64) in one launch, with exactly the
+// arithmetic of the two k_panel launches (the second with kp = k0).
+// Workgroup b owns rows rowbase = k1 + 64 b + [0, 64) in both panels:
+// workgroup 0's rows are the second panel's diagonal rows (the block L21),
+// the others' rows lie below it.  Every workgroup forms L21 = A21 X1^T
+// itself; workgroup 0 writes it to the workspace (A21 stays readable for the
+// other workgroups during the launch) and k_diag_copy moves it into A.
+__global__ __launch_bounds__(256) void k_panel2(double* A, int n, int k0, int nb2, double* Linv,
+                                                double* L21w, int* fail) {
+  const int failed = *fail;
+  __shared__ double D[NB][NB + 1];
+  __shared__ double X[NB][NB + 1];
+  __shared__ double Rs[NB][NB + 1];
+  __shared__ double L2[NB][NB + 1];
+  __shared__ double rcp[NB];
+  __shared__ double colw[4][NB];
+  __shared__ double Tb[3][16][17];
+  __shared__ int s_fail;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int k1 = k0 + NB;
+  const long rowbase = (long)k1 + (long)blockIdx.x * NB;
+  const bool diag_rows = blockIdx.x == 0;
+  // loads: D1, the own rows (first panel's columns), A21 and D2, all of a
+  // thread's loads in flight together
+  double d2v[NB * NB / 256];
+  {
+    double dv[NB * NB / 256], rv[NB * NB / 256], av[NB * NB / 256];
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+      const long row = rowbase + r;
+      dv[q] = (c <= r) ? A[(long)(k0 + r) * n + k0 + c] : 0.0;
+      rv[q] = row < n ? A[row * n + k0 + c] : 0.0;
+      av[q] = r < nb2 ? A[(long)(k1 + r) * n + k0 + c] : 0.0;
+      d2v[q] = (r < nb2) ? ((c <= r) ? A[(long)(k1 + r) * n + k1 + c] : 0.0) : (r == c ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+      D[r][c] = dv[q];
+      Rs[r][c] = rv[q];
+      L2[r][c] = av[q];
+    }
+  }
+  __syncthreads();
+  if (failed) return;
+  // ---- first panel (no pending update) ----
+  int jdone = panel_factor(D, colw, &s_fail, NB, wave, lane);
+  if (jdone < NB) {
+    if (diag_rows) {
+      double* Lg = Linv + ((size_t)((n + NB - 1) / NB) + k0 / NB) * NB * NB;
+      for (int e = tid; e < NB * NB; e += 256) Lg[e] = D[e >> 6][e & 63];
+      if (tid == 0) *fail = k0 + jdone + 1;
+    }
+    // the rows keep their input values
+    return;
+  }
+  panel_inverse(D, X, rcp, Tb, wave, lane, tid);
+  if (diag_rows) panel_publish(D, X, n, k0, NB, Linv, tid);
+  {
+    v4d ar[4], al[4];
+    panel_times_xt(Rs, X, ar, wave, lr, lk);   // own rows' L (first panel)
+    panel_times_xt(L2, X, al, wave, lr, lk);   // L21
+    __syncthreads();                           // Rs, L2 and X have been read
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * wave + lk + 4 * r, c = 16 * bj + lr;
+        const long row = rowbase + i;
+        Rs[i][c] = ar[bj][r];
+        L2[i][c] = al[bj][r];
+        if (diag_rows) {
+          L21w[(size_t)(k0 / (2 * NB)) * NB * NB + i * NB + c] = al[bj][r];
+        } else if (row < n) {
+          A[row * n + k0 + c] = ar[bj][r];
+        }
+      }
+  }
+  // ---- second panel: D2 = A22 - L21 L21^T, R2 = A_rows - L_r1 L21^T ----
+  {
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+      D[r][c] = d2v[q];
+    }
+    v4d accR[4], accD[4];
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const long row = rowbase + 16 * wave + lk + 4 * r;
+        const int cc = 16 * bj + lr;
+        accR[bj][r] = (!diag_rows && row < n && cc < nb2) ? A[row * n + k1 + cc] : 0.0;
+      }
+      accD[bj] = (v4d){0.0, 0.0, 0.0, 0.0};
+    }
+    __syncthreads();   // Rs, L2 and D are in place
+    const long arow = rowbase + 16 * wave + lr;
+#pragma unroll
+    for (int kk = 0; kk < NB; kk += 4) {
+      const double a_r = (!diag_rows && arow < n) ? -Rs[16 * wave + lr][kk + lk] : 0.0;
+      const double a_d = -L2[16 * wave + lr][kk + lk];
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++) {
+        const double b = L2[16 * bj + lr][kk + lk];
+        accR[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r, b, accR[bj], 0, 0, 0);
+        accD[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_d, b, accD[bj], 0, 0, 0);
+      }
+    }
+    __syncthreads();   // Rs has been read
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * wave + lk + 4 * r, c = 16 * bj + lr;
+        Rs[i][c] = accR[bj][r];
+        if (c <= i && i < nb2) D[i][c] += accD[bj][r];
+      }
+  }
+  __syncthreads();
+  jdone = panel_factor(D, colw, &s_fail, nb2, wave, lane);
+  if (jdone < nb2) {
+    if (diag_rows) {
+      double* Lg = Linv + ((size_t)((n + NB - 1) / NB) + k1 / NB) * NB * NB;
+      for (int e = tid; e < NB * NB; e += 256) Lg[e] = D[e >> 6][e & 63];
+      if (tid == 0) *fail = k1 + jdone + 1;
+    } else {
+      for (int e = tid; e < NB * NB; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        const long row = rowbase + r;
+        if (row < n && c < nb2) A[row * n + k1 + c] = Rs[r][c];
+      }
+    }
+    return;
+  }
+  panel_inverse(D, X, rcp, Tb, wave, lane, tid);
+  if (diag_rows) {
+    panel_publish(D, X, n, k1, nb2, Linv, tid);
+    return;
+  }
+  v4d acc[4];
+  panel_times_xt(Rs, X, acc, wave, lr, lk);
+#pragma unroll
+  for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const long row = rowbase + 16 * wave + lk + 4 * r;
+      const int cc = 16 * bj + lr;
+      if (row < n && cc < nb2) A[row * n + k1 + cc] = acc[bj][r];
+    }
+}
+
 // Diagonal blocks' L from the workspace into A (lower triangles): every
 // block before a failure, and the factored columns of the failing block.
+// With merged outer panels (k_panel2) also the block L21 below the first
+// diagonal block of every outer panel whose first panel completed.
 __global__ __launch_bounds__(256) void k_diag_copy(double* A, int n, const double* Linv,
-                                                   const int* fail) {
+                                                   const double* L21w, int merged, const int* fail) {
   const int k0 = blockIdx.x * NB, nb = (n - k0) < NB ? (n - k0) : NB;
   const int f = *fail;
+  const int jf = f ? f - 1 : n;
+  if (merged && (blockIdx.x & 1) == 0 && n - k0 > NB && jf >= k0 + NB) {
+    const double* Lw = L21w + (size_t)(blockIdx.x >> 1) * NB * NB;
+    const int nb2 = (n - k0 - NB) < NB ? (n - k0 - NB) : NB;
+    for (int e = threadIdx.x; e < NB * NB; e += 256) {
+      const int r = e >> 6, c = e & 63;
+      if (r < nb2) A[(long)(k0 + NB + r) * n + k0 + c] = Lw[e];
+    }
+  }
   int lim = nb;
   if (f) {
-    const int jf = f - 1;
     if (k0 > jf) return;
     if (jf < k0 + nb) lim = jf - k0;
   }
@@ -887,8 +1071,30 @@ static int inner_panel(double* A, int n, int k0, int kp, double* Linv, int* fail
   return 0;
 }
 
+// Workspace of the merged outer panels' L21 blocks, after the inverses and
+// the diagonal blocks' L in Linv.
+static size_t l21_offset(int n) { return 2 * (size_t)((n + NB - 1) / NB) * NB * NB; }
+static size_t l21_elems(int n) { return (size_t)((n + NBO - 1) / NBO) * NB * NB; }
+
+// both inner panels of an outer panel in one launch (k_panel2) unless
+// NQS_HIP_PANEL_MERGE=0
+static int panel_merge() {
+  const char* e = getenv("NQS_HIP_PANEL_MERGE");
+  return !(e && e[0] == '0');
+}
+
 // outer panel at K0: its two inner panels
 static int outer_panel(double* A, int n, int K0, double* Linv, int* fail_dev, hipStream_t st) {
+  if (n - K0 > NB && panel_merge()) {
+    const int nb2 = (n - K0 - NB) < NB ? (n - K0 - NB) : NB;
+    const int grid = (n - K0 - NB + NB - 1) / NB;
+    KScope ks("potrf_panel", st);
+    hipLaunchKernelGGL(k_panel2, dim3((unsigned)grid), dim3(256), 0, st, A, n, K0, nb2, Linv,
+                       Linv + l21_offset(n), fail_dev);
+    count_launch();
+    NQS_HIP_CHECK(hipGetLastError());
+    return 0;
+  }
   if (inner_panel(A, n, K0, -1, Linv, fail_dev, st)) return -1;
   if (n - K0 > NB && inner_panel(A, n, K0 + NB, K0, Linv, fail_dev, st)) return -1;
   return 0;
@@ -983,7 +1189,7 @@ int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) 
     w = w1;
   }
   hipLaunchKernelGGL(k_diag_copy, dim3((unsigned)((n + NB - 1) / NB)), dim3(256), 0, st, A, n, Linv,
-                     fail_dev);
+                     Linv + l21_offset(n), panel_merge(), fail_dev);
   count_launch();
   NQS_HIP_CHECK(hipGetLastError());
   int fail = 0;
@@ -1026,9 +1232,7 @@ int launch_potrf_repair(double* A, int n, int jf, const double* orig, hipStream_
 constexpr int SB = 256;
 
 static size_t sb_count(int n) { return (size_t)((n + SB - 1) / SB); }
-size_t potrf_linv_elems(int n) {
-  return 2 * (size_t)((n + NB - 1) / NB) * NB * NB + 4 * sb_count(n) * SB * SB;
-}
+size_t potrf_linv_elems(int n) { return l21_offset(n) + l21_elems(n) + 4 * sb_count(n) * SB * SB; }
 
 // acc (+)= sign * A B over 64 x 64 LDS tiles; the thread owns rows
 // rb .. rb + 3 and columns cb .. cb + 3
@@ -1347,7 +1551,7 @@ __global__ __launch_bounds__(256) void k_sbsv_bwd(const double* __restrict__ L,
 int launch_potrs(const double* L, const double* Linv, int n, const double* b, double* x,
                  double* w, hipStream_t st) {
   const int nsb = (int)sb_count(n);
-  double* X = const_cast<double*>(Linv) + 2 * (size_t)((n + NB - 1) / NB) * NB * NB;
+  double* X = const_cast<double*>(Linv) + l21_offset(n) + l21_elems(n);
   double* XT = X + (size_t)nsb * SB * SB;
   double* PT = XT + (size_t)nsb * SB * SB;
   double* Q = PT + (size_t)nsb * SB * SB;

@@ -356,6 +356,31 @@ def test_cholesky_failure_state_matches_unblocked(hip, orc):
         assert ok == (1 if rc == 0 else 0)
 
 
+def test_merged_panels_bit_equal(hip, monkeypatch):
+    """Both inner panels of an outer panel in one launch give the factor and
+    the solution of the two-launch schedule bit for bit, including a
+    failing factorisation and partial last panels."""
+    rng = np.random.default_rng(21)
+    cases = [(700, None), (1030, None), (321, None), (193, None), (600, 450)]
+    for B, rank in cases:
+        X = rng.standard_normal((B, rank if rank else B + 37))
+        A = np.ascontiguousarray(X @ X.T / X.shape[1] + (0.0 if rank else 0.05) * np.eye(B))
+        b = np.ascontiguousarray(rng.standard_normal(B))
+        out = {}
+        for merge in ("0", "1"):
+            monkeypatch.setenv("NQS_HIP_PANEL_MERGE", merge)
+            Aw = A.copy()
+            x = np.zeros(B)
+            ok = hip.nqs_hip_cholesky_solve(ptr(Aw), B, ptr(b), ptr(x))
+            out[merge] = (ok, Aw, x)
+        print(f"B={B} rank={rank}: ok={out['1'][0]}", flush=True)
+        assert out["0"][0] == out["1"][0]
+        np.testing.assert_array_equal(out["0"][1], out["1"][1])
+        np.testing.assert_array_equal(out["0"][2], out["1"][2])
+        if not rank:
+            np.testing.assert_allclose(A @ out["1"][2], b, atol=1e-8)
+
+
 def test_large_shapes_consistent_with_oracle(hip, orc):
     """Configuration shapes of the benchmark (6x6 periodic Heisenberg + Marshall,
     complex RBM alpha=4; kagome 2x4 periodic, complex RBM M=96; J1-J2 10x10
