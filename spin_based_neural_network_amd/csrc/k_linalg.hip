@@ -537,16 +537,16 @@ int launch_gemm_nt(int M, int N, int K, double alpha, const double* A, long lda,
 
 // --------------------------------------------------------------------------
 // Cholesky, right-looking.  Outer panels of 128 columns, each made of two
-// 64-column inner panels; one launch (k_panel) per inner panel factors the
-// diagonal block, inverts it, and solves every row below it (the second
-// inner panel first takes the first's rank-64 update inside the same
-// launch).  The trailing matrix takes one update per wide panel of outer
-// panels (rank 512 by default; inside a wide panel each outer panel's
-// rank-128 update reaches the wide panel's later columns first) on the
-// large-tile MFMA GEMM.  With lookahead, the next wide panel's columns are
-// updated first and that panel is factored on a second stream while the
-// rest of the trailing matrix is updated.  The inverses of the 64x64
-// diagonal blocks are kept for the triangular solves.
+// 64-column inner panels; one launch (k_panel2) factors both: per inner
+// panel the diagonal block is factored and inverted and every row below it
+// solved (the second inner panel first takes the first's rank-64 update).
+// The trailing matrix takes one update per wide panel of outer panels (rank
+// 512 by default; inside a wide panel each outer panel's rank-128 update
+// reaches the wide panel's later columns first) on the large-tile MFMA GEMM.
+// With lookahead, the next wide panel's columns are updated first and that
+// panel is factored on a second stream while the rest of the trailing matrix
+// is updated.  The inverses of the 64x64 diagonal blocks are kept for the
+// triangular solves.
 // --------------------------------------------------------------------------
 constexpr int NB = 64, NBO = 128;
 
@@ -1157,26 +1157,42 @@ int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) 
     return (e && e[0] == '0') ? 0 : 1;
   }();
   hipStream_t side = lookahead ? g_look : st;
+  // The panel chain stays on the side stream: step k's lookahead update (the
+  // next wide panel's columns, after the previous trailing update) and the
+  // next wide panel follow each other there without a stream hop; the main
+  // stream runs the rest of the trailing update after the lookahead update,
+  // concurrently with the next wide panel (disjoint column ranges).
+  auto event = [&](size_t i) -> hipEvent_t {
+    while (g_ev.size() <= i) {
+      hipEvent_t e;
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+      g_ev.push_back(e);
+    }
+    return g_ev[i];
+  };
+  hipEvent_t ev_first = event(0);
+  if (!ev_first) return -1;
+  NQS_HIP_CHECK(hipEventRecord(ev_first, st));
+  NQS_HIP_CHECK(hipStreamWaitEvent(side, ev_first, 0));
   for (int k = 0;; k++) {
     const int R0 = K0 + w;
     if (R0 >= n) break;
     int w1 = panel_width(n - R0);
     if (w1 > n - R0) w1 = n - R0;
-    while ((int)g_ev.size() < 2 * (k + 1)) {
-      hipEvent_t e;
-      NQS_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-      g_ev.push_back(e);
-    }
+    hipEvent_t ev_panel = event(2 * k + 1), ev_rest = event(2 * k + 2);   // ev_panel: after the lookahead
+    if (!ev_panel || !ev_rest) return -1;
+    if (k > 0) NQS_HIP_CHECK(hipStreamWaitEvent(side, g_ev[2 * k], 0));   // rest of step k - 1
     const double* L = A + (long)R0 * n + K0;
     // next wide panel's columns [R0, R0 + w1), rows >= R0 (lower part)
     if (gemm_big_impl(n - R0, w1, w, -1.0, L, n, L, n, 1.0, A + (long)R0 * n + R0, n, 0, 1,
-                      fail_dev, "gemm_potrf", st))
+                      fail_dev, "gemm_potrf", side))
       return -1;
-    NQS_HIP_CHECK(hipEventRecord(g_ev[2 * k], st));
-    NQS_HIP_CHECK(hipStreamWaitEvent(side, g_ev[2 * k], 0));
+    NQS_HIP_CHECK(hipEventRecord(ev_panel, side));
     if (wide_panel(A, n, R0, w1, Linv, fail_dev, side)) return -1;
-    NQS_HIP_CHECK(hipEventRecord(g_ev[2 * k + 1], side));
-    // the rest of the trailing matrix: rows and columns >= R0 + w1
+    // the rest of the trailing matrix: rows and columns >= R0 + w1, after
+    // the lookahead update (so that the lookahead runs alone) and thereby
+    // after this step's wide panel
+    NQS_HIP_CHECK(hipStreamWaitEvent(st, ev_panel, 0));
     const int R1 = R0 + w1;
     if (R1 < n) {
       const double* L1 = A + (long)R1 * n + K0;
@@ -1184,10 +1200,15 @@ int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) 
                         fail_dev, "gemm_potrf", st))
         return -1;
     }
-    NQS_HIP_CHECK(hipStreamWaitEvent(st, g_ev[2 * k + 1], 0));
+    NQS_HIP_CHECK(hipEventRecord(ev_rest, st));
     K0 = R0;
     w = w1;
   }
+  // everything on the side stream before the final copy
+  static hipEvent_t ev_end = nullptr;
+  if (!ev_end) NQS_HIP_CHECK(hipEventCreateWithFlags(&ev_end, hipEventDisableTiming));
+  NQS_HIP_CHECK(hipEventRecord(ev_end, side));
+  NQS_HIP_CHECK(hipStreamWaitEvent(st, ev_end, 0));
   hipLaunchKernelGGL(k_diag_copy, dim3((unsigned)((n + NB - 1) / NB)), dim3(256), 0, st, A, n, Linv,
                      Linv + l21_offset(n), panel_merge(), fail_dev);
   count_launch();
