@@ -1731,9 +1731,25 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
   a->m64.valid = 0;
   a->m32.valid = 0;
   CKR(nqs::launch_sum(delta, P, delta, 1.0, sc + 5, g_stream));
+  // scalars, parameters and energies through one pinned staging buffer:
+  // three copies, one synchronisation
+  static double* stage = nullptr;
+  static size_t stage_n = 0;
+  const size_t need = 8 + (size_t)P + (size_t)B;
+  if (stage_n < need) {
+    if (stage) (void)hipHostFree(stage);
+    stage = nullptr;
+    stage_n = 0;
+    CK(hipHostMalloc((void**)&stage, sizeof(double) * need, hipHostMallocDefault));
+    stage_n = need;
+  }
+  CK(hipMemcpyAsync(stage, sc, sizeof(double) * 6, hipMemcpyDeviceToHost, g_stream));
+  CK(hipMemcpyAsync(stage + 8, a->d_par.p, sizeof(double) * P, hipMemcpyDeviceToHost, g_stream));
+  CK(hipMemcpyAsync(stage + 8 + P, E, sizeof(double) * B, hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
   double hs[6];
-  CK(hipMemcpyAsync(hs, sc, sizeof(double) * 6, hipMemcpyDeviceToHost, g_stream));
-  CKR(ansatz_download(a));
+  memcpy(hs, stage, sizeof(hs));
+  memcpy(a->h.data(), stage + 8, sizeof(double) * P);
   tm.mark();
   if (info) {
     info->mean_energy = hs[0];
@@ -1744,9 +1760,7 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
     info->converged = converged;
   }
   // per-chain energy series for tau_int
-  s->last_E.resize(B);
-  CK(hipMemcpyAsync(s->last_E.data(), E, sizeof(double) * B, hipMemcpyDeviceToHost, g_stream));
-  CK(hipStreamSynchronize(g_stream));
+  s->last_E.assign(stage + 8 + P, stage + 8 + P + B);
   s->last_rounds = rounds;
   tm.finish(s);
   if (!in_run) a->dev_fresh = 0;
