@@ -21,6 +21,7 @@
 #include <atomic>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/nqs/nqs_ansatz.h"
@@ -297,7 +298,7 @@ bool recognise(nqs_log_amp_fn_t fn, void* user, AmpSource& src) {
 // ---------------------------------------------------------------------------
 struct Workspace {
   DBuf spins, t_re, t_im, la, ar, e_re, e_im, O, Oi, colwork, u1, u2, vecs, T, diag, y, scal,
-      cg, fail, ham, sublat, hostla, hostar, linv, T2, Sd, elw;
+      cg, fail, ham, sublat, hostla, hostar, linv, T2, Sd, elw, fwd;
 };
 Workspace g_ws;
 
@@ -411,6 +412,16 @@ int forward_t(nqs_ansatz* a, const int* d_spins, int B, double* la, double* ar) 
   const size_t tb = sizeof(R) * (size_t)B * (size_t)std::max(a->M, 1);
   CKR(g_ws.t_re.ensure(tb));
   CKR(g_ws.t_im.ensure(tb));
+  // many FP64 RBM configurations, t only: theta on FP64 MFMA
+  if (std::is_same<R, double>::value && !la && !ar && B >= 512 && a->M >= 1 &&
+      (a->kind == NQS_ANSATZ_RBM || a->kind == NQS_ANSATZ_COMPLEX_RBM)) {
+    CKR(g_ws.fwd.ensure(sizeof(double) * (size_t)B * a->N));
+    ModelView<double> vd;
+    CKR(model_view<double>(a, vd));
+    return nqs::launch_forward_t_gemm(vd, a->d_par.as<double>(), d_spins, B,
+                                      g_ws.t_re.as<double>(), g_ws.t_im.as<double>(),
+                                      g_ws.fwd.as<double>(), g_stream);
+  }
   return nqs::launch_forward<R>(v, d_spins, B, g_ws.t_re.as<R>(), g_ws.t_im.as<R>(), la, ar,
                                 g_stream);
 }

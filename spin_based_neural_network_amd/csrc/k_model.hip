@@ -200,6 +200,62 @@ int launch_forward(const ModelView<R>& m, const int* spins, int B, R* t_re, R* t
 }
 
 // --------------------------------------------------------------------------
+// t = tanh(theta) of many configurations with theta = S W^T + b formed on
+// FP64 MFMA (launch_gemm_nt): the spins as doubles (B x N) times the
+// row-major W (M x N) of the parameter vector, into t itself; then
+// t <- tanh(t + b) in place (complex: both parts from W_re and W_im).
+// --------------------------------------------------------------------------
+__global__ void k_spins_f64(const int* __restrict__ s, long n, double* __restrict__ out) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    out[e] = (double)s[e];
+}
+
+template <bool CPLX>
+__global__ void k_theta_finish(double* __restrict__ t_re, double* __restrict__ t_im,
+                               const double* __restrict__ b_re, const double* __restrict__ b_im,
+                               long B, int M) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < B * M;
+       e += (long)gridDim.x * blockDim.x) {
+    const int h = (int)(e % M);
+    if (CPLX) {
+      double tr, ti;
+      ctanh<double>(t_re[e] + b_re[h], t_im[e] + b_im[h], tr, ti);
+      t_re[e] = tr;
+      t_im[e] = ti;
+    } else {
+      t_re[e] = tanh(t_re[e] + b_re[h]);
+    }
+  }
+}
+
+int launch_forward_t_gemm(const ModelView<double>& m, const double* d_par, const int* spins, int B,
+                          double* t_re, double* t_im, double* sd, hipStream_t st) {
+  if (B <= 0) return 0;
+  const int N = m.N, M = m.M;
+  const long Pr = (long)N + M + (long)M * N;
+  KScope ks("forward", st);
+  long blocks = ((long)B * N + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_spins_f64, dim3((unsigned)blocks), dim3(256), 0, st, spins, (long)B * N, sd);
+  count_launch();
+  if (launch_gemm_nt(B, M, N, 1.0, sd, N, d_par + N + M, N, 0.0, t_re, M, 0, st)) return -1;
+  if (m.cplx &&
+      launch_gemm_nt(B, M, N, 1.0, sd, N, d_par + Pr + N + M, N, 0.0, t_im, M, 0, st))
+    return -1;
+  blocks = ((long)B * M + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (m.cplx)
+    hipLaunchKernelGGL(k_theta_finish<true>, dim3((unsigned)blocks), dim3(256), 0, st, t_re, t_im,
+                       m.b_re, m.b_im, (long)B, M);
+  else
+    hipLaunchKernelGGL(k_theta_finish<false>, dim3((unsigned)blocks), dim3(256), 0, st, t_re, t_im,
+                       m.b_re, m.b_im, (long)B, M);
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// --------------------------------------------------------------------------
 // O_k rows.  Real block layout [a (N) | b (M) | W (M x N)], imaginary block
 // (complex RBM) the same at offset Pr.
 //   real RBM / mean-field : [s | t | t s]

@@ -74,6 +74,10 @@ void free_model(ModelBuffers<R>& mb);
 template <typename R>
 int launch_forward(const ModelView<R>& m, const int* spins, int B, R* t_re, R* t_im,
                    double* la, double* ar, hipStream_t st);
+// t only, FP64, theta = S W^T + b on FP64 MFMA (d_par: the parameter
+// vector, sd: B*N doubles of scratch)
+int launch_forward_t_gemm(const ModelView<double>& m, const double* d_par, const int* spins, int B,
+                          double* t_re, double* t_im, double* sd, hipStream_t st);
 // O rows [B][P] from spins and t: mode 0 real-projected, 1 holomorphic.
 template <typename R>
 int launch_gen_O(const ModelView<R>& m, int kind, const int* spins, int B,
