@@ -30,8 +30,19 @@ __device__ inline int select_site(const uint64_t* sp, int nw, int N, int want, i
     uint64_t z = (want ? sp[w] : ~sp[w]) & mask;
     int cnt = __popcll(z);
     if (k < cnt) {
-      for (int t = 0; t < k; t++) z &= z - 1ULL;
-      return 64 * w + __ffsll((long long)z) - 1;
+      // position of the (k+1)-th set bit: binary search over halves
+      int pos = 0;
+#pragma unroll
+      for (int width = 32; width >= 1; width >>= 1) {
+        const uint64_t low = z & ((1ULL << width) - 1ULL);
+        const int c = __popcll(low);
+        if (k >= c) {
+          k -= c;
+          z >>= width;
+          pos += width;
+        }
+      }
+      return 64 * w + pos;
     }
     k -= cnt;
   }
@@ -142,7 +153,7 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
       // table; the pair form reads the second site's rows as well)
       auto factors = [&](auto pair_tag) {
         constexpr bool PAIR = decltype(pair_tag)::value;
-        constexpr int QB = 4;
+        constexpr int QB = HPL >= 16 ? 8 : 4;
 #pragma unroll
         for (int q0 = 0; q0 < HPL; q0 += QB) {
           R cir[QB], sir[QB], cii[QB], sii[QB], cjr[QB], sjr[QB], cji[QB], sji[QB];
