@@ -258,7 +258,11 @@ def gpu_run(args, w):
     log(f"[bench] workload {args.workload}: N={st['N']} P={st['P']} samples/iter="
         f"{w['chains'] * w['per_chain']}")
     lib.nqs_sampler_thermalize(st["s"])
-    # one profiling step: which kernel dominates the iteration
+    for i in range(args.warmup):
+        step(lib, w, st)
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    # one profiling step after the warm-up (first-touch costs of the work
+    # buffers are out of it): which kernel dominates the iteration
     lib.nqs_hip_kernel_timing(",".join(TAGS).encode())
     step(lib, w, st)
     per = {}
@@ -270,9 +274,6 @@ def gpu_run(args, w):
     dom = max(per, key=lambda t: per[t][1]) if per else "gemm"
     log("[bench] kernel split of one iteration (launches, ms): " +
         ", ".join(f"{t}={c}/{ms:.2f}" for t, (c, ms) in sorted(per.items(), key=lambda kv: -kv[1][1])))
-    for i in range(args.warmup):
-        step(lib, w, st)
-        log(f"[bench] warmup {i + 1}/{args.warmup} done")
     C, R = w["chains"], w["per_chain"]
     series = np.zeros((C, 0))
     buf = np.zeros(C * R)

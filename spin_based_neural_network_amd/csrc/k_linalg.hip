@@ -404,6 +404,87 @@ static int gemm_nt_impl(int M, int N, int K, double alpha, const double* A, long
 // --------------------------------------------------------------------------
 constexpr int BT = 128, BKC = 32;
 
+// K loop shared by the large-tile kernels: acc += A[rows] B[cols]^T over K.
+// Each chunk of 32 is one scheduling region per MFMA step: the next chunk's
+// global loads go out during steps 0-3 (two rows per thread per step), the
+// operands of step s+1 are read from LDS while the MFMAs of step s issue, and
+// the next chunk's LDS stores are interleaved with step 7. Loads use clamped
+// addresses without branches; columns past K are zeroed when stored, rows
+// past M / N hold duplicates whose products are never written.
+__device__ __attribute__((always_inline)) inline void tile_kloop(
+    const double* A, long lda, const double* __restrict__ Bm, long ldb, int M, int N, int K,
+    int row0, int col0, v4d (&acc)[4][2], double (&As)[2][BT][BKC + 1],
+    double (&Bs)[2][BT][BKC + 1]) {
+  if (K <= 0) return;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wr = (wave >> 2) * 64, wc = (wave & 3) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int kk = tid & 31, lrow = tid >> 5;
+  const double* pa[8];
+  const double* pb[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    pa[q] = A + (long)min(row0 + lrow + 16 * q, M - 1) * lda;
+    pb[q] = Bm + (long)min(col0 + lrow + 16 * q, N - 1) * ldb;
+  }
+  double ra[8], rb[8];
+  const int nch = (K + BKC - 1) / BKC;
+  auto gload = [&](int q, int k0) {
+    const int gk = min(k0 + kk, K - 1);
+    ra[q] = pa[q][gk];
+    rb[q] = pb[q][gk];
+  };
+  auto sstore = [&](int buf, int q, int k0) {
+    const bool ok = k0 + kk < K;
+    As[buf][lrow + 16 * q][kk] = ok ? ra[q] : 0.0;
+    Bs[buf][lrow + 16 * q][kk] = ok ? rb[q] : 0.0;
+  };
+  __syncthreads();   // a previous K loop's last reads of LDS are done
+#pragma unroll
+  for (int q = 0; q < 8; q++) gload(q, 0);
+#pragma unroll
+  for (int q = 0; q < 8; q++) sstore(0, q, 0);
+  __syncthreads();
+  double a[2][4], b[2][2];
+#pragma unroll
+  for (int i = 0; i < 4; i++) a[0][i] = As[0][wr + 16 * i + lr][lk];
+#pragma unroll
+  for (int j = 0; j < 2; j++) b[0][j] = Bs[0][wc + 16 * j + lr][lk];
+  for (int c = 0; c < nch; c++) {
+    const int buf = c & 1, k1 = (c + 1) * BKC;
+#pragma unroll
+    for (int s = 0; s < BKC / 4; s++) {
+      const int cur = s & 1;
+      if (s < 4) {
+        gload(2 * s, k1);
+        gload(2 * s + 1, k1);
+      }
+      if (s + 1 < BKC / 4) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) a[cur ^ 1][i] = As[buf][wr + 16 * i + lr][4 * (s + 1) + lk];
+#pragma unroll
+        for (int j = 0; j < 2; j++) b[cur ^ 1][j] = Bs[buf][wc + 16 * j + lr][4 * (s + 1) + lk];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[cur][i], b[cur][j], acc[i][j], 0, 0, 0);
+      if (s == BKC / 4 - 1) {
+        // the other buffer's last reads were before the previous barrier
+#pragma unroll
+        for (int q = 0; q < 8; q++) sstore(buf ^ 1, q, k1);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; i++) a[0][i] = As[buf ^ 1][wr + 16 * i + lr][lk];
+#pragma unroll
+    for (int j = 0; j < 2; j++) b[0][j] = Bs[buf ^ 1][wc + 16 * j + lr][lk];
+  }
+}
+
 template <int USE>
 __global__ __launch_bounds__(512) void k_gemm_big(int M, int N, int K, double alpha,
                                                   const double* A, long lda,
@@ -411,8 +492,7 @@ __global__ __launch_bounds__(512) void k_gemm_big(int M, int N, int K, double al
                                                   double beta, double* C, long ldc, int lower,
                                                   int mask, int tiles_n, const int* skip) {
   // 8 waves, 2 per SIMD: wave (wm, wn) computes rows wm*64 + [0, 64) and
-  // columns wn*32 + [0, 32) of the tile as 4x2 MFMA blocks; the operands of
-  // step kk + 4 are read from LDS while the MFMAs of step kk issue.
+  // columns wn*32 + [0, 32) of the tile as 4x2 MFMA blocks (tile_kloop).
   const int skipped = skip ? *skip : 0;
   __shared__ double As[2][BT][BKC + 1];
   __shared__ double Bs[2][BT][BKC + 1];
@@ -428,8 +508,6 @@ __global__ __launch_bounds__(512) void k_gemm_big(int M, int N, int K, double al
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-  double ra[8], rb[8];
-  const int nch = (K + BKC - 1) / BKC;
   // C of an accumulating update is loaded before the K loop, so its latency
   // hides behind the MFMAs
   double cpre[4][2][4];
@@ -447,52 +525,7 @@ __global__ __launch_bounds__(512) void k_gemm_big(int M, int N, int K, double al
   }
   // the skip flag is read with the C tile; nothing is written before the test
   if (skipped) return;
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int e = tid + 512 * q, r = e >> 5, kk = e & 31, gk = k0 + kk;
-      const int ga = row0 + r, gb = col0 + r;
-      ra[q] = (ga < M && gk < K) ? A[(long)ga * lda + gk] : 0.0;
-      rb[q] = (gb < N && gk < K) ? Bm[(long)gb * ldb + gk] : 0.0;
-    }
-  };
-  auto sstore = [&](int buf) {
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int e = tid + 512 * q, r = e >> 5, kk = e & 31;
-      As[buf][r][kk] = ra[q];
-      Bs[buf][r][kk] = rb[q];
-    }
-  };
-  gload(0);
-  sstore(0);
-  __syncthreads();
-  for (int c = 0; c < nch; c++) {
-    const int buf = c & 1;
-    if (c + 1 < nch) gload((c + 1) * BKC);
-    double a[2][4], b[2][2];
-#pragma unroll
-    for (int i = 0; i < 4; i++) a[0][i] = As[buf][wr + 16 * i + lr][lk];
-#pragma unroll
-    for (int j = 0; j < 2; j++) b[0][j] = Bs[buf][wc + 16 * j + lr][lk];
-#pragma unroll
-    for (int s = 0; s < BKC / 4; s++) {
-      const int cur = s & 1;
-      if (s + 1 < BKC / 4) {
-#pragma unroll
-        for (int i = 0; i < 4; i++) a[cur ^ 1][i] = As[buf][wr + 16 * i + lr][4 * (s + 1) + lk];
-#pragma unroll
-        for (int j = 0; j < 2; j++) b[cur ^ 1][j] = Bs[buf][wc + 16 * j + lr][4 * (s + 1) + lk];
-      }
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[cur][i], b[cur][j], acc[i][j], 0, 0, 0);
-    }
-    if (c + 1 < nch) sstore(buf ^ 1);
-    __syncthreads();
-  }
+  tile_kloop(A, lda, Bm, ldb, M, N, K, row0, col0, acc, As, Bs);
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -1706,61 +1739,11 @@ __global__ __launch_bounds__(512) void k_gram_fused(int B, const unsigned long l
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-  // acc += X[rows] X[cols]^T over K columns of the row-major B x K matrix X
-  auto kloop = [&](const double* __restrict__ X, int K) {
-    double ra[8], rb[8];
-    const int nch = (K + BKC - 1) / BKC;
-    auto gload = [&](int k0) {
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const int e = tid + 512 * q, r = e >> 5, kk = e & 31, gk = k0 + kk;
-        const int ga = row0 + r, gb = col0 + r;
-        ra[q] = (ga < B && gk < K) ? X[(long)ga * K + gk] : 0.0;
-        rb[q] = (gb < B && gk < K) ? X[(long)gb * K + gk] : 0.0;
-      }
-    };
-    auto sstore = [&](int buf) {
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const int e = tid + 512 * q, r = e >> 5, kk = e & 31;
-        As[buf][r][kk] = ra[q];
-        Bs[buf][r][kk] = rb[q];
-      }
-    };
-    __syncthreads();   // the previous K loop's last reads of LDS are done
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    for (int c = 0; c < nch; c++) {
-      const int buf = c & 1;
-      if (c + 1 < nch) gload((c + 1) * BKC);
-      double a[2][4], b[2][2];
-#pragma unroll
-      for (int i = 0; i < 4; i++) a[0][i] = As[buf][wr + 16 * i + lr][lk];
-#pragma unroll
-      for (int j = 0; j < 2; j++) b[0][j] = Bs[buf][wc + 16 * j + lr][lk];
-#pragma unroll
-      for (int s = 0; s < BKC / 4; s++) {
-        const int cur = s & 1;
-        if (s + 1 < BKC / 4) {
-#pragma unroll
-          for (int i = 0; i < 4; i++) a[cur ^ 1][i] = As[buf][wr + 16 * i + lr][4 * (s + 1) + lk];
-#pragma unroll
-          for (int j = 0; j < 2; j++) b[cur ^ 1][j] = Bs[buf][wc + 16 * j + lr][4 * (s + 1) + lk];
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-          for (int j = 0; j < 2; j++)
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[cur][i], b[cur][j], acc[i][j], 0, 0, 0);
-      }
-      if (c + 1 < nch) sstore(buf ^ 1);
-      __syncthreads();
-    }
-  };
+  // acc += X[rows] X[cols]^T over the Ku columns of the row-major B x Ku
+  // factors
   if (Ku > 0) {
-    kloop(U1, Ku);
-    if (U2) kloop(U2, Ku);
+    tile_kloop(U1, Ku, U1, Ku, B, B, Ku, row0, col0, acc, As, Bs);
+    if (U2) tile_kloop(U2, Ku, U2, Ku, B, B, Ku, row0, col0, acc, As, Bs);
   }
 #pragma unroll
   for (int j = 0; j < 2; j++) {
