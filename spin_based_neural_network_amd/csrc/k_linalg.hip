@@ -1207,14 +1207,26 @@ int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) 
   if (!ev_first) return -1;
   NQS_HIP_CHECK(hipEventRecord(ev_first, st));
   NQS_HIP_CHECK(hipStreamWaitEvent(side, ev_first, 0));
+  // A rest-of-trailing-matrix update of more than `early` columns starts as
+  // soon as the wide panel it applies is factored, concurrently with the
+  // lookahead update (disjoint columns); a smaller one waits for the
+  // lookahead update so that the lookahead runs alone
+  // (NQS_HIP_POTRF_EARLY_REST, default 8192)
+  static const int early = [] {
+    const char* e = getenv("NQS_HIP_POTRF_EARLY_REST");
+    return e ? atoi(e) : 8192;
+  }();
+  hipEvent_t ev_wp = ev_first;   // the wide panel at K0 is factored
   for (int k = 0;; k++) {
     const int R0 = K0 + w;
     if (R0 >= n) break;
     int w1 = panel_width(n - R0);
     if (w1 > n - R0) w1 = n - R0;
-    hipEvent_t ev_panel = event(2 * k + 1), ev_rest = event(2 * k + 2);   // ev_panel: after the lookahead
-    if (!ev_panel || !ev_rest) return -1;
-    if (k > 0) NQS_HIP_CHECK(hipStreamWaitEvent(side, g_ev[2 * k], 0));   // rest of step k - 1
+    // events 3k + 1 .. 3k + 3: after the lookahead, after the rest, after
+    // the wide panel
+    hipEvent_t ev_panel = event(3 * k + 1), ev_rest = event(3 * k + 2), ev_wp1 = event(3 * k + 3);
+    if (!ev_panel || !ev_rest || !ev_wp1) return -1;
+    if (k > 0) NQS_HIP_CHECK(hipStreamWaitEvent(side, g_ev[3 * k - 1], 0));   // rest of step k - 1
     const double* L = A + (long)R0 * n + K0;
     // next wide panel's columns [R0, R0 + w1), rows >= R0 (lower part)
     if (gemm_big_impl(n - R0, w1, w, -1.0, L, n, L, n, 1.0, A + (long)R0 * n + R0, n, 0, 1,
@@ -1222,11 +1234,11 @@ int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) 
       return -1;
     NQS_HIP_CHECK(hipEventRecord(ev_panel, side));
     if (wide_panel(A, n, R0, w1, Linv, fail_dev, side)) return -1;
-    // the rest of the trailing matrix: rows and columns >= R0 + w1, after
-    // the lookahead update (so that the lookahead runs alone) and thereby
-    // after this step's wide panel
-    NQS_HIP_CHECK(hipStreamWaitEvent(st, ev_panel, 0));
+    NQS_HIP_CHECK(hipEventRecord(ev_wp1, side));
+    // the rest of the trailing matrix: rows and columns >= R0 + w1
     const int R1 = R0 + w1;
+    NQS_HIP_CHECK(hipStreamWaitEvent(st, (lookahead && n - R1 > early) ? ev_wp : ev_panel, 0));
+    ev_wp = ev_wp1;
     if (R1 < n) {
       const double* L1 = A + (long)R1 * n + K0;
       if (gemm_big_impl(n - R1, n - R1, w, -1.0, L1, n, L1, n, 1.0, A + (long)R1 * n + R1, n, 1, 1,
