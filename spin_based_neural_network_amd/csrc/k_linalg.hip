@@ -588,6 +588,25 @@ constexpr int NB = 64, NBO = 128;
 // once every panel has read its input), the merged outer panels' L21 blocks,
 // then the triangular solves' 256 x 256 super-block inverses and couplings
 
+// 1/d and 1/sqrt(d) for d > 0 from the hardware estimates and two
+// Newton-Raphson steps each (error within an ulp or two of the rounded
+// quotient / root; the pivot chain is shorter than with IEEE division and
+// square root)
+__device__ __attribute__((always_inline)) inline double recip_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __attribute__((always_inline)) inline double rsqrt_nr(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+  double e = fma(-d * y, y, 1.0);   // 1 - d y^2
+  y = fma(0.5 * y, e, y);
+  e = fma(-d * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
 // Step 2 of a panel: D = L L^T by blocks of 16 columns.  Wave jb factors
 // its own 16 columns with the rows in its lanes (only 1/pivot is on the
 // column chain, L_ij = a_ij / sqrt(d) is formed after the update; each
@@ -618,12 +637,12 @@ __device__ __attribute__((always_inline)) inline int panel_factor(double (*D)[NB
         const bool live = j < nb && jf == nb;
         const bool ok = live && d > 0.0;
         jf = (live && !ok) ? j : jf;
-        const double rd = 1.0 / d;
+        const double rd = recip_nr(d);
         const double si = (ok && lane > j) ? ci * rd : 0.0;
 #pragma unroll
         for (int t = jj + 1; t < 16; t++) rr[t] -= si * colw[wave][16 * jb + t];
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const double rs = sqrt(rd);   // 1 / sqrt(d)
+        const double rs = rsqrt_nr(d);
         rr[jj] = !ok ? ci : ((lane == j) ? d * rs : ((lane > j) ? ci * rs : ci));
       }
 #pragma unroll
@@ -659,10 +678,10 @@ __device__ __attribute__((always_inline)) inline int panel_factor(double (*D)[NB
 // Step 3: X = L^-1 by 16 x 16 blocks.  (a) wave w inverts diagonal block w
 // (lane c < 16 solves L_ww x = e_c with reciprocal pivots); (b) one block
 // diagonal at a time, X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj, the blocks of a
-// diagonal on separate waves (T_ij staged in LDS).  Rows beyond nb are
+// diagonal on separate waves (two MFMA products each).  Rows beyond nb are
 // identity rows of L, so their part of X is the identity.  All 256 threads.
 __device__ __attribute__((always_inline)) inline void panel_inverse(double (*D)[NB + 1], double (*X)[NB + 1], double* rcp,
-                                     double (*Tb)[16][17], int wave, int lane, int tid) {
+                                     int wave, int lane, int tid) {
   for (int e = tid; e < NB * NB; e += 256) X[e >> 6][e & 63] = 0.0;
   if (tid < NB) rcp[tid] = 1.0 / D[tid][tid];
   __syncthreads();
@@ -683,32 +702,23 @@ __device__ __attribute__((always_inline)) inline void panel_inverse(double (*D)[
     for (int q = 0; q < 16; q++) X[b0 + q][b0 + c] = x[q];
   }
   __syncthreads();
+  // (b) on MFMA, wave w forms block (w + dist, w): T = L[bi rows][bj, bi)
+  // X[bj, bi)[bj cols], then X = -X_ii T.  T's accumulator is the second
+  // product's B operand as it stands: lane l holds T[l/16 + 4r][l%16], the
+  // B element (k = 4r + l/16, col = l%16) of k step r.
+  const int lr = lane & 15, lk = lane >> 4;
   for (int dist = 1; dist < 4; dist++) {
-    const int bi = 16 * (wave + dist), bj = 16 * wave;   // block (wave + dist, wave)
-    const int r = lane >> 2, c0 = 4 * (lane & 3);
     if (wave + dist < 4) {
+      const int bi = 16 * (wave + dist), bj = 16 * wave;
+      v4d t = (v4d){0.0, 0.0, 0.0, 0.0};
+      for (int m0 = bj; m0 < bi; m0 += 4)
+        t = __builtin_amdgcn_mfma_f64_16x16x4f64(D[bi + lr][m0 + lk], X[m0 + lk][bj + lr], t, 0, 0, 0);
+      v4d x = (v4d){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        double t0 = 0.0, t1 = 0.0;
-        for (int m = bj; m < bi; m += 2) {
-          t0 += D[bi + r][m] * X[m][bj + c0 + u];
-          t1 += D[bi + r][m + 1] * X[m + 1][bj + c0 + u];
-        }
-        Tb[wave][r][c0 + u] = t0 + t1;
-      }
-    }
-    __syncthreads();
-    if (wave + dist < 4) {
+      for (int r = 0; r < 4; r++)
+        x = __builtin_amdgcn_mfma_f64_16x16x4f64(-X[bi + lr][bi + 4 * r + lk], t[r], x, 0, 0, 0);
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        double x0 = 0.0, x1 = 0.0;
-#pragma unroll
-        for (int m = 0; m < 16; m += 2) {
-          x0 -= X[bi + r][bi + m] * Tb[wave][m][c0 + u];
-          x1 -= X[bi + r][bi + m + 1] * Tb[wave][m + 1][c0 + u];
-        }
-        X[bi + r][bj + c0 + u] = x0 + x1;
-      }
+      for (int r = 0; r < 4; r++) X[bi + lk + 4 * r][bj + lr] = x[r];
     }
     __syncthreads();
   }
@@ -762,7 +772,6 @@ __global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb,
   __shared__ double Rs[NB][NB + 1];
   __shared__ double rcp[NB];
   __shared__ double colw[4][NB];
-  __shared__ double Tb[3][16][17];
   __shared__ int s_fail;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lr = lane & 15, lk = lane >> 4;
@@ -844,7 +853,7 @@ __global__ __launch_bounds__(256) void k_panel(double* A, int n, int k0, int nb,
     }
     return;
   }
-  panel_inverse(D, X, rcp, Tb, wave, lane, tid);
+  panel_inverse(D, X, rcp, wave, lane, tid);
   if (blockIdx.x == 0) panel_publish(D, X, n, k0, nb, Linv, tid);
   // step 4: rows = R Dinv^T
   v4d acc[4];
@@ -876,7 +885,6 @@ __global__ __launch_bounds__(256) void k_panel2(double* A, int n, int k0, int nb
   __shared__ double L2[NB][NB + 1];
   __shared__ double rcp[NB];
   __shared__ double colw[4][NB];
-  __shared__ double Tb[3][16][17];
   __shared__ int s_fail;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lr = lane & 15, lk = lane >> 4;
@@ -918,7 +926,7 @@ __global__ __launch_bounds__(256) void k_panel2(double* A, int n, int k0, int nb
     // the rows keep their input values
     return;
   }
-  panel_inverse(D, X, rcp, Tb, wave, lane, tid);
+  panel_inverse(D, X, rcp, wave, lane, tid);
   if (diag_rows) panel_publish(D, X, n, k0, NB, Linv, tid);
   {
     v4d ar[4], al[4];
@@ -997,7 +1005,7 @@ __global__ __launch_bounds__(256) void k_panel2(double* A, int n, int k0, int nb
     }
     return;
   }
-  panel_inverse(D, X, rcp, Tb, wave, lane, tid);
+  panel_inverse(D, X, rcp, wave, lane, tid);
   if (diag_rows) {
     panel_publish(D, X, n, k1, nb2, Linv, tid);
     return;
