@@ -138,6 +138,11 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
         j = select_site(sp, nw, N, 1, l);   // a down site  (s_j = -1)
       }
       const R si = bit_of(sp, i) ? (R)-1 : (R)1;
+      // log of the uniform draw that a downhill proposal uses, computed before
+      // the proposal's ratio so that its latency overlaps the table loads;
+      // the stream advances only if the draw is used
+      uint64_t rng_u = rng;
+      const double lu = log(xs_uniform(rng_u));
       // hidden factors of this lane; cosh d, sinh d and w = cosh d + t sinh d
       // of each unit are kept for the update of t on acceptance
       R pr = 1, pi = 0;
@@ -235,7 +240,10 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
       const double d = 2.0 * (off + dl);
       bool acc;
       if (d >= 0.0) acc = true;
-      else acc = (log(xs_uniform(rng)) < d);
+      else {
+        acc = lu < d;
+        rng = rng_u;
+      }
       ntot++;
       if (acc) {
         nacc++;
