@@ -22,6 +22,13 @@ namespace nqs {
 
 __device__ inline int bit_of(const uint64_t* sp, int i) { return (int)((sp[i >> 6] >> (i & 63)) & 1ULL); }
 
+// a wave-uniform 64-bit value into scalar registers
+__device__ inline uint64_t uniform_u64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // index of the k-th (0-based) site whose bit equals `want` among N sites
 __device__ inline int select_site(const uint64_t* sp, int nw, int N, int want, int k) {
   for (int w = 0; w < nw; w++) {
@@ -58,6 +65,13 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
 #pragma unroll
   for (int w = 0; w < 4; w++) sp[w] = (w < nw) ? cs.spins[(size_t)c * nw + w] : 0ULL;
   uint64_t rng = cs.rng[c];
+  if constexpr (L == 64) {
+    // one chain per wave: the configuration and the stream are wave-uniform
+    // (scalar registers); site choice and bookkeeping run on the scalar unit
+#pragma unroll
+    for (int w = 0; w < 4; w++) sp[w] = uniform_u64(sp[w]);
+    rng = uniform_u64(rng);
+  }
 
   // theta_h = b_h + sum_i W_hi s_i in site order; the site loop is outer so
   // that the loads of all of a lane's units for one site are in flight
@@ -117,23 +131,29 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
   double off = la_now - cs.la[c];
   double la_cur = la_now;
 
+  // exchange moves keep the numbers of up and down sites: both counts and
+  // the reciprocals of the draw moduli are fixed for the launch
+  int nd = 0;
+  for (int w = 0; w < nw; w++) {
+    int valid = N - 64 * w;
+    uint64_t mask = valid >= 64 ? ~0ULL : ((1ULL << valid) - 1ULL);
+    nd += __popcll(sp[w] & mask);
+  }
+  const int nu = N - nd;
+  const double rN = 1.0 / N, rnu = nu > 0 ? 1.0 / nu : 0.0, rnd = nd > 0 ? 1.0 / nd : 0.0;
+
   unsigned long long nacc = 0, ntot = 0;
   for (int r = 0; r < ml.rounds; r++) {
     for (int stp = 0; stp < ml.steps; stp++) {
       int i, j = -1;
+      // draws reduced on the vector unit, then back to scalar registers
+      auto uni = [](int v) { return L == 64 ? __builtin_amdgcn_readfirstlane(v) : v; };
       if (ml.move == 0) {
-        i = mod_small(xs_next(rng), (uint32_t)N);
+        i = uni(mod_recip(xs_next(rng), (uint32_t)N, rN));
       } else {
-        int nd = 0;
-        for (int w = 0; w < nw; w++) {
-          int valid = N - 64 * w;
-          uint64_t mask = valid >= 64 ? ~0ULL : ((1ULL << valid) - 1ULL);
-          nd += __popcll(sp[w] & mask);
-        }
-        const int nu = N - nd;
         if (nu == 0 || nd == 0) { ntot++; continue; }
-        const int k = mod_small(xs_next(rng), (uint32_t)nu);
-        const int l = mod_small(xs_next(rng), (uint32_t)nd);
+        const int k = uni(mod_recip(xs_next(rng), (uint32_t)nu, rnu));
+        const int l = uni(mod_recip(xs_next(rng), (uint32_t)nd, rnd));
         i = select_site(sp, nw, N, 0, k);   // an up site   (s_i = +1)
         j = select_site(sp, nw, N, 1, l);   // a down site  (s_j = -1)
       }
@@ -244,6 +264,8 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
         acc = lu < d;
         rng = rng_u;
       }
+      // every lane of the chain holds the same sums, hence the same decision
+      if constexpr (L == 64) acc = __builtin_amdgcn_readfirstlane((int)acc) != 0;
       ntot++;
       if (acc) {
         nacc++;

@@ -47,6 +47,23 @@ __host__ __device__ inline int mod_small(uint64_t x, uint32_t n) {
   else if (r >= (int64_t)n) r -= n;
   return (int)r;
 }
+// The same x mod n with rn = 1/n (rounded) supplied: each quotient estimate
+// trunc(t * rn) is within one of floor(t / n) (t < 2^53, relative error of
+// the product below 2^-52), and one correction step makes the remainder
+// exact; no division on the device.
+__host__ __device__ inline int mod_recip(uint64_t x, uint32_t n, double rn) {
+  const uint32_t hi = (uint32_t)(x >> 32);
+  int64_t q = (int64_t)((double)hi * rn);
+  int64_t rh = (int64_t)hi - q * (int64_t)n;
+  if (rh < 0) rh += n;
+  else if (rh >= (int64_t)n) rh -= n;
+  const uint64_t t = ((uint64_t)rh << 32) | (uint32_t)x;
+  q = (int64_t)((double)t * rn);
+  int64_t r = (int64_t)t - q * (int64_t)n;
+  if (r < 0) r += n;
+  else if (r >= (int64_t)n) r -= n;
+  return (int)r;
+}
 __host__ __device__ inline double xs_uniform(uint64_t& s) {
   return (double)(xs_next(s) >> 11) * (1.0 / 9007199254740992.0);
 }
