@@ -1567,9 +1567,7 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
   CKR(g_ws.vecs.ensure(sizeof(double) * 9 * (size_t)P));
   if (ofree) {
     const int nblk = a->kind == NQS_ANSATZ_COMPLEX_RBM ? 2 : 1;
-    CKR(g_ws.colwork.ensure(sizeof(double) * (size_t)std::max(
-                                nqs::factored_delta_work_elems(B, N, a->M, nblk),
-                                (B + 127) / 128 * B)));
+    CKR(g_ws.colwork.ensure(sizeof(double) * nqs::gram_factored_work_elems(B, N, a->M, nblk)));
   } else {
     CKR(g_ws.colwork.ensure(sizeof(double) * nqs::colsum_work_elems(B, P)));
   }

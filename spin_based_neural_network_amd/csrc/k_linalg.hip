@@ -1692,59 +1692,56 @@ __global__ void k_spins_to_mask(const int* __restrict__ s, int B, int N, int nw,
     if (s[b * N + i] < 0) m |= 1ull << (i - 64 * w);
   out[e] = m;
 }
-// r_i = (sum_{j<=i} G_ij + sum_{j>i} G_ji) / B from the lower triangle, in
-// fixed order: row part (one wave per row, coalesced), column part (two
-// stages over 128-row chunks, coalesced across columns)
-__global__ __launch_bounds__(256) void k_sym_rowpart(const double* __restrict__ G, int B,
-                                                     double* __restrict__ part) {
+// Row means of the factored Gram without the Gram: with Sigma_s = sum_j s_j,
+// Sigma_u = sum_j u_j and K = S^T U (all from the factored product with unit
+// weights), sum_j G_ij = s_i . (Sigma_s + Y_i) + u_i . Sigma_u, Y = U K^T,
+// since sum_j (s_i . s_j)(u_i . u_j) = s_i^T K u_i.
+// Km_z[a][h] = sign_z W_z[h][a] from the product's W block (the imaginary
+// block carries the conjugation sign -1)
+__global__ void k_gram_kmat(const double* __restrict__ dl, int N, int M, int nblk,
+                            double* __restrict__ Km) {
+  const long per = (long)N * M;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= nblk * per) return;
+  const int z = (int)(e / per);
+  const long q = e - z * per;
+  const int a = (int)(q / M), h = (int)(q - (long)a * M);
+  const double v = dl[(long)z * ((long)N + M + per) + N + M + (long)h * N + a];
+  Km[e] = z ? -v : v;
+}
+// r_i = sum_j G_ij / B, one wave per row, fixed-order butterfly
+__global__ __launch_bounds__(256) void k_gram_rowmean(const int* __restrict__ spins, int B, int N,
+                                                      int M, const double* __restrict__ U1,
+                                                      const double* __restrict__ U2,
+                                                      const double* __restrict__ Y,
+                                                      const double* __restrict__ dl,
+                                                      double* __restrict__ r) {
   const long i = blockIdx.x * 4L + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   if (i >= B) return;
-  const double* row = G + i * B;
+  const long P1 = (long)N + M + (long)M * N;
   double s = 0.0;
-  for (long j = lane; j <= i; j += 64) s += row[j];
+  for (int a = lane; a < N; a += 64)
+    s += (double)spins[i * N + a] * (dl[a] + (Y ? Y[i * N + a] : 0.0));
+  for (int h = lane; h < M; h += 64) {
+    s += U1[i * M + h] * dl[N + h];
+    if (U2) s -= U2[i * M + h] * dl[P1 + N + h];
+  }
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
-  if (lane == 0) part[i] = s;
-}
-__global__ __launch_bounds__(256) void k_sym_colpart(const double* __restrict__ G, int B,
-                                                     double* __restrict__ work) {
-  const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const long c = blockIdx.y;
-  if (j >= B) return;
-  const long i0 = c * 128, i1 = (i0 + 128 < B) ? i0 + 128 : B;
-  double s = 0.0;
-  for (long i = (i0 > j + 1 ? i0 : j + 1); i < i1; i++) s += G[i * B + j];
-  work[c * B + j] = s;
-}
-__global__ __launch_bounds__(256) void k_sym_rowmean(const double* __restrict__ part,
-                                                     const double* __restrict__ work, long nch,
-                                                     int B, double* __restrict__ r) {
-  const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (j >= B) return;
-  double s = part[j];
-  for (long c = 0; c < nch; c++) s += work[c * B + j];
-  r[j] = s / B;
-}
-
-__global__ void k_gram_center(double* T, const double* __restrict__ r, const double* g, int B,
-                              double eps) {
-  const long total = (long)B * B;
-  const double gg = *g, inv = 1.0 / B;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
-       e += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(e / B), j = (int)(e % B);
-    if (j <= i) T[e] = (T[e] - r[i] - r[j] + gg) * inv + (i == j ? eps : 0.0);
-  }
+  if (lane == 0) r[i] = s / B;
 }
 
 // Factored Gram in one pass: per 128 x 128 lower tile, UU = U1 U1^T
 // (+ U2 U2^T) (K = Ku each) on FP64 MFMA with the same tiling as k_gemm_big;
-// in the epilogue SS_ij = N - 2 popcount(mask_i ^ mask_j) (exact) and
-// G = (1 + SS)(1 + UU) - 1 is written once.
+// in the epilogue SS_ij = N - 2 popcount(mask_i ^ mask_j) (exact),
+// G = (1 + SS)(1 + UU) - 1, and the centred, shifted
+// T_ij = (G_ij - r_i - r_j + g) / B + eps [i == j] is written once.
 __global__ __launch_bounds__(512) void k_gram_fused(int B, const unsigned long long* __restrict__ mask,
                                                     int N, int nw, const double* __restrict__ U1,
                                                     const double* __restrict__ U2, int Ku,
+                                                    const double* __restrict__ rm,
+                                                    const double* __restrict__ gm, double eps,
                                                     double* __restrict__ T) {
   __shared__ double As[2][BT][BKC + 1];
   __shared__ double Bs[2][BT][BKC + 1];
@@ -1765,10 +1762,12 @@ __global__ __launch_bounds__(512) void k_gram_fused(int B, const unsigned long l
     tile_kloop(U1, Ku, U1, Ku, B, B, Ku, row0, col0, acc, As, Bs);
     if (U2) tile_kloop(U2, Ku, U2, Ku, B, B, Ku, row0, col0, acc, As, Bs);
   }
+  const double gg = *gm, inv = 1.0 / B;
 #pragma unroll
   for (int j = 0; j < 2; j++) {
     const int gc = col0 + wc + 16 * j + lr;
     const unsigned long long* mc = mask + (size_t)min(gc, B - 1) * nw;
+    const double rc = rm[min(gc, B - 1)];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -1779,42 +1778,53 @@ __global__ __launch_bounds__(512) void k_gram_fused(int B, const unsigned long l
           int d = 0;
           for (int w = 0; w < nw; w++) d += __popcll(mr[w] ^ mc[w]);
           const double ss = (double)(N - 2 * d);
-          T[(long)gr * B + gc] = (1.0 + ss) * (1.0 + acc[i][j][r]) - 1.0;
+          const double G = (1.0 + ss) * (1.0 + acc[i][j][r]) - 1.0;
+          T[(long)gr * B + gc] = (G - rm[gr] - rc + gg) * inv + (gr == gc ? eps : 0.0);
         }
       }
   }
+}
+
+size_t gram_factored_work_elems(int B, int N, int M, int nblk) {
+  return (size_t)factored_delta_work_elems(B, N, M, nblk) +
+         (size_t)nblk * ((size_t)N + M + (size_t)M * N) + (size_t)nblk * N * M + (size_t)B * N;
 }
 
 int launch_gram_factored(const int* spins, int B, int N, const double* U1, const double* U2, int Ku,
                          double eps, double* T, double* work, double* Sd, double* r, double* g,
                          hipStream_t st) {
   // the spin masks occupy the first B * nw words of the B * (N + 2) scratch Sd
-  const int nw = (N + 63) / 64;
+  const int nw = (N + 63) / 64, nblk = U2 ? 2 : 1;
   unsigned long long* mask = (unsigned long long*)Sd;
   hipLaunchKernelGGL(k_spins_to_mask, dim3((unsigned)(((long)B * nw + 255) / 256)), dim3(256), 0, st,
                      spins, B, N, nw, mask);
   count_launch();
+  // row means first (Sigma_s, Sigma_u, S^T U from the factored product with
+  // unit weights; Y = U K^T), so that the Gram is written centred
+  double* dl = work + factored_delta_work_elems(B, N, Ku, nblk);
+  double* Km = dl + (size_t)nblk * ((size_t)N + Ku + (size_t)Ku * N);
+  double* Y = Km + (size_t)nblk * N * Ku;
+  if (launch_factored_delta(spins, B, N, Ku, U1, U2, nullptr, 1.0, work, dl, st)) return -1;
+  if (Ku > 0) {
+    const long ne = (long)nblk * N * Ku;
+    hipLaunchKernelGGL(k_gram_kmat, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, dl, N, Ku,
+                       nblk, Km);
+    count_launch();
+    if (launch_gemm_nt(B, N, Ku, 1.0, U1, Ku, Km, Ku, 0.0, Y, N, 0, st)) return -1;
+    if (U2 && launch_gemm_nt(B, N, Ku, 1.0, U2, Ku, Km + (size_t)N * Ku, Ku, 1.0, Y, N, 0, st))
+      return -1;
+  }
+  hipLaunchKernelGGL(k_gram_rowmean, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, st, spins, B, N, Ku,
+                     U1, U2, Ku > 0 ? Y : nullptr, dl, r);
+  count_launch();
+  if (launch_sum(r, B, nullptr, 1.0 / B, g, st)) return -1;
   {
     KScope ks("gram_f", st);
     const long tmb = (B + BT - 1) / BT;
     hipLaunchKernelGGL(k_gram_fused, dim3((unsigned)(tmb * (tmb + 1) / 2)), dim3(512), 0, st, B, mask,
-                       N, nw, U1, U2, Ku, T);
+                       N, nw, U1, U2, Ku, r, g, eps, T);
     count_launch();
   }
-  long tb = ((long)B * B + 255) / 256;
-  if (tb > 65536) tb = 65536;
-  {
-    const long nch = (B + 127) / 128;
-    double* part = Sd + (size_t)B * nw;   // B doubles after the masks
-    hipLaunchKernelGGL(k_sym_rowpart, dim3((B + 3) / 4), dim3(256), 0, st, T, B, part);
-    hipLaunchKernelGGL(k_sym_colpart, dim3((B + 255) / 256, (unsigned)nch), dim3(256), 0, st, T, B,
-                       work);
-    hipLaunchKernelGGL(k_sym_rowmean, dim3((B + 255) / 256), dim3(256), 0, st, part, work, nch, B, r);
-    count_launch(3);
-  }
-  if (launch_sum(r, B, nullptr, 1.0 / B, g, st)) return -1;
-  hipLaunchKernelGGL(k_gram_center, dim3((unsigned)tb), dim3(256), 0, st, T, r, g, B, eps);
-  count_launch();
   NQS_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -1854,7 +1864,7 @@ __global__ __launch_bounds__(256) void k_factored_ot(const int* __restrict__ spi
     const int b = bk + lk;
     const bool ok = b < be;
     const int bc = ok ? b : bs;
-    const double wb = ok ? w[bc] : 0.0;
+    const double wb = ok ? (w ? w[bc] : 1.0) : 0.0;
     double av[2], bv[2];
 #pragma unroll
     for (int q = 0; q < 2; q++) {

@@ -254,14 +254,17 @@ int launch_potrs(const double* L, const double* Linv, int n, const double* b, do
                  double* w, hipStream_t st);
 // Factored MinSR Gram T = (O_c O_c^T)/B + eps I (lower triangle) for O rows
 // (s, u, u (x) s); u . u' = U1 U1^T (+ U2 U2^T when U2 is given), Ku columns
-// each (Ku = 0 for mean-field).  work: ceil(B/128)*B scratch, Sd: B*(N+2)
-// scratch, r: B scratch, g: device scalar.
+// each (Ku = 0 for mean-field).  work: gram_factored_work_elems(B, N, Ku,
+// 1 or 2) doubles, Sd: B*(N+2) scratch, r: B scratch (row means),
+// g: device scalar.
+size_t gram_factored_work_elems(int B, int N, int M, int nblk);
 int launch_gram_factored(const int* spins, int B, int N, const double* U1, const double* U2, int Ku,
                          double eps, double* T, double* work, double* Sd, double* r, double* g,
                          hipStream_t st);
 // MinSR update of an RBM from its factors: delta = alpha * O^T w over the
 // parameter blocks (O rows (s, u, u (x) s); u = U1, then u = -U2 with a
-// zero visible part when U2 is given), w = centred solution.  work:
+// zero visible part when U2 is given), w = centred solution (null: unit
+// weights).  work:
 // factored_delta_work_elems(B, N, M, 1 or 2) doubles.
 int factored_delta_work_elems(int B, int N, int M, int nblk);
 int launch_factored_delta(const int* spins, int B, int N, int M, const double* U1,
