@@ -258,19 +258,26 @@ def gpu_run(args, w):
     log(f"[bench] workload {args.workload}: N={st['N']} P={st['P']} samples/iter="
         f"{w['chains'] * w['per_chain']}")
     lib.nqs_sampler_thermalize(st["s"])
+
+    def profiled_step():
+        lib.nqs_hip_kernel_timing(",".join(TAGS).encode())
+        step(lib, w, st)
+        out = {}
+        for t in TAGS:
+            c, ms = ctypes.c_longlong(0), ctypes.c_double(0)
+            if lib.nqs_hip_kernel_stats(t.encode(), ctypes.byref(c), ctypes.byref(ms)) == 0 and c.value:
+                out[t] = (c.value, ms.value)
+        lib.nqs_hip_kernel_timing(None)
+        return out
+
+    # which kernel dominates the iteration: the split of a step after the
+    # warm-up (first-touch costs of the work buffers are out of it), or of
+    # the first step where later steps replay one graph (no per-kernel times)
+    per = profiled_step()
     for i in range(args.warmup):
         step(lib, w, st)
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
-    # one profiling step after the warm-up (first-touch costs of the work
-    # buffers are out of it): which kernel dominates the iteration
-    lib.nqs_hip_kernel_timing(",".join(TAGS).encode())
-    step(lib, w, st)
-    per = {}
-    for t in TAGS:
-        c, ms = ctypes.c_longlong(0), ctypes.c_double(0)
-        if lib.nqs_hip_kernel_stats(t.encode(), ctypes.byref(c), ctypes.byref(ms)) == 0 and c.value:
-            per[t] = (c.value, ms.value)
-    lib.nqs_hip_kernel_timing(None)
+    per = profiled_step() or per
     dom = max(per, key=lambda t: per[t][1]) if per else "gemm"
     log("[bench] kernel split of one iteration (launches, ms): " +
         ", ".join(f"{t}={c}/{ms:.2f}" for t, (c, ms) in sorted(per.items(), key=lambda kv: -kv[1][1])))
