@@ -396,21 +396,25 @@ static int gemm_nt_impl(int M, int N, int K, double alpha, const double* A, long
 
 // --------------------------------------------------------------------------
 // Large-tile NT GEMM for the Cholesky trailing updates and the direct Gram:
-// workgroup tile 128x128, 4 waves of 64x64 (4x4 MFMA blocks), K in chunks of
+// workgroup tile 128x128, 8 waves of 64x32 (4x2 MFMA blocks), K in chunks of
 // 32 double-buffered through LDS (global loads of chunk c+1 in flight while
-// chunk c feeds the MFMAs).  Each load instruction of a wave reads two rows'
-// 32 consecutive doubles (coalesced).  Same lower / mask / skip conventions
-// as k_gemm_nt.
+// chunk c feeds the MFMAs), one workgroup per CU: a Cholesky panel beside a
+// trailing update then finds whole CUs as soon as update tiles retire.  The
+// factored Gram uses chunks of 16 instead (68 KB of LDS, at most 128 VGPRs:
+// two workgroups per CU, one's barrier waits covered by the other's MFMAs).
+// Same lower / mask / skip conventions as k_gemm_nt.
 // --------------------------------------------------------------------------
-constexpr int BT = 128, BKC = 32;
+constexpr int BT = 128;
 
-// K loop shared by the large-tile kernels: acc += A[rows] B[cols]^T over K.
-// Each chunk of 32 is one scheduling region per MFMA step: the next chunk's
-// global loads go out during steps 0-3 (two rows per thread per step), the
-// operands of step s+1 are read from LDS while the MFMAs of step s issue, and
-// the next chunk's LDS stores are interleaved with step 7. Loads use clamped
-// addresses without branches; columns past K are zeroed when stored, rows
-// past M / N hold duplicates whose products are never written.
+// K loop shared by the large-tile kernels: acc += A[rows] B[cols]^T over K,
+// in chunks of BKC (16 or 32) columns.  Each MFMA step of a chunk is one
+// scheduling region: the next chunk's global loads go out during the first
+// steps (two rows per thread per step), the operands of step s+1 are read
+// from LDS while the MFMAs of step s issue, and the next chunk's LDS stores
+// are interleaved with the last step.  Loads use clamped addresses without
+// branches; columns past K are zeroed when stored, rows past M / N hold
+// duplicates whose products are never written.
+template <int BKC>
 __device__ __attribute__((always_inline)) inline void tile_kloop(
     const double* A, long lda, const double* __restrict__ Bm, long ldb, int M, int N, int K,
     int row0, int col0, v4d (&acc)[4][2], double (&As)[2][BT][BKC + 1],
@@ -419,31 +423,47 @@ __device__ __attribute__((always_inline)) inline void tile_kloop(
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wr = (wave >> 2) * 64, wc = (wave & 3) * 32;
   const int lr = lane & 15, lk = lane >> 4;
-  const int kk = tid & 31, lrow = tid >> 5;
-  const double* pa[8];
-  const double* pb[8];
+  // loader: thread tid fills rows tid / BKC + (512 / BKC) q at column
+  // tid % BKC
+  constexpr int NQ = BT * BKC / 512, STEPS = BKC / 4;
+  const int kk = tid & (BKC - 1), lrow = tid / BKC;
+  // row starts: pointers with one workgroup per CU; 32-bit row indices with
+  // two (128 VGPRs)
+  constexpr bool PTR = BKC == 32;
+  const double* pa[PTR ? NQ : 1];
+  const double* pb[PTR ? NQ : 1];
+  int oa[NQ], ob[NQ];
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    pa[q] = A + (long)min(row0 + lrow + 16 * q, M - 1) * lda;
-    pb[q] = Bm + (long)min(col0 + lrow + 16 * q, N - 1) * ldb;
+  for (int q = 0; q < NQ; q++) {
+    oa[q] = min(row0 + lrow + (512 / BKC) * q, M - 1);
+    ob[q] = min(col0 + lrow + (512 / BKC) * q, N - 1);
+    if constexpr (PTR) {
+      pa[q] = A + (long)oa[q] * lda;
+      pb[q] = Bm + (long)ob[q] * ldb;
+    }
   }
-  double ra[8], rb[8];
+  double ra[NQ], rb[NQ];
   const int nch = (K + BKC - 1) / BKC;
   auto gload = [&](int q, int k0) {
     const int gk = min(k0 + kk, K - 1);
-    ra[q] = pa[q][gk];
-    rb[q] = pb[q][gk];
+    if constexpr (PTR) {
+      ra[q] = pa[q][gk];
+      rb[q] = pb[q][gk];
+    } else {
+      ra[q] = A[(long)oa[q] * lda + gk];
+      rb[q] = Bm[(long)ob[q] * ldb + gk];
+    }
   };
   auto sstore = [&](int buf, int q, int k0) {
     const bool ok = k0 + kk < K;
-    As[buf][lrow + 16 * q][kk] = ok ? ra[q] : 0.0;
-    Bs[buf][lrow + 16 * q][kk] = ok ? rb[q] : 0.0;
+    As[buf][lrow + (512 / BKC) * q][kk] = ok ? ra[q] : 0.0;
+    Bs[buf][lrow + (512 / BKC) * q][kk] = ok ? rb[q] : 0.0;
   };
   __syncthreads();   // a previous K loop's last reads of LDS are done
 #pragma unroll
-  for (int q = 0; q < 8; q++) gload(q, 0);
+  for (int q = 0; q < NQ; q++) gload(q, 0);
 #pragma unroll
-  for (int q = 0; q < 8; q++) sstore(0, q, 0);
+  for (int q = 0; q < NQ; q++) sstore(0, q, 0);
   __syncthreads();
   double a[2][4], b[2][2];
 #pragma unroll
@@ -453,13 +473,13 @@ __device__ __attribute__((always_inline)) inline void tile_kloop(
   for (int c = 0; c < nch; c++) {
     const int buf = c & 1, k1 = (c + 1) * BKC;
 #pragma unroll
-    for (int s = 0; s < BKC / 4; s++) {
+    for (int s = 0; s < STEPS; s++) {
       const int cur = s & 1;
-      if (s < 4) {
+      if (2 * s < NQ) {
         gload(2 * s, k1);
         gload(2 * s + 1, k1);
       }
-      if (s + 1 < BKC / 4) {
+      if (s + 1 < STEPS) {
 #pragma unroll
         for (int i = 0; i < 4; i++) a[cur ^ 1][i] = As[buf][wr + 16 * i + lr][4 * (s + 1) + lk];
 #pragma unroll
@@ -470,10 +490,10 @@ __device__ __attribute__((always_inline)) inline void tile_kloop(
 #pragma unroll
         for (int j = 0; j < 2; j++)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[cur][i], b[cur][j], acc[i][j], 0, 0, 0);
-      if (s == BKC / 4 - 1) {
+      if (s == STEPS - 1) {
         // the other buffer's last reads were before the previous barrier
 #pragma unroll
-        for (int q = 0; q < 8; q++) sstore(buf ^ 1, q, k1);
+        for (int q = 0; q < NQ; q++) sstore(buf ^ 1, q, k1);
       }
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -494,8 +514,8 @@ __global__ __launch_bounds__(512) void k_gemm_big(int M, int N, int K, double al
   // 8 waves, 2 per SIMD: wave (wm, wn) computes rows wm*64 + [0, 64) and
   // columns wn*32 + [0, 32) of the tile as 4x2 MFMA blocks (tile_kloop).
   const int skipped = skip ? *skip : 0;
-  __shared__ double As[2][BT][BKC + 1];
-  __shared__ double Bs[2][BT][BKC + 1];
+  __shared__ double As[2][BT][32 + 1];
+  __shared__ double Bs[2][BT][32 + 1];
   int tm, tn;
   if (lower) lower_tile(blockIdx.x, tm, tn);
   else { tm = blockIdx.x / tiles_n; tn = blockIdx.x % tiles_n; }
@@ -523,9 +543,9 @@ __global__ __launch_bounds__(512) void k_gemm_big(int M, int N, int K, double al
           cpre[i][j][r] = (gr < M && gc < N && (!mask || gc <= gr)) ? C[(long)gr * ldc + gc] : 0.0;
         }
   }
-  // the skip flag is read with the C tile; nothing is written before the test
+  // nothing is written before the skip flag is tested
   if (skipped) return;
-  tile_kloop(A, lda, Bm, ldb, M, N, K, row0, col0, acc, As, Bs);
+  tile_kloop<32>(A, lda, Bm, ldb, M, N, K, row0, col0, acc, As, Bs);
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -1737,14 +1757,14 @@ __global__ __launch_bounds__(256) void k_gram_rowmean(const int* __restrict__ sp
 // in the epilogue SS_ij = N - 2 popcount(mask_i ^ mask_j) (exact),
 // G = (1 + SS)(1 + UU) - 1, and the centred, shifted
 // T_ij = (G_ij - r_i - r_j + g) / B + eps [i == j] is written once.
-__global__ __launch_bounds__(512) void k_gram_fused(int B, const unsigned long long* __restrict__ mask,
+__global__ __launch_bounds__(512, 2) void k_gram_fused(int B, const unsigned long long* __restrict__ mask,
                                                     int N, int nw, const double* __restrict__ U1,
                                                     const double* __restrict__ U2, int Ku,
                                                     const double* __restrict__ rm,
                                                     const double* __restrict__ gm, double eps,
                                                     double* __restrict__ T) {
-  __shared__ double As[2][BT][BKC + 1];
-  __shared__ double Bs[2][BT][BKC + 1];
+  __shared__ double As[2][BT][16 + 1];
+  __shared__ double Bs[2][BT][16 + 1];
   int tm, tn;
   lower_tile(blockIdx.x, tm, tn);
   const int row0 = tm * BT, col0 = tn * BT;
@@ -1759,8 +1779,8 @@ __global__ __launch_bounds__(512) void k_gram_fused(int B, const unsigned long l
   // acc += X[rows] X[cols]^T over the Ku columns of the row-major B x Ku
   // factors
   if (Ku > 0) {
-    tile_kloop(U1, Ku, U1, Ku, B, B, Ku, row0, col0, acc, As, Bs);
-    if (U2) tile_kloop(U2, Ku, U2, Ku, B, B, Ku, row0, col0, acc, As, Bs);
+    tile_kloop<16>(U1, Ku, U1, Ku, B, B, Ku, row0, col0, acc, As, Bs);
+    if (U2) tile_kloop<16>(U2, Ku, U2, Ku, B, B, Ku, row0, col0, acc, As, Bs);
   }
   const double gg = *gm, inv = 1.0 / B;
 #pragma unroll
