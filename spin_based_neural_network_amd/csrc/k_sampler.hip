@@ -258,13 +258,16 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
       if (j >= 0) dvis += (R)2 * m.a_re[j] * si;   // s_j = -s_i
       const double dl = (double)lsum + (double)dvis;
       const double d = 2.0 * (off + dl);
+      // every lane of the chain holds the same sums, hence the same decision
+      // (wave-uniform with one chain per wave: the stream stays scalar)
+      bool down = d < 0.0;
+      if constexpr (L == 64) down = __builtin_amdgcn_readfirstlane((int)down) != 0;
       bool acc;
-      if (d >= 0.0) acc = true;
+      if (!down) acc = true;
       else {
         acc = lu < d;
         rng = rng_u;
       }
-      // every lane of the chain holds the same sums, hence the same decision
       if constexpr (L == 64) acc = __builtin_amdgcn_readfirstlane((int)acc) != 0;
       ntot++;
       if (acc) {

@@ -47,21 +47,24 @@ __host__ __device__ inline int mod_small(uint64_t x, uint32_t n) {
   else if (r >= (int64_t)n) r -= n;
   return (int)r;
 }
-// The same x mod n with rn = 1/n (rounded) supplied: each quotient estimate
-// trunc(t * rn) is within one of floor(t / n) (t < 2^53, relative error of
-// the product below 2^-52), and one correction step makes the remainder
-// exact; no division on the device.
+// The same x mod n with rn = 1/n (rounded) supplied, in double precision
+// throughout: every operand and intermediate is an integer below 2^53 (hi,
+// rh 2^32 + lo), each quotient estimate trunc(t * rn) is within one of
+// floor(t / n) (relative error of the product below 2^-52), t - q n comes out
+// of one fused multiply-add exactly, and one correction step makes the
+// remainder exact; no integer division or 64-bit integer arithmetic.
 __host__ __device__ inline int mod_recip(uint64_t x, uint32_t n, double rn) {
-  const uint32_t hi = (uint32_t)(x >> 32);
-  int64_t q = (int64_t)((double)hi * rn);
-  int64_t rh = (int64_t)hi - q * (int64_t)n;
-  if (rh < 0) rh += n;
-  else if (rh >= (int64_t)n) rh -= n;
-  const uint64_t t = ((uint64_t)rh << 32) | (uint32_t)x;
-  q = (int64_t)((double)t * rn);
-  int64_t r = (int64_t)t - q * (int64_t)n;
-  if (r < 0) r += n;
-  else if (r >= (int64_t)n) r -= n;
+  const double dn = (double)n;
+  const double th = (double)(uint32_t)(x >> 32);
+  double q = trunc(th * rn);
+  double rh = fma(-q, dn, th);
+  rh += rh < 0.0 ? dn : 0.0;
+  rh -= rh >= dn ? dn : 0.0;
+  const double t = rh * 4294967296.0 + (double)(uint32_t)x;
+  q = trunc(t * rn);
+  double r = fma(-q, dn, t);
+  r += r < 0.0 ? dn : 0.0;
+  r -= r >= dn ? dn : 0.0;
   return (int)r;
 }
 __host__ __device__ inline double xs_uniform(uint64_t& s) {
