@@ -163,16 +163,11 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
       // the stream advances only if the draw is used
       uint64_t rng_u = rng;
       const double lu = log(xs_uniform(rng_u));
-      // hidden factors of this lane; cosh d, sinh d and w = cosh d + t sinh d
-      // of each unit are kept for the update of t on acceptance
+      // hidden factors w = cosh d + t sinh d of this lane's units, and each
+      // unit's tanh(theta + d) = (t cosh d + sinh d) / w should the proposal
+      // be accepted (computed with the factors, off the decision's path)
       R pr = 1, pi = 0;
-      R kchr[HPL], kchi[HPL], kshr[HPL], kshi[HPL], kwr[HPL], kwi[HPL];
-#pragma unroll
-      for (int q = 0; q < HPL; q++) {
-        kchr[q] = 1;
-        kchi[q] = kshr[q] = kshi[q] = kwi[q] = 0;
-        kwr[q] = 1;
-      }
+      R tnr[HPL], tni[HPL];
       // table rows of the flipped sites for the lane's units, in groups of
       // QB units whose loads are issued together (rows clamped into the
       // table; the pair form reads the second site's rows as well)
@@ -225,8 +220,6 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
                 shr = -si * (sir[u] * cjr[u] - cir[u] * sjr[u]);
               }
             }
-            kchr[q] = chr;
-            kshr[q] = shr;
             if (CPLX) {
               // w = cosh d + t sinh d
               const R wr = chr + (tr[q] * shr - ti[q] * shi);
@@ -234,14 +227,15 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
               const R nr = pr * wr - pi * wi;
               pi = pr * wi + pi * wr;
               pr = nr;
-              kchi[q] = chi;
-              kshi[q] = shi;
-              kwr[q] = wr;
-              kwi[q] = wi;
+              const R tr_n = (tr[q] * chr - ti[q] * chi) + shr;
+              const R ti_n = (tr[q] * chi + ti[q] * chr) + shi;
+              const R inv = (R)1 / (wr * wr + wi * wi);
+              tnr[q] = (tr_n * wr + ti_n * wi) * inv;
+              tni[q] = (ti_n * wr - tr_n * wi) * inv;
             } else {
               const R wr = chr + tr[q] * shr;
               pr *= wr;
-              kwr[q] = wr;
+              tnr[q] = (tr[q] * chr + shr) / wr;
             }
           }
         }
@@ -276,22 +270,14 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
         off = 0.0;
         sp[i >> 6] ^= 1ULL << (i & 63);
         if (j >= 0) sp[j >> 6] ^= 1ULL << (j & 63);
-        // tanh(theta + d) = (t cosh d + sinh d) / (cosh d + t sinh d): the
-        // denominator is the unit's factor w of the accepted proposal
-        // (w != 0: a zero factor gives d = -inf, never accepted)
+        // the accepted proposal's tanh(theta + d); its denominator w is not
+        // zero (a zero factor gives d = -inf, never accepted)
 #pragma unroll
         for (int q = 0; q < HPL; q++) {
           const int h = lane + L * q;
           if (h < M) {
-            if (CPLX) {
-              const R nr = (tr[q] * kchr[q] - ti[q] * kchi[q]) + kshr[q];
-              const R ni = (tr[q] * kchi[q] + ti[q] * kchr[q]) + kshi[q];
-              const R inv = (R)1 / (kwr[q] * kwr[q] + kwi[q] * kwi[q]);
-              tr[q] = (nr * kwr[q] + ni * kwi[q]) * inv;
-              ti[q] = (ni * kwr[q] - nr * kwi[q]) * inv;
-            } else {
-              tr[q] = (tr[q] * kchr[q] + kshr[q]) / kwr[q];
-            }
+            tr[q] = tnr[q];
+            if (CPLX) ti[q] = tni[q];
           }
         }
       }
