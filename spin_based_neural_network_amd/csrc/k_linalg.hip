@@ -1042,6 +1042,195 @@ __global__ __launch_bounds__(256) void k_panel2(double* A, int n, int k0, int nb
     }
 }
 
+// k_panel2 with R 64-row blocks per workgroup (workgroup 0 keeps the second
+// diagonal block's rows): fewer workgroups hold CUs for the diagonal work,
+// which every workgroup repeats, while a concurrent trailing update needs
+// them.  The own rows' second-panel columns (A - L_r1 L21^T) go through A
+// between the two factorisations.  Every product is the one k_panel2 forms
+// for the same rows, so the results are the same bit for bit.
+__global__ __launch_bounds__(256) void k_panel2_rows(double* A, int n, int k0, int nb2,
+                                                     double* Linv, double* L21w, int* fail,
+                                                     int R) {
+  const int failed = *fail;
+  __shared__ double D[NB][NB + 1];
+  __shared__ double X[NB][NB + 1];
+  __shared__ double Rs[NB][NB + 1];
+  __shared__ double L2[NB][NB + 1];
+  __shared__ double rcp[NB];
+  __shared__ double colw[4][NB];
+  __shared__ int s_fail;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int k1 = k0 + NB;
+  const int nblk = (n - k1 + NB - 1) / NB;
+  const bool diag_rows = blockIdx.x == 0;
+  const int b0 = diag_rows ? 0 : 1 + ((int)blockIdx.x - 1) * R;
+  const int b1 = diag_rows ? 1 : min(nblk, b0 + R);
+  // the own rows of block blk, first panel's columns, into Rs
+  auto load_rows = [&](int blk, int col0, int ncol) {
+    const long rowbase = (long)k1 + (long)blk * NB;
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+      const long row = rowbase + r;
+      Rs[r][c] = (row < n && c < ncol) ? A[row * n + col0 + c] : 0.0;
+    }
+  };
+  double d2v[NB * NB / 256];
+  {
+    const long rowbase = (long)k1 + (long)b0 * NB;
+    double dv[NB * NB / 256], rv[NB * NB / 256], av[NB * NB / 256];
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+      const long row = rowbase + r;
+      dv[q] = (c <= r) ? A[(long)(k0 + r) * n + k0 + c] : 0.0;
+      rv[q] = row < n ? A[row * n + k0 + c] : 0.0;
+      av[q] = r < nb2 ? A[(long)(k1 + r) * n + k0 + c] : 0.0;
+      d2v[q] = (r < nb2) ? ((c <= r) ? A[(long)(k1 + r) * n + k1 + c] : 0.0) : (r == c ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+      D[r][c] = dv[q];
+      Rs[r][c] = rv[q];
+      L2[r][c] = av[q];
+    }
+  }
+  __syncthreads();
+  if (failed) return;
+  // ---- first panel ----
+  int jdone = panel_factor(D, colw, &s_fail, NB, wave, lane);
+  if (jdone < NB) {
+    if (diag_rows) {
+      double* Lg = Linv + ((size_t)((n + NB - 1) / NB) + k0 / NB) * NB * NB;
+      for (int e = tid; e < NB * NB; e += 256) Lg[e] = D[e >> 6][e & 63];
+      if (tid == 0) *fail = k0 + jdone + 1;
+    }
+    return;
+  }
+  panel_inverse(D, X, rcp, wave, lane, tid);
+  if (diag_rows) panel_publish(D, X, n, k0, NB, Linv, tid);
+  {
+    v4d al[4];
+    panel_times_xt(L2, X, al, wave, lr, lk);   // L21
+    for (int blk = b0; blk < b1; blk++) {
+      const long rowbase = (long)k1 + (long)blk * NB;
+      if (blk > b0) {
+        __syncthreads();   // the previous block's Rs has been read
+        load_rows(blk, k0, NB);
+        __syncthreads();
+      }
+      v4d ar[4];
+      panel_times_xt(Rs, X, ar, wave, lr, lk);   // own rows' L (first panel)
+      __syncthreads();                           // Rs (and L2) have been read
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = 16 * wave + lk + 4 * r, c = 16 * bj + lr;
+          const long row = rowbase + i;
+          Rs[i][c] = ar[bj][r];
+          if (blk == b0) L2[i][c] = al[bj][r];
+          if (diag_rows) {
+            L21w[(size_t)(k0 / (2 * NB)) * NB * NB + i * NB + c] = al[bj][r];
+          } else if (row < n) {
+            A[row * n + k0 + c] = ar[bj][r];
+          }
+        }
+      if (diag_rows) continue;
+      // own rows' second-panel columns: A_rows - L_r1 L21^T, back into A
+      v4d accR[4];
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const long row = rowbase + 16 * wave + lk + 4 * r;
+          const int cc = 16 * bj + lr;
+          accR[bj][r] = (row < n && cc < nb2) ? A[row * n + k1 + cc] : 0.0;
+        }
+      __syncthreads();   // Rs and L2 are in place
+      const long arow = rowbase + 16 * wave + lr;
+#pragma unroll
+      for (int kk = 0; kk < NB; kk += 4) {
+        const double a_r = arow < n ? -Rs[16 * wave + lr][kk + lk] : 0.0;
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++)
+          accR[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r, L2[16 * bj + lr][kk + lk], accR[bj], 0,
+                                                          0, 0);
+      }
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const long row = rowbase + 16 * wave + lk + 4 * r;
+          const int cc = 16 * bj + lr;
+          if (row < n && cc < nb2) A[row * n + k1 + cc] = accR[bj][r];
+        }
+    }
+  }
+  // ---- second panel: D2 = A22 - L21 L21^T ----
+  {
+    __syncthreads();   // L2 is in place, the last Rs has been read
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; q++) {
+      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+      D[r][c] = d2v[q];
+    }
+    v4d accD[4];
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++) accD[bj] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < NB; kk += 4) {
+      const double a_d = -L2[16 * wave + lr][kk + lk];
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++)
+        accD[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_d, L2[16 * bj + lr][kk + lk], accD[bj], 0, 0,
+                                                        0);
+    }
+    __syncthreads();   // D is in place
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * wave + lk + 4 * r, c = 16 * bj + lr;
+        if (c <= i && i < nb2) D[i][c] += accD[bj][r];
+      }
+  }
+  __syncthreads();
+  jdone = panel_factor(D, colw, &s_fail, nb2, wave, lane);
+  if (jdone < nb2) {
+    if (diag_rows) {
+      double* Lg = Linv + ((size_t)((n + NB - 1) / NB) + k1 / NB) * NB * NB;
+      for (int e = tid; e < NB * NB; e += 256) Lg[e] = D[e >> 6][e & 63];
+      if (tid == 0) *fail = k1 + jdone + 1;
+    }
+    // the updated, unsolved rows are already in A
+    return;
+  }
+  panel_inverse(D, X, rcp, wave, lane, tid);
+  if (diag_rows) {
+    panel_publish(D, X, n, k1, nb2, Linv, tid);
+    return;
+  }
+  for (int blk = b0; blk < b1; blk++) {
+    const long rowbase = (long)k1 + (long)blk * NB;
+    __syncthreads();   // the previous reads of Rs are done
+    load_rows(blk, k1, nb2);
+    __syncthreads();
+    v4d acc[4];
+    panel_times_xt(Rs, X, acc, wave, lr, lk);
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const long row = rowbase + 16 * wave + lk + 4 * r;
+        const int cc = 16 * bj + lr;
+        if (row < n && cc < nb2) A[row * n + k1 + cc] = acc[bj][r];
+      }
+  }
+}
+
 // Diagonal blocks' L from the workspace into A (lower triangles): every
 // block before a failure, and the factored columns of the failing block.
 // With merged outer panels (k_panel2) also the block L21 below the first
@@ -1144,14 +1333,22 @@ static int panel_merge() {
   return !(e && e[0] == '0');
 }
 
-// outer panel at K0: its two inner panels
-static int outer_panel(double* A, int n, int K0, double* Linv, int* fail_dev, hipStream_t st) {
+// outer panel at K0: its two inner panels (rows: 64-row blocks per
+// workgroup of the merged launch)
+static int outer_panel(double* A, int n, int K0, double* Linv, int* fail_dev, hipStream_t st,
+                       int rows = 1) {
   if (n - K0 > NB && panel_merge()) {
     const int nb2 = (n - K0 - NB) < NB ? (n - K0 - NB) : NB;
-    const int grid = (n - K0 - NB + NB - 1) / NB;
+    const int nblk = (n - K0 - NB + NB - 1) / NB;
     KScope ks("potrf_panel", st);
-    hipLaunchKernelGGL(k_panel2, dim3((unsigned)grid), dim3(256), 0, st, A, n, K0, nb2, Linv,
-                       Linv + l21_offset(n), fail_dev);
+    if (rows > 1 && nblk > 1) {
+      const int grid = 1 + (nblk - 1 + rows - 1) / rows;
+      hipLaunchKernelGGL(k_panel2_rows, dim3((unsigned)grid), dim3(256), 0, st, A, n, K0, nb2, Linv,
+                         Linv + l21_offset(n), fail_dev, rows);
+    } else {
+      hipLaunchKernelGGL(k_panel2, dim3((unsigned)nblk), dim3(256), 0, st, A, n, K0, nb2, Linv,
+                         Linv + l21_offset(n), fail_dev);
+    }
     count_launch();
     NQS_HIP_CHECK(hipGetLastError());
     return 0;
@@ -1165,10 +1362,10 @@ static int outer_panel(double* A, int n, int K0, double* Linv, int* fail_dev, hi
 // 128-column outer panel is factored, then the wide panel's columns right
 // of it take its rank-128 update
 static int wide_panel(double* A, int n, int K0, int width, double* Linv, int* fail_dev,
-                      hipStream_t st) {
+                      hipStream_t st, int rows = 1) {
   const int end = (K0 + width) < n ? K0 + width : n;
   for (int j = K0; j < end; j += NBO) {
-    if (outer_panel(A, n, j, Linv, fail_dev, st)) return -1;
+    if (outer_panel(A, n, j, Linv, fail_dev, st, rows)) return -1;
     const int c0 = j + NBO;
     if (c0 < end) {
       const double* L = A + (long)c0 * n + j;
@@ -1178,6 +1375,18 @@ static int wide_panel(double* A, int n, int K0, int width, double* Linv, int* fa
     }
   }
   return 0;
+}
+
+// read at each wide panel (as NQS_HIP_PANEL_MERGE), so that a test can
+// compare the schedules in one process
+static int panel_rows() {
+  const char* e = getenv("NQS_HIP_PANEL_ROWS");
+  const int v = e ? atoi(e) : 4;
+  return v < 1 ? 1 : (v > 16 ? 16 : v);
+}
+static int panel_rows_all() {
+  const char* e = getenv("NQS_HIP_PANEL_ROWS_ALL");
+  return (e && e[0] == '1') ? 1 : 0;
 }
 
 // Width of the wide panel that starts with `rem` columns left to factor:
@@ -1211,7 +1420,7 @@ int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) 
   count_launch();
   int K0 = 0, w = panel_width(n);
   if (w > n) w = n;
-  if (wide_panel(A, n, 0, w, Linv, fail_dev, st)) return -1;
+  if (wide_panel(A, n, 0, w, Linv, fail_dev, st, panel_rows_all() ? panel_rows() : 1)) return -1;
   // NQS_HIP_POTRF_LOOKAHEAD=0: the same schedule on one stream
   static const int lookahead = [] {
     const char* e = getenv("NQS_HIP_POTRF_LOOKAHEAD");
@@ -1261,7 +1470,13 @@ int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) 
                       fail_dev, "gemm_potrf", side))
       return -1;
     NQS_HIP_CHECK(hipEventRecord(ev_panel, side));
-    if (wide_panel(A, n, R0, w1, Linv, fail_dev, side)) return -1;
+    // while the rest update is the critical path (the same test as the
+    // early rest update) the panels run on fewer, fuller workgroups
+    // (NQS_HIP_PANEL_ROWS blocks of 64 rows each, default 4) and leave CUs
+    // to it; NQS_HIP_PANEL_ROWS_ALL=1 uses them for every panel
+    const int R1p = R0 + w1;
+    const int rows = (panel_rows_all() || (lookahead && n - R1p > early)) ? panel_rows() : 1;
+    if (wide_panel(A, n, R0, w1, Linv, fail_dev, side, rows)) return -1;
     NQS_HIP_CHECK(hipEventRecord(ev_wp1, side));
     // the rest of the trailing matrix: rows and columns >= R0 + w1
     const int R1 = R0 + w1;

@@ -381,6 +381,33 @@ def test_merged_panels_bit_equal(hip, monkeypatch):
             np.testing.assert_allclose(A @ out["1"][2], b, atol=1e-8)
 
 
+def test_panel_row_blocks_bit_equal(hip, monkeypatch):
+    """Merged panels with several 64-row blocks per workgroup (fewer
+    workgroups, second-panel rows through memory) give the factor and the
+    solution of one block per workgroup bit for bit, including a failing
+    factorisation and ragged block counts."""
+    rng = np.random.default_rng(22)
+    cases = [(700, None), (1030, None), (193, None), (600, 450)]
+    monkeypatch.setenv("NQS_HIP_PANEL_MERGE", "1")
+    for B, rank in cases:
+        X = rng.standard_normal((B, rank if rank else B + 37))
+        A = np.ascontiguousarray(X @ X.T / X.shape[1] + (0.0 if rank else 0.05) * np.eye(B))
+        b = np.ascontiguousarray(rng.standard_normal(B))
+        out = {}
+        for rows, every in (("1", "0"), ("3", "1"), ("2", "1")):
+            monkeypatch.setenv("NQS_HIP_PANEL_ROWS", rows)
+            monkeypatch.setenv("NQS_HIP_PANEL_ROWS_ALL", every)
+            Aw = A.copy()
+            x = np.zeros(B)
+            ok = hip.nqs_hip_cholesky_solve(ptr(Aw), B, ptr(b), ptr(x))
+            out[rows] = (ok, Aw, x)
+        print(f"B={B} rank={rank}: ok={out['1'][0]}", flush=True)
+        for rows in ("3", "2"):
+            assert out["1"][0] == out[rows][0]
+            np.testing.assert_array_equal(out["1"][1], out[rows][1])
+            np.testing.assert_array_equal(out["1"][2], out[rows][2])
+
+
 def test_large_shapes_consistent_with_oracle(hip, orc):
     """Configuration shapes of the benchmark (6x6 periodic Heisenberg + Marshall,
     complex RBM alpha=4; kagome 2x4 periodic, complex RBM M=96; J1-J2 10x10
