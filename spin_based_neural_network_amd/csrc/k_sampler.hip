@@ -355,6 +355,13 @@ int launch_mh(const ModelView<R>& m, ChainState& cs, const MHLaunch& ml, hipStre
   KScope ks("mh", st);
   if (m.M <= 64) {
     if (c) mh_go<R, true, 16, 4>(m, cs, ml, st); else mh_go<R, false, 16, 4>(m, cs, ml, st);
+  } else if (m.M <= 128) {
+    // one chain per wave with as few idle unit slots per lane as the
+    // register budget allows (the chain's latency grows with the units a
+    // lane owns)
+    if (c) mh_go<R, true, 64, 2>(m, cs, ml, st); else mh_go<R, false, 64, 2>(m, cs, ml, st);
+  } else if (m.M <= 192) {
+    if (c) mh_go<R, true, 64, 3>(m, cs, ml, st); else mh_go<R, false, 64, 3>(m, cs, ml, st);
   } else if (m.M <= 256) {
     if (c) mh_go<R, true, 64, 4>(m, cs, ml, st); else mh_go<R, false, 64, 4>(m, cs, ml, st);
   } else if (m.M <= 832) {
