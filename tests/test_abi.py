@@ -5,7 +5,9 @@
 * the struct layouts the library was compiled with match the ctypes mirror
   (and therefore the reference's include/nqs/nqs_config.h:82-120 layout);
 * the option defaults agree on both sides;
-* the reference's header set is covered by ours for the path's functions.
+* the reference's header set is covered by ours for the path's functions;
+* a C caller written against the reference API compiles and links against
+  the library alone, and fails loudly without a device.
 """
 import ctypes
 import os
@@ -130,3 +132,57 @@ def test_layout_matches_reference_headers(tmp_path):
         subprocess.run(["gcc", "-std=c11", "-I", inc, str(c), "-o", str(exe)], check=True)
         outs.append(subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout)
     assert outs[0] == outs[1]
+
+
+CALLER = r"""
+#include <stdio.h>
+#include "nqs/nqs_config.h"
+#include "nqs/nqs_ansatz.h"
+#include "nqs/nqs_sampler.h"
+#include "nqs/nqs_optimizer.h"
+#include "nqs_hip/nqs_hip.h"
+
+int main(void) {
+  nqs_config_t cfg = nqs_config_defaults();
+  cfg.ansatz = NQS_ANSATZ_COMPLEX_RBM; cfg.rbm_hidden_units = 96;
+  cfg.hamiltonian = NQS_HAM_KAGOME_HEISENBERG; cfg.kagome_pbc = 1;
+  cfg.num_samples = 2048 * 8; cfg.num_thermalize = 96; cfg.num_decorrelate = 24;
+  nqs_ansatz_t *a = nqs_ansatz_create(&cfg, 24);
+  if (!a) { printf("no-device\n"); return 3; }
+  nqs_hip_options_t opt = nqs_hip_options_defaults();
+  opt.num_chains = 2048;
+  opt.move = NQS_HIP_MOVE_LEARNED;
+  opt.sz_zero_start = 1;
+  opt.gram = NQS_HIP_GRAM_FACTORED;
+  nqs_sampler_t *s = nqs_hip_sampler_create_ex(24, &cfg, &opt, nqs_ansatz_log_amp, a);
+  nqs_sampler_t *plain = nqs_sampler_create(24, &cfg, nqs_ansatz_log_amp, a);
+  double trace[2];
+  cfg.num_iterations = 2;
+  int rc = nqs_sr_run_minsr(&cfg, 2, 4, a, s, NULL, NULL, trace);
+  printf("rc=%d E=%.6f\n", rc, trace[1]);
+  nqs_sampler_free(plain);
+  nqs_sampler_free(s);
+  nqs_ansatz_free(a);
+  return rc;
+}
+"""
+
+
+def test_reference_style_caller_compiles_and_links(lib, tmp_path, capfd):
+    """A C caller written against the reference API (the C example of the
+    integration notes, in the style of the reference's tests/test_nqs_minsr.c:118)
+    compiles with gcc against include/ and links against libnqs_hip.so alone.
+    Without a device it runs and fails loudly at the first handle."""
+    src = tmp_path / "caller.c"
+    src.write_text(CALLER)
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(abi.LIB_PATH)
+    r = subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", INC, str(src), "-o", str(exe),
+                        "-L", libdir, "-lnqs_hip", "-Wl,-rpath," + libdir, "-lm"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present: compile and link checked only")
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 3 and out.stdout.strip() == "no-device", (out.stdout, out.stderr)
+    assert "no HIP device" in out.stderr
