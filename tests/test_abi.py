@@ -10,6 +10,7 @@
   the library alone, and fails loudly without a device.
 """
 import ctypes
+import math
 import os
 import re
 import subprocess
@@ -168,11 +169,7 @@ int main(void) {
 """
 
 
-def test_reference_style_caller_compiles_and_links(lib, tmp_path, capfd):
-    """A C caller written against the reference API (the C example of the
-    integration notes, in the style of the reference's tests/test_nqs_minsr.c:118)
-    compiles with gcc against include/ and links against libnqs_hip.so alone.
-    Without a device it runs and fails loudly at the first handle."""
+def _build_caller(tmp_path):
     src = tmp_path / "caller.c"
     src.write_text(CALLER)
     exe = tmp_path / "caller"
@@ -181,8 +178,31 @@ def test_reference_style_caller_compiles_and_links(lib, tmp_path, capfd):
                         "-L", libdir, "-lnqs_hip", "-Wl,-rpath," + libdir, "-lm"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    return exe
+
+
+def test_reference_style_caller_compiles_and_links(lib, tmp_path, capfd):
+    """A C caller written against the reference API (the C example of the
+    integration notes, in the style of the reference's tests/test_nqs_minsr.c:118)
+    compiles with gcc against include/ and links against libnqs_hip.so alone.
+    Without a device it runs and fails loudly at the first handle."""
+    exe = _build_caller(tmp_path)
     if os.path.exists("/dev/kfd"):
         pytest.skip("a GPU is present: compile and link checked only")
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 3 and out.stdout.strip() == "no-device", (out.stdout, out.stderr)
     assert "no HIP device" in out.stderr
+
+
+@pytest.mark.gpu
+def test_reference_style_caller_runs(lib, tmp_path):
+    """The same C caller on a GPU: two MinSR iterations of the 24-site kagome
+    complex RBM with learned moves and the factored Gram, in a process of its
+    own that loads nothing but libnqs_hip.so."""
+    exe = _build_caller(tmp_path)
+    print("[caller] kagome 2x4 PBC, cRBM M=96, 16384 samples, 2 MinSR iterations", flush=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=100)
+    assert out.returncode == 0, (out.stdout, out.stderr)
+    m = re.match(r"rc=0 E=(\S+)", out.stdout.strip())
+    assert m, out.stdout
+    assert math.isfinite(float(m.group(1))), out.stdout
