@@ -513,6 +513,13 @@ NQS_EXPORT nqs_ansatz_t* nqs_ansatz_create(const nqs_config_t* cfg, int num_site
   const int kind = cfg->ansatz;
   if (kind != NQS_ANSATZ_LEGACY_MLP && kind != NQS_ANSATZ_RBM && kind != NQS_ANSATZ_COMPLEX_RBM)
     return nullptr;
+  if (num_sites > nqs::kMaxSites) {
+    // every device kernel keeps a configuration as at most 4 64-bit words;
+    // refusing here keeps log_amp / local_energy from returning zeros later
+    fprintf(stderr, "nqs_hip: nqs_ansatz_create: this library supports N <= %d sites (got %d)\n",
+            nqs::kMaxSites, num_sites);
+    return nullptr;
+  }
   if (stream_ready()) return nullptr;
   nqs_ansatz* a = new nqs_ansatz();
   static long long next_id = 0;

@@ -585,7 +585,11 @@ int launch_eloc(const ModelView<R>& m, const HamDev& h, const int* spins, int B,
                 void* ws, size_t ws_bytes) {
   if (B <= 0) return 0;
   const int T = h.n_flip + h.n_pair;
-  if (m.N > kMaxSites || T >= 65536) return -1;
+  if (m.N > kMaxSites || T >= 65536) {
+    fprintf(stderr, "nqs_hip: local energies support N <= %d sites and < 65536 off-diagonal "
+                    "terms (got N = %d, %d terms)\n", kMaxSites, m.N, T);
+    return -1;
+  }
   const bool c = m.cplx != 0;
   if (ws && ws_bytes >= eloc_ws_bytes(m, h, B))
     return c ? eloc_terms_go<R, true>(m, h, spins, B, t_re, t_im, e_re, e_im, ws, st)

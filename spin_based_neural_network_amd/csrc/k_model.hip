@@ -185,7 +185,10 @@ template <typename R>
 int launch_forward(const ModelView<R>& m, const int* spins, int B, R* t_re, R* t_im,
                    double* la, double* ar, hipStream_t st) {
   if (B <= 0) return 0;
-  if (m.N > kMaxSites) return -1;
+  if (m.N > kMaxSites) {
+    fprintf(stderr, "nqs_hip: amplitudes support N <= %d sites (got %d)\n", kMaxSites, m.N);
+    return -1;
+  }
   const int grid = (B + 3) / 4 < 65535 ? (B + 3) / 4 : 65535;
   KScope ks("forward", st);
   if (m.cplx)

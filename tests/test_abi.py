@@ -109,6 +109,16 @@ def test_no_gpu_fails_loudly(lib, capfd):
     assert "no HIP device" in capfd.readouterr().err
 
 
+def test_oversized_lattice_refused(lib, capfd):
+    """N above the device kernels' limit (256 sites) is refused at creation
+    with a message, instead of amplitudes and energies of zero later; the
+    refusal comes before any device call, so it holds with or without a GPU."""
+    for kind in (5, 6):
+        cfg = abi.config_defaults(ansatz=kind, rbm_hidden_units=4)
+        assert not lib.nqs_ansatz_create(ctypes.byref(cfg), 257)
+        assert "N <= 256" in capfd.readouterr().err
+
+
 REF_INC = "/root/reference/include"
 
 

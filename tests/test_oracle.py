@@ -354,3 +354,118 @@ def test_tau_int_ar1(orc):
         tau = orc.orc_tau_int(ptr(x), n)
         exact = (1 + rho) / (2 * (1 - rho))
         assert abs(tau - exact) / exact < 0.08
+
+
+# ---- periodic square lattices: anchors outside the oracle's own ED ----------
+def test_ed_periodic_rings_known_answers(orc):
+    """Heisenberg rings through the periodic square-lattice bond set (Ly = 1):
+    the 4-site ring E0 = -2 J and the 6-site ring E0 = -2.802775637732 J held
+    by the reference's own tests (tests/test_nqs_kspace_ed.c:38-61)."""
+    heis = config_defaults(hamiltonian=1)
+    assert abs(_ed(orc, heis, 4, 1, per=1) - (-2.0)) < 1e-9
+    assert abs(_ed(orc, heis, 6, 1, per=1) - (-2.802775637732)) < 1e-8
+    # XXZ with Jxy = Jz is the same Hamiltonian
+    xxz = config_defaults(hamiltonian=4, j_coupling=1.0, j_z_coupling=1.0)
+    assert abs(_ed(orc, xxz, 6, 1, per=1) - (-2.802775637732)) < 1e-8
+
+
+def test_ed_periodic_4x4_heisenberg_literature(orc):
+    """4x4 periodic Heisenberg square lattice: E0/N = -0.7017802 (the
+    standard exact-diagonalisation value of the 16-site torus), which also
+    fixes that both wrap bonds of every row and column are present (a
+    missing wrap bond raises E0 by O(0.1))."""
+    e = _ed(orc, config_defaults(hamiltonian=1), 4, 4, per=1, n_down=8)
+    assert abs(e / 16 - (-0.7017802)) < 2e-7
+
+
+def _translation_invariant_rbm(orc, cfg, Lx, Ly, seed):
+    """An RBM whose hidden unit (tx, ty) carries the filter w translated by
+    (tx, ty) on the torus, with equal hidden biases and equal visible
+    biases: psi(T s) = psi(s) for every lattice translation T."""
+    N = Lx * Ly
+    a = orc.orc_ansatz_create(ctypes.byref(cfg), N)
+    P = orc.orc_ansatz_num_params(a)
+    p = np.ctypeslib.as_array(ctypes.cast(orc.orc_ansatz_params_raw(a), ctypes.POINTER(ctypes.c_double)), (P,))
+    rng = np.random.default_rng(seed)
+    filt = rng.normal(0.0, 0.4, N)
+    M = N
+    p[:N] = 0.13
+    p[N:N + M] = -0.07
+    W = np.zeros((M, N))
+    for tx in range(Lx):
+        for ty in range(Ly):
+            h = tx * Ly + ty
+            for x in range(Lx):
+                for y in range(Ly):
+                    W[h, ((x + tx) % Lx) * Ly + (y + ty) % Ly] = filt[x * Ly + y]
+    p[N + M:] = W.ravel()
+    return a
+
+
+@pytest.mark.parametrize("ham,Lx,Ly", [(1, 4, 4), (2, 4, 4), (2, 3, 4), (0, 5, 3), (4, 4, 3)])
+def test_periodic_local_energy_translation_invariant(orc, ham, Lx, Ly):
+    """With a translation-invariant amplitude, E_loc(T s) = E_loc(s) for every
+    translation T of a periodic lattice; a missing or misplaced wrap bond
+    breaks it (the open twin fails this test)."""
+    N = Lx * Ly
+    cfg = config_defaults(ansatz=5, rbm_hidden_units=N, hamiltonian=ham, j2_coupling=0.5,
+                          transverse_field=0.9, j_z_coupling=0.6)
+    a = _translation_invariant_rbm(orc, cfg, Lx, Ly, seed=ham * 10 + Lx)
+    fn = fnptr(orc, "orc_ansatz_log_amp")
+    rng = np.random.default_rng(4)
+    base = rng.choice([-1, 1], size=(6, Lx, Ly)).astype(np.int32)
+    rows, idx = [], []
+    for k in range(6):
+        for tx in range(Lx):
+            for ty in range(Ly):
+                rows.append(np.roll(base[k], (tx, ty), axis=(0, 1)).ravel())
+                idx.append(k)
+    spins = np.ascontiguousarray(np.array(rows, dtype=np.int32))
+    re, im = np.zeros(len(rows)), np.zeros(len(rows))
+    for per, must_hold in ((1, True), (0, False)):
+        orc.orc_local_energy_batch_ex(ctypes.byref(cfg), Lx, Ly, per, ptr(spins), len(rows), fn, a,
+                                      ptr(re), ptr(im))
+        spread = max(np.ptp(re[np.array(idx) == k]) for k in range(6))
+        if must_hold:
+            assert spread < 1e-11 * max(1.0, np.max(np.abs(re)))
+        else:
+            assert spread > 1e-3
+    orc.orc_ansatz_free(a)
+
+
+def test_j1j2_4x4_periodic_mean_local_energy_equals_exact(orc):
+    """<E_loc> over |psi|^2 equals <psi|H|psi> by enumeration for the J1-J2
+    model on the 4x4 torus (J2/J1 = 0.5, complex RBM), every configuration."""
+    Lx = Ly = 4
+    N = 16
+    cfg = config_defaults(ansatz=6, hamiltonian=2, rbm_hidden_units=5, rbm_init_scale=0.5,
+                          j2_coupling=0.5, rng_seed=11)
+    a = orc.orc_ansatz_create(ctypes.byref(cfg), N)
+    fn = fnptr(orc, "orc_ansatz_log_amp")
+    dim = 1 << N
+    st = np.arange(dim)
+    spins = np.ascontiguousarray(np.where((st[:, None] >> np.arange(N)) & 1, -1, 1).astype(np.int32))
+    re, im = np.zeros(dim), np.zeros(dim)
+    orc.orc_local_energy_batch_ex(ctypes.byref(cfg), Lx, Ly, 1, ptr(spins), dim, fn, a, ptr(re), ptr(im))
+    la, ar = np.zeros(dim), np.zeros(dim)
+    for b in range(dim):
+        orc.orc_ansatz_log_amp(ptr(spins[b]), N, a, ptr(la[b:]), ptr(ar[b:]))
+    w = np.exp(2 * (la - la.max()))
+    w /= w.sum()
+    exact = orc.orc_exact_energy(ctypes.byref(cfg), Lx, Ly, 1, fn, a)
+    assert abs(np.sum(w * re) - exact) < 1e-10 * max(1.0, abs(exact))
+    assert abs(np.sum(w * im)) < 1e-10
+    orc.orc_ansatz_free(a)
+
+
+def test_oracle_under_address_and_ub_sanitizers():
+    """Every oracle entry point, driven once by oracle/sanity_main.c in a
+    build with AddressSanitizer and UndefinedBehaviorSanitizer (leak
+    detection on, any report aborts)."""
+    import subprocess
+    import os
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    r = subprocess.run(["make", "-s", "-C", here, "sanitize"], capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "oracle sanity ok" in r.stdout
