@@ -1,28 +1,30 @@
 """Benchmark of the NQS VMC hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
-                    [--precision fp64|fp32] [--gram direct|factored] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5]
+                    [--budget K] [--precision fp64|fp32] [--gram direct|factored|auto]
+                    [--move flip|exchange|learned] [--no-cpu] [--no-c3]
 
 A step is one full SR iteration (Metropolis sampling of every chain, local
 energies, O_k, MinSR / CG-SR solve, parameter update) through the library's
-C ABI.  Default workload (the 6x6 Heisenberg configuration of the metric):
-periodic 6x6 Heisenberg AFM with the Marshall sign, Sz = 0, complex RBM
-alpha = 4 (M = 144, Re/Im ~ N(0, 0.01)), 1024 chains x 8 samples = 8192
-samples per iteration, exchange moves with one sweep (N proposals) between
-samples, MinSR with shift 1e-4.  Prints one JSON line.
+C ABI.  Default workload: the metric's configuration, kagome Heisenberg on
+the 2x4-cell periodic cluster (N = 24, Sz = 0), complex RBM alpha = 4
+(M = 96), 2048 chains x 8 = 16384 samples per iteration, MinSR, run twice:
+with plain exchange Metropolis and with the learned autoregressive proposal
+(Adam lr 1e-3, retrained every 10 iterations on the last 16384 samples).
+Each leg first optimises its own wavefunction for the workload's budget of
+iterations (C5: 300 holomorphic SR iterations, untimed) and reports the
+energy error against exact diagonalisation (E0 = -10.759897248084), then
+times K MinSR iterations at that state.  The line also carries the J1-J2
+10x10 SR-iteration time against the reference CPU (configuration 3).
+Prints one JSON line.
 
-value = effective samples per second of the sampling phase: sum over chains
-of n_c / (2 tau_int) with tau_int from the pooled per-chain autocorrelation
-of E_loc (Sokal window, c = 5), divided by the sampling-stage time of the
-timed iterations (HIP events on the library's stream).  ess_per_s_iteration
-divides by the whole SR-iteration wall time instead; ms_per_step is the
-SR-iteration wall time.
-
-    --move exchange|learned|flip   override the workload's move kind
-    --converge K                   after the timed steps, run K more SR
-                                   iterations and report the energy trace's
-                                   tail (last 50) and its error vs exact
-                                   diagonalisation where it is known
+value = effective samples per second of the sampling phase (the learned
+leg for C5): sum over chains of n_c / (2 tau_int) with tau_int from the
+pooled per-chain autocorrelation of E_loc (Sokal window, c = 5; 95%
+interval from a jackknife over chain groups), divided by the sampling-stage
+time of the timed iterations (HIP events on the library's stream).
+ess_per_s_iteration divides by the whole SR-iteration wall time instead;
+ms_per_step is the SR-iteration wall time.
 """
 import argparse
 import ctypes
@@ -38,11 +40,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from spin_based_neural_network_amd import abi  # noqa: E402
-from spin_based_neural_network_amd.stats import pooled_tau  # noqa: E402
+from spin_based_neural_network_amd.stats import pooled_tau, tau_ci  # noqa: E402
 
 METRIC = "effective samples/s & SR-iter wall time, learned vs plain MH, same ΔE vs ED"
 PEAK_FP64 = 78.6e12          # MI355X dense FP64 (vector / MFMA), FLOP/s
 PEAK_HBM = 8.0e12            # MI355X HBM3E, B/s
+LEARNED_LR = 1e-3            # Adam rate of the learned proposal (configuration 5)
 
 WORKLOADS = {
     # 6x6 periodic Heisenberg + Marshall, complex RBM alpha=4, MinSR
@@ -63,11 +66,14 @@ WORKLOADS = {
                ham=2, Lx=10, Ly=10, kind=5, M=800, chains=4096, per_chain=4, move=1, sz0=1,
                periodic=1, marshall=False, method="minsr", shift=1e-4, lr=1e-2, ed=None),
     # kagome 2x4 PBC (N=24), complex RBM M=96, MinSR
-    "c5": dict(desc="kagome Heisenberg 2x4 cells PBC (N=24), Sz=0; complex RBM M=96; 2048 "
-                    "chains x 8 = 16384/iter; exchange moves; MinSR",
+    "c5": dict(desc="kagome Heisenberg 2x4 cells PBC (N=24), J=1, Sz=0; complex RBM M=96 "
+                    "(P=4848 real); 2048 chains x 8 = 16384/iter; MinSR shift 1e-4 lr 0.01; "
+                    "learned AR-MLP proposal 24-64-64-24 (BF16, Adam lr 1e-3, retrained every "
+                    "10 iterations on the last 16384 samples, MH-corrected) against plain "
+                    "exchange Metropolis (1 sweep between samples)",
                ham=6, Lx=2, Ly=4, kind=6, M=96, chains=2048, per_chain=8, move=1, sz0=1,
                periodic=1, marshall=False, method="minsr", shift=1e-4, lr=1e-2,
-               ed=-10.759897248084),
+               ed=-10.759897248084, opt_method="holo", opt_lr=2e-2, opt_shift=1e-3, budget=300),
     # the same system optimised as the reference's kagome runs do (holomorphic
     # SR, lr 0.02, shift 1e-3: scripts/research_kagome_N12_convergence.c:68-69);
     # the real-projected MinSR of c5 cannot move the phases of psi
@@ -90,7 +96,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def setup(lib, w, args):
+def setup(lib, w, args, move=None):
+    """Ansatz (random init N(0, 0.01)) and sampler of the workload; `move`
+    overrides the workload's move kind ("flip", "exchange", "learned")."""
     N = 3 * w["Lx"] * w["Ly"] if w["ham"] == 6 else w["Lx"] * w["Ly"]
     cfg = abi.config_defaults(ansatz=w["kind"], hamiltonian=w["ham"], rbm_hidden_units=w["M"],
                               j_coupling=1.0, j2_coupling=0.5, transverse_field=1.0, kagome_pbc=1,
@@ -108,13 +116,19 @@ def setup(lib, w, args):
     if w["marshall"]:
         keep = abi.MarshallWrapper(fn, a, w["Lx"], w["Ly"])
         fn, user = abi.fnptr(lib, "nqs_marshall_log_amp"), ctypes.addressof(keep)
-    move = {"flip": abi.MOVE_FLIP, "exchange": abi.MOVE_EXCHANGE,
-            "learned": abi.MOVE_LEARNED}.get(args.move, w["move"])
-    opt = abi.options_defaults(num_chains=w["chains"], move=move, sz_zero_start=w["sz0"],
+    mv = {"flip": abi.MOVE_FLIP, "exchange": abi.MOVE_EXCHANGE,
+          "learned": abi.MOVE_LEARNED}.get(move or args.move, w["move"])
+    opt = abi.options_defaults(num_chains=w["chains"], move=mv, sz_zero_start=w["sz0"],
                                periodic=w["periodic"],
                                precision=abi.FP32 if args.precision == "fp32" else abi.FP64,
                                gram={"direct": abi.GRAM_DIRECT, "factored": abi.GRAM_FACTORED,
                                      "auto": abi.GRAM_AUTO}[args.gram])
+    if mv == abi.MOVE_LEARNED:
+        # the learned proposal as the configuration states it: Adam at 1e-3,
+        # retrained every 10 iterations on the last 16384 chain samples
+        opt.ar_lr = LEARNED_LR
+        opt.ar_train_every = 10
+        opt.ar_train_window = 16384
     s = lib.nqs_hip_sampler_create_ex(N, ctypes.byref(cfg), ctypes.byref(opt), fn, user)
     if not s:
         raise RuntimeError("nqs_hip_sampler_create_ex failed")
@@ -124,18 +138,52 @@ def setup(lib, w, args):
     gram_b = B * B * (N + 2 * w["M"]) if factored else B * B * P
     dual = args.gram == "auto" and B * P * P + P ** 3 / 3 < gram_b + B ** 3 / 3
     return dict(cfg=cfg, a=a, s=s, fn=fn, user=user, keep=keep, N=N, P=P, opt=opt,
-                factored=factored and not dual, dual=dual)
+                factored=factored and not dual, dual=dual, move=mv)
 
 
-def step(lib, w, st):
+def step(lib, w, st, method=None, cfg=None):
     info = abi.SrInfo()
     fname = {"minsr": "nqs_sr_step_minsr_full", "cg": "nqs_sr_step_custom_full",
-             "holo": "nqs_sr_step_holomorphic_full"}[w["method"]]
-    rc = getattr(lib, fname)(ctypes.byref(st["cfg"]), w["Lx"], w["Ly"], st["a"], st["s"], st["fn"],
-                             st["user"], None, None, ctypes.byref(info))
+             "holo": "nqs_sr_step_holomorphic_full"}[method or w["method"]]
+    rc = getattr(lib, fname)(ctypes.byref(cfg or st["cfg"]), w["Lx"], w["Ly"], st["a"], st["s"],
+                             st["fn"], st["user"], None, None, ctypes.byref(info))
     if rc != 0:
         raise RuntimeError(f"{fname} returned {rc}")
     return info
+
+
+def optimise(lib, w, st, budget):
+    """`budget` iterations of the workload's energy optimiser from the
+    current state (for C5 the reference's kagome recipe, holomorphic SR with
+    lr 0.02 and shift 1e-3: the real-projected MinSR of the timed step cannot
+    move the phases a non-stoquastic ground state needs).  Returns the
+    trace's tail (last 50 iterations: mean, binned error) and its error
+    against exact diagonalisation where that is known."""
+    method = w.get("opt_method", w["method"])
+    cfg = abi.NqsConfig.from_buffer_copy(st["cfg"])
+    cfg.learning_rate = w.get("opt_lr", cfg.learning_rate)
+    cfg.sr_diag_shift = w.get("opt_shift", cfg.sr_diag_shift)
+    trace = []
+    t0 = time.perf_counter()
+    for i in range(budget):
+        info = step(lib, w, st, method, cfg)
+        trace.append(info.mean_energy)
+        if (i + 1) % 50 == 0:
+            log(f"[bench] optimise {i + 1}/{budget}: E = {info.mean_energy:.6f}")
+    tail = np.array(trace[-50:])
+    nbin = 5
+    bins = tail[: len(tail) // nbin * nbin].reshape(nbin, -1).mean(axis=1)
+    out = dict(optimizer={"holo": "holomorphic CG-SR", "minsr": "MinSR", "cg": "CG-SR"}[method],
+               iterations=budget, lr=cfg.learning_rate, shift=cfg.sr_diag_shift,
+               wall_s=time.perf_counter() - t0, tail_mean=float(tail.mean()),
+               tail_err=float(bins.std(ddof=1) / np.sqrt(nbin)) if len(tail) >= nbin else None,
+               trace_every_10=[float(x) for x in trace[::10]])
+    if w["ed"] is not None:
+        out["e_ed"] = w["ed"]
+        out["delta_e_rel"] = (out["tail_mean"] - w["ed"]) / abs(w["ed"])
+    if not np.all(np.isfinite(trace)):
+        out["nonfinite_at"] = int(np.argmin(np.isfinite(trace)))
+    return out
 
 
 # kernel symbol (as rocprofv3 names it) behind each timing tag
@@ -252,12 +300,20 @@ def kernel_work(tag, w, st, launches, iters):
     return "hbm", B * N * 4, "B"
 
 
-def gpu_run(args, w):
+def gpu_run(args, w, move=None, budget=0):
+    """One sampler leg: set up, thermalise, optimise `budget` iterations
+    (untimed; the energy error is reported), then W warm-up and K timed SR
+    iterations of the workload's step at that state."""
     lib = abi.load_library()
-    st = setup(lib, w, args)
-    log(f"[bench] workload {args.workload}: N={st['N']} P={st['P']} samples/iter="
+    st = setup(lib, w, args, move)
+    mname = {abi.MOVE_FLIP: "flip", abi.MOVE_EXCHANGE: "exchange", abi.MOVE_LEARNED: "learned"}[st["move"]]
+    log(f"[bench] workload {args.workload} ({mname} moves): N={st['N']} P={st['P']} samples/iter="
         f"{w['chains'] * w['per_chain']}")
     lib.nqs_sampler_thermalize(st["s"])
+    opt = optimise(lib, w, st, budget) if budget > 0 else None
+    if opt:
+        log(f"[bench] optimised {budget} iterations ({opt['optimizer']}): tail E = "
+            f"{opt['tail_mean']:.6f}" + (f", dE = {opt['delta_e_rel']:.3e}" if "delta_e_rel" in opt else ""))
 
     def profiled_step():
         lib.nqs_hip_kernel_timing(",".join(TAGS).encode())
@@ -301,58 +357,17 @@ def gpu_run(args, w):
     lib.nqs_hip_kernel_stats(dom.encode(), ctypes.byref(c), ctypes.byref(ms))
     lib.nqs_hip_kernel_timing(None)
     log(f"[bench] timed {args.steps} iterations in {elapsed:.3f} s")
-    tau = pooled_tau(series)
+    tau, tau_lo, tau_hi = tau_ci(series)
     n_total = series.shape[0] * series.shape[1]
     ess = n_total / (2.0 * tau)
     sample_s = sum(sm[0] for sm in stages) * 1e-3 if len(stages) == args.steps else None
     learned = None
-    if args.move == "learned":
+    if st["move"] == abi.MOVE_LEARNED:
         acc, props, ntr, nll = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_int(), ctypes.c_double()
         lib.nqs_hip_sampler_learned_stats(st["s"], ctypes.byref(acc), ctypes.byref(props),
                                           ctypes.byref(ntr), ctypes.byref(nll))
         learned = dict(acceptance=acc.value, proposals=props.value, trainings=ntr.value,
-                       last_nll=nll.value)
-    conv = None
-    if args.converge > 0:
-        trace = list(energies)
-        tc = time.perf_counter()
-        for i in range(args.converge):
-            info = step(lib, w, st)
-            trace.append(info.mean_energy)
-            if (i + 1) % 50 == 0:
-                log(f"[bench] converge {i + 1}/{args.converge}: E = {info.mean_energy:.6f}")
-        # effective samples per second at the optimised psi: the sampler's
-        # mixing matters most once |psi|^2 has structure
-        ser = np.zeros((C, 0))
-        smp_ms = []
-        for i in range(max(args.steps, 4)):
-            info = step(lib, w, st)
-            trace.append(info.mean_energy)
-            n = lib.nqs_hip_sampler_last_energies(st["s"], abi.ptr(buf), C * R)
-            ser = np.concatenate([ser, buf[: C * n].reshape(C, n)], axis=1)
-            sm = np.zeros(7)
-            if lib.nqs_hip_last_stage_ms(st["s"], abi.ptr(sm)) == 0:
-                smp_ms.append(sm[0])
-        tau_end = pooled_tau(ser)
-        # sampling-phase time where the library reports stages (a CG-SR step
-        # replayed as one graph has none: then the whole iterations)
-        t_end = sum(smp_ms) * 1e-3 if len(smp_ms) == max(args.steps, 4) else None
-        ess_end = ser.size / (2.0 * tau_end) / t_end if t_end else None
-        tail = np.array(trace[-50:])
-        nbin = 5
-        bins = tail[: len(tail) // nbin * nbin].reshape(nbin, -1).mean(axis=1)
-        conv = dict(iterations=len(trace), wall_s=time.perf_counter() - tc,
-                    tau_int_end=tau_end, ess_per_s_end=ess_end,
-                    sample_ms_end=float(np.mean(smp_ms)) if t_end else None,
-                    tail_mean=float(tail.mean()), tail_err=float(bins.std(ddof=1) / np.sqrt(nbin)),
-                    trace_every_10=[float(x) for x in trace[::10]])
-        if w["ed"] is not None:
-            conv["e_ed"] = w["ed"]
-            conv["delta_e_rel"] = (conv["tail_mean"] - w["ed"]) / abs(w["ed"])
-        if args.move == "learned":
-            acc = ctypes.c_double()
-            lib.nqs_hip_sampler_learned_stats(st["s"], ctypes.byref(acc), None, None, None)
-            conv["learned_acceptance"] = acc.value
+                       last_nll=nll.value, adam_lr=LEARNED_LR)
     avg_ms = ms.value / max(c.value, 1)
     launches, iters = c.value, args.steps
     if not c.value:
@@ -366,16 +381,19 @@ def gpu_run(args, w):
     else:
         achieved, peak, runit = work / (avg_ms * 1e-3) / 1e9, PEAK_HBM / 1e9, "GB/s"
     res = dict(
-        ess_per_s=ess / sample_s if sample_s else ess / elapsed, ess_per_s_iteration=ess / elapsed,
+        move=mname, ess_per_s=ess / sample_s if sample_s else ess / elapsed,
+        ess_per_s_iteration=ess / elapsed,
         sample_ms=1e3 * sample_s / args.steps if sample_s else None, learned=learned,
-        convergence=conv, ms_per_step=1e3 * elapsed / args.steps, tau=tau,
+        optimised=opt, ms_per_step=1e3 * elapsed / args.steps, tau=tau, tau_ci95=[tau_lo, tau_hi],
+        samples_per_chain=int(series.shape[1]),
         energies=energies, acceptance=infos[-1]["acceptance_ratio"],
         stage_ms=np.mean(stages, axis=0).tolist() if stages else None,
         roofline=dict(kernel=dom, symbol=SYMBOL.get(dom), bound=bound, achieved=achieved, peak=peak,
                       unit=runit, frac=achieved / peak, traffic=traffic_bytes(args.workload, dom),
                       launches=c.value, avg_ms=avg_ms,
                       work_per_launch=work, work_unit=unit),
-        split={t: dict(launches=cc, ms=mm) for t, (cc, mm) in per.items()}, N=st["N"], P=st["P"])
+        split={t: dict(launches=cc, ms=mm) for t, (cc, mm) in per.items()}, N=st["N"], P=st["P"],
+        dual=st["dual"], factored=st["factored"])
     lib.nqs_sampler_free(st["s"])
     lib.nqs_ansatz_free(st["a"])
     return res
@@ -515,17 +533,18 @@ def gpu_run_multi(args, w):
     iters = [infos[i].cg_iterations for i in range(n)]
     energies = [infos[i].mean_energy for i in range(n)]
     conv = None
-    if args.converge > 0:
-        trace = np.zeros((n, args.converge))
-        for k in range(args.converge):
+    conv_iters = args.budget or 0
+    if conv_iters > 0:
+        trace = np.zeros((n, conv_iters))
+        for k in range(conv_iters):
             lib.nqs_hip_multi_step(m, infos)
             trace[:, k] = [infos[i].mean_energy for i in range(n)]
             if (k + 1) % 50 == 0:
-                log(f"[bench] converge {k + 1}/{args.converge}")
+                log(f"[bench] converge {k + 1}/{conv_iters}")
         tail = trace[:, -20:].mean(axis=1)
         e0 = np.array([tfim_pbc_closed_form(N, h) for N, h in inst])
         rel = (tail - e0) / np.abs(e0)
-        conv = dict(iterations=args.converge, delta_e_rel=[float(x) for x in rel],
+        conv = dict(iterations=conv_iters, delta_e_rel=[float(x) for x in rel],
                     delta_e_rel_max=float(rel.max()), delta_e_rel_median=float(np.median(rel)))
     dom = max(per, key=lambda t: per[t][1])
     c, tot_ms = per[dom]
@@ -712,16 +731,19 @@ def cpu_baseline(w, gpu):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=12)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--gram", default="auto", choices=["direct", "factored", "auto"])
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the HBM copy probe on MIB mebibytes (counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="C5: skip the J1-J2 10x10 SR-iteration comparison with the reference CPU")
     ap.add_argument("--move", default=None, choices=["flip", "exchange", "learned"])
-    ap.add_argument("--converge", type=int, default=0, metavar="K")
+    ap.add_argument("--budget", type=int, default=None, metavar="K",
+                    help="optimisation iterations before timing (default: the workload's, C5 300)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of "
@@ -736,11 +758,20 @@ def main():
     w = WORKLOADS[args.workload]
     if w.get("multi"):
         return main_multi(args, w)
-    gpu = gpu_run(args, w)
+    budget = args.budget if args.budget is not None else w.get("budget", 0)
+    if args.workload == "c5" and args.move is None:
+        return main_c5(args, w, budget)
+    gpu = gpu_run(args, w, None, budget)
     cpu = None
     if not args.no_cpu:
         log("[bench] timing the reference CPU path on a bounded sample")
         cpu = cpu_baseline(w, gpu)
+    out = line(args, w, gpu, cpu)
+    print(json.dumps(out))
+
+
+def line(args, w, gpu, cpu):
+    """The bench JSON line of one sampler leg."""
     out = {
         "metric": METRIC,
         "value": gpu["ess_per_s"],
@@ -757,11 +788,12 @@ def main():
         "config": {"workload": args.workload + ": " + w["desc"], "N": gpu["N"], "P": gpu["P"],
                    "samples_per_iter": w["chains"] * w["per_chain"], "chains": w["chains"],
                    "method": w["method"], "gram": args.gram, "precision": args.precision,
-                   "move": args.move or {0: "flip", 1: "exchange", 2: "learned"}[w["move"]]},
+                   "move": gpu["move"]},
         "sr_iter_ms": gpu["ms_per_step"],
         "sample_ms": gpu["sample_ms"],
         "ess_per_s_iteration": gpu["ess_per_s_iteration"],
         "tau_int": gpu["tau"],
+        "tau_int_ci95": gpu["tau_ci95"],
         "acceptance": gpu["acceptance"],
         "energy_last": gpu["energies"][-1],
         "stage_ms": dict(zip(["sample", "eloc", "O_reduce", "solve", "update"], gpu["stage_ms"][:5]))
@@ -773,11 +805,74 @@ def main():
     }
     if gpu["learned"]:
         out["learned"] = gpu["learned"]
-    if gpu["convergence"]:
-        out["convergence"] = gpu["convergence"]
+    if gpu["optimised"]:
+        out["optimised"] = gpu["optimised"]
     if cpu:
         out["speedup_vs_cpu_iter"] = cpu["sr_iter_s"] / (gpu["ms_per_step"] * 1e-3)
+    return out
+
+
+def leg_summary(g):
+    d = dict(ess_per_s=g["ess_per_s"], ess_per_s_iteration=g["ess_per_s_iteration"],
+             sr_iter_ms=g["ms_per_step"], sample_ms=g["sample_ms"], tau_int=g["tau"],
+             tau_int_ci95=g["tau_ci95"], samples_per_chain=g["samples_per_chain"],
+             acceptance=g["acceptance"])
+    if g["learned"]:
+        d["learned"] = g["learned"]
+    if g["optimised"]:
+        o = g["optimised"]
+        d["delta_e"] = {k: o.get(k) for k in ("optimizer", "iterations", "lr", "shift", "tail_mean",
+                                               "tail_err", "e_ed", "delta_e_rel", "wall_s",
+                                               "nonfinite_at")}
+    return d
+
+
+def main_c5(args, w, budget):
+    """The metric's configuration: the same C5 system with plain exchange
+    Metropolis and with the learned proposal.  Each leg optimises its own
+    wavefunction for `budget` iterations (energy error vs exact
+    diagonalisation reported), then times K MinSR iterations at that state;
+    value = the learned leg's effective samples per second of the sampling
+    phase, ms_per_step its SR-iteration time."""
+    legs = {mv: gpu_run(args, w, mv, budget) for mv in ("exchange", "learned")}
+    g = legs["learned"]
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] timing the reference CPU path on a bounded sample")
+        cpu = cpu_baseline(w, g)
+    out = line(args, w, g, cpu)
+    out["data"] = (f"synthetic: random-init parameters N(0,0.01), chains from random Sz=0 states; "
+                   f"each sampler leg first optimises its own wavefunction for {budget} iterations "
+                   f"({w.get('opt_method', w['method'])} SR, untimed), then times {args.steps} "
+                   f"MinSR iterations at that state")
+    out["samplers"] = {mv: leg_summary(legs[mv]) for mv in legs}
+    ex = legs["exchange"]
+    out["learned_vs_exchange"] = dict(
+        ess_per_s=g["ess_per_s"] / ex["ess_per_s"],
+        ess_per_s_iteration=g["ess_per_s_iteration"] / ex["ess_per_s_iteration"],
+        sr_iter_ms=g["ms_per_step"] / ex["ms_per_step"])
+    if not args.no_c3:
+        out["c3_vs_reference_cpu"] = c3_comparison(args)
     print(json.dumps(out))
+
+
+def c3_comparison(args):
+    """J1-J2 10x10 (configuration 3): GPU SR-iteration time (plain exchange
+    MH, MinSR on the 16384^2 Gram) against the reference CPU iteration on 1
+    core, for the >= 50x target."""
+    w3 = WORKLOADS["c3"]
+    a3 = argparse.Namespace(**vars(args))
+    a3.workload, a3.steps, a3.warmup, a3.move = "c3", 5, 2, None
+    g3 = gpu_run(a3, w3, None, 0)
+    out = dict(workload="c3: " + w3["desc"], sr_iter_ms=g3["ms_per_step"],
+               roofline={k: g3["roofline"][k] for k in ("kernel", "achieved", "unit", "frac")})
+    if not args.no_cpu:
+        log("[bench] c3: timing the reference CPU path on a bounded sample")
+        cpu = cpu_baseline(w3, g3)
+        out.update(cpu_sr_iter_s=cpu["sr_iter_s"], cpu_kind=cpu["kind"], cpu_cores=cpu["cores"],
+                   cpu_sample=cpu["sample"],
+                   speedup_vs_cpu_iter=cpu["sr_iter_s"] / (g3["ms_per_step"] * 1e-3))
+    return out
 
 
 def main_multi(args, w):

@@ -127,6 +127,42 @@ int nqs_sr_run_minsr(const nqs_config_t *cfg, int size_x, int size_y,
                      nqs_log_amp_fn_t log_amp_fn, void *log_amp_user,
                      double *out_energy_trace);
 
+/* nqs_optimizer.h:206-213 / nqs_optimizer.c:904-993.  Real-time tVMC,
+ * forward Euler: F_k = 2[Cov(R_k, E_im) - Cov(I_k, E_re)] over holomorphic
+ * O = R + iI, (G + eps I) delta = F with the holomorphic metric G,
+ * theta <- theta + dt delta; update_norm = |delta| |dt|. */
+int nqs_tvmc_step_real_time(const nqs_config_t *cfg, double dt, int size_x,
+                            int size_y, nqs_ansatz_t *ansatz,
+                            nqs_sampler_t *sampler, nqs_log_amp_fn_t log_amp_fn,
+                            void *log_amp_user, nqs_sr_step_info_t *out_info);
+
+/* nqs_optimizer.h:242-251 / nqs_optimizer.c:562-716.  Excited-state SR:
+ * local energies augmented by penalty_mu r(s) conj(<r>),
+ * r = psi_ref / psi (log-ratio clamped to +-10), holomorphic SR on them;
+ * mean_energy and variance_energy report the physical energy. */
+int nqs_sr_step_excited(const nqs_config_t *cfg, int size_x, int size_y,
+                        nqs_ansatz_t *ansatz, nqs_sampler_t *sampler,
+                        nqs_log_amp_fn_t log_amp_fn, void *log_amp_user,
+                        nqs_log_amp_fn_t ref_log_amp_fn, void *ref_log_amp_user,
+                        double penalty_mu, nqs_sr_step_info_t *out_info);
+
+/* nqs_optimizer.h:253-262 / nqs_optimizer.c:718-742: thermalise, then
+ * num_iterations excited steps; trace[it] = physical mean energy. */
+int nqs_sr_run_excited(const nqs_config_t *cfg, int size_x, int size_y,
+                       nqs_ansatz_t *ansatz, nqs_sampler_t *sampler,
+                       nqs_log_amp_fn_t log_amp_fn, void *log_amp_user,
+                       nqs_log_amp_fn_t ref_log_amp_fn, void *ref_log_amp_user,
+                       double penalty_mu, double *out_energy_trace);
+
+/* nqs_optimizer.h:267-274 / nqs_optimizer.c:830-902.  Heun: k1 = delta(theta),
+ * k2 = delta(theta + dt k1) from a second batch, theta <- theta +
+ * dt (k1 + k2) / 2; info from the first stage, cg_iterations summed,
+ * converged = both stages converged. */
+int nqs_tvmc_step_heun(const nqs_config_t *cfg, double dt, int size_x, int size_y,
+                       nqs_ansatz_t *ansatz, nqs_sampler_t *sampler,
+                       nqs_log_amp_fn_t log_amp_fn, void *log_amp_user,
+                       nqs_sr_step_info_t *out_info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,12 +47,13 @@ typedef enum {
 typedef enum {
     NQS_HIP_GRAM_DIRECT   = 0, /* T = O_c O_c^T over the materialised O       */
     NQS_HIP_GRAM_FACTORED = 1, /* RBM / complex RBM: T from (S S^T, Th Th^T)  */
-    NQS_HIP_GRAM_AUTO     = 2  /* the cheaper of the sample-space solve
+    NQS_HIP_GRAM_AUTO     = 2, /* the cheaper of the sample-space solve
                                   (factored Gram where the ansatz allows it)
                                   and the parameter-space solve
                                   (O_c^T O_c / Ns + eps I) delta = O_c^T eps / Ns,
                                   which gives the same delta (push-through
                                   identity), by operation count              */
+    NQS_HIP_GRAM_PARAMETER = 3 /* always the parameter-space solve           */
 } nqs_hip_gram_t;
 
 typedef struct {
@@ -214,6 +215,29 @@ int nqs_hip_last_stage_ms(const nqs_sampler_t *s, double out[7]);
  * fail (x = 0), -1 on errors. */
 int nqs_hip_gram(const double *O, int B, long P, double scale, double *T);
 int nqs_hip_cholesky_solve(double *T, int n, const double *b, double *x);
+
+/* The MinSR update of nqs_sr_step_minsr_full (src/nqs/nqs_optimizer.c:
+ * 1119-1177) for caller-given samples spins [B][N] and local energies
+ * e_loc [B], with the ansatz's current parameters (real O for a real RBM,
+ * the real-projected O for a complex RBM, as the reference's MinSR uses):
+ * eps_i = E_i - mean E, O_c = O - mean O, and
+ *   gram = NQS_HIP_GRAM_DIRECT:   (O_c O_c^T / B + shift I) y = eps over
+ *                                 the materialised O, delta = O_c^T y / B;
+ *   gram = NQS_HIP_GRAM_FACTORED: the same system with the Gram formed from
+ *                                 the RBM factors (RBM / complex RBM only);
+ *   gram = NQS_HIP_GRAM_PARAMETER: (O_c^T O_c / B + shift I) delta =
+ *                                 O_c^T eps / B (same delta by the
+ *                                 push-through identity).
+ * Failure rule as the reference: one retry with +1e-6 on the diagonal of
+ * the partially factored matrix, then delta = 0.  out_delta [P] receives
+ * delta (the parameters are not changed).  Sample space only: out_y [B]
+ * (may be NULL) receives y, and out_T [B][B] (may be NULL) the shifted
+ * Gram before factorisation, its lower triangle (j <= i) defined.
+ * Returns 1 when a factorisation succeeded, 0 when both attempts failed,
+ * -1 on errors. */
+int nqs_hip_minsr_solve(nqs_ansatz_t *a, const int *spins, int B, const double *e_loc,
+                        double shift, int gram, double *out_delta, double *out_y,
+                        double *out_T);
 
 /* Live kernel timing with HIP events on the library's stream.
  * nqs_hip_kernel_timing("eloc,gemm") starts timing the listed kernel tags

@@ -943,6 +943,210 @@ int orc_sr_run_holomorphic_full(const nqs_config_t *cfg, int Lx, int Ly,
     return 0;
 }
 
+/* Holomorphic force 2[(R^T ea + I^T eb)/Ns - Rm <ea> - Im <eb>] and the
+ * column means of R and I (the sums of nqs_optimizer.c:496-509, 677-690,
+ * 797-810, with (ea, eb) the two energy series each caller pairs). */
+static void holo_force(const sr_buf *b, const double *ea, const double *eb, double sa,
+                       const double *means, double *Rm, double *Im, double *F) {
+    long P = b->P;
+    int B = b->B;
+    double ma = 0.0, mb = 0.0;
+    if (means) {
+        ma = means[0];
+        mb = means[1];
+    } else {
+        for (int i = 0; i < B; i++) { ma += ea[i]; mb += eb[i]; }
+        ma /= (double)B;
+        mb /= (double)B;
+    }
+    for (long k = 0; k < P; k++) {
+        double sR = 0.0, sI = 0.0, sRa = 0.0, sIb = 0.0;
+        for (int i = 0; i < B; i++) {
+            double rk = b->O[(size_t)i * P + k], ik = b->Oi[(size_t)i * P + k];
+            sR += rk;
+            sI += ik;
+            sRa += rk * ea[i];
+            sIb += ik * eb[i];
+        }
+        Rm[k] = sR / (double)B;
+        Im[k] = sI / (double)B;
+        F[k] = 2.0 * ((sRa + sa * sIb) / (double)B - Rm[k] * ma - sa * Im[k] * mb);
+    }
+}
+
+/* Excited-state SR with the orthogonality penalty mu |<r>|^2,
+ * r(s) = psi_ref(s) / psi(s) with the log-ratio clamped to +-10:
+ * nqs_optimizer.c:562-716.  The local energy is augmented by
+ * mu r(s) conj(<r>); the force is the holomorphic one of the augmented
+ * energies; info reports the physical energy. */
+int orc_sr_step_excited(const nqs_config_t *cfg, int Lx, int Ly, orc_ansatz_t *a,
+                        orc_sampler_t *s, orc_log_amp_fn fn, void *user,
+                        orc_log_amp_fn ref_fn, void *ref_user, double mu,
+                        orc_sr_info_t *info) {
+    if (!cfg || !a || !s || !ref_fn) return -1;
+    if (!fn) { fn = orc_ansatz_log_amp; user = a; }
+    sr_buf b;
+    if (sr_collect(cfg, Lx, Ly, a, s, fn, user, 1, NULL, NULL, NULL, &b)) return -1;
+    long P = b.P;
+    int B = b.B, N = b.N;
+    double *rr = malloc((size_t)B * sizeof(double)), *ri = malloc((size_t)B * sizeof(double));
+    for (int i = 0; i < B; i++) {
+        const int *sp = b.batch + (size_t)i * N;
+        double lp, ap, lr, ar;
+        fn(sp, N, user, &lp, &ap);
+        ref_fn(sp, N, ref_user, &lr, &ar);
+        double dl = lr - lp;
+        if (dl > 10.0) dl = 10.0;
+        if (dl < -10.0) dl = -10.0;
+        const double m = exp(dl), dth = ar - ap;
+        rr[i] = m * cos(dth);
+        ri[i] = m * sin(dth);
+    }
+    double Rr = 0.0, Ri = 0.0, erm = 0.0, eim = 0.0, er2 = 0.0;
+    for (int i = 0; i < B; i++) {
+        Rr += rr[i];
+        Ri += ri[i];
+        erm += b.E[i];
+        eim += b.Ei[i];
+        er2 += b.E[i] * b.E[i];
+    }
+    Rr /= (double)B;
+    Ri /= (double)B;
+    erm /= (double)B;
+    eim /= (double)B;
+    er2 /= (double)B;
+    for (int i = 0; i < B; i++) {
+        b.E[i] += mu * (rr[i] * Rr + ri[i] * Ri);
+        b.Ei[i] += mu * (ri[i] * Rr - rr[i] * Ri);
+    }
+    /* the augmented means in closed form, as nqs_optimizer.c:654-657 */
+    const double means[2] = {erm + mu * (Rr * Rr + Ri * Ri), eim + mu * (Ri * Rr - Rr * Ri)};
+    double *Rm = calloc(P, sizeof(double)), *Im = calloc(P, sizeof(double));
+    double *F = calloc(P, sizeof(double)), *d = calloc(P, sizeof(double));
+    holo_force(&b, b.E, b.Ei, 1.0, means, Rm, Im, F);
+    int conv = 0;
+    int it = orc_cg_solve_complex(b.O, b.Oi, B, P, Rm, Im, cfg->sr_diag_shift, F,
+                                  cfg->sr_cg_max_iters, cfg->sr_cg_tol, d, &conv);
+    orc_ansatz_apply_update(a, d, -cfg->learning_rate);
+    if (info) {
+        info->mean_energy = erm;
+        info->variance_energy = er2 - erm * erm;
+        info->update_norm = sqrt(dotv(d, d, P)) * cfg->learning_rate;
+        info->acceptance_ratio = orc_sampler_acceptance_ratio(s);
+        info->cg_iterations = it;
+        info->converged = conv;
+    }
+    free(rr); free(ri); free(Rm); free(Im); free(F); free(d);
+    sr_free(&b);
+    return 0;
+}
+
+/* nqs_optimizer.c:718-742 */
+int orc_sr_run_excited(const nqs_config_t *cfg, int Lx, int Ly, orc_ansatz_t *a,
+                       orc_sampler_t *s, orc_log_amp_fn fn, void *user,
+                       orc_log_amp_fn ref_fn, void *ref_user, double mu, double *trace) {
+    if (!cfg || !a || !s || !ref_fn) return -1;
+    orc_sampler_thermalize(s);
+    for (int it = 0; it < cfg->num_iterations; it++) {
+        orc_sr_info_t inf;
+        int rc = orc_sr_step_excited(cfg, Lx, Ly, a, s, fn, user, ref_fn, ref_user, mu, &inf);
+        if (rc) return rc;
+        if (trace) trace[it] = inf.mean_energy;
+    }
+    return 0;
+}
+
+/* Real-time tVMC direction Re(S)^-1 Im(F) at the current parameters from a
+ * fresh batch, F_k = 2[Cov(R_k, E_im) - Cov(I_k, E_re)]
+ * (nqs_optimizer.c:747-828); the batch has the sampler's site count (the
+ * reference's Heun helper sizes it Lx * Ly, which is the same number on
+ * square lattices). */
+static int tvmc_delta(const nqs_config_t *cfg, int Lx, int Ly, orc_ansatz_t *a,
+                      orc_sampler_t *s, orc_log_amp_fn fn, void *user, double *d,
+                      double *em_out, double *var_out, int *it_out, int *conv_out) {
+    sr_buf b;
+    if (sr_collect(cfg, Lx, Ly, a, s, fn, user, 1, NULL, NULL, NULL, &b)) return -1;
+    long P = b.P;
+    int B = b.B;
+    double erm = 0.0, er2 = 0.0;
+    for (int i = 0; i < B; i++) { erm += b.E[i]; er2 += b.E[i] * b.E[i]; }
+    erm /= (double)B;
+    er2 /= (double)B;
+    double *Rm = calloc(P, sizeof(double)), *Im = calloc(P, sizeof(double));
+    double *F = calloc(P, sizeof(double));
+    /* 2[(R^T E_im - I^T E_re)/Ns - Rm <E_im> + Im <E_re>] */
+    holo_force(&b, b.Ei, b.E, -1.0, NULL, Rm, Im, F);
+    int conv = 0;
+    int it = orc_cg_solve_complex(b.O, b.Oi, B, P, Rm, Im, cfg->sr_diag_shift, F,
+                                  cfg->sr_cg_max_iters, cfg->sr_cg_tol, d, &conv);
+    if (em_out) *em_out = erm;
+    if (var_out) *var_out = er2 - erm * erm;
+    if (it_out) *it_out = it;
+    if (conv_out) *conv_out = conv;
+    free(Rm); free(Im); free(F);
+    sr_free(&b);
+    return 0;
+}
+
+/* Forward Euler theta <- theta + dt delta: nqs_optimizer.c:904-993 */
+int orc_tvmc_step_real_time(const nqs_config_t *cfg, double dt, int Lx, int Ly,
+                            orc_ansatz_t *a, orc_sampler_t *s, orc_log_amp_fn fn,
+                            void *user, orc_sr_info_t *info) {
+    if (!cfg || !a || !s) return -1;
+    if (!fn) { fn = orc_ansatz_log_amp; user = a; }
+    long P = orc_ansatz_num_params(a);
+    double *d = calloc(P, sizeof(double));
+    double em, var;
+    int it, conv;
+    if (tvmc_delta(cfg, Lx, Ly, a, s, fn, user, d, &em, &var, &it, &conv)) { free(d); return -1; }
+    orc_ansatz_apply_update(a, d, dt);
+    if (info) {
+        info->mean_energy = em;
+        info->variance_energy = var;
+        info->update_norm = sqrt(dotv(d, d, P)) * fabs(dt);
+        info->acceptance_ratio = orc_sampler_acceptance_ratio(s);
+        info->cg_iterations = it;
+        info->converged = conv;
+    }
+    free(d);
+    return 0;
+}
+
+/* Heun: k1 = delta(theta), k2 = delta(theta + dt k1) from a second batch,
+ * theta <- theta + dt (k1 + k2) / 2: nqs_optimizer.c:830-902 */
+int orc_tvmc_step_heun(const nqs_config_t *cfg, double dt, int Lx, int Ly,
+                       orc_ansatz_t *a, orc_sampler_t *s, orc_log_amp_fn fn,
+                       void *user, orc_sr_info_t *info) {
+    if (!cfg || !a || !s) return -1;
+    if (!fn) { fn = orc_ansatz_log_amp; user = a; }
+    long P = orc_ansatz_num_params(a);
+    double *k1 = calloc(P, sizeof(double)), *k2 = calloc(P, sizeof(double));
+    double *th0 = malloc((size_t)P * sizeof(double));
+    memcpy(th0, orc_ansatz_params_raw(a), (size_t)P * sizeof(double));
+    double e1 = 0.0, v1 = 0.0;
+    int it1 = 0, c1 = 0, it2 = 0, c2 = 0;
+    int rc = tvmc_delta(cfg, Lx, Ly, a, s, fn, user, k1, &e1, &v1, &it1, &c1);
+    if (!rc) {
+        orc_ansatz_apply_update(a, k1, dt);
+        rc = tvmc_delta(cfg, Lx, Ly, a, s, fn, user, k2, NULL, NULL, &it2, &c2);
+    }
+    memcpy(orc_ansatz_params_raw(a), th0, (size_t)P * sizeof(double));
+    if (!rc) {
+        for (long k = 0; k < P; k++) k1[k] = 0.5 * (k1[k] + k2[k]);
+        orc_ansatz_apply_update(a, k1, dt);
+        if (info) {
+            info->mean_energy = e1;
+            info->variance_energy = v1;
+            info->update_norm = sqrt(dotv(k1, k1, P)) * fabs(dt);
+            info->acceptance_ratio = orc_sampler_acceptance_ratio(s);
+            info->cg_iterations = it1 + it2;
+            info->converged = c1 && c2;
+        }
+    }
+    free(k1); free(k2); free(th0);
+    return rc;
+}
+
 /* Unblocked lower Cholesky in place, upper triangle zeroed on success:
  * nqs_optimizer.c:1026-1043.  Only the lower triangle is read. */
 int orc_cholesky_lower(double *A, int n) {

@@ -155,6 +155,22 @@ int orc_sr_step_holomorphic(const nqs_config_t *cfg, int Lx, int Ly, orc_ansatz_
 int orc_sr_run_holomorphic(const nqs_config_t *cfg, int Lx, int Ly, orc_ansatz_t *a,
                            orc_sampler_t *s, orc_log_amp_fn fn, void *user, double *trace);
 
+/* Excited-state SR (orthogonality penalty) and real-time tVMC
+ * (src/nqs/nqs_optimizer.c:562-993). */
+int orc_sr_step_excited(const nqs_config_t *cfg, int Lx, int Ly, orc_ansatz_t *a,
+                        orc_sampler_t *s, orc_log_amp_fn fn, void *user,
+                        orc_log_amp_fn ref_fn, void *ref_user, double mu,
+                        orc_sr_info_t *info);
+int orc_sr_run_excited(const nqs_config_t *cfg, int Lx, int Ly, orc_ansatz_t *a,
+                       orc_sampler_t *s, orc_log_amp_fn fn, void *user,
+                       orc_log_amp_fn ref_fn, void *ref_user, double mu, double *trace);
+int orc_tvmc_step_real_time(const nqs_config_t *cfg, double dt, int Lx, int Ly,
+                            orc_ansatz_t *a, orc_sampler_t *s, orc_log_amp_fn fn,
+                            void *user, orc_sr_info_t *info);
+int orc_tvmc_step_heun(const nqs_config_t *cfg, double dt, int Lx, int Ly,
+                       orc_ansatz_t *a, orc_sampler_t *s, orc_log_amp_fn fn,
+                       void *user, orc_sr_info_t *info);
+
 /* Solver building blocks on caller-provided matrices (row-major O[Ns][P]). */
 int  orc_cg_solve(const double *O, long Ns, long P, const double *Omean,
                   double eps, const double *F, int max_iters, double tol,

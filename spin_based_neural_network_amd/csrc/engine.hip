@@ -263,6 +263,20 @@ int model_view(nqs_ansatz* a, ModelView<R>& v) {
   return 0;
 }
 
+// the view for kernels that price moves with the ratio tables (sampler,
+// local energies): refused, with the reason on stderr, when a weight lies
+// beyond the tables' range
+template <typename R>
+int ratio_view(nqs_ansatz* a, ModelView<R>& v) {
+  CKR(model_view<R>(a, v));
+  nqs::ModelBuffers<R>& mb = model_bufs<R>(a);
+  if (mb.ratio_ok == 0) {
+    (void)nqs::report_ratio_range<R>(a->d_par.as<double>(), a->N, a->M, mb.range_flag, g_stream);
+    return -1;
+  }
+  return 0;
+}
+
 bool is_ansatz_cb(nqs_log_amp_fn_t fn, void* user) {
   return fn == nqs_ansatz_log_amp && user != nullptr;
 }
@@ -297,8 +311,8 @@ bool recognise(nqs_log_amp_fn_t fn, void* user, AmpSource& src) {
 // workspace shared by all calls (single stream, serialised)
 // ---------------------------------------------------------------------------
 struct Workspace {
-  DBuf spins, t_re, t_im, la, ar, e_re, e_im, O, Oi, colwork, u1, u2, vecs, T, diag, y, scal,
-      cg, fail, ham, sublat, hostla, hostar, linv, T2, Sd, elw, fwd;
+  DBuf spins, t_re, t_im, q_re, q_im, la, ar, e_re, e_im, O, Oi, colwork, u1, u2, vecs, T, diag,
+      y, scal, cg, fail, ham, sublat, hostla, hostar, linv, T2, Sd, elw, fwd, xr, heun;
 };
 Workspace g_ws;
 
@@ -405,13 +419,22 @@ int ham_upload(const nqs::HamHost& H, const AmpSource& src, DBuf& store, DBuf& s
 }
 
 // forward pass of B device configurations into the workspace t buffers
+// (and, with want_q, the small logistic members the local energies use)
 template <typename R>
-int forward_t(nqs_ansatz* a, const int* d_spins, int B, double* la, double* ar) {
+int forward_t(nqs_ansatz* a, const int* d_spins, int B, double* la, double* ar,
+              bool want_q = false) {
   ModelView<R> v;
   CKR(model_view<R>(a, v));
   const size_t tb = sizeof(R) * (size_t)B * (size_t)std::max(a->M, 1);
   CKR(g_ws.t_re.ensure(tb));
   CKR(g_ws.t_im.ensure(tb));
+  R *q_re = nullptr, *q_im = nullptr;
+  if (want_q) {
+    CKR(g_ws.q_re.ensure(tb));
+    CKR(g_ws.q_im.ensure(tb));
+    q_re = g_ws.q_re.as<R>();
+    q_im = g_ws.q_im.as<R>();
+  }
   // many FP64 RBM configurations, t only: theta on FP64 MFMA
   if (std::is_same<R, double>::value && !la && !ar && B >= 512 && a->M >= 1 &&
       (a->kind == NQS_ANSATZ_RBM || a->kind == NQS_ANSATZ_COMPLEX_RBM)) {
@@ -420,10 +443,11 @@ int forward_t(nqs_ansatz* a, const int* d_spins, int B, double* la, double* ar) 
     CKR(model_view<double>(a, vd));
     return nqs::launch_forward_t_gemm(vd, a->d_par.as<double>(), d_spins, B,
                                       g_ws.t_re.as<double>(), g_ws.t_im.as<double>(),
-                                      g_ws.fwd.as<double>(), g_stream);
+                                      g_ws.fwd.as<double>(), g_stream, (double*)q_re,
+                                      (double*)q_im);
   }
   return nqs::launch_forward<R>(v, d_spins, B, g_ws.t_re.as<R>(), g_ws.t_im.as<R>(), la, ar,
-                                g_stream);
+                                g_stream, q_re, q_im);
 }
 
 // E_loc of B device configurations (device amplitude source)
@@ -437,13 +461,14 @@ int eloc_device_T(const nqs_config_t* cfg, int Lx, int Ly, int per, const AmpSou
             src.a->N);
     return -1;
   }
-  CKR(forward_t<R>(src.a, d_spins, B, nullptr, nullptr));
   ModelView<R> v;
-  CKR(model_view<R>(src.a, v));
+  CKR(ratio_view<R>(src.a, v));
+  CKR(forward_t<R>(src.a, d_spins, B, nullptr, nullptr, true));
   const size_t wb = nqs::eloc_ws_bytes<R>(v, H, B);
   CKR(g_ws.elw.ensure(wb));
-  return nqs::launch_eloc<R>(v, H, d_spins, B, g_ws.t_re.as<R>(), g_ws.t_im.as<R>(), d_re, d_im,
-                             g_stream, g_ws.elw.p, wb);
+  return nqs::launch_eloc<R>(v, H, d_spins, B, g_ws.t_re.as<R>(), g_ws.t_im.as<R>(),
+                             g_ws.q_re.as<R>(), g_ws.q_im.as<R>(), d_re, d_im, g_stream,
+                             g_ws.elw.p, wb);
 }
 
 int eloc_device(const nqs_config_t* cfg, int Lx, int Ly, int per, int prec, const AmpSource& src,
@@ -974,7 +999,8 @@ template <typename R>
 int learned_rounds(nqs_sampler* s, int rounds, int n_out, int* d_out) {
   const int C = s->cs.C;
   ModelView<R> mv;
-  CKR(model_view<R>(s->src.a, mv));
+  if (s->opt.ar_local > 0) CKR(ratio_view<R>(s->src.a, mv));   // exchange moves between
+  else CKR(model_view<R>(s->src.a, mv));
   nqs::ArRounds ar;
   ar.constrained = 1;
   ar.lacc = s->lacc.as<unsigned long long>();
@@ -1040,11 +1066,11 @@ int sampler_advance(nqs_sampler* s, int steps, int rounds, int n_out, int* d_out
       ml.out = d_out;
       if (s->opt.precision == NQS_HIP_FP32) {
         ModelView<float> v;
-        CKR(model_view<float>(s->src.a, v));
+        CKR(ratio_view<float>(s->src.a, v));
         CKR(nqs::launch_mh<float>(v, s->cs, ml, g_stream));
       } else {
         ModelView<double> v;
-        CKR(model_view<double>(s->src.a, v));
+        CKR(ratio_view<double>(s->src.a, v));
         CKR(nqs::launch_mh<double>(v, s->cs, ml, g_stream));
       }
     }
@@ -1454,10 +1480,201 @@ int cg_device(long P, const double* F, double* x, int max_iters, double tol, App
   return 0;
 }
 
+// MinSR solve (src/nqs/nqs_optimizer.c:1119-1177) on device data, shared by
+// nqs_sr_step_minsr_full and nqs_hip_minsr_solve.  Inputs: the samples, the
+// uncentred O [B][P] with its column means (null / unused when the factored
+// Gram is used without O), the energies E with their mean in sc[0].  The
+// sample-space solve: eps_i = E_i - <E>, T = O_c O_c^T / B + eps I (direct
+// over O, or factored from the RBM's t = tanh theta left in the workspace by
+// the caller), Cholesky with the reference's retry rule, delta = O_c^T y / B.
+// The parameter-space solve (dual): (O_c^T O_c / B + eps I) delta =
+// O_c^T eps / B.  Optional host copies of T (before factorisation) and y.
+struct MinsrArgs {
+  nqs_ansatz* a = nullptr;
+  const int* d_samples = nullptr;
+  int B = 0;
+  double* O = nullptr;        // [B][P], centred in place (null: factored, no O)
+  const double* Om = nullptr; // column means of O
+  const double* E = nullptr;  // [B]
+  double* sc = nullptr;       // device scalars, sc[0] = <E>
+  double* work = nullptr;     // colsum / factored work
+  double* rhs = nullptr;      // [P] scratch (parameter space)
+  double eps = 0.0;
+  bool factored = false, dual = false;
+  double* delta = nullptr;    // [P] out (device)
+  double* h_T = nullptr;      // optional [B][B] host copy of T (sample space)
+  double* h_y = nullptr;      // optional [B] host copy of y (sample space)
+};
+
+int minsr_core(const MinsrArgs& m, int& converged) {
+  nqs_ansatz* a = m.a;
+  const int B = m.B, N = a->N;
+  const long P = a->P;
+  const double eps = m.eps;
+  double* O = m.O;
+  double* sc = m.sc;
+  double* work = m.work;
+  double* delta = m.delta;
+  const bool ofree = m.factored && !m.dual;
+  converged = 0;
+  double* u = g_ws.u1.as<double>();
+  // eps_i = E_i - <E>, O_c = O - <O>  (nqs_optimizer.c:1119-1126)
+  CKR(nqs::launch_sub_scalar(u, m.E, B, sc + 0, g_stream));
+  if (!ofree) CKR(nqs::launch_center(O, B, P, m.Om, g_stream));
+  CKR(g_ws.diag.ensure(sizeof(double) * B));
+  CKR(g_ws.y.ensure(sizeof(double) * B * 2));
+  CKR(g_ws.fail.ensure(sizeof(int) * 4));
+  double* T = nullptr;
+  const int cpl = a->kind == NQS_ANSATZ_COMPLEX_RBM;
+  auto gram_factored = [&](double* out) -> int {
+    CKR(g_ws.Sd.ensure(sizeof(double) * (size_t)B * (N + 2)));
+    return nqs::launch_gram_factored(m.d_samples, B, N, g_ws.t_re.as<double>(),
+                                     cpl ? g_ws.t_im.as<double>() : nullptr, a->M, eps, out,
+                                     work, g_ws.Sd.as<double>(), g_ws.u2.as<double>(), sc + 6,
+                                     g_stream);
+  };
+  if (m.dual) {
+    // parameter-space MinSR: delta = S^-1 (O_c^T eps / Ns)
+    CKR(g_ws.T.ensure(sizeof(double) * (size_t)P * P));
+    CKR(g_ws.T2.ensure(sizeof(double) * (size_t)P * B));
+    T = g_ws.T.as<double>();
+    double* OT = g_ws.T2.as<double>();
+    double* rhs = m.rhs;
+    CKR(nqs::launch_transpose(O, B, P, OT, g_stream));
+    CKR(nqs::launch_gemm_nt((int)P, (int)P, B, 1.0 / B, OT, B, OT, B, 0.0, T, P, 1, g_stream));
+    CKR(nqs::launch_add_diag(T, (int)P, P, eps, g_stream));
+    CKR(nqs::launch_colsum(O, B, P, u, 1.0 / B, rhs, work, g_stream));
+    CKR(g_ws.linv.ensure(sizeof(double) * nqs::potrf_linv_elems((int)P)));
+    CKR(g_ws.y.ensure(sizeof(double) * 2 * (size_t)P));
+    double* Linv = g_ws.linv.as<double>();
+    int f = nqs::launch_potrf(T, (int)P, Linv, g_ws.fail.as<int>(), g_stream);
+    if (f < 0) return -1;
+    if (f > 0) {
+      // as in the sample-space solve: the partially factored matrix with
+      // +1e-6 on its diagonal is factored once more (input columns from a
+      // second evaluation of S)
+      CKR(g_ws.Oi.ensure(sizeof(double) * (size_t)P * P));
+      double* S2 = g_ws.Oi.as<double>();
+      CKR(nqs::launch_gemm_nt((int)P, (int)P, B, 1.0 / B, OT, B, OT, B, 0.0, S2, P, 1, g_stream));
+      CKR(nqs::launch_add_diag(S2, (int)P, P, eps, g_stream));
+      CKR(nqs::launch_potrf_repair(T, (int)P, f - 1, S2, g_stream));
+      CKR(nqs::launch_add_diag(T, (int)P, P, 1e-6, g_stream));
+      f = nqs::launch_potrf(T, (int)P, Linv, g_ws.fail.as<int>(), g_stream);
+      if (f < 0) return -1;
+    }
+    if (f == 0) {
+      CKR(nqs::launch_potrs(T, Linv, (int)P, rhs, delta, g_ws.y.as<double>(), g_stream));
+      converged = 1;
+    } else {
+      fprintf(stderr, "nqs_sr_step_minsr: Cholesky breakdown after retry; increase "
+                      "sr_diag_shift or num_samples. Update set to 0.\n");
+      CKR(nqs::launch_fill(delta, 0.0, P, g_stream));
+    }
+    return 0;
+  }
+  CKR(g_ws.T.ensure(sizeof(double) * (size_t)B * B));
+  T = g_ws.T.as<double>();
+  if (m.factored) {
+    CKR(gram_factored(T));
+  } else {
+    CKR(nqs::launch_gemm_nt(B, B, (int)P, 1.0 / B, O, P, O, P, 0.0, T, B, 1, g_stream));
+    CKR(nqs::launch_add_diag(T, B, B, eps, g_stream));
+  }
+  if (m.h_T) {
+    CK(hipMemcpyAsync(m.h_T, T, sizeof(double) * (size_t)B * B, hipMemcpyDeviceToHost, g_stream));
+    CK(hipStreamSynchronize(g_stream));
+  }
+  CKR(g_ws.linv.ensure(sizeof(double) * nqs::potrf_linv_elems(B)));
+  double* Linv = g_ws.linv.as<double>();
+  int f = nqs::launch_potrf(T, B, Linv, g_ws.fail.as<int>(), g_stream);
+  if (f < 0) return -1;
+  if (f > 0) {
+    // the reference retries on its partially factored matrix with +1e-6 on
+    // the diagonal (nqs_optimizer.c:1156-1160): rebuild that matrix (input
+    // columns from a second Gram evaluation) and factor again
+    CKR(g_ws.T2.ensure(sizeof(double) * (size_t)B * B));
+    double* T2 = g_ws.T2.as<double>();
+    if (m.factored) {
+      CKR(gram_factored(T2));
+    } else {
+      CKR(nqs::launch_gemm_nt(B, B, (int)P, 1.0 / B, O, P, O, P, 0.0, T2, B, 1, g_stream));
+      CKR(nqs::launch_add_diag(T2, B, B, eps, g_stream));
+    }
+    CKR(nqs::launch_potrf_repair(T, B, f - 1, T2, g_stream));
+    CKR(nqs::launch_add_diag(T, B, B, 1e-6, g_stream));
+    f = nqs::launch_potrf(T, B, Linv, g_ws.fail.as<int>(), g_stream);
+    if (f < 0) return -1;
+  }
+  double* y = g_ws.y.as<double>();
+  if (f == 0) {
+    CKR(nqs::launch_potrs(T, Linv, B, u, y, y + B, g_stream));
+    converged = 1;
+  } else {
+    fprintf(stderr, "nqs_sr_step_minsr: Cholesky breakdown after retry; increase "
+                    "sr_diag_shift or num_samples. Update set to 0.\n");
+    CKR(nqs::launch_fill(y, 0.0, B, g_stream));
+  }
+  if (m.h_y) {
+    CK(hipMemcpyAsync(m.h_y, y, sizeof(double) * B, hipMemcpyDeviceToHost, g_stream));
+    CK(hipStreamSynchronize(g_stream));
+  }
+  // delta_k = (1/Ns) sum_i O_c[i][k] y_i = (1/Ns) sum_i O[i][k] (y_i - mean y)
+  if (ofree) {
+    double* yc = y + B;
+    CKR(nqs::launch_sum(y, B, nullptr, 1.0 / B, sc + 7, g_stream));
+    CKR(nqs::launch_sub_scalar(yc, y, B, sc + 7, g_stream));
+    CKR(nqs::launch_factored_delta(m.d_samples, B, N, a->M, g_ws.t_re.as<double>(),
+                                   cpl ? g_ws.t_im.as<double>() : nullptr, yc, 1.0 / B, work,
+                                   delta, g_stream));
+  } else {
+    CKR(nqs::launch_colsum(O, B, P, y, 1.0 / B, delta, work, g_stream));
+  }
+  return 0;
+}
+
+// Variants of the holomorphic step (src/nqs/nqs_optimizer.c:562-993):
+// excited-state SR (local energies augmented by the orthogonality penalty
+// mu r conj(<r>), r = psi_ref / psi) and real-time tVMC (force
+// 2[Cov(R, E_im) - Cov(I, E_re)], theta += dt delta).  apply = false leaves
+// the parameters alone and copies delta to keep_delta (Heun's two stages).
+struct DynStep {
+  int kind = 0;                 // 1 excited, 2 tVMC
+  double mu = 0.0;
+  nqs_log_amp_fn_t ref_fn = nullptr;
+  void* ref_user = nullptr;
+  double dt = 0.0;
+  bool apply = true;
+  double* keep_delta = nullptr;
+};
+
+// log|psi| and arg psi of the B samples under a callback: on the device for
+// a recognised ansatz (plus the Marshall sign), else through the host
+// callback on the host copies
+int logamp_samples(nqs_log_amp_fn_t fn, void* user, int N, const int* d_samples,
+                   const int* h_samples, int B, double* d_la, double* d_ar) {
+  AmpSource src;
+  if (recognise(fn, user, src) && src.a->N == N) {
+    CKR(ansatz_upload(src.a));
+    CKR(forward_t<double>(src.a, d_samples, B, d_la, d_ar));
+    if (src.marshall) CKR(nqs::launch_marshall_arg(d_samples, B, N, src.mlx, src.mly, d_ar, g_stream));
+    return 0;
+  }
+  if (!h_samples) return -1;
+  std::vector<double> la(B), ar(B);
+  for (int b = 0; b < B; b++) {
+    fn(h_samples + (size_t)b * N, N, user, &la[b], &ar[b]);
+    g_host_calls++;
+  }
+  CK(hipMemcpyAsync(d_la, la.data(), sizeof(double) * B, hipMemcpyHostToDevice, g_stream));
+  CK(hipMemcpyAsync(d_ar, ar.data(), sizeof(double) * B, hipMemcpyHostToDevice, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
 int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz* a,
                  nqs_sampler* s, nqs_log_amp_fn_t fn, void* user, nqs_gradient_fn_t gfn,
                  nqs_holomorphic_gradient_fn_t hfn, void* guser, nqs_sr_step_info_t* info,
-                 bool in_run) {
+                 bool in_run, const DynStep* dyn = nullptr) {
   const int N = s->N;
   const long P = a->P;
   const int B = cfg->num_samples;
@@ -1493,8 +1710,11 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
   const int rounds = (B + C - 1) / C;
   CKR(s->d_samples.ensure(sizeof(int) * (size_t)B * N));
   std::vector<int> h_samples;
-  const bool need_host_samples = !s->device || gfn || hfn;
-  AmpSource esrc;
+  AmpSource esrc, rsrc;
+  const bool excited = dyn && dyn->kind == 1, tvmc = dyn && dyn->kind == 2;
+  const bool need_host_samples =
+      !s->device || gfn || hfn ||
+      (excited && !(recognise(fn, user, rsrc) && recognise(dyn->ref_fn, dyn->ref_user, rsrc)));
   const bool e_dev = recognise(fn, user, esrc) && esrc.a->N == N;
   if (need_host_samples || !e_dev) h_samples.resize((size_t)B * N);
   CKR(sampler_advance(s, k, rounds, B, s->d_samples.as<int>(),
@@ -1514,6 +1734,27 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
     CKR(eloc_callback(cfg, Lx, Ly, s->opt.periodic, h_samples.data(), B, fn, user, E, Ei,
                       d_samples));
   }
+  CKR(g_ws.scal.ensure(sizeof(double) * 16));
+  double* sc = g_ws.scal.as<double>();      // [0] <E_re> [1] <E_re^2> [2] <E_im> [3,4] u means
+  if (excited) {
+    // physical statistics first (they are what info reports), then
+    // r = psi_ref / psi, its means, the augmented energies and their means
+    // in closed form (sc[10], sc[11]), nqs_optimizer.c:605-657
+    CKR(nqs::launch_sum(E, B, nullptr, 1.0 / B, sc + 0, g_stream));
+    CKR(nqs::launch_sum(E, B, E, 1.0 / B, sc + 1, g_stream));
+    CKR(nqs::launch_sum(Ei, B, nullptr, 1.0 / B, sc + 2, g_stream));
+    CKR(g_ws.xr.ensure(sizeof(double) * 6 * (size_t)B));
+    double* xr = g_ws.xr.as<double>();
+    const int* hs = h_samples.empty() ? nullptr : h_samples.data();
+    CKR(logamp_samples(fn, user, N, d_samples, hs, B, xr, xr + B));
+    CKR(logamp_samples(dyn->ref_fn, dyn->ref_user, N, d_samples, hs, B, xr + 2 * B, xr + 3 * B));
+    CKR(nqs::launch_excited_ratio(xr, xr + B, xr + 2 * B, xr + 3 * B, B, xr + 4 * B, xr + 5 * B,
+                                  g_stream));
+    CKR(nqs::launch_sum(xr + 4 * B, B, nullptr, 1.0 / B, sc + 8, g_stream));
+    CKR(nqs::launch_sum(xr + 5 * B, B, nullptr, 1.0 / B, sc + 9, g_stream));
+    CKR(nqs::launch_excited_augment(E, Ei, xr + 4 * B, xr + 5 * B, sc + 8, dyn->mu, B, g_stream));
+    CKR(nqs::launch_excited_means(sc + 0, sc + 2, sc + 8, dyn->mu, sc + 10, g_stream));
+  }
   tm.mark();
   // MinSR path: sample space (T = O_c O_c^T / Ns + eps I, Ns x Ns) or, with
   // NQS_HIP_GRAM_AUTO when it costs less, parameter space
@@ -1524,7 +1765,7 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
   const bool rbm_like = (a->kind == NQS_ANSATZ_RBM || a->kind == NQS_ANSATZ_COMPLEX_RBM) && !gfn;
   const bool factored = method == SR_MINSR && rbm_like &&
                         (s->opt.gram == NQS_HIP_GRAM_FACTORED || s->opt.gram == NQS_HIP_GRAM_AUTO);
-  bool dual = false;
+  bool dual = method == SR_MINSR && s->opt.gram == NQS_HIP_GRAM_PARAMETER;
   if (method == SR_MINSR && s->opt.gram == NQS_HIP_GRAM_AUTO) {
     const double b = (double)B, p = (double)P;
     const double gram_b = factored ? b * b * (N + 2.0 * a->M) : b * b * p;
@@ -1578,10 +1819,8 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
   } else {
     CKR(g_ws.colwork.ensure(sizeof(double) * nqs::colsum_work_elems(B, P)));
   }
-  CKR(g_ws.scal.ensure(sizeof(double) * 16));
   CKR(g_ws.u1.ensure(sizeof(double) * B));
   CKR(g_ws.u2.ensure(sizeof(double) * B));
-  double* sc = g_ws.scal.as<double>();      // [0] <E_re> [1] <E_re^2> [2] <E_im> [3,4] u means
   double* vb = g_ws.vecs.as<double>();
   double* delta = vb;                        // [0, P)
   double* F = vb + P;                        // [P, 2P)
@@ -1590,15 +1829,28 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
   double* c1 = vb + 7 * P;
   double* c2 = vb + 8 * P;
   double* work = g_ws.colwork.as<double>();
-  CKR(nqs::launch_sum(E, B, nullptr, 1.0 / B, sc + 0, g_stream));
-  CKR(nqs::launch_sum(E, B, E, 1.0 / B, sc + 1, g_stream));
-  CKR(nqs::launch_sum(Ei, B, nullptr, 1.0 / B, sc + 2, g_stream));
+  if (!excited) {   // (the excited step summed the physical energies before augmenting)
+    CKR(nqs::launch_sum(E, B, nullptr, 1.0 / B, sc + 0, g_stream));
+    CKR(nqs::launch_sum(E, B, E, 1.0 / B, sc + 1, g_stream));
+    CKR(nqs::launch_sum(Ei, B, nullptr, 1.0 / B, sc + 2, g_stream));
+  }
   if (!ofree) CKR(nqs::launch_colsum(O, B, P, nullptr, 1.0 / B, Om, work, g_stream));
-  if (holo) {
+  if (holo && tvmc) {
+    // F = 2[(R^T E_im - I^T E_re)/Ns - <R><E_im> + <I><E_re>]
+    // (nqs_optimizer.c:797-810): second term and mean carry the minus sign
+    CKR(nqs::launch_colsum(Oi, B, P, nullptr, 1.0 / B, Im, work, g_stream));
+    CKR(nqs::launch_colsum(O, B, P, Ei, 1.0 / B, c1, work, g_stream));
+    CKR(nqs::launch_colsum(Oi, B, P, E, -1.0 / B, c2, work, g_stream));
+    CKR(nqs::launch_sum(E, B, nullptr, -1.0 / B, sc + 12, g_stream));
+    CKR(nqs::launch_qgt_finish(F, c1, Om, sc + 2, c2, Im, sc + 12, 2.0, 0.0, c1, P, g_stream));
+  } else if (holo) {
+    // holomorphic force; the excited step's with the augmented energies and
+    // their closed-form means
     CKR(nqs::launch_colsum(Oi, B, P, nullptr, 1.0 / B, Im, work, g_stream));
     CKR(nqs::launch_colsum(O, B, P, E, 1.0 / B, c1, work, g_stream));
     CKR(nqs::launch_colsum(Oi, B, P, Ei, 1.0 / B, c2, work, g_stream));
-    CKR(nqs::launch_qgt_finish(F, c1, Om, sc + 0, c2, Im, sc + 2, 2.0, 0.0, c1, P, g_stream));
+    CKR(nqs::launch_qgt_finish(F, c1, Om, sc + (excited ? 10 : 0), c2, Im, sc + (excited ? 11 : 2),
+                               2.0, 0.0, c1, P, g_stream));
   } else if (method == SR_CG) {
     CKR(nqs::launch_colsum(O, B, P, E, 1.0 / B, c1, work, g_stream));
     CKR(nqs::launch_qgt_finish(F, c1, Om, sc + 0, nullptr, nullptr, nullptr, 1.0, 0.0, c1, P,
@@ -1609,112 +1861,21 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
   int iters = 0, converged = 0;
   const double eps = cfg->sr_diag_shift;
   if (method == SR_MINSR) {
-    double* u = g_ws.u1.as<double>();
-    // eps_i = E_i - <E>, O_c = O - <O>  (nqs_optimizer.c:1119-1126)
-    CKR(nqs::launch_sub_scalar(u, E, B, sc + 0, g_stream));
-    if (!ofree) CKR(nqs::launch_center(O, B, P, Om, g_stream));
-    CKR(g_ws.diag.ensure(sizeof(double) * B));
-    CKR(g_ws.y.ensure(sizeof(double) * B * 2));
-    CKR(g_ws.fail.ensure(sizeof(int) * 4));
-    double* T = nullptr;
-    const int cpl = a->kind == NQS_ANSATZ_COMPLEX_RBM;
-    auto gram_factored = [&](double* out) -> int {
-      CKR(g_ws.Sd.ensure(sizeof(double) * (size_t)B * (N + 2)));
-      return nqs::launch_gram_factored(d_samples, B, N, g_ws.t_re.as<double>(),
-                                       cpl ? g_ws.t_im.as<double>() : nullptr, a->M, eps, out,
-                                       work, g_ws.Sd.as<double>(), g_ws.u2.as<double>(), sc + 6,
-                                       g_stream);
-    };
-    if (dual) {
-      // parameter-space MinSR: delta = S^-1 (O_c^T eps / Ns)
-      CKR(g_ws.T.ensure(sizeof(double) * (size_t)P * P));
-      CKR(g_ws.T2.ensure(sizeof(double) * (size_t)P * B));
-      T = g_ws.T.as<double>();
-      double* OT = g_ws.T2.as<double>();
-      double* rhs = c1;
-      CKR(nqs::launch_transpose(O, B, P, OT, g_stream));
-      CKR(nqs::launch_gemm_nt((int)P, (int)P, B, 1.0 / B, OT, B, OT, B, 0.0, T, P, 1, g_stream));
-      CKR(nqs::launch_add_diag(T, (int)P, P, eps, g_stream));
-      CKR(nqs::launch_colsum(O, B, P, u, 1.0 / B, rhs, work, g_stream));
-      CKR(g_ws.linv.ensure(sizeof(double) * nqs::potrf_linv_elems((int)P)));
-      CKR(g_ws.y.ensure(sizeof(double) * 2 * (size_t)P));
-      double* Linv = g_ws.linv.as<double>();
-      int f = nqs::launch_potrf(T, (int)P, Linv, g_ws.fail.as<int>(), g_stream);
-      if (f < 0) return -1;
-      if (f > 0) {
-        // as in the sample-space solve: the partially factored matrix with
-        // +1e-6 on its diagonal is factored once more (input columns from a
-        // second evaluation of S)
-        CKR(g_ws.Oi.ensure(sizeof(double) * (size_t)P * P));
-        double* S2 = g_ws.Oi.as<double>();
-        CKR(nqs::launch_gemm_nt((int)P, (int)P, B, 1.0 / B, OT, B, OT, B, 0.0, S2, P, 1, g_stream));
-        CKR(nqs::launch_add_diag(S2, (int)P, P, eps, g_stream));
-        CKR(nqs::launch_potrf_repair(T, (int)P, f - 1, S2, g_stream));
-        CKR(nqs::launch_add_diag(T, (int)P, P, 1e-6, g_stream));
-        f = nqs::launch_potrf(T, (int)P, Linv, g_ws.fail.as<int>(), g_stream);
-        if (f < 0) return -1;
-      }
-      if (f == 0) {
-        CKR(nqs::launch_potrs(T, Linv, (int)P, rhs, delta, g_ws.y.as<double>(), g_stream));
-        converged = 1;
-      } else {
-        fprintf(stderr, "nqs_sr_step_minsr: Cholesky breakdown after retry; increase "
-                        "sr_diag_shift or num_samples. Update set to 0.\n");
-        CKR(nqs::launch_fill(delta, 0.0, P, g_stream));
-      }
-    } else if (factored) {
-      CKR(g_ws.T.ensure(sizeof(double) * (size_t)B * B));
-      T = g_ws.T.as<double>();
-      CKR(gram_factored(T));
-    } else {
-      CKR(g_ws.T.ensure(sizeof(double) * (size_t)B * B));
-      T = g_ws.T.as<double>();
-      CKR(nqs::launch_gemm_nt(B, B, (int)P, 1.0 / B, O, P, O, P, 0.0, T, B, 1, g_stream));
-      CKR(nqs::launch_add_diag(T, B, B, eps, g_stream));
-    }
-    if (!dual) {
-    CKR(g_ws.linv.ensure(sizeof(double) * nqs::potrf_linv_elems(B)));
-    double* Linv = g_ws.linv.as<double>();
-    int f = nqs::launch_potrf(T, B, Linv, g_ws.fail.as<int>(), g_stream);
-    if (f < 0) return -1;
-    if (f > 0) {
-      // the reference retries on its partially factored matrix with +1e-6 on
-      // the diagonal (nqs_optimizer.c:1156-1160): rebuild that matrix (input
-      // columns from a second Gram evaluation) and factor again
-      CKR(g_ws.T2.ensure(sizeof(double) * (size_t)B * B));
-      double* T2 = g_ws.T2.as<double>();
-      if (factored) {
-        CKR(gram_factored(T2));
-      } else {
-        CKR(nqs::launch_gemm_nt(B, B, (int)P, 1.0 / B, O, P, O, P, 0.0, T2, B, 1, g_stream));
-        CKR(nqs::launch_add_diag(T2, B, B, eps, g_stream));
-      }
-      CKR(nqs::launch_potrf_repair(T, B, f - 1, T2, g_stream));
-      CKR(nqs::launch_add_diag(T, B, B, 1e-6, g_stream));
-      f = nqs::launch_potrf(T, B, Linv, g_ws.fail.as<int>(), g_stream);
-      if (f < 0) return -1;
-    }
-    double* y = g_ws.y.as<double>();
-    if (f == 0) {
-      CKR(nqs::launch_potrs(T, Linv, B, u, y, y + B, g_stream));
-      converged = 1;
-    } else {
-      fprintf(stderr, "nqs_sr_step_minsr: Cholesky breakdown after retry; increase "
-                      "sr_diag_shift or num_samples. Update set to 0.\n");
-      CKR(nqs::launch_fill(y, 0.0, B, g_stream));
-    }
-    // delta_k = (1/Ns) sum_i O_c[i][k] y_i = (1/Ns) sum_i O[i][k] (y_i - mean y)
-    if (ofree) {
-      double* yc = y + B;
-      CKR(nqs::launch_sum(y, B, nullptr, 1.0 / B, sc + 7, g_stream));
-      CKR(nqs::launch_sub_scalar(yc, y, B, sc + 7, g_stream));
-      CKR(nqs::launch_factored_delta(d_samples, B, N, a->M, g_ws.t_re.as<double>(),
-                                     cpl ? g_ws.t_im.as<double>() : nullptr, yc, 1.0 / B, work,
-                                     delta, g_stream));
-    } else {
-      CKR(nqs::launch_colsum(O, B, P, y, 1.0 / B, delta, work, g_stream));
-    }
-    }
+    MinsrArgs m;
+    m.a = a;
+    m.d_samples = d_samples;
+    m.B = B;
+    m.O = O;
+    m.Om = Om;
+    m.E = E;
+    m.sc = sc;
+    m.work = work;
+    m.rhs = c1;
+    m.eps = eps;
+    m.factored = factored;
+    m.dual = dual;
+    m.delta = delta;
+    CKR(minsr_core(m, converged));
   } else {
     double* u1 = g_ws.u1.as<double>();
     double* u2 = g_ws.u2.as<double>();
@@ -1743,9 +1904,14 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
   }
   tm.mark();
   // ---- 6. update -----------------------------------------------------------------
-  CKR(nqs::launch_axpy(a->d_par.as<double>(), delta, -cfg->learning_rate, P, g_stream));
-  a->m64.valid = 0;
-  a->m32.valid = 0;
+  const double step = tvmc ? dyn->dt : -cfg->learning_rate;
+  if (!dyn || dyn->apply) {
+    CKR(nqs::launch_axpy(a->d_par.as<double>(), delta, step, P, g_stream));
+    a->m64.valid = 0;
+    a->m32.valid = 0;
+  }
+  if (dyn && dyn->keep_delta)
+    CK(hipMemcpyAsync(dyn->keep_delta, delta, sizeof(double) * P, hipMemcpyDeviceToDevice, g_stream));
   CKR(nqs::launch_sum(delta, P, delta, 1.0, sc + 5, g_stream));
   // scalars, parameters and energies through one pinned staging buffer:
   // three copies, one synchronisation
@@ -1770,7 +1936,7 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
   if (info) {
     info->mean_energy = hs[0];
     info->variance_energy = hs[1] - hs[0] * hs[0];
-    info->update_norm = sqrt(hs[5]) * cfg->learning_rate;
+    info->update_norm = sqrt(hs[5]) * fabs(step);
     info->acceptance_ratio = nqs_sampler_acceptance_ratio(s);
     info->cg_iterations = (method == SR_MINSR) ? 0 : iters;
     info->converged = converged;
@@ -1785,16 +1951,20 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
 
 int sr_run_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz* a, nqs_sampler* s,
                 nqs_log_amp_fn_t fn, void* user, nqs_gradient_fn_t gfn,
-                nqs_holomorphic_gradient_fn_t hfn, void* guser, double* trace) {
+                nqs_holomorphic_gradient_fn_t hfn, void* guser, double* trace,
+                const DynStep* dyn = nullptr) {
   // a host callback may read the parameters between steps: keep the host
   // copy authoritative then
   AmpSource src;
-  const bool pure_device = s->device && (!fn || recognise(fn, user, src)) && !gfn && !hfn;
+  AmpSource rsrc;
+  const bool pure_device = s->device && (!fn || recognise(fn, user, src)) && !gfn && !hfn &&
+                           (!dyn || dyn->kind != 1 || recognise(dyn->ref_fn, dyn->ref_user, rsrc));
   a->dev_fresh = 0;
   sampler_advance(s, s->cfg.num_thermalize, s->cfg.num_thermalize > 0 ? 1 : 0, 0, nullptr, nullptr);
   for (int it = 0; it < cfg->num_iterations; it++) {
     nqs_sr_step_info_t info;
-    int rc = sr_step_impl(method, cfg, Lx, Ly, a, s, fn, user, gfn, hfn, guser, &info, pure_device);
+    int rc = sr_step_impl(method, cfg, Lx, Ly, a, s, fn, user, gfn, hfn, guser, &info, pure_device,
+                          dyn);
     if (rc) {
       a->dev_fresh = 0;
       return rc;
@@ -1916,6 +2086,116 @@ NQS_EXPORT int nqs_sr_run_minsr(const nqs_config_t* cfg, int size_x, int size_y,
                      nullptr, nullptr, nullptr, out_energy_trace);
 }
 
+NQS_EXPORT int nqs_sr_step_excited(const nqs_config_t* cfg, int size_x, int size_y,
+                                   nqs_ansatz_t* ansatz, nqs_sampler_t* sampler,
+                                   nqs_log_amp_fn_t log_amp_fn, void* log_amp_user,
+                                   nqs_log_amp_fn_t ref_log_amp_fn, void* ref_log_amp_user,
+                                   double penalty_mu, nqs_sr_step_info_t* out_info) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !ansatz || !sampler || !ref_log_amp_fn) return -1;
+  DynStep d;
+  d.kind = 1;
+  d.mu = penalty_mu;
+  d.ref_fn = ref_log_amp_fn;
+  d.ref_user = ref_log_amp_user;
+  return sr_step_impl(SR_HOLO, cfg, size_x, size_y, ansatz, sampler, log_amp_fn, log_amp_user,
+                      nullptr, nullptr, nullptr, out_info, false, &d);
+}
+
+NQS_EXPORT int nqs_sr_run_excited(const nqs_config_t* cfg, int size_x, int size_y,
+                                  nqs_ansatz_t* ansatz, nqs_sampler_t* sampler,
+                                  nqs_log_amp_fn_t log_amp_fn, void* log_amp_user,
+                                  nqs_log_amp_fn_t ref_log_amp_fn, void* ref_log_amp_user,
+                                  double penalty_mu, double* out_energy_trace) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !ansatz || !sampler || !ref_log_amp_fn) return -1;
+  DynStep d;
+  d.kind = 1;
+  d.mu = penalty_mu;
+  d.ref_fn = ref_log_amp_fn;
+  d.ref_user = ref_log_amp_user;
+  return sr_run_impl(SR_HOLO, cfg, size_x, size_y, ansatz, sampler, log_amp_fn, log_amp_user,
+                     nullptr, nullptr, nullptr, out_energy_trace, &d);
+}
+
+NQS_EXPORT int nqs_tvmc_step_real_time(const nqs_config_t* cfg, double dt, int size_x, int size_y,
+                                       nqs_ansatz_t* ansatz, nqs_sampler_t* sampler,
+                                       nqs_log_amp_fn_t log_amp_fn, void* log_amp_user,
+                                       nqs_sr_step_info_t* out_info) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !ansatz || !sampler) return -1;
+  DynStep d;
+  d.kind = 2;
+  d.dt = dt;
+  return sr_step_impl(SR_HOLO, cfg, size_x, size_y, ansatz, sampler, log_amp_fn, log_amp_user,
+                      nullptr, nullptr, nullptr, out_info, false, &d);
+}
+
+// Heun (nqs_optimizer.c:830-902): k1 = delta(theta) and k2 = delta(theta +
+// dt k1), each from its own batch, then theta <- theta + dt (k1 + k2) / 2.
+// The parameters stay on the device between the stages; the batch has the
+// sampler's site count (the reference sizes its Heun batch Lx * Ly, the
+// same number on square lattices).
+NQS_EXPORT int nqs_tvmc_step_heun(const nqs_config_t* cfg, double dt, int size_x, int size_y,
+                                  nqs_ansatz_t* ansatz, nqs_sampler_t* sampler,
+                                  nqs_log_amp_fn_t log_amp_fn, void* log_amp_user,
+                                  nqs_sr_step_info_t* out_info) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !ansatz || !sampler) return -1;
+  nqs_ansatz* a = ansatz;
+  const long P = a->P;
+  if (P <= 0 || stream_ready()) return -1;
+  CKR(g_ws.heun.ensure(sizeof(double) * 3 * (size_t)P));
+  double* th0 = g_ws.heun.as<double>();
+  double* k1 = th0 + P;
+  double* k2 = th0 + 2 * P;
+  a->dev_fresh = 0;
+  CKR(ansatz_upload(a));
+  a->dev_fresh = 1;
+  CK(hipMemcpyAsync(th0, a->d_par.p, sizeof(double) * P, hipMemcpyDeviceToDevice, g_stream));
+  DynStep d;
+  d.kind = 2;
+  d.dt = dt;
+  d.apply = false;
+  d.keep_delta = k1;
+  nqs_sr_step_info_t i1, i2;
+  int rc = sr_step_impl(SR_HOLO, cfg, size_x, size_y, a, sampler, log_amp_fn, log_amp_user,
+                        nullptr, nullptr, nullptr, &i1, true, &d);
+  if (rc == 0) {
+    CKR(nqs::launch_axpy(a->d_par.as<double>(), k1, dt, P, g_stream));
+    a->m64.valid = 0;
+    a->m32.valid = 0;
+    d.keep_delta = k2;
+    rc = sr_step_impl(SR_HOLO, cfg, size_x, size_y, a, sampler, log_amp_fn, log_amp_user, nullptr,
+                      nullptr, nullptr, &i2, true, &d);
+  }
+  // theta0 back, then + dt (k1 + k2) / 2 (k1 + k2 formed first, scaled by
+  // dt / 2: the same rounding as dt ((k1 + k2) / 2))
+  CK(hipMemcpyAsync(a->d_par.p, th0, sizeof(double) * P, hipMemcpyDeviceToDevice, g_stream));
+  a->m64.valid = 0;
+  a->m32.valid = 0;
+  if (rc == 0) {
+    CKR(nqs::launch_axpy(k1, k2, 1.0, P, g_stream));
+    CKR(nqs::launch_axpy(a->d_par.as<double>(), k1, 0.5 * dt, P, g_stream));
+    double* sc = g_ws.scal.as<double>();
+    CKR(nqs::launch_sum(k1, P, k1, 1.0, sc + 13, g_stream));
+    double n2 = 0.0;
+    CK(hipMemcpyAsync(&n2, sc + 13, sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    CKR(ansatz_download(a));
+    if (out_info) {
+      *out_info = i1;
+      out_info->update_norm = 0.5 * sqrt(n2) * fabs(dt);
+      out_info->acceptance_ratio = nqs_sampler_acceptance_ratio(sampler);
+      out_info->cg_iterations = i1.cg_iterations + i2.cg_iterations;
+      out_info->converged = i1.converged && i2.converged;
+    }
+  } else {
+    (void)ansatz_download(a);
+  }
+  a->dev_fresh = 0;
+  return rc;
+}
+
 // ===========================================================================
 // extensions
 // ===========================================================================
@@ -2001,6 +2281,84 @@ NQS_EXPORT int nqs_hip_cholesky_solve(double* T, int n, const double* b, double*
   CK(hipMemcpyAsync(x, y, sizeof(double) * n, hipMemcpyDeviceToHost, g_stream));
   CK(hipStreamSynchronize(g_stream));
   return f == 0 ? 1 : 0;
+}
+
+NQS_EXPORT int nqs_hip_minsr_solve(nqs_ansatz_t* a, const int* spins, int B, const double* e_loc,
+                                   double shift, int gram, double* out_delta, double* out_y,
+                                   double* out_T) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !spins || !e_loc || !out_delta || B <= 0) return -1;
+  const bool rbm_like = a->kind == NQS_ANSATZ_RBM || a->kind == NQS_ANSATZ_COMPLEX_RBM;
+  if (gram != NQS_HIP_GRAM_DIRECT && gram != NQS_HIP_GRAM_FACTORED &&
+      gram != NQS_HIP_GRAM_PARAMETER) {
+    fprintf(stderr, "nqs_hip_minsr_solve: gram must be DIRECT, FACTORED or PARAMETER\n");
+    return -1;
+  }
+  if ((gram == NQS_HIP_GRAM_FACTORED && !rbm_like) ||
+      (gram == NQS_HIP_GRAM_PARAMETER && (out_y || out_T))) {
+    fprintf(stderr, "nqs_hip_minsr_solve: unsupported combination (factored Gram needs an RBM; "
+                    "y and T exist in sample space only)\n");
+    return -1;
+  }
+  if (stream_ready()) return -1;
+  const int N = a->N;
+  const long P = a->P;
+  const bool factored = gram == NQS_HIP_GRAM_FACTORED, dual = gram == NQS_HIP_GRAM_PARAMETER;
+  a->dev_fresh = 0;
+  CKR(ansatz_upload(a));
+  CKR(upload_spins(spins, (long)B * N));
+  const int* d_samples = g_ws.spins.as<int>();
+  CKR(g_ws.e_re.ensure(sizeof(double) * B));
+  double* E = g_ws.e_re.as<double>();
+  CK(hipMemcpyAsync(E, e_loc, sizeof(double) * B, hipMemcpyHostToDevice, g_stream));
+  double* O = nullptr;
+  if (factored) {
+    CKR(forward_t<double>(a, d_samples, B, nullptr, nullptr));
+  } else {
+    CKR(g_ws.O.ensure(sizeof(double) * (size_t)B * (size_t)P));
+    O = g_ws.O.as<double>();
+    CKR(forward_t<double>(a, d_samples, B, nullptr, nullptr));
+    ModelView<double> v;
+    CKR(model_view<double>(a, v));
+    CKR(nqs::launch_gen_O<double>(v, a->kind, d_samples, B, g_ws.t_re.as<double>(),
+                                  g_ws.t_im.as<double>(), 0, O, nullptr, P, g_stream));
+  }
+  CKR(g_ws.vecs.ensure(sizeof(double) * 9 * (size_t)P));
+  if (factored) {
+    const int nblk = a->kind == NQS_ANSATZ_COMPLEX_RBM ? 2 : 1;
+    CKR(g_ws.colwork.ensure(sizeof(double) * nqs::gram_factored_work_elems(B, N, a->M, nblk)));
+  } else {
+    CKR(g_ws.colwork.ensure(sizeof(double) * nqs::colsum_work_elems(B, P)));
+  }
+  CKR(g_ws.scal.ensure(sizeof(double) * 16));
+  CKR(g_ws.u1.ensure(sizeof(double) * B));
+  CKR(g_ws.u2.ensure(sizeof(double) * B));
+  double* sc = g_ws.scal.as<double>();
+  double* vb = g_ws.vecs.as<double>();
+  double* work = g_ws.colwork.as<double>();
+  CKR(nqs::launch_sum(E, B, nullptr, 1.0 / B, sc + 0, g_stream));
+  if (O) CKR(nqs::launch_colsum(O, B, P, nullptr, 1.0 / B, vb + 5 * P, work, g_stream));
+  MinsrArgs m;
+  m.a = a;
+  m.d_samples = d_samples;
+  m.B = B;
+  m.O = O;
+  m.Om = vb + 5 * P;
+  m.E = E;
+  m.sc = sc;
+  m.work = work;
+  m.rhs = vb + 7 * P;
+  m.eps = shift;
+  m.factored = factored;
+  m.dual = dual;
+  m.delta = vb;
+  m.h_T = out_T;
+  m.h_y = out_y;
+  int converged = 0;
+  CKR(minsr_core(m, converged));
+  CK(hipMemcpyAsync(out_delta, vb, sizeof(double) * P, hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return converged ? 1 : 0;
 }
 
 // Streaming copy of `bytes` bytes between two device buffers, `reps` times;
@@ -2294,8 +2652,11 @@ int multi_set_split(nqs_hip_multi* m, int target) {
     m->mp[i].nsplit = ns;
   }
   nqs::multi_layout(m->mp.data(), (int)m->mp.size(), m->mb);
+  // the host table is rewritten by the next call (the autotuner sets several
+  // targets in a row): the copy must have read it before this returns
   CK(hipMemcpyAsync(m->d_mp.p, m->mp.data(), sizeof(nqs::MultiProb) * m->mp.size(),
                     hipMemcpyHostToDevice, g_stream));
+  CK(hipStreamSynchronize(g_stream));
   m->mb.probs = m->d_mp.as<nqs::MultiProb>();
   m->target = target;
   return 0;
@@ -2347,6 +2708,7 @@ int multi_build_jobs(nqs_hip_multi* m) {
   CKR(m->d_jobs.ensure(sizeof(nqs::MHJob) * m->jobs.size()));
   CK(hipMemcpyAsync(m->d_jobs.p, m->jobs.data(), sizeof(nqs::MHJob) * m->jobs.size(),
                     hipMemcpyHostToDevice, g_stream));
+  CK(hipStreamSynchronize(g_stream));
   return 0;
 }
 
@@ -2357,8 +2719,9 @@ int multi_issue_front(nqs_hip_multi* m) {
   CKR(multi_fork(m));
   for (size_t i = 0; i < m->p.size(); i++) {
     nqs_ansatz* a = m->p[i].a;
+    // inside the captured graph: the range verdict is read after the step
     CKR(nqs::launch_prep<double>(a->d_par.as<double>(), a->kind, a->N, a->M, a->m64,
-                                 m->side[i % nqs_hip_multi::kSide]));
+                                 m->side[i % nqs_hip_multi::kSide], 0));
   }
   CKR(multi_join(m));
   if (m->jobs.empty()) CKR(multi_build_jobs(m));
@@ -2376,9 +2739,11 @@ int multi_issue_front(nqs_hip_multi* m) {
     const int* smp = q.s->d_samples.as<int>();
     const size_t tb = (size_t)q.B * a->M;
     double* t = q.t.as<double>();
-    CKR(nqs::launch_forward<double>(v, smp, q.B, t, t + tb, nullptr, nullptr, st));
+    // real RBM: t in the first half, the small logistic members in the second
+    CKR(nqs::launch_forward<double>(v, smp, q.B, t, nullptr, nullptr, nullptr, st, t + tb,
+                                    nullptr));
     double* E = q.E.as<double>();
-    CKR(nqs::launch_eloc<double>(v, q.H, smp, q.B, t, t + tb, E, E + q.B, st));
+    CKR(nqs::launch_eloc<double>(v, q.H, smp, q.B, t, nullptr, t + tb, nullptr, E, E + q.B, st));
     double* stats = q.stats.as<double>();
     CKR(nqs::launch_sum(E, q.B, nullptr, 1.0 / q.B, stats + 0, st));
     CKR(nqs::launch_sum(E, q.B, E, 1.0 / q.B, stats + 1, st));
@@ -2502,6 +2867,15 @@ int multi_step(nqs_hip_multi* m, nqs_sr_step_info_t* infos) {
   m->step_ms = hipEventElapsedTime(&tms, m->ev_t0, m->ev_t1) == hipSuccess ? tms : -1.0;
   multi_fill_info(m, infos);
   for (nqs_hip_multi::Prob& q : m->p) q.a->m64.valid = 0;
+  // weights beyond the ratio tables' range (flagged by the tables of this
+  // step): the step's moves and energies are not trusted
+  for (size_t i = 0; i < m->p.size(); i++)
+    if (m->h_out[i * nqs::kMultiOut + 10] != 0.0) {
+      fprintf(stderr, "nqs_hip_multi_step: problem %zu has a weight |Re W| beyond the range of "
+                      "the FP64 amplitude-ratio tables (|Re W| <= %g); the step is refused\n", i,
+              nqs::RatioRange<double>::max_w);
+      return -1;
+    }
   m->steps_done++;
   return 0;
 }
@@ -2566,7 +2940,10 @@ nqs_hip_multi* multi_new(int n, const nqs_config_t* cfgs, const int* size_x, con
     a->dev_fresh = 0;
     if (ansatz_upload(a)) return fail(i, "parameter upload");
     a->dev_fresh = 1;
+    ModelView<double> mv0;
+    if (ratio_view<double>(a, mv0)) return fail(i, "amplitude-ratio tables");
     nqs::MultiProb& mp = m->mp[i];
+    mp.range_flag = a->m64.range_flag;
     mp.B = q.B;
     mp.N = N;
     mp.M = M;

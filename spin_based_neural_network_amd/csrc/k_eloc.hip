@@ -4,17 +4,22 @@
 // Heisenberg :87-139, J1-J2 :146-205, kagome :374-445, real path = real
 // part of the complex path :495-875).  The reference re-evaluates
 // log psi(s') from scratch for every connected configuration (:24-45);
-// here each connected configuration is priced from the sample's
-// tanh(theta) and the tables C = cosh(2W), S = sinh(2W):
-//   psi(s')/psi(s) = exp(sum_flipped -2 a_i s_i) * prod_h [cosh d_h + t_h sinh d_h]
-// which needs no transcendental per hidden unit.
+// here each connected configuration is priced from the sample's logistic
+// pair (p, q) of every hidden input theta (logistic_small, nqs_common.h) and
+// the tables EP = e^{2W}, EM = e^{-2W}:
+//   psi(s')/psi(s) = exp(sum_flipped -2 a_i s_i) * prod_h [p_h e^{d_h} + q_h e^{-d_h}]
+// with d_h the change of theta_h.  For a real RBM both terms of every factor
+// are positive, so no factor loses digits however large |theta| is (the
+// tanh form cosh d + tanh(theta) sinh d cancels once tanh rounds to +-1), and
+// no transcendental is evaluated per hidden unit.  Running products are
+// rescaled by powers of two every RatioRange<R>::rescale units.
 //
 // Work decomposition: a workgroup takes SB consecutive samples, compacts the
 // active (sample, term) pairs (all flips; pairs with s_a != s_b) into a
 // sample-major list in LDS, and gives one lane per active pair.  The hidden
 // dimension is walked in tiles of HT units whose table rows (HT x N) and
-// t values (SB x HT) are staged in LDS; each lane keeps a running product
-// renormalised once per tile.  Per-sample sums run in a fixed order.
+// logistic members (SB x HT) are staged in LDS.  Per-sample sums run in a
+// fixed order.
 #include "nqs_kernels.h"
 
 namespace nqs {
@@ -36,16 +41,38 @@ __device__ inline int block_exclusive_scan(int v, int* sh, int& total) {
   return incl - v;
 }
 
+// p and q of one hidden unit from the staged sign of Re t and small member
+template <typename R, bool CPLX>
+__device__ inline void pq_of(R t_sign, R sr, R si, R& pr, R& pi, R& qr, R& qi) {
+  logistic_unpack<R>(sr, CPLX ? si : (R)0, t_sign < (R)0 ? 1 : 0, pr, pi, qr, qi);
+}
+
+// running product rescaled by a power of two (exact), exponent accumulated
+template <typename R, bool CPLX>
+__device__ inline void rescale2(R& pr, R& pi, int& ex) {
+  int e2;
+  if (CPLX) {
+    (void)frexp(fmax(fabs(pr), fabs(pi)), &e2);
+    pr = ldexp(pr, -e2);
+    pi = ldexp(pi, -e2);
+  } else {
+    pr = frexp(pr, &e2);
+  }
+  ex += e2;
+}
+
 template <typename R, bool CPLX, int IPT>
 __global__ __launch_bounds__(256) void k_eloc(ModelView<R> m, HamDev H,
                                               const int* __restrict__ spins, int B,
                                               const R* __restrict__ t_re,
-                                              const R* __restrict__ t_im,
+                                              const R* __restrict__ q_re,
+                                              const R* __restrict__ q_im,
                                               double* __restrict__ e_re,
                                               double* __restrict__ e_im, int SB, int HT) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int scan_sh[256];
   __shared__ int n_items_sh;
+  constexpr int RS = RatioRange<R>::rescale;
   const int N = m.N, M = m.M;
   const int T = H.n_flip + H.n_pair;
   const int tid = threadIdx.x;
@@ -55,12 +82,13 @@ __global__ __launch_bounds__(256) void k_eloc(ModelView<R> m, HamDev H,
   signed char* sp = (signed char*)carve((size_t)SB * N);
   int* items = (int*)carve(sizeof(int) * (size_t)SB * T);
   int* starts = (int*)carve(sizeof(int) * (SB + 1));
-  R* cr = (R*)carve(sizeof(R) * (size_t)HT * N);
-  R* sr = (R*)carve(sizeof(R) * (size_t)HT * N);
-  R* ci = CPLX ? (R*)carve(sizeof(R) * (size_t)HT * N) : nullptr;
-  R* si = CPLX ? (R*)carve(sizeof(R) * (size_t)HT * N) : nullptr;
-  R* ttr = (R*)carve(sizeof(R) * (size_t)SB * HT);
-  R* tti = CPLX ? (R*)carve(sizeof(R) * (size_t)SB * HT) : nullptr;
+  R* epr = (R*)carve(sizeof(R) * (size_t)HT * N);
+  R* emr = (R*)carve(sizeof(R) * (size_t)HT * N);
+  R* epi = CPLX ? (R*)carve(sizeof(R) * (size_t)HT * N) : nullptr;
+  R* emi = CPLX ? (R*)carve(sizeof(R) * (size_t)HT * N) : nullptr;
+  R* tsg = (R*)carve(sizeof(R) * (size_t)SB * HT);
+  R* lqr = (R*)carve(sizeof(R) * (size_t)SB * HT);
+  R* lqi = CPLX ? (R*)carve(sizeof(R) * (size_t)SB * HT) : nullptr;
   double* res_r = (double*)carve(sizeof(double) * (size_t)SB * T);
   double* res_i = (double*)carve(sizeof(double) * (size_t)SB * T);
 
@@ -92,6 +120,9 @@ __global__ __launch_bounds__(256) void k_eloc(ModelView<R> m, HamDev H,
       }
       base += total;
     }
+    // every wave's counts must be in starts before thread 0 turns them into
+    // offsets
+    __syncthreads();
     if (tid == 0) {
       n_items_sh = base;
       for (int s = 0; s < nsb; s++) starts[s + 1] += starts[s];
@@ -100,12 +131,12 @@ __global__ __launch_bounds__(256) void k_eloc(ModelView<R> m, HamDev H,
     const int n_items = n_items_sh;
 
     // ---- per-item state ----
-    R Lm[IPT], Pr[IPT], Pi[IPT], sgn[IPT];
-    int isb[IPT], ia[IPT], ib[IPT];
+    R Pr[IPT], Pi[IPT], sgn[IPT];
+    int Ex[IPT], isb[IPT], ia[IPT], ib[IPT];
 #pragma unroll
     for (int q = 0; q < IPT; q++) {
       const int idx = tid + q * (int)blockDim.x;
-      Lm[q] = 0; Pr[q] = 1; Pi[q] = 0; sgn[q] = 1; isb[q] = -1; ia[q] = 0; ib[q] = -1;
+      Pr[q] = 1; Pi[q] = 0; Ex[q] = 0; sgn[q] = 1; isb[q] = -1; ia[q] = 0; ib[q] = -1;
       if (idx < n_items) {
         const int it = items[idx];
         const int sb = it >> 16, t = it & 0xFFFF;
@@ -123,86 +154,90 @@ __global__ __launch_bounds__(256) void k_eloc(ModelView<R> m, HamDev H,
       // flight at once (indices clamped into the tile, stores guarded)
       const int tn = hn * N;
       for (int e0 = 0; e0 < tn; e0 += 8 * 256) {
-        R vcr[8], vsr[8], vci[8], vsi[8];
+        R vpr[8], vmr[8], vpi[8], vmi[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
           const int e = min(e0 + tid + 256 * u, tn - 1);
-          vcr[u] = m.C_re[toff + e];
-          vsr[u] = m.S_re[toff + e];
-          if (CPLX) { vci[u] = m.C_im[toff + e]; vsi[u] = m.S_im[toff + e]; }
+          vpr[u] = m.EP_re[toff + e];
+          vmr[u] = m.EM_re[toff + e];
+          if (CPLX) { vpi[u] = m.EP_im[toff + e]; vmi[u] = m.EM_im[toff + e]; }
         }
 #pragma unroll
         for (int u = 0; u < 8; u++) {
           const int e = e0 + tid + 256 * u;
           if (e < tn) {
-            cr[e] = vcr[u];
-            sr[e] = vsr[u];
-            if (CPLX) { ci[e] = vci[u]; si[e] = vsi[u]; }
+            epr[e] = vpr[u];
+            emr[e] = vmr[u];
+            if (CPLX) { epi[e] = vpi[u]; emi[e] = vmi[u]; }
           }
         }
       }
       for (int e = tid; e < nsb * hn; e += blockDim.x) {
         const int s = e / hn, hh = e - s * hn;
         const size_t g = (size_t)(b0 + s) * M + h0 + hh;
-        ttr[s * HT + hh] = t_re[g];
-        if (CPLX) tti[s * HT + hh] = t_im[g];
+        tsg[s * HT + hh] = t_re[g];
+        lqr[s * HT + hh] = q_re[g];
+        if (CPLX) lqi[s * HT + hh] = q_im[g];
       }
       __syncthreads();
 #pragma unroll
       for (int q = 0; q < IPT; q++) {
         if (isb[q] < 0) continue;
         const int a = ia[q], bb = ib[q];
-        const R s_a = sgn[q];
-        const R* tt = ttr + isb[q] * HT;
-        const R* ttI = CPLX ? tti + isb[q] * HT : nullptr;
+        const bool up = sgn[q] > (R)0;
+        const R* ts = tsg + isb[q] * HT;
+        const R* lr = lqr + isb[q] * HT;
+        const R* li = CPLX ? lqi + isb[q] * HT : nullptr;
         R pr = Pr[q], pim = Pi[q];
+        int ex = Ex[q];
         for (int hh = 0; hh < hn; hh++) {
           const int ea = hh * N + a;
-          R chr, chi = 0, shr, shi = 0;
-          if (bb < 0) {
-            chr = cr[ea];
-            shr = -s_a * sr[ea];
-            if (CPLX) { chi = ci[ea]; shi = -s_a * si[ea]; }
-          } else {
-            const int eb = hh * N + bb;
-            if (CPLX) {
-              const R car = cr[ea], cai = ci[ea], sar = sr[ea], sai = si[ea];
-              const R cbr = cr[eb], cbi = ci[eb], sbr = sr[eb], sbi = si[eb];
-              chr = (car * cbr - cai * cbi) - (sar * sbr - sai * sbi);
-              chi = (car * cbi + cai * cbr) - (sar * sbi + sai * sbr);
-              const R xr = (sar * cbr - sai * cbi) - (car * sbr - cai * sbi);
-              const R xi = (sar * cbi + sai * cbr) - (car * sbi + cai * sbr);
-              shr = -s_a * xr;
-              shi = -s_a * xi;
-            } else {
-              const R car = cr[ea], sar = sr[ea], cbr = cr[eb], sbr = sr[eb];
-              chr = car * cbr - sar * sbr;
-              shr = -s_a * (sar * cbr - car * sbr);
+          // e^{d} (xr, xi) and e^{-d} (yr, yi) of this unit for the term
+          R xr, xi = 0, yr, yi = 0;
+          {
+            const R par_ = epr[ea], mar = emr[ea];
+            const R pai = CPLX ? epi[ea] : (R)0, mai = CPLX ? emi[ea] : (R)0;
+            // site a alone: s_a = +1 gives e^{d} = EM_a, e^{-d} = EP_a
+            R dr = up ? mar : par_, di = up ? mai : pai;
+            R er = up ? par_ : mar, ei = up ? pai : mai;
+            if (bb >= 0) {
+              // second site with s_b = -s_a: e^{d} *= (s_a = +1 ? EP_b : EM_b)
+              const int eb = hh * N + bb;
+              const R pbr = epr[eb], mbr = emr[eb];
+              const R pbi = CPLX ? epi[eb] : (R)0, mbi = CPLX ? emi[eb] : (R)0;
+              const R fr = up ? pbr : mbr, fi = up ? pbi : mbi;
+              const R gr = up ? mbr : pbr, gi = up ? mbi : pbi;
+              if (CPLX) {
+                const R nr = dr * fr - di * fi;
+                di = dr * fi + di * fr;
+                dr = nr;
+                const R mr2 = er * gr - ei * gi;
+                ei = er * gi + ei * gr;
+                er = mr2;
+              } else {
+                dr *= fr;
+                er *= gr;
+              }
             }
+            xr = dr; xi = di; yr = er; yi = ei;
           }
+          R ppr, ppi, qqr, qqi;
+          pq_of<R, CPLX>(ts[hh], lr[hh], CPLX ? li[hh] : (R)0, ppr, ppi, qqr, qqi);
           if (CPLX) {
-            const R t_r = tt[hh], t_i = ttI[hh];
-            const R wr = chr + (t_r * shr - t_i * shi);
-            const R wi = chi + (t_r * shi + t_i * shr);
+            const R wr = (ppr * xr - ppi * xi) + (qqr * yr - qqi * yi);
+            const R wi = (ppr * xi + ppi * xr) + (qqr * yi + qqi * yr);
             const R nr = pr * wr - pim * wi;
             pim = pr * wi + pim * wr;
             pr = nr;
           } else {
-            pr *= chr + tt[hh] * shr;
+            pr *= ppr * xr + qqr * yr;
           }
+          if ((hh % RS) == RS - 1) rescale2<R, CPLX>(pr, pim, ex);
         }
-        // renormalise once per tile
-        if (CPLX) {
-          const R mag = sqrt(pr * pr + pim * pim);
-          Lm[q] += log(mag);
-          pr /= mag;
-          pim /= mag;
-        } else {
-          Lm[q] += log(pr);
-          pr = 1;
-        }
+        rescale2<R, CPLX>(pr, pim, ex);
         Pr[q] = pr;
         Pi[q] = pim;
+        Ex[q] = ex;
       }
       __syncthreads();
     }
@@ -222,8 +257,13 @@ __global__ __launch_bounds__(256) void k_eloc(ModelView<R> m, HamDev H,
         vr += 2.0 * (double)m.a_re[bb] * s_a;
         if (CPLX) vi += 2.0 * (double)m.a_im[bb] * s_a;
       }
-      double mag = exp((double)Lm[q] + vr);
       double ph_r = (double)Pr[q], ph_i = CPLX ? (double)Pi[q] : 0.0;
+      const double mg = CPLX ? sqrt(ph_r * ph_r + ph_i * ph_i) : fabs(ph_r);
+      // a zero factor (psi(s') = 0) contributes nothing
+      const double lm = mg > 0.0 ? log(mg) + (double)Ex[q] * 0.69314718055994530942 : -INFINITY;
+      const double mag = exp(lm + vr);
+      ph_r = mg > 0.0 ? ph_r / mg : 0.0;
+      ph_i = mg > 0.0 ? ph_i / mg : 0.0;
       if (CPLX) {
         const double c = cos(vi), s = sin(vi);
         const double nr = ph_r * c - ph_i * s;
@@ -261,61 +301,64 @@ __global__ __launch_bounds__(256) void k_eloc(ModelView<R> m, HamDev H,
 // ---------------------------------------------------------------------------
 // Term-major local energies for many samples.  The off-diagonal terms
 // (flips, then pairs, in the Hamiltonian's order) get hidden-unit rows of
-// their own once per call:
-//   flip of a:    Cq = C_a,                 Sq = S_a
-//   pair (a, b):  Cq = C_a C_b - S_a S_b,   Sq = S_a C_b - C_a S_b
-// (C = cosh 2W, S = sinh 2W, complex products for complex RBMs), so that the
-// factor of hidden unit h in psi(s')/psi(s) is Cq + t_h (-s_a) Sq.  A wave
-// then holds 64 samples in its lanes and 8 terms in registers; per tile of
-// hidden units, the samples' t_h and the workgroup's 32 term rows are staged
-// in LDS by coalesced loads (term rows are read at wave-uniform addresses),
-// and the running products are rescaled by a power of two per tile.  Each
-// (term, sample) contribution goes to res[t][b] (zero for inactive pairs);
-// a second kernel adds the diagonal part and the contributions in term
-// order, so E_loc sums in the same order as the per-sample item lists.
+// their own once per call, e^{d} and e^{-d} of the term for s_a = +1:
+//   flip of a:    Fp = EM_a,             Fm = EP_a
+//   pair (a, b):  Fp = EM_a EP_b,        Fm = EP_a EM_b
+// (complex products for complex RBMs); a sample with s_a = -1 swaps the two,
+// so the factor of hidden unit h in psi(s')/psi(s) is
+//   s_a = +1: p_h Fp + q_h Fm,   s_a = -1: q_h Fp + p_h Fm.
+// A wave holds 64 samples in its lanes and 8 terms in registers; per tile of
+// hidden units, the samples' logistic members and the workgroup's 32 term
+// rows are staged in LDS by coalesced loads (term rows are read at
+// wave-uniform addresses), and the running products are rescaled by a power
+// of two every RatioRange<R>::rescale units.  Each (term, sample)
+// contribution goes to res[t][b] (zero for inactive pairs); a second kernel
+// adds the diagonal part and the contributions in term order, so E_loc sums
+// in the same order as the per-sample item lists.
 // ---------------------------------------------------------------------------
 template <typename R, bool CPLX>
-__global__ void k_term_tables(ModelView<R> m, HamDev H, int Tp, R* __restrict__ Cr,
-                              R* __restrict__ Ci, R* __restrict__ Sr, R* __restrict__ Si) {
+__global__ void k_term_tables(ModelView<R> m, HamDev H, int Tp, R* __restrict__ Fpr,
+                              R* __restrict__ Fpi, R* __restrict__ Fmr, R* __restrict__ Fmi) {
   const int T = H.n_flip + H.n_pair, M = m.M;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)Tp * M;
        e += (long)gridDim.x * blockDim.x) {
     const int t = (int)(e / M), h = (int)(e - (long)t * M);
     if (t >= T) {
-      // padding rows: factor 1
-      Cr[e] = (R)1;
-      Sr[e] = (R)0;
-      if (CPLX) { Ci[e] = (R)0; Si[e] = (R)0; }
+      // padding rows: factor p + q = 1
+      Fpr[e] = (R)1;
+      Fmr[e] = (R)1;
+      if (CPLX) { Fpi[e] = (R)0; Fmi[e] = (R)0; }
       continue;
     }
     const int a = t < H.n_flip ? H.flip_i[t] : H.pair_a[t - H.n_flip];
     const int b = t < H.n_flip ? -1 : H.pair_b[t - H.n_flip];
-    const R car = m.CT_re[(size_t)a * M + h], sar = m.ST_re[(size_t)a * M + h];
-    const R cai = CPLX ? m.CT_im[(size_t)a * M + h] : (R)0;
-    const R sai = CPLX ? m.ST_im[(size_t)a * M + h] : (R)0;
+    const size_t ea = (size_t)a * M + h;
+    const R mar = m.EMT_re[ea], par_ = m.EPT_re[ea];
+    const R mai = CPLX ? m.EMT_im[ea] : (R)0, pai = CPLX ? m.EPT_im[ea] : (R)0;
     if (b < 0) {
-      Cr[e] = car;
-      Sr[e] = sar;
-      if (CPLX) { Ci[e] = cai; Si[e] = sai; }
+      Fpr[e] = mar;
+      Fmr[e] = par_;
+      if (CPLX) { Fpi[e] = mai; Fmi[e] = pai; }
       continue;
     }
-    const R cbr = m.CT_re[(size_t)b * M + h], sbr = m.ST_re[(size_t)b * M + h];
+    const size_t eb = (size_t)b * M + h;
+    const R pbr = m.EPT_re[eb], mbr = m.EMT_re[eb];
     if (CPLX) {
-      const R cbi = m.CT_im[(size_t)b * M + h], sbi = m.ST_im[(size_t)b * M + h];
-      Cr[e] = (car * cbr - cai * cbi) - (sar * sbr - sai * sbi);
-      Ci[e] = (car * cbi + cai * cbr) - (sar * sbi + sai * sbr);
-      Sr[e] = (sar * cbr - sai * cbi) - (car * sbr - cai * sbi);
-      Si[e] = (sar * cbi + sai * cbr) - (car * sbi + cai * sbr);
+      const R pbi = m.EPT_im[eb], mbi = m.EMT_im[eb];
+      Fpr[e] = mar * pbr - mai * pbi;
+      Fpi[e] = mar * pbi + mai * pbr;
+      Fmr[e] = par_ * mbr - pai * mbi;
+      Fmi[e] = par_ * mbi + pai * mbr;
     } else {
-      Cr[e] = car * cbr - sar * sbr;
-      Sr[e] = sar * cbr - car * sbr;
+      Fpr[e] = mar * pbr;
+      Fmr[e] = par_ * mbr;
     }
   }
 }
 
 constexpr int kEtTerms = 8;    // terms per wave (4 waves per workgroup)
 
-// hidden units per tile (the products are rescaled once per tile)
+// hidden units per tile
 template <typename R, bool CPLX>
 struct EtTile { static constexpr int H = (sizeof(R) == 8 && !CPLX) ? 32 : 16; };
 
@@ -323,19 +366,23 @@ template <typename R, bool CPLX>
 __global__ __launch_bounds__(256) void k_eloc_terms(ModelView<R> m, HamDev H,
                                                     const int* __restrict__ spins, int B,
                                                     const R* __restrict__ t_re,
-                                                    const R* __restrict__ t_im,
-                                                    const R* __restrict__ Cr,
-                                                    const R* __restrict__ Ci,
-                                                    const R* __restrict__ Sr,
-                                                    const R* __restrict__ Si,
+                                                    const R* __restrict__ q_re,
+                                                    const R* __restrict__ q_im,
+                                                    const R* __restrict__ Fpr,
+                                                    const R* __restrict__ Fpi,
+                                                    const R* __restrict__ Fmr,
+                                                    const R* __restrict__ Fmi,
                                                     double* __restrict__ res_r,
                                                     double* __restrict__ res_i) {
   constexpr int HT = EtTile<R, CPLX>::H;
+  constexpr int RS = RatioRange<R>::rescale;
+  static_assert(HT % RS == 0, "tiles hold whole rescaling groups");
   constexpr int NTB = 4 * kEtTerms;                 // terms per workgroup
-  __shared__ R tl_r[64][HT + 1];
+  __shared__ R tl_s[64][HT + 1];                     // Re t (its sign: which member)
+  __shared__ R tl_r[64][HT + 1];                     // small logistic member
   __shared__ R tl_i[CPLX ? 64 : 1][HT + 1];
-  __shared__ R tc_r[NTB][HT], ts_r[NTB][HT];
-  __shared__ R tc_i[CPLX ? NTB : 1][HT], ts_i[CPLX ? NTB : 1][HT];
+  __shared__ R tp_r[NTB][HT], tm_r[NTB][HT];
+  __shared__ R tp_i[CPLX ? NTB : 1][HT], tm_i[CPLX ? NTB : 1][HT];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int T = H.n_flip + H.n_pair, M = m.M, N = m.N;
@@ -344,16 +391,14 @@ __global__ __launch_bounds__(256) void k_eloc_terms(ModelView<R> m, HamDev H,
   const int tb0 = blockIdx.y * NTB;                 // workgroup's first term
   const int t0 = tb0 + wave * kEtTerms;             // wave's first term
   const int nt = max(0, min(kEtTerms, T - t0));
-  R msa[kEtTerms];
-  unsigned active = 0;
+  unsigned active = 0, upmask = 0;
 #pragma unroll
   for (int j = 0; j < kEtTerms; j++) {
-    msa[j] = (R)0;
     if (j < nt) {
       const int t = t0 + j;
       const int a = t < H.n_flip ? H.flip_i[t] : H.pair_a[t - H.n_flip];
       const int sa = spins[(size_t)bl * N + a];
-      msa[j] = (R)(-sa);
+      upmask |= sa > 0 ? (1u << j) : 0u;
       const bool act = t < H.n_flip || spins[(size_t)bl * N + H.pair_b[t - H.n_flip]] != sa;
       active |= act ? (1u << j) : 0u;
     }
@@ -366,72 +411,75 @@ __global__ __launch_bounds__(256) void k_eloc_terms(ModelView<R> m, HamDev H,
     const int hn = min(HT, M - h0);
     __syncthreads();
     {
-      // t of the 64 samples and the workgroup's term rows for this tile; all
-      // loads of a thread in flight together (indices clamped into the tile)
+      // logistic members of the 64 samples and the workgroup's term rows for
+      // this tile; all loads of a thread in flight together (indices clamped
+      // into the tile)
       constexpr int NQT = 64 * HT / 256, NQC = NTB * HT / 256;
-      R vtr[NQT], vti[NQT], vcr[NQC], vci[NQC], vsr[NQC], vsi[NQC];
+      R vts[NQT], vtr[NQT], vti[NQT], vpr[NQC], vpi[NQC], vmr[NQC], vmi[NQC];
 #pragma unroll
       for (int u = 0; u < NQT; u++) {
         const int e = tid + 256 * u, sm = e / HT, hh = e - sm * HT;
         const size_t g = (size_t)min(b0 + sm, B - 1) * M + h0 + min(hh, hn - 1);
-        vtr[u] = t_re[g];
-        if (CPLX) vti[u] = t_im[g];
+        vts[u] = t_re[g];
+        vtr[u] = q_re[g];
+        if (CPLX) vti[u] = q_im[g];
       }
 #pragma unroll
       for (int u = 0; u < NQC; u++) {
         const int e = tid + 256 * u, r = e / HT, hh = e - r * HT;
         const size_t g = (size_t)(tb0 + r) * M + h0 + min(hh, hn - 1);
-        vcr[u] = Cr[g];
-        vsr[u] = Sr[g];
-        if (CPLX) { vci[u] = Ci[g]; vsi[u] = Si[g]; }
+        vpr[u] = Fpr[g];
+        vmr[u] = Fmr[g];
+        if (CPLX) { vpi[u] = Fpi[g]; vmi[u] = Fmi[g]; }
       }
 #pragma unroll
       for (int u = 0; u < NQT; u++) {
         const int e = tid + 256 * u, sm = e / HT, hh = e - sm * HT;
+        tl_s[sm][hh] = vts[u];
         tl_r[sm][hh] = vtr[u];
         if (CPLX) tl_i[sm][hh] = vti[u];
       }
 #pragma unroll
       for (int u = 0; u < NQC; u++) {
         const int e = tid + 256 * u, r = e / HT, hh = e - r * HT;
-        tc_r[r][hh] = vcr[u];
-        ts_r[r][hh] = vsr[u];
-        if (CPLX) { tc_i[r][hh] = vci[u]; ts_i[r][hh] = vsi[u]; }
+        tp_r[r][hh] = vpr[u];
+        tm_r[r][hh] = vmr[u];
+        if (CPLX) { tp_i[r][hh] = vpi[u]; tm_i[r][hh] = vmi[u]; }
       }
     }
     __syncthreads();
     if (nt == 0) continue;
     const int rw = wave * kEtTerms;
-    for (int hh = 0; hh < hn; hh++) {
-      const R tr = tl_r[lane][hh];
-      const R ti = CPLX ? tl_i[lane][hh] : (R)0;
-      // term rows are padded to whole workgroups: no test on j
+    for (int g0 = 0; g0 < hn; g0 += RS) {
 #pragma unroll
-      for (int j = 0; j < kEtTerms; j++) {
-        if (CPLX) {
-          const R sr = msa[j] * ts_r[rw + j][hh], si = msa[j] * ts_i[rw + j][hh];
-          const R wr = tc_r[rw + j][hh] + (tr * sr - ti * si);
-          const R wi = tc_i[rw + j][hh] + (tr * si + ti * sr);
-          const R nr = pr[j] * wr - pim[j] * wi;
-          pim[j] = pr[j] * wi + pim[j] * wr;
-          pr[j] = nr;
-        } else {
-          pr[j] *= tc_r[rw + j][hh] + tr * msa[j] * ts_r[rw + j][hh];
+      for (int u = 0; u < RS; u++) {
+        const int hh = g0 + u;
+        if (hh >= hn) break;
+        R ppr, ppi, qqr, qqi;
+        pq_of<R, CPLX>(tl_s[lane][hh], tl_r[lane][hh], CPLX ? tl_i[lane][hh] : (R)0, ppr, ppi,
+                       qqr, qqi);
+        // term rows are padded to whole workgroups: no test on j
+#pragma unroll
+        for (int j = 0; j < kEtTerms; j++) {
+          const bool up = (upmask >> j) & 1u;
+          const R ar = up ? ppr : qqr, ai = up ? ppi : qqi;
+          const R br = up ? qqr : ppr, bi = up ? qqi : ppi;
+          if (CPLX) {
+            const R fpr = tp_r[rw + j][hh], fpi = tp_i[rw + j][hh];
+            const R fmr = tm_r[rw + j][hh], fmi = tm_i[rw + j][hh];
+            const R wr = (ar * fpr - ai * fpi) + (br * fmr - bi * fmi);
+            const R wi = (ar * fpi + ai * fpr) + (br * fmi + bi * fmr);
+            const R nr = pr[j] * wr - pim[j] * wi;
+            pim[j] = pr[j] * wi + pim[j] * wr;
+            pr[j] = nr;
+          } else {
+            pr[j] *= ar * tp_r[rw + j][hh] + br * tm_r[rw + j][hh];
+          }
         }
       }
-    }
-    // rescale by a power of two (exact)
+      // rescale by a power of two (exact)
 #pragma unroll
-    for (int j = 0; j < kEtTerms; j++) {
-      int e2;
-      if (CPLX) {
-        (void)frexp(fmax(fabs(pr[j]), fabs(pim[j])), &e2);
-        pr[j] = ldexp(pr[j], -e2);
-        pim[j] = ldexp(pim[j], -e2);
-      } else {
-        pr[j] = frexp(pr[j], &e2);
-      }
-      ex[j] += e2;
+      for (int j = 0; j < kEtTerms; j++) rescale2<R, CPLX>(pr[j], pim[j], ex[j]);
     }
   }
   if (b0 + lane >= B) return;
@@ -447,26 +495,23 @@ __global__ __launch_bounds__(256) void k_eloc_terms(ModelView<R> m, HamDev H,
     }
     const int a = t < H.n_flip ? H.flip_i[t] : H.pair_a[t - H.n_flip];
     const int bb = t < H.n_flip ? -1 : H.pair_b[t - H.n_flip];
-    const double s_a = -(double)msa[j];
+    const double s_a = ((upmask >> j) & 1u) ? 1.0 : -1.0;
     double vr = -2.0 * (double)m.a_re[a] * s_a;
     double vi = CPLX ? -2.0 * (double)m.a_im[a] * s_a : 0.0;
     if (bb >= 0) {
       vr += 2.0 * (double)m.a_re[bb] * s_a;
       if (CPLX) vi += 2.0 * (double)m.a_im[bb] * s_a;
     }
-    double lm, ph_r = 1.0, ph_i = 0.0;
+    const double xr = (double)pr[j], xi = CPLX ? (double)pim[j] : 0.0;
+    const double mg = CPLX ? sqrt(xr * xr + xi * xi) : fabs(xr);
+    // a zero factor (psi(s') = 0) contributes nothing
+    const double lm = mg > 0.0 ? log(mg) + (double)ex[j] * 0.69314718055994530942 : -INFINITY;
+    double ph_r = mg > 0.0 ? xr / mg : 0.0, ph_i = mg > 0.0 ? xi / mg : 0.0;
     if (CPLX) {
-      const double xr = (double)pr[j], xi = (double)pim[j];
-      const double mg = sqrt(xr * xr + xi * xi);
-      lm = log(mg) + (double)ex[j] * 0.69314718055994530942;
-      ph_r = xr / mg;
-      ph_i = xi / mg;
       const double c = cos(vi), sn = sin(vi);
       const double nr = ph_r * c - ph_i * sn;
       ph_i = ph_r * sn + ph_i * c;
       ph_r = nr;
-    } else {
-      lm = log((double)pr[j]) + (double)ex[j] * 0.69314718055994530942;
     }
     const double mag = exp(lm + vr);
     double sign = 1.0;
@@ -515,15 +560,15 @@ size_t eloc_ws_bytes(const ModelView<R>& m, const HamDev& h, int B) {
 
 template <typename R, bool CPLX>
 static int eloc_terms_go(const ModelView<R>& m, const HamDev& h, const int* spins, int B,
-                         const R* t_re, const R* t_im, double* e_re, double* e_im, void* ws,
-                         hipStream_t st) {
+                         const R* t_re, const R* q_re, const R* q_im, double* e_re, double* e_im,
+                         void* ws, hipStream_t st) {
   const int T = h.n_flip + h.n_pair, M = m.M, Tp = padded_terms(T);
   const size_t TM = (size_t)Tp * M;
   unsigned char* p = (unsigned char*)ws;
-  R* Cr = (R*)p;
-  R* Sr = Cr + TM;
-  R* Ci = CPLX ? Sr + TM : nullptr;
-  R* Si = CPLX ? Ci + TM : nullptr;
+  R* Fpr = (R*)p;
+  R* Fmr = Fpr + TM;
+  R* Fpi = CPLX ? Fmr + TM : nullptr;
+  R* Fmi = CPLX ? Fpi + TM : nullptr;
   const size_t tab = TM * sizeof(R) * (CPLX ? 4 : 2);
   double* res_r = (double*)(p + ((tab + 255) & ~(size_t)255));
   double* res_i = res_r + (size_t)T * B;
@@ -533,12 +578,12 @@ static int eloc_terms_go(const ModelView<R>& m, const HamDev& h, const int* spin
       long blocks = ((long)TM + 255) / 256;
       if (blocks > 4096) blocks = 4096;
       hipLaunchKernelGGL((k_term_tables<R, CPLX>), dim3((unsigned)blocks), dim3(256), 0, st, m, h,
-                         Tp, Cr, Ci, Sr, Si);
+                         Tp, Fpr, Fpi, Fmr, Fmi);
       count_launch();
     }
     const dim3 grid((B + 63) / 64, Tp / (4 * kEtTerms));
-    hipLaunchKernelGGL((k_eloc_terms<R, CPLX>), grid, dim3(256), 0, st, m, h, spins, B, t_re, t_im,
-                       Cr, Ci, Sr, Si, res_r, res_i);
+    hipLaunchKernelGGL((k_eloc_terms<R, CPLX>), grid, dim3(256), 0, st, m, h, spins, B, t_re, q_re,
+                       q_im, Fpr, Fpi, Fmr, Fmi, res_r, res_i);
     count_launch();
   }
   hipLaunchKernelGGL(k_eloc_terms_sum, dim3((B + 255) / 256), dim3(256), 0, st, h, spins, B, m.N,
@@ -550,13 +595,14 @@ static int eloc_terms_go(const ModelView<R>& m, const HamDev& h, const int* spin
 
 template <typename R, bool CPLX, int IPT>
 static int eloc_go(const ModelView<R>& m, const HamDev& h, const int* spins, int B,
-                   const R* t_re, const R* t_im, double* e_re, double* e_im, hipStream_t st) {
+                   const R* t_re, const R* q_re, const R* q_im, double* e_re, double* e_im,
+                   hipStream_t st) {
   const int N = m.N, T = h.n_flip + h.n_pair;
   int SB = (256 * IPT) / (T > 0 ? T : 1);
   if (SB < 1) SB = 1;
   if (SB > 64) SB = 64;
   if (SB > B) SB = B;
-  const int ntab = CPLX ? 4 : 2, nt = CPLX ? 2 : 1;
+  const int ntab = CPLX ? 4 : 2, nt = CPLX ? 3 : 2;
   auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
   const size_t fixed = al((size_t)SB * N) + al(4 * (size_t)SB * T) + al(4 * (size_t)(SB + 1)) +
                        2 * al(8 * (size_t)SB * T);
@@ -573,7 +619,7 @@ static int eloc_go(const ModelView<R>& m, const HamDev& h, const int* spins, int
   if (grid > 65535) grid = 65535;
   KScope ks("eloc", st);
   hipLaunchKernelGGL((k_eloc<R, CPLX, IPT>), dim3(grid), dim3(256), bytes, st, m, h, spins, B,
-                     t_re, t_im, e_re, e_im, SB, HT);
+                     t_re, q_re, q_im, e_re, e_im, SB, HT);
   count_launch();
   NQS_HIP_CHECK(hipGetLastError());
   return 0;
@@ -581,8 +627,9 @@ static int eloc_go(const ModelView<R>& m, const HamDev& h, const int* spins, int
 
 template <typename R>
 int launch_eloc(const ModelView<R>& m, const HamDev& h, const int* spins, int B,
-                const R* t_re, const R* t_im, double* e_re, double* e_im, hipStream_t st,
-                void* ws, size_t ws_bytes) {
+                const R* t_re, const R* t_im, const R* q_re, const R* q_im, double* e_re,
+                double* e_im, hipStream_t st, void* ws, size_t ws_bytes) {
+  (void)t_im;   // only the sign of Re t is read (which logistic member q holds)
   if (B <= 0) return 0;
   const int T = h.n_flip + h.n_pair;
   if (m.N > kMaxSites || T >= 65536) {
@@ -590,19 +637,23 @@ int launch_eloc(const ModelView<R>& m, const HamDev& h, const int* spins, int B,
                     "terms (got N = %d, %d terms)\n", kMaxSites, m.N, T);
     return -1;
   }
+  if (m.M > 0 && !q_re) {
+    fprintf(stderr, "nqs_hip: local energies need the logistic members of the forward pass\n");
+    return -1;
+  }
   const bool c = m.cplx != 0;
   if (ws && ws_bytes >= eloc_ws_bytes(m, h, B))
-    return c ? eloc_terms_go<R, true>(m, h, spins, B, t_re, t_im, e_re, e_im, ws, st)
-             : eloc_terms_go<R, false>(m, h, spins, B, t_re, t_im, e_re, e_im, ws, st);
+    return c ? eloc_terms_go<R, true>(m, h, spins, B, t_re, q_re, q_im, e_re, e_im, ws, st)
+             : eloc_terms_go<R, false>(m, h, spins, B, t_re, q_re, q_im, e_re, e_im, ws, st);
   if (T <= 256) {
-    return c ? eloc_go<R, true, 1>(m, h, spins, B, t_re, t_im, e_re, e_im, st)
-             : eloc_go<R, false, 1>(m, h, spins, B, t_re, t_im, e_re, e_im, st);
+    return c ? eloc_go<R, true, 1>(m, h, spins, B, t_re, q_re, q_im, e_re, e_im, st)
+             : eloc_go<R, false, 1>(m, h, spins, B, t_re, q_re, q_im, e_re, e_im, st);
   } else if (T <= 512) {
-    return c ? eloc_go<R, true, 2>(m, h, spins, B, t_re, t_im, e_re, e_im, st)
-             : eloc_go<R, false, 2>(m, h, spins, B, t_re, t_im, e_re, e_im, st);
+    return c ? eloc_go<R, true, 2>(m, h, spins, B, t_re, q_re, q_im, e_re, e_im, st)
+             : eloc_go<R, false, 2>(m, h, spins, B, t_re, q_re, q_im, e_re, e_im, st);
   } else if (T <= 1024) {
-    return c ? eloc_go<R, true, 4>(m, h, spins, B, t_re, t_im, e_re, e_im, st)
-             : eloc_go<R, false, 4>(m, h, spins, B, t_re, t_im, e_re, e_im, st);
+    return c ? eloc_go<R, true, 4>(m, h, spins, B, t_re, q_re, q_im, e_re, e_im, st)
+             : eloc_go<R, false, 4>(m, h, spins, B, t_re, q_re, q_im, e_re, e_im, st);
   }
   fprintf(stderr, "nqs_hip: more than 1024 off-diagonal terms per sample is not supported\n");
   return -1;
@@ -649,8 +700,8 @@ int launch_eloc_from_logamps(const HamDev& h, const int* spins, int B, int N, in
   return 0;
 }
 
-template int launch_eloc<double>(const ModelView<double>&, const HamDev&, const int*, int, const double*, const double*, double*, double*, hipStream_t, void*, size_t);
-template int launch_eloc<float>(const ModelView<float>&, const HamDev&, const int*, int, const float*, const float*, double*, double*, hipStream_t, void*, size_t);
+template int launch_eloc<double>(const ModelView<double>&, const HamDev&, const int*, int, const double*, const double*, const double*, const double*, double*, double*, hipStream_t, void*, size_t);
+template int launch_eloc<float>(const ModelView<float>&, const HamDev&, const int*, int, const float*, const float*, const float*, const float*, double*, double*, hipStream_t, void*, size_t);
 template size_t eloc_ws_bytes<double>(const ModelView<double>&, const HamDev&, int);
 template size_t eloc_ws_bytes<float>(const ModelView<float>&, const HamDev&, int);
 

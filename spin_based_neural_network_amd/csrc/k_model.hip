@@ -9,13 +9,13 @@
 namespace nqs {
 
 // --------------------------------------------------------------------------
-// tables: a, b, W^T, cosh(2W), sinh(2W) in both h-major and site-major order
+// tables: a, b, W^T, e^{2W} and e^{-2W} in both h-major and site-major order
 // --------------------------------------------------------------------------
 template <typename R>
 __global__ void k_prep(const double* __restrict__ par, int kind, int N, int M,
-                       ModelView<R> v, R* a_re, R* a_im, R* b_re, R* b_im, R* WT_re,
-                       R* WT_im, R* C_re, R* C_im, R* S_re, R* S_im, R* CT_re,
-                       R* CT_im, R* ST_re, R* ST_im) {
+                       R* a_re, R* a_im, R* b_re, R* b_im, R* WT_re, R* WT_im, R* EP_re,
+                       R* EP_im, R* EM_re, R* EM_im, R* EPT_re, R* EPT_im, R* EMT_re,
+                       R* EMT_im, int* range_flag) {
   const long Pr = (long)N + M + (long)M * N;
   const int cplx = (kind == 6);
   const long total = (long)M * N;
@@ -38,23 +38,48 @@ __global__ void k_prep(const double* __restrict__ par, int kind, int N, int M,
     double wi = cplx ? par[Pr + N + M + e] : 0.0;
     long et = (long)i * M + h;
     WT_re[et] = (R)wr;
-    // cosh / sinh of 2w for complex w = x + iy:
-    // cosh = cosh x cos y + i sinh x sin y ; sinh = sinh x cos y + i cosh x sin y
-    double x = 2.0 * wr, y = 2.0 * wi;
-    double ch = cosh(x), sh = sinh(x), cy = cos(y), sy = sin(y);
-    R cr = (R)(ch * cy), sr = (R)(sh * cy);
-    C_re[e] = cr;  S_re[e] = sr;  CT_re[et] = cr;  ST_re[et] = sr;
+    // beyond the served range the entries are not formed (the launcher
+    // reports the weight and refuses the tables)
+    if (!(fabs(wr) <= RatioRange<R>::max_w)) {
+      *range_flag = 1;
+      wr = 0.0;
+    }
+    // e^{+-2w} for complex w = x + iy: e^{+-2x} (cos 2y +- i sin 2y)
+    const double ep = exp(2.0 * wr), em = exp(-2.0 * wr);
+    double sy = 0.0, cy = 1.0;
+    if (cplx) sincos(2.0 * wi, &sy, &cy);
+    const R pr = (R)(ep * cy), mr = (R)(em * cy);
+    EP_re[e] = pr;  EM_re[e] = mr;  EPT_re[et] = pr;  EMT_re[et] = mr;
     if (cplx) {
-      R ci = (R)(sh * sy), si = (R)(ch * sy);
+      const R pi = (R)(ep * sy), mi = (R)(-em * sy);
       WT_im[et] = (R)wi;
-      C_im[e] = ci;  S_im[e] = si;  CT_im[et] = ci;  ST_im[et] = si;
+      EP_im[e] = pi;  EM_im[e] = mi;  EPT_im[et] = pi;  EMT_im[et] = mi;
     }
   }
 }
 
+__global__ void k_max_abs(const double* __restrict__ x, long n, double* __restrict__ out) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (long e = threadIdx.x; e < n; e += blockDim.x) m = fmax(m, fabs(x[e]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+__global__ void k_flag_reset(int* flag) { *flag = 0; }
+
+// check = 1: wait for the tables and set mb.ratio_ok (0: a weight lies
+// beyond the served range); check = 0 (inside a captured graph): the flag is
+// left in mb.range_flag for the caller to read after the stream synchronises
+// (mb.ratio_ok = -1).  a, b and W^T serve amplitudes either way.
 template <typename R>
 int launch_prep(const double* d_par, int kind, int N, int M, ModelBuffers<R>& mb,
-                hipStream_t st) {
+                hipStream_t st, int check) {
   const int cplx = (kind == 6);
   const size_t MN = (size_t)M * N;
   const size_t need = 2 * (size_t)N + 2 * (size_t)M + 10 * MN + 16;
@@ -64,12 +89,13 @@ int launch_prep(const double* d_par, int kind, int N, int M, ModelBuffers<R>& mb
     NQS_HIP_CHECK(hipMalloc(&mb.base, need * sizeof(R)));
     mb.count = need;
   }
+  if (!mb.range_flag) NQS_HIP_CHECK(hipMalloc(&mb.range_flag, 2 * sizeof(double)));
   R* p = mb.base;
   auto take = [&](size_t n) { R* q = p; p += n; return q; };
   R *a_re = take(N), *a_im = take(N), *b_re = take(M), *b_im = take(M);
-  R *WT_re = take(MN), *WT_im = take(MN), *C_re = take(MN), *C_im = take(MN);
-  R *S_re = take(MN), *S_im = take(MN), *CT_re = take(MN), *CT_im = take(MN);
-  R *ST_re = take(MN), *ST_im = take(MN);
+  R *WT_re = take(MN), *WT_im = take(MN), *EP_re = take(MN), *EP_im = take(MN);
+  R *EM_re = take(MN), *EM_im = take(MN), *EPT_re = take(MN), *EPT_im = take(MN);
+  R *EMT_re = take(MN), *EMT_im = take(MN);
   ModelView<R>& v = mb.view;
   v.N = N;
   v.M = M;
@@ -77,26 +103,52 @@ int launch_prep(const double* d_par, int kind, int N, int M, ModelBuffers<R>& mb
   v.a_re = a_re;  v.a_im = cplx ? a_im : nullptr;
   v.b_re = b_re;  v.b_im = cplx ? b_im : nullptr;
   v.WT_re = WT_re;  v.WT_im = cplx ? WT_im : nullptr;
-  v.C_re = C_re;  v.C_im = cplx ? C_im : nullptr;
-  v.S_re = S_re;  v.S_im = cplx ? S_im : nullptr;
-  v.CT_re = CT_re;  v.CT_im = cplx ? CT_im : nullptr;
-  v.ST_re = ST_re;  v.ST_im = cplx ? ST_im : nullptr;
+  v.EP_re = EP_re;  v.EP_im = cplx ? EP_im : nullptr;
+  v.EM_re = EM_re;  v.EM_im = cplx ? EM_im : nullptr;
+  v.EPT_re = EPT_re;  v.EPT_im = cplx ? EPT_im : nullptr;
+  v.EMT_re = EMT_re;  v.EMT_im = cplx ? EMT_im : nullptr;
+  hipLaunchKernelGGL(k_flag_reset, dim3(1), dim3(1), 0, st, mb.range_flag);
+  count_launch();
   long total = (long)MN + N + M;
   int blocks = (int)((total + 255) / 256);
   if (blocks > 4096) blocks = 4096;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_prep<R>, dim3(blocks), dim3(256), 0, st, d_par, kind, N, M, v,
-                     a_re, a_im, b_re, b_im, WT_re, WT_im, C_re, C_im, S_re, S_im,
-                     CT_re, CT_im, ST_re, ST_im);
+  hipLaunchKernelGGL(k_prep<R>, dim3(blocks), dim3(256), 0, st, d_par, kind, N, M,
+                     a_re, a_im, b_re, b_im, WT_re, WT_im, EP_re, EP_im, EM_re, EM_im,
+                     EPT_re, EPT_im, EMT_re, EMT_im, mb.range_flag);
   count_launch();
   NQS_HIP_CHECK(hipGetLastError());
   mb.valid = 1;
+  mb.ratio_ok = -1;
+  if (!check) return 0;
+  int flag = 0;
+  NQS_HIP_CHECK(hipMemcpyAsync(&flag, mb.range_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  NQS_HIP_CHECK(hipStreamSynchronize(st));
+  mb.ratio_ok = flag ? 0 : 1;
+  return 0;
+}
+
+// the largest |Re W| (real block [N + M, N + M + M N) of the parameters),
+// reported on stderr as the reason the ratio tables are refused
+template <typename R>
+int report_ratio_range(const double* d_par, int N, int M, int* scratch, hipStream_t st) {
+  double* dmax = reinterpret_cast<double*>(scratch) + 1;
+  hipLaunchKernelGGL(k_max_abs, dim3(1), dim3(256), 0, st, d_par + N + M, (long)M * N, dmax);
+  count_launch();
+  double wmax = 0.0;
+  NQS_HIP_CHECK(hipMemcpyAsync(&wmax, dmax, sizeof(double), hipMemcpyDeviceToHost, st));
+  NQS_HIP_CHECK(hipStreamSynchronize(st));
+  fprintf(stderr, "nqs_hip: a weight |Re W| = %.6g lies beyond the range of the %s amplitude-ratio "
+                  "tables (|Re W| <= %g); moves and local energies are refused\n", wmax,
+          sizeof(R) == 8 ? "FP64" : "FP32", RatioRange<R>::max_w);
   return 0;
 }
 
 template <typename R>
 void free_model(ModelBuffers<R>& mb) {
   if (mb.base) (void)hipFree(mb.base);
+  if (mb.range_flag) (void)hipFree(mb.range_flag);
+  mb.range_flag = nullptr;
   mb.base = nullptr;
   mb.count = 0;
   mb.valid = 0;
@@ -117,7 +169,8 @@ template <typename R, bool CPLX>
 __global__ __launch_bounds__(256) void k_forward(ModelView<R> m, const int* __restrict__ spins,
                                                  int B, R* __restrict__ t_re,
                                                  R* __restrict__ t_im, double* la,
-                                                 double* ar) {
+                                                 double* ar, R* __restrict__ q_re,
+                                                 R* __restrict__ q_im) {
   const int N = m.N, M = m.M;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -143,6 +196,13 @@ __global__ __launch_bounds__(256) void k_forward(ModelView<R> m, const int* __re
         } else {
           t_re[(size_t)b * M + h] = tanh(xr);
         }
+      }
+      if (q_re) {
+        R sr, si;
+        int is_p;
+        logistic_small<R>(xr, xi, sr, si, is_p);
+        q_re[(size_t)b * M + h] = sr;
+        if (CPLX) q_im[(size_t)b * M + h] = si;
       }
       if (la) {
         if (CPLX && ar) {
@@ -183,7 +243,7 @@ __global__ __launch_bounds__(256) void k_forward(ModelView<R> m, const int* __re
 
 template <typename R>
 int launch_forward(const ModelView<R>& m, const int* spins, int B, R* t_re, R* t_im,
-                   double* la, double* ar, hipStream_t st) {
+                   double* la, double* ar, hipStream_t st, R* q_re, R* q_im) {
   if (B <= 0) return 0;
   if (m.N > kMaxSites) {
     fprintf(stderr, "nqs_hip: amplitudes support N <= %d sites (got %d)\n", kMaxSites, m.N);
@@ -193,10 +253,10 @@ int launch_forward(const ModelView<R>& m, const int* spins, int B, R* t_re, R* t
   KScope ks("forward", st);
   if (m.cplx)
     hipLaunchKernelGGL((k_forward<R, true>), dim3(grid), dim3(256), 0, st, m, spins, B,
-                       t_re, t_im, la, ar);
+                       t_re, t_im, la, ar, q_re, q_im);
   else
     hipLaunchKernelGGL((k_forward<R, false>), dim3(grid), dim3(256), 0, st, m, spins, B,
-                       t_re, t_im, la, ar);
+                       t_re, t_im, la, ar, q_re, q_im);
   count_launch();
   NQS_HIP_CHECK(hipGetLastError());
   return 0;
@@ -216,23 +276,33 @@ __global__ void k_spins_f64(const int* __restrict__ s, long n, double* __restric
 template <bool CPLX>
 __global__ void k_theta_finish(double* __restrict__ t_re, double* __restrict__ t_im,
                                const double* __restrict__ b_re, const double* __restrict__ b_im,
-                               long B, int M) {
+                               long B, int M, double* __restrict__ q_re,
+                               double* __restrict__ q_im) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < B * M;
        e += (long)gridDim.x * blockDim.x) {
     const int h = (int)(e % M);
+    const double xr = t_re[e] + b_re[h], xi = CPLX ? t_im[e] + b_im[h] : 0.0;
+    if (q_re) {
+      double sr, si;
+      int is_p;
+      logistic_small<double>(xr, xi, sr, si, is_p);
+      q_re[e] = sr;
+      if (CPLX) q_im[e] = si;
+    }
     if (CPLX) {
       double tr, ti;
-      ctanh<double>(t_re[e] + b_re[h], t_im[e] + b_im[h], tr, ti);
+      ctanh<double>(xr, xi, tr, ti);
       t_re[e] = tr;
       t_im[e] = ti;
     } else {
-      t_re[e] = tanh(t_re[e] + b_re[h]);
+      t_re[e] = tanh(xr);
     }
   }
 }
 
 int launch_forward_t_gemm(const ModelView<double>& m, const double* d_par, const int* spins, int B,
-                          double* t_re, double* t_im, double* sd, hipStream_t st) {
+                          double* t_re, double* t_im, double* sd, hipStream_t st, double* q_re,
+                          double* q_im) {
   if (B <= 0) return 0;
   const int N = m.N, M = m.M;
   const long Pr = (long)N + M + (long)M * N;
@@ -249,10 +319,10 @@ int launch_forward_t_gemm(const ModelView<double>& m, const double* d_par, const
   if (blocks > 65536) blocks = 65536;
   if (m.cplx)
     hipLaunchKernelGGL(k_theta_finish<true>, dim3((unsigned)blocks), dim3(256), 0, st, t_re, t_im,
-                       m.b_re, m.b_im, (long)B, M);
+                       m.b_re, m.b_im, (long)B, M, q_re, q_im);
   else
     hipLaunchKernelGGL(k_theta_finish<false>, dim3((unsigned)blocks), dim3(256), 0, st, t_re, t_im,
-                       m.b_re, m.b_im, (long)B, M);
+                       m.b_re, m.b_im, (long)B, M, q_re, q_im);
   count_launch();
   NQS_HIP_CHECK(hipGetLastError());
   return 0;
@@ -332,12 +402,14 @@ int launch_gen_O(const ModelView<R>& m, int kind, const int* spins, int B,
   return 0;
 }
 
-template int launch_prep<double>(const double*, int, int, int, ModelBuffers<double>&, hipStream_t);
-template int launch_prep<float>(const double*, int, int, int, ModelBuffers<float>&, hipStream_t);
+template int launch_prep<double>(const double*, int, int, int, ModelBuffers<double>&, hipStream_t, int);
+template int launch_prep<float>(const double*, int, int, int, ModelBuffers<float>&, hipStream_t, int);
+template int report_ratio_range<double>(const double*, int, int, int*, hipStream_t);
+template int report_ratio_range<float>(const double*, int, int, int*, hipStream_t);
 template void free_model<double>(ModelBuffers<double>&);
 template void free_model<float>(ModelBuffers<float>&);
-template int launch_forward<double>(const ModelView<double>&, const int*, int, double*, double*, double*, double*, hipStream_t);
-template int launch_forward<float>(const ModelView<float>&, const int*, int, float*, float*, double*, double*, hipStream_t);
+template int launch_forward<double>(const ModelView<double>&, const int*, int, double*, double*, double*, double*, hipStream_t, double*, double*);
+template int launch_forward<float>(const ModelView<float>&, const int*, int, float*, float*, double*, double*, hipStream_t, float*, float*);
 template int launch_gen_O<double>(const ModelView<double>&, int, const int*, int, const double*, const double*, int, double*, double*, long, hipStream_t);
 template int launch_gen_O<float>(const ModelView<float>&, int, const int*, int, const float*, const float*, int, double*, double*, long, hipStream_t);
 

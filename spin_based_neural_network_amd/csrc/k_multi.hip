@@ -378,6 +378,7 @@ __global__ void k_mb_gather(MultiBatch mb, double* out) {
   o[7] = sc->rs_new;
   o[8] = sc->rs0;
   o[9] = t ? (double)a / (double)t : 0.0;
+  o[10] = q.range_flag ? (double)*q.range_flag : 0.0;
 }
 
 }  // namespace

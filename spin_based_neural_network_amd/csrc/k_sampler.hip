@@ -7,13 +7,17 @@
 // n_down) are this library's extension; both are restated in the oracle.
 //
 // Layout: L lanes per chain (L | 64), each lane owns hidden units
-// h = lane + L*j (j < HPL) and keeps theta_h and tanh(theta_h) in registers;
-// the spin configuration is a bitmask in registers and every lane of the
-// chain advances an identical copy of the chain's xorshift state, so no
-// broadcast is needed.  A proposal's |psi'/psi| is the product over h of
-// cosh(theta_h + d_h)/cosh(theta_h) = cosh d_h + tanh(theta_h) sinh d_h read
-// from the site-major cosh(2W)/sinh(2W) tables: one log per lane and a
-// log2(L)-step butterfly per proposal, theta updated incrementally on accept.
+// h = lane + L*j (j < HPL) and keeps the logistic pair of theta_h
+// (p = 1/(1 + e^{-2 theta}), q = 1 - p: the member of smaller modulus and a
+// bit saying which, nqs_common.h) in registers; the spin configuration is a
+// bitmask in registers and every lane of the chain advances an identical
+// copy of the chain's xorshift state, so no broadcast is needed.  A
+// proposal's |psi'/psi| is the product over h of
+//   cosh(theta_h + d_h)/cosh(theta_h) = p_h e^{d_h} + q_h e^{-d_h}
+// with e^{+-d} read from the site-major e^{2W} / e^{-2W} tables (two
+// positive terms for a real RBM: no cancellation at any |theta|): one log
+// per lane and a log2(L)-step butterfly per proposal; on acceptance the
+// pair becomes (p e^{d} / w, q e^{-d} / w), again without cancellation.
 #include <type_traits>
 
 #include "nqs_kernels.h"
@@ -76,7 +80,8 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
   // theta_h = b_h + sum_i W_hi s_i in site order; the site loop is outer so
   // that the loads of all of a lane's units for one site are in flight
   // together (rows clamped into the table, products s * W exact)
-  R thr[HPL], thi[HPL], tr[HPL], ti[HPL];
+  R thr[HPL], thi[HPL], lr[HPL], li[HPL];
+  unsigned isp = 0;   // bit j: lr/li of unit j hold p (else q)
 #pragma unroll
   for (int j = 0; j < HPL; j++) {
     const int h = lane + L * j;
@@ -103,8 +108,10 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
   }
 #pragma unroll
   for (int j = 0; j < HPL; j++) {
-    if (CPLX) ctanh<R>(thr[j], thi[j], tr[j], ti[j]);
-    else { tr[j] = tanh(thr[j]); ti[j] = 0; }
+    int f;
+    logistic_small<R>(thr[j], CPLX ? thi[j] : (R)0, lr[j], li[j], f);
+    if (!CPLX) li[j] = 0;
+    isp |= (unsigned)f << j;
   }
 
   // log|psi(s)| with the current parameters, and the chain's cached value
@@ -163,34 +170,40 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
       // the stream advances only if the draw is used
       uint64_t rng_u = rng;
       const double lu = log(xs_uniform(rng_u));
-      // hidden factors w = cosh d + t sinh d of this lane's units, and each
-      // unit's tanh(theta + d) = (t cosh d + sinh d) / w should the proposal
-      // be accepted (computed with the factors, off the decision's path)
+      // hidden factors w = p e^{d} + q e^{-d} of this lane's units, and each
+      // unit's next logistic pair (p e^{d} / w, q e^{-d} / w) should the
+      // proposal be accepted (computed with the factors, off the decision's
+      // path); the running product is rescaled by a power of two every RS
+      // units
+      constexpr int RS = RatioRange<R>::rescale;
       R pr = 1, pi = 0;
-      R tnr[HPL], tni[HPL];
+      int pex = 0;
+      R nlr[HPL], nli[HPL];
+      unsigned nisp = 0;
       // table rows of the flipped sites for the lane's units, in groups of
       // QB units whose loads are issued together (rows clamped into the
       // table; the pair form reads the second site's rows as well)
       auto factors = [&](auto pair_tag) {
         constexpr bool PAIR = decltype(pair_tag)::value;
         constexpr int QB = HPL >= 16 ? 8 : 4;
+        const bool up = si > (R)0;
 #pragma unroll
         for (int q0 = 0; q0 < HPL; q0 += QB) {
-          R cir[QB], sir[QB], cii[QB], sii[QB], cjr[QB], sjr[QB], cji[QB], sji[QB];
+          R pir[QB], mir[QB], pii[QB], mii[QB], pjr[QB], mjr[QB], pji[QB], mji[QB];
 #pragma unroll
           for (int u = 0; u < QB; u++) {
             const int q = q0 + u;
             if (q < HPL) {
               const int h = lane + L * q;
               const size_t ei = (size_t)i * M + (h < M ? h : 0);
-              cir[u] = m.CT_re[ei];
-              sir[u] = m.ST_re[ei];
-              if (CPLX) { cii[u] = m.CT_im[ei]; sii[u] = m.ST_im[ei]; }
+              pir[u] = m.EPT_re[ei];
+              mir[u] = m.EMT_re[ei];
+              if (CPLX) { pii[u] = m.EPT_im[ei]; mii[u] = m.EMT_im[ei]; }
               if (PAIR) {
                 const size_t ej = (size_t)j * M + (h < M ? h : 0);
-                cjr[u] = m.CT_re[ej];
-                sjr[u] = m.ST_re[ej];
-                if (CPLX) { cji[u] = m.CT_im[ej]; sji[u] = m.ST_im[ej]; }
+                pjr[u] = m.EPT_re[ej];
+                mjr[u] = m.EMT_re[ej];
+                if (CPLX) { pji[u] = m.EPT_im[ej]; mji[u] = m.EMT_im[ej]; }
               }
             }
           }
@@ -200,42 +213,60 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
             if (q >= HPL) continue;
             const int h = lane + L * q;
             if (h >= M) continue;
-            R chr, chi = 0, shr, shi = 0;   // cosh d, sinh d
-            if (!PAIR) {
-              chr = cir[u];
-              shr = -si * sir[u];
-              if (CPLX) { chi = cii[u]; shi = -si * sii[u]; }
-            } else {
-              // d = -2 s_i (W_i - W_j): cosh d = Ci Cj - Si Sj,
-              // sinh d = -s_i (Si Cj - Ci Sj)
+            // e^{d} (xr, xi), e^{-d} (yr, yi): d = -2 s_i W_i (+ 2 s_i W_j)
+            R xr = up ? mir[u] : pir[u], xi = CPLX ? (up ? mii[u] : pii[u]) : (R)0;
+            R yr = up ? pir[u] : mir[u], yi = CPLX ? (up ? pii[u] : mii[u]) : (R)0;
+            if (PAIR) {
+              const R fr = up ? pjr[u] : mjr[u], gr = up ? mjr[u] : pjr[u];
               if (CPLX) {
-                chr = (cir[u] * cjr[u] - cii[u] * cji[u]) - (sir[u] * sjr[u] - sii[u] * sji[u]);
-                chi = (cir[u] * cji[u] + cii[u] * cjr[u]) - (sir[u] * sji[u] + sii[u] * sjr[u]);
-                R ar = (sir[u] * cjr[u] - sii[u] * cji[u]) - (cir[u] * sjr[u] - cii[u] * sji[u]);
-                R ai = (sir[u] * cji[u] + sii[u] * cjr[u]) - (cir[u] * sji[u] + cii[u] * sjr[u]);
-                shr = -si * ar;
-                shi = -si * ai;
+                const R fi = up ? pji[u] : mji[u], gi = up ? mji[u] : pji[u];
+                const R nxr = xr * fr - xi * fi;
+                xi = xr * fi + xi * fr;
+                xr = nxr;
+                const R nyr = yr * gr - yi * gi;
+                yi = yr * gi + yi * gr;
+                yr = nyr;
               } else {
-                chr = cir[u] * cjr[u] - sir[u] * sjr[u];
-                shr = -si * (sir[u] * cjr[u] - cir[u] * sjr[u]);
+                xr *= fr;
+                yr *= gr;
               }
             }
+            R ppr, ppi, qqr, qqi;
+            logistic_unpack<R>(lr[q], CPLX ? li[q] : (R)0, (isp >> q) & 1u, ppr, ppi, qqr, qqi);
             if (CPLX) {
-              // w = cosh d + t sinh d
-              const R wr = chr + (tr[q] * shr - ti[q] * shi);
-              const R wi = chi + (tr[q] * shi + ti[q] * shr);
+              const R ar = ppr * xr - ppi * xi, ai = ppr * xi + ppi * xr;   // p e^{d}
+              const R br = qqr * yr - qqi * yi, bi = qqr * yi + qqi * yr;   // q e^{-d}
+              const R wr = ar + br, wi = ai + bi;
               const R nr = pr * wr - pi * wi;
               pi = pr * wi + pi * wr;
               pr = nr;
-              const R tr_n = (tr[q] * chr - ti[q] * chi) + shr;
-              const R ti_n = (tr[q] * chi + ti[q] * chr) + shi;
               const R inv = (R)1 / (wr * wr + wi * wi);
-              tnr[q] = (tr_n * wr + ti_n * wi) * inv;
-              tni[q] = (ti_n * wr - tr_n * wi) * inv;
+              const R npr = (ar * wr + ai * wi) * inv, npi = (ai * wr - ar * wi) * inv;
+              const R nqr = (br * wr + bi * wi) * inv, nqi = (bi * wr - br * wi) * inv;
+              const bool small_p = npr * npr + npi * npi < nqr * nqr + nqi * nqi;
+              nlr[q] = small_p ? npr : nqr;
+              nli[q] = small_p ? npi : nqi;
+              nisp |= small_p ? (1u << q) : 0u;
             } else {
-              const R wr = chr + tr[q] * shr;
-              pr *= wr;
-              tnr[q] = (tr[q] * chr + shr) / wr;
+              const R a = ppr * xr, b = qqr * yr;    // both positive
+              const R w = a + b;
+              pr *= w;
+              const R inv = (R)1 / w;
+              const R np = a * inv, nq = b * inv;
+              const bool small_p = np < nq;
+              nlr[q] = small_p ? np : nq;
+              nisp |= small_p ? (1u << q) : 0u;
+            }
+            if ((q % RS) == RS - 1) {
+              int e2;
+              if (CPLX) {
+                (void)frexp(fmax(fabs(pr), fabs(pi)), &e2);
+                pr = ldexp(pr, -e2);
+                pi = ldexp(pi, -e2);
+              } else {
+                pr = frexp(pr, &e2);
+              }
+              pex += e2;
             }
           }
         }
@@ -244,9 +275,12 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
         if (j >= 0) factors(std::true_type{});
         else factors(std::false_type{});
       }
+      // log of this lane's product (an exact power of two folded in; a zero
+      // factor gives -inf, never accepted)
       R lm = CPLX ? (R)0.5 * log(pr * pr + pi * pi) : log(pr);
+      lm += (R)pex * (R)0.69314718055994530942;
       if (M == 0) lm = 0;
-      R lsum = group_allreduce_sum<L>(lm);
+      const R lsum = group_allreduce_sum<L>(lm);
       // visible part: Re a_i (s_i' - s_i) = -2 a_i s_i per flipped site
       R dvis = (R)-2 * m.a_re[i] * si;
       if (j >= 0) dvis += (R)2 * m.a_re[j] * si;   // s_j = -s_i
@@ -270,16 +304,17 @@ __device__ inline void mh_chain(const ModelView<R>& m, const ChainState& cs, con
         off = 0.0;
         sp[i >> 6] ^= 1ULL << (i & 63);
         if (j >= 0) sp[j >> 6] ^= 1ULL << (j & 63);
-        // the accepted proposal's tanh(theta + d); its denominator w is not
+        // the accepted proposal's logistic pairs; their denominator w is not
         // zero (a zero factor gives d = -inf, never accepted)
 #pragma unroll
         for (int q = 0; q < HPL; q++) {
           const int h = lane + L * q;
           if (h < M) {
-            tr[q] = tnr[q];
-            if (CPLX) ti[q] = tni[q];
+            lr[q] = nlr[q];
+            if (CPLX) li[q] = nli[q];
           }
         }
+        isp = nisp;
       }
     }
     if (ml.record) {

@@ -154,6 +154,44 @@ __device__ inline void ctanh<float>(float x, float y, float& tr, float& ti) {
   ctanh_f(x, y, tr, ti);
 }
 
+// The logistic pair of a hidden input theta = x + iy,
+//   p = 1 / (1 + e^{-2 theta}),  q = 1 / (1 + e^{2 theta}),  p + q = 1,
+// prices a change d of theta without cancellation:
+//   cosh(theta + d) / cosh(theta) = p e^{d} + q e^{-d},
+// a sum of two positive terms for real theta (the form cosh d + tanh(theta)
+// sinh d loses every digit once tanh rounds to +-1 against a large -d), and
+// after the change p' = p e^{d} / w, q' = q e^{-d} / w with w the ratio.
+// Only the member of smaller modulus is kept ("small": q when x >= 0, else
+// p), from z = e^{-2|x|} e^{-+2iy} (|z| <= 1): small = z / (1 + z); the
+// other is 1 - small, which keeps its relative accuracy since its modulus is
+// at least 1/2.  The flag tells which one small is (1: small = p).
+template <typename R>
+__device__ inline void logistic_small(R x, R y, R& sr, R& si, int& is_p) {
+  const R E = exp((R)-2 * fabs(x));
+  R s2 = 0, c2 = 1;
+  if (y != (R)0) {
+    if constexpr (sizeof(R) == 8) sincos((double)(2 * y), (double*)&s2, (double*)&c2);
+    else sincosf((float)(2 * y), (float*)&s2, (float*)&c2);
+  }
+  const R sg = x >= (R)0 ? (R)1 : (R)-1;
+  R den = (R)1 + (R)2 * E * c2 + E * E;   // |1 + z|^2
+  const R floor_ = sizeof(R) == 8 ? (R)1e-300 : (R)1e-30;
+  if (den < floor_) den = floor_;
+  const R r = (R)1 / den;
+  sr = (E * c2 + E * E) * r;
+  si = -sg * E * s2 * r;
+  is_p = x < (R)0 ? 1 : 0;
+}
+// p and q from small and its flag
+template <typename R>
+__device__ inline void logistic_unpack(R sr, R si, int is_p, R& pr, R& pi, R& qr, R& qi) {
+  const R lr = (R)1 - sr, li = -si;
+  pr = is_p ? sr : lr;
+  pi = is_p ? si : li;
+  qr = is_p ? lr : sr;
+  qi = is_p ? li : si;
+}
+
 // log|2 cosh z| and arg(2 cosh z) in the overflow-safe form the reference
 // uses (src/nqs/nqs_ansatz.c:309-318), FP64.
 __device__ inline void logcosh2_c(double x, double y, double& la, double& ar) {

@@ -9,21 +9,38 @@ namespace nqs {
 // Device view of an ansatz's derived tables in arithmetic type R.  Built by
 // launch_prep from the FP64 parameter vector (reference layout,
 // src/nqs/nqs_ansatz.c:36-48): a (N), b (M), W (M x N) per real / imaginary
-// block.  C = cosh(2W) and S = sinh(2W) give the ratio of one hidden factor
-// after flipping sites: cosh(theta + d)/cosh(theta) = cosh d + tanh(theta) sinh d
-// with d = -2 s_i W_hi (d's cosh / sinh for two flipped sites follow from the
-// addition theorems).  Imaginary arrays are null for real kinds.
+// block.  EP = e^{2W} and EM = e^{-2W} give the ratio of one hidden factor
+// after flipping sites with the logistic pair (p, q) of theta
+// (logistic_small in nqs_common.h): cosh(theta + d)/cosh(theta) =
+// p e^{d} + q e^{-d}, with e^{d} = EM, e^{-d} = EP for the flip of an up site
+// (d = -2 s_i W_hi) and the products of two sites' entries for a pair.
+// Imaginary arrays are null for real kinds.
 template <typename R>
 struct ModelView {
   int N = 0, M = 0, cplx = 0;
   const R *a_re = nullptr, *a_im = nullptr;
   const R *b_re = nullptr, *b_im = nullptr;
   const R *WT_re = nullptr, *WT_im = nullptr;      // [N][M]
-  const R *C_re = nullptr, *C_im = nullptr;        // [M][N]
-  const R *S_re = nullptr, *S_im = nullptr;        // [M][N]
-  const R *CT_re = nullptr, *CT_im = nullptr;      // [N][M]
-  const R *ST_re = nullptr, *ST_im = nullptr;      // [N][M]
+  const R *EP_re = nullptr, *EP_im = nullptr;      // [M][N]  e^{2W}
+  const R *EM_re = nullptr, *EM_im = nullptr;      // [M][N]  e^{-2W}
+  const R *EPT_re = nullptr, *EPT_im = nullptr;    // [N][M]
+  const R *EMT_re = nullptr, *EMT_im = nullptr;    // [N][M]
 };
+
+// Largest |Re W| the ratio tables serve, and how many hidden units the
+// kernels multiply between two rescalings of a running product.  A pair
+// changes Re theta by up to 4 max|Re W| per unit, so the bounds keep every
+// table entry and every product between rescalings inside the type's range
+// (FP64: 8 x 80 < 709; FP32: 2 x 32 < 88), and a small logistic member that
+// underflows to zero is negligible beside the other term of its ratio
+// (2 x 80 < 745 for FP64, 2 x 32 < 87 for FP32).  launch_prep refuses
+// larger weights.
+template <typename R>
+struct RatioRange;
+template <>
+struct RatioRange<double> { static constexpr double max_w = 20.0; static constexpr int rescale = 8; };
+template <>
+struct RatioRange<float> { static constexpr double max_w = 8.0; static constexpr int rescale = 2; };
 
 template <typename R>
 struct ModelBuffers {
@@ -31,6 +48,8 @@ struct ModelBuffers {
   size_t count = 0;
   ModelView<R> view;
   int valid = 0;
+  int* range_flag = nullptr;   // device: set by the table kernel when |Re W| > max_w
+  int ratio_ok = -1;           // 1: ratio tables usable, 0: refused, -1: not read
 };
 
 // Hamiltonian as term lists (built on the host in the reference's bond
@@ -64,20 +83,32 @@ struct ChainState {
 };
 
 // ---- model (k_model.hip) --------------------------------------------------
+// Tables of the ansatz.  check = 1 waits for them and sets mb.ratio_ok
+// (0: a weight beyond RatioRange, the ratio tables must not be used);
+// check = 0 (graph capture) leaves the verdict in *mb.range_flag (nonzero:
+// refused) for the caller to read later.  Amplitudes (a, b, W^T) are valid
+// either way.
 template <typename R>
 int launch_prep(const double* d_par, int kind, int N, int M, ModelBuffers<R>& mb,
-                hipStream_t st);
+                hipStream_t st, int check = 1);
+// message on stderr naming the largest |Re W| (scratch: mb.range_flag)
+template <typename R>
+int report_ratio_range(const double* d_par, int N, int M, int* scratch, hipStream_t st);
 template <typename R>
 void free_model(ModelBuffers<R>& mb);
-// theta = b + W s per sample; writes t = tanh(theta) [B][M] (re / im) and,
-// when la != null, log|psi| and arg psi [B] in FP64.
+// theta = b + W s per sample; writes t = tanh(theta) [B][M] (re / im),
+// when q_re != null the small member of the logistic pair of theta [B][M]
+// (re / im; which member it is follows from the sign of Re t: p when
+// Re t < 0), and, when la != null, log|psi| and arg psi [B] in FP64.
 template <typename R>
 int launch_forward(const ModelView<R>& m, const int* spins, int B, R* t_re, R* t_im,
-                   double* la, double* ar, hipStream_t st);
-// t only, FP64, theta = S W^T + b on FP64 MFMA (d_par: the parameter
-// vector, sd: B*N doubles of scratch)
+                   double* la, double* ar, hipStream_t st, R* q_re = nullptr,
+                   R* q_im = nullptr);
+// t (and optionally the small logistic member) only, FP64, theta = S W^T + b
+// on FP64 MFMA (d_par: the parameter vector, sd: B*N doubles of scratch)
 int launch_forward_t_gemm(const ModelView<double>& m, const double* d_par, const int* spins, int B,
-                          double* t_re, double* t_im, double* sd, hipStream_t st);
+                          double* t_re, double* t_im, double* sd, hipStream_t st,
+                          double* q_re = nullptr, double* q_im = nullptr);
 // O rows [B][P] from spins and t: mode 0 real-projected, 1 holomorphic.
 template <typename R>
 int launch_gen_O(const ModelView<R>& m, int kind, const int* spins, int B,
@@ -207,12 +238,14 @@ int launch_spins_to_bits(const int* spins, long n, int N, uint64_t* out, hipStre
 // With a workspace of at least eloc_ws_bytes(m, h, B) bytes the term-major
 // kernels run (term tables, 64 samples per wave, fixed-order sums);
 // without one, the per-sample item-list kernel.
+// t and q: tanh(theta) and the small logistic member of every sample's
+// hidden inputs, as launch_forward writes them.
 template <typename R>
 size_t eloc_ws_bytes(const ModelView<R>& m, const HamDev& h, int B);
 template <typename R>
 int launch_eloc(const ModelView<R>& m, const HamDev& h, const int* spins, int B,
-                const R* t_re, const R* t_im, double* e_re, double* e_im,
-                hipStream_t st, void* ws = nullptr, size_t ws_bytes = 0);
+                const R* t_re, const R* t_im, const R* q_re, const R* q_im, double* e_re,
+                double* e_im, hipStream_t st, void* ws = nullptr, size_t ws_bytes = 0);
 // E_loc from externally evaluated log-amplitudes of every connected
 // configuration (host-callback adaptor): la/ar of sample b at [b][0], of
 // its k-th connected configuration at [b][1+k] in term order.
@@ -334,6 +367,7 @@ struct MultiProb {
   const double* stats = nullptr; // [0] <E> (reference order), read by the force
   double* norm2 = nullptr;       // |delta|^2
   double* par = nullptr;         // [P] parameters, reference layout
+  const int* range_flag = nullptr;  // the ansatz tables' range verdict (nonzero: refused)
   // first block of this problem in each batched launch
   // (blk_g: one G product per problem, CG; blk_g2: two, the force)
   int blk_prep = 0, blk_z = 0, blk_rows = 0, blk_g = 0, blk_g2 = 0, blk_vec = 0;
@@ -356,9 +390,25 @@ int launch_multi_cg_iter(const MultiBatch& mb, int matvec_only, hipStream_t st);
 // par -= lr * x (reference layout), |x|^2
 int launch_multi_apply(const MultiBatch& mb, hipStream_t st);
 // per problem kMultiOut doubles: <E>, <E^2>, |delta|^2, CG iterations,
-// converged, breakdown, pAp, rs_new, rs0, accepted / proposed moves
-constexpr int kMultiOut = 10;
+// converged, breakdown, pAp, rs_new, rs0, accepted / proposed moves, the
+// tables' range flag
+constexpr int kMultiOut = 11;
 int launch_multi_gather(const MultiBatch& mb, double* out, hipStream_t st);
+
+// ---- excited-state SR and Marshall sign helpers (k_dyn.hip) -----------------
+// r = psi_ref / psi per sample from the two log-amplitudes (log-ratio
+// clamped to [-10, 10])
+int launch_excited_ratio(const double* la_p, const double* ar_p, const double* la_r,
+                         const double* ar_r, int B, double* r_re, double* r_im, hipStream_t st);
+// E += mu r conj(<r>), rm = {<r_re>, <r_im>} on the device
+int launch_excited_augment(double* e_re, double* e_im, const double* r_re, const double* r_im,
+                           const double* rm, double mu, int B, hipStream_t st);
+// closed-form augmented means of <E_re>, <E_im> (device scalars) -> out[2]
+int launch_excited_means(const double* er, const double* ei, const double* rm, double mu,
+                         double* out, hipStream_t st);
+// arg += pi where the Marshall sublattice holds an odd number of down spins
+int launch_marshall_arg(const int* spins, int B, int N, int Lx, int Ly, double* ar,
+                        hipStream_t st);
 
 // ---- bookkeeping ------------------------------------------------------------
 void count_launch(int n = 1);

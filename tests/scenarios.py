@@ -94,6 +94,17 @@ RUN = [
     dict(name="run_holo_kagome_2x2_crbm", method="holo", cfg=dict(ansatz=CRBM, rbm_hidden_units=12, hamiltonian=KAGOME, num_samples=64, num_thermalize=32, num_decorrelate=2, num_iterations=4, sr_diag_shift=1e-2, learning_rate=0.02), Lx=2, Ly=2),
 ]
 
+# Excited-state SR (orthogonality penalty against a second ansatz) and
+# real-time tVMC (Euler, Heun): src/nqs/nqs_optimizer.c:562-993
+DYN = [
+    dict(name="excited_tfim_2x2_rbm", method="excited", cfg=dict(ansatz=RBM, num_samples=64, num_thermalize=32, num_decorrelate=1, sr_diag_shift=1e-2, rbm_init_scale=0.3, learning_rate=0.05), ref=dict(ansatz=RBM, rbm_init_scale=0.5, rng_seed=7), mu=1.0, Lx=2, Ly=2, steps=2),
+    dict(name="excited_kagome_2x2_crbm", method="excited", cfg=dict(ansatz=CRBM, rbm_hidden_units=12, hamiltonian=KAGOME, num_samples=128, num_thermalize=32, num_decorrelate=2, sr_diag_shift=1e-2, rbm_init_scale=0.3, learning_rate=0.02), ref=dict(ansatz=CRBM, rbm_hidden_units=12, rbm_init_scale=0.4, rng_seed=13), mu=2.0, Lx=2, Ly=2, steps=2),
+    dict(name="tvmc_euler_heis_2x2_crbm", method="euler", cfg=dict(ansatz=CRBM, rbm_hidden_units=8, hamiltonian=HEIS, num_samples=64, num_thermalize=32, num_decorrelate=2, sr_diag_shift=1e-2, rbm_init_scale=0.3), dt=0.02, Lx=2, Ly=2, steps=2),
+    dict(name="tvmc_heun_tfim_4x1_crbm", method="heun", cfg=dict(ansatz=CRBM, rbm_hidden_units=8, num_samples=64, num_thermalize=32, num_decorrelate=2, sr_diag_shift=1e-2, rbm_init_scale=0.3, transverse_field=0.8), dt=0.05, Lx=4, Ly=1, steps=2),
+    dict(name="tvmc_euler_kagome_2x2_crbm", method="euler", cfg=dict(ansatz=CRBM, rbm_hidden_units=12, hamiltonian=KAGOME, num_samples=64, num_thermalize=32, num_decorrelate=2, sr_diag_shift=1e-2, rbm_init_scale=0.3), dt=0.02, Lx=2, Ly=2, steps=1),
+    dict(name="run_excited_tfim_2x2_rbm", method="run_excited", cfg=dict(ansatz=RBM, num_samples=64, num_thermalize=32, num_decorrelate=1, num_iterations=3, sr_diag_shift=1e-2, rbm_init_scale=0.3, learning_rate=0.05), ref=dict(ansatz=RBM, rbm_init_scale=0.5, rng_seed=7), mu=1.0, Lx=2, Ly=2),
+]
+
 
 def _cfg(d):
     return config_defaults(**d)
@@ -280,8 +291,51 @@ def run_run(r, sc):
     return {"trace": tr, "params": p}
 
 
+def run_dyn(r, sc):
+    cfg = _cfg(sc["cfg"])
+    Lx, Ly = sc["Lx"], sc["Ly"]
+    N = _n_sites(cfg, Lx, Ly)
+    h = r.ansatz(cfg, N)
+    fn, user = r.callback(h, None)
+    ref = None
+    if "ref" in sc:
+        ref = r.ansatz(_cfg(sc["ref"]), N)
+    smp = r.f("sampler_create")(N, ctypes.byref(cfg), fn, user)
+    m = sc["method"]
+    if m == "run_excited":
+        tr = np.zeros(cfg.num_iterations)
+        rc = r.f("sr_run_excited")(ctypes.byref(cfg), Lx, Ly, h, smp, fn, user, r.amp_fn, ref,
+                                   sc["mu"], ptr(tr))
+        assert rc == 0
+        out = {"trace": tr, "params": r.params(h)}
+    else:
+        r.f("sampler_thermalize")(smp)
+        params = [r.params(h)]
+        infos = []
+        for _ in range(sc["steps"]):
+            info = SrInfo()
+            if m == "excited":
+                rc = r.f("sr_step_excited")(ctypes.byref(cfg), Lx, Ly, h, smp, fn, user, r.amp_fn,
+                                            ref, sc["mu"], ctypes.byref(info))
+            else:
+                name = "tvmc_step_real_time" if m == "euler" else "tvmc_step_heun"
+                rc = r.f(name)(ctypes.byref(cfg), sc["dt"], Lx, Ly, h, smp, fn, user,
+                               ctypes.byref(info))
+            assert rc == 0
+            params.append(r.params(h))
+            infos.append([info.mean_energy, info.variance_energy, info.update_norm,
+                          info.acceptance_ratio, info.cg_iterations, info.converged])
+        out = {"params": np.array(params), "info": np.array(infos)}
+    r.f("sampler_free")(smp)
+    if ref:
+        r.f("ansatz_free")(ref)
+    r.f("ansatz_free")(h)
+    return out
+
+
 KINDS = {"init": (INIT, run_init), "amp": (AMP, run_amp), "eloc": (ELOC, run_eloc),
-         "sampler": (SAMPLER, run_sampler), "sr": (SR, run_sr), "run": (RUN, run_run)}
+         "sampler": (SAMPLER, run_sampler), "sr": (SR, run_sr), "run": (RUN, run_run),
+         "dyn": (DYN, run_dyn)}
 
 
 def run_all(lib, prefix, kinds=None, names=None):

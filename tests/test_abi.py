@@ -80,14 +80,13 @@ def test_option_defaults_agree(lib):
 
 def test_reference_header_functions_covered():
     """Every path function of the reference headers this library replaces
-    (ansatz, sampler, local energy, SR / holomorphic / MinSR, Marshall) has a
-    declaration here; tVMC and the excited-state step are out of scope."""
+    (ansatz, sampler, local energy, SR / holomorphic / MinSR, excited-state
+    SR, real-time tVMC, Marshall) has a declaration here."""
     ref = "/root/reference/include/nqs"
     if not os.path.isdir(ref):
         pytest.skip("reference checkout not present")
     ours = set(declared_functions())
-    skip = {"nqs_tvmc_step_real_time", "nqs_tvmc_step_heun", "nqs_sr_step_excited",
-            "nqs_sr_run_excited", "nqs_config_defaults"}
+    skip = {"nqs_config_defaults"}
     for h in ("nqs_ansatz.h", "nqs_sampler.h", "nqs_gradient.h", "nqs_optimizer.h",
               "nqs_marshall.h"):
         txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(ref, h)).read(), flags=re.S)

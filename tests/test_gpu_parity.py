@@ -117,6 +117,26 @@ def test_sr_run(hip_results, golden, sc):
     _close(got["params"], ref["params"], tol * 10)
 
 
+@pytest.mark.parametrize("sc", scenarios.DYN, ids=lambda s: s["name"])
+def test_dynamics_step(hip_results, golden, sc):
+    """Excited-state SR and real-time tVMC (Euler, Heun) on the device
+    against the reference golden vectors.  These are CG-family steps (the
+    holomorphic CG of src/nqs/nqs_optimizer.c:358-401), so the bar is the CG
+    family's: parameters within 1e-4 of the update size, energies within
+    1e-7, iteration counts within one (or 10%)."""
+    got = hip_results[f"dyn/{sc['name']}"]
+    ref = golden_case(golden, f"dyn/{sc['name']}")
+    if "trace" in ref:
+        _close(got["trace"], ref["trace"], 1e-6)
+        _close(got["params"], ref["params"], 1e-5)
+        return
+    step = np.abs(np.diff(ref["params"], axis=0)).max()
+    err = np.abs(got["params"] - ref["params"]).max()
+    assert err <= 1e-4 * step + 1e-12, f"param err {err:.3e} vs update size {step:.3e}"
+    _close(got["info"][:, :4], ref["info"][:, :4], 1e-7)
+    assert np.all(np.abs(got["info"][:, 4] - ref["info"][:, 4]) <= np.maximum(1, 0.1 * ref["info"][:, 4]))
+
+
 @pytest.mark.parametrize("sc", [s for s in scenarios.SR if s["method"] in ("cg", "holo")],
                          ids=lambda s: s["name"])
 def test_cg_step_solver_contract(hip, orc, sc):

@@ -87,6 +87,17 @@ def test_sr_runs(orc_results, golden, sc):
     _close(got["params"], ref["params"], 1e-10)
 
 
+@pytest.mark.parametrize("sc", scenarios.DYN, ids=lambda s: s["name"])
+def test_dynamics_match_reference_bit_exact(orc_results, golden, sc):
+    """Excited-state SR, real-time tVMC (Euler / Heun) and the excited run
+    against the compiled reference: the oracle sums in the reference's order,
+    so parameters and info fields agree bit for bit."""
+    got = orc_results[f"dyn/{sc['name']}"]
+    ref = golden_case(golden, f"dyn/{sc['name']}")
+    for k in ref:
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+
+
 def test_minsr_breakdown_zero_update(golden):
     ref = golden_case(golden, "sr/minsr_retry_tfim_4x1_rbm")
     # the reference reports the failed factorisation and applies no update
