@@ -216,6 +216,21 @@ int nqs_hip_last_stage_ms(const nqs_sampler_t *s, double out[7]);
 int nqs_hip_gram(const double *O, int B, long P, double scale, double *T);
 int nqs_hip_cholesky_solve(double *T, int n, const double *b, double *x);
 
+/* nqs_hip_cholesky_solve on device memory, allocating nothing: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc).  It waits
+ * for its own work before returning (the failure flag decides the retry), so
+ * it is not itself a graph-capturable call.  d_T [n][n] (row-major, lower
+ * triangle read; factored in place as nqs_hip_cholesky_solve leaves it),
+ * d_b [n] and d_x [n] are device pointers the caller owns (d_b and d_x
+ * distinct).  Every work buffer lives
+ * in one static device array of 64 MiB that the library holds on each GPU it
+ * runs on until the process ends: n <= 2319 fits, larger n is refused with a
+ * message (-1).  Calls that overlap on one GPU share that array and race.
+ * Returns 1 on success, 0 when both factorisation attempts fail (x = 0),
+ * -1 on errors. */
+int nqs_hip_cholesky_solve_noalloc(double *d_T, int n, const double *d_b, double *d_x);
+
 /* The MinSR update of nqs_sr_step_minsr_full (src/nqs/nqs_optimizer.c:
  * 1119-1177) for caller-given samples spins [B][N] and local energies
  * e_loc [B], with the ansatz's current parameters (real O for a real RBM,

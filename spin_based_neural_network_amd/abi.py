@@ -173,6 +173,7 @@ HIP_API = {
     "nqs_hip_copy_probe": (c_double, [c_long, c_int]),
     "nqs_hip_kernel_stats": (c_int, [c_char_p, POINTER(c_longlong), POINTER(c_double)]),
     "nqs_hip_cholesky_solve": (c_int, [_P, c_int, _P, _P]),
+    "nqs_hip_cholesky_solve_noalloc": (c_int, [_P, c_int, _P, _P]),
     "nqs_hip_ar_create": (c_void_p, [c_int, c_ulonglong, c_double]),
     "nqs_hip_ar_free": (None, [_P]),
     "nqs_hip_ar_num_params": (c_long, [_P]),

@@ -2283,6 +2283,82 @@ NQS_EXPORT int nqs_hip_cholesky_solve(double* T, int n, const double* b, double*
   return f == 0 ? 1 : 0;
 }
 
+// ---- allocation-free Cholesky solve ------------------------------------------
+// Device memory of nqs_hip_cholesky_solve_noalloc: one file-scope array that
+// every GPU the library runs on holds (its own copy, until the process ends).
+// Layout for an n x n system: the failure flag (first 256 bytes), the copy of
+// the input the reference's retry repairs from (n^2 doubles), the block
+// inverses of the factorisation (potrf_linv_elems(n)), and the solve's two
+// vectors (2n), each region 256-byte aligned.  64 MiB serve every n <= 2319.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// the array's device address on the current device, looked up once per
+// device with that device current (the name itself is a host-side stand-in)
+double* scratch_device_ptr() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    fprintf(stderr, "nqs_hip: no usable current device for the allocation-free scratch\n");
+    return nullptr;
+  }
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "nqs_hip: hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev,
+              hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_dev[dev] = (double*)p;
+  }
+  return g_scratch_dev[dev];
+}
+
+// offsets (in doubles) of the regions, each 256-byte aligned as hipMalloc's
+size_t noalloc_align(size_t d) { return (d + 31) & ~(size_t)31; }
+size_t noalloc_linv(int n) { return noalloc_align(32 + (size_t)n * n); }
+size_t noalloc_w(int n) { return noalloc_align(noalloc_linv(n) + nqs::potrf_linv_elems(n)); }
+size_t noalloc_doubles(int n) { return noalloc_w(n) + 2 * (size_t)n; }
+}  // namespace
+
+NQS_EXPORT int nqs_hip_cholesky_solve_noalloc(double* d_T, int n, const double* d_b, double* d_x) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!d_T || !d_b || !d_x || n <= 0) return -1;
+  const size_t need = noalloc_doubles(n) * sizeof(double);
+  if (need > kNoAllocScratchBytes) {
+    int nmax = 1;
+    while (noalloc_doubles(nmax + 1) * sizeof(double) <= kNoAllocScratchBytes) nmax++;
+    fprintf(stderr, "nqs_hip_cholesky_solve_noalloc: n = %d needs %zu bytes of scratch; the "
+                    "library holds %zu (n <= %d)\n", n, need, kNoAllocScratchBytes, nmax);
+    return -1;
+  }
+  if (stream_ready()) return -1;
+  double* base = scratch_device_ptr();
+  if (!base) return -1;
+  int* fail = reinterpret_cast<int*>(base);          // zeroed by launch_potrf's own kernel
+  double* orig = base + 32;
+  double* Linv = base + noalloc_linv(n);
+  double* w = base + noalloc_w(n);
+  // the input, for the retry's repair of the partially factored columns
+  CK(hipMemcpyAsync(orig, d_T, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice, g_stream));
+  int f = nqs::launch_potrf(d_T, n, Linv, fail, g_stream);
+  if (f < 0) return -1;
+  if (f > 0) {
+    CKR(nqs::launch_potrf_repair(d_T, n, f - 1, orig, g_stream));
+    CKR(nqs::launch_add_diag(d_T, n, n, 1e-6, g_stream));
+    f = nqs::launch_potrf(d_T, n, Linv, fail, g_stream);
+    if (f < 0) return -1;
+  }
+  if (f == 0) CKR(nqs::launch_potrs(d_T, Linv, n, d_b, d_x, w, g_stream));
+  else CKR(nqs::launch_fill(d_x, 0.0, n, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return f == 0 ? 1 : 0;
+}
+
 NQS_EXPORT int nqs_hip_minsr_solve(nqs_ansatz_t* a, const int* spins, int B, const double* e_loc,
                                    double shift, int gram, double* out_delta, double* out_y,
                                    double* out_T) {
