@@ -206,7 +206,7 @@ def traffic_bytes(workload, tag):
     return None if v is None else v["fetch"] + v["write"]
 
 
-TAGS = ["mh", "ar_mh", "forward", "eloc", "gen_O", "colsum", "gemm", "gemm_potrf", "potrf_panel",
+TAGS = ["mh", "ar_mh", "forward", "eloc", "gen_O", "colsum", "gemm", "gemm_small", "gemm_potrf", "potrf_panel",
         "trsv", "trsv_inv", "gram_f", "multi_prep", "multi_gemm_z", "multi_rows", "multi_gemm_g", "multi_vec"]
 
 

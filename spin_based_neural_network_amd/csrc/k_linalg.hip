@@ -377,7 +377,7 @@ static int gemm_nt_impl(int M, int N, int K, double alpha, const double* A, long
   if (lower && M != N) return -1;
   if (mask < 0) mask = lower;
   const int use = !skip ? 0 : (mask ? 1 : 2);
-  KScope ks(use == 0 ? "gemm" : (use == 1 ? "gemm_inner" : "gemm_trsm"), st);
+  KScope ks(use == 0 ? "gemm_small" : (use == 1 ? "gemm_inner" : "gemm_trsm"), st);
   if (use == 0)
     hipLaunchKernelGGL(k_gemm_nt<0>, dim3((unsigned)tiles), dim3(256), 0, st, M, N, K, alpha, A,
                        lda, B, ldb, beta, C, ldc, lower, mask, tn, skip);
@@ -400,11 +400,17 @@ static int gemm_nt_impl(int M, int N, int K, double alpha, const double* A, long
 // 32 double-buffered through LDS (global loads of chunk c+1 in flight while
 // chunk c feeds the MFMAs), one workgroup per CU: a Cholesky panel beside a
 // trailing update then finds whole CUs as soon as update tiles retire.  The
-// factored Gram uses chunks of 16 instead (68 KB of LDS, at most 128 VGPRs:
+// factored Gram uses chunks of 16 instead (72 KB of LDS, at most 128 VGPRs:
 // two workgroups per CU, one's barrier waits covered by the other's MFMAs).
 // Same lower / mask / skip conventions as k_gemm_nt.
 // --------------------------------------------------------------------------
 constexpr int BT = 128;
+// LDS row stride of the K chunks: BKC + 2 doubles.  A half-wave's operand
+// read touches rows lr (16) x columns lk (2); with a stride of 2 mod 32
+// doubles the 8-byte bank of (lr, lk) is 2 lr + lk (BKC = 32) or
+// 18 lr + lk (BKC = 16), 32 distinct banks, where a stride of BKC + 1 maps
+// (lr, lk) and (lr + 1, lk - 1) to one bank.
+constexpr int kLdsPad = 2;
 
 // K loop shared by the large-tile kernels: acc += A[rows] B[cols]^T over K,
 // in chunks of BKC (16 or 32) columns.  Each MFMA step of a chunk is one
@@ -417,8 +423,8 @@ constexpr int BT = 128;
 template <int BKC>
 __device__ __attribute__((always_inline)) inline void tile_kloop(
     const double* A, long lda, const double* __restrict__ Bm, long ldb, int M, int N, int K,
-    int row0, int col0, v4d (&acc)[4][2], double (&As)[2][BT][BKC + 1],
-    double (&Bs)[2][BT][BKC + 1]) {
+    int row0, int col0, v4d (&acc)[4][2], double (&As)[2][BT][BKC + kLdsPad],
+    double (&Bs)[2][BT][BKC + kLdsPad]) {
   if (K <= 0) return;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wr = (wave >> 2) * 64, wc = (wave & 3) * 32;
@@ -514,8 +520,8 @@ __global__ __launch_bounds__(512) void k_gemm_big(int M, int N, int K, double al
   // 8 waves, 2 per SIMD: wave (wm, wn) computes rows wm*64 + [0, 64) and
   // columns wn*32 + [0, 32) of the tile as 4x2 MFMA blocks (tile_kloop).
   const int skipped = skip ? *skip : 0;
-  __shared__ double As[2][BT][32 + 1];
-  __shared__ double Bs[2][BT][32 + 1];
+  __shared__ double As[2][BT][32 + kLdsPad];
+  __shared__ double Bs[2][BT][32 + kLdsPad];
   int tm, tn;
   if (lower) lower_tile(blockIdx.x, tm, tn);
   else { tm = blockIdx.x / tiles_n; tn = blockIdx.x % tiles_n; }
@@ -1978,8 +1984,8 @@ __global__ __launch_bounds__(512, 2) void k_gram_fused(int B, const unsigned lon
                                                     const double* __restrict__ rm,
                                                     const double* __restrict__ gm, double eps,
                                                     double* __restrict__ T) {
-  __shared__ double As[2][BT][16 + 1];
-  __shared__ double Bs[2][BT][16 + 1];
+  __shared__ double As[2][BT][16 + kLdsPad];
+  __shared__ double Bs[2][BT][16 + kLdsPad];
   int tm, tn;
   lower_tile(blockIdx.x, tm, tn);
   const int row0 = tm * BT, col0 = tn * BT;
