@@ -129,6 +129,10 @@ def setup(lib, w, args, move=None):
         opt.ar_lr = LEARNED_LR
         opt.ar_train_every = 10
         opt.ar_train_window = 16384
+        if getattr(args, "ar_local", None) is not None:
+            opt.ar_local = args.ar_local
+        if getattr(args, "ar_epochs", None) is not None:
+            opt.ar_epochs = args.ar_epochs
     s = lib.nqs_hip_sampler_create_ex(N, ctypes.byref(cfg), ctypes.byref(opt), fn, user)
     if not s:
         raise RuntimeError("nqs_hip_sampler_create_ex failed")
@@ -744,6 +748,10 @@ def main():
     ap.add_argument("--move", default=None, choices=["flip", "exchange", "learned"])
     ap.add_argument("--budget", type=int, default=None, metavar="K",
                     help="optimisation iterations before timing (default: the workload's, C5 300)")
+    ap.add_argument("--ar-local", type=int, default=None, metavar="K",
+                    help="learned proposal: exchange moves before each learned move")
+    ap.add_argument("--ar-epochs", type=int, default=None, metavar="E",
+                    help="learned proposal: passes over the window per retraining")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of "
