@@ -1596,3 +1596,112 @@ int orc_sr_run_holomorphic(const nqs_config_t *cfg, int Lx, int Ly, orc_ansatz_t
                            orc_sampler_t *s, orc_log_amp_fn fn, void *user, double *trace) {
     return orc_sr_run_holomorphic_full(cfg, Lx, Ly, a, s, fn, user, NULL, NULL, trace);
 }
+
+/* ====================================================================== */
+/* sample diagnostics                                                      */
+/* ====================================================================== */
+
+/* nqs_diagnostics.c:14-92: a fresh batch, the complex gradient of every
+ * sample, Tr(S) = sum_k <|O_k|^2> - |<O_k>|^2 with values in (-1e-12, 0)
+ * taken as 0. */
+int orc_compute_chi_F(const nqs_config_t *cfg, int size_x, int size_y,
+                      orc_ansatz_t *a, orc_sampler_t *s,
+                      double *out_trace_S, double *out_per_param) {
+    (void)size_x; (void)size_y;
+    if (!cfg || !a || !s || !out_trace_S) return -1;
+    const long P = orc_ansatz_num_params(a);
+    const int N = orc_sampler_num_sites(s), B = cfg->num_samples;
+    if (P <= 0 || B <= 0) return -1;
+    int *smp = malloc((size_t)B * N * sizeof(int));
+    double *acc = calloc((size_t)3 * P, sizeof(double));   /* sum re, sum im, sum |O|^2 */
+    double *g = malloc((size_t)2 * P * sizeof(double));
+    int rc = (!smp || !acc || !g) ? -1 : orc_sampler_batch(s, B, smp);
+    for (int b = 0; rc == 0 && b < B; b++) {
+        if (orc_ansatz_logpsi_gradient_complex(a, smp + (size_t)b * N, N, g, g + P)) { rc = -1; break; }
+        for (long k = 0; k < P; k++) {
+            acc[k] += g[k];
+            acc[P + k] += g[P + k];
+            acc[2 * P + k] += g[k] * g[k] + g[P + k] * g[P + k];
+        }
+    }
+    if (rc == 0) {
+        const double inv = 1.0 / B;
+        double tr = 0.0;
+        for (long k = 0; k < P; k++) {
+            const double mr = acc[k] * inv, mi = acc[P + k] * inv;
+            double v = acc[2 * P + k] * inv - (mr * mr + mi * mi);
+            if (v < 0.0 && v > -1e-12) v = 0.0;
+            tr += v;
+        }
+        *out_trace_S = tr;
+        if (out_per_param) *out_per_param = tr / (double)P;
+    }
+    free(smp); free(acc); free(g);
+    return rc;
+}
+
+/* nqs_diagnostics.c:113-222: per cell (cx-major) the up-triangle bonds
+ * A-B, A-C, B-C and, with periodic cells or cx, cy >= 1, the down-triangle
+ * bonds A-Bm, A-Cm, Bm-Cm (Bm: B of cell (cx-1, cy), Cm: C of cell
+ * (cx, cy-1)); for every opposite-spin bond of every sample the complex
+ * ratio psi(s with both spins flipped) / psi(s) from two log-amplitude
+ * evaluations, summed per sublattice class in that order. */
+int orc_compute_kagome_bond_phase(const nqs_config_t *cfg, int Lx, int Ly,
+                                  orc_ansatz_t *a, orc_sampler_t *s,
+                                  double *out_re, double *out_im, long *out_counts) {
+    if (!cfg || !a || !s || !out_re || !out_im) return -1;
+    if (cfg->hamiltonian != NQS_HAM_KAGOME_HEISENBERG) return -2;
+    if (Lx < 1 || Ly < 1) return -1;
+    const int N = 3 * Lx * Ly, B = cfg->num_samples;
+    if (orc_sampler_num_sites(s) != N || B <= 0) return -1;
+    /* bond list of one sample, in the order of the walk */
+    int (*bond)[2] = malloc((size_t)6 * Lx * Ly * sizeof *bond);
+    int nb = 0;
+    for (int cx = 0; cx < Lx; cx++)
+        for (int cy = 0; cy < Ly; cy++) {
+            const int A = 3 * (cx * Ly + cy);
+            bond[nb][0] = A; bond[nb][1] = A + 1; nb++;
+            bond[nb][0] = A; bond[nb][1] = A + 2; nb++;
+            bond[nb][0] = A + 1; bond[nb][1] = A + 2; nb++;
+            if (!cfg->kagome_pbc && (cx == 0 || cy == 0)) continue;
+            const int bm = 3 * (((cx - 1 + Lx) % Lx) * Ly + cy) + 1;
+            const int cm = 3 * (cx * Ly + (cy - 1 + Ly) % Ly) + 2;
+            bond[nb][0] = A; bond[nb][1] = bm; nb++;
+            bond[nb][0] = A; bond[nb][1] = cm; nb++;
+            bond[nb][0] = bm; bond[nb][1] = cm; nb++;
+        }
+    int *smp = malloc((size_t)B * N * sizeof(int));
+    int *w = malloc((size_t)N * sizeof(int));
+    double sr[3] = {0, 0, 0}, si[3] = {0, 0, 0};
+    long cnt[3] = {0, 0, 0};
+    int rc = orc_sampler_batch(s, B, smp);
+    for (int b = 0; rc == 0 && b < B; b++) {
+        const int *x = smp + (size_t)b * N;
+        memcpy(w, x, (size_t)N * sizeof(int));
+        double l0, p0;
+        orc_ansatz_log_amp(x, N, a, &l0, &p0);
+        for (int t = 0; t < nb; t++) {
+            const int u = bond[t][0], v = bond[t][1];
+            if (x[u] == x[v]) continue;
+            const int su = u % 3, sv = v % 3;
+            const int lo = su < sv ? su : sv, hi = su < sv ? sv : su;
+            const int c = lo == 0 ? (hi == 1 ? 0 : 1) : 2;
+            w[u] = -x[u]; w[v] = -x[v];
+            double l1, p1;
+            orc_ansatz_log_amp(w, N, a, &l1, &p1);
+            w[u] = x[u]; w[v] = x[v];
+            const double m = exp(l1 - l0);
+            sr[c] += m * cos(p1 - p0);
+            si[c] += m * sin(p1 - p0);
+            cnt[c]++;
+        }
+    }
+    if (rc == 0)
+        for (int c = 0; c < 3; c++) {
+            out_re[c] = cnt[c] ? sr[c] / (double)cnt[c] : 0.0;
+            out_im[c] = cnt[c] ? si[c] / (double)cnt[c] : 0.0;
+            if (out_counts) out_counts[c] = cnt[c];
+        }
+    free(bond); free(smp); free(w);
+    return rc;
+}

@@ -197,6 +197,15 @@ double orc_ed_ground_energy(const nqs_config_t *cfg, int size_x, int size_y,
 double orc_exact_energy(const nqs_config_t *cfg, int size_x, int size_y,
                         int periodic, orc_log_amp_fn fn, void *user);
 
+/* ---- sample diagnostics (src/nqs/nqs_diagnostics.c) ------------------ */
+int orc_compute_chi_F(const nqs_config_t *cfg, int size_x, int size_y,
+                      orc_ansatz_t *a, orc_sampler_t *s,
+                      double *out_trace_S, double *out_per_param);
+int orc_compute_kagome_bond_phase(const nqs_config_t *cfg, int Lx_cells, int Ly_cells,
+                                  orc_ansatz_t *a, orc_sampler_t *s,
+                                  double *out_mean_re, double *out_mean_im,
+                                  long *out_counts);
+
 /* ---- statistics ----------------------------------------------------- */
 double orc_tau_int(const double *x, int n);
 

@@ -146,6 +146,8 @@ API = {
     "sr_step_minsr_full": (c_int, [_CFG, c_int, c_int, _P, _P, _P, _P, _P, _P, _INFO]),
     "sr_run_minsr": (c_int, [_CFG, c_int, c_int, _P, _P, _P, _P, _P]),
     "sr_step_excited": (c_int, [_CFG, c_int, c_int, _P, _P, _P, _P, _P, _P, c_double, _INFO]),
+    "compute_chi_F": (c_int, [_CFG, c_int, c_int, _P, _P, _P, _P]),
+    "compute_kagome_bond_phase": (c_int, [_CFG, c_int, c_int, _P, _P, _P, _P, _P]),
     "sr_run_excited": (c_int, [_CFG, c_int, c_int, _P, _P, _P, _P, _P, _P, c_double, _P]),
     "tvmc_step_real_time": (c_int, [_CFG, c_double, c_int, c_int, _P, _P, _P, _P, _INFO]),
     "tvmc_step_heun": (c_int, [_CFG, c_double, c_int, c_int, _P, _P, _P, _P, _INFO]),

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/nqs/nqs_ansatz.h"
+#include "../../include/nqs/nqs_diagnostics.h"
 #include "../../include/nqs/nqs_gradient.h"
 #include "../../include/nqs/nqs_marshall.h"
 #include "../../include/nqs/nqs_optimizer.h"
@@ -312,7 +313,8 @@ bool recognise(nqs_log_amp_fn_t fn, void* user, AmpSource& src) {
 // ---------------------------------------------------------------------------
 struct Workspace {
   DBuf spins, t_re, t_im, q_re, q_im, la, ar, e_re, e_im, O, Oi, colwork, u1, u2, vecs, T, diag,
-      y, scal, cg, fail, ham, sublat, hostla, hostar, linv, T2, Sd, elw, fwd, xr, heun;
+      y, scal, cg, fail, ham, sublat, hostla, hostar, linv, T2, Sd, elw, fwd, xr, heun, bond_ham,
+      bond_sub;
 };
 Workspace g_ws;
 
@@ -2357,6 +2359,169 @@ NQS_EXPORT int nqs_hip_cholesky_solve_noalloc(double* d_T, int n, const double* 
   else CKR(nqs::launch_fill(d_x, 0.0, n, g_stream));
   CK(hipStreamSynchronize(g_stream));
   return f == 0 ? 1 : 0;
+}
+
+// ===========================================================================
+// sample diagnostics (src/nqs/nqs_diagnostics.c)
+// ===========================================================================
+namespace {
+
+// the sampler's next cfg->num_samples configurations, as nqs_sampler_batch
+// draws them, into s->d_samples (and h_out when given)
+int diag_batch(nqs_sampler* s, int B, int* h_out) {
+  const int C = s->opt.num_chains;
+  const int k = s->cfg.num_decorrelate > 0 ? s->cfg.num_decorrelate : 1;
+  const int rounds = (B + C - 1) / C;
+  CKR(s->d_samples.ensure(sizeof(int) * (size_t)B * s->N));
+  std::vector<int> tmp;
+  if (!h_out && !s->device) {
+    tmp.resize((size_t)B * s->N);
+    h_out = tmp.data();
+  }
+  CKR(sampler_advance(s, k, rounds, B, s->d_samples.as<int>(), h_out));
+  return 0;
+}
+
+}  // namespace
+
+// Tr(S) of the quantum geometric tensor over a fresh batch
+// (nqs_diagnostics.c:14-92): holomorphic O rows of every sample on the
+// device, column moments in a fixed order.
+NQS_EXPORT int nqs_compute_chi_F(const nqs_config_t* cfg, int size_x, int size_y,
+                                 nqs_ansatz_t* a, nqs_sampler_t* s, double* out_trace_S,
+                                 double* out_per_param) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  (void)size_x;
+  (void)size_y;
+  if (!cfg || !a || !s || !out_trace_S) return -1;
+  const long P = a->P;
+  const int B = cfg->num_samples;
+  if (P <= 0 || B <= 0) return -1;
+  if (a->N != s->N) {
+    fprintf(stderr, "nqs_compute_chi_F: the ansatz has %d sites and the sampler %d\n", a->N, s->N);
+    return -1;
+  }
+  if (stream_ready()) return -1;
+  CKR(diag_batch(s, B, nullptr));
+  a->dev_fresh = 0;
+  CKR(ansatz_upload(a));
+  const int* d_samples = s->d_samples.as<int>();
+  CKR(forward_t<double>(a, d_samples, B, nullptr, nullptr));
+  const bool cplx = a->kind == NQS_ANSATZ_COMPLEX_RBM;
+  CKR(g_ws.O.ensure(sizeof(double) * (size_t)B * (size_t)P));
+  if (cplx) CKR(g_ws.Oi.ensure(sizeof(double) * (size_t)B * (size_t)P));
+  double* Or = g_ws.O.as<double>();
+  double* Oi = cplx ? g_ws.Oi.as<double>() : nullptr;
+  ModelView<double> v;
+  CKR(model_view<double>(a, v));
+  CKR(nqs::launch_gen_O<double>(v, a->kind, d_samples, B, g_ws.t_re.as<double>(),
+                                g_ws.t_im.as<double>(), cplx ? 1 : 0, Or, Oi, P, g_stream));
+  CKR(g_ws.colwork.ensure(sizeof(double) * nqs::chi_work_elems(P)));
+  CKR(g_ws.scal.ensure(sizeof(double) * 16));
+  CKR(nqs::launch_chi_trace(Or, Oi, B, P, g_ws.colwork.as<double>(), g_ws.scal.as<double>(),
+                            g_stream));
+  double tr = 0.0;
+  CK(hipMemcpyAsync(&tr, g_ws.scal.p, sizeof(double), hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  *out_trace_S = tr;
+  if (out_per_param) *out_per_param = tr / (double)P;
+  return 0;
+}
+
+// Per-bond-class mean of psi(s_ij)/psi(s) over the opposite-spin kagome
+// bonds of a fresh batch (nqs_diagnostics.c:113-222).  Bonds in the
+// reference's order: per cell (cx, cy), cx-major, the up triangle (A,B),
+// (A,C), (B,C), then, with periodic cells or cx, cy >= 1, the down triangle
+// (A,Bm), (A,Cm), (Bm,Cm) with Bm the B site of cell (cx-1, cy) and Cm the C
+// site of cell (cx, cy-1).  Class of a bond: A-B 0, A-C 1, B-C 2.  The
+// ratios of all (bond, sample) pairs come from the E_loc pair kernel; they
+// are added per class in the reference's order (sample, then bond).
+NQS_EXPORT int nqs_compute_kagome_bond_phase(const nqs_config_t* cfg, int Lx_cells, int Ly_cells,
+                                             nqs_ansatz_t* a, nqs_sampler_t* s,
+                                             double* out_mean_re, double* out_mean_im,
+                                             long* out_counts) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!cfg || !a || !s || !out_mean_re || !out_mean_im) return -1;
+  if (cfg->hamiltonian != NQS_HAM_KAGOME_HEISENBERG) return -2;
+  if (Lx_cells < 1 || Ly_cells < 1) return -1;
+  const int N = 3 * Lx_cells * Ly_cells, pbc = cfg->kagome_pbc;
+  if (s->N != N) return -1;
+  const int B = cfg->num_samples;
+  if (B <= 0) return -1;
+  if (a->N != N) {
+    fprintf(stderr, "nqs_compute_kagome_bond_phase: the ansatz has %d sites, the lattice %d\n",
+            a->N, N);
+    return -1;
+  }
+  if (stream_ready()) return -1;
+  nqs::HamHost hh;
+  hh.n_sites = N;
+  std::vector<int> cls;
+  auto sub_class = [](int u, int w) {
+    const int lo = std::min(u % 3, w % 3), hi = std::max(u % 3, w % 3);
+    return lo == 0 ? (hi == 1 ? 0 : 1) : 2;
+  };
+  for (int cx = 0; cx < Lx_cells; cx++)
+    for (int cy = 0; cy < Ly_cells; cy++) {
+      const int A = 3 * (cx * Ly_cells + cy), Bs = A + 1, Cs = A + 2;
+      const int up[3][2] = {{A, Bs}, {A, Cs}, {Bs, Cs}};
+      for (const auto& e : up) {
+        hh.pair_a.push_back(e[0]);
+        hh.pair_b.push_back(e[1]);
+      }
+      if (!pbc && (cx == 0 || cy == 0)) continue;
+      const int cxm = (cx - 1 + Lx_cells) % Lx_cells, cym = (cy - 1 + Ly_cells) % Ly_cells;
+      const int Bm = 3 * (cxm * Ly_cells + cy) + 1, Cm = 3 * (cx * Ly_cells + cym) + 2;
+      const int dn[3][2] = {{A, Bm}, {A, Cm}, {Bm, Cm}};
+      for (const auto& e : dn) {
+        hh.pair_a.push_back(e[0]);
+        hh.pair_b.push_back(e[1]);
+      }
+    }
+  const int T = (int)hh.pair_a.size();
+  for (int t = 0; t < T; t++) cls.push_back(sub_class(hh.pair_a[t], hh.pair_b[t]));
+  hh.pair_c.assign(T, 1.0);
+  std::vector<int> h_samples((size_t)B * N);
+  CKR(diag_batch(s, B, h_samples.data()));
+  AmpSource src;
+  src.a = a;
+  nqs::HamDev H;
+  CKR(ham_upload(hh, src, g_ws.bond_ham, g_ws.bond_sub, H));
+  a->dev_fresh = 0;
+  CKR(ansatz_upload(a));
+  ModelView<double> v;
+  CKR(ratio_view<double>(a, v));
+  const int* d_samples = s->d_samples.as<int>();
+  CKR(forward_t<double>(a, d_samples, B, nullptr, nullptr, true));
+  const size_t wb = nqs::eloc_ws_bytes<double>(v, H, B);
+  CKR(g_ws.elw.ensure(wb));
+  double* res[2] = {nullptr, nullptr};
+  CKR(nqs::launch_pair_ratios<double>(v, H, d_samples, B, g_ws.t_re.as<double>(),
+                                      g_ws.q_re.as<double>(), g_ws.q_im.as<double>(), g_ws.elw.p,
+                                      wb, res, g_stream));
+  std::vector<double> rr((size_t)T * B), ri((size_t)T * B);
+  CK(hipMemcpyAsync(rr.data(), res[0], sizeof(double) * rr.size(), hipMemcpyDeviceToHost,
+                    g_stream));
+  CK(hipMemcpyAsync(ri.data(), res[1], sizeof(double) * ri.size(), hipMemcpyDeviceToHost,
+                    g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  double sr[3] = {0, 0, 0}, si[3] = {0, 0, 0};
+  long cnt[3] = {0, 0, 0};
+  for (int b = 0; b < B; b++) {
+    const int* sp = h_samples.data() + (size_t)b * N;
+    for (int t = 0; t < T; t++) {
+      if (sp[hh.pair_a[t]] == sp[hh.pair_b[t]]) continue;
+      sr[cls[t]] += rr[(size_t)t * B + b];
+      si[cls[t]] += ri[(size_t)t * B + b];
+      cnt[cls[t]]++;
+    }
+  }
+  for (int c = 0; c < 3; c++) {
+    out_mean_re[c] = cnt[c] ? sr[c] / (double)cnt[c] : 0.0;
+    out_mean_im[c] = cnt[c] ? si[c] / (double)cnt[c] : 0.0;
+    if (out_counts) out_counts[c] = cnt[c];
+  }
+  return 0;
 }
 
 NQS_EXPORT int nqs_hip_minsr_solve(nqs_ansatz_t* a, const int* spins, int B, const double* e_loc,

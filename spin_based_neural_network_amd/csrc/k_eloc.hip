@@ -558,10 +558,13 @@ size_t eloc_ws_bytes(const ModelView<R>& m, const HamDev& h, int B) {
   return ((tab + 255) & ~(size_t)255) + 2 * T * (size_t)B * sizeof(double) + 256;
 }
 
+// With e_re == nullptr only the per-(term, sample) contributions are formed
+// (res_r / res_i in the workspace, term-major); their addresses go to
+// *res_out.
 template <typename R, bool CPLX>
 static int eloc_terms_go(const ModelView<R>& m, const HamDev& h, const int* spins, int B,
                          const R* t_re, const R* q_re, const R* q_im, double* e_re, double* e_im,
-                         void* ws, hipStream_t st) {
+                         void* ws, hipStream_t st, double** res_out = nullptr) {
   const int T = h.n_flip + h.n_pair, M = m.M, Tp = padded_terms(T);
   const size_t TM = (size_t)Tp * M;
   unsigned char* p = (unsigned char*)ws;
@@ -586,11 +589,34 @@ static int eloc_terms_go(const ModelView<R>& m, const HamDev& h, const int* spin
                        q_im, Fpr, Fpi, Fmr, Fmi, res_r, res_i);
     count_launch();
   }
-  hipLaunchKernelGGL(k_eloc_terms_sum, dim3((B + 255) / 256), dim3(256), 0, st, h, spins, B, m.N,
-                     res_r, res_i, e_re, e_im);
-  count_launch();
+  if (res_out) {
+    res_out[0] = res_r;
+    res_out[1] = res_i;
+  }
+  if (e_re) {
+    hipLaunchKernelGGL(k_eloc_terms_sum, dim3((B + 255) / 256), dim3(256), 0, st, h, spins, B,
+                       m.N, res_r, res_i, e_re, e_im);
+    count_launch();
+  }
   NQS_HIP_CHECK(hipGetLastError());
   return 0;
+}
+
+template <typename R>
+int launch_pair_ratios(const ModelView<R>& m, const HamDev& h, const int* spins, int B,
+                       const R* t_re, const R* q_re, const R* q_im, void* ws, size_t ws_bytes,
+                       double** res, hipStream_t st) {
+  if (B <= 0) return 0;
+  if (m.N > kMaxSites || h.n_flip + h.n_pair >= 65536 || !ws || ws_bytes < eloc_ws_bytes(m, h, B) ||
+      (m.M > 0 && !q_re)) {
+    fprintf(stderr, "nqs_hip: amplitude ratios need N <= %d, < 65536 terms, the logistic members "
+                    "and an E_loc workspace\n", kMaxSites);
+    return -1;
+  }
+  return m.cplx ? eloc_terms_go<R, true>(m, h, spins, B, t_re, q_re, q_im, nullptr, nullptr, ws, st,
+                                         res)
+                : eloc_terms_go<R, false>(m, h, spins, B, t_re, q_re, q_im, nullptr, nullptr, ws,
+                                          st, res);
 }
 
 template <typename R, bool CPLX, int IPT>
@@ -702,6 +728,7 @@ int launch_eloc_from_logamps(const HamDev& h, const int* spins, int B, int N, in
 
 template int launch_eloc<double>(const ModelView<double>&, const HamDev&, const int*, int, const double*, const double*, const double*, const double*, double*, double*, hipStream_t, void*, size_t);
 template int launch_eloc<float>(const ModelView<float>&, const HamDev&, const int*, int, const float*, const float*, const float*, const float*, double*, double*, hipStream_t, void*, size_t);
+template int launch_pair_ratios<double>(const ModelView<double>&, const HamDev&, const int*, int, const double*, const double*, const double*, void*, size_t, double**, hipStream_t);
 template size_t eloc_ws_bytes<double>(const ModelView<double>&, const HamDev&, int);
 template size_t eloc_ws_bytes<float>(const ModelView<float>&, const HamDev&, int);
 

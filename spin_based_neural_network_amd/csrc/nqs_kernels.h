@@ -246,12 +246,26 @@ template <typename R>
 int launch_eloc(const ModelView<R>& m, const HamDev& h, const int* spins, int B,
                 const R* t_re, const R* t_im, const R* q_re, const R* q_im, double* e_re,
                 double* e_im, hipStream_t st, void* ws = nullptr, size_t ws_bytes = 0);
+// psi(s')/psi(s) of every (term, sample) pair of h with coefficient and
+// sublattice sign applied (zero for inactive pairs), term-major [T][B], in
+// the workspace (eloc_ws_bytes): res[0] real parts, res[1] imaginary parts.
+template <typename R>
+int launch_pair_ratios(const ModelView<R>& m, const HamDev& h, const int* spins, int B,
+                       const R* t_re, const R* q_re, const R* q_im, void* ws, size_t ws_bytes,
+                       double** res, hipStream_t st);
 // E_loc from externally evaluated log-amplitudes of every connected
 // configuration (host-callback adaptor): la/ar of sample b at [b][0], of
 // its k-th connected configuration at [b][1+k] in term order.
 int launch_eloc_from_logamps(const HamDev& h, const int* spins, int B, int N,
                              int n_terms, const double* la, const double* ar,
                              double* e_re, double* e_im, hipStream_t st);
+
+// ---- sample diagnostics (k_diag.hip) --------------------------------------
+// out[0] = sum_k (<|O_k|^2> - |<O_k>|^2) over rows O_re [B][P] (+ i O_im,
+// may be null); work: chi_work_elems(P) doubles
+size_t chi_work_elems(long P);
+int launch_chi_trace(const double* O_re, const double* O_im, long B, long P, double* work,
+                     double* out, hipStream_t st);
 
 // ---- linear algebra (k_linalg.hip), FP64 ---------------------------------
 // out[k] = alpha * sum_b O[b][k] * (w ? w[b] : 1)  (deterministic order)

@@ -333,9 +333,47 @@ def run_dyn(r, sc):
     return out
 
 
+# sample diagnostics (src/nqs/nqs_diagnostics.c): Tr(S) and the kagome
+# bond-class ratio means over a fresh batch after thermalisation
+DIAG = [
+    dict(name="chi_tfim_2x2_crbm", what="chi", cfg=dict(ansatz=CRBM, rbm_hidden_units=8, rbm_init_scale=0.05, num_samples=1024, num_thermalize=256, num_decorrelate=2, rng_seed=0xC4F0), Lx=2, Ly=2),
+    dict(name="chi_tfim_2x2_mf", what="chi", cfg=dict(ansatz=MF, num_samples=512, num_thermalize=256, num_decorrelate=2, rng_seed=0xC4F1), Lx=2, Ly=2),
+    dict(name="chi_heis_4x2_rbm", what="chi", cfg=dict(ansatz=RBM, hamiltonian=HEIS, rbm_hidden_units=12, rbm_init_scale=0.4, num_samples=700, num_thermalize=64, num_decorrelate=3, rng_seed=91), Lx=4, Ly=2),
+    dict(name="chi_kagome_2x2_crbm", what="chi", cfg=dict(ansatz=CRBM, hamiltonian=KAGOME, rbm_hidden_units=24, rbm_init_scale=0.3, num_samples=600, num_thermalize=96, num_decorrelate=4, rng_seed=92), Lx=2, Ly=2),
+    dict(name="bond_kagome_2x2_pbc_crbm", what="bond", cfg=dict(ansatz=CRBM, hamiltonian=KAGOME, kagome_pbc=1, rbm_hidden_units=12, rbm_init_scale=0.05, num_samples=512, num_thermalize=256, num_decorrelate=2, rng_seed=0xC0BB), Lx=2, Ly=2),
+    dict(name="bond_kagome_3x2_obc_crbm", what="bond", cfg=dict(ansatz=CRBM, hamiltonian=KAGOME, kagome_pbc=0, rbm_hidden_units=18, rbm_init_scale=0.5, num_samples=300, num_thermalize=64, num_decorrelate=3, rng_seed=93), Lx=3, Ly=2),
+    dict(name="bond_kagome_1x2_pbc_rbm", what="bond", cfg=dict(ansatz=RBM, hamiltonian=KAGOME, kagome_pbc=1, rbm_hidden_units=6, rbm_init_scale=0.6, num_samples=256, num_thermalize=64, num_decorrelate=2, rng_seed=94), Lx=1, Ly=2),
+]
+
+
+def run_diag(r, sc):
+    cfg = _cfg(sc["cfg"])
+    Lx, Ly = sc["Lx"], sc["Ly"]
+    N = _n_sites(cfg, Lx, Ly)
+    h = r.ansatz(cfg, N)
+    smp = r.f("sampler_create")(N, ctypes.byref(cfg), r.amp_fn, h)
+    r.f("sampler_thermalize")(smp)
+    if sc["what"] == "chi":
+        tr, pp = ctypes.c_double(0), ctypes.c_double(0)
+        assert r.f("compute_chi_F")(ctypes.byref(cfg), Lx, Ly, h, smp, ctypes.byref(tr),
+                                    ctypes.byref(pp)) == 0
+        out = {"trace": np.array([tr.value]), "per_param": np.array([pp.value])}
+    else:
+        re, im = np.zeros(3), np.zeros(3)
+        cnt = np.zeros(3, dtype=np.int64)
+        assert r.f("compute_kagome_bond_phase")(ctypes.byref(cfg), Lx, Ly, h, smp, ptr(re), ptr(im),
+                                                ptr(cnt)) == 0
+        out = {"re": re, "im": im, "counts": cnt}
+    cur = r.f("sampler_current")(smp)
+    out["current"] = np.ctypeslib.as_array(ctypes.cast(cur, ctypes.POINTER(ctypes.c_int)), (N,)).copy()
+    r.f("sampler_free")(smp)
+    r.f("ansatz_free")(h)
+    return out
+
+
 KINDS = {"init": (INIT, run_init), "amp": (AMP, run_amp), "eloc": (ELOC, run_eloc),
          "sampler": (SAMPLER, run_sampler), "sr": (SR, run_sr), "run": (RUN, run_run),
-         "dyn": (DYN, run_dyn)}
+         "dyn": (DYN, run_dyn), "diag": (DIAG, run_diag)}
 
 
 def run_all(lib, prefix, kinds=None, names=None):

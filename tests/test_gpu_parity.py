@@ -137,6 +137,22 @@ def test_dynamics_step(hip_results, golden, sc):
     assert np.all(np.abs(got["info"][:, 4] - ref["info"][:, 4]) <= np.maximum(1, 0.1 * ref["info"][:, 4]))
 
 
+@pytest.mark.parametrize("sc", scenarios.DIAG, ids=lambda s: s["name"])
+def test_diagnostics(hip_results, golden, sc):
+    """nqs_compute_chi_F (O rows and column moments on the device) and
+    nqs_compute_kagome_bond_phase (pair ratios from the local-energy kernel)
+    against the reference golden vectors: same batch (final configuration
+    and class counts equal), values within 1e-10."""
+    got = hip_results[f"diag/{sc['name']}"]
+    ref = golden_case(golden, f"diag/{sc['name']}")
+    np.testing.assert_array_equal(got["current"], ref["current"])
+    if "counts" in ref:
+        np.testing.assert_array_equal(got["counts"], ref["counts"])
+    for k in ref:
+        if k not in ("current", "counts"):
+            _close(got[k], ref[k], 1e-10)
+
+
 @pytest.mark.parametrize("sc", [s for s in scenarios.SR if s["method"] in ("cg", "holo")],
                          ids=lambda s: s["name"])
 def test_cg_step_solver_contract(hip, orc, sc):

@@ -98,6 +98,22 @@ def test_dynamics_match_reference_bit_exact(orc_results, golden, sc):
         np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
 
 
+@pytest.mark.parametrize("sc", scenarios.DIAG, ids=lambda s: s["name"])
+def test_diagnostics_match_reference(orc_results, golden, sc):
+    """Tr(S) and the kagome bond-class ratio means against the compiled
+    reference (src/nqs/nqs_diagnostics.c): the same batch (sampler
+    bit-exact, final configuration and class counts equal), sums in the
+    reference's order."""
+    got = orc_results[f"diag/{sc['name']}"]
+    ref = golden_case(golden, f"diag/{sc['name']}")
+    np.testing.assert_array_equal(got["current"], ref["current"])
+    if "counts" in ref:
+        np.testing.assert_array_equal(got["counts"], ref["counts"])
+    for k in ref:
+        if k not in ("current", "counts"):
+            _close(got[k], ref[k], 1e-12)
+
+
 def test_minsr_breakdown_zero_update(golden):
     ref = golden_case(golden, "sr/minsr_retry_tfim_4x1_rbm")
     # the reference reports the failed factorisation and applies no update
