@@ -748,6 +748,9 @@ def main():
     ap.add_argument("--move", default=None, choices=["flip", "exchange", "learned"])
     ap.add_argument("--budget", type=int, default=None, metavar="K",
                     help="optimisation iterations before timing (default: the workload's, C5 300)")
+    ap.add_argument("--reference-records", default=None, metavar="DIR",
+                    help="only write the reference benchmark's records (bench_nqs.c workloads, "
+                         "bench_common.h schema) under DIR/nqs/")
     ap.add_argument("--ar-local", type=int, default=None, metavar="K",
                     help="learned proposal: exchange moves before each learned move")
     ap.add_argument("--ar-epochs", type=int, default=None, metavar="E",
@@ -757,6 +760,10 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of "
               f"this path is a design, not a build", file=sys.stderr)
         sys.exit(2)
+    if args.reference_records:
+        from spin_based_neural_network_amd.bench_records import write_records
+        write_records(args.reference_records, log=log)
+        return
     if args.copy_probe:
         lib = abi.load_library()
         nbytes = args.copy_probe << 20

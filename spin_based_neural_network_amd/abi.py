@@ -76,6 +76,12 @@ class SrInfo(Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class LanczosResult(Structure):
+    """lanczos_result_t (include/mps/lanczos.h)."""
+    _fields_ = [("eigenvalue", c_double), ("iterations", c_int), ("converged", c_int),
+                ("residual_norm", c_double)]
+
+
 class MarshallWrapper(Structure):
     _fields_ = [("base_log_amp", c_void_p), ("base_user", c_void_p),
                 ("size_x", c_int), ("size_y", c_int)]
@@ -199,6 +205,18 @@ HIP_API = {
 }
 
 
+# exact diagonalisation (include/nqs/nqs_lanczos.h), the reference's names
+_LR = POINTER(LanczosResult)
+ED_API = {
+    "nqs_exact_energy_kagome_heisenberg": (c_int, [_P, c_int, c_int, c_double, c_int, _P]),
+    "nqs_exact_energy_kagome_heisenberg_with_cb": (c_int, [_P, _P, c_int, c_int, c_double, c_int,
+                                                           _P]),
+    "nqs_lanczos_e0_kagome_heisenberg_projected_lean": (c_int, [c_int, c_int, c_double, c_int, _P,
+                                                                _P, c_int, c_int, c_double, _P,
+                                                                _LR]),
+}
+
+
 def bind(lib, prefix, table=API, strict=True):
     """Attach restype/argtypes for ``prefix + name`` of every table entry."""
     for name, (res, args) in table.items():
@@ -242,6 +260,7 @@ def load_library(path=None):
     lib = ctypes.CDLL(p, mode=ctypes.RTLD_LOCAL)
     bind(lib, "nqs_")
     bind(lib, "", HIP_API)
+    bind(lib, "", ED_API)
     lib.nqs_hip_options_defaults.restype = HipOptions
     lib.nqs_hip_options_defaults.argtypes = []
     if path is None:

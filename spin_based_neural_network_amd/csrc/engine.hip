@@ -26,6 +26,7 @@
 
 #include "../../include/nqs/nqs_ansatz.h"
 #include "../../include/nqs/nqs_diagnostics.h"
+#include "../../include/nqs/nqs_lanczos.h"
 #include "../../include/nqs/nqs_gradient.h"
 #include "../../include/nqs/nqs_marshall.h"
 #include "../../include/nqs/nqs_optimizer.h"
@@ -2366,6 +2367,29 @@ NQS_EXPORT int nqs_hip_cholesky_solve_noalloc(double* d_T, int n, const double* 
 // ===========================================================================
 namespace {
 
+// kagome bonds in the reference's enumeration (nqs_diagnostics.c:141-197,
+// nqs_lanczos.c:343-371): per cell (cx, cy), cx-major, the up triangle
+// (A,B), (A,C), (B,C), then, with periodic cells or cx, cy >= 1, the down
+// triangle (A,Bm), (A,Cm), (Bm,Cm) with Bm the B site of cell (cx-1, cy) and
+// Cm the C site of cell (cx, cy-1)
+std::vector<std::array<int, 2>> kagome_bonds(int Lx, int Ly, int pbc) {
+  std::vector<std::array<int, 2>> out;
+  for (int cx = 0; cx < Lx; cx++)
+    for (int cy = 0; cy < Ly; cy++) {
+      const int A = 3 * (cx * Ly + cy);
+      out.push_back({A, A + 1});
+      out.push_back({A, A + 2});
+      out.push_back({A + 1, A + 2});
+      if (!pbc && (cx == 0 || cy == 0)) continue;
+      const int Bm = 3 * (((cx - 1 + Lx) % Lx) * Ly + cy) + 1;
+      const int Cm = 3 * (cx * Ly + (cy - 1 + Ly) % Ly) + 2;
+      out.push_back({A, Bm});
+      out.push_back({A, Cm});
+      out.push_back({Bm, Cm});
+    }
+  return out;
+}
+
 // the sampler's next cfg->num_samples configurations, as nqs_sampler_batch
 // draws them, into s->d_samples (and h_out when given)
 int diag_batch(nqs_sampler* s, int B, int* h_out) {
@@ -2461,23 +2485,10 @@ NQS_EXPORT int nqs_compute_kagome_bond_phase(const nqs_config_t* cfg, int Lx_cel
     const int lo = std::min(u % 3, w % 3), hi = std::max(u % 3, w % 3);
     return lo == 0 ? (hi == 1 ? 0 : 1) : 2;
   };
-  for (int cx = 0; cx < Lx_cells; cx++)
-    for (int cy = 0; cy < Ly_cells; cy++) {
-      const int A = 3 * (cx * Ly_cells + cy), Bs = A + 1, Cs = A + 2;
-      const int up[3][2] = {{A, Bs}, {A, Cs}, {Bs, Cs}};
-      for (const auto& e : up) {
-        hh.pair_a.push_back(e[0]);
-        hh.pair_b.push_back(e[1]);
-      }
-      if (!pbc && (cx == 0 || cy == 0)) continue;
-      const int cxm = (cx - 1 + Lx_cells) % Lx_cells, cym = (cy - 1 + Ly_cells) % Ly_cells;
-      const int Bm = 3 * (cxm * Ly_cells + cy) + 1, Cm = 3 * (cx * Ly_cells + cym) + 2;
-      const int dn[3][2] = {{A, Bm}, {A, Cm}, {Bm, Cm}};
-      for (const auto& e : dn) {
-        hh.pair_a.push_back(e[0]);
-        hh.pair_b.push_back(e[1]);
-      }
-    }
+  for (const auto& e : kagome_bonds(Lx_cells, Ly_cells, pbc)) {
+    hh.pair_a.push_back(e[0]);
+    hh.pair_b.push_back(e[1]);
+  }
   const int T = (int)hh.pair_a.size();
   for (int t = 0; t < T; t++) cls.push_back(sub_class(hh.pair_a[t], hh.pair_b[t]));
   hh.pair_c.assign(T, 1.0);
@@ -2520,6 +2531,285 @@ NQS_EXPORT int nqs_compute_kagome_bond_phase(const nqs_config_t* cfg, int Lx_cel
     out_mean_re[c] = cnt[c] ? sr[c] / (double)cnt[c] : 0.0;
     out_mean_im[c] = cnt[c] ? si[c] / (double)cnt[c] : 0.0;
     if (out_counts) out_counts[c] = cnt[c];
+  }
+  return 0;
+}
+
+// ===========================================================================
+// exact diagonalisation of the kagome Heisenberg model (src/nqs/nqs_lanczos.c)
+// ===========================================================================
+namespace {
+
+// device buffer released at scope exit
+struct ScopedBuf {
+  DBuf b;
+  ~ScopedBuf() { b.release(); }
+};
+
+// smallest eigenvalue of the symmetric tridiagonal matrix with diagonal
+// d[0..K-1] and off-diagonal e[0..K-2]: bisection on the Sturm count
+// (number of negative pivots of T - x I) inside the Gershgorin interval,
+// down to adjacent doubles
+double tridiag_smallest(const double* d, const double* e, int K) {
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < K; i++) {
+    const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < K ? fabs(e[i]) : 0.0);
+    lo = std::min(lo, d[i] - r);
+    hi = std::max(hi, d[i] + r);
+  }
+  auto below = [&](double x) {
+    int c = 0;
+    double q = d[0] - x;
+    for (int i = 0;;) {
+      if (q < 0.0) c++;
+      if (++i >= K) break;
+      if (q == 0.0) q = 1e-300;
+      q = d[i] - x - e[i - 1] * e[i - 1] / q;
+    }
+    return c;
+  };
+  for (int it = 0; it < 2000; it++) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (below(mid) >= 1) hi = mid;
+    else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// the reference's deterministic start vector (src/mps/lanczos.c:852-870):
+// xorshift64 from seed ^ dim, element i = (x_i >> 11) / 2^53 - 1/2;
+// generated on the host in slices and uploaded
+int ed_seed_vector(double* d_v, long dim, unsigned long long seed) {
+  const long slice = 1L << 22;
+  std::vector<double> h((size_t)std::min(dim, slice));
+  unsigned long long x = seed ^ (unsigned long long)dim;
+  for (long s0 = 0; s0 < dim; s0 += slice) {
+    const long n = std::min(slice, dim - s0);
+    for (long i = 0; i < n; i++) {
+      x ^= x << 13;
+      x ^= x >> 7;
+      x ^= x << 17;
+      h[(size_t)i] = (double)(x >> 11) / 9007199254740992.0 - 0.5;
+    }
+    CK(hipMemcpyAsync(d_v + s0, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, g_stream));
+    CK(hipStreamSynchronize(g_stream));
+  }
+  return 0;
+}
+
+double ed_read(const double* d) {
+  double v = 0.0;
+  if (hipMemcpyAsync(&v, d, sizeof(double), hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
+      hipStreamSynchronize(g_stream) != hipSuccess)
+    return NAN;
+  return v;
+}
+
+int ed_upload_bonds(int Lx, int Ly, int pbc, DBuf& store, int& nb) {
+  const auto bonds = kagome_bonds(Lx, Ly, pbc);
+  nb = (int)bonds.size();
+  std::vector<int> flat;
+  for (const auto& e : bonds) {
+    flat.push_back(e[0]);
+    flat.push_back(e[1]);
+  }
+  CKR(store.ensure(sizeof(int) * flat.size() + 16));
+  CK(hipMemcpyAsync(store.p, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice,
+                    g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+// <psi|H|psi>/<psi|psi> with psi = cos(arg) exp(log|psi| - max) over the
+// 2^N basis states; amplitudes of a device source or of a host callback
+int kagome_exact_energy(nqs_log_amp_fn_t fn, void* user, int Lx, int Ly, double J, int pbc,
+                        double* out) {
+  const int N = 3 * Lx * Ly;
+  if (Lx < 1 || Ly < 1 || N > 24) {
+    fprintf(stderr, "nqs_exact_energy_kagome_heisenberg: N = 3 Lx Ly must be 3..24\n");
+    return -1;
+  }
+  const long dim = 1L << N;
+  ScopedBuf la, ar, psi, hpsi, part, scal, bonds, spins;
+  CKR(la.b.ensure(sizeof(double) * dim));
+  CKR(ar.b.ensure(sizeof(double) * dim));
+  AmpSource src;
+  if (recognise(fn, user, src) && src.a->N == N) {
+    nqs_ansatz* a = src.a;
+    a->dev_fresh = 0;
+    CKR(ansatz_upload(a));
+    ModelView<double> v;
+    CKR(model_view<double>(a, v));
+    const long chunk = std::min(dim, 1L << 20);
+    CKR(spins.b.ensure(sizeof(int) * (size_t)chunk * N));
+    for (long s0 = 0; s0 < dim; s0 += chunk) {
+      const long n = std::min(chunk, dim - s0);
+      CKR(nqs::launch_ed_state_spins(s0, n, N, spins.b.as<int>(), g_stream));
+      CKR(nqs::launch_forward<double>(v, spins.b.as<int>(), (int)n, nullptr, nullptr,
+                                      la.b.as<double>() + s0, ar.b.as<double>() + s0, g_stream));
+      if (src.marshall)
+        CKR(nqs::launch_marshall_arg(spins.b.as<int>(), (int)n, N, src.mlx, src.mly,
+                                     ar.b.as<double>() + s0, g_stream));
+    }
+  } else {
+    std::vector<double> hl(dim), ha(dim);
+    std::vector<int> sp(N);
+    for (long s = 0; s < dim; s++) {
+      for (int i = 0; i < N; i++) sp[i] = ((s >> i) & 1) ? -1 : 1;
+      fn(sp.data(), N, user, &hl[s], &ha[s]);
+      g_host_calls++;
+    }
+    CK(hipMemcpyAsync(la.b.p, hl.data(), sizeof(double) * dim, hipMemcpyHostToDevice, g_stream));
+    CK(hipMemcpyAsync(ar.b.p, ha.data(), sizeof(double) * dim, hipMemcpyHostToDevice, g_stream));
+  }
+  CKR(psi.b.ensure(sizeof(double) * dim));
+  CKR(hpsi.b.ensure(sizeof(double) * dim));
+  CKR(part.b.ensure(sizeof(double) * nqs::ed_partials()));
+  CKR(scal.b.ensure(sizeof(double) * 8));
+  double* sc = scal.b.as<double>();
+  CKR(nqs::launch_ed_psi(la.b.as<double>(), ar.b.as<double>(), dim, psi.b.as<double>(),
+                         part.b.as<double>(), sc, g_stream));
+  int nb = 0;
+  CKR(ed_upload_bonds(Lx, Ly, pbc, bonds.b, nb));
+  CKR(nqs::launch_ed_matvec_kagome(psi.b.as<double>(), hpsi.b.as<double>(), dim, bonds.b.as<int>(),
+                                   nb, J, part.b.as<double>(), sc + 2, g_stream));
+  const double den = ed_read(sc + 1), num = ed_read(sc + 2);
+  if (!(den > 0.0)) return -1;
+  *out = num / den;
+  return 0;
+}
+
+}  // namespace
+
+NQS_EXPORT int nqs_exact_energy_kagome_heisenberg(nqs_ansatz_t* a, int Lx_cells, int Ly_cells,
+                                                  double J, int pbc, double* out_energy) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !out_energy) return -1;
+  if (stream_ready()) return -1;
+  return kagome_exact_energy(nqs_ansatz_log_amp, a, Lx_cells, Ly_cells, J, pbc, out_energy);
+}
+
+NQS_EXPORT int nqs_exact_energy_kagome_heisenberg_with_cb(nqs_log_amp_fn_t log_amp, void* user,
+                                                          int Lx_cells, int Ly_cells, double J,
+                                                          int pbc, double* out_energy) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!log_amp || !out_energy) return -1;
+  if (stream_ready()) return -1;
+  return kagome_exact_energy(log_amp, user, Lx_cells, Ly_cells, J, pbc, out_energy);
+}
+
+// Lean projecting Lanczos (src/mps/lanczos.c:824-955 through
+// src/nqs/nqs_lanczos.c:629-647): three vectors and the projector's output
+// in HBM; alpha and beta are device reductions read back every step, the
+// tridiagonal's smallest eigenvalue is taken on the host.
+NQS_EXPORT int nqs_lanczos_e0_kagome_heisenberg_projected_lean(
+    int Lx_cells, int Ly_cells, double J, int pbc, const int* perm, const double* characters,
+    int G, int max_iters, double tol, double* out_eigenvalue, lanczos_result_t* out_result) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!out_eigenvalue || !perm || !characters || G <= 0) return -1;
+  const int N = 3 * Lx_cells * Ly_cells;
+  if (Lx_cells < 1 || Ly_cells < 1 || N > 30 || max_iters <= 0) return -1;
+  if (out_result) *out_result = lanczos_result_t{0.0, 0, 0, 0.0};
+  // the projector's bit tables: bit i of T(g) s is bit perm[g][i] of s, so
+  // bit j of s lands at position pinv_g[j]
+  const int nchunk = (N + 2) / 3;
+  std::vector<unsigned long long> tab((size_t)G * nchunk * 8, 0ULL);
+  bool identity = G == 1 && characters[0] == 1.0;
+  for (int g = 0; g < G; g++) {
+    std::vector<int> pinv(N, -1);
+    for (int i = 0; i < N; i++) {
+      const int j = perm[(size_t)g * N + i];
+      if (j < 0 || j >= N || pinv[j] >= 0) {
+        fprintf(stderr, "nqs_lanczos_e0_kagome_heisenberg_projected_lean: perm[%d] is not a "
+                        "permutation of 0..%d\n", g, N - 1);
+        return -1;
+      }
+      pinv[j] = i;
+      if (j != i) identity = false;
+    }
+    for (int c = 0; c < nchunk; c++)
+      for (int v = 0; v < 8; v++) {
+        unsigned long long m = 0;
+        for (int b = 0; b < 3; b++)
+          if (((v >> b) & 1) && 3 * c + b < N) m |= 1ULL << pinv[3 * c + b];
+        tab[((size_t)g * nchunk + c) * 8 + v] = m;
+      }
+  }
+  if (stream_ready()) return -1;
+  const long dim = 1L << N;
+  if (max_iters > dim) max_iters = (int)dim;
+  ScopedBuf vb[4], part, scal, bonds, dtab, dchi;
+  for (auto& b : vb) CKR(b.b.ensure(sizeof(double) * dim));
+  CKR(part.b.ensure(sizeof(double) * nqs::ed_partials()));
+  CKR(scal.b.ensure(sizeof(double) * 8));
+  CKR(dtab.b.ensure(sizeof(unsigned long long) * tab.size()));
+  CKR(dchi.b.ensure(sizeof(double) * G));
+  CK(hipMemcpyAsync(dtab.b.p, tab.data(), sizeof(unsigned long long) * tab.size(),
+                    hipMemcpyHostToDevice, g_stream));
+  CK(hipMemcpyAsync(dchi.b.p, characters, sizeof(double) * G, hipMemcpyHostToDevice, g_stream));
+  int nb = 0;
+  CKR(ed_upload_bonds(Lx_cells, Ly_cells, pbc, bonds.b, nb));
+  double* sc = scal.b.as<double>();
+  double* pt = part.b.as<double>();
+  // v_cur, v_prev, w, projector output (rotated by pointer)
+  double *vc = vb[0].b.as<double>(), *vp = vb[1].b.as<double>(), *w = vb[2].b.as<double>(),
+         *tmp = vb[3].b.as<double>();
+  auto project_norm2 = [&](double*& x) -> double {
+    if (identity) {
+      if (nqs::launch_ed_scale(x, x, 1.0, dim, pt, sc + 1, g_stream)) return NAN;
+    } else {
+      if (nqs::launch_ed_project(x, tmp, dim, dtab.b.as<unsigned long long>(),
+                                 dchi.b.as<double>(), G, nchunk, pt, sc + 1, g_stream))
+        return NAN;
+      std::swap(x, tmp);
+    }
+    return ed_read(sc + 1);
+  };
+  CKR(ed_seed_vector(vc, dim, 0xA5A5A5A5A5A5A5A5ULL));
+  double n2 = project_norm2(vc);
+  if (!(sqrt(n2) >= 1e-14)) {
+    // the projector removed the start vector: the reference's second seed
+    CKR(ed_seed_vector(vc, dim, 0xBEEFBABEULL));
+    n2 = project_norm2(vc);
+  }
+  if (!std::isfinite(n2)) return -1;
+  if (n2 > 0.0) CKR(nqs::launch_ed_scale(vc, vc, 1.0 / sqrt(n2), dim, pt, nullptr, g_stream));
+  std::vector<double> alpha(max_iters), beta(max_iters + 1, 0.0), d(max_iters), e(max_iters);
+  double lambda = 0.0, prev = 0.0;
+  int k = 0, converged = 0;
+  for (k = 0; k < max_iters; k++) {
+    CKR(nqs::launch_ed_matvec_kagome(vc, w, dim, bonds.b.as<int>(), nb, J, pt, sc, g_stream));
+    alpha[k] = ed_read(sc);
+    CKR(nqs::launch_ed_axpy2(w, vc, alpha[k], k > 0 ? vp : nullptr, beta[k], dim, pt,
+                             identity ? sc + 1 : nullptr, g_stream));
+    const double b2 = identity ? ed_read(sc + 1) : project_norm2(w);
+    if (!std::isfinite(alpha[k]) || !std::isfinite(b2)) return -1;
+    beta[k + 1] = sqrt(b2);
+    if (beta[k + 1] < 1e-14) {
+      k++;
+      break;
+    }
+    const int K = k + 1;
+    for (int i = 0; i < K; i++) d[i] = alpha[i];
+    for (int i = 0; i + 1 < K; i++) e[i] = beta[i + 1];
+    prev = lambda;
+    lambda = tridiag_smallest(d.data(), e.data(), K);
+    if (k > 5 && fabs(lambda - prev) < tol) {
+      converged = 1;
+      k++;
+      break;
+    }
+    // v_prev <- v_cur, v_cur <- w / beta (into the old v_prev's buffer)
+    std::swap(vp, vc);
+    CKR(nqs::launch_ed_scale(w, vc, 1.0 / beta[k + 1], dim, pt, nullptr, g_stream));
+  }
+  *out_eigenvalue = lambda;
+  if (out_result) {
+    out_result->eigenvalue = lambda;
+    out_result->iterations = k;
+    out_result->converged = converged;
+    out_result->residual_norm = k > 0 ? fabs(lambda - prev) : 0.0;
   }
   return 0;
 }

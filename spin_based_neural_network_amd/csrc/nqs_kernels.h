@@ -267,6 +267,25 @@ size_t chi_work_elems(long P);
 int launch_chi_trace(const double* O_re, const double* O_im, long B, long P, double* work,
                      double* out, hipStream_t st);
 
+// ---- kagome exact diagonalisation on the 2^N basis (k_ed.hip) -------------
+// Every reduction writes ed_partials() partials and reduces them into a
+// device scalar in a fixed order.  bonds: device [nb][2] site pairs.
+int ed_partials();
+int launch_ed_matvec_kagome(const double* in, double* out, long dim, const int* bonds, int nb,
+                            double J, double* part, double* dot, hipStream_t st);
+int launch_ed_axpy2(double* w, const double* v, double a, const double* u, double b, long dim,
+                    double* part, double* norm2, hipStream_t st);
+// tab: device [G][nchunk][8] OR-masks, chi: device [G]
+int launch_ed_project(const double* in, double* out, long dim, const unsigned long long* tab,
+                      const double* chi, int G, int nchunk, double* part, double* norm2,
+                      hipStream_t st);
+int launch_ed_scale(const double* x, double* y, double c, long dim, double* part, double* norm2,
+                    hipStream_t st);
+int launch_ed_state_spins(long s0, long n, int N, int* out, hipStream_t st);
+// psi = cos(ar) exp(la - max la); scal[0] = max la, scal[1] = sum psi^2
+int launch_ed_psi(const double* la, const double* ar, long dim, double* psi, double* part,
+                  double* scal, hipStream_t st);
+
 // ---- linear algebra (k_linalg.hip), FP64 ---------------------------------
 // out[k] = alpha * sum_b O[b][k] * (w ? w[b] : 1)  (deterministic order)
 int launch_colsum(const double* O, long B, long P, const double* w, double alpha,

@@ -71,6 +71,23 @@ def ed_anchors(lib):
         assert rc == 0
         out[f"ed/kagome_{Lx}x{Ly}_pbc{pbc}"] = np.array([e.value])
         lib.nqs_ansatz_free(a)
+    # the reference's lean Lanczos (identity group): E_0, iterations, converged
+    from spin_based_neural_network_amd.abi import LanczosResult
+    f = lib.nqs_lanczos_e0_kagome_heisenberg_projected_lean
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, ctypes.c_int, c_d, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_int, ctypes.c_int, c_d, ctypes.c_void_p, ctypes.POINTER(LanczosResult)]
+    for (Lx, Ly, pbc, iters, tol) in [(1, 1, 1, 50, 1e-9), (2, 2, 1, 300, 1e-11),
+                                      (2, 2, 0, 300, 1e-11), (1, 2, 1, 300, 1e-11)]:
+        N = 3 * Lx * Ly
+        perm = np.arange(N, dtype=np.int32)
+        chi = np.ones(1)
+        e = c_d(0)
+        lr = LanczosResult()
+        assert f(Lx, Ly, 1.0, pbc, perm.ctypes.data, chi.ctypes.data, 1, iters, tol,
+                 ctypes.byref(e), ctypes.byref(lr)) == 0
+        out[f"lean/kagome_{Lx}x{Ly}_pbc{pbc}"] = np.array([e.value, lr.iterations, lr.converged,
+                                                          lr.residual_norm])
     # deterministic variational energies <psi|H|psi> of a real RBM
     f = lib.nqs_exact_energy_tfim
     f.restype = ctypes.c_int
