@@ -11,7 +11,7 @@ import ctypes
 import os
 from ctypes import POINTER, c_double, c_int, c_uint, c_ulonglong, c_void_p
 
-from spin_based_neural_network_amd.abi import (API, HipOptions, NqsConfig,
+from spin_based_neural_network_amd.abi import (API, ED_API, SYM_API, HipOptions, NqsConfig,
                                                bind)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -62,4 +62,6 @@ def load_reference():
         return None
     lib = ctypes.CDLL(REF_LIB, mode=ctypes.RTLD_LOCAL)
     bind(lib, "nqs_")
+    bind(lib, "", SYM_API, strict=False)
+    bind(lib, "", ED_API, strict=False)
     return lib

@@ -1705,3 +1705,165 @@ int orc_compute_kagome_bond_phase(const nqs_config_t *cfg, int Lx, int Ly,
     free(bond); free(smp); free(w);
     return rc;
 }
+
+/* ====================================================================== */
+/* symmetry projection                                                     */
+/* ====================================================================== */
+
+/* The orbit of one configuration: (g.s)_i = s_{perm[g][i]}, base
+ * log-amplitudes lp[g], arg[g], and m = max_g lp[g].  rows: G * N ints. */
+static double orbit_amps(orc_log_amp_fn fn, void *user, const int *perm, int G, int N,
+                         const int *s, int *rows, double *lp, double *arg) {
+    double m = -1e300;
+    for (int g = 0; g < G; g++) {
+        int *r = rows + (size_t)g * N;
+        for (int i = 0; i < N; i++) r[i] = s[perm[(size_t)g * N + i]];
+        fn(r, N, user, &lp[g], &arg[g]);
+        if (lp[g] > m) m = lp[g];
+    }
+    return m;
+}
+
+/* nqs_symproj.c:30-80 / nqs_translation.c:24-72: real combination
+ * S = sum chi cos(arg) e^{lp - m}, log|S| + m - log(G)/2, arg 0 or pi */
+static void orbit_log_amp(orc_log_amp_fn fn, void *user, const int *perm, const double *chi,
+                          int G, int N, const int *s, double *ola, double *oar) {
+    int *rows = malloc((size_t)G * N * sizeof(int));
+    double *lp = malloc((size_t)G * sizeof(double)), *ar = malloc((size_t)G * sizeof(double));
+    const double m = orbit_amps(fn, user, perm, G, N, s, rows, lp, ar);
+    double S = 0.0;
+    for (int g = 0; g < G; g++) S += chi[g] * cos(ar[g]) * exp(lp[g] - m);
+    if (fabs(S) > 0.0) {
+        if (ola) *ola = log(fabs(S)) + m - 0.5 * log((double)G);
+        if (oar) *oar = S < 0.0 ? M_PI : 0.0;
+    } else {
+        if (ola) *ola = -1e300;
+        if (oar) *oar = 0.0;
+    }
+    free(rows); free(lp); free(ar);
+}
+
+/* nqs_symproj.c:82-153 (real weights), 155-263 (complex weights): base
+ * gradients of a at the orbit members, weighted by c_g / sum c, c_g =
+ * chi e^{lp - m} cos(arg) (real) or chi e^{lp - m + i arg} (complex);
+ * zero when the sum vanishes, members of zero weight skipped */
+static int orbit_gradient(orc_log_amp_fn fn, void *user, const int *perm, const double *chi,
+                          int G, int N, orc_ansatz_t *a, const int *s, int cplx,
+                          double *ore, double *oim) {
+    const long P = orc_ansatz_num_params(a);
+    int *rows = malloc((size_t)G * N * sizeof(int));
+    double *lp = malloc((size_t)G * sizeof(double)), *ar = malloc((size_t)G * sizeof(double));
+    double *cr = malloc((size_t)G * sizeof(double)), *ci = calloc((size_t)G, sizeof(double));
+    double *tr = malloc((size_t)P * sizeof(double)), *ti = malloc((size_t)P * sizeof(double));
+    const double m = orbit_amps(fn, user, perm, G, N, s, rows, lp, ar);
+    double dr = 0.0, di = 0.0;
+    for (int g = 0; g < G; g++) {
+        if (cplx) {
+            const double mag = chi[g] * exp(lp[g] - m);
+            cr[g] = mag * cos(ar[g]);
+            ci[g] = mag * sin(ar[g]);
+        } else {
+            cr[g] = chi[g] * cos(ar[g]) * exp(lp[g] - m);
+        }
+        dr += cr[g];
+        di += ci[g];
+    }
+    memset(ore, 0, (size_t)P * sizeof(double));
+    if (cplx) memset(oim, 0, (size_t)P * sizeof(double));
+    const double d2 = dr * dr + di * di;
+    int rc = 0;
+    if (!(cplx ? d2 == 0.0 : dr == 0.0)) {
+        for (int g = 0; g < G && rc == 0; g++) {
+            const double wr = cplx ? (cr[g] * dr + ci[g] * di) / d2 : cr[g] / dr;
+            const double wi = cplx ? (ci[g] * dr - cr[g] * di) / d2 : 0.0;
+            if (wr == 0.0 && wi == 0.0) continue;
+            const int *r = rows + (size_t)g * N;
+            if (!cplx) {
+                rc = orc_ansatz_logpsi_gradient(a, r, N, tr);
+                for (long k = 0; rc == 0 && k < P; k++) ore[k] += wr * tr[k];
+            } else {
+                rc = orc_ansatz_logpsi_gradient_complex(a, r, N, tr, ti);
+                for (long k = 0; rc == 0 && k < P; k++) {
+                    ore[k] += wr * tr[k] - wi * ti[k];
+                    oim[k] += wr * ti[k] + wi * tr[k];
+                }
+            }
+        }
+    }
+    free(rows); free(lp); free(ar); free(cr); free(ci); free(tr); free(ti);
+    return rc;
+}
+
+void orc_symproj_log_amp(const int *spins, int num_sites, void *user, double *ola, double *oar) {
+    const orc_symproj_wrapper_t *w = user;
+    if (!w || !w->base_log_amp || !w->perm || !w->characters || num_sites != w->num_sites ||
+        w->num_sites <= 0 || w->num_group_elements <= 0) {
+        if (ola) *ola = 0.0;
+        if (oar) *oar = 0.0;
+        return;
+    }
+    orbit_log_amp(w->base_log_amp, w->base_user, w->perm, w->characters, w->num_group_elements,
+                  w->num_sites, spins, ola, oar);
+}
+
+int orc_symproj_gradient(void *u, orc_ansatz_t *a, const int *s, int n, double *out) {
+    const orc_symproj_wrapper_t *w = u;
+    if (!w || !a || !s || !out || n != w->num_sites || n <= 0 || w->num_group_elements <= 0)
+        return -1;
+    return orbit_gradient(w->base_log_amp, w->base_user, w->perm, w->characters,
+                          w->num_group_elements, n, a, s, 0, out, NULL);
+}
+
+int orc_symproj_gradient_complex(void *u, orc_ansatz_t *a, const int *s, int n, double *re,
+                                 double *im) {
+    const orc_symproj_wrapper_t *w = u;
+    if (!w || !a || !s || !re || !im || n != w->num_sites || n <= 0 ||
+        w->num_group_elements <= 0)
+        return -1;
+    return orbit_gradient(w->base_log_amp, w->base_user, w->perm, w->characters,
+                          w->num_group_elements, n, a, s, 1, re, im);
+}
+
+/* nqs_translation.c:36-48: element (tx, ty), tx-major, output site x Ly + y
+ * takes site ((x + tx) mod Lx) Ly + (y + ty) mod Ly */
+static void translation_table(int Lx, int Ly, int *perm, double *chi) {
+    const int N = Lx * Ly;
+    int g = 0;
+    for (int tx = 0; tx < Lx; tx++)
+        for (int ty = 0; ty < Ly; ty++, g++) {
+            chi[g] = 1.0;
+            for (int x = 0; x < Lx; x++)
+                for (int y = 0; y < Ly; y++)
+                    perm[(size_t)g * N + x * Ly + y] = ((x + tx) % Lx) * Ly + (y + ty) % Ly;
+        }
+}
+
+void orc_translation_log_amp(const int *spins, int num_sites, void *user, double *ola,
+                             double *oar) {
+    const orc_translation_wrapper_t *w = user;
+    const int N = w ? w->size_x * w->size_y : 0;
+    if (!w || N <= 0 || num_sites != N) {
+        if (ola) *ola = 0.0;
+        if (oar) *oar = 0.0;
+        return;
+    }
+    int *perm = malloc((size_t)N * N * sizeof(int));
+    double *chi = malloc((size_t)N * sizeof(double));
+    translation_table(w->size_x, w->size_y, perm, chi);
+    orbit_log_amp(w->base_log_amp, w->base_user, perm, chi, N, N, spins, ola, oar);
+    free(perm); free(chi);
+}
+
+int orc_translation_gradient(void *u, orc_ansatz_t *a, const int *s, int n, double *out) {
+    const orc_translation_wrapper_t *w = u;
+    if (!w || !a || !s || !out) return -1;
+    const int N = w->size_x * w->size_y;
+    if (N <= 0 || n != N) return -1;
+    int *perm = malloc((size_t)N * N * sizeof(int));
+    double *chi = malloc((size_t)N * sizeof(double));
+    translation_table(w->size_x, w->size_y, perm, chi);
+    const int rc = orbit_gradient(w->base_log_amp, w->base_user, perm, chi, N, N, a, s, 0, out,
+                                  NULL);
+    free(perm); free(chi);
+    return rc;
+}

@@ -197,6 +197,30 @@ double orc_ed_ground_energy(const nqs_config_t *cfg, int size_x, int size_y,
 double orc_exact_energy(const nqs_config_t *cfg, int size_x, int size_y,
                         int periodic, orc_log_amp_fn fn, void *user);
 
+/* ---- symmetry projection (src/nqs/nqs_symproj.c, nqs_translation.c) -- */
+typedef struct {                 /* layout of nqs_symproj_wrapper_t */
+    orc_log_amp_fn base_log_amp;
+    void *base_user;
+    int num_sites, num_group_elements;
+    const int *perm;
+    const double *characters;
+} orc_symproj_wrapper_t;
+typedef struct {                 /* layout of nqs_translation_wrapper_t */
+    orc_log_amp_fn base_log_amp;
+    void *base_user;
+    int size_x, size_y;
+} orc_translation_wrapper_t;
+void orc_symproj_log_amp(const int *spins, int num_sites, void *user,
+                         double *out_log_abs, double *out_arg);
+int  orc_symproj_gradient(void *grad_user, orc_ansatz_t *a, const int *spins,
+                          int num_sites, double *out_grad);
+int  orc_symproj_gradient_complex(void *grad_user, orc_ansatz_t *a, const int *spins,
+                                  int num_sites, double *out_re, double *out_im);
+void orc_translation_log_amp(const int *spins, int num_sites, void *user,
+                             double *out_log_abs, double *out_arg);
+int  orc_translation_gradient(void *grad_user, orc_ansatz_t *a, const int *spins,
+                              int num_sites, double *out_grad);
+
 /* ---- sample diagnostics (src/nqs/nqs_diagnostics.c) ------------------ */
 int orc_compute_chi_F(const nqs_config_t *cfg, int size_x, int size_y,
                       orc_ansatz_t *a, orc_sampler_t *s,

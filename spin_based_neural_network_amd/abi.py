@@ -82,6 +82,18 @@ class LanczosResult(Structure):
                 ("residual_norm", c_double)]
 
 
+class SymprojWrapper(Structure):
+    """nqs_symproj_wrapper_t (include/nqs/nqs_symproj.h)."""
+    _fields_ = [("base_log_amp", c_void_p), ("base_user", c_void_p), ("num_sites", c_int),
+                ("num_group_elements", c_int), ("perm", c_void_p), ("characters", c_void_p)]
+
+
+class TranslationWrapper(Structure):
+    """nqs_translation_wrapper_t (include/nqs/nqs_translation.h)."""
+    _fields_ = [("base_log_amp", c_void_p), ("base_user", c_void_p), ("size_x", c_int),
+                ("size_y", c_int)]
+
+
 class MarshallWrapper(Structure):
     _fields_ = [("base_log_amp", c_void_p), ("base_user", c_void_p),
                 ("size_x", c_int), ("size_y", c_int)]
@@ -153,6 +165,11 @@ API = {
     "sr_run_minsr": (c_int, [_CFG, c_int, c_int, _P, _P, _P, _P, _P]),
     "sr_step_excited": (c_int, [_CFG, c_int, c_int, _P, _P, _P, _P, _P, _P, c_double, _INFO]),
     "compute_chi_F": (c_int, [_CFG, c_int, c_int, _P, _P, _P, _P]),
+    "symproj_log_amp": (None, [_P, c_int, _P, _P, _P]),
+    "symproj_gradient": (c_int, [_P, _P, _P, c_int, _P]),
+    "symproj_gradient_complex": (c_int, [_P, _P, _P, c_int, _P, _P]),
+    "translation_log_amp": (None, [_P, c_int, _P, _P, _P]),
+    "translation_gradient": (c_int, [_P, _P, _P, c_int, _P]),
     "compute_kagome_bond_phase": (c_int, [_CFG, c_int, c_int, _P, _P, _P, _P, _P]),
     "sr_run_excited": (c_int, [_CFG, c_int, c_int, _P, _P, _P, _P, _P, _P, c_double, _P]),
     "tvmc_step_real_time": (c_int, [_CFG, c_double, c_int, c_int, _P, _P, _P, _P, _INFO]),
@@ -217,6 +234,46 @@ ED_API = {
 }
 
 
+# kagome group tables (include/nqs/nqs_symproj.h), the reference's names
+SYM_API = {
+    "nqs_kagome_translation_perm": (c_int, [c_int, c_int, _P, _P]),
+    "nqs_kagome_p2_perm": (c_int, [c_int, c_int, _P, _P, _P]),
+    "nqs_kagome_p3_perm": (c_int, [c_int, _P, _P, _P]),
+    "nqs_kagome_p6_perm": (c_int, [c_int, _P, _P, _P]),
+    "nqs_kagome_p6m_perm": (c_int, [c_int, _P, _P, _P]),
+    "nqs_kagome_p6m_perm_irrep": (c_int, [c_int, c_int, _P, _P, _P]),
+    "nqs_kagome_p6m_project_inplace": (None, [_P, c_int, c_int, _P, _P]),
+}
+
+
+def kagome_group(lib, kind, *size, irrep=0):
+    """(perm [G][N] int32, characters [G]) from a library's builder:
+    kind in translation, p2, p3, p6, p6m (irrep: nqs_symproj_kagome_irrep_t).
+    The C arrays are copied and released with libc free."""
+    perm_p = ctypes.POINTER(c_int)()
+    chi_p = ctypes.POINTER(c_double)()
+    G = c_int(0)
+    name = f"nqs_kagome_{kind}_perm" + ("_irrep" if kind == "p6m" else "")
+    f = getattr(lib, name)
+    if kind == "translation":
+        rc = f(*size, ctypes.byref(perm_p), ctypes.byref(G))
+    elif kind == "p6m":
+        rc = f(*size, irrep, ctypes.byref(perm_p), ctypes.byref(chi_p), ctypes.byref(G))
+    else:
+        rc = f(*size, ctypes.byref(perm_p), ctypes.byref(chi_p), ctypes.byref(G))
+    if rc != 0:
+        raise RuntimeError(f"{name} returned {rc}")
+    N = 3 * int(np.prod(size)) if kind in ("translation", "p2") else 3 * size[0] * size[0]
+    perm = np.ctypeslib.as_array(perm_p, (G.value * N,)).reshape(G.value, N).copy()
+    chi = (np.ctypeslib.as_array(chi_p, (G.value,)).copy() if kind != "translation"
+           else np.ones(G.value))
+    libc = ctypes.CDLL(None)
+    libc.free(ctypes.cast(perm_p, c_void_p))
+    if kind != "translation":
+        libc.free(ctypes.cast(chi_p, c_void_p))
+    return perm.astype(np.int32), chi
+
+
 def bind(lib, prefix, table=API, strict=True):
     """Attach restype/argtypes for ``prefix + name`` of every table entry."""
     for name, (res, args) in table.items():
@@ -261,6 +318,7 @@ def load_library(path=None):
     bind(lib, "nqs_")
     bind(lib, "", HIP_API)
     bind(lib, "", ED_API)
+    bind(lib, "", SYM_API)
     lib.nqs_hip_options_defaults.restype = HipOptions
     lib.nqs_hip_options_defaults.argtypes = []
     if path is None:

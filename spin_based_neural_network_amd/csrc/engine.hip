@@ -27,6 +27,8 @@
 #include "../../include/nqs/nqs_ansatz.h"
 #include "../../include/nqs/nqs_diagnostics.h"
 #include "../../include/nqs/nqs_lanczos.h"
+#include "../../include/nqs/nqs_symproj.h"
+#include "../../include/nqs/nqs_translation.h"
 #include "../../include/nqs/nqs_gradient.h"
 #include "../../include/nqs/nqs_marshall.h"
 #include "../../include/nqs/nqs_optimizer.h"
@@ -310,12 +312,129 @@ bool recognise(nqs_log_amp_fn_t fn, void* user, AmpSource& src) {
 }
 
 // ---------------------------------------------------------------------------
+// group-orbit amplitude sources (nqs_symproj_log_amp, nqs_translation_log_amp
+// over a device amplitude source): the (perm, characters) tables on the
+// device, cached by address, size and content
+// ---------------------------------------------------------------------------
+struct OrbitTable {
+  const int* hp = nullptr;
+  const double* hc = nullptr;
+  int G = 0, N = 0;
+  uint64_t hash = 0;
+  DBuf perm, chi;
+};
+
+uint64_t orbit_hash(const int* perm, const double* chi, int G, int N) {
+  uint64_t h = 1469598103934665603ULL;
+  auto mix = [&](const void* p, size_t n) {
+    const unsigned char* c = (const unsigned char*)p;
+    for (size_t i = 0; i < n; i++) h = (h ^ c[i]) * 1099511628211ULL;
+  };
+  mix(perm, sizeof(int) * (size_t)G * N);
+  mix(chi, sizeof(double) * (size_t)G);
+  return h;
+}
+
+OrbitTable* orbit_table(const int* perm, const double* chi, int G, int N) {
+  static OrbitTable cache[4];
+  static int next = 0;
+  const uint64_t h = orbit_hash(perm, chi, G, N);
+  for (OrbitTable& t : cache)
+    if (t.hp == perm && t.hc == chi && t.G == G && t.N == N && t.hash == h) return &t;
+  for (int g = 0; g < G; g++)
+    for (int i = 0; i < N; i++)
+      if (perm[(size_t)g * N + i] < 0 || perm[(size_t)g * N + i] >= N) {
+        fprintf(stderr, "nqs_hip: symmetry table entry perm[%d][%d] = %d is not a site\n", g, i,
+                perm[(size_t)g * N + i]);
+        return nullptr;
+      }
+  OrbitTable& t = cache[next];
+  next = (next + 1) % 4;
+  if (t.perm.ensure(sizeof(int) * (size_t)G * N) || t.chi.ensure(sizeof(double) * G)) return nullptr;
+  if (hipMemcpyAsync(t.perm.p, perm, sizeof(int) * (size_t)G * N, hipMemcpyHostToDevice, g_stream) ||
+      hipMemcpyAsync(t.chi.p, chi, sizeof(double) * G, hipMemcpyHostToDevice, g_stream) ||
+      hipStreamSynchronize(g_stream))
+    return nullptr;
+  t.hp = perm;
+  t.hc = chi;
+  t.G = G;
+  t.N = N;
+  t.hash = h;
+  return &t;
+}
+
+// nqs_translation's table (nqs_translation.c:36-48): element (tx, ty),
+// tx-major, takes site x Ly + y from ((x + tx) mod Lx) Ly + (y + ty) mod Ly
+struct TranslationTable {
+  int Lx = 0, Ly = 0;
+  std::vector<int> perm;
+  std::vector<double> chi;
+};
+
+const TranslationTable& translation_table(int Lx, int Ly) {
+  static TranslationTable t;
+  if (t.Lx != Lx || t.Ly != Ly) {
+    t.Lx = Lx;
+    t.Ly = Ly;
+    const int N = Lx * Ly;
+    t.perm.assign((size_t)N * N, 0);
+    t.chi.assign(N, 1.0);
+    int g = 0;
+    for (int tx = 0; tx < Lx; tx++)
+      for (int ty = 0; ty < Ly; ty++, g++)
+        for (int x = 0; x < Lx; x++)
+          for (int y = 0; y < Ly; y++)
+            t.perm[(size_t)g * N + x * Ly + y] = ((x + tx) % Lx) * Ly + (y + ty) % Ly;
+  }
+  return t;
+}
+
+struct OrbitSource {
+  AmpSource base;
+  OrbitTable* tab = nullptr;
+};
+
+bool recognise_orbit(nqs_log_amp_fn_t fn, void* user, OrbitSource& o) {
+  o = OrbitSource();
+  if (!user) return false;
+  nqs_log_amp_fn_t bfn = nullptr;
+  void* buser = nullptr;
+  const int* perm = nullptr;
+  const double* chi = nullptr;
+  int G = 0, N = 0;
+  if (fn == nqs_symproj_log_amp) {
+    const nqs_symproj_wrapper_t* w = (const nqs_symproj_wrapper_t*)user;
+    bfn = w->base_log_amp;
+    buser = w->base_user;
+    perm = w->perm;
+    chi = w->characters;
+    G = w->num_group_elements;
+    N = w->num_sites;
+  } else if (fn == nqs_translation_log_amp) {
+    const nqs_translation_wrapper_t* w = (const nqs_translation_wrapper_t*)user;
+    if (w->size_x <= 0 || w->size_y <= 0) return false;
+    const TranslationTable& t = translation_table(w->size_x, w->size_y);
+    bfn = w->base_log_amp;
+    buser = w->base_user;
+    perm = t.perm.data();
+    chi = t.chi.data();
+    N = G = w->size_x * w->size_y;
+  } else {
+    return false;
+  }
+  if (!bfn || !perm || !chi || G <= 0 || N <= 0) return false;
+  if (!recognise(bfn, buser, o.base) || o.base.a->N != N) return false;
+  o.tab = orbit_table(perm, chi, G, N);
+  return o.tab != nullptr;
+}
+
+// ---------------------------------------------------------------------------
 // workspace shared by all calls (single stream, serialised)
 // ---------------------------------------------------------------------------
 struct Workspace {
   DBuf spins, t_re, t_im, q_re, q_im, la, ar, e_re, e_im, O, Oi, colwork, u1, u2, vecs, T, diag,
       y, scal, cg, fail, ham, sublat, hostla, hostar, linv, T2, Sd, elw, fwd, xr, heun, bond_ham,
-      bond_sub;
+      bond_sub, orb_rows, orb_lp, orb_ar, orb_O, orb_Oi, orb_w, orb_cfg;
 };
 Workspace g_ws;
 
@@ -481,6 +600,91 @@ int eloc_device(const nqs_config_t* cfg, int Lx, int Ly, int per, int prec, cons
   return eloc_device_T<double>(cfg, Lx, Ly, per, src, d_spins, B, d_re, d_im);
 }
 
+// log|psi| and arg psi of B device configurations under a group-orbit
+// source: the orbit rows of a slice of configurations, one batched forward
+// of the base ansatz, the real combination per configuration
+int orbit_log_amp(const OrbitSource& o, const int* d_spins, long B, double* d_la, double* d_ar) {
+  nqs_ansatz* a = o.base.a;
+  const int N = a->N, G = o.tab->G;
+  CKR(ansatz_upload(a));
+  ModelView<double> v;
+  CKR(model_view<double>(a, v));
+  const long slice = std::max(1L, (1L << 20) / G);
+  const long rows = std::min(B, slice) * G;
+  CKR(g_ws.orb_rows.ensure(sizeof(int) * (size_t)rows * N));
+  CKR(g_ws.orb_lp.ensure(sizeof(double) * (size_t)rows));
+  CKR(g_ws.orb_ar.ensure(sizeof(double) * (size_t)rows));
+  int* r = g_ws.orb_rows.as<int>();
+  double* lp = g_ws.orb_lp.as<double>();
+  double* ar = g_ws.orb_ar.as<double>();
+  for (long b0 = 0; b0 < B; b0 += slice) {
+    const int n = (int)std::min(slice, B - b0);
+    CKR(nqs::launch_orbit_spins(d_spins + (size_t)b0 * N, n, N, o.tab->perm.as<int>(), G, r,
+                                g_stream));
+    CKR(nqs::launch_forward<double>(v, r, n * G, nullptr, nullptr, lp, ar, g_stream));
+    if (o.base.marshall)
+      CKR(nqs::launch_marshall_arg(r, n * G, N, o.base.mlx, o.base.mly, ar, g_stream));
+    CKR(nqs::launch_orbit_combine(lp, ar, o.tab->chi.as<double>(), n, G, d_la + b0,
+                                  d_ar ? d_ar + b0 : nullptr, g_stream));
+  }
+  return 0;
+}
+
+// O rows [B][P] (and the imaginary rows with cplx) of the projected
+// amplitude: weights from the orbit source's base amplitudes, base
+// gradients of ansatz a at the orbit rows (real-projected, or holomorphic
+// with cplx), summed per configuration
+int orbit_gradients(const OrbitSource& o, nqs_ansatz* a, const int* d_spins, long B, bool cplx,
+                    double* O, double* Oi) {
+  nqs_ansatz* ab = o.base.a;
+  const int N = a->N, G = o.tab->G;
+  const long P = a->P;
+  CKR(ansatz_upload(ab));
+  CKR(ansatz_upload(a));
+  ModelView<double> vb, va;
+  CKR(model_view<double>(ab, vb));
+  CKR(model_view<double>(a, va));
+  long slice = std::max(1L, (1L << 26) / ((long)G * P));
+  slice = std::min(slice, std::max(1L, (1L << 20) / G));
+  const long rows = std::min(B, slice) * G;
+  CKR(g_ws.orb_rows.ensure(sizeof(int) * (size_t)rows * N));
+  CKR(g_ws.orb_lp.ensure(sizeof(double) * (size_t)rows));
+  CKR(g_ws.orb_ar.ensure(sizeof(double) * (size_t)rows));
+  CKR(g_ws.orb_O.ensure(sizeof(double) * (size_t)rows * P));
+  if (cplx) CKR(g_ws.orb_Oi.ensure(sizeof(double) * (size_t)rows * P));
+  CKR(g_ws.orb_w.ensure(sizeof(double) * 2 * (size_t)rows));
+  int* r = g_ws.orb_rows.as<int>();
+  double* lp = g_ws.orb_lp.as<double>();
+  double* ar = g_ws.orb_ar.as<double>();
+  double* w = g_ws.orb_w.as<double>();
+  for (long b0 = 0; b0 < B; b0 += slice) {
+    const int n = (int)std::min(slice, B - b0);
+    CKR(nqs::launch_orbit_spins(d_spins + (size_t)b0 * N, n, N, o.tab->perm.as<int>(), G, r,
+                                g_stream));
+    CKR(nqs::launch_forward<double>(vb, r, n * G, nullptr, nullptr, lp, ar, g_stream));
+    if (o.base.marshall)
+      CKR(nqs::launch_marshall_arg(r, n * G, N, o.base.mlx, o.base.mly, ar, g_stream));
+    CKR(forward_t<double>(a, r, n * G, nullptr, nullptr));
+    CKR(nqs::launch_gen_O<double>(va, a->kind, r, n * G, g_ws.t_re.as<double>(),
+                                  g_ws.t_im.as<double>(), cplx ? 1 : 0, g_ws.orb_O.as<double>(),
+                                  cplx ? g_ws.orb_Oi.as<double>() : nullptr, P, g_stream));
+    CKR(nqs::launch_orbit_grad(lp, ar, o.tab->chi.as<double>(), n, G, cplx ? 1 : 0,
+                               g_ws.orb_O.as<double>(), cplx ? g_ws.orb_Oi.as<double>() : nullptr,
+                               P, w, w + rows, O + (size_t)b0 * P,
+                               cplx ? Oi + (size_t)b0 * P : nullptr, g_stream));
+  }
+  return 0;
+}
+
+// the gradient callbacks of the projection wrappers over a device source
+bool orbit_gradient_cb(bool holo, nqs_gradient_fn_t gfn, nqs_holomorphic_gradient_fn_t hfn,
+                       void* guser, OrbitSource& o) {
+  if (holo) return hfn == nqs_symproj_gradient_complex && recognise_orbit(nqs_symproj_log_amp, guser, o);
+  if (gfn == nqs_symproj_gradient) return recognise_orbit(nqs_symproj_log_amp, guser, o);
+  if (gfn == nqs_translation_gradient) return recognise_orbit(nqs_translation_log_amp, guser, o);
+  return false;
+}
+
 // E_loc with a host callback: the callback is evaluated on the sample and on
 // each connected configuration; the sums run on the device.
 int eloc_callback(const nqs_config_t* cfg, int Lx, int Ly, int per, const int* h_spins, int B,
@@ -492,6 +696,39 @@ int eloc_callback(const nqs_config_t* cfg, int Lx, int Ly, int per, const int* h
   const nqs::HamHost& HH = g_ham.host;
   const int N = HH.n_sites;
   const int T = H.n_flip + H.n_pair;
+  OrbitSource orb;
+  if (recognise_orbit(fn, user, orb) && orb.base.a->N == N) {
+    // a group-orbit source over a device ansatz: every sample and each of
+    // its connected configurations (the sample itself for inactive pairs)
+    // as rows [B][T + 1][N], their amplitudes in device batches
+    const size_t R = (size_t)B * (T + 1);
+    std::vector<int> rows(R * N);
+    for (int b = 0; b < B; b++) {
+      const int* sp = h_spins + (size_t)b * N;
+      for (int t = 0; t <= T; t++) {
+        int* o = rows.data() + ((size_t)b * (T + 1) + t) * N;
+        memcpy(o, sp, sizeof(int) * N);
+        if (t == 0) continue;
+        const int k = t - 1;
+        if (k < H.n_flip) o[HH.flip_i[k]] = -o[HH.flip_i[k]];
+        else if (sp[HH.pair_a[k - H.n_flip]] != sp[HH.pair_b[k - H.n_flip]]) {
+          o[HH.pair_a[k - H.n_flip]] = -o[HH.pair_a[k - H.n_flip]];
+          o[HH.pair_b[k - H.n_flip]] = -o[HH.pair_b[k - H.n_flip]];
+        }
+      }
+    }
+    CKR(g_ws.orb_cfg.ensure(sizeof(int) * rows.size()));
+    CK(hipMemcpyAsync(g_ws.orb_cfg.p, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice,
+                      g_stream));
+    CKR(g_ws.hostla.ensure(sizeof(double) * R));
+    CKR(g_ws.hostar.ensure(sizeof(double) * R));
+    CKR(orbit_log_amp(orb, g_ws.orb_cfg.as<int>(), (long)R, g_ws.hostla.as<double>(),
+                      g_ws.hostar.as<double>()));
+    CKR(nqs::launch_eloc_from_logamps(H, d_spins, B, N, T, g_ws.hostla.as<double>(),
+                                      g_ws.hostar.as<double>(), d_re, d_im, g_stream));
+    CK(hipStreamSynchronize(g_stream));
+    return 0;
+  }
   std::vector<double> la((size_t)B * (T + 1), 0.0), ar((size_t)B * (T + 1), 0.0);
   std::vector<int> scr(N);
   for (int b = 0; b < B; b++) {
@@ -1788,6 +2025,10 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
     // this ansatz have just left it in the workspace)
     if (!(e_dev && esrc.a == a && prec == NQS_HIP_FP64))
       CKR(forward_t<double>(a, d_samples, B, nullptr, nullptr));
+  } else if (OrbitSource og; orbit_gradient_cb(holo, gfn, hfn, guser, og)) {
+    // gradients of a symmetry-projected amplitude: orbit rows of every
+    // sample, weights and base gradients on the device
+    CKR(orbit_gradients(og, a, d_samples, B, holo, O, Oi));
   } else if ((!holo && gfn) || (holo && hfn)) {
     std::vector<double> hr((size_t)B * P), hi(holo ? (size_t)B * P : 0);
     for (int b = 0; b < B; b++) {
@@ -2536,7 +2777,7 @@ NQS_EXPORT int nqs_compute_kagome_bond_phase(const nqs_config_t* cfg, int Lx_cel
 }
 
 // ===========================================================================
-// exact diagonalisation of the kagome Heisenberg model (src/nqs/nqs_lanczos.c)
+// symmetry projection (src/nqs/nqs_symproj.c, src/nqs/nqs_translation.c)
 // ===========================================================================
 namespace {
 
@@ -2545,6 +2786,261 @@ struct ScopedBuf {
   DBuf b;
   ~ScopedBuf() { b.release(); }
 };
+
+// (lp, arg) of the G orbit members g.s of one configuration: one device
+// batch when the base amplitude is a device source, else G callback calls
+int orbit_member_amps(nqs_log_amp_fn_t bfn, void* buser, const int* perm, int G, int N,
+                      const int* spins, std::vector<int>& rows, std::vector<double>& lp,
+                      std::vector<double>& ar) {
+  rows.resize((size_t)G * N);
+  for (int g = 0; g < G; g++)
+    for (int i = 0; i < N; i++) {
+      const int j = perm[(size_t)g * N + i];
+      if (j < 0 || j >= N) return -1;
+      rows[(size_t)g * N + i] = spins[j];
+    }
+  lp.assign(G, 0.0);
+  ar.assign(G, 0.0);
+  AmpSource bs;
+  if (recognise(bfn, buser, bs) && bs.a->N == N) {
+    if (stream_ready()) return -1;
+    CKR(ansatz_upload(bs.a));
+    ModelView<double> v;
+    CKR(model_view<double>(bs.a, v));
+    CKR(g_ws.orb_cfg.ensure(sizeof(int) * rows.size()));
+    CKR(g_ws.orb_lp.ensure(sizeof(double) * 2 * G));
+    double* d = g_ws.orb_lp.as<double>();
+    CK(hipMemcpyAsync(g_ws.orb_cfg.p, rows.data(), sizeof(int) * rows.size(),
+                      hipMemcpyHostToDevice, g_stream));
+    CKR(nqs::launch_forward<double>(v, g_ws.orb_cfg.as<int>(), G, nullptr, nullptr, d, d + G,
+                                    g_stream));
+    if (bs.marshall)
+      CKR(nqs::launch_marshall_arg(g_ws.orb_cfg.as<int>(), G, N, bs.mlx, bs.mly, d + G, g_stream));
+    CK(hipMemcpyAsync(lp.data(), d, sizeof(double) * G, hipMemcpyDeviceToHost, g_stream));
+    CK(hipMemcpyAsync(ar.data(), d + G, sizeof(double) * G, hipMemcpyDeviceToHost, g_stream));
+    CK(hipStreamSynchronize(g_stream));
+    return 0;
+  }
+  for (int g = 0; g < G; g++) {
+    bfn(rows.data() + (size_t)g * N, N, buser, &lp[g], &ar[g]);
+    g_host_calls++;
+  }
+  return 0;
+}
+
+// the real combination (nqs_symproj.c:56-79)
+void orbit_combine_host(const std::vector<double>& lp, const std::vector<double>& ar,
+                        const double* chi, int G, double* out_la, double* out_arg) {
+  double m = -1e300;
+  for (int g = 0; g < G; g++) m = lp[g] > m ? lp[g] : m;
+  double S = 0.0;
+  for (int g = 0; g < G; g++) S += chi[g] * cos(ar[g]) * exp(lp[g] - m);
+  if (fabs(S) > 0.0) {
+    if (out_la) *out_la = log(fabs(S)) + m - 0.5 * log((double)G);
+    if (out_arg) *out_arg = S < 0.0 ? M_PI : 0.0;
+  } else {
+    if (out_la) *out_la = -1e300;
+    if (out_arg) *out_arg = 0.0;
+  }
+}
+
+// sum_g w_g grad log psi_a(g.s): weights from the orbit members' base
+// amplitudes (real or complex), base gradients of the G rows on the device,
+// accumulated on the host in g order (nqs_symproj.c:120-150, 220-262)
+int orbit_gradient_one(nqs_log_amp_fn_t bfn, void* buser, const int* perm, const double* chi,
+                       int G, nqs_ansatz* a, const int* spins, int N, bool cplx, double* out_re,
+                       double* out_im) {
+  std::vector<int> rows;
+  std::vector<double> lp, ar;
+  CKR(orbit_member_amps(bfn, buser, perm, G, N, spins, rows, lp, ar));
+  const long P = a->P;
+  double m = -1e300;
+  for (int g = 0; g < G; g++) m = lp[g] > m ? lp[g] : m;
+  std::vector<double> cr(G), ci(G, 0.0);
+  double dr = 0.0, di = 0.0;
+  for (int g = 0; g < G; g++) {
+    if (cplx) {
+      const double mag = chi[g] * exp(lp[g] - m);
+      cr[g] = mag * cos(ar[g]);
+      ci[g] = mag * sin(ar[g]);
+    } else {
+      cr[g] = chi[g] * cos(ar[g]) * exp(lp[g] - m);
+    }
+    dr += cr[g];
+    di += ci[g];
+  }
+  std::fill(out_re, out_re + P, 0.0);
+  if (cplx) std::fill(out_im, out_im + P, 0.0);
+  const double d2 = dr * dr + di * di;
+  if ((cplx && d2 == 0.0) || (!cplx && dr == 0.0)) return 0;
+  // base gradients of all G rows in one device batch
+  if (stream_ready()) return -1;
+  CKR(ansatz_upload(a));
+  CKR(upload_spins(rows.data(), (long)rows.size()));
+  CKR(forward_t<double>(a, g_ws.spins.as<int>(), G, nullptr, nullptr));
+  ModelView<double> v;
+  CKR(model_view<double>(a, v));
+  CKR(g_ws.orb_O.ensure(sizeof(double) * (size_t)G * P));
+  if (cplx) CKR(g_ws.orb_Oi.ensure(sizeof(double) * (size_t)G * P));
+  CKR(nqs::launch_gen_O<double>(v, a->kind, g_ws.spins.as<int>(), G, g_ws.t_re.as<double>(),
+                                g_ws.t_im.as<double>(), cplx ? 1 : 0, g_ws.orb_O.as<double>(),
+                                cplx ? g_ws.orb_Oi.as<double>() : nullptr, P, g_stream));
+  std::vector<double> gr((size_t)G * P), gi(cplx ? (size_t)G * P : 0);
+  CK(hipMemcpyAsync(gr.data(), g_ws.orb_O.p, sizeof(double) * gr.size(), hipMemcpyDeviceToHost,
+                    g_stream));
+  if (cplx)
+    CK(hipMemcpyAsync(gi.data(), g_ws.orb_Oi.p, sizeof(double) * gi.size(), hipMemcpyDeviceToHost,
+                      g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  for (int g = 0; g < G; g++) {
+    double wr, wi = 0.0;
+    if (cplx) {
+      wr = (cr[g] * dr + ci[g] * di) / d2;
+      wi = (ci[g] * dr - cr[g] * di) / d2;
+    } else {
+      wr = cr[g] / dr;
+    }
+    if (wr == 0.0 && wi == 0.0) continue;
+    const double* r = gr.data() + (size_t)g * P;
+    if (!cplx) {
+      for (long k = 0; k < P; k++) out_re[k] += wr * r[k];
+      continue;
+    }
+    const double* q = gi.data() + (size_t)g * P;
+    for (long k = 0; k < P; k++) {
+      out_re[k] += wr * r[k] - wi * q[k];
+      out_im[k] += wr * q[k] + wi * r[k];
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+NQS_EXPORT void nqs_symproj_log_amp(const int* spins, int num_sites, void* user,
+                                    double* out_log_abs, double* out_arg) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  const nqs_symproj_wrapper_t* w = (const nqs_symproj_wrapper_t*)user;
+  auto zero = [&] {
+    if (out_log_abs) *out_log_abs = 0.0;
+    if (out_arg) *out_arg = 0.0;
+  };
+  if (!w || !w->base_log_amp || !w->perm || !w->characters || !spins) return zero();
+  const int N = w->num_sites, G = w->num_group_elements;
+  if (num_sites != N || N <= 0 || G <= 0) return zero();
+  std::vector<int> rows;
+  std::vector<double> lp, ar;
+  if (orbit_member_amps(w->base_log_amp, w->base_user, w->perm, G, N, spins, rows, lp, ar))
+    return zero();
+  orbit_combine_host(lp, ar, w->characters, G, out_log_abs, out_arg);
+}
+
+NQS_EXPORT int nqs_symproj_gradient(void* grad_user, nqs_ansatz_t* ansatz, const int* spins,
+                                    int num_sites, double* out_grad) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!grad_user || !ansatz || !spins || !out_grad) return -1;
+  const nqs_symproj_wrapper_t* w = (const nqs_symproj_wrapper_t*)grad_user;
+  const int N = w->num_sites, G = w->num_group_elements;
+  if (num_sites != N || N <= 0 || G <= 0 || ansatz->N != N) return -1;
+  return orbit_gradient_one(w->base_log_amp, w->base_user, w->perm, w->characters, G, ansatz,
+                            spins, N, false, out_grad, nullptr);
+}
+
+NQS_EXPORT int nqs_symproj_gradient_complex(void* grad_user, nqs_ansatz_t* ansatz,
+                                            const int* spins, int num_sites, double* out_grad_re,
+                                            double* out_grad_im) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!grad_user || !ansatz || !spins || !out_grad_re || !out_grad_im) return -1;
+  const nqs_symproj_wrapper_t* w = (const nqs_symproj_wrapper_t*)grad_user;
+  const int N = w->num_sites, G = w->num_group_elements;
+  if (num_sites != N || N <= 0 || G <= 0 || ansatz->N != N) return -1;
+  return orbit_gradient_one(w->base_log_amp, w->base_user, w->perm, w->characters, G, ansatz,
+                            spins, N, true, out_grad_re, out_grad_im);
+}
+
+NQS_EXPORT void nqs_translation_log_amp(const int* spins, int num_sites, void* user,
+                                        double* out_log_abs, double* out_arg) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  const nqs_translation_wrapper_t* w = (const nqs_translation_wrapper_t*)user;
+  auto zero = [&] {
+    if (out_log_abs) *out_log_abs = 0.0;
+    if (out_arg) *out_arg = 0.0;
+  };
+  if (!w || !w->base_log_amp || !spins || w->size_x <= 0 || w->size_y <= 0) return zero();
+  const int N = w->size_x * w->size_y;
+  if (num_sites != N) return zero();
+  const TranslationTable& t = translation_table(w->size_x, w->size_y);
+  std::vector<int> rows;
+  std::vector<double> lp, ar;
+  if (orbit_member_amps(w->base_log_amp, w->base_user, t.perm.data(), N, N, spins, rows, lp, ar))
+    return zero();
+  orbit_combine_host(lp, ar, t.chi.data(), N, out_log_abs, out_arg);
+}
+
+NQS_EXPORT int nqs_translation_gradient(void* grad_user, nqs_ansatz_t* ansatz, const int* spins,
+                                        int num_sites, double* out_grad) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!grad_user || !ansatz || !spins || !out_grad) return -1;
+  const nqs_translation_wrapper_t* w = (const nqs_translation_wrapper_t*)grad_user;
+  if (w->size_x <= 0 || w->size_y <= 0) return -1;
+  const int N = w->size_x * w->size_y;
+  if (num_sites != N || ansatz->N != N) return -1;
+  const TranslationTable& t = translation_table(w->size_x, w->size_y);
+  return orbit_gradient_one(w->base_log_amp, w->base_user, t.perm.data(), t.chi.data(), N, ansatz,
+                            spins, N, false, out_grad, nullptr);
+}
+
+// psi <- (1/G) sum_g chi_g T(g) psi on the device (the ED projector)
+NQS_EXPORT void nqs_kagome_p6m_project_inplace(double* psi, int N, int G, const int* perm,
+                                               const double* characters) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!psi || !perm || !characters || N <= 0 || N > 30 || G <= 0) return;
+  const int nchunk = (N + 2) / 3;
+  std::vector<unsigned long long> tab((size_t)G * nchunk * 8, 0ULL);
+  for (int g = 0; g < G; g++) {
+    std::vector<int> pinv(N, 0);
+    for (int i = 0; i < N; i++) {
+      const int j = perm[(size_t)g * N + i];
+      if (j < 0 || j >= N) {
+        fprintf(stderr, "nqs_kagome_p6m_project_inplace: perm[%d][%d] = %d is not a site\n", g,
+                i, j);
+        return;
+      }
+      pinv[j] = i;
+    }
+    for (int c = 0; c < nchunk; c++)
+      for (int v = 0; v < 8; v++) {
+        unsigned long long mk = 0;
+        for (int b = 0; b < 3; b++)
+          if (((v >> b) & 1) && 3 * c + b < N) mk |= 1ULL << pinv[3 * c + b];
+        tab[((size_t)g * nchunk + c) * 8 + v] = mk;
+      }
+  }
+  if (stream_ready()) return;
+  const long dim = 1L << N;
+  ScopedBuf in, out, part, scal, dtab, dchi;
+  if (in.b.ensure(sizeof(double) * dim) || out.b.ensure(sizeof(double) * dim) ||
+      part.b.ensure(sizeof(double) * nqs::ed_partials()) || scal.b.ensure(64) ||
+      dtab.b.ensure(sizeof(unsigned long long) * tab.size()) || dchi.b.ensure(sizeof(double) * G))
+    return;
+  if (hipMemcpyAsync(in.b.p, psi, sizeof(double) * dim, hipMemcpyHostToDevice, g_stream) ||
+      hipMemcpyAsync(dtab.b.p, tab.data(), sizeof(unsigned long long) * tab.size(),
+                     hipMemcpyHostToDevice, g_stream) ||
+      hipMemcpyAsync(dchi.b.p, characters, sizeof(double) * G, hipMemcpyHostToDevice, g_stream))
+    return;
+  if (nqs::launch_ed_project(in.b.as<double>(), out.b.as<double>(), dim,
+                             dtab.b.as<unsigned long long>(), dchi.b.as<double>(), G, nchunk,
+                             part.b.as<double>(), scal.b.as<double>(), g_stream))
+    return;
+  if (hipMemcpyAsync(psi, out.b.p, sizeof(double) * dim, hipMemcpyDeviceToHost, g_stream) ||
+      hipStreamSynchronize(g_stream))
+    return;
+}
+
+// ===========================================================================
+// exact diagonalisation of the kagome Heisenberg model (src/nqs/nqs_lanczos.c)
+// ===========================================================================
+namespace {
 
 // smallest eigenvalue of the symmetric tridiagonal matrix with diagonal
 // d[0..K-1] and off-diagonal e[0..K-2]: bisection on the Sturm count
@@ -2651,6 +3147,14 @@ int kagome_exact_energy(nqs_log_amp_fn_t fn, void* user, int Lx, int Ly, double 
       if (src.marshall)
         CKR(nqs::launch_marshall_arg(spins.b.as<int>(), (int)n, N, src.mlx, src.mly,
                                      ar.b.as<double>() + s0, g_stream));
+    }
+  } else if (OrbitSource orb; recognise_orbit(fn, user, orb) && orb.base.a->N == N) {
+    const long chunk = std::min(dim, 1L << 16);
+    CKR(spins.b.ensure(sizeof(int) * (size_t)chunk * N));
+    for (long s0 = 0; s0 < dim; s0 += chunk) {
+      const long n = std::min(chunk, dim - s0);
+      CKR(nqs::launch_ed_state_spins(s0, n, N, spins.b.as<int>(), g_stream));
+      CKR(orbit_log_amp(orb, spins.b.as<int>(), n, la.b.as<double>() + s0, ar.b.as<double>() + s0));
     }
   } else {
     std::vector<double> hl(dim), ha(dim);

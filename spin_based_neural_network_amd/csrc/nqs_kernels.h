@@ -260,6 +260,18 @@ int launch_eloc_from_logamps(const HamDev& h, const int* spins, int B, int N,
                              int n_terms, const double* la, const double* ar,
                              double* e_re, double* e_im, hipStream_t st);
 
+// ---- group orbits of configurations (k_orbit.hip) ------------------------
+// out [B*G][N]: row b G + g is g.s_b, (g.s)_i = s_{perm[g][i]}
+int launch_orbit_spins(const int* spins, int B, int N, const int* perm, int G, int* out,
+                       hipStream_t st);
+// real combination of the orbit rows' (lp, ar) [B][G] into log|psi|, arg
+int launch_orbit_combine(const double* lp, const double* ar, const double* chi, int B, int G,
+                         double* la_out, double* ar_out, hipStream_t st);
+// weights w [B][G] (w_re, w_im scratch) and O_sym [B][P] = sum_g w O[bG+g]
+int launch_orbit_grad(const double* lp, const double* ar, const double* chi, int B, int G,
+                      int cplx, const double* O_re, const double* O_im, long P, double* w_re,
+                      double* w_im, double* out_re, double* out_im, hipStream_t st);
+
 // ---- sample diagnostics (k_diag.hip) --------------------------------------
 // out[0] = sum_k (<|O_k|^2> - |<O_k>|^2) over rows O_re [B][P] (+ i O_im,
 // may be null); work: chi_work_elems(P) doubles

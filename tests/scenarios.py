@@ -11,8 +11,9 @@ import ctypes
 
 import numpy as np
 
-from spin_based_neural_network_amd.abi import (MarshallWrapper, SrInfo,
-                                               config_defaults, fnptr, ptr)
+from spin_based_neural_network_amd.abi import (MarshallWrapper, SrInfo, SymprojWrapper,
+                                               TranslationWrapper, config_defaults, fnptr,
+                                               kagome_group, ptr)
 
 RBM, CRBM, MF = 5, 6, 0
 TFIM, HEIS, J1J2, XXZ, KAGOME = 0, 1, 2, 4, 6
@@ -119,6 +120,7 @@ class Runner:
 
     def __init__(self, lib, prefix):
         self.lib = lib
+        self.tables = None
         self.p = prefix
         self.f = lambda name: getattr(lib, prefix + name)
         self.amp_fn = fnptr(lib, prefix + "ansatz_log_amp")
@@ -371,13 +373,119 @@ def run_diag(r, sc):
     return out
 
 
+# symmetry projection (src/nqs/nqs_symproj.c, src/nqs/nqs_translation.c):
+# group tables from the library's builders (or, for a library without
+# them, Runner.tables), projected amplitudes and gradients of random
+# configurations, short SR runs with the projected callbacks, and the
+# exact energy of the projected state
+SYM = [
+    dict(name="p6m_b1_kagome_2x2_crbm", what="amp", group=("p6m", (2,), 2), cfg=dict(ansatz=CRBM, hamiltonian=KAGOME, rbm_hidden_units=8, rbm_init_scale=0.3, rng_seed=31), Lx=2, Ly=2, n=12),
+    dict(name="p6m_e1_kagome_2x2_crbm", what="amp", group=("p6m", (2,), 4), cfg=dict(ansatz=CRBM, hamiltonian=KAGOME, rbm_hidden_units=6, rbm_init_scale=0.4, rng_seed=36), Lx=2, Ly=2, n=10),
+    dict(name="p2_a1_kagome_3x2_rbm", what="amp", group=("p2", (3, 2), 0), cfg=dict(ansatz=RBM, hamiltonian=KAGOME, rbm_hidden_units=10, rbm_init_scale=0.5, rng_seed=32), Lx=3, Ly=2, n=12),
+    dict(name="p3_kagome_2x2_crbm", what="amp", group=("p3", (2,), 0), cfg=dict(ansatz=CRBM, hamiltonian=KAGOME, rbm_hidden_units=8, rbm_init_scale=0.3, rng_seed=37), Lx=2, Ly=2, n=8),
+    dict(name="transl_tfim_3x2_rbm", what="amp", translation=(3, 2), cfg=dict(ansatz=RBM, rbm_hidden_units=6, rbm_init_scale=0.4, rng_seed=33), Lx=3, Ly=2, n=12),
+    dict(name="run_holo_p6m_b1_kagome_2x2", what="holo_run", group=("p6m", (2,), 2), cfg=dict(ansatz=CRBM, hamiltonian=KAGOME, kagome_pbc=1, rbm_hidden_units=8, rbm_init_scale=0.05, num_samples=96, num_thermalize=64, num_decorrelate=2, num_iterations=3, learning_rate=0.005, sr_diag_shift=0.05, sr_cg_max_iters=200, sr_cg_tol=1e-8, rng_seed=0xB1B1B1B1), Lx=2, Ly=2),
+    dict(name="run_cg_transl_tfim_4x1_rbm", what="cg_run", translation=(4, 1), cfg=dict(ansatz=RBM, rbm_hidden_units=8, rbm_init_scale=0.1, num_samples=64, num_thermalize=32, num_decorrelate=2, num_iterations=3, sr_diag_shift=1e-2, learning_rate=0.05), Lx=4, Ly=1),
+    dict(name="exact_p6m_b1_kagome_2x2_crbm", what="exact", group=("p6m", (2,), 2), cfg=dict(ansatz=CRBM, hamiltonian=KAGOME, rbm_hidden_units=8, rbm_init_scale=0.3, rng_seed=35), Lx=2, Ly=2),
+]
+
+
+def _sym_tables(r, sc):
+    if getattr(r, "tables", None) is not None:
+        return r.tables[sc["name"]]
+    kind, size, irrep = sc["group"]
+    return kagome_group(r.lib, kind, *size, irrep=irrep)
+
+
+def run_sym(r, sc):
+    """Empty for a library that lacks the group builders (and no tables
+    were given) or the exact-energy entry point."""
+    if "group" in sc and getattr(r, "tables", None) is None and not hasattr(r.lib, "nqs_kagome_p6m_perm_irrep"):
+        return {}
+    if sc["what"] == "exact" and not hasattr(r.lib, "nqs_exact_energy_kagome_heisenberg_with_cb"):
+        return {}
+    cfg = _cfg(sc["cfg"])
+    Lx, Ly = sc["Lx"], sc["Ly"]
+    N = _n_sites(cfg, Lx, Ly)
+    h = r.ansatz(cfg, N)
+    out = {}
+    if "group" in sc:
+        perm, chi = _sym_tables(r, sc)
+        perm = np.ascontiguousarray(perm, dtype=np.int32)
+        chi = np.ascontiguousarray(chi, dtype=float)
+        w = SymprojWrapper(r.amp_fn, h, N, perm.shape[0], perm.ctypes.data, chi.ctypes.data)
+        names = ("symproj_log_amp", "symproj_gradient", "symproj_gradient_complex")
+        out.update(perm=perm, chi=chi)
+    else:
+        w = TranslationWrapper(r.amp_fn, h, *sc["translation"])
+        names = ("translation_log_amp", "translation_gradient", None)
+    fn = fnptr(r.lib, r.p + names[0])
+    user = ctypes.addressof(w)
+    if sc["what"] == "amp":
+        s = spins_random(sc["n"], N, 4000 + N)
+        la, ar = r.log_amp(fn, user, s)
+        P = r.f("ansatz_num_params")(h)
+        g = np.zeros((4, P))
+        gr, gi = np.zeros((4, P)), np.zeros((4, P))
+        # gradients at the 4 configurations of largest projected amplitude
+        # (where the orbit sum cancels to rounding level both the amplitude
+        # and the weights are rounding noise)
+        top = np.sort(np.argsort(-la, kind="stable")[:4])
+        for k, b in enumerate(top):
+            row = np.ascontiguousarray(s[b])
+            assert r.f(names[1])(user, h, ptr(row), N, ptr(g[k])) == 0
+            if names[2]:
+                assert r.f(names[2])(user, h, ptr(row), N, ptr(gr[k]), ptr(gi[k])) == 0
+        out.update(spins=s, log_abs=la, arg=ar, grad=g, grad_rows=top)
+        if names[2]:
+            out.update(gradc_re=gr, gradc_im=gi)
+    elif sc["what"] in ("holo_run", "cg_run"):
+        smp = r.f("sampler_create")(N, ctypes.byref(cfg), fn, user)
+        tr = np.zeros(cfg.num_iterations)
+        if sc["what"] == "holo_run":
+            gfn = fnptr(r.lib, r.p + names[2])
+            rc = r.f("sr_run_holomorphic_full")(ctypes.byref(cfg), Lx, Ly, h, smp, fn, user, gfn,
+                                                user, ptr(tr))
+        else:
+            gfn = fnptr(r.lib, r.p + names[1])
+            rc = r.f("sr_run_custom_full")(ctypes.byref(cfg), Lx, Ly, h, smp, fn, user, gfn, user,
+                                           ptr(tr))
+        assert rc == 0
+        out.update(trace=tr, params=r.params(h))
+        r.f("sampler_free")(smp)
+    else:
+        e = ctypes.c_double(0)
+        assert r.lib.nqs_exact_energy_kagome_heisenberg_with_cb(fn, user, Lx, Ly, 1.0, 1,
+                                                                ctypes.byref(e)) == 0
+        out["energy"] = np.array([e.value])
+    r.f("ansatz_free")(h)
+    return out
+
+
 KINDS = {"init": (INIT, run_init), "amp": (AMP, run_amp), "eloc": (ELOC, run_eloc),
          "sampler": (SAMPLER, run_sampler), "sr": (SR, run_sr), "run": (RUN, run_run),
-         "dyn": (DYN, run_dyn), "diag": (DIAG, run_diag)}
+         "dyn": (DYN, run_dyn), "diag": (DIAG, run_diag), "sym": (SYM, run_sym)}
 
 
-def run_all(lib, prefix, kinds=None, names=None):
+def sym_close(got, ref, amp_tol, grad_tol):
+    """Projected amplitudes where the orbit sum does not cancel to rounding
+    level (log|psi| within 20 of the largest), signs, gradient rows and
+    values; cancelled configurations must come out cancelled."""
+    la, lr = np.asarray(got["log_abs"]), np.asarray(ref["log_abs"])
+    live = lr > lr.max() - 20.0
+    assert np.max(np.abs(la[live] - lr[live])) <= amp_tol * max(1.0, np.max(np.abs(lr[live])))
+    np.testing.assert_array_equal(np.asarray(got["arg"])[live], np.asarray(ref["arg"])[live])
+    assert np.all(la[~live] < lr.max() - 15.0)
+    np.testing.assert_array_equal(got["grad_rows"], ref["grad_rows"])
+    for k in ("grad", "gradc_re", "gradc_im"):
+        if k in ref:
+            scale = max(1.0, float(np.max(np.abs(ref[k]))))
+            assert np.max(np.abs(np.asarray(got[k]) - ref[k])) <= grad_tol * scale, k
+
+
+def run_all(lib, prefix, kinds=None, names=None, tables=None):
     r = Runner(lib, prefix)
+    r.tables = tables
     res = {}
     for kind, (cases, fn) in KINDS.items():
         if kinds and kind not in kinds:

@@ -114,6 +114,28 @@ def test_diagnostics_match_reference(orc_results, golden, sc):
             _close(got[k], ref[k], 1e-12)
 
 
+@pytest.fixture(scope="module")
+def orc_sym(orc, golden):
+    tables = {sc["name"]: (golden[f"sym/{sc['name']}/perm"], golden[f"sym/{sc['name']}/chi"])
+              for sc in scenarios.SYM if "group" in sc}
+    return scenarios.run_all(orc, "orc_", kinds=["sym"], tables=tables)
+
+
+@pytest.mark.parametrize("sc", [s for s in scenarios.SYM if s["what"] != "exact"],
+                         ids=lambda s: s["name"])
+def test_symmetry_projection_matches_reference(orc_sym, golden, sc):
+    """Projected amplitudes, real and complex gradients and SR runs with the
+    projection callbacks (src/nqs/nqs_symproj.c, nqs_translation.c) against
+    the compiled reference, the group tables taken from its builders."""
+    got = orc_sym[f"sym/{sc['name']}"]
+    ref = golden_case(golden, f"sym/{sc['name']}")
+    if "trace" in ref:
+        _close(got["trace"], ref["trace"], 1e-10)
+        _close(got["params"], ref["params"], 1e-10)
+        return
+    scenarios.sym_close(got, ref, 1e-12, 1e-11)
+
+
 def test_minsr_breakdown_zero_update(golden):
     ref = golden_case(golden, "sr/minsr_retry_tfim_4x1_rbm")
     # the reference reports the failed factorisation and applies no update
