@@ -10,8 +10,8 @@
  * solves.  The CG stopping rule (relative residual tol^2 * |r0|^2, breakdown
  * |pAp| < 1e-30), MinSR's Cholesky retry (+1e-6 on the diagonal, zero
  * update on a second failure) and every nqs_sr_step_info_t field follow the
- * reference definitions.  tVMC and the excited-state penalty step are not
- * part of this library.
+ * reference definitions.  The excited-state penalty step and real-time
+ * tVMC (Euler and Heun) reuse the same stages.
  */
 #ifndef NQS_OPTIMIZER_H
 #define NQS_OPTIMIZER_H

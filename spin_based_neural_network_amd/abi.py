@@ -231,6 +231,31 @@ ED_API = {
     "nqs_lanczos_e0_kagome_heisenberg_projected_lean": (c_int, [c_int, c_int, c_double, c_int, _P,
                                                                 _P, c_int, c_int, c_double, _P,
                                                                 _LR]),
+    "nqs_lanczos_e0_kagome_heisenberg_projected_lean_eigvec": (
+        c_int, [c_int, c_int, c_double, c_int, _P, _P, c_int, c_int, c_double, _P, _P, _LR]),
+    "nqs_materialise_state": (c_int, [_P, c_int, c_int, _P, _P]),
+    "nqs_materialise_state_with_cb": (c_int, [_P, _P, c_int, c_int, _P, _P]),
+    "nqs_materialise_state_with_cb_N": (c_int, [_P, _P, c_int, _P, _P]),
+    "nqs_exact_energy_tfim": (c_int, [_P, c_int, c_int, c_double, c_double, _P]),
+    "nqs_exact_energy_heisenberg": (c_int, [_P, c_int, c_int, c_double, c_double, _P]),
+    "nqs_lanczos_refine_tfim": (c_int, [_P, c_int, c_int, c_double, c_double, c_int, c_double,
+                                        _P, _P, _LR]),
+    "nqs_lanczos_refine_heisenberg": (c_int, [_P, c_int, c_int, c_double, c_double, c_int,
+                                              c_double, _P, _P, _LR]),
+    "nqs_lanczos_refine_kagome_heisenberg": (c_int, [_P, c_int, c_int, c_double, c_int, c_int,
+                                                     c_double, _P, _P, _LR]),
+    "nqs_lanczos_refine_kagome_heisenberg_with_cb": (c_int, [_P, _P, c_int, c_int, c_double,
+                                                             c_int, c_int, c_double, _P, _P,
+                                                             _LR]),
+    "nqs_lanczos_k_lowest_kagome_heisenberg": (c_int, [_P, c_int, c_int, c_double, c_int, c_int,
+                                                       c_int, _P, _LR]),
+    "nqs_lanczos_k_lowest_kagome_heisenberg_with_cb": (c_int, [_P, _P, c_int, c_int, c_double,
+                                                               c_int, c_int, c_int, _P, _LR]),
+    "nqs_lanczos_k_lowest_kagome_heisenberg_projected": (
+        c_int, [_P, _P, c_int, c_int, c_double, c_int, _P, _P, c_int, c_int, c_int, _P, _LR]),
+    "nqs_lanczos_refine_kagome_heisenberg_projected": (
+        c_int, [_P, _P, c_int, c_int, c_double, c_int, _P, _P, c_int, c_int, c_double, _P, _P,
+                _LR]),
 }
 
 

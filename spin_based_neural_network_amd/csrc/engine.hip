@@ -85,6 +85,8 @@ struct DBuf {
     if (b == 0) b = 16;
     if (hipMalloc(&p, b) != hipSuccess) {
       fprintf(stderr, "nqs_hip: hipMalloc(%zu) failed\n", b);
+      (void)hipGetLastError();   // clear the error so later launch checks see their own
+      p = nullptr;
       return -1;
     }
     bytes = b;
@@ -2990,32 +2992,51 @@ NQS_EXPORT int nqs_translation_gradient(void* grad_user, nqs_ansatz_t* ansatz, c
                             spins, N, false, out_grad, nullptr);
 }
 
+namespace {
+
+// the projector's bit tables: bit i of T(g) s is bit perm[g][i] of s, so
+// bit j of s lands at position pinv_g[j]; tab [G][nchunk][8] holds, for
+// every 3-bit chunk c of s and its value v, the OR of the bits it sets in
+// T(g) s.  strict: every row must be a permutation of 0..N-1.
+int proj_tables(const int* perm, int G, int N, bool strict, const char* who,
+                std::vector<unsigned long long>& tab, bool* identity) {
+  const int nchunk = (N + 2) / 3;
+  tab.assign((size_t)G * nchunk * 8, 0ULL);
+  bool id = true;
+  for (int g = 0; g < G; g++) {
+    std::vector<int> pinv(N, -1);
+    for (int i = 0; i < N; i++) {
+      const int j = perm[(size_t)g * N + i];
+      if (j < 0 || j >= N || (strict && pinv[j] >= 0)) {
+        fprintf(stderr, "%s: perm[%d] is not a permutation of 0..%d\n", who, g, N - 1);
+        return -1;
+      }
+      pinv[j] = i;
+      if (j != i) id = false;
+    }
+    for (int c = 0; c < nchunk; c++)
+      for (int v = 0; v < 8; v++) {
+        unsigned long long m = 0;
+        for (int b = 0; b < 3; b++)
+          if (((v >> b) & 1) && 3 * c + b < N && pinv[3 * c + b] >= 0)
+            m |= 1ULL << pinv[3 * c + b];
+        tab[((size_t)g * nchunk + c) * 8 + v] = m;
+      }
+  }
+  if (identity) *identity = id;
+  return 0;
+}
+
+}  // namespace
+
 // psi <- (1/G) sum_g chi_g T(g) psi on the device (the ED projector)
 NQS_EXPORT void nqs_kagome_p6m_project_inplace(double* psi, int N, int G, const int* perm,
                                                const double* characters) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (!psi || !perm || !characters || N <= 0 || N > 30 || G <= 0) return;
   const int nchunk = (N + 2) / 3;
-  std::vector<unsigned long long> tab((size_t)G * nchunk * 8, 0ULL);
-  for (int g = 0; g < G; g++) {
-    std::vector<int> pinv(N, 0);
-    for (int i = 0; i < N; i++) {
-      const int j = perm[(size_t)g * N + i];
-      if (j < 0 || j >= N) {
-        fprintf(stderr, "nqs_kagome_p6m_project_inplace: perm[%d][%d] = %d is not a site\n", g,
-                i, j);
-        return;
-      }
-      pinv[j] = i;
-    }
-    for (int c = 0; c < nchunk; c++)
-      for (int v = 0; v < 8; v++) {
-        unsigned long long mk = 0;
-        for (int b = 0; b < 3; b++)
-          if (((v >> b) & 1) && 3 * c + b < N) mk |= 1ULL << pinv[3 * c + b];
-        tab[((size_t)g * nchunk + c) * 8 + v] = mk;
-      }
-  }
+  std::vector<unsigned long long> tab;
+  if (proj_tables(perm, G, N, false, "nqs_kagome_p6m_project_inplace", tab, nullptr)) return;
   if (stream_ready()) return;
   const long dim = 1L << N;
   ScopedBuf in, out, part, scal, dtab, dchi;
@@ -3038,42 +3059,119 @@ NQS_EXPORT void nqs_kagome_p6m_project_inplace(double* psi, int N, int G, const 
 }
 
 // ===========================================================================
-// exact diagonalisation of the kagome Heisenberg model (src/nqs/nqs_lanczos.c)
+// exact diagonalisation on the 2^N basis (src/nqs/nqs_lanczos.c,
+// src/mps/lanczos.c)
 // ===========================================================================
 namespace {
 
-// smallest eigenvalue of the symmetric tridiagonal matrix with diagonal
-// d[0..K-1] and off-diagonal e[0..K-2]: bisection on the Sturm count
-// (number of negative pivots of T - x I) inside the Gershgorin interval,
-// down to adjacent doubles
-double tridiag_smallest(const double* d, const double* e, int K) {
+// number of eigenvalues below x of the symmetric tridiagonal matrix with
+// diagonal d[0..K-1] and off-diagonal e[0..K-2]: the negative pivots of the
+// LDL^T factorisation of T - x I (Sturm count)
+int sturm_below(const double* d, const double* e, int K, double x) {
+  int c = 0;
+  double q = d[0] - x;
+  for (int i = 0;;) {
+    if (q < 0.0) c++;
+    if (++i >= K) break;
+    if (q == 0.0) q = 1e-300;
+    q = d[i] - x - e[i - 1] * e[i - 1] / q;
+  }
+  return c;
+}
+
+// the idx-th smallest eigenvalue (0-based) of that matrix: bisection on the
+// Sturm count inside the Gershgorin interval, down to adjacent doubles
+double tridiag_eig(const double* d, const double* e, int K, int idx) {
   double lo = 1e300, hi = -1e300;
   for (int i = 0; i < K; i++) {
     const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < K ? fabs(e[i]) : 0.0);
     lo = std::min(lo, d[i] - r);
     hi = std::max(hi, d[i] + r);
   }
-  auto below = [&](double x) {
-    int c = 0;
-    double q = d[0] - x;
-    for (int i = 0;;) {
-      if (q < 0.0) c++;
-      if (++i >= K) break;
-      if (q == 0.0) q = 1e-300;
-      q = d[i] - x - e[i - 1] * e[i - 1] / q;
-    }
-    return c;
-  };
+  const double pad = 1e-12 * (hi - lo) + 1e-300;
+  lo -= pad;
+  hi += pad;
   for (int it = 0; it < 2000; it++) {
     const double mid = 0.5 * (lo + hi);
     if (mid <= lo || mid >= hi) break;
-    if (below(mid) >= 1) hi = mid;
+    if (sturm_below(d, e, K, mid) >= idx + 1) hi = mid;
     else lo = mid;
   }
   return 0.5 * (lo + hi);
 }
 
-// the reference's deterministic start vector (src/mps/lanczos.c:852-870):
+double tridiag_smallest(const double* d, const double* e, int K) {
+  return tridiag_eig(d, e, K, 0);
+}
+
+// unit eigenvector z of that matrix for its eigenvalue lam by inverse
+// iteration: Gaussian elimination of T - lam I with partial pivoting (U
+// with two superdiagonals; a zero pivot becomes eps |T|), three solves from
+// a fixed start vector, each rescaled by its largest entry
+void tridiag_vec(const double* d, const double* e, int K, double lam, double* z) {
+  if (K == 1) {
+    z[0] = 1.0;
+    return;
+  }
+  double nrm = 0.0;
+  for (int i = 0; i < K; i++)
+    nrm = std::max(nrm, fabs(d[i] - lam) + (i > 0 ? fabs(e[i - 1]) : 0.0) +
+                            (i + 1 < K ? fabs(e[i]) : 0.0));
+  const double tiny = nrm > 0.0 ? 2.220446049250313e-16 * nrm : 1e-300;
+  std::vector<double> u0(K), u1(K, 0.0), u2(K, 0.0), l(K, 0.0);
+  std::vector<char> sw(K, 0);
+  // p: the pivot candidate row (columns i, i+1, i+2), q: row i+1 as stored
+  double p0 = d[0] - lam, p1 = e[0], p2 = 0.0;
+  for (int i = 0; i + 1 < K; i++) {
+    double q0 = e[i], q1 = d[i + 1] - lam, q2 = i + 2 < K ? e[i + 1] : 0.0;
+    if (fabs(q0) > fabs(p0)) {
+      std::swap(p0, q0);
+      std::swap(p1, q1);
+      std::swap(p2, q2);
+      sw[i] = 1;
+    }
+    if (p0 == 0.0) p0 = tiny;
+    const double m = q0 / p0;
+    l[i] = m;
+    u0[i] = p0;
+    u1[i] = p1;
+    u2[i] = p2;
+    p0 = q1 - m * p1;
+    p1 = q2 - m * p2;
+    p2 = 0.0;
+  }
+  u0[K - 1] = p0 == 0.0 ? tiny : p0;
+  std::vector<double> b(K);
+  uint64_t st = 0x9E3779B97F4A7C15ULL;
+  for (int i = 0; i < K; i++) {
+    st ^= st << 13;
+    st ^= st >> 7;
+    st ^= st << 17;
+    b[i] = 0.5 + (double)(st >> 11) / 9007199254740992.0;
+  }
+  for (int it = 0; it < 3; it++) {
+    for (int i = 0; i + 1 < K; i++) {
+      if (sw[i]) std::swap(b[i], b[i + 1]);
+      b[i + 1] -= l[i] * b[i];
+    }
+    for (int i = K - 1; i >= 0; i--) {
+      double x = b[i];
+      if (i + 1 < K) x -= u1[i] * b[i + 1];
+      if (i + 2 < K) x -= u2[i] * b[i + 2];
+      b[i] = x / u0[i];
+    }
+    double mx = 0.0;
+    for (int i = 0; i < K; i++) mx = std::max(mx, fabs(b[i]));
+    if (!(mx > 0.0) || !std::isfinite(mx)) break;
+    for (int i = 0; i < K; i++) b[i] /= mx;
+  }
+  double n2 = 0.0;
+  for (int i = 0; i < K; i++) n2 += b[i] * b[i];
+  const double inv = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
+  for (int i = 0; i < K; i++) z[i] = b[i] * inv;
+}
+
+// the reference's deterministic start vector (src/mps/lanczos.c:160-176):
 // xorshift64 from seed ^ dim, element i = (x_i >> 11) / 2^53 - 1/2;
 // generated on the host in slices and uploaded
 int ed_seed_vector(double* d_v, long dim, unsigned long long seed) {
@@ -3102,34 +3200,120 @@ double ed_read(const double* d) {
   return v;
 }
 
-int ed_upload_bonds(int Lx, int Ly, int pbc, DBuf& store, int& nb) {
-  const auto bonds = kagome_bonds(Lx, Ly, pbc);
-  nb = (int)bonds.size();
+// H on the device: bond list and coefficients
+struct EdHam {
+  ScopedBuf bonds;
+  nqs::EdOp op;
+};
+
+int ed_ham_bonds(const std::vector<std::array<int, 2>>& bl, int N, EdHam& h) {
   std::vector<int> flat;
-  for (const auto& e : bonds) {
+  for (const auto& e : bl) {
     flat.push_back(e[0]);
     flat.push_back(e[1]);
   }
-  CKR(store.ensure(sizeof(int) * flat.size() + 16));
-  CK(hipMemcpyAsync(store.p, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice,
-                    g_stream));
+  CKR(h.bonds.b.ensure(sizeof(int) * flat.size() + 16));
+  if (!flat.empty())
+    CK(hipMemcpyAsync(h.bonds.b.p, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice,
+                      g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  h.op.bonds = h.bonds.b.as<int>();
+  h.op.nb = (int)bl.size();
+  h.op.N = N;
+  return 0;
+}
+
+// kagome Heisenberg J S_u . S_v (nqs_lanczos.c:336-432)
+int ed_ham_kagome(int Lx, int Ly, double J, int pbc, EdHam& h) {
+  CKR(ed_ham_bonds(kagome_bonds(Lx, Ly, pbc), 3 * Lx * Ly, h));
+  h.op.jd = 0.25 * J;
+  h.op.jo = 0.5 * J;
+  h.op.field = 0.0;
+  return 0;
+}
+
+// open Lx x Ly square lattice, site x Ly + y, bonds to (x + 1, y), (x, y + 1)
+std::vector<std::array<int, 2>> square_open_bonds(int Lx, int Ly) {
+  std::vector<std::array<int, 2>> bl;
+  for (int x = 0; x < Lx; x++)
+    for (int y = 0; y < Ly; y++) {
+      if (x + 1 < Lx) bl.push_back({x * Ly + y, (x + 1) * Ly + y});
+      if (y + 1 < Ly) bl.push_back({x * Ly + y, x * Ly + y + 1});
+    }
+  return bl;
+}
+
+// XXZ J/2 (S+S- + S-S+) + Jz Sz Sz on the open square lattice
+// (nqs_lanczos.c:224-281)
+int ed_ham_heisenberg(int Lx, int Ly, double J, double Jz, EdHam& h) {
+  CKR(ed_ham_bonds(square_open_bonds(Lx, Ly), Lx * Ly, h));
+  h.op.jd = 0.25 * Jz;
+  h.op.jo = 0.5 * J;
+  h.op.field = 0.0;
+  return 0;
+}
+
+// TFIM -J sz sz - Gamma sx on the open square lattice (nqs_lanczos.c:150-191)
+int ed_ham_tfim(int Lx, int Ly, double J, double Gamma, EdHam& h) {
+  CKR(ed_ham_bonds(square_open_bonds(Lx, Ly), Lx * Ly, h));
+  h.op.jd = -J;
+  h.op.jo = 0.0;
+  h.op.field = -Gamma;
+  return 0;
+}
+
+// the sector projector P = (1/G) sum_g chi_g T(g) on the device; identity
+// when G = 1 with the identity permutation and chi = 1
+struct EdProj {
+  bool identity = true;
+  int G = 1, nchunk = 0;
+  ScopedBuf tab, chi;
+};
+
+int ed_proj_build(const int* perm, const double* chars, int G, int N, const char* who,
+                  EdProj& p) {
+  std::vector<unsigned long long> tab;
+  bool id = true;
+  if (proj_tables(perm, G, N, true, who, tab, &id)) return -1;
+  p.identity = id && G == 1 && chars[0] == 1.0;
+  p.G = G;
+  p.nchunk = (N + 2) / 3;
+  if (stream_ready()) return -1;
+  CKR(p.tab.b.ensure(sizeof(unsigned long long) * tab.size()));
+  CKR(p.chi.b.ensure(sizeof(double) * G));
+  CK(hipMemcpyAsync(p.tab.b.p, tab.data(), sizeof(unsigned long long) * tab.size(),
+                    hipMemcpyHostToDevice, g_stream));
+  CK(hipMemcpyAsync(p.chi.b.p, chars, sizeof(double) * G, hipMemcpyHostToDevice, g_stream));
   CK(hipStreamSynchronize(g_stream));
   return 0;
 }
 
-// <psi|H|psi>/<psi|psi> with psi = cos(arg) exp(log|psi| - max) over the
-// 2^N basis states; amplitudes of a device source or of a host callback
-int kagome_exact_energy(nqs_log_amp_fn_t fn, void* user, int Lx, int Ly, double J, int pbc,
-                        double* out) {
-  const int N = 3 * Lx * Ly;
-  if (Lx < 1 || Ly < 1 || N > 24) {
-    fprintf(stderr, "nqs_exact_energy_kagome_heisenberg: N = 3 Lx Ly must be 3..24\n");
-    return -1;
+// x <- P x (out of place into tmp, then the pointers swap; in place for the
+// identity); returns |x|^2
+double ed_project_norm2(const EdProj& P, double*& x, double*& tmp, long dim, double* pt,
+                        double* sc) {
+  if (P.identity) {
+    if (nqs::launch_ed_scale(x, x, 1.0, dim, pt, sc + 1, g_stream)) return NAN;
+  } else {
+    if (nqs::launch_ed_project(x, tmp, dim, P.tab.b.as<unsigned long long>(),
+                               P.chi.b.as<double>(), P.G, P.nchunk, pt, sc + 1, g_stream))
+      return NAN;
+    std::swap(x, tmp);
   }
+  return ed_read(sc + 1);
+}
+
+double ed_norm2(const double* x, long dim, double* pt, double* sc) {
+  if (nqs::launch_ed_scale(x, const_cast<double*>(x), 1.0, dim, pt, sc + 1, g_stream)) return NAN;
+  return ed_read(sc + 1);
+}
+
+// log|psi| and arg of every basis state (bit i set <-> s_i = -1) into the
+// device arrays la, ar: a device ansatz source, an orbit source, or one
+// host call per state
+int ed_log_amps(nqs_log_amp_fn_t fn, void* user, int N, double* la, double* ar) {
   const long dim = 1L << N;
-  ScopedBuf la, ar, psi, hpsi, part, scal, bonds, spins;
-  CKR(la.b.ensure(sizeof(double) * dim));
-  CKR(ar.b.ensure(sizeof(double) * dim));
+  ScopedBuf spins;
   AmpSource src;
   if (recognise(fn, user, src) && src.a->N == N) {
     nqs_ansatz* a = src.a;
@@ -3142,159 +3326,308 @@ int kagome_exact_energy(nqs_log_amp_fn_t fn, void* user, int Lx, int Ly, double 
     for (long s0 = 0; s0 < dim; s0 += chunk) {
       const long n = std::min(chunk, dim - s0);
       CKR(nqs::launch_ed_state_spins(s0, n, N, spins.b.as<int>(), g_stream));
-      CKR(nqs::launch_forward<double>(v, spins.b.as<int>(), (int)n, nullptr, nullptr,
-                                      la.b.as<double>() + s0, ar.b.as<double>() + s0, g_stream));
+      CKR(nqs::launch_forward<double>(v, spins.b.as<int>(), (int)n, nullptr, nullptr, la + s0,
+                                      ar + s0, g_stream));
       if (src.marshall)
-        CKR(nqs::launch_marshall_arg(spins.b.as<int>(), (int)n, N, src.mlx, src.mly,
-                                     ar.b.as<double>() + s0, g_stream));
+        CKR(nqs::launch_marshall_arg(spins.b.as<int>(), (int)n, N, src.mlx, src.mly, ar + s0,
+                                     g_stream));
     }
-  } else if (OrbitSource orb; recognise_orbit(fn, user, orb) && orb.base.a->N == N) {
+    return 0;
+  }
+  if (OrbitSource orb; recognise_orbit(fn, user, orb) && orb.base.a->N == N) {
     const long chunk = std::min(dim, 1L << 16);
     CKR(spins.b.ensure(sizeof(int) * (size_t)chunk * N));
     for (long s0 = 0; s0 < dim; s0 += chunk) {
       const long n = std::min(chunk, dim - s0);
       CKR(nqs::launch_ed_state_spins(s0, n, N, spins.b.as<int>(), g_stream));
-      CKR(orbit_log_amp(orb, spins.b.as<int>(), n, la.b.as<double>() + s0, ar.b.as<double>() + s0));
+      CKR(orbit_log_amp(orb, spins.b.as<int>(), n, la + s0, ar + s0));
     }
-  } else {
-    std::vector<double> hl(dim), ha(dim);
-    std::vector<int> sp(N);
-    for (long s = 0; s < dim; s++) {
-      for (int i = 0; i < N; i++) sp[i] = ((s >> i) & 1) ? -1 : 1;
-      fn(sp.data(), N, user, &hl[s], &ha[s]);
-      g_host_calls++;
-    }
-    CK(hipMemcpyAsync(la.b.p, hl.data(), sizeof(double) * dim, hipMemcpyHostToDevice, g_stream));
-    CK(hipMemcpyAsync(ar.b.p, ha.data(), sizeof(double) * dim, hipMemcpyHostToDevice, g_stream));
+    return 0;
   }
+  std::vector<double> hl(dim), ha(dim);
+  std::vector<int> sp(N);
+  for (long s = 0; s < dim; s++) {
+    for (int i = 0; i < N; i++) sp[i] = ((s >> i) & 1) ? -1 : 1;
+    fn(sp.data(), N, user, &hl[s], &ha[s]);
+    g_host_calls++;
+  }
+  CK(hipMemcpyAsync(la, hl.data(), sizeof(double) * dim, hipMemcpyHostToDevice, g_stream));
+  CK(hipMemcpyAsync(ar, ha.data(), sizeof(double) * dim, hipMemcpyHostToDevice, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+// psi = Re psi / |Re psi| over the 2^N states into d_psi, with
+// Re psi(s) = cos(arg) exp(log|psi| - max log|psi|) (nqs_lanczos.c:115-149);
+// a warning on stderr when Im psi holds more than 1e-6 of the norm (:39-62).
+// -1 when N is outside 1..24.
+int ed_materialise(nqs_log_amp_fn_t fn, void* user, int N, double* d_psi) {
+  if (!fn || N <= 0 || N > 24) return -1;
+  const long dim = 1L << N;
+  ScopedBuf la, ar, part, scal;
+  CKR(la.b.ensure(sizeof(double) * dim));
+  CKR(ar.b.ensure(sizeof(double) * dim));
+  CKR(part.b.ensure(sizeof(double) * nqs::ed_partials()));
+  CKR(scal.b.ensure(sizeof(double) * 8));
+  CKR(ed_log_amps(fn, user, N, la.b.as<double>(), ar.b.as<double>()));
+  double* sc = scal.b.as<double>();
+  CKR(nqs::launch_ed_psi(la.b.as<double>(), ar.b.as<double>(), dim, d_psi, part.b.as<double>(),
+                         sc, g_stream));
+  const double n2 = ed_read(sc + 1), tot = ed_read(sc + 2);
+  if (!std::isfinite(n2) || !std::isfinite(tot)) return -1;
+  const double frac = tot > 0.0 ? (tot - n2) / tot : 0.0;
+  if (frac > 1e-6)
+    fprintf(stderr, "nqs_lanczos: warning: Im(psi) holds %.3e of the wavefunction norm; the "
+                    "materialised state is Re(psi), which is exact only for stoquastic ground "
+                    "states\n", frac);
+  const double inv = n2 > 0.0 ? 1.0 / sqrt(n2) : 1.0;
+  CKR(nqs::launch_ed_scale(d_psi, d_psi, inv, dim, part.b.as<double>(), nullptr, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+// <psi|H|psi> / <psi|psi> of the materialised state
+int ed_exact_energy(nqs_log_amp_fn_t fn, void* user, int N, const EdHam& H, double* out) {
+  const long dim = 1L << N;
+  ScopedBuf psi, hpsi, part, scal;
   CKR(psi.b.ensure(sizeof(double) * dim));
   CKR(hpsi.b.ensure(sizeof(double) * dim));
   CKR(part.b.ensure(sizeof(double) * nqs::ed_partials()));
   CKR(scal.b.ensure(sizeof(double) * 8));
+  CKR(ed_materialise(fn, user, N, psi.b.as<double>()));
   double* sc = scal.b.as<double>();
-  CKR(nqs::launch_ed_psi(la.b.as<double>(), ar.b.as<double>(), dim, psi.b.as<double>(),
-                         part.b.as<double>(), sc, g_stream));
-  int nb = 0;
-  CKR(ed_upload_bonds(Lx, Ly, pbc, bonds.b, nb));
-  CKR(nqs::launch_ed_matvec_kagome(psi.b.as<double>(), hpsi.b.as<double>(), dim, bonds.b.as<int>(),
-                                   nb, J, part.b.as<double>(), sc + 2, g_stream));
-  const double den = ed_read(sc + 1), num = ed_read(sc + 2);
-  if (!(den > 0.0)) return -1;
+  const double den = ed_norm2(psi.b.as<double>(), dim, part.b.as<double>(), sc);
+  CKR(nqs::launch_ed_matvec(H.op, psi.b.as<double>(), hpsi.b.as<double>(), dim,
+                            part.b.as<double>(), sc + 2, g_stream));
+  const double num = ed_read(sc + 2);
+  if (!(den > 0.0) || !std::isfinite(num)) return -1;
   *out = num / den;
   return 0;
 }
 
-}  // namespace
+// Full-reorthogonalisation Lanczos (src/mps/lanczos.c:119-306: smallest Ritz
+// value, stop when |beta_{k+1} z_{k}| < tol after at least two steps;
+// :308-433: the k smallest Ritz values after max_iters steps; :435-711: the
+// same with a projector applied to every new Krylov vector).  The basis V
+// [max_iters][dim] stays in HBM.  The reference's modified Gram-Schmidt
+// sweep over V_0..V_k is done as two passes of classical Gram-Schmidt
+// (h = V^T w, w -= V h, twice), which leaves w orthogonal to the basis to
+// working precision in two sweeps over V instead of k + 1 dependent ones.
+// The tridiagonal's eigenvalues come from Sturm bisection, the Ritz
+// vector's coefficients from inverse iteration.
+struct FullLanczos {
+  const EdHam* H = nullptr;
+  const EdProj* P = nullptr;     // null: no projection
+  int N = 0;
+  const double* seed = nullptr;  // device start vector; null: the deterministic one
+  int max_iters = 0;
+  double tol = 0.0;
+  int k = 0;                     // 0: smallest with convergence test, > 0: k smallest
+};
 
-NQS_EXPORT int nqs_exact_energy_kagome_heisenberg(nqs_ansatz_t* a, int Lx_cells, int Ly_cells,
-                                                  double J, int pbc, double* out_energy) {
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
-  if (!a || !out_energy) return -1;
-  if (stream_ready()) return -1;
-  return kagome_exact_energy(nqs_ansatz_log_amp, a, Lx_cells, Ly_cells, J, pbc, out_energy);
-}
-
-NQS_EXPORT int nqs_exact_energy_kagome_heisenberg_with_cb(nqs_log_amp_fn_t log_amp, void* user,
-                                                          int Lx_cells, int Ly_cells, double J,
-                                                          int pbc, double* out_energy) {
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
-  if (!log_amp || !out_energy) return -1;
-  if (stream_ready()) return -1;
-  return kagome_exact_energy(log_amp, user, Lx_cells, Ly_cells, J, pbc, out_energy);
-}
-
-// Lean projecting Lanczos (src/mps/lanczos.c:824-955 through
-// src/nqs/nqs_lanczos.c:629-647): three vectors and the projector's output
-// in HBM; alpha and beta are device reductions read back every step, the
-// tridiagonal's smallest eigenvalue is taken on the host.
-NQS_EXPORT int nqs_lanczos_e0_kagome_heisenberg_projected_lean(
-    int Lx_cells, int Ly_cells, double J, int pbc, const int* perm, const double* characters,
-    int G, int max_iters, double tol, double* out_eigenvalue, lanczos_result_t* out_result) {
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
-  if (!out_eigenvalue || !perm || !characters || G <= 0) return -1;
-  const int N = 3 * Lx_cells * Ly_cells;
-  if (Lx_cells < 1 || Ly_cells < 1 || N > 30 || max_iters <= 0) return -1;
-  if (out_result) *out_result = lanczos_result_t{0.0, 0, 0, 0.0};
-  // the projector's bit tables: bit i of T(g) s is bit perm[g][i] of s, so
-  // bit j of s lands at position pinv_g[j]
-  const int nchunk = (N + 2) / 3;
-  std::vector<unsigned long long> tab((size_t)G * nchunk * 8, 0ULL);
-  bool identity = G == 1 && characters[0] == 1.0;
-  for (int g = 0; g < G; g++) {
-    std::vector<int> pinv(N, -1);
-    for (int i = 0; i < N; i++) {
-      const int j = perm[(size_t)g * N + i];
-      if (j < 0 || j >= N || pinv[j] >= 0) {
-        fprintf(stderr, "nqs_lanczos_e0_kagome_heisenberg_projected_lean: perm[%d] is not a "
-                        "permutation of 0..%d\n", g, N - 1);
-        return -1;
-      }
-      pinv[j] = i;
-      if (j != i) identity = false;
-    }
-    for (int c = 0; c < nchunk; c++)
-      for (int v = 0; v < 8; v++) {
-        unsigned long long m = 0;
-        for (int b = 0; b < 3; b++)
-          if (((v >> b) & 1) && 3 * c + b < N) m |= 1ULL << pinv[3 * c + b];
-        tab[((size_t)g * nchunk + c) * 8 + v] = m;
-      }
-  }
-  if (stream_ready()) return -1;
-  const long dim = 1L << N;
+int full_lanczos(const FullLanczos& q, double* out_vec, double* out_vals, lanczos_result_t* out) {
+  if (q.N <= 0 || q.N > 30 || q.max_iters <= 0 || q.k < 0) return -1;
+  const long dim = 1L << q.N;
+  if (out) *out = lanczos_result_t{0.0, 0, 0, 0.0};
+  int max_iters = q.max_iters;
   if (max_iters > dim) max_iters = (int)dim;
-  ScopedBuf vb[4], part, scal, bonds, dtab, dchi;
-  for (auto& b : vb) CKR(b.b.ensure(sizeof(double) * dim));
+  int kk = std::min(q.k, max_iters);
+  const bool proj = q.P && !q.P->identity;
+  ScopedBuf vb, wb, tb, part, hb, scal;
+  if (vb.b.ensure(sizeof(double) * (size_t)max_iters * dim)) {
+    fprintf(stderr, "nqs_lanczos: the Krylov basis of %d vectors of 2^%d doubles does not fit\n",
+            max_iters, q.N);
+    return -1;
+  }
+  CKR(wb.b.ensure(sizeof(double) * dim));
+  if (proj) CKR(tb.b.ensure(sizeof(double) * dim));
+  CKR(part.b.ensure(sizeof(double) *
+                    std::max<size_t>(nqs::ed_partials(), nqs::ed_mdot_partials(dim, max_iters))));
+  CKR(hb.b.ensure(sizeof(double) * (max_iters + 1)));
+  CKR(scal.b.ensure(sizeof(double) * 8));
+  double* V = vb.b.as<double>();
+  double *wp = wb.b.as<double>(), *tp = tb.b.as<double>();
+  double *pt = part.b.as<double>(), *sc = scal.b.as<double>(), *h = hb.b.as<double>();
+  auto start_norm2 = [&]() -> double {
+    return q.P ? ed_project_norm2(*q.P, wp, tp, dim, pt, sc) : ed_norm2(wp, dim, pt, sc);
+  };
+  // start vector in w, then V_0 = w / |w|; a caller's start vector is not
+  // projected
+  double n2;
+  if (q.seed) {
+    CK(hipMemcpyAsync(wp, q.seed, sizeof(double) * dim, hipMemcpyDeviceToDevice, g_stream));
+    n2 = ed_norm2(wp, dim, pt, sc);
+  } else {
+    CKR(ed_seed_vector(wp, dim, 0xA5A5A5A5A5A5A5A5ULL));
+    n2 = start_norm2();
+  }
+  if (!(sqrt(n2) >= 1e-14)) {
+    CKR(ed_seed_vector(wp, dim, 0xBEEFBABEULL));
+    n2 = start_norm2();
+  }
+  if (!std::isfinite(n2)) return -1;
+  CKR(nqs::launch_ed_scale(wp, V, n2 > 0.0 ? 1.0 / sqrt(n2) : 1.0, dim, pt, nullptr, g_stream));
+  std::vector<double> alpha(max_iters), beta(max_iters + 1, 0.0), d(max_iters), e(max_iters),
+      z(max_iters);
+  auto load_t = [&](int K) {
+    for (int i = 0; i < K; i++) d[i] = alpha[i];
+    for (int i = 0; i + 1 < K; i++) e[i] = beta[i + 1];
+  };
+  // Ritz vector sum_j z_j V_j / |.| into out_vec (host)
+  auto ritz = [&](int K) -> int {
+    std::vector<double> nz(K);
+    for (int i = 0; i < K; i++) nz[i] = -z[i];
+    CK(hipMemcpyAsync(h, nz.data(), sizeof(double) * K, hipMemcpyHostToDevice, g_stream));
+    CKR(nqs::launch_ed_mupdate(V, dim, K, h, wp, 0, pt, sc + 1, g_stream));
+    const double r2 = ed_read(sc + 1);
+    if (!std::isfinite(r2)) return -1;
+    if (r2 > 0.0) CKR(nqs::launch_ed_scale(wp, wp, 1.0 / sqrt(r2), dim, pt, nullptr, g_stream));
+    CK(hipMemcpyAsync(out_vec, wp, sizeof(double) * dim, hipMemcpyDeviceToHost, g_stream));
+    CK(hipStreamSynchronize(g_stream));
+    return 0;
+  };
+  double resid = 0.0;
+  int kb;
+  for (kb = 0; kb < max_iters; kb++) {
+    double* vk = V + (size_t)kb * dim;
+    CKR(nqs::launch_ed_matvec(q.H->op, vk, wp, dim, pt, sc, g_stream));
+    alpha[kb] = ed_read(sc);
+    CKR(nqs::launch_ed_axpy2(wp, vk, alpha[kb], kb > 0 ? vk - dim : nullptr, beta[kb], dim, pt,
+                             nullptr, g_stream));
+    for (int pass = 0; pass < 2; pass++) {
+      CKR(nqs::launch_ed_mdot(V, dim, kb + 1, wp, pt, h, g_stream));
+      CKR(nqs::launch_ed_mupdate(V, dim, kb + 1, h, wp, 1, pt,
+                                 (pass == 1 && !q.P) ? sc + 1 : nullptr, g_stream));
+    }
+    const double b2 = q.P ? ed_project_norm2(*q.P, wp, tp, dim, pt, sc) : ed_read(sc + 1);
+    if (!std::isfinite(alpha[kb]) || !std::isfinite(b2)) return -1;
+    beta[kb + 1] = sqrt(b2);
+    if (beta[kb + 1] < 1e-14) {
+      kb++;
+      break;
+    }
+    if (q.k == 0) {
+      const int K = kb + 1;
+      load_t(K);
+      const double lam = tridiag_eig(d.data(), e.data(), K, 0);
+      tridiag_vec(d.data(), e.data(), K, lam, z.data());
+      resid = fabs(beta[kb + 1] * z[K - 1]);
+      if (resid < q.tol && kb >= 1) {
+        if (out_vec) CKR(ritz(K));
+        out_vals[0] = lam;
+        if (out) *out = lanczos_result_t{lam, K, 1, resid};
+        return 0;
+      }
+    }
+    if (kb + 1 < max_iters)
+      CKR(nqs::launch_ed_scale(wp, V + (size_t)(kb + 1) * dim, 1.0 / beta[kb + 1], dim, pt,
+                               nullptr, g_stream));
+  }
+  const int K = std::max(kb, 1);
+  load_t(K);
+  if (q.k == 0) {
+    const double lam = tridiag_eig(d.data(), e.data(), K, 0);
+    if (out_vec) {
+      tridiag_vec(d.data(), e.data(), K, lam, z.data());
+      CKR(ritz(K));
+    }
+    out_vals[0] = lam;
+    if (out) *out = lanczos_result_t{lam, kb, 0, resid};
+    return 0;
+  }
+  kk = std::min(kk, K);
+  for (int i = 0; i < kk; i++) out_vals[i] = tridiag_eig(d.data(), e.data(), K, i);
+  if (out)
+    *out = lanczos_result_t{tridiag_eig(d.data(), e.data(), K, 0), K, beta[K] < 1e-10 ? 1 : 0,
+                            beta[K]};
+  return 0;
+}
+
+// the reference's seeded drivers: start from the materialised Re psi when
+// it exists (N <= 24), else from the deterministic start vector
+int seeded_lanczos(nqs_log_amp_fn_t fn, void* user, const EdHam& H, const EdProj* P, int N,
+                   int max_iters, double tol, int k, double* out_vec, double* out_vals,
+                   lanczos_result_t* out) {
+  if (N <= 0 || N > 30) return -1;
+  ScopedBuf seed;
+  const double* sp = nullptr;
+  if (fn && N <= 24 && max_iters > 0) {
+    CKR(seed.b.ensure(sizeof(double) * (1L << N)));
+    if (ed_materialise(fn, user, N, seed.b.as<double>()) == 0) sp = seed.b.as<double>();
+  }
+  FullLanczos q;
+  q.H = &H;
+  q.P = P;
+  q.N = N;
+  q.seed = sp;
+  q.max_iters = max_iters;
+  q.tol = tol;
+  q.k = k;
+  return full_lanczos(q, out_vec, out_vals, out);
+}
+
+// Memory-lean projecting Lanczos (src/mps/lanczos.c:824-970): three vectors
+// and the projector's output in HBM; alpha and beta are device reductions
+// read back every step, the tridiagonal's smallest eigenvalue is taken on
+// the host.  Stops when the smallest Ritz value moves by less than tol after
+// more than 5 steps or when beta < 1e-14 (then the previous step's Ritz
+// value is returned, as in the reference).
+//
+// With out_vec (src/mps/lanczos.c:972-1130): the same first pass, then the
+// Ritz value and its coefficients z from the final tridiagonal, then a
+// replay of the recurrence from the same start vector accumulating
+// sum_k z_k v_k; the kernels reduce in a fixed order, so the replay
+// reproduces the first pass's vectors bit for bit.
+int lean_lanczos(const EdHam& H, const EdProj& P, int N, int max_iters, double tol,
+                 double* out_val, double* out_vec, lanczos_result_t* out) {
+  const long dim = 1L << N;
+  if (out) *out = lanczos_result_t{0.0, 0, 0, 0.0};
+  if (max_iters > dim) max_iters = (int)dim;
+  const int nvec = out_vec ? 6 : 4;
+  ScopedBuf vb[6], part, scal;
+  for (int i = 0; i < nvec; i++) CKR(vb[i].b.ensure(sizeof(double) * dim));
   CKR(part.b.ensure(sizeof(double) * nqs::ed_partials()));
   CKR(scal.b.ensure(sizeof(double) * 8));
-  CKR(dtab.b.ensure(sizeof(unsigned long long) * tab.size()));
-  CKR(dchi.b.ensure(sizeof(double) * G));
-  CK(hipMemcpyAsync(dtab.b.p, tab.data(), sizeof(unsigned long long) * tab.size(),
-                    hipMemcpyHostToDevice, g_stream));
-  CK(hipMemcpyAsync(dchi.b.p, characters, sizeof(double) * G, hipMemcpyHostToDevice, g_stream));
-  int nb = 0;
-  CKR(ed_upload_bonds(Lx_cells, Ly_cells, pbc, bonds.b, nb));
   double* sc = scal.b.as<double>();
   double* pt = part.b.as<double>();
-  // v_cur, v_prev, w, projector output (rotated by pointer)
+  // v_cur, v_prev, w, projector output (rotated by pointer); start vector
+  // and Ritz vector for the two-pass variant
   double *vc = vb[0].b.as<double>(), *vp = vb[1].b.as<double>(), *w = vb[2].b.as<double>(),
          *tmp = vb[3].b.as<double>();
-  auto project_norm2 = [&](double*& x) -> double {
-    if (identity) {
-      if (nqs::launch_ed_scale(x, x, 1.0, dim, pt, sc + 1, g_stream)) return NAN;
-    } else {
-      if (nqs::launch_ed_project(x, tmp, dim, dtab.b.as<unsigned long long>(),
-                                 dchi.b.as<double>(), G, nchunk, pt, sc + 1, g_stream))
-        return NAN;
-      std::swap(x, tmp);
-    }
-    return ed_read(sc + 1);
-  };
   CKR(ed_seed_vector(vc, dim, 0xA5A5A5A5A5A5A5A5ULL));
-  double n2 = project_norm2(vc);
+  double n2 = ed_project_norm2(P, vc, tmp, dim, pt, sc);
   if (!(sqrt(n2) >= 1e-14)) {
     // the projector removed the start vector: the reference's second seed
     CKR(ed_seed_vector(vc, dim, 0xBEEFBABEULL));
-    n2 = project_norm2(vc);
+    n2 = ed_project_norm2(P, vc, tmp, dim, pt, sc);
   }
   if (!std::isfinite(n2)) return -1;
   if (n2 > 0.0) CKR(nqs::launch_ed_scale(vc, vc, 1.0 / sqrt(n2), dim, pt, nullptr, g_stream));
+  double* seed = out_vec ? vb[4].b.as<double>() : nullptr;
+  if (seed)
+    CK(hipMemcpyAsync(seed, vc, sizeof(double) * dim, hipMemcpyDeviceToDevice, g_stream));
   std::vector<double> alpha(max_iters), beta(max_iters + 1, 0.0), d(max_iters), e(max_iters);
+  // w = P (H v_cur - alpha v_cur - beta_k v_prev); alpha from the matvec's
+  // dot when fresh, |w|^2 returned
+  auto step = [&](int k, bool fresh) -> double {
+    if (nqs::launch_ed_matvec(H.op, vc, w, dim, pt, sc, g_stream)) return NAN;
+    if (fresh) alpha[k] = ed_read(sc);
+    if (nqs::launch_ed_axpy2(w, vc, alpha[k], k > 0 ? vp : nullptr, beta[k], dim, pt,
+                             P.identity ? sc + 1 : nullptr, g_stream))
+      return NAN;
+    return P.identity ? ed_read(sc + 1) : ed_project_norm2(P, w, tmp, dim, pt, sc);
+  };
   double lambda = 0.0, prev = 0.0;
-  int k = 0, converged = 0;
+  int k = 0, converged = 0, K = 0;
   for (k = 0; k < max_iters; k++) {
-    CKR(nqs::launch_ed_matvec_kagome(vc, w, dim, bonds.b.as<int>(), nb, J, pt, sc, g_stream));
-    alpha[k] = ed_read(sc);
-    CKR(nqs::launch_ed_axpy2(w, vc, alpha[k], k > 0 ? vp : nullptr, beta[k], dim, pt,
-                             identity ? sc + 1 : nullptr, g_stream));
-    const double b2 = identity ? ed_read(sc + 1) : project_norm2(w);
+    const double b2 = step(k, true);
     if (!std::isfinite(alpha[k]) || !std::isfinite(b2)) return -1;
     beta[k + 1] = sqrt(b2);
+    K = k + 1;
     if (beta[k + 1] < 1e-14) {
       k++;
       break;
     }
-    const int K = k + 1;
     for (int i = 0; i < K; i++) d[i] = alpha[i];
     for (int i = 0; i + 1 < K; i++) e[i] = beta[i + 1];
     prev = lambda;
@@ -3308,14 +3641,279 @@ NQS_EXPORT int nqs_lanczos_e0_kagome_heisenberg_projected_lean(
     std::swap(vp, vc);
     CKR(nqs::launch_ed_scale(w, vc, 1.0 / beta[k + 1], dim, pt, nullptr, g_stream));
   }
-  *out_eigenvalue = lambda;
-  if (out_result) {
-    out_result->eigenvalue = lambda;
-    out_result->iterations = k;
-    out_result->converged = converged;
-    out_result->residual_norm = k > 0 ? fabs(lambda - prev) : 0.0;
+  if (!out_vec) {
+    *out_val = lambda;
+    if (out) *out = lanczos_result_t{lambda, k, converged, k > 0 ? fabs(lambda - prev) : 0.0};
+    return 0;
   }
+  // second pass: Ritz value and coefficients of the final tridiagonal
+  for (int i = 0; i < K; i++) d[i] = alpha[i];
+  for (int i = 0; i + 1 < K; i++) e[i] = beta[i + 1];
+  lambda = tridiag_eig(d.data(), e.data(), K, 0);
+  std::vector<double> z(K);
+  tridiag_vec(d.data(), e.data(), K, lambda, z.data());
+  // third pass: replay from the start vector, acc = sum_k z_k v_k
+  double* acc = vb[5].b.as<double>();
+  CK(hipMemcpyAsync(vc, seed, sizeof(double) * dim, hipMemcpyDeviceToDevice, g_stream));
+  CKR(nqs::launch_ed_scale(seed, acc, z[0], dim, pt, nullptr, g_stream));
+  for (int j = 0; j + 1 < K; j++) {
+    if (!std::isfinite(step(j, false))) return -1;
+    if (beta[j + 1] < 1e-14) break;
+    std::swap(vp, vc);
+    CKR(nqs::launch_ed_scale(w, vc, 1.0 / beta[j + 1], dim, pt, nullptr, g_stream));
+    CKR(nqs::launch_ed_axpy2(acc, vc, -z[j + 1], nullptr, 0.0, dim, pt, nullptr, g_stream));
+  }
+  const double a2 = ed_norm2(acc, dim, pt, sc);
+  if (!std::isfinite(a2)) return -1;
+  if (a2 > 0.0) CKR(nqs::launch_ed_scale(acc, acc, 1.0 / sqrt(a2), dim, pt, nullptr, g_stream));
+  CK(hipMemcpyAsync(out_vec, acc, sizeof(double) * dim, hipMemcpyDeviceToHost, g_stream));
+  CK(hipStreamSynchronize(g_stream));
+  *out_val = lambda;
+  const double dl = fabs(lambda - prev);
+  if (out) *out = lanczos_result_t{lambda, K, (K >= 6 && dl < tol) ? 1 : 0, K >= 2 ? dl : 0.0};
   return 0;
+}
+
+// the materialised state as a malloc'd host vector (the reference's
+// nqs_materialise_state* contract: the caller frees it)
+int materialise_host(nqs_log_amp_fn_t fn, void* user, int N, double** out_psi, long* out_dim) {
+  if (!fn || !out_psi || !out_dim || N <= 0 || N > 24) return -1;
+  if (stream_ready()) return -1;
+  const long dim = 1L << N;
+  ScopedBuf psi;
+  CKR(psi.b.ensure(sizeof(double) * dim));
+  CKR(ed_materialise(fn, user, N, psi.b.as<double>()));
+  double* h = (double*)malloc(sizeof(double) * dim);
+  if (!h) return -1;
+  if (hipMemcpyAsync(h, psi.b.p, sizeof(double) * dim, hipMemcpyDeviceToHost, g_stream) ||
+      hipStreamSynchronize(g_stream)) {
+    free(h);
+    return -1;
+  }
+  *out_psi = h;
+  *out_dim = dim;
+  return 0;
+}
+
+bool kagome_size_ok(int Lx, int Ly, int max_n, const char* who) {
+  if (Lx >= 1 && Ly >= 1 && 3 * Lx * Ly <= max_n) return true;
+  fprintf(stderr, "%s: N = 3 Lx Ly must be 3..%d\n", who, max_n);
+  return false;
+}
+
+bool square_size_ok(int Lx, int Ly, int max_n, const char* who) {
+  if (Lx >= 1 && Ly >= 1 && Lx * Ly <= max_n) return true;
+  fprintf(stderr, "%s: N = Lx Ly must be 1..%d\n", who, max_n);
+  return false;
+}
+
+}  // namespace
+
+NQS_EXPORT int nqs_materialise_state(nqs_ansatz_t* a, int Lx, int Ly, double** out_psi,
+                                     long* out_dim) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || Lx <= 0 || Ly <= 0) return -1;
+  return materialise_host(nqs_ansatz_log_amp, a, Lx * Ly, out_psi, out_dim);
+}
+
+NQS_EXPORT int nqs_materialise_state_with_cb(nqs_log_amp_fn_t log_amp, void* user, int Lx, int Ly,
+                                             double** out_psi, long* out_dim) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (Lx <= 0 || Ly <= 0) return -1;
+  return materialise_host(log_amp, user, Lx * Ly, out_psi, out_dim);
+}
+
+NQS_EXPORT int nqs_materialise_state_with_cb_N(nqs_log_amp_fn_t log_amp, void* user, int N,
+                                               double** out_psi, long* out_dim) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  return materialise_host(log_amp, user, N, out_psi, out_dim);
+}
+
+NQS_EXPORT int nqs_exact_energy_tfim(nqs_ansatz_t* a, int Lx, int Ly, double J, double Gamma,
+                                     double* out_energy) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !out_energy || !square_size_ok(Lx, Ly, 24, "nqs_exact_energy_tfim")) return -1;
+  if (stream_ready()) return -1;
+  EdHam H;
+  CKR(ed_ham_tfim(Lx, Ly, J, Gamma, H));
+  return ed_exact_energy(nqs_ansatz_log_amp, a, Lx * Ly, H, out_energy);
+}
+
+NQS_EXPORT int nqs_lanczos_refine_tfim(nqs_ansatz_t* a, int Lx, int Ly, double J, double Gamma,
+                                       int max_iters, double tol, double* out_eigenvalue,
+                                       double* out_eigenvector, lanczos_result_t* out_result) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !out_eigenvalue || !square_size_ok(Lx, Ly, 30, "nqs_lanczos_refine_tfim")) return -1;
+  if (stream_ready()) return -1;
+  EdHam H;
+  CKR(ed_ham_tfim(Lx, Ly, J, Gamma, H));
+  // the reference starts the TFIM refiner from its deterministic vector
+  return seeded_lanczos(nullptr, nullptr, H, nullptr, Lx * Ly, max_iters, tol, 0, out_eigenvector,
+                        out_eigenvalue, out_result);
+}
+
+NQS_EXPORT int nqs_exact_energy_heisenberg(nqs_ansatz_t* a, int Lx, int Ly, double J, double Jz,
+                                           double* out_energy) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !out_energy || !square_size_ok(Lx, Ly, 24, "nqs_exact_energy_heisenberg")) return -1;
+  if (stream_ready()) return -1;
+  EdHam H;
+  CKR(ed_ham_heisenberg(Lx, Ly, J, Jz, H));
+  return ed_exact_energy(nqs_ansatz_log_amp, a, Lx * Ly, H, out_energy);
+}
+
+NQS_EXPORT int nqs_lanczos_refine_heisenberg(nqs_ansatz_t* a, int Lx, int Ly, double J, double Jz,
+                                             int max_iters, double tol, double* out_eigenvalue,
+                                             double* out_eigenvector,
+                                             lanczos_result_t* out_result) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !out_eigenvalue || !square_size_ok(Lx, Ly, 30, "nqs_lanczos_refine_heisenberg"))
+    return -1;
+  if (stream_ready()) return -1;
+  EdHam H;
+  CKR(ed_ham_heisenberg(Lx, Ly, J, Jz, H));
+  return seeded_lanczos(nqs_ansatz_log_amp, a, H, nullptr, Lx * Ly, max_iters, tol, 0,
+                        out_eigenvector, out_eigenvalue, out_result);
+}
+
+NQS_EXPORT int nqs_exact_energy_kagome_heisenberg(nqs_ansatz_t* a, int Lx_cells, int Ly_cells,
+                                                  double J, int pbc, double* out_energy) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!a || !out_energy ||
+      !kagome_size_ok(Lx_cells, Ly_cells, 24, "nqs_exact_energy_kagome_heisenberg"))
+    return -1;
+  if (stream_ready()) return -1;
+  EdHam H;
+  CKR(ed_ham_kagome(Lx_cells, Ly_cells, J, pbc, H));
+  return ed_exact_energy(nqs_ansatz_log_amp, a, 3 * Lx_cells * Ly_cells, H, out_energy);
+}
+
+NQS_EXPORT int nqs_exact_energy_kagome_heisenberg_with_cb(nqs_log_amp_fn_t log_amp, void* user,
+                                                          int Lx_cells, int Ly_cells, double J,
+                                                          int pbc, double* out_energy) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!log_amp || !out_energy ||
+      !kagome_size_ok(Lx_cells, Ly_cells, 24, "nqs_exact_energy_kagome_heisenberg"))
+    return -1;
+  if (stream_ready()) return -1;
+  EdHam H;
+  CKR(ed_ham_kagome(Lx_cells, Ly_cells, J, pbc, H));
+  return ed_exact_energy(log_amp, user, 3 * Lx_cells * Ly_cells, H, out_energy);
+}
+
+NQS_EXPORT int nqs_lanczos_refine_kagome_heisenberg_with_cb(
+    nqs_log_amp_fn_t log_amp, void* user, int Lx_cells, int Ly_cells, double J, int pbc,
+    int max_iters, double tol, double* out_eigenvalue, double* out_eigenvector,
+    lanczos_result_t* out_result) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!log_amp || !out_eigenvalue ||
+      !kagome_size_ok(Lx_cells, Ly_cells, 30, "nqs_lanczos_refine_kagome_heisenberg"))
+    return -1;
+  if (stream_ready()) return -1;
+  EdHam H;
+  CKR(ed_ham_kagome(Lx_cells, Ly_cells, J, pbc, H));
+  return seeded_lanczos(log_amp, user, H, nullptr, 3 * Lx_cells * Ly_cells, max_iters, tol, 0,
+                        out_eigenvector, out_eigenvalue, out_result);
+}
+
+NQS_EXPORT int nqs_lanczos_refine_kagome_heisenberg(nqs_ansatz_t* a, int Lx_cells, int Ly_cells,
+                                                    double J, int pbc, int max_iters, double tol,
+                                                    double* out_eigenvalue,
+                                                    double* out_eigenvector,
+                                                    lanczos_result_t* out_result) {
+  if (!a) return -1;
+  return nqs_lanczos_refine_kagome_heisenberg_with_cb(nqs_ansatz_log_amp, a, Lx_cells, Ly_cells,
+                                                      J, pbc, max_iters, tol, out_eigenvalue,
+                                                      out_eigenvector, out_result);
+}
+
+NQS_EXPORT int nqs_lanczos_k_lowest_kagome_heisenberg_with_cb(
+    nqs_log_amp_fn_t log_amp, void* user, int Lx_cells, int Ly_cells, double J, int pbc,
+    int max_iters, int k, double* out_eigenvalues, lanczos_result_t* out_result) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!log_amp || !out_eigenvalues || k <= 0 ||
+      !kagome_size_ok(Lx_cells, Ly_cells, 30, "nqs_lanczos_k_lowest_kagome_heisenberg"))
+    return -1;
+  if (stream_ready()) return -1;
+  EdHam H;
+  CKR(ed_ham_kagome(Lx_cells, Ly_cells, J, pbc, H));
+  return seeded_lanczos(log_amp, user, H, nullptr, 3 * Lx_cells * Ly_cells, max_iters, 0.0, k,
+                        nullptr, out_eigenvalues, out_result);
+}
+
+NQS_EXPORT int nqs_lanczos_k_lowest_kagome_heisenberg(nqs_ansatz_t* a, int Lx_cells, int Ly_cells,
+                                                      double J, int pbc, int max_iters, int k,
+                                                      double* out_eigenvalues,
+                                                      lanczos_result_t* out_result) {
+  if (!a) return -1;
+  return nqs_lanczos_k_lowest_kagome_heisenberg_with_cb(nqs_ansatz_log_amp, a, Lx_cells, Ly_cells,
+                                                        J, pbc, max_iters, k, out_eigenvalues,
+                                                        out_result);
+}
+
+NQS_EXPORT int nqs_lanczos_k_lowest_kagome_heisenberg_projected(
+    nqs_log_amp_fn_t log_amp, void* user, int Lx_cells, int Ly_cells, double J, int pbc,
+    const int* perm, const double* characters, int G, int max_iters, int k,
+    double* out_eigenvalues, lanczos_result_t* out_result) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!log_amp || !out_eigenvalues || !perm || !characters || G <= 0 || k <= 0 ||
+      !kagome_size_ok(Lx_cells, Ly_cells, 30, "nqs_lanczos_k_lowest_kagome_heisenberg_projected"))
+    return -1;
+  const int N = 3 * Lx_cells * Ly_cells;
+  EdProj P;
+  CKR(ed_proj_build(perm, characters, G, N, "nqs_lanczos_k_lowest_kagome_heisenberg_projected",
+                    P));
+  EdHam H;
+  CKR(ed_ham_kagome(Lx_cells, Ly_cells, J, pbc, H));
+  return seeded_lanczos(log_amp, user, H, &P, N, max_iters, 0.0, k, nullptr, out_eigenvalues,
+                        out_result);
+}
+
+NQS_EXPORT int nqs_lanczos_refine_kagome_heisenberg_projected(
+    nqs_log_amp_fn_t log_amp, void* user, int Lx_cells, int Ly_cells, double J, int pbc,
+    const int* perm, const double* characters, int G, int max_iters, double tol,
+    double* out_eigenvalue, double* out_eigenvector, lanczos_result_t* out_result) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!log_amp || !out_eigenvalue || !perm || !characters || G <= 0 ||
+      !kagome_size_ok(Lx_cells, Ly_cells, 30, "nqs_lanczos_refine_kagome_heisenberg_projected"))
+    return -1;
+  const int N = 3 * Lx_cells * Ly_cells;
+  EdProj P;
+  CKR(ed_proj_build(perm, characters, G, N, "nqs_lanczos_refine_kagome_heisenberg_projected", P));
+  EdHam H;
+  CKR(ed_ham_kagome(Lx_cells, Ly_cells, J, pbc, H));
+  return seeded_lanczos(log_amp, user, H, &P, N, max_iters, tol, 0, out_eigenvector,
+                        out_eigenvalue, out_result);
+}
+
+NQS_EXPORT int nqs_lanczos_e0_kagome_heisenberg_projected_lean(
+    int Lx_cells, int Ly_cells, double J, int pbc, const int* perm, const double* characters,
+    int G, int max_iters, double tol, double* out_eigenvalue, lanczos_result_t* out_result) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!out_eigenvalue || !perm || !characters || G <= 0) return -1;
+  if (Lx_cells < 1 || Ly_cells < 1 || 3 * Lx_cells * Ly_cells > 30 || max_iters <= 0) return -1;
+  const int N = 3 * Lx_cells * Ly_cells;
+  EdProj P;
+  CKR(ed_proj_build(perm, characters, G, N, "nqs_lanczos_e0_kagome_heisenberg_projected_lean", P));
+  EdHam H;
+  CKR(ed_ham_kagome(Lx_cells, Ly_cells, J, pbc, H));
+  return lean_lanczos(H, P, N, max_iters, tol, out_eigenvalue, nullptr, out_result);
+}
+
+NQS_EXPORT int nqs_lanczos_e0_kagome_heisenberg_projected_lean_eigvec(
+    int Lx_cells, int Ly_cells, double J, int pbc, const int* perm, const double* characters,
+    int G, int max_iters, double tol, double* out_eigenvalue, double* out_eigenvector,
+    lanczos_result_t* out_result) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!out_eigenvalue || !out_eigenvector || !perm || !characters || G <= 0) return -1;
+  if (Lx_cells < 1 || Ly_cells < 1 || 3 * Lx_cells * Ly_cells > 30 || max_iters <= 0) return -1;
+  const int N = 3 * Lx_cells * Ly_cells;
+  EdProj P;
+  CKR(ed_proj_build(perm, characters, G, N,
+                    "nqs_lanczos_e0_kagome_heisenberg_projected_lean_eigvec", P));
+  EdHam H;
+  CKR(ed_ham_kagome(Lx_cells, Ly_cells, J, pbc, H));
+  return lean_lanczos(H, P, N, max_iters, tol, out_eigenvalue, out_eigenvector, out_result);
 }
 
 NQS_EXPORT int nqs_hip_minsr_solve(nqs_ansatz_t* a, const int* spins, int B, const double* e_loc,

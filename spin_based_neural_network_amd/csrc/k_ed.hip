@@ -1,22 +1,28 @@
-// k_ed.hip — exact diagonalisation of the kagome Heisenberg model on the
-// full 2^N basis (N <= 30), for the lean Lanczos of engine.hip.
+// k_ed.hip — exact diagonalisation on the full 2^N basis (N <= 30): the
+// matrix-vector products of src/nqs/nqs_lanczos.c and the vector kernels of
+// the Lanczos drivers in engine.hip (src/mps/lanczos.c).
 //
 // Basis state s: bit i set <-> s_i = -1 (src/nqs/nqs_lanczos.c:16-20).
-// H = J sum_bonds S_u . S_v:  diagonal (J/4) s_u s_v per bond, and for an
-// antiparallel bond (J/2) |s with u, v flipped> (nqs_lanczos.c:380-432).
+// H = sum_bonds [jd s_u s_v + jo (|s with u, v flipped> if s_u != s_v)]
+//     + field sum_i |s with i flipped>:
+//   Heisenberg XXZ (nqs_lanczos.c:228-281, kagome :343-432): jd = Jz / 4,
+//   jo = J / 2, field = 0;  TFIM (:150-191): jd = -J, jo = 0, field = -Gamma.
 // One thread per basis state; a wave's 64 consecutive states read, for one
-// bond mask m, the 64 states s ^ m, which lie in one aligned 512-byte block
+// flip mask m, the 64 states s ^ m, which lie in one aligned 512-byte block
 // (or permute inside it when m touches the low 6 bits), so every gather is
-// a coalesced block read.  The bond list (at most 6 per cell, 60 at N = 30)
-// is staged in LDS.
+// a coalesced block read.  The bond list (at most 64 bonds) is staged in LDS.
 //
 // Sector projection (nqs_symproj.c:786-851): out[s] = (1/G) sum_g chi_g
 // in[s_g], where bit i of s_g is bit perm_g[i] of s; s_g is assembled from
 // per-(g, 3-bit chunk) tables of OR-masks staged in LDS.
 //
+// Full reorthogonalisation against a Krylov basis V [nv][dim] in HBM:
+// k_ed_mdot forms h_j = <V_j, w> for every j (eight basis vectors per sweep
+// over w), k_ed_mupdate applies w -= sum_j h_j V_j reading each V_j once.
+//
 // Reductions (dot products, norms) write one partial per workgroup in a
-// fixed grid; k_ed_finish adds them in workgroup order, so every scalar of
-// the recurrence is reproducible run to run.
+// fixed grid; the finish kernels add them in a fixed order, so every scalar
+// of the recurrence is reproducible run to run.
 #include "nqs_kernels.h"
 
 namespace nqs {
@@ -38,10 +44,11 @@ __device__ inline double block_sum_256(double v, double* red) {
 }
 
 // out = H in; part[blockIdx] = sum_s in[s] out[s] over the block's states
-__global__ __launch_bounds__(256) void k_ed_matvec_kagome(const double* __restrict__ in,
-                                                          double* __restrict__ out, long dim,
-                                                          const int* __restrict__ bonds, int nb,
-                                                          double J, double* __restrict__ part) {
+__global__ __launch_bounds__(256) void k_ed_matvec(const double* __restrict__ in,
+                                                   double* __restrict__ out, long dim,
+                                                   const int* __restrict__ bonds, int nb,
+                                                   double jd, double jo, double field, int N,
+                                                   double* __restrict__ part) {
   __shared__ unsigned long long msk[kEdMaxBonds];
   __shared__ int bu[kEdMaxBonds], bv[kEdMaxBonds];
   __shared__ double red[256];
@@ -51,7 +58,6 @@ __global__ __launch_bounds__(256) void k_ed_matvec_kagome(const double* __restri
     msk[b] = (1ULL << bu[b]) | (1ULL << bv[b]);
   }
   __syncthreads();
-  const double jd = 0.25 * J, jo = 0.5 * J;
   double dot = 0.0;
   for (long s = blockIdx.x * 256L + threadIdx.x; s < dim; s += (long)gridDim.x * 256) {
     const unsigned long long us = (unsigned long long)s;
@@ -60,8 +66,10 @@ __global__ __launch_bounds__(256) void k_ed_matvec_kagome(const double* __restri
     for (int b = 0; b < nb; b++) {
       const int anti = (int)(((us >> bu[b]) ^ (us >> bv[b])) & 1ULL);
       diag += anti ? -jd : jd;
-      if (anti) off += jo * in[(long)(us ^ msk[b])];
+      if (anti && jo != 0.0) off += jo * in[(long)(us ^ msk[b])];
     }
+    if (field != 0.0)
+      for (int i = 0; i < N; i++) off += field * in[(long)(us ^ (1ULL << i))];
     const double y = diag * x + off;
     out[s] = y;
     dot += x * y;
@@ -180,22 +188,94 @@ __global__ void k_ed_max_finish(const double* __restrict__ part, int n, double* 
 }
 
 // psi[s] = cos(ar[s]) exp(la[s] - lmax[0]) (real part of the amplitude
-// relative to the largest); part: sum psi^2
+// relative to the largest); part: sum psi^2, part2 (when given): sum of
+// exp(2 (la[s] - lmax[0])), the squared norm of the complex amplitude
 __global__ __launch_bounds__(256) void k_ed_psi(const double* __restrict__ la,
                                                 const double* __restrict__ ar,
                                                 const double* __restrict__ lmax, long dim,
                                                 double* __restrict__ psi,
-                                                double* __restrict__ part) {
+                                                double* __restrict__ part,
+                                                double* __restrict__ part2) {
   __shared__ double red[256];
   const double m = lmax[0];
-  double acc = 0.0;
+  double acc = 0.0, tot = 0.0;
   for (long s = blockIdx.x * 256L + threadIdx.x; s < dim; s += (long)gridDim.x * 256) {
-    const double v = cos(ar ? ar[s] : 0.0) * exp(la[s] - m);
+    const double a = exp(la[s] - m);
+    const double v = cos(ar ? ar[s] : 0.0) * a;
     psi[s] = v;
     acc += v * v;
+    tot += a * a;
   }
   const double t = block_sum_256(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
+  if (part2) {
+    const double t2 = block_sum_256(tot, red);
+    if (threadIdx.x == 0) part2[blockIdx.x] = t2;
+  }
+}
+
+// part[j * gridDim.x + blockIdx] = sum over the block's states of
+// V[j][s] w[s], j < nv; eight basis vectors per sweep over w
+__global__ __launch_bounds__(256) void k_ed_mdot(const double* __restrict__ V, long dim, int nv,
+                                                 const double* __restrict__ w,
+                                                 double* __restrict__ part) {
+  __shared__ double red[8][256];
+  const int tid = threadIdx.x;
+  for (int j0 = 0; j0 < nv; j0 += 8) {
+    const int nj = nv - j0 < 8 ? nv - j0 : 8;
+    double acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) acc[q] = 0.0;
+    for (long s = blockIdx.x * 256L + tid; s < dim; s += (long)gridDim.x * 256) {
+      const double x = w[s];
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (q < nj) acc[q] += V[(size_t)(j0 + q) * dim + s] * x;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) red[q][tid] = acc[q];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (tid < h) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) red[q][tid] += red[q][tid + h];
+      }
+      __syncthreads();
+    }
+    if (tid < nj) part[(size_t)(j0 + tid) * gridDim.x + blockIdx.x] = red[tid][0];
+    __syncthreads();
+  }
+}
+
+// out[j] = sum of the n partials of row j (one workgroup per row)
+__global__ __launch_bounds__(256) void k_ed_finish_rows(const double* __restrict__ part, int n,
+                                                        double* __restrict__ out) {
+  __shared__ double red[256];
+  const double* p = part + (size_t)blockIdx.x * n;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += p[i];
+  const double t = block_sum_256(acc, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+// w[s] = (keep ? w[s] : 0) - sum_{j < nv} h[j] V[j][s];
+// part[blockIdx] = sum w^2 when part != null
+__global__ __launch_bounds__(256) void k_ed_mupdate(const double* __restrict__ V, long dim, int nv,
+                                                    const double* __restrict__ h,
+                                                    double* __restrict__ w, int keep,
+                                                    double* __restrict__ part) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (long s = blockIdx.x * 256L + threadIdx.x; s < dim; s += (long)gridDim.x * 256) {
+    double x = keep ? w[s] : 0.0;
+    for (int j = 0; j < nv; j++) x -= h[j] * V[(size_t)j * dim + s];
+    w[s] = x;
+    acc += x * x;
+  }
+  if (part) {
+    const double t = block_sum_256(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+  }
 }
 
 static int ed_grid(long dim) {
@@ -203,18 +283,18 @@ static int ed_grid(long dim) {
   return (int)(g < kEdBlocks ? g : kEdBlocks);
 }
 
-int ed_partials() { return kEdBlocks; }
+int ed_partials() { return 2 * kEdBlocks; }
 
-int launch_ed_matvec_kagome(const double* in, double* out, long dim, const int* bonds, int nb,
-                            double J, double* part, double* dot, hipStream_t st) {
-  if (nb > kEdMaxBonds) {
-    fprintf(stderr, "nqs_hip: kagome ED supports at most %d bonds\n", kEdMaxBonds);
+int launch_ed_matvec(const EdOp& op, const double* in, double* out, long dim, double* part,
+                     double* dot, hipStream_t st) {
+  if (op.nb > kEdMaxBonds) {
+    fprintf(stderr, "nqs_hip: exact diagonalisation supports at most %d bonds\n", kEdMaxBonds);
     return -1;
   }
   const int g = ed_grid(dim);
   KScope ks("ed_matvec", st);
-  hipLaunchKernelGGL(k_ed_matvec_kagome, dim3(g), dim3(256), 0, st, in, out, dim, bonds, nb, J,
-                     part);
+  hipLaunchKernelGGL(k_ed_matvec, dim3(g), dim3(256), 0, st, in, out, dim, op.bonds, op.nb, op.jd,
+                     op.jo, op.field, op.N, part);
   hipLaunchKernelGGL(k_ed_finish, dim3(1), dim3(256), 0, st, part, g, dot);
   count_launch();
   count_launch();
@@ -282,9 +362,41 @@ int launch_ed_psi(const double* la, const double* ar, long dim, double* psi, dou
   const int g = ed_grid(dim);
   hipLaunchKernelGGL(k_ed_max, dim3(g), dim3(256), 0, st, la, dim, part);
   hipLaunchKernelGGL(k_ed_max_finish, dim3(1), dim3(64), 0, st, part, g, scal);
-  hipLaunchKernelGGL(k_ed_psi, dim3(g), dim3(256), 0, st, la, ar, scal, dim, psi, part);
+  hipLaunchKernelGGL(k_ed_psi, dim3(g), dim3(256), 0, st, la, ar, scal, dim, psi, part,
+                     part + kEdBlocks);
   hipLaunchKernelGGL(k_ed_finish, dim3(1), dim3(256), 0, st, part, g, scal + 1);
-  for (int i = 0; i < 4; i++) count_launch();
+  hipLaunchKernelGGL(k_ed_finish, dim3(1), dim3(256), 0, st, part + kEdBlocks, g, scal + 2);
+  for (int i = 0; i < 5; i++) count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+size_t ed_mdot_partials(long dim, int nv) { return (size_t)ed_grid(dim) * (nv > 0 ? nv : 1); }
+
+int launch_ed_mdot(const double* V, long dim, int nv, const double* w, double* part, double* h,
+                   hipStream_t st) {
+  if (nv <= 0) return 0;
+  const int g = ed_grid(dim);
+  KScope ks("ed_reorth", st);
+  hipLaunchKernelGGL(k_ed_mdot, dim3(g), dim3(256), 0, st, V, dim, nv, w, part);
+  hipLaunchKernelGGL(k_ed_finish_rows, dim3(nv), dim3(256), 0, st, part, g, h);
+  count_launch();
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_ed_mupdate(const double* V, long dim, int nv, const double* h, double* w, int keep,
+                      double* part, double* norm2, hipStream_t st) {
+  const int g = ed_grid(dim);
+  KScope ks("ed_reorth", st);
+  hipLaunchKernelGGL(k_ed_mupdate, dim3(g), dim3(256), 0, st, V, dim, nv, h, w, keep,
+                     norm2 ? part : nullptr);
+  count_launch();
+  if (norm2) {
+    hipLaunchKernelGGL(k_ed_finish, dim3(1), dim3(256), 0, st, part, g, norm2);
+    count_launch();
+  }
   NQS_HIP_CHECK(hipGetLastError());
   return 0;
 }

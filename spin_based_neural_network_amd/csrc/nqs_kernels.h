@@ -279,12 +279,19 @@ size_t chi_work_elems(long P);
 int launch_chi_trace(const double* O_re, const double* O_im, long B, long P, double* work,
                      double* out, hipStream_t st);
 
-// ---- kagome exact diagonalisation on the 2^N basis (k_ed.hip) -------------
-// Every reduction writes ed_partials() partials and reduces them into a
-// device scalar in a fixed order.  bonds: device [nb][2] site pairs.
+// ---- exact diagonalisation on the 2^N basis (k_ed.hip) --------------------
+// Every reduction writes partials (part: at least ed_partials() doubles) and
+// reduces them into a device scalar in a fixed order.
 int ed_partials();
-int launch_ed_matvec_kagome(const double* in, double* out, long dim, const int* bonds, int nb,
-                            double J, double* part, double* dot, hipStream_t st);
+// H = sum_bonds [jd s_u s_v + jo flip(u, v) if s_u != s_v] + field sum_i flip(i)
+struct EdOp {
+  const int* bonds = nullptr;   // device [nb][2] site pairs
+  int nb = 0;
+  double jd = 0.0, jo = 0.0, field = 0.0;
+  int N = 0;                    // sites flipped by the field term
+};
+int launch_ed_matvec(const EdOp& op, const double* in, double* out, long dim, double* part,
+                     double* dot, hipStream_t st);
 int launch_ed_axpy2(double* w, const double* v, double a, const double* u, double b, long dim,
                     double* part, double* norm2, hipStream_t st);
 // tab: device [G][nchunk][8] OR-masks, chi: device [G]
@@ -294,7 +301,15 @@ int launch_ed_project(const double* in, double* out, long dim, const unsigned lo
 int launch_ed_scale(const double* x, double* y, double c, long dim, double* part, double* norm2,
                     hipStream_t st);
 int launch_ed_state_spins(long s0, long n, int N, int* out, hipStream_t st);
-// psi = cos(ar) exp(la - max la); scal[0] = max la, scal[1] = sum psi^2
+// Krylov basis V [nv][dim]: h[j] = <V_j, w> (part: ed_mdot_partials(dim, nv)
+// doubles); w = (keep ? w : 0) - sum_j h[j] V_j, norm2 = |w|^2 when given
+size_t ed_mdot_partials(long dim, int nv);
+int launch_ed_mdot(const double* V, long dim, int nv, const double* w, double* part, double* h,
+                   hipStream_t st);
+int launch_ed_mupdate(const double* V, long dim, int nv, const double* h, double* w, int keep,
+                      double* part, double* norm2, hipStream_t st);
+// psi = cos(ar) exp(la - max la); scal[0] = max la, scal[1] = sum psi^2,
+// scal[2] = sum exp(2 (la - max la))
 int launch_ed_psi(const double* la, const double* ar, long dim, double* psi, double* part,
                   double* scal, hipStream_t st);
 

@@ -11,9 +11,9 @@ import ctypes
 
 import numpy as np
 
-from spin_based_neural_network_amd.abi import (MarshallWrapper, SrInfo, SymprojWrapper,
-                                               TranslationWrapper, config_defaults, fnptr,
-                                               kagome_group, ptr)
+from spin_based_neural_network_amd.abi import (LanczosResult, MarshallWrapper, SrInfo,
+                                               SymprojWrapper, TranslationWrapper,
+                                               config_defaults, fnptr, kagome_group, ptr)
 
 RBM, CRBM, MF = 5, 6, 0
 TFIM, HEIS, J1J2, XXZ, KAGOME = 0, 1, 2, 4, 6
@@ -462,9 +462,119 @@ def run_sym(r, sc):
     return out
 
 
+# Lanczos drivers of include/nqs/nqs_lanczos.h (src/nqs/nqs_lanczos.c,
+# src/mps/lanczos.c).  "sym" seeds through a p6m sector projection of the
+# ansatz (the research driver scripts/research_kagome_b1_train.c:91-114,
+# 166-205: complex RBM, 24 hidden units, init 0.05, seed 0xB1B1B1B1).
+_B1 = dict(ansatz=CRBM, hamiltonian=KAGOME, kagome_pbc=1, rbm_hidden_units=24,
+           rbm_init_scale=0.05, rng_seed=0xB1B1B1B1)
+LANC = [
+    dict(name="refine_symB1_kagome_2x2", what="refine_cb", sym=("p6m", (2,), 2), cfg=_B1, Lx=2, Ly=2, pbc=1, iters=200, tol=1e-12),
+    dict(name="klow_symB1_kagome_2x2", what="klow_cb", sym=("p6m", (2,), 2), cfg=_B1, Lx=2, Ly=2, pbc=1, iters=300, k=4),
+    dict(name="klow_rbm_kagome_2x2", what="klow", cfg=dict(ansatz=RBM, hamiltonian=KAGOME, rbm_hidden_units=12, rbm_init_scale=0.2, rng_seed=5), Lx=2, Ly=2, pbc=1, iters=120, k=4),
+    dict(name="refine_rbm_kagome_1x2_obc", what="refine", cfg=dict(ansatz=RBM, hamiltonian=KAGOME, rbm_hidden_units=6, rbm_init_scale=0.3, rng_seed=6), Lx=1, Ly=2, pbc=0, iters=64, tol=1e-12),
+    dict(name="klowproj_A1_kagome_2x2", what="klow_proj", sym=("p6m", (2,), 0), group=("p6m", (2,), 0), cfg=dict(_B1, rng_seed=0xA1A1), Lx=2, Ly=2, pbc=1, iters=150, k=4),
+    dict(name="refineproj_B1_kagome_2x2", what="refine_proj", sym=("p6m", (2,), 2), group=("p6m", (2,), 2), cfg=_B1, Lx=2, Ly=2, pbc=1, iters=200, tol=1e-12),
+    dict(name="lean_eigvec_B1_kagome_2x2", what="lean_eigvec", group=("p6m", (2,), 2), Lx=2, Ly=2, pbc=1, iters=300, tol=1e-11),
+    dict(name="refine_tfim_3x3", what="tfim", cfg=dict(ansatz=RBM, rbm_init_scale=0.1), Lx=3, Ly=3, J=1.0, h=0.5, iters=200, tol=1e-12),
+    dict(name="refine_heis_4x2_xxz", what="heis", cfg=dict(ansatz=RBM, hamiltonian=HEIS, rbm_hidden_units=8, rbm_init_scale=0.2, rng_seed=9), Lx=4, Ly=2, J=1.0, h=0.7, iters=200, tol=1e-12),
+    dict(name="materialise_crbm_3x3", what="materialise", cfg=dict(ansatz=CRBM, rbm_hidden_units=6, rbm_init_scale=0.3, rng_seed=8), Lx=3, Ly=3),
+    dict(name="exact_heis_3x3_xxz", what="exact_heis", cfg=dict(ansatz=RBM, hamiltonian=HEIS, rbm_hidden_units=9, rbm_init_scale=0.3, rng_seed=10), Lx=3, Ly=3, J=1.0, h=0.6),
+]
+
+
+def run_lanc(r, sc):
+    """Empty for a library without the Lanczos drivers."""
+    L = r.lib
+    if not hasattr(L, "nqs_lanczos_k_lowest_kagome_heisenberg_projected"):
+        return {}
+    Lx, Ly = sc["Lx"], sc["Ly"]
+    what = sc["what"]
+    out = {}
+    lr = LanczosResult()
+    e = ctypes.c_double(0)
+    info = lambda: np.array([lr.eigenvalue, lr.iterations, lr.converged, lr.residual_norm])
+    if what == "lean_eigvec":
+        N = 3 * Lx * Ly
+        perm, chi = _sym_tables(r, sc)
+        perm = np.ascontiguousarray(perm, dtype=np.int32)
+        chi = np.ascontiguousarray(chi, dtype=float)
+        vec = np.zeros(1 << N)
+        assert L.nqs_lanczos_e0_kagome_heisenberg_projected_lean_eigvec(
+            Lx, Ly, 1.0, sc["pbc"], ptr(perm), ptr(chi), perm.shape[0], sc["iters"], sc["tol"],
+            ctypes.byref(e), ptr(vec), ctypes.byref(lr)) == 0
+        return dict(evals=np.array([e.value]), info=info(), vec=vec)
+    cfg = _cfg(sc["cfg"])
+    N = _n_sites(cfg, Lx, Ly)
+    h = r.ansatz(cfg, N)
+    fn, user = r.amp_fn, h
+    if "sym" in sc:
+        kind, size, irrep = sc["sym"]
+        perm, chi = _sym_tables(r, dict(name=sc["name"], group=sc["sym"]))
+        perm = np.ascontiguousarray(perm, dtype=np.int32)
+        chi = np.ascontiguousarray(chi, dtype=float)
+        w = SymprojWrapper(r.amp_fn, h, N, perm.shape[0], perm.ctypes.data, chi.ctypes.data)
+        r._keep += [w, perm, chi]
+        fn, user = fnptr(L, r.p + "symproj_log_amp"), ctypes.addressof(w)
+    if "group" in sc:
+        gp, gc = _sym_tables(r, sc)
+        gp = np.ascontiguousarray(gp, dtype=np.int32)
+        gc = np.ascontiguousarray(gc, dtype=float)
+    vec = np.zeros(1 << N)
+    if what in ("refine", "refine_cb"):
+        if what == "refine":
+            rc = L.nqs_lanczos_refine_kagome_heisenberg(h, Lx, Ly, 1.0, sc["pbc"], sc["iters"],
+                                                        sc["tol"], ctypes.byref(e), ptr(vec),
+                                                        ctypes.byref(lr))
+        else:
+            rc = L.nqs_lanczos_refine_kagome_heisenberg_with_cb(fn, user, Lx, Ly, 1.0, sc["pbc"],
+                                                                sc["iters"], sc["tol"],
+                                                                ctypes.byref(e), ptr(vec),
+                                                                ctypes.byref(lr))
+        assert rc == 0
+        out = dict(evals=np.array([e.value]), info=info(), vec=vec)
+    elif what in ("klow", "klow_cb", "klow_proj"):
+        ev = np.zeros(sc["k"])
+        if what == "klow":
+            rc = L.nqs_lanczos_k_lowest_kagome_heisenberg(h, Lx, Ly, 1.0, sc["pbc"], sc["iters"],
+                                                          sc["k"], ptr(ev), ctypes.byref(lr))
+        elif what == "klow_cb":
+            rc = L.nqs_lanczos_k_lowest_kagome_heisenberg_with_cb(fn, user, Lx, Ly, 1.0, sc["pbc"],
+                                                                  sc["iters"], sc["k"], ptr(ev),
+                                                                  ctypes.byref(lr))
+        else:
+            rc = L.nqs_lanczos_k_lowest_kagome_heisenberg_projected(
+                fn, user, Lx, Ly, 1.0, sc["pbc"], ptr(gp), ptr(gc), gp.shape[0], sc["iters"],
+                sc["k"], ptr(ev), ctypes.byref(lr))
+        assert rc == 0
+        out = dict(evals=ev, info=info())
+    elif what == "refine_proj":
+        assert L.nqs_lanczos_refine_kagome_heisenberg_projected(
+            fn, user, Lx, Ly, 1.0, sc["pbc"], ptr(gp), ptr(gc), gp.shape[0], sc["iters"],
+            sc["tol"], ctypes.byref(e), ptr(vec), ctypes.byref(lr)) == 0
+        out = dict(evals=np.array([e.value]), info=info(), vec=vec)
+    elif what in ("tfim", "heis"):
+        f = L.nqs_lanczos_refine_tfim if what == "tfim" else L.nqs_lanczos_refine_heisenberg
+        assert f(h, Lx, Ly, sc["J"], sc["h"], sc["iters"], sc["tol"], ctypes.byref(e), ptr(vec),
+                 ctypes.byref(lr)) == 0
+        out = dict(evals=np.array([e.value]), info=info(), vec=vec)
+    elif what == "materialise":
+        pp = ctypes.POINTER(ctypes.c_double)()
+        dim = ctypes.c_long(0)
+        assert L.nqs_materialise_state(h, Lx, Ly, ctypes.byref(pp), ctypes.byref(dim)) == 0
+        out = dict(psi=np.ctypeslib.as_array(pp, (dim.value,)).copy())
+        ctypes.CDLL(None).free(ctypes.cast(pp, ctypes.c_void_p))
+    elif what == "exact_heis":
+        assert L.nqs_exact_energy_heisenberg(h, Lx, Ly, sc["J"], sc["h"], ctypes.byref(e)) == 0
+        out = dict(energy=np.array([e.value]))
+    r.f("ansatz_free")(h)
+    return out
+
+
 KINDS = {"init": (INIT, run_init), "amp": (AMP, run_amp), "eloc": (ELOC, run_eloc),
          "sampler": (SAMPLER, run_sampler), "sr": (SR, run_sr), "run": (RUN, run_run),
-         "dyn": (DYN, run_dyn), "diag": (DIAG, run_diag), "sym": (SYM, run_sym)}
+         "dyn": (DYN, run_dyn), "diag": (DIAG, run_diag), "sym": (SYM, run_sym),
+         "lanc": (LANC, run_lanc)}
 
 
 def sym_close(got, ref, amp_tol, grad_tol):
