@@ -1,7 +1,7 @@
 """Benchmark of the NQS VMC hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5]
-                    [--budget K] [--precision fp64|fp32] [--gram direct|factored|auto]
+                    [--budget K] [--precision fp64|fp32|mixed] [--gram direct|factored|auto]
                     [--move flip|exchange|learned] [--no-cpu] [--no-c3]
 
 A step is one full SR iteration (Metropolis sampling of every chain, local
@@ -44,6 +44,7 @@ from spin_based_neural_network_amd.stats import pooled_tau, tau_ci  # noqa: E402
 
 METRIC = "effective samples/s & SR-iter wall time, learned vs plain MH, same ΔE vs ED"
 PEAK_FP64 = 78.6e12          # MI355X dense FP64 (vector / MFMA), FLOP/s
+PEAK_BF16 = 2.5e15           # MI355X dense BF16 MFMA, FLOP/s
 PEAK_HBM = 8.0e12            # MI355X HBM3E, B/s
 LEARNED_LR = 1e-3            # Adam rate of the learned proposal (configuration 5)
 
@@ -120,7 +121,8 @@ def setup(lib, w, args, move=None):
           "learned": abi.MOVE_LEARNED}.get(move or args.move, w["move"])
     opt = abi.options_defaults(num_chains=w["chains"], move=mv, sz_zero_start=w["sz0"],
                                periodic=w["periodic"],
-                               precision=abi.FP32 if args.precision == "fp32" else abi.FP64,
+                               precision={"fp64": abi.FP64, "fp32": abi.FP32,
+                                          "mixed": abi.MIXED}[args.precision],
                                gram={"direct": abi.GRAM_DIRECT, "factored": abi.GRAM_FACTORED,
                                      "auto": abi.GRAM_AUTO}[args.gram])
     if mv == abi.MOVE_LEARNED:
@@ -195,7 +197,7 @@ SYMBOL = {"gemm_potrf": "nqs::k_gemm_big<1>", "gemm": "nqs::k_gemm_big<0>",
           "potrf_panel": "nqs::k_panel",
           "eloc": "nqs::k_eloc_terms<double, true>", "gen_O": "nqs::k_gen_O<double>",
           "colsum": "nqs::k_colsum_partial", "mh": "nqs::k_mh<double, true, 64, 4>",
-          "trsv": "nqs::k_sbsv_fwd"}
+          "trsv": "nqs::k_sbsv_fwd", "gram_x3": "nqs::k_gemm_x3"}
 
 
 def traffic_bytes(workload, tag):
@@ -210,7 +212,8 @@ def traffic_bytes(workload, tag):
     return None if v is None else v["fetch"] + v["write"]
 
 
-TAGS = ["mh", "ar_mh", "forward", "eloc", "gen_O", "colsum", "gemm", "gemm_small", "gemm_potrf", "potrf_panel",
+TAGS = ["mh", "ar_mh", "forward", "eloc", "gen_O", "colsum", "gemm", "gemm_small", "gram_x3",
+        "gemm_potrf", "potrf_panel",
         "trsv", "trsv_inv", "gram_f", "multi_prep", "multi_gemm_z", "multi_rows", "multi_gemm_g", "multi_vec"]
 
 
@@ -280,12 +283,22 @@ def kernel_work(tag, w, st, launches, iters):
             return "mfma", B * (B + 1) * kdim / per_iter, "FLOP"
         # lower triangle of T = O_c O_c^T: B(B+1)/2 dot products of length P
         return "mfma", B * (B + 1) * P / per_iter, "FLOP"
+    if tag == "gram_x3":
+        # the mixed mode's Gram on the split-BF16 MFMA (three BF16 products
+        # per algorithmic multiply-add): lower triangle of S = O_c^T O_c
+        # (K = B) or of T = O_c O_c^T (K = P)
+        if st["dual"]:
+            return "mfma", P * (P + 1) * B / per_iter, "FLOP"
+        return "mfma", B * (B + 1) * P / per_iter, "FLOP"
+    # the Cholesky factors the P x P matrix in the parameter-space solve
+    n_chol = P if st["dual"] else B
     if tag == "gemm_potrf":
-        return "mfma", potrf_gemm_flops(B) / per_iter, "FLOP"
+        return "mfma", potrf_gemm_flops(n_chol) / per_iter, "FLOP"
     if tag == "potrf_panel":
         # per inner panel: factor + inverse of the 64x64 block, and the rows
         # below solved by a 64-wide product (and the preceding panel's
         # rank-64 update for every second panel), averaged over the panels
+        B = n_chol
         npan = (B + 63) // 64
         tot = sum(2 * 64 ** 3 / 3 + 2 * (B - 64 * (k + 1)) * 64 * 64 * (2 if k % 2 else 1)
                   for k in range(npan))
@@ -381,7 +394,8 @@ def gpu_run(args, w, move=None, budget=0):
         avg_ms, iters = ms_prof / launches, 1
     bound, work, unit = kernel_work(dom, w, st, launches, iters)
     if bound == "mfma":
-        achieved, peak, runit = work / (avg_ms * 1e-3) / 1e12, PEAK_FP64 / 1e12, "TFLOP/s"
+        pk = PEAK_BF16 if dom == "gram_x3" else PEAK_FP64
+        achieved, peak, runit = work / (avg_ms * 1e-3) / 1e12, pk / 1e12, "TFLOP/s"
     else:
         achieved, peak, runit = work / (avg_ms * 1e-3) / 1e9, PEAK_HBM / 1e9, "GB/s"
     res = dict(
@@ -395,7 +409,9 @@ def gpu_run(args, w, move=None, budget=0):
         roofline=dict(kernel=dom, symbol=SYMBOL.get(dom), bound=bound, achieved=achieved, peak=peak,
                       unit=runit, frac=achieved / peak, traffic=traffic_bytes(args.workload, dom),
                       launches=c.value, avg_ms=avg_ms,
-                      work_per_launch=work, work_unit=unit),
+                      work_per_launch=work, work_unit=unit,
+                      **({"mfma_products_per_flop": 3, "mfma_util": 3 * achieved / peak}
+                         if dom == "gram_x3" else {})),
         split={t: dict(launches=cc, ms=mm) for t, (cc, mm) in per.items()}, N=st["N"], P=st["P"],
         dual=st["dual"], factored=st["factored"])
     lib.nqs_sampler_free(st["s"])
@@ -738,7 +754,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "mixed"])
     ap.add_argument("--gram", default="auto", choices=["direct", "factored", "auto"])
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the HBM copy probe on MIB mebibytes (counter calibration)")
@@ -798,7 +814,7 @@ def line(args, w, gpu, cpu):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": {"fp64": "f64", "fp32": "f32+f64", "mixed": "bf16x3+f32+f64"}[args.precision],
         "data": "synthetic (random-init parameters N(0,0.01), chains started at random Sz=0 states)",
         "config": {"workload": args.workload + ": " + w["desc"], "N": gpu["N"], "P": gpu["P"],
                    "samples_per_iter": w["chains"] * w["per_chain"], "chains": w["chains"],
@@ -815,7 +831,9 @@ def line(args, w, gpu, cpu):
         if gpu["stage_ms"] else None,
         "kernel_split_ms": {k: v["ms"] for k, v in gpu["split"].items()},
         "roofline": {k: gpu["roofline"][k] for k in ("bound", "achieved", "peak", "unit", "frac",
-                                                     "traffic", "kernel", "symbol", "avg_ms")},
+                                                     "traffic", "kernel", "symbol", "avg_ms",
+                                                     "mfma_products_per_flop", "mfma_util")
+                     if k in gpu["roofline"]},
         "cpu_baseline": cpu,
     }
     if gpu["learned"]:

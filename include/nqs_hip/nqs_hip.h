@@ -40,8 +40,18 @@ typedef enum {
 
 typedef enum {
     NQS_HIP_FP64 = 0,          /* all arithmetic FP64 (parity mode)           */
-    NQS_HIP_FP32 = 1           /* amplitudes, E_loc, moves and O in FP32;
+    NQS_HIP_FP32 = 1,          /* amplitudes, E_loc, moves and O in FP32;
                                   reductions, Gram, solve and update FP64     */
+    NQS_HIP_MIXED = 2          /* as FP32, with theta = S W^T of the O rows
+                                  on BF16 MFMA (spins exact, weights split
+                                  in three BF16 parts) and the dense MinSR
+                                  Gram (O_c^T O_c or O_c O_c^T) on FP16
+                                  MFMA: entries scaled by a power of two and
+                                  split into hi + lo halves, products as
+                                  hi hi + hi lo + lo hi (about 3 * 2^-22 per
+                                  product), FP32 accumulation over at most
+                                  2048 terms, FP64 across them; the shifted
+                                  solve, reductions and delta stay FP64     */
 } nqs_hip_precision_t;
 
 typedef enum {
@@ -253,6 +263,14 @@ int nqs_hip_cholesky_solve_noalloc(double *d_T, int n, const double *d_b, double
 int nqs_hip_minsr_solve(nqs_ansatz_t *a, const int *spins, int B, const double *e_loc,
                         double shift, int gram, double *out_delta, double *out_y,
                         double *out_T);
+
+/* nqs_hip_minsr_solve with a sampler's options: opt->gram selects the Gram
+ * as above, opt->precision the arithmetic (NQS_HIP_FP32: O from FP32
+ * amplitudes; NQS_HIP_MIXED: additionally the dense Gram on the split
+ * 16-bit MFMA).  Same returns. */
+int nqs_hip_minsr_solve_ex(nqs_ansatz_t *a, const int *spins, int B, const double *e_loc,
+                           double shift, const nqs_hip_options_t *opt, double *out_delta,
+                           double *out_y, double *out_T);
 
 /* Live kernel timing with HIP events on the library's stream.
  * nqs_hip_kernel_timing("eloc,gemm") starts timing the listed kernel tags

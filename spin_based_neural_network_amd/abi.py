@@ -28,7 +28,7 @@ ANSATZ_RBM, ANSATZ_CRBM, ANSATZ_FVIT_C = 5, 6, 7
 HAM_TFIM, HAM_HEIS, HAM_J1J2, HAM_KITAEV, HAM_XXZ, HAM_KH, HAM_KAGOME = range(7)
 # nqs_hip_move_t / nqs_hip_precision_t / nqs_hip_gram_t
 MOVE_FLIP, MOVE_EXCHANGE, MOVE_LEARNED = 0, 1, 2
-FP64, FP32 = 0, 1
+FP64, FP32, MIXED = 0, 1, 2
 GRAM_DIRECT, GRAM_FACTORED, GRAM_AUTO, GRAM_PARAMETER = 0, 1, 2, 3
 
 
@@ -194,6 +194,7 @@ HIP_API = {
     "nqs_hip_config_layout": (c_int, [_P, c_int]),
     "nqs_hip_gram": (c_int, [_P, c_int, c_long, c_double, _P]),
     "nqs_hip_minsr_solve": (c_int, [_P, _P, c_int, _P, c_double, c_int, _P, _P, _P]),
+    "nqs_hip_minsr_solve_ex": (c_int, [_P, _P, c_int, _P, c_double, _OPT, _P, _P, _P]),
     "nqs_hip_kernel_timing": (c_int, [c_char_p]),
     "nqs_hip_copy_probe": (c_double, [c_long, c_int]),
     "nqs_hip_kernel_stats": (c_int, [c_char_p, POINTER(c_longlong), POINTER(c_double)]),

@@ -436,8 +436,12 @@ bool recognise_orbit(nqs_log_amp_fn_t fn, void* user, OrbitSource& o) {
 struct Workspace {
   DBuf spins, t_re, t_im, q_re, q_im, la, ar, e_re, e_im, O, Oi, colwork, u1, u2, vecs, T, diag,
       y, scal, cg, fail, ham, sublat, hostla, hostar, linv, T2, Sd, elw, fwd, xr, heun, bond_ham,
-      bond_sub, orb_rows, orb_lp, orb_ar, orb_O, orb_Oi, orb_w, orb_cfg;
+      bond_sub, orb_rows, orb_lp, orb_ar, orb_O, orb_Oi, orb_w, orb_cfg, o32, x3part, x3max;
 };
+
+// FP32 arithmetic for amplitudes, local energies, moves and O (the FP32 and
+// mixed modes)
+inline bool low_prec(int p) { return p == NQS_HIP_FP32 || p == NQS_HIP_MIXED; }
 Workspace g_ws;
 
 // Hamiltonian on the device, rebuilt when the request changes.
@@ -546,7 +550,7 @@ int ham_upload(const nqs::HamHost& H, const AmpSource& src, DBuf& store, DBuf& s
 // (and, with want_q, the small logistic members the local energies use)
 template <typename R>
 int forward_t(nqs_ansatz* a, const int* d_spins, int B, double* la, double* ar,
-              bool want_q = false) {
+              bool want_q = false, bool x3 = false) {
   ModelView<R> v;
   CKR(model_view<R>(a, v));
   const size_t tb = sizeof(R) * (size_t)B * (size_t)std::max(a->M, 1);
@@ -569,6 +573,16 @@ int forward_t(nqs_ansatz* a, const int* d_spins, int B, double* la, double* ar,
                                       g_ws.t_re.as<double>(), g_ws.t_im.as<double>(),
                                       g_ws.fwd.as<double>(), g_stream, (double*)q_re,
                                       (double*)q_im);
+  }
+  // the mixed mode's FP32 t of an RBM: theta on the split-BF16 MFMA
+  if (std::is_same<R, float>::value && x3 && !la && !ar && !want_q && a->M >= 1 &&
+      (a->kind == NQS_ANSATZ_RBM || a->kind == NQS_ANSATZ_COMPLEX_RBM)) {
+    CKR(g_ws.fwd.ensure(sizeof(__bf16) * 2 * (size_t)nqs::theta_x3_split_elems(a->M, a->N)));
+    ModelView<float> vf;
+    CKR(model_view<float>(a, vf));
+    return nqs::launch_forward_t_x3(vf, a->d_par.as<double>(), d_spins, B,
+                                    g_ws.t_re.as<float>(), g_ws.t_im.as<float>(),
+                                    g_ws.fwd.as<__bf16>(), g_stream);
   }
   return nqs::launch_forward<R>(v, d_spins, B, g_ws.t_re.as<R>(), g_ws.t_im.as<R>(), la, ar,
                                 g_stream, q_re, q_im);
@@ -597,7 +611,7 @@ int eloc_device_T(const nqs_config_t* cfg, int Lx, int Ly, int per, const AmpSou
 
 int eloc_device(const nqs_config_t* cfg, int Lx, int Ly, int per, int prec, const AmpSource& src,
                 const int* d_spins, int B, double* d_re, double* d_im) {
-  if (prec == NQS_HIP_FP32)
+  if (low_prec(prec))
     return eloc_device_T<float>(cfg, Lx, Ly, per, src, d_spins, B, d_re, d_im);
   return eloc_device_T<double>(cfg, Lx, Ly, per, src, d_spins, B, d_re, d_im);
 }
@@ -1296,7 +1310,7 @@ int sampler_advance(nqs_sampler* s, int steps, int rounds, int n_out, int* d_out
     const bool learned_batch = s->ar && d_out;
     if (learned_batch) CKR(ar_maybe_train(s));
     if (learned_batch && s->trained) {
-      if (s->opt.precision == NQS_HIP_FP32) CKR(learned_rounds<float>(s, rounds, n_out, d_out));
+      if (low_prec(s->opt.precision)) CKR(learned_rounds<float>(s, rounds, n_out, d_out));
       else CKR(learned_rounds<double>(s, rounds, n_out, d_out));
     } else {
       nqs::MHLaunch ml;
@@ -1306,7 +1320,7 @@ int sampler_advance(nqs_sampler* s, int steps, int rounds, int n_out, int* d_out
       ml.record = d_out ? 1 : 0;
       ml.n_out = n_out;
       ml.out = d_out;
-      if (s->opt.precision == NQS_HIP_FP32) {
+      if (low_prec(s->opt.precision)) {
         ModelView<float> v;
         CKR(ratio_view<float>(s->src.a, v));
         CKR(nqs::launch_mh<float>(v, s->cs, ml, g_stream));
@@ -1743,6 +1757,7 @@ struct MinsrArgs {
   double* rhs = nullptr;      // [P] scratch (parameter space)
   double eps = 0.0;
   bool factored = false, dual = false;
+  bool mixed = false;         // dense Gram on the split-BF16 MFMA (FP32 O)
   double* delta = nullptr;    // [P] out (device)
   double* h_T = nullptr;      // optional [B][B] host copy of T (sample space)
   double* h_y = nullptr;      // optional [B] host copy of y (sample space)
@@ -1778,13 +1793,36 @@ int minsr_core(const MinsrArgs& m, int& converged) {
   if (m.dual) {
     // parameter-space MinSR: delta = S^-1 (O_c^T eps / Ns)
     CKR(g_ws.T.ensure(sizeof(double) * (size_t)P * P));
-    CKR(g_ws.T2.ensure(sizeof(double) * (size_t)P * B));
     T = g_ws.T.as<double>();
-    double* OT = g_ws.T2.as<double>();
     double* rhs = m.rhs;
-    CKR(nqs::launch_transpose(O, B, P, OT, g_stream));
-    CKR(nqs::launch_gemm_nt((int)P, (int)P, B, 1.0 / B, OT, B, OT, B, 0.0, T, P, 1, g_stream));
-    CKR(nqs::launch_add_diag(T, (int)P, P, eps, g_stream));
+    // S = O_c^T O_c / Ns (lower triangle): FP64 MFMA over the transposed O,
+    // or the split-BF16 MFMA over its FP32 transpose (K padded with zeros)
+    const long Kp = nqs::x3_pad(B);
+    double* OT = nullptr;
+    float* OT32 = nullptr;
+    if (m.mixed) {
+      CKR(g_ws.o32.ensure(sizeof(float) * (size_t)P * Kp));
+      CKR(g_ws.x3max.ensure(sizeof(float) * nqs::x3_max_elems(B, P, Kp, 1)));
+      OT32 = g_ws.o32.as<float>();
+      CKR(nqs::launch_to_f32_t(O, B, P, OT32, Kp, g_ws.x3max.as<float>(), g_stream));
+    } else {
+      CKR(g_ws.T2.ensure(sizeof(double) * (size_t)P * B));
+      OT = g_ws.T2.as<double>();
+      CKR(nqs::launch_transpose(O, B, P, OT, g_stream));
+    }
+    if (m.mixed)
+      CKR(g_ws.x3part.ensure(sizeof(float) * nqs::x3_part_elems((int)P, (int)P, Kp) + 16));
+    auto gram_param = [&](double* out) -> int {
+      if (m.mixed)
+        CKR(nqs::launch_gemm_x3((int)P, (int)P, (int)Kp, 1.0 / B, OT32, Kp, OT32, Kp,
+                                g_ws.x3max.as<float>(), g_ws.x3max.as<float>(), out, P, 1,
+                                g_ws.x3part.as<float>(), g_stream));
+      else
+        CKR(nqs::launch_gemm_nt((int)P, (int)P, B, 1.0 / B, OT, B, OT, B, 0.0, out, P, 1,
+                                g_stream));
+      return nqs::launch_add_diag(out, (int)P, P, eps, g_stream);
+    };
+    CKR(gram_param(T));
     CKR(nqs::launch_colsum(O, B, P, u, 1.0 / B, rhs, work, g_stream));
     CKR(g_ws.linv.ensure(sizeof(double) * nqs::potrf_linv_elems((int)P)));
     CKR(g_ws.y.ensure(sizeof(double) * 2 * (size_t)P));
@@ -1797,8 +1835,7 @@ int minsr_core(const MinsrArgs& m, int& converged) {
       // second evaluation of S)
       CKR(g_ws.Oi.ensure(sizeof(double) * (size_t)P * P));
       double* S2 = g_ws.Oi.as<double>();
-      CKR(nqs::launch_gemm_nt((int)P, (int)P, B, 1.0 / B, OT, B, OT, B, 0.0, S2, P, 1, g_stream));
-      CKR(nqs::launch_add_diag(S2, (int)P, P, eps, g_stream));
+      CKR(gram_param(S2));
       CKR(nqs::launch_potrf_repair(T, (int)P, f - 1, S2, g_stream));
       CKR(nqs::launch_add_diag(T, (int)P, P, 1e-6, g_stream));
       f = nqs::launch_potrf(T, (int)P, Linv, g_ws.fail.as<int>(), g_stream);
@@ -1816,11 +1853,30 @@ int minsr_core(const MinsrArgs& m, int& converged) {
   }
   CKR(g_ws.T.ensure(sizeof(double) * (size_t)B * B));
   T = g_ws.T.as<double>();
+  // T = O_c O_c^T / Ns + eps I over the materialised O: FP64 MFMA, or the
+  // split-BF16 MFMA over its FP32 copy (columns padded with zeros)
+  const long Pp = nqs::x3_pad(P);
+  if (!m.factored && m.mixed) {
+    CKR(g_ws.o32.ensure(sizeof(float) * (size_t)B * Pp));
+    CKR(g_ws.x3part.ensure(sizeof(float) * nqs::x3_part_elems(B, B, Pp) + 16));
+    CKR(g_ws.x3max.ensure(sizeof(float) * nqs::x3_max_elems(B, P, Pp, 0)));
+    CKR(nqs::launch_to_f32_pad(O, B, P, g_ws.o32.as<float>(), Pp, g_ws.x3max.as<float>(),
+                               g_stream));
+  }
+  auto gram_dense = [&](double* out) -> int {
+    if (m.mixed)
+      CKR(nqs::launch_gemm_x3(B, B, (int)Pp, 1.0 / B, g_ws.o32.as<float>(), Pp,
+                              g_ws.o32.as<float>(), Pp, g_ws.x3max.as<float>(),
+                              g_ws.x3max.as<float>(), out, B, 1, g_ws.x3part.as<float>(),
+                              g_stream));
+    else
+      CKR(nqs::launch_gemm_nt(B, B, (int)P, 1.0 / B, O, P, O, P, 0.0, out, B, 1, g_stream));
+    return nqs::launch_add_diag(out, B, B, eps, g_stream);
+  };
   if (m.factored) {
     CKR(gram_factored(T));
   } else {
-    CKR(nqs::launch_gemm_nt(B, B, (int)P, 1.0 / B, O, P, O, P, 0.0, T, B, 1, g_stream));
-    CKR(nqs::launch_add_diag(T, B, B, eps, g_stream));
+    CKR(gram_dense(T));
   }
   if (m.h_T) {
     CK(hipMemcpyAsync(m.h_T, T, sizeof(double) * (size_t)B * B, hipMemcpyDeviceToHost, g_stream));
@@ -1839,8 +1895,7 @@ int minsr_core(const MinsrArgs& m, int& converged) {
     if (m.factored) {
       CKR(gram_factored(T2));
     } else {
-      CKR(nqs::launch_gemm_nt(B, B, (int)P, 1.0 / B, O, P, O, P, 0.0, T2, B, 1, g_stream));
-      CKR(nqs::launch_add_diag(T2, B, B, eps, g_stream));
+      CKR(gram_dense(T2));
     }
     CKR(nqs::launch_potrf_repair(T, B, f - 1, T2, g_stream));
     CKR(nqs::launch_add_diag(T, B, B, 1e-6, g_stream));
@@ -2012,7 +2067,9 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
     const double b = (double)B, p = (double)P;
     const double gram_b = factored ? b * b * (N + 2.0 * a->M) : b * b * p;
     const double primal = gram_b + b * b * b / 3.0;
-    const double dualc = b * p * p + p * p * p / 3.0;
+    // the parameter-space Gram of the mixed mode runs on the split-BF16
+    // MFMA at about an eighth of the FP64 MFMA's time per product
+    const double dualc = (prec == NQS_HIP_MIXED ? 0.125 : 1.0) * b * p * p + p * p * p / 3.0;
     dual = dualc < primal;
   }
   const bool ofree = factored && !dual;
@@ -2043,8 +2100,8 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
     if (holo) CK(hipMemcpyAsync(Oi, hi.data(), ob, hipMemcpyHostToDevice, g_stream));
     CK(hipStreamSynchronize(g_stream));
   } else {
-    if (prec == NQS_HIP_FP32) {
-      CKR(forward_t<float>(a, d_samples, B, nullptr, nullptr));
+    if (low_prec(prec)) {
+      CKR(forward_t<float>(a, d_samples, B, nullptr, nullptr, false, prec == NQS_HIP_MIXED));
       ModelView<float> v;
       CKR(model_view<float>(a, v));
       CKR(nqs::launch_gen_O<float>(v, a->kind, d_samples, B, g_ws.t_re.as<float>(),
@@ -2120,6 +2177,7 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
     m.eps = eps;
     m.factored = factored;
     m.dual = dual;
+    m.mixed = prec == NQS_HIP_MIXED;
     m.delta = delta;
     CKR(minsr_core(m, converged));
   } else {
@@ -3916,21 +3974,22 @@ NQS_EXPORT int nqs_lanczos_e0_kagome_heisenberg_projected_lean_eigvec(
   return lean_lanczos(H, P, N, max_iters, tol, out_eigenvalue, out_eigenvector, out_result);
 }
 
-NQS_EXPORT int nqs_hip_minsr_solve(nqs_ansatz_t* a, const int* spins, int B, const double* e_loc,
-                                   double shift, int gram, double* out_delta, double* out_y,
-                                   double* out_T) {
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
+namespace {
+
+int minsr_solve_impl(nqs_ansatz_t* a, const int* spins, int B, const double* e_loc, double shift,
+                     int gram, int prec, double* out_delta, double* out_y, double* out_T,
+                     const char* who) {
   if (!a || !spins || !e_loc || !out_delta || B <= 0) return -1;
   const bool rbm_like = a->kind == NQS_ANSATZ_RBM || a->kind == NQS_ANSATZ_COMPLEX_RBM;
   if (gram != NQS_HIP_GRAM_DIRECT && gram != NQS_HIP_GRAM_FACTORED &&
       gram != NQS_HIP_GRAM_PARAMETER) {
-    fprintf(stderr, "nqs_hip_minsr_solve: gram must be DIRECT, FACTORED or PARAMETER\n");
+    fprintf(stderr, "%s: gram must be DIRECT, FACTORED or PARAMETER\n", who);
     return -1;
   }
   if ((gram == NQS_HIP_GRAM_FACTORED && !rbm_like) ||
       (gram == NQS_HIP_GRAM_PARAMETER && (out_y || out_T))) {
-    fprintf(stderr, "nqs_hip_minsr_solve: unsupported combination (factored Gram needs an RBM; "
-                    "y and T exist in sample space only)\n");
+    fprintf(stderr, "%s: unsupported combination (factored Gram needs an RBM; "
+                    "y and T exist in sample space only)\n", who);
     return -1;
   }
   if (stream_ready()) return -1;
@@ -3950,11 +4009,19 @@ NQS_EXPORT int nqs_hip_minsr_solve(nqs_ansatz_t* a, const int* spins, int B, con
   } else {
     CKR(g_ws.O.ensure(sizeof(double) * (size_t)B * (size_t)P));
     O = g_ws.O.as<double>();
-    CKR(forward_t<double>(a, d_samples, B, nullptr, nullptr));
-    ModelView<double> v;
-    CKR(model_view<double>(a, v));
-    CKR(nqs::launch_gen_O<double>(v, a->kind, d_samples, B, g_ws.t_re.as<double>(),
-                                  g_ws.t_im.as<double>(), 0, O, nullptr, P, g_stream));
+    if (low_prec(prec)) {
+      CKR(forward_t<float>(a, d_samples, B, nullptr, nullptr, false, prec == NQS_HIP_MIXED));
+      ModelView<float> v;
+      CKR(model_view<float>(a, v));
+      CKR(nqs::launch_gen_O<float>(v, a->kind, d_samples, B, g_ws.t_re.as<float>(),
+                                   g_ws.t_im.as<float>(), 0, O, nullptr, P, g_stream));
+    } else {
+      CKR(forward_t<double>(a, d_samples, B, nullptr, nullptr));
+      ModelView<double> v;
+      CKR(model_view<double>(a, v));
+      CKR(nqs::launch_gen_O<double>(v, a->kind, d_samples, B, g_ws.t_re.as<double>(),
+                                    g_ws.t_im.as<double>(), 0, O, nullptr, P, g_stream));
+    }
   }
   CKR(g_ws.vecs.ensure(sizeof(double) * 9 * (size_t)P));
   if (factored) {
@@ -3984,6 +4051,7 @@ NQS_EXPORT int nqs_hip_minsr_solve(nqs_ansatz_t* a, const int* spins, int B, con
   m.eps = shift;
   m.factored = factored;
   m.dual = dual;
+  m.mixed = prec == NQS_HIP_MIXED;
   m.delta = vb;
   m.h_T = out_T;
   m.h_y = out_y;
@@ -3992,6 +4060,30 @@ NQS_EXPORT int nqs_hip_minsr_solve(nqs_ansatz_t* a, const int* spins, int B, con
   CK(hipMemcpyAsync(out_delta, vb, sizeof(double) * P, hipMemcpyDeviceToHost, g_stream));
   CK(hipStreamSynchronize(g_stream));
   return converged ? 1 : 0;
+}
+
+}  // namespace
+
+NQS_EXPORT int nqs_hip_minsr_solve(nqs_ansatz_t* a, const int* spins, int B, const double* e_loc,
+                                   double shift, int gram, double* out_delta, double* out_y,
+                                   double* out_T) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  return minsr_solve_impl(a, spins, B, e_loc, shift, gram, NQS_HIP_FP64, out_delta, out_y, out_T,
+                          "nqs_hip_minsr_solve");
+}
+
+NQS_EXPORT int nqs_hip_minsr_solve_ex(nqs_ansatz_t* a, const int* spins, int B,
+                                      const double* e_loc, double shift,
+                                      const nqs_hip_options_t* opt, double* out_delta,
+                                      double* out_y, double* out_T) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!opt) return -1;
+  if (opt->precision < NQS_HIP_FP64 || opt->precision > NQS_HIP_MIXED) {
+    fprintf(stderr, "nqs_hip_minsr_solve_ex: unknown precision %d\n", opt->precision);
+    return -1;
+  }
+  return minsr_solve_impl(a, spins, B, e_loc, shift, opt->gram, opt->precision, out_delta, out_y,
+                          out_T, "nqs_hip_minsr_solve_ex");
 }
 
 // Streaming copy of `bytes` bytes between two device buffers, `reps` times;

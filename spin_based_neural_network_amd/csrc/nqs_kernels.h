@@ -272,6 +272,35 @@ int launch_orbit_grad(const double* lp, const double* ar, const double* chi, int
                       int cplx, const double* O_re, const double* O_im, long P, double* w_re,
                       double* w_im, double* out_re, double* out_im, hipStream_t st);
 
+// ---- mixed precision: split 16-bit MFMA products (k_mixed.hip) -------------
+// C = alpha A B^T with A [M][K], B [N][K] FP32 (K a multiple of 32, rows
+// 16-byte aligned), C FP64: entries scaled by a power of two from the
+// largest |entry| (device amax / bmax) and split into hi + lo half
+// precision, each product as three FP16 MFMA terms, FP32 accumulation;
+// lower = 1: entries with column <= row of a square product only.  K sums
+// longer than 2048 run as splits of at most 2048 columns into part
+// (x3_part_elems(M, N, K) floats), added in FP64.
+long x3_pad(long k);   // K padded to what the split product reads (zeros)
+size_t x3_part_elems(int M, int N, long K);
+int launch_gemm_x3(int M, int N, int K, double alpha, const float* A, long lda, const float* B,
+                   long ldb, const float* amax, const float* bmax, double* C, long ldc, int lower,
+                   float* part, hipStream_t st);
+// FP32 staging of a Gram operand with amax[0] = largest |entry|; amax holds
+// x3_max_elems(...) floats (block maxima after the first)
+long x3_max_elems(long R, long C, long ld, int transpose);
+// out [C][ldk] = FP32 transpose of in [R][C], zero for rows R..ldk-1
+int launch_to_f32_t(const double* in, long R, long C, float* out, long ldk, float* amax,
+                    hipStream_t st);
+// out [R][ldc] = FP32 copy of in [R][C], zero for columns C..ldc-1
+int launch_to_f32_pad(const double* in, long R, long C, float* out, long ldc, float* amax,
+                      hipStream_t st);
+// t = tanh(S W^T + b) in FP32 for B configurations (RBM / complex RBM), theta
+// on the BF16 MFMA with W split three ways; wsplit: theta_x3_split_elems(M, N)
+// bf16 per weight block (twice that for a complex RBM)
+int theta_x3_split_elems(int M, int N);
+int launch_forward_t_x3(const ModelView<float>& m, const double* d_par, const int* spins, int B,
+                        float* t_re, float* t_im, __bf16* wsplit, hipStream_t st);
+
 // ---- sample diagnostics (k_diag.hip) --------------------------------------
 // out[0] = sum_k (<|O_k|^2> - |<O_k>|^2) over rows O_re [B][P] (+ i O_im,
 // may be null); work: chi_work_elems(P) doubles
