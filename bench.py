@@ -1,7 +1,7 @@
 """Benchmark of the NQS VMC hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5]
-                    [--budget K] [--precision fp64|fp32|mixed] [--gram direct|factored|auto]
+                    [--budget K] [--precision mixed|fp64|fp32] [--gram direct|factored|auto]
                     [--move flip|exchange|learned] [--no-cpu] [--no-c3]
 
 A step is one full SR iteration (Metropolis sampling of every chain, local
@@ -754,7 +754,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "fp32", "mixed"])
     ap.add_argument("--gram", default="auto", choices=["direct", "factored", "auto"])
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the HBM copy probe on MIB mebibytes (counter calibration)")
@@ -814,7 +814,7 @@ def line(args, w, gpu, cpu):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32+f64", "mixed": "bf16x3+f32+f64"}[args.precision],
+        "dtype": {"fp64": "f64", "fp32": "f32+f64", "mixed": "f32+bf16+f16x3+f64"}[args.precision],
         "data": "synthetic (random-init parameters N(0,0.01), chains started at random Sz=0 states)",
         "config": {"workload": args.workload + ": " + w["desc"], "N": gpu["N"], "P": gpu["P"],
                    "samples_per_iter": w["chains"] * w["per_chain"], "chains": w["chains"],

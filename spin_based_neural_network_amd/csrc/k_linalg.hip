@@ -512,19 +512,13 @@ __device__ __attribute__((always_inline)) inline void tile_kloop(
 }
 
 template <int USE>
-__global__ __launch_bounds__(512) void k_gemm_big(int M, int N, int K, double alpha,
-                                                  const double* A, long lda,
-                                                  const double* __restrict__ Bm, long ldb,
-                                                  double beta, double* C, long ldc, int lower,
-                                                  int mask, int tiles_n, const int* skip) {
-  // 8 waves, 2 per SIMD: wave (wm, wn) computes rows wm*64 + [0, 64) and
-  // columns wn*32 + [0, 32) of the tile as 4x2 MFMA blocks (tile_kloop).
-  const int skipped = skip ? *skip : 0;
-  __shared__ double As[2][BT][32 + kLdsPad];
-  __shared__ double Bs[2][BT][32 + kLdsPad];
+__device__ __attribute__((always_inline)) inline void gemm_big_tile(
+    int M, int N, int K, double alpha, const double* A, long lda, const double* __restrict__ Bm,
+    long ldb, double beta, double* C, long ldc, int lower, int mask, int tiles_n, int skipped,
+    int t, double (&As)[2][BT][32 + kLdsPad], double (&Bs)[2][BT][32 + kLdsPad]) {
   int tm, tn;
-  if (lower) lower_tile(blockIdx.x, tm, tn);
-  else { tm = blockIdx.x / tiles_n; tn = blockIdx.x % tiles_n; }
+  if (lower) lower_tile(t, tm, tn);
+  else { tm = t / tiles_n; tn = t % tiles_n; }
   const int row0 = tm * BT, col0 = tn * BT;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wr = (wave >> 2) * 64, wc = (wave & 3) * 32;
@@ -567,20 +561,42 @@ __global__ __launch_bounds__(512) void k_gemm_big(int M, int N, int K, double al
       }
 }
 
+template <int USE>
+__global__ __launch_bounds__(512) void k_gemm_big(int M, int N, int K, double alpha,
+                                                  const double* A, long lda,
+                                                  const double* __restrict__ Bm, long ldb,
+                                                  double beta, double* C, long ldc, int lower,
+                                                  int mask, int tiles_n, const int* skip,
+                                                  int ntiles) {
+  // 8 waves, 2 per SIMD: wave (wm, wn) computes rows wm*64 + [0, 64) and
+  // columns wn*32 + [0, 32) of the tile as 4x2 MFMA blocks (tile_kloop).
+  // A grid smaller than the tile count walks the tiles (capped launches
+  // leave CUs to a concurrent panel chain).
+  const int skipped = skip ? *skip : 0;
+  __shared__ double As[2][BT][32 + kLdsPad];
+  __shared__ double Bs[2][BT][32 + kLdsPad];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
+    gemm_big_tile<USE>(M, N, K, alpha, A, lda, Bm, ldb, beta, C, ldc, lower, mask, tiles_n,
+                       skipped, t, As, Bs);
+}
+
+// max_wg > 0 caps the grid (the workgroups walk the tiles)
 static int gemm_big_impl(int M, int N, int K, double alpha, const double* A, long lda,
                          const double* B, long ldb, double beta, double* C, long ldc, int lower,
-                         int mask, const int* skip, const char* tag, hipStream_t st) {
+                         int mask, const int* skip, const char* tag, hipStream_t st,
+                         int max_wg = 0) {
   if (M <= 0 || N <= 0) return 0;
   const int tm = (M + BT - 1) / BT, tn = (N + BT - 1) / BT;
   if (lower && M != N) return -1;
   const long tiles = lower ? (long)tm * (tm + 1) / 2 : (long)tm * tn;
+  const long grid = (max_wg > 0 && tiles > max_wg) ? max_wg : tiles;
   KScope ks(tag, st);
   if (skip)
-    hipLaunchKernelGGL(k_gemm_big<1>, dim3((unsigned)tiles), dim3(512), 0, st, M, N, K, alpha, A,
-                       lda, B, ldb, beta, C, ldc, lower, mask, tn, skip);
+    hipLaunchKernelGGL(k_gemm_big<1>, dim3((unsigned)grid), dim3(512), 0, st, M, N, K, alpha, A,
+                       lda, B, ldb, beta, C, ldc, lower, mask, tn, skip, (int)tiles);
   else
-    hipLaunchKernelGGL(k_gemm_big<0>, dim3((unsigned)tiles), dim3(512), 0, st, M, N, K, alpha, A,
-                       lda, B, ldb, beta, C, ldc, lower, mask, tn, skip);
+    hipLaunchKernelGGL(k_gemm_big<0>, dim3((unsigned)grid), dim3(512), 0, st, M, N, K, alpha, A,
+                       lda, B, ldb, beta, C, ldc, lower, mask, tn, skip, (int)tiles);
   count_launch();
   NQS_HIP_CHECK(hipGetLastError());
   return 0;
@@ -1383,6 +1399,15 @@ static int wide_panel(double* A, int n, int K0, int width, double* Linv, int* fa
   return 0;
 }
 
+// workgroups of a capped rest-of-trailing-matrix update (one per CU; the
+// panel chain runs on the others): NQS_HIP_POTRF_REST_WG, default 192 of
+// the 256 CUs, 0 = no cap.  Read at each factorisation.
+static int rest_cap() {
+  const char* e = getenv("NQS_HIP_POTRF_REST_WG");
+  const int v = e ? atoi(e) : 192;
+  return v < 0 ? 0 : v;
+}
+
 // read at each wide panel (as NQS_HIP_PANEL_MERGE), so that a test can
 // compare the schedules in one process
 static int panel_rows() {
@@ -1490,8 +1515,11 @@ int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) 
     ev_wp = ev_wp1;
     if (R1 < n) {
       const double* L1 = A + (long)R1 * n + K0;
+      // with lookahead the rest update leaves rest_cap() workgroup slots'
+      // worth of CUs free, so that the next wide panel's launches on the side
+      // stream find CUs instead of queueing behind the update's tiles
       if (gemm_big_impl(n - R1, n - R1, w, -1.0, L1, n, L1, n, 1.0, A + (long)R1 * n + R1, n, 1, 1,
-                        fail_dev, "gemm_potrf", st))
+                        fail_dev, "gemm_potrf", st, lookahead ? rest_cap() : 0))
         return -1;
     }
     NQS_HIP_CHECK(hipEventRecord(ev_rest, st));
