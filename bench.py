@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5]
                     [--budget K] [--precision mixed|fp64|fp32] [--gram direct|factored|auto]
-                    [--move flip|exchange|learned] [--no-cpu] [--no-c3]
+                    [--move flip|exchange|learned] [--no-cpu] [--c3]
 
 A step is one full SR iteration (Metropolis sampling of every chain, local
 energies, O_k, MinSR / CG-SR solve, parameter update) through the library's
@@ -14,9 +14,10 @@ with plain exchange Metropolis and with the learned autoregressive proposal
 Each leg first optimises its own wavefunction for the workload's budget of
 iterations (C5: 300 holomorphic SR iterations, untimed) and reports the
 energy error against exact diagonalisation (E0 = -10.759897248084), then
-times K MinSR iterations at that state.  The line also carries the J1-J2
-10x10 SR-iteration time against the reference CPU (configuration 3).
-Prints one JSON line.
+times K MinSR iterations at that state.  With --c3 the line also carries the
+J1-J2 10x10 SR-iteration time against the reference CPU (configuration 3);
+`--workload c3` benches that configuration on its own.  Prints one JSON
+line.
 
 value = effective samples per second of the sampling phase (the learned
 leg for C5): sum over chains of n_c / (2 tau_int) with tau_int from the
@@ -105,8 +106,8 @@ def setup(lib, w, args, move=None):
                               j_coupling=1.0, j2_coupling=0.5, transverse_field=1.0, kagome_pbc=1,
                               num_samples=w["chains"] * w["per_chain"], num_thermalize=4 * N,
                               num_decorrelate=N, rng_seed=20260,
-                              learning_rate=w["lr"], sr_diag_shift=w["shift"],
-                              sr_cg_max_iters=100, sr_cg_tol=1e-6)
+                              learning_rate=args.lr if getattr(args, "lr", None) else w["lr"],
+                              sr_diag_shift=w["shift"], sr_cg_max_iters=100, sr_cg_tol=1e-6)
     a = lib.nqs_ansatz_create(ctypes.byref(cfg), N)
     if not a:
         raise RuntimeError("nqs_ansatz_create failed")
@@ -759,9 +760,11 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the HBM copy probe on MIB mebibytes (counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-c3", action="store_true",
-                    help="C5: skip the J1-J2 10x10 SR-iteration comparison with the reference CPU")
+    ap.add_argument("--c3", action="store_true",
+                    help="C5: add the J1-J2 10x10 SR-iteration comparison with the reference CPU")
     ap.add_argument("--move", default=None, choices=["flip", "exchange", "learned"])
+    ap.add_argument("--lr", type=float, default=None,
+                    help="learning rate of the timed SR iterations (default: the workload's)")
     ap.add_argument("--budget", type=int, default=None, metavar="K",
                     help="optimisation iterations before timing (default: the workload's, C5 300)")
     ap.add_argument("--reference-records", default=None, metavar="DIR",
@@ -884,7 +887,7 @@ def main_c5(args, w, budget):
         ess_per_s=g["ess_per_s"] / ex["ess_per_s"],
         ess_per_s_iteration=g["ess_per_s_iteration"] / ex["ess_per_s_iteration"],
         sr_iter_ms=g["ms_per_step"] / ex["ms_per_step"])
-    if not args.no_c3:
+    if args.c3:
         out["c3_vs_reference_cpu"] = c3_comparison(args)
     print(json.dumps(out))
 

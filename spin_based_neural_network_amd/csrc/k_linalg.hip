@@ -1400,11 +1400,13 @@ static int wide_panel(double* A, int n, int K0, int width, double* Linv, int* fa
 }
 
 // workgroups of a capped rest-of-trailing-matrix update (one per CU; the
-// panel chain runs on the others): NQS_HIP_POTRF_REST_WG, default 192 of
-// the 256 CUs, 0 = no cap.  Read at each factorisation.
+// panel chain runs on the others): NQS_HIP_POTRF_REST_WG, default 0 = no cap
+// (a cap of 160-192 of the 256 CUs measured within noise at n = 4848 and
+// 8192 and slower where the rest update is the critical path).  Read at
+// each factorisation.
 static int rest_cap() {
   const char* e = getenv("NQS_HIP_POTRF_REST_WG");
-  const int v = e ? atoi(e) : 192;
+  const int v = e ? atoi(e) : 0;
   return v < 0 ? 0 : v;
 }
 
