@@ -194,7 +194,8 @@ def optimise(lib, w, st, budget):
 
 
 # kernel symbol (as rocprofv3 names it) behind each timing tag
-SYMBOL = {"gemm_potrf": "nqs::k_gemm_big<1>", "gemm": "nqs::k_gemm_big<0>",
+SYMBOL = {"gemm_potrf": "nqs::k_gemm_big<1>", "gemm_chain": "nqs::k_gemm_lat",
+          "gemm": "nqs::k_gemm_big<0>",
           "potrf_panel": "nqs::k_panel",
           "eloc": "nqs::k_eloc_terms<double, true>", "gen_O": "nqs::k_gen_O<double>",
           "colsum": "nqs::k_colsum_partial", "mh": "nqs::k_mh<double, true, 64, 4>",
@@ -214,7 +215,7 @@ def traffic_bytes(workload, tag):
 
 
 TAGS = ["mh", "ar_mh", "forward", "eloc", "gen_O", "colsum", "gemm", "gemm_small", "gram_x3",
-        "gemm_potrf", "potrf_panel",
+        "gemm_potrf", "gemm_chain", "potrf_panel",
         "trsv", "trsv_inv", "gram_f", "multi_prep", "multi_gemm_z", "multi_rows", "multi_gemm_g", "multi_vec"]
 
 
@@ -233,12 +234,15 @@ def potrf_widths():
     return rules or [(512, 0)]
 
 
-def potrf_gemm_flops(B):
+def potrf_gemm_flops(B, part="rest"):
     """FLOP of the blocked Cholesky's GEMM updates of a B x B matrix (lower
     elements only): wide panels made of 128-column outer panels; inside a
     wide panel each outer panel's rank-128 update reaches the wide panel's
     later columns, then the next wide panel's columns and the rest of the
-    trailing triangle take one update of the wide panel's rank."""
+    trailing triangle take one update of the wide panel's rank.  part
+    "rest": the rest-of-trailing-matrix updates (`gemm_potrf`); "chain": the
+    updates inside the wide panels and of the next wide panel's columns
+    (`gemm_chain`)."""
     rules = potrf_widths()
 
     def width(rem):
@@ -257,14 +261,15 @@ def potrf_gemm_flops(B):
 
     k0 = 0
     wd = width(B)
-    tot = inside(0, wd)
+    chain, rest = inside(0, wd), 0
     while k0 + wd < B:
         r0 = k0 + wd
         w1 = width(B - r0)
         m = B - r0 - w1
-        tot += 2 * wd * lower_cols(r0, w1) + inside(r0, w1) + 2 * wd * (m * (m + 1) // 2)
+        chain += 2 * wd * lower_cols(r0, w1) + inside(r0, w1)
+        rest += 2 * wd * (m * (m + 1) // 2)
         k0, wd = r0, w1
-    return tot
+    return chain if part == "chain" else rest
 
 
 def kernel_work(tag, w, st, launches, iters):
@@ -294,7 +299,9 @@ def kernel_work(tag, w, st, launches, iters):
     # the Cholesky factors the P x P matrix in the parameter-space solve
     n_chol = P if st["dual"] else B
     if tag == "gemm_potrf":
-        return "mfma", potrf_gemm_flops(n_chol) / per_iter, "FLOP"
+        return "mfma", potrf_gemm_flops(n_chol, "rest") / per_iter, "FLOP"
+    if tag == "gemm_chain":
+        return "mfma", potrf_gemm_flops(n_chol, "chain") / per_iter, "FLOP"
     if tag == "potrf_panel":
         # per inner panel: factor + inverse of the 64x64 block, and the rows
         # below solved by a 64-wide product (and the preceding panel's

@@ -611,6 +611,162 @@ int launch_gemm_nt(int M, int N, int K, double alpha, const double* A, long lda,
 }
 
 // --------------------------------------------------------------------------
+// Latency-bound NT GEMM for the Cholesky's panel-chain updates (the next
+// wide panel's columns, and the rank-128 updates inside a wide panel): few
+// output columns (128 .. 512) and K = 128 .. 512, so the launch holds few
+// 128 x 128 tiles and its time is one tile's MFMA time (an FP64 MFMA pipe
+// does 16 multiply-adds per cycle: a 128 x 128 x 512 tile is 55 us on one
+// CU).  Tiles of 64 x 64 (4 waves of 32 x 32, 2 x 2 MFMA blocks) cut that
+// to a quarter and spread the update over four times as many CUs; 70 KB of
+// LDS leaves room for two workgroups per CU.  K in chunks of 32, double
+// buffered: the next chunk's global loads are issued at the top of a chunk
+// and stored to the other LDS buffer after its last MFMA step.  Each output
+// accumulates its K steps of 4 in the same order as k_gemm_big, and the
+// Cholesky's alpha = -1, beta = 1 make the epilogue exact up to the one
+// rounding of C - acc, so the two kernels give the same bits.  mask: only
+// elements with gc <= gr are written (tiles wholly above the diagonal exit
+// at once); skip: nothing is written when the device flag is non-zero.
+// --------------------------------------------------------------------------
+constexpr int LT = 64, LBK = 32;
+
+__global__ __launch_bounds__(256, 2) void k_gemm_lat(int M, int N, int K, double alpha,
+                                                     const double* A, long lda,
+                                                     const double* __restrict__ Bm, long ldb,
+                                                     double beta, double* C, long ldc, int mask,
+                                                     int tiles_n, const int* skip) {
+  constexpr int NQ = LT * LBK / 256, STEPS = LBK / 4;
+  __shared__ double As[2][LT][LBK + kLdsPad];
+  __shared__ double Bs[2][LT][LBK + kLdsPad];
+  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
+  const int row0 = tm * LT, col0 = tn * LT;
+  if (mask && col0 > row0 + LT - 1) return;
+  if (skip && *skip) return;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  double cpre[2][2][4];
+  if (beta != 0.0) {
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int gr = row0 + wr + 16 * i + lk + 4 * r;
+          const int gc = col0 + wc + 16 * j + lr;
+          cpre[i][j][r] = (gr < M && gc < N && (!mask || gc <= gr)) ? C[(long)gr * ldc + gc] : 0.0;
+        }
+  }
+  v4d acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+  // loader: thread tid moves rows tid / 32 + 8 q, column tid % 32 of a chunk
+  const int kk = tid & (LBK - 1), lrow = tid / LBK;
+  const double* pa[NQ];
+  const double* pb[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    pa[q] = A + (long)min(row0 + lrow + 8 * q, M - 1) * lda;
+    pb[q] = Bm + (long)min(col0 + lrow + 8 * q, N - 1) * ldb;
+  }
+  double va[NQ], vb[NQ];
+  auto gload = [&](int k0) {
+    const int gk = min(k0 + kk, K - 1);
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      va[q] = pa[q][gk];
+      vb[q] = pb[q][gk];
+    }
+  };
+  auto sstore = [&](int buf, int k0) {
+    const bool ok = k0 + kk < K;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      As[buf][lrow + 8 * q][kk] = ok ? va[q] : 0.0;
+      Bs[buf][lrow + 8 * q][kk] = ok ? vb[q] : 0.0;
+    }
+  };
+  const int nch = (K + LBK - 1) / LBK;
+  gload(0);
+  sstore(0, 0);
+  __syncthreads();
+  double a[2][2], b[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++) a[0][i] = As[0][wr + 16 * i + lr][lk];
+#pragma unroll
+  for (int j = 0; j < 2; j++) b[0][j] = Bs[0][wc + 16 * j + lr][lk];
+  for (int c = 0; c < nch; c++) {
+    const int buf = c & 1, k1 = (c + 1) * LBK;
+    const bool more = c + 1 < nch;
+    if (more) gload(k1);
+#pragma unroll
+    for (int s = 0; s < STEPS; s++) {
+      const int cur = s & 1;
+      if (s + 1 < STEPS) {
+#pragma unroll
+        for (int i = 0; i < 2; i++) a[cur ^ 1][i] = As[buf][wr + 16 * i + lr][4 * (s + 1) + lk];
+#pragma unroll
+        for (int j = 0; j < 2; j++) b[cur ^ 1][j] = Bs[buf][wc + 16 * j + lr][4 * (s + 1) + lk];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[cur][i], b[cur][j], acc[i][j], 0, 0, 0);
+      // the other buffer's last reads were before the previous barrier
+      if (s == STEPS - 1 && more) sstore(buf ^ 1, k1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+    if (more) {
+#pragma unroll
+      for (int i = 0; i < 2; i++) a[0][i] = As[buf ^ 1][wr + 16 * i + lr][lk];
+#pragma unroll
+      for (int j = 0; j < 2; j++) b[0][j] = Bs[buf ^ 1][wc + 16 * j + lr][lk];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gr = row0 + wr + 16 * i + lk + 4 * r;
+        const int gc = col0 + wc + 16 * j + lr;
+        if (gr < M && gc < N && (!mask || gc <= gr)) {
+          double* cp = C + (long)gr * ldc + gc;
+          *cp = (beta == 0.0) ? alpha * acc[i][j][r] : alpha * acc[i][j][r] + beta * cpre[i][j][r];
+        }
+      }
+}
+
+// NQS_HIP_POTRF_LAT=0: the panel-chain updates take the 128 x 128 tiles of
+// k_gemm_big instead (same bits).  Read at each launch.
+static int chain_lat() {
+  const char* e = getenv("NQS_HIP_POTRF_LAT");
+  return !(e && e[0] == '0');
+}
+
+// C (rows x cols, lower elements when mask) -= A B^T for a panel-chain
+// update; `skip` is the Cholesky's failure flag
+static int gemm_chain(int M, int N, int K, const double* A, long lda, double* C, long ldc,
+                      int mask, const int* skip, hipStream_t st) {
+  if (M <= 0 || N <= 0) return 0;
+  if (!chain_lat())
+    return gemm_big_impl(M, N, K, -1.0, A, lda, A, lda, 1.0, C, ldc, 0, mask, skip, "gemm_chain",
+                         st);
+  const int tm = (M + LT - 1) / LT, tn = (N + LT - 1) / LT;
+  KScope ks("gemm_chain", st);
+  hipLaunchKernelGGL(k_gemm_lat, dim3((unsigned)(tm * tn)), dim3(256), 0, st, M, N, K, -1.0, A,
+                     lda, A, lda, 1.0, C, ldc, mask, tn, skip);
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// --------------------------------------------------------------------------
 // Cholesky, right-looking.  Outer panels of 128 columns, each made of two
 // 64-column inner panels; one launch (k_panel2) factors both: per inner
 // panel the diagonal block is factored and inverted and every row below it
@@ -1391,8 +1547,7 @@ static int wide_panel(double* A, int n, int K0, int width, double* Linv, int* fa
     const int c0 = j + NBO;
     if (c0 < end) {
       const double* L = A + (long)c0 * n + j;
-      if (gemm_big_impl(n - c0, end - c0, NBO, -1.0, L, n, L, n, 1.0, A + (long)c0 * n + c0, n, 0,
-                        1, fail_dev, "gemm_potrf", st))
+      if (gemm_chain(n - c0, end - c0, NBO, L, n, A + (long)c0 * n + c0, n, 1, fail_dev, st))
         return -1;
     }
   }
@@ -1499,9 +1654,7 @@ int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) 
     if (k > 0) NQS_HIP_CHECK(hipStreamWaitEvent(side, g_ev[3 * k - 1], 0));   // rest of step k - 1
     const double* L = A + (long)R0 * n + K0;
     // next wide panel's columns [R0, R0 + w1), rows >= R0 (lower part)
-    if (gemm_big_impl(n - R0, w1, w, -1.0, L, n, L, n, 1.0, A + (long)R0 * n + R0, n, 0, 1,
-                      fail_dev, "gemm_potrf", side))
-      return -1;
+    if (gemm_chain(n - R0, w1, w, L, n, A + (long)R0 * n + R0, n, 1, fail_dev, side)) return -1;
     NQS_HIP_CHECK(hipEventRecord(ev_panel, side));
     // while the rest update is the critical path (the same test as the
     // early rest update) the panels run on fewer, fuller workgroups

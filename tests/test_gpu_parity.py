@@ -447,6 +447,33 @@ def test_merged_panels_bit_equal(hip, monkeypatch):
             np.testing.assert_allclose(A @ out["1"][2], b, atol=1e-8)
 
 
+def test_chain_updates_small_tiles_bit_equal(hip, monkeypatch):
+    """The panel chain's updates (inside a wide panel and of the next wide
+    panel's columns) on 64 x 64 tiles give the factor and the solution of the
+    128 x 128-tile GEMM bit for bit: same K order per element, and C - acc
+    rounded once.  Cases: ragged sizes, several wide panels (lookahead on),
+    and a failing factorisation."""
+    rng = np.random.default_rng(23)
+    cases = [(1030, None), (2100, None), (193, None), (1300, 1000)]
+    for B, rank in cases:
+        X = rng.standard_normal((B, rank if rank else B + 37))
+        A = np.ascontiguousarray(X @ X.T / X.shape[1] + (0.0 if rank else 0.05) * np.eye(B))
+        b = np.ascontiguousarray(rng.standard_normal(B))
+        out = {}
+        for lat in ("0", "1"):
+            monkeypatch.setenv("NQS_HIP_POTRF_LAT", lat)
+            Aw = A.copy()
+            x = np.zeros(B)
+            ok = hip.nqs_hip_cholesky_solve(ptr(Aw), B, ptr(b), ptr(x))
+            out[lat] = (ok, Aw, x)
+        print(f"B={B} rank={rank}: ok={out['1'][0]}", flush=True)
+        assert out["0"][0] == out["1"][0]
+        np.testing.assert_array_equal(out["0"][1], out["1"][1])
+        np.testing.assert_array_equal(out["0"][2], out["1"][2])
+        if not rank:
+            np.testing.assert_allclose(A @ out["1"][2], b, atol=1e-8)
+
+
 def test_panel_row_blocks_bit_equal(hip, monkeypatch):
     """Merged panels with several 64-row blocks per workgroup (fewer
     workgroups, second-panel rows through memory) give the factor and the
