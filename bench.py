@@ -196,7 +196,7 @@ def optimise(lib, w, st, budget):
 # kernel symbol (as rocprofv3 names it) behind each timing tag
 SYMBOL = {"gemm_potrf": "nqs::k_gemm_big<1>", "gemm_chain": "nqs::k_gemm_lat",
           "gemm": "nqs::k_gemm_big<0>",
-          "potrf_panel": "nqs::k_panel",
+          "potrf_panel": "nqs::k_panel2",
           "eloc": "nqs::k_eloc_terms<double, true>", "gen_O": "nqs::k_gen_O<double>",
           "colsum": "nqs::k_colsum_partial", "mh": "nqs::k_mh<double, true, 64, 4>",
           "trsv": "nqs::k_sbsv_fwd", "gram_x3": "nqs::k_gemm_x3"}
@@ -305,12 +305,13 @@ def kernel_work(tag, w, st, launches, iters):
     if tag == "potrf_panel":
         # per inner panel: factor + inverse of the 64x64 block, and the rows
         # below solved by a 64-wide product (and the preceding panel's
-        # rank-64 update for every second panel), averaged over the panels
+        # rank-64 update for every second panel); one launch factors the two
+        # inner panels of a 128-column outer panel
         B = n_chol
         npan = (B + 63) // 64
         tot = sum(2 * 64 ** 3 / 3 + 2 * (B - 64 * (k + 1)) * 64 * 64 * (2 if k % 2 else 1)
                   for k in range(npan))
-        return "mfma", tot / npan, "FLOP"
+        return "mfma", tot / per_iter, "FLOP"
     if tag in ("multi_gemm_g", "multi_gemm_z"):
         # one problem: one QGT product per CG iteration, 2 (M+1)(N+1) B FLOP
         return "mfma", 2.0 * (M + 1) * (N + 1) * B, "FLOP"

@@ -22,8 +22,12 @@
 // k_x3_reduce: the FP32 accumulation error stays that of a 2048-term sum.
 // Global FP32 loads of the next chunk are in flight while the current
 // chunk's MFMAs issue; the split into hi / lo happens once per element when
-// it is staged in LDS (double-buffered, rows padded to 40 halves so a
-// quarter-wave's 16-byte operand reads fall in distinct banks).  With
+// it is staged in LDS (double-buffered; rows of 32 halves, 64 bytes, whose
+// four 16-byte groups are permuted by an XOR with (row / 2) % 4: the eight
+// lanes of an operand read (rows r..r+7, one group) and the sixteen of a
+// staging store (two rows, eight 8-byte pieces each) then fall in distinct
+// banks, and the 64 KB of a workgroup's buffers leave room for two
+// workgroups per CU).  With
 // lower = 1 only tiles on or below the diagonal run and only entries with
 // column <= row are written.
 //
@@ -44,7 +48,10 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int XT = 128;       // output tile edge
 constexpr int XK = 32;        // K per chunk
-constexpr int XLD = XK + 8;   // LDS row stride (halves)
+constexpr int XLD = XK;       // LDS row stride (halves)
+
+// half offset of the 8-half group g of row r in the swizzled LDS rows
+__device__ inline int x3_grp(int r, int g) { return 8 * (g ^ ((r >> 1) & 3)); }
 
 __device__ inline void x3_lower_tile(int t, int& tm, int& tn) {
   int m = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
@@ -122,13 +129,14 @@ __global__ __launch_bounds__(256) void k_gemm_x3(int M, int N, int K, double alp
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int e = 256 * q + tid, r = e >> 3, c4 = 4 * (e & 7);
+      const int pc = x3_grp(r, c4 >> 3) + (c4 & 7);
       f16x4 hi, lo;
       x3_split(ra[q], sa, hi, lo);
-      *(f16x4*)&sAh[buf][r][c4] = hi;
-      *(f16x4*)&sAl[buf][r][c4] = lo;
+      *(f16x4*)&sAh[buf][r][pc] = hi;
+      *(f16x4*)&sAl[buf][r][pc] = lo;
       x3_split(rb[q], sb, hi, lo);
-      *(f16x4*)&sBh[buf][r][c4] = hi;
-      *(f16x4*)&sBl[buf][r][c4] = lo;
+      *(f16x4*)&sBh[buf][r][pc] = hi;
+      *(f16x4*)&sBl[buf][r][pc] = lo;
     }
   };
   f32x4 acc[4][4];
@@ -146,12 +154,14 @@ __global__ __launch_bounds__(256) void k_gemm_x3(int M, int N, int K, double alp
     const int buf = c & 1;
     if (c + 1 < nch) gload((c + 1) * XK);
     f16x8 ah[4], al[4], bh[4], bl[4];
+    // rows 16 i + l16 of the wave's blocks: (row / 2) % 4 = (l16 / 2) % 4
+    const int pg = x3_grp(l16, lg);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      ah[i] = *(const f16x8*)&sAh[buf][wr + 16 * i + l16][8 * lg];
-      al[i] = *(const f16x8*)&sAl[buf][wr + 16 * i + l16][8 * lg];
-      bh[i] = *(const f16x8*)&sBh[buf][wc + 16 * i + l16][8 * lg];
-      bl[i] = *(const f16x8*)&sBl[buf][wc + 16 * i + l16][8 * lg];
+      ah[i] = *(const f16x8*)&sAh[buf][wr + 16 * i + l16][pg];
+      al[i] = *(const f16x8*)&sAl[buf][wr + 16 * i + l16][pg];
+      bh[i] = *(const f16x8*)&sBh[buf][wc + 16 * i + l16][pg];
+      bl[i] = *(const f16x8*)&sBl[buf][wc + 16 * i + l16][pg];
     }
     // the small terms first, sixteen independent accumulators between
     // dependent MFMAs
