@@ -1758,10 +1758,22 @@ struct MinsrArgs {
   double eps = 0.0;
   bool factored = false, dual = false;
   bool mixed = false;         // dense Gram on the split-BF16 MFMA (FP32 O)
+  bool fdual = false;         // mixed parameter-space solve of an RBM without O:
+                              // O_c^T and the force from the spins and FP32 factors
   double* delta = nullptr;    // [P] out (device)
   double* h_T = nullptr;      // optional [B][B] host copy of T (sample space)
   double* h_y = nullptr;      // optional [B] host copy of y (sample space)
 };
+
+// NQS_HIP_FACTORED_DUAL=0: the mixed parameter-space MinSR of an RBM goes
+// through the materialised O (read at each step); so does an RBM with more
+// than 256 staged factor rows per sample
+bool factored_dual_on(const nqs_ansatz* a) {
+  const char* e = getenv("NQS_HIP_FACTORED_DUAL");
+  if (e && e[0] == '0') return false;
+  // the generator stages N + M (complex: N + 2M) factor rows per sample
+  return a->N + (a->kind == NQS_ANSATZ_COMPLEX_RBM ? 2 : 1) * a->M <= 256;
+}
 
 int minsr_core(const MinsrArgs& m, int& converged) {
   nqs_ansatz* a = m.a;
@@ -1777,7 +1789,7 @@ int minsr_core(const MinsrArgs& m, int& converged) {
   double* u = g_ws.u1.as<double>();
   // eps_i = E_i - <E>, O_c = O - <O>  (nqs_optimizer.c:1119-1126)
   CKR(nqs::launch_sub_scalar(u, m.E, B, sc + 0, g_stream));
-  if (!ofree) CKR(nqs::launch_center(O, B, P, m.Om, g_stream));
+  if (O) CKR(nqs::launch_center(O, B, P, m.Om, g_stream));
   CKR(g_ws.diag.ensure(sizeof(double) * B));
   CKR(g_ws.y.ensure(sizeof(double) * B * 2));
   CKR(g_ws.fail.ensure(sizeof(int) * 4));
@@ -1804,7 +1816,12 @@ int minsr_core(const MinsrArgs& m, int& converged) {
       CKR(g_ws.o32.ensure(sizeof(float) * (size_t)P * Kp));
       CKR(g_ws.x3max.ensure(sizeof(float) * nqs::x3_max_elems(B, P, Kp, 1)));
       OT32 = g_ws.o32.as<float>();
-      CKR(nqs::launch_to_f32_t(O, B, P, OT32, Kp, g_ws.x3max.as<float>(), g_stream));
+      if (m.fdual)
+        CKR(nqs::launch_oct_factored(m.d_samples, B, N, a->M, g_ws.t_re.as<float>(),
+                                     cpl ? g_ws.t_im.as<float>() : nullptr, m.Om, P, OT32, Kp,
+                                     g_ws.x3max.as<float>(), g_stream));
+      else
+        CKR(nqs::launch_to_f32_t(O, B, P, OT32, Kp, g_ws.x3max.as<float>(), g_stream));
     } else {
       CKR(g_ws.T2.ensure(sizeof(double) * (size_t)P * B));
       OT = g_ws.T2.as<double>();
@@ -1823,7 +1840,17 @@ int minsr_core(const MinsrArgs& m, int& converged) {
       return nqs::launch_add_diag(out, (int)P, P, eps, g_stream);
     };
     CKR(gram_param(T));
-    CKR(nqs::launch_colsum(O, B, P, u, 1.0 / B, rhs, work, g_stream));
+    if (m.fdual) {
+      // O_c^T eps / Ns = O^T (eps - <eps>) / Ns from the factors
+      double* uc = g_ws.u2.as<double>();
+      CKR(nqs::launch_sum(u, B, nullptr, 1.0 / B, sc + 7, g_stream));
+      CKR(nqs::launch_sub_scalar(uc, u, B, sc + 7, g_stream));
+      CKR(nqs::launch_factored_delta(m.d_samples, B, N, a->M, g_ws.t_re.as<float>(),
+                                     cpl ? g_ws.t_im.as<float>() : nullptr, uc, 1.0 / B, work,
+                                     rhs, g_stream));
+    } else {
+      CKR(nqs::launch_colsum(O, B, P, u, 1.0 / B, rhs, work, g_stream));
+    }
     CKR(g_ws.linv.ensure(sizeof(double) * nqs::potrf_linv_elems((int)P)));
     CKR(g_ws.y.ensure(sizeof(double) * 2 * (size_t)P));
     double* Linv = g_ws.linv.as<double>();
@@ -2073,17 +2100,24 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
     dual = dualc < primal;
   }
   const bool ofree = factored && !dual;
+  // the mixed parameter-space solve of an RBM forms O_c^T in FP32 straight
+  // from the spins and the FP32 factors, and the means and force from factor
+  // sums, so O itself is never written (NQS_HIP_FACTORED_DUAL=0: through O)
+  const bool fdual = method == SR_MINSR && dual && prec == NQS_HIP_MIXED && rbm_like && !holo &&
+                     factored_dual_on(a);
   // ---- 3. O matrix -----------------------------------------------------------
   const size_t ob = sizeof(double) * (size_t)B * (size_t)P;
-  if (!ofree) CKR(g_ws.O.ensure(ob));
+  if (!ofree && !fdual) CKR(g_ws.O.ensure(ob));
   if (holo) CKR(g_ws.Oi.ensure(ob));
-  double* O = ofree ? nullptr : g_ws.O.as<double>();
+  double* O = (ofree || fdual) ? nullptr : g_ws.O.as<double>();
   double* Oi = holo ? g_ws.Oi.as<double>() : nullptr;
   if (ofree) {
     // t in FP64 for the factored Gram and update (the FP64 local energies of
     // this ansatz have just left it in the workspace)
     if (!(e_dev && esrc.a == a && prec == NQS_HIP_FP64))
       CKR(forward_t<double>(a, d_samples, B, nullptr, nullptr));
+  } else if (fdual) {
+    CKR(forward_t<float>(a, d_samples, B, nullptr, nullptr, false, true));
   } else if (OrbitSource og; orbit_gradient_cb(holo, gfn, hfn, guser, og)) {
     // gradients of a symmetry-projected amplitude: orbit rows of every
     // sample, weights and base gradients on the device
@@ -2119,6 +2153,9 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
   if (ofree) {
     const int nblk = a->kind == NQS_ANSATZ_COMPLEX_RBM ? 2 : 1;
     CKR(g_ws.colwork.ensure(sizeof(double) * nqs::gram_factored_work_elems(B, N, a->M, nblk)));
+  } else if (fdual) {
+    const int nblk = a->kind == NQS_ANSATZ_COMPLEX_RBM ? 2 : 1;
+    CKR(g_ws.colwork.ensure(sizeof(double) * nqs::factored_delta_work_elems(B, N, a->M, nblk)));
   } else {
     CKR(g_ws.colwork.ensure(sizeof(double) * nqs::colsum_work_elems(B, P)));
   }
@@ -2137,7 +2174,13 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
     CKR(nqs::launch_sum(E, B, E, 1.0 / B, sc + 1, g_stream));
     CKR(nqs::launch_sum(Ei, B, nullptr, 1.0 / B, sc + 2, g_stream));
   }
-  if (!ofree) CKR(nqs::launch_colsum(O, B, P, nullptr, 1.0 / B, Om, work, g_stream));
+  if (fdual)
+    CKR(nqs::launch_factored_delta(d_samples, B, N, a->M, g_ws.t_re.as<float>(),
+                                   a->kind == NQS_ANSATZ_COMPLEX_RBM ? g_ws.t_im.as<float>()
+                                                                     : nullptr,
+                                   nullptr, 1.0 / B, work, Om, g_stream));
+  else if (!ofree)
+    CKR(nqs::launch_colsum(O, B, P, nullptr, 1.0 / B, Om, work, g_stream));
   if (holo && tvmc) {
     // F = 2[(R^T E_im - I^T E_re)/Ns - <R><E_im> + <I><E_re>]
     // (nqs_optimizer.c:797-810): second term and mean carry the minus sign
@@ -2178,6 +2221,7 @@ int sr_step_impl(int method, const nqs_config_t* cfg, int Lx, int Ly, nqs_ansatz
     m.factored = factored;
     m.dual = dual;
     m.mixed = prec == NQS_HIP_MIXED;
+    m.fdual = fdual;
     m.delta = delta;
     CKR(minsr_core(m, converged));
   } else {
@@ -3996,6 +4040,7 @@ int minsr_solve_impl(nqs_ansatz_t* a, const int* spins, int B, const double* e_l
   const int N = a->N;
   const long P = a->P;
   const bool factored = gram == NQS_HIP_GRAM_FACTORED, dual = gram == NQS_HIP_GRAM_PARAMETER;
+  const bool fdual = dual && prec == NQS_HIP_MIXED && rbm_like && factored_dual_on(a);
   a->dev_fresh = 0;
   CKR(ansatz_upload(a));
   CKR(upload_spins(spins, (long)B * N));
@@ -4006,6 +4051,8 @@ int minsr_solve_impl(nqs_ansatz_t* a, const int* spins, int B, const double* e_l
   double* O = nullptr;
   if (factored) {
     CKR(forward_t<double>(a, d_samples, B, nullptr, nullptr));
+  } else if (fdual) {
+    CKR(forward_t<float>(a, d_samples, B, nullptr, nullptr, false, true));
   } else {
     CKR(g_ws.O.ensure(sizeof(double) * (size_t)B * (size_t)P));
     O = g_ws.O.as<double>();
@@ -4027,6 +4074,9 @@ int minsr_solve_impl(nqs_ansatz_t* a, const int* spins, int B, const double* e_l
   if (factored) {
     const int nblk = a->kind == NQS_ANSATZ_COMPLEX_RBM ? 2 : 1;
     CKR(g_ws.colwork.ensure(sizeof(double) * nqs::gram_factored_work_elems(B, N, a->M, nblk)));
+  } else if (fdual) {
+    const int nblk = a->kind == NQS_ANSATZ_COMPLEX_RBM ? 2 : 1;
+    CKR(g_ws.colwork.ensure(sizeof(double) * nqs::factored_delta_work_elems(B, N, a->M, nblk)));
   } else {
     CKR(g_ws.colwork.ensure(sizeof(double) * nqs::colsum_work_elems(B, P)));
   }
@@ -4038,6 +4088,11 @@ int minsr_solve_impl(nqs_ansatz_t* a, const int* spins, int B, const double* e_l
   double* work = g_ws.colwork.as<double>();
   CKR(nqs::launch_sum(E, B, nullptr, 1.0 / B, sc + 0, g_stream));
   if (O) CKR(nqs::launch_colsum(O, B, P, nullptr, 1.0 / B, vb + 5 * P, work, g_stream));
+  if (fdual)
+    CKR(nqs::launch_factored_delta(d_samples, B, N, a->M, g_ws.t_re.as<float>(),
+                                   a->kind == NQS_ANSATZ_COMPLEX_RBM ? g_ws.t_im.as<float>()
+                                                                     : nullptr,
+                                   nullptr, 1.0 / B, work, vb + 5 * P, g_stream));
   MinsrArgs m;
   m.a = a;
   m.d_samples = d_samples;
@@ -4052,6 +4107,7 @@ int minsr_solve_impl(nqs_ansatz_t* a, const int* spins, int B, const double* e_l
   m.factored = factored;
   m.dual = dual;
   m.mixed = prec == NQS_HIP_MIXED;
+  m.fdual = fdual;
   m.delta = vb;
   m.h_T = out_T;
   m.h_y = out_y;

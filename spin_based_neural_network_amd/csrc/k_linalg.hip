@@ -2265,15 +2265,16 @@ int launch_gram_factored(const int* spins, int B, int N, const double* U1, const
 // --------------------------------------------------------------------------
 constexpr int kFdChunk = 128;
 
+template <typename UT>
 __global__ __launch_bounds__(256) void k_factored_ot(const int* __restrict__ spins, int B, int N,
-                                                     int M, const double* __restrict__ U1,
-                                                     const double* __restrict__ U2,
+                                                     int M, const UT* __restrict__ U1,
+                                                     const UT* __restrict__ U2,
                                                      const double* __restrict__ w, int tiles_n,
                                                      int nsplit, double* __restrict__ Zp) {
   const int M1 = M + 1, N1 = N + 1;
   const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
   const int split = blockIdx.y, z = blockIdx.z;
-  const double* U = z ? U2 : U1;
+  const UT* U = z ? U2 : U1;
   const double sg = z ? -1.0 : 1.0;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lr = lane & 15, lk = lane >> 4;
@@ -2293,7 +2294,7 @@ __global__ __launch_bounds__(256) void k_factored_ot(const int* __restrict__ spi
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       const int h = hb + 16 * q + lr, i = ib + 16 * q + lr;
-      const double uh = h < M ? sg * U[(size_t)bc * M + h] : 1.0;
+      const double uh = h < M ? sg * (double)U[(size_t)bc * M + h] : 1.0;
       av[q] = h < M1 ? wb * uh : 0.0;
       const double si = i < N ? (double)spins[(size_t)bc * N + i] : 1.0;
       bv[q] = i < N1 ? si : 0.0;
@@ -2338,20 +2339,31 @@ int factored_delta_work_elems(int B, int N, int M, int nblk) {
   return nblk * ((B + kFdChunk - 1) / kFdChunk) * (M + 1) * (N + 1);
 }
 
-int launch_factored_delta(const int* spins, int B, int N, int M, const double* U1,
-                          const double* U2, const double* w, double alpha, double* work,
-                          double* delta, hipStream_t st) {
+template <typename UT>
+static int factored_delta_impl(const int* spins, int B, int N, int M, const UT* U1, const UT* U2,
+                               const double* w, double alpha, double* work, double* delta,
+                               hipStream_t st) {
   const int nsplit = (B + kFdChunk - 1) / kFdChunk, nblk = U2 ? 2 : 1;
   const int tiles_m = (M + 1 + 63) / 64, tiles_n = (N + 1 + 63) / 64;
   KScope ks("colsum", st);
-  hipLaunchKernelGGL(k_factored_ot, dim3(tiles_m * tiles_n, nsplit, nblk), dim3(256), 0, st, spins, B,
-                     N, M, U1, U2, w, tiles_n, nsplit, work);
+  hipLaunchKernelGGL(k_factored_ot<UT>, dim3(tiles_m * tiles_n, nsplit, nblk), dim3(256), 0, st,
+                     spins, B, N, M, U1, U2, w, tiles_n, nsplit, work);
   const long ne = (long)nblk * (M + 1) * (N + 1);
   hipLaunchKernelGGL(k_factored_delta, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, work, N,
                      M, nsplit, nblk, alpha, delta);
   count_launch(2);
   NQS_HIP_CHECK(hipGetLastError());
   return 0;
+}
+int launch_factored_delta(const int* spins, int B, int N, int M, const double* U1,
+                          const double* U2, const double* w, double alpha, double* work,
+                          double* delta, hipStream_t st) {
+  return factored_delta_impl(spins, B, N, M, U1, U2, w, alpha, work, delta, st);
+}
+int launch_factored_delta(const int* spins, int B, int N, int M, const float* U1,
+                          const float* U2, const double* w, double alpha, double* work,
+                          double* delta, hipStream_t st) {
+  return factored_delta_impl(spins, B, N, M, U1, U2, w, alpha, work, delta, st);
 }
 
 // plain streaming copy (HBM probe used to calibrate byte counters)

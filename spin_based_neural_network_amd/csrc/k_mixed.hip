@@ -50,6 +50,14 @@ constexpr int XT = 128;       // output tile edge
 constexpr int XK = 32;        // K per chunk
 constexpr int XLD = XK;       // LDS row stride (halves)
 
+// lane l's value of v in every lane (l wave-uniform)
+__device__ inline double x3_readlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // half offset of the 8-half group g of row r in the swizzled LDS rows
 __device__ inline int x3_grp(int r, int g) { return 8 * (g ^ ((r >> 1) & 3)); }
 
@@ -475,6 +483,104 @@ long x3_max_elems(long R, long C, long ld, int transpose) {
   if (transpose) return ((C + 63) / 64) * ((ld + 63) / 64) + 1;
   long g = (R * ld + 255) / 256;
   return (g > 16384 ? 16384 : g) + 1;
+}
+
+// Block (x, y): samples 64 x + [0, 64), weight rows of the hidden units in
+// slice y of gridDim.y (slice 0 also the bias rows).  The block's spins and
+// factors are staged in LDS sample-minor (row stride 65 floats:
+// conflict-free stores and reads), then wave w takes rows w, w + 4, ...
+// (the parameter decode is wave-uniform) and its 64 lanes the samples, so
+// each row segment is one 256-byte store.  An entry is formed in FP64 (an
+// FP32 factor times a spin, exact), centred there and rounded once;
+// samples past B give zeros.
+constexpr int kOctRows = 256;   // N + M (+ M) staged rows at most
+
+__global__ __launch_bounds__(256) void k_oct_factored(const int* __restrict__ spins, int B, int N,
+                                                      int M, const float* __restrict__ U1,
+                                                      const float* __restrict__ U2,
+                                                      const double* __restrict__ Om, long P,
+                                                      float* __restrict__ out, long ldk,
+                                                      float* __restrict__ bmax) {
+  __shared__ float stg[kOctRows][65];
+  __shared__ float red[256];
+  const long b0 = (long)blockIdx.x * 64;
+  const int nz = U2 ? 2 : 1;
+  const int nvalid = (int)min(64L, max(0L, (long)B - b0));
+  for (int e = threadIdx.x; e < 64 * N; e += 256) {
+    const int bl = e / N, i = e - bl * N;
+    stg[i][bl] = bl < nvalid ? (float)spins[(b0 + bl) * N + i] : 0.f;
+  }
+  for (int z = 0; z < nz; z++) {
+    const float* U = z ? U2 : U1;
+    for (int e = threadIdx.x; e < 64 * M; e += 256) {
+      const int bl = e / M, h = e - bl * M;
+      stg[N + z * M + h][bl] = bl < nvalid ? U[(b0 + bl) * M + h] : 0.f;
+    }
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const bool live = lane < nvalid, inpad = b0 + lane < ldk;
+  const int hlo = (int)((long)blockIdx.y * M / gridDim.y);
+  const int hhi = (int)((long)(blockIdx.y + 1) * M / gridDim.y);
+  const long Pr = (long)N + M + (long)M * N;
+  float mx = 0.f;
+  auto put = [&](long k, double o, double om) {
+    const float v = live ? (float)(o - om) : 0.f;
+    if (inpad) out[k * ldk + b0 + lane] = v;
+    mx = fmaxf(mx, fabsf(v));
+  };
+  for (int z = 0; z < nz; z++) {
+    const double sg = z ? -1.0 : 1.0;
+    const long kz = z * Pr;
+    // visible biases (zero in the second block), hidden biases
+    for (int r = blockIdx.y ? N + M : wave; r < N + M; r += 4) {
+      const double o = r < N ? (z ? 0.0 : (double)stg[r][lane])
+                             : sg * (double)stg[N + z * M + (r - N)][lane];
+      put(kz + r, o, Om[kz + r]);
+    }
+    // weights, row h*N + i of the block; with N <= 64 the means of the
+    // next unit's rows are loaded (lane i: row i) while this unit's are
+    // stored, and taken by readlane
+    const long kw = kz + N + M;
+    const bool pre = N <= 64;
+    double omv = (pre && hlo < hhi && lane < N) ? Om[kw + (long)hlo * N + lane] : 0.0;
+    for (int h = hlo; h < hhi; h++) {
+      const double omn = (pre && h + 1 < hhi && lane < N) ? Om[kw + (long)(h + 1) * N + lane] : 0.0;
+      const double u = sg * (double)stg[N + z * M + h][lane];
+      const long kh = kw + (long)h * N;
+      for (int i = wave; i < N; i += 4)
+        put(kh + i, u * (double)stg[i][lane], pre ? x3_readlane(omv, i) : Om[kh + i]);
+      omv = omn;
+    }
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bmax[(long)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+int launch_oct_factored(const int* spins, int B, int N, int M, const float* U1, const float* U2,
+                        const double* Om, long P, float* out, long ldk, float* amax,
+                        hipStream_t st) {
+  if (B <= 0 || P <= 0) return 0;
+  const long Pr = (long)N + M + (long)M * N;
+  if (P != (U2 ? 2 * Pr : Pr) || ldk < B || N + (U2 ? 2 : 1) * M > kOctRows) return -1;
+  // up to 8 hidden-unit slices per 64 samples (more blocks in flight); one
+  // block maximum each, fewer than x3_max_elems(B, P, ldk, 1)
+  const int gy = M < 8 ? (M > 0 ? M : 1) : 8;
+  const dim3 grid((unsigned)((ldk + 63) / 64), (unsigned)gy);
+  const long nb = (long)grid.x * grid.y;
+  KScope ks("gen_O", st);
+  hipLaunchKernelGGL(k_oct_factored, grid, dim3(256), 0, st, spins, B, N, M, U1, U2, Om, P, out,
+                     ldk, amax + 1);
+  hipLaunchKernelGGL(k_absmax_finish, dim3(1), dim3(256), 0, st, amax + 1, nb, amax);
+  count_launch();
+  count_launch();
+  NQS_HIP_CHECK(hipGetLastError());
+  return 0;
 }
 
 int launch_to_f32_t(const double* in, long R, long C, float* out, long ldk, float* amax,

@@ -289,6 +289,15 @@ int launch_gemm_x3(int M, int N, int K, double alpha, const float* A, long lda, 
 // x3_max_elems(...) floats (block maxima after the first)
 long x3_max_elems(long R, long C, long ld, int transpose);
 // out [C][ldk] = FP32 transpose of in [R][C], zero for rows R..ldk-1
+// out [P][ldk] = (O - Om)^T in FP32 for an RBM's real-projected O rows
+// (s, u, u (x) s; then 0, -v, -v (x) s with v = U2 when given), generated
+// from the spins [B][N] and the FP32 factors U1, U2 [B][M] (columns B..ldk
+// zero), with the block maxima of |out| and their maximum in amax[0]
+// (x3_max_elems(B, P, ldk, 1) floats); N + M (+ M) <= 256 (kOctRows), else
+// -1
+int launch_oct_factored(const int* spins, int B, int N, int M, const float* U1, const float* U2,
+                        const double* Om, long P, float* out, long ldk, float* amax,
+                        hipStream_t st);
 int launch_to_f32_t(const double* in, long R, long C, float* out, long ldk, float* amax,
                     hipStream_t st);
 // out [R][ldc] = FP32 copy of in [R][C], zero for columns C..ldc-1
@@ -391,6 +400,10 @@ int launch_gram_factored(const int* spins, int B, int N, const double* U1, const
 int factored_delta_work_elems(int B, int N, int M, int nblk);
 int launch_factored_delta(const int* spins, int B, int N, int M, const double* U1,
                           const double* U2, const double* w, double alpha, double* work,
+                          double* delta, hipStream_t st);
+// the same sums over FP32 factors (each factor widened exactly, FP64 MFMA)
+int launch_factored_delta(const int* spins, int B, int N, int M, const float* U1,
+                          const float* U2, const double* w, double alpha, double* work,
                           double* delta, hipStream_t st);
 int launch_add_diag(double* A, int n, long ld, double v, hipStream_t st);
 // out [C][R] = in [R][C]^T

@@ -30,9 +30,9 @@ pytestmark = pytest.mark.gpu
 CRBM, KAGOME = 6, 6
 
 
-def _setup(hip, orc, Lx, Ly, M, B, seed):
+def _setup(hip, orc, Lx, Ly, M, B, seed, ansatz=CRBM):
     N = 3 * Lx * Ly
-    cfg = config_defaults(ansatz=CRBM, hamiltonian=KAGOME, kagome_pbc=1, rbm_hidden_units=M,
+    cfg = config_defaults(ansatz=ansatz, hamiltonian=KAGOME, kagome_pbc=1, rbm_hidden_units=M,
                           rbm_init_scale=0.1, rng_seed=seed)
     a = hip.nqs_ansatz_create(ctypes.byref(cfg), N)
     o = orc.orc_ansatz_create(ctypes.byref(cfg), N)
@@ -111,3 +111,28 @@ def test_mixed_parameter_space_update(hip, orc, shift):
     assert rel_dual < 1e-6
     assert relmx < (1e-4 if shift >= 1e-2 else 5e-3)
     assert cos > 0.99999
+
+
+@pytest.mark.parametrize("ansatz,M,B", [(CRBM, 96, 4096), (5, 40, 1000)])
+def test_mixed_parameter_space_without_o_matrix(hip, orc, monkeypatch, ansatz, M, B):
+    """The mixed parameter-space solve of an RBM takes O_c^T (FP32) from the
+    spins and factors and the means and force from FP64 factor sums, without
+    forming O: against the path through the materialised O on the same
+    samples (same FP32 entries up to the rounding of the column means, so
+    the updates agree far inside the mixed mode's bound), and against the
+    FP64 update.  Cases: the C5 complex RBM and a real RBM with B = 1000
+    (samples padded to 1024)."""
+    a, P, spins, e, Oc = _setup(hip, orc, 2, 4, M, B, 905, ansatz=ansatz)
+    shift = 1e-2
+    print(f"[mixed] parameter space without O: ansatz {ansatz} P={P} B={B}", flush=True)
+    monkeypatch.setenv("NQS_HIP_FACTORED_DUAL", "0")
+    d_mat, _ = _solve(hip, a, spins, e, shift, MIXED, GRAM_PARAMETER, False)
+    monkeypatch.setenv("NQS_HIP_FACTORED_DUAL", "1")
+    d_fac, _ = _solve(hip, a, spins, e, shift, MIXED, GRAM_PARAMETER, False)
+    d64, _ = _solve(hip, a, spins, e, shift, FP64, GRAM_PARAMETER, False)
+    hip.nqs_ansatz_free(a)
+    rel = np.linalg.norm(d_fac - d_mat) / np.linalg.norm(d_mat)
+    rel64 = np.linalg.norm(d_fac - d64) / np.linalg.norm(d64)
+    print(f"[mixed] |d(no O) - d(O)| / |d| = {rel:.3e}; against FP64 {rel64:.3e}", flush=True)
+    assert rel < 1e-5
+    assert rel64 < 1e-4
