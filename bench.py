@@ -195,6 +195,10 @@ def optimise(lib, w, st, budget):
 
 # kernel symbol (as rocprofv3 names it) behind each timing tag
 SYMBOL = {"gemm_potrf": "nqs::k_gemm_big<1>", "gemm_chain": "nqs::k_gemm_lat",
+          "multi_gemm_g": "nqs::(anonymous namespace)::k_mb_gemm_g",
+          "multi_gemm_z": "nqs::(anonymous namespace)::k_mb_gemm_z",
+          "multi_rows": "nqs::(anonymous namespace)::k_mb_rows",
+          "multi_vec": "nqs::(anonymous namespace)::k_mb_finish + k_mb_update + k_mb_dir",
           "gemm": "nqs::k_gemm_big<0>",
           "potrf_panel": "nqs::k_panel2",
           "eloc": "nqs::k_eloc_terms<double, true>", "gen_O": "nqs::k_gen_O<double>",
@@ -525,8 +529,11 @@ def gpu_run_multi(args, w):
     chosen = lib.nqs_hip_multi_tuning(m, tg, us, 8)
     log("[bench] autotune (G tiles x splits target: CG product us): " +
         ", ".join(f"{tg[i]}: {us[i]:.1f}" for i in range(6)) + f" -> {chosen}")
-    # one step launch by launch for the per-kernel split
+    # one step launch by launch (CG as one launch per phase and iteration,
+    # the timed configuration) for the per-kernel split, then one with the
+    # CG in the persistent cooperative launch for comparison
     lib.nqs_hip_multi_use_graph(m, 0)
+    lib.nqs_hip_multi_use_persistent(m, 0)
     lib.nqs_hip_kernel_timing(",".join(MULTI_TAGS).encode())
     lib.nqs_hip_multi_step(m, infos)
     per = {}
@@ -535,8 +542,25 @@ def gpu_run_multi(args, w):
         if lib.nqs_hip_kernel_stats(t.encode(), ctypes.byref(c), ctypes.byref(ms)) == 0 and c.value:
             per[t] = (c.value, ms.value)
     lib.nqs_hip_kernel_timing(None)
+    iters_seq = [infos[i].cg_iterations for i in range(n)]
+    lib.nqs_hip_multi_use_persistent(m, 1)
+    lib.nqs_hip_kernel_timing(",".join(MULTI_TAGS + ["multi_cg"]).encode())
+    lib.nqs_hip_multi_step(m, infos)
+    per_p = {}
+    for t in MULTI_TAGS + ["multi_cg"]:
+        c, ms = ctypes.c_longlong(0), ctypes.c_double(0)
+        if lib.nqs_hip_kernel_stats(t.encode(), ctypes.byref(c), ctypes.byref(ms)) == 0 and c.value:
+            per_p[t] = (c.value, ms.value)
+    lib.nqs_hip_kernel_timing(None)
+    lib.nqs_hip_multi_use_persistent(m, 0)
     lib.nqs_hip_multi_use_graph(m, 1)
-    iters_prof = [infos[i].cg_iterations for i in range(n)]
+    iters_prof = iters_seq
+    iters_per = [infos[i].cg_iterations for i in range(n)]
+    seq_iter_us = 1e3 * sum(per.get(t, (0, 0.0))[1] for t in
+                            ("multi_gemm_z", "multi_rows", "multi_gemm_g", "multi_vec")) / max(iters_seq)
+    cg_iter_us = 1e3 * per_p["multi_cg"][1] / max(iters_per) if "multi_cg" in per_p else None
+    log(f"[bench] CG iteration of all problems: launch sequence {seq_iter_us:.1f} us (kernel time), "
+        f"persistent launch {cg_iter_us if cg_iter_us is None else round(cg_iter_us, 1)} us")
     log("[bench] kernel split of one step (launches, ms): " +
         ", ".join(f"{t}={c}/{ms:.2f}" for t, (c, ms) in sorted(per.items(), key=lambda kv: -kv[1][1])))
     for i in range(args.warmup):
@@ -597,9 +621,12 @@ def gpu_run_multi(args, w):
                 ms_per_step=ms_step, taus=taus,
                 iters=iters, energies=energies, inst=inst, convergence=conv, chosen=chosen,
                 tuning={int(tg[i]): float(us[i]) for i in range(6)},
-                roofline=dict(kernel=dom, symbol=None, bound=bound, achieved=achieved, peak=peak,
-                              unit=runit, frac=achieved / peak, traffic=None, avg_ms=avg_ms),
-                split={t: dict(launches=cc, ms=mm) for t, (cc, mm) in per.items()})
+                roofline=dict(kernel=dom, symbol=SYMBOL.get(dom), bound=bound, achieved=achieved,
+                              peak=peak, unit=runit, frac=achieved / peak, traffic=None,
+                              avg_ms=avg_ms),
+                cg_iter_us=dict(launch_sequence=seq_iter_us, persistent_cooperative=cg_iter_us),
+                split={t: dict(launches=cc, ms=mm) for t, (cc, mm) in per.items()},
+                split_persistent={t: dict(launches=cc, ms=mm) for t, (cc, mm) in per_p.items()})
 
 
 def cpu_baseline_multi(w, gpu):
@@ -940,13 +967,15 @@ def main_multi(args, w):
         "data": "synthetic (random-init parameters N(0,0.01), chains started at random states)",
         "config": {"workload": args.workload + ": " + w["desc"], "instances": len(gpu["inst"]),
                    "samples_per_instance": w["chains"] * w["per_chain"], "method": "cg",
-                   "split_target": gpu["chosen"]},
+                   "split_target": gpu["chosen"], "cg_launch": "launch sequence in the graph"},
         "sr_iter_ms": gpu["ms_per_step"],
         "sample_ms": gpu["sample_ms"],
         "ess_per_s_iteration": gpu["ess_per_s_iteration"],
         "cg_iterations": gpu["iters"],
         "autotune_us": gpu["tuning"],
         "kernel_split_ms": {k: v["ms"] for k, v in gpu["split"].items()},
+        "kernel_split_persistent_ms": {k: v["ms"] for k, v in gpu["split_persistent"].items()},
+        "cg_iter_us": gpu["cg_iter_us"],
         "roofline": gpu["roofline"],
         "cpu_baseline": cpu,
     }

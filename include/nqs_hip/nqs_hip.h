@@ -190,6 +190,13 @@ int nqs_hip_sampler_learned_stats(const nqs_sampler_t *s, double *acceptance,
  * product times (microseconds); returns the chosen target.
  * use_graph(m, 0): later steps are issued launch by launch instead of as
  * the captured graph (same kernels and results; for per-kernel timing).
+ * use_persistent(m, on): the CG iterations of all problems in one
+ * cooperative launch with grid barriers between phases (on; or
+ * NQS_HIP_MULTI_PERSIST=1 at creation), or one launch per phase and
+ * iteration (off, the default: on the MI355X the six grid barriers per CG
+ * iteration cost more than the captured graph's launch gaps); same results
+ * bit for bit.  A device that refuses the cooperative launch gets the launch
+ * sequence, with a message.
  * last_energies: the local energies of problem i's last batch, sample
  * r*C + c in slot r*C + c (chain c, round r); returns the count. */
 typedef struct nqs_hip_multi nqs_hip_multi_t;
@@ -201,6 +208,7 @@ int nqs_hip_multi_run(nqs_hip_multi_t *m, int num_iterations, double *out_traces
 int nqs_hip_multi_sync(nqs_hip_multi_t *m);
 int nqs_hip_multi_tuning(const nqs_hip_multi_t *m, int *targets, double *matvec_us, int capacity);
 int nqs_hip_multi_use_graph(nqs_hip_multi_t *m, int on);
+int nqs_hip_multi_use_persistent(nqs_hip_multi_t *m, int on);
 /* device time of the last step's Metropolis launches (all problems), ms */
 int nqs_hip_multi_last_sample_ms(const nqs_hip_multi_t *m, double *ms);
 int nqs_hip_multi_last_energies(const nqs_hip_multi_t *m, int i, double *out, int capacity);

@@ -218,6 +218,7 @@ HIP_API = {
     "nqs_hip_multi_tuning": (c_int, [_P, _P, _P, c_int]),
     "nqs_hip_multi_free": (None, [_P]),
     "nqs_hip_multi_use_graph": (c_int, [_P, c_int]),
+    "nqs_hip_multi_use_persistent": (c_int, [_P, c_int]),
     "nqs_hip_multi_last_sample_ms": (c_int, [_P, _P]),
     "nqs_hip_multi_last_energies": (c_int, [_P, c_int, _P, c_int]),
 }

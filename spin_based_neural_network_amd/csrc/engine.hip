@@ -4390,6 +4390,11 @@ struct nqs_hip_multi {
   double* h_out = nullptr;              // pinned, kMultiOut per problem
   long long steps_done = 0;
   int tuned = 0, use_graph = 1;
+  // the CG iterations in one cooperative launch (NQS_HIP_MULTI_PERSIST=1 at
+  // creation, or nqs_hip_multi_use_persistent), the captured graph then
+  // holding the step up to the CG start; off by default: its six grid
+  // barriers per CG iteration cost more than the graph's launch gaps
+  int persist = 0;
   // Metropolis jobs of all problems, grouped by kernel variant:
   // (variant, first job, jobs, blocks)
   std::vector<nqs::MHJob> jobs;
@@ -4533,7 +4538,20 @@ int multi_issue_front(nqs_hip_multi* m) {
 }
 
 int multi_issue_back(nqs_hip_multi* m) {
-  for (int it = 0; it < m->max_iters; it++) CKR(nqs::launch_multi_cg_iter(m->mb, 0, g_stream));
+  if (m->persist && nqs::launch_multi_cg_persist(m->mb, m->max_iters, g_stream) != 0) {
+    fprintf(stderr, "nqs_hip_multi_step: the CG runs as a launch sequence from here on\n");
+    m->persist = 0;
+    if (m->exec) {
+      (void)hipGraphExecDestroy(m->exec);
+      m->exec = nullptr;
+    }
+    if (m->graph) {
+      (void)hipGraphDestroy(m->graph);
+      m->graph = nullptr;
+    }
+  }
+  if (!m->persist)
+    for (int it = 0; it < m->max_iters; it++) CKR(nqs::launch_multi_cg_iter(m->mb, 0, g_stream));
   CKR(nqs::launch_multi_apply(m->mb, g_stream));
   CKR(nqs::launch_multi_gather(m->mb, m->d_out.as<double>(), g_stream));
   CK(hipMemcpyAsync(m->h_out, m->d_out.p, sizeof(double) * nqs::kMultiOut * m->p.size(),
@@ -4541,10 +4559,15 @@ int multi_issue_back(nqs_hip_multi* m) {
   return 0;
 }
 
-int multi_issue_step(nqs_hip_multi* m) {
+// the step up to the CG start
+int multi_issue_pre(nqs_hip_multi* m) {
   CKR(multi_issue_front(m));
   CKR(nqs::launch_multi_prep(m->mb, g_stream));
-  CKR(nqs::launch_multi_force(m->mb, g_stream));
+  return nqs::launch_multi_force(m->mb, g_stream);
+}
+
+int multi_issue_step(nqs_hip_multi* m) {
+  CKR(multi_issue_pre(m));
   return multi_issue_back(m);
 }
 
@@ -4609,7 +4632,8 @@ void multi_fill_info(nqs_hip_multi* m, nqs_sr_step_info_t* infos) {
 int multi_capture(nqs_hip_multi* m) {
   const int kt = g_kt_on.exchange(0);
   hipError_t e = hipStreamBeginCapture(g_stream, hipStreamCaptureModeRelaxed);
-  int rc = e == hipSuccess ? multi_issue_step(m) : -1;
+  // the cooperative CG launch stays outside the graph
+  int rc = e == hipSuccess ? (m->persist ? multi_issue_pre(m) : multi_issue_step(m)) : -1;
   hipGraph_t g = nullptr;
   hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(g_stream, &g) : e;
   g_kt_on = kt;
@@ -4640,6 +4664,7 @@ int multi_step(nqs_hip_multi* m, nqs_sr_step_info_t* infos) {
   } else {
     if (!m->exec) CKR(multi_capture(m));
     CK(hipGraphLaunch(m->exec, g_stream));
+    if (m->persist) CKR(multi_issue_back(m));
   }
   CK(hipEventRecord(m->ev_t1, g_stream));
   CK(hipStreamSynchronize(g_stream));
@@ -4670,6 +4695,10 @@ nqs_hip_multi* multi_new(int n, const nqs_config_t* cfgs, const int* size_x, con
   if (n <= 0 || !cfgs || !size_x || !size_y || !ansatz || !samplers) return nullptr;
   if (stream_ready()) return nullptr;
   nqs_hip_multi* m = new nqs_hip_multi();
+  {
+    const char* e = getenv("NQS_HIP_MULTI_PERSIST");
+    m->persist = (e && e[0] == '1') ? 1 : 0;
+  }
   m->p.resize(n);
   m->mp.resize(n);
   auto fail = [&](int i, const char* why) -> nqs_hip_multi_t* {
@@ -4896,6 +4925,20 @@ NQS_EXPORT int nqs_hip_multi_use_graph(nqs_hip_multi_t* m, int on) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (!m) return -1;
   m->use_graph = on ? 1 : 0;
+  return 0;
+}
+
+NQS_EXPORT int nqs_hip_multi_use_persistent(nqs_hip_multi_t* m, int on) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!m) return -1;
+  if ((on ? 1 : 0) != m->persist) {
+    // the captured graph's extent depends on the mode
+    if (m->exec) (void)hipGraphExecDestroy(m->exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    m->exec = nullptr;
+    m->graph = nullptr;
+  }
+  m->persist = on ? 1 : 0;
   return 0;
 }
 

@@ -18,6 +18,8 @@
 // Scalar reductions over a problem's blocks use one arrival counter: every
 // block stores its partial, the block that arrives last sums the partials in
 // block order (deterministic) and resets the counter.
+#include <hip/hip_cooperative_groups.h>
+
 #include "nqs_kernels.h"
 
 namespace nqs {
@@ -155,13 +157,13 @@ __global__ __launch_bounds__(256) void k_mb_prep(MultiBatch mb) {
 }
 
 // Zt = V S1^T  (M+1) x B, K = N+1
-__global__ __launch_bounds__(256) void k_mb_gemm_z(MultiBatch mb) {
-  const int qi = find_prob<F_Z>(mb, blockIdx.x);
+__device__ inline void mb_gemm_z(const MultiBatch& mb, int blk) {
+  const int qi = find_prob<F_Z>(mb, blk);
   const MultiProb& q = mb.probs[qi];
   if (cg_done(q)) return;
   const int M1 = q.M + 1, N1 = q.N + 1;
   const int tiles_n = (q.B + MT - 1) / MT;
-  const int t = blockIdx.x - q.blk_z;
+  const int t = blk - q.blk_z;
   tile_nt(M1, q.B, 0, N1, q.p, N1, q.S1, N1, q.Zt, q.B, (t / tiles_n) * MT, (t % tiles_n) * MT);
 }
 
@@ -169,13 +171,14 @@ __global__ __launch_bounds__(256) void k_mb_gemm_z(MultiBatch mb) {
 // Block: 64 samples x 4 groups of hidden rows (h = g, g + 4, ...); the four
 // group sums are added in group order, so every thread of a sample sees the
 // same u_b.
-__global__ __launch_bounds__(256) void k_mb_rows(MultiBatch mb) {
+__device__ inline void mb_rows(const MultiBatch& mb, int blk) {
   __shared__ double red[4][64];
-  const int qi = find_prob<F_ROWS>(mb, blockIdx.x);
+  __syncthreads();   // a previous work item of this block has read red
+  const int qi = find_prob<F_ROWS>(mb, blk);
   const MultiProb& q = mb.probs[qi];
   if (cg_done(q)) return;
   const int bl = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const int b = (blockIdx.x - q.blk_rows) * 64 + bl;
+  const int b = (blk - q.blk_rows) * 64 + bl;
   const long B = q.B;
   double u = 0.0;
   if (b < q.B)
@@ -190,13 +193,13 @@ __global__ __launch_bounds__(256) void k_mb_rows(MultiBatch mb) {
 // Gp[slot][split] = A[:, Kchunk] S1t[:, Kchunk]^T, (M+1) x (N+1).  With two
 // slots (the force) slot 0 takes A = T1t (weights 1) and slot 1 A = TWt
 // (weights E); with one slot (CG) A = TWt (weights O p).
-__global__ __launch_bounds__(256) void k_mb_gemm_g(MultiBatch mb, int force) {
-  const int qi = force ? find_prob<F_G2>(mb, blockIdx.x) : find_prob<F_G>(mb, blockIdx.x);
+__device__ inline void mb_gemm_g(const MultiBatch& mb, int blk, int force) {
+  const int qi = force ? find_prob<F_G2>(mb, blk) : find_prob<F_G>(mb, blk);
   const MultiProb& q = mb.probs[qi];
   if (!force && cg_done(q)) return;
   const int M1 = q.M + 1, N1 = q.N + 1;
   const int tm = (M1 + MT - 1) / MT, tn = (N1 + MT - 1) / MT;
-  int t = blockIdx.x - (force ? q.blk_g2 : q.blk_g);
+  int t = blk - (force ? q.blk_g2 : q.blk_g);
   const int per_slot = tm * tn * q.nsplit;
   const int slot = t / per_slot;
   t -= slot * per_slot;
@@ -259,12 +262,12 @@ __global__ __launch_bounds__(kMultiVecBlock) void k_mb_force(MultiBatch mb) {
 
 // Ap = (O^T u)/B - Om mean(u) + eps p, pAp; alpha = rs / pAp or breakdown
 // when |pAp| < 1e-30 (nqs_optimizer.c:103-112)
-__global__ __launch_bounds__(kMultiVecBlock) void k_mb_finish(MultiBatch mb) {
+__device__ inline void mb_finish(const MultiBatch& mb, int gblk) {
   __shared__ double red[kMultiVecBlock];
-  const int qi = find_prob<F_VEC>(mb, blockIdx.x);
+  const int qi = find_prob<F_VEC>(mb, gblk);
   const MultiProb& q = mb.probs[qi];
   if (cg_done(q)) return;
-  const int blk = blockIdx.x - q.blk_vec, nblk = vec_blocks(q);
+  const int blk = gblk - q.blk_vec, nblk = vec_blocks(q);
   const int L = (q.M + 1) * (q.N + 1);
   const int e = blk * kMultiVecBlock + threadIdx.x;
   double pa = 0.0;
@@ -294,12 +297,12 @@ __global__ __launch_bounds__(kMultiVecBlock) void k_mb_finish(MultiBatch mb) {
 
 // x += alpha p, r -= alpha Ap, rs_new = r.r; converged when
 // rs_new <= tol^2 rs0, else beta = rs_new / rs (nqs_optimizer.c:113-136)
-__global__ __launch_bounds__(kMultiVecBlock) void k_mb_update(MultiBatch mb) {
+__device__ inline void mb_update(const MultiBatch& mb, int gblk) {
   __shared__ double red[kMultiVecBlock];
-  const int qi = find_prob<F_VEC>(mb, blockIdx.x);
+  const int qi = find_prob<F_VEC>(mb, gblk);
   const MultiProb& q = mb.probs[qi];
   if (cg_done(q)) return;
-  const int blk = blockIdx.x - q.blk_vec, nblk = vec_blocks(q);
+  const int blk = gblk - q.blk_vec, nblk = vec_blocks(q);
   const int L = (q.M + 1) * (q.N + 1);
   const int e = blk * kMultiVecBlock + threadIdx.x;
   const double a = q.sc->alpha;
@@ -327,13 +330,57 @@ __global__ __launch_bounds__(kMultiVecBlock) void k_mb_update(MultiBatch mb) {
 }
 
 // p = r + beta p
-__global__ __launch_bounds__(kMultiVecBlock) void k_mb_dir(MultiBatch mb) {
-  const int qi = find_prob<F_VEC>(mb, blockIdx.x);
+__device__ inline void mb_dir(const MultiBatch& mb, int gblk) {
+  const int qi = find_prob<F_VEC>(mb, gblk);
   const MultiProb& q = mb.probs[qi];
   if (cg_done(q)) return;
   const int L = (q.M + 1) * (q.N + 1);
-  const int e = (blockIdx.x - q.blk_vec) * kMultiVecBlock + threadIdx.x;
+  const int e = (gblk - q.blk_vec) * kMultiVecBlock + threadIdx.x;
   if (e < L) q.p[e] = q.r[e] + q.sc->beta * q.p[e];
+}
+
+// the CG launches: one block per work item
+__global__ __launch_bounds__(256) void k_mb_gemm_z(MultiBatch mb) { mb_gemm_z(mb, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_mb_rows(MultiBatch mb) { mb_rows(mb, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_mb_gemm_g(MultiBatch mb, int force) {
+  mb_gemm_g(mb, blockIdx.x, force);
+}
+__global__ __launch_bounds__(kMultiVecBlock) void k_mb_finish(MultiBatch mb) {
+  mb_finish(mb, blockIdx.x);
+}
+__global__ __launch_bounds__(kMultiVecBlock) void k_mb_update(MultiBatch mb) {
+  mb_update(mb, blockIdx.x);
+}
+__global__ __launch_bounds__(kMultiVecBlock) void k_mb_dir(MultiBatch mb) { mb_dir(mb, blockIdx.x); }
+
+// Every CG iteration of every problem in one cooperative launch: the six
+// phases of an iteration walk their work items (the blocks of the launch
+// sequence above, same device functions, so the same results bit for bit)
+// grid-stride, with a grid barrier between phases.  Every block passes every
+// barrier: the phase loops only skip items, and the loop ends for all blocks
+// together when, after an iteration's last barrier, every problem reads done
+// (its flags were written before that barrier).  The launch is cooperative,
+// so it either has all its blocks resident or does not start.
+__global__ __launch_bounds__(256) void k_mb_cg_persist(MultiBatch mb, int iters) {
+  static_assert(kMultiVecBlock == 256, "one block size for every phase");
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  for (int it = 0; it < iters; it++) {
+    for (int b = blockIdx.x; b < mb.nblk_z; b += gridDim.x) mb_gemm_z(mb, b);
+    grid.sync();
+    for (int b = blockIdx.x; b < mb.nblk_rows; b += gridDim.x) mb_rows(mb, b);
+    grid.sync();
+    for (int b = blockIdx.x; b < mb.nblk_g; b += gridDim.x) mb_gemm_g(mb, b, 0);
+    grid.sync();
+    for (int b = blockIdx.x; b < mb.nblk_vec; b += gridDim.x) mb_finish(mb, b);
+    grid.sync();
+    for (int b = blockIdx.x; b < mb.nblk_vec; b += gridDim.x) mb_update(mb, b);
+    grid.sync();
+    for (int b = blockIdx.x; b < mb.nblk_vec; b += gridDim.x) mb_dir(mb, b);
+    grid.sync();
+    bool all = true;
+    for (int i = 0; i < mb.n; i++) all = all && cg_done(mb.probs[i]);
+    if (all) break;
+  }
 }
 
 // theta -= lr delta in the reference layout [a | b | W], |delta|^2
@@ -457,6 +504,51 @@ int launch_multi_cg_iter(const MultiBatch& mb, int matvec_only, hipStream_t st) 
   }
   count_launch(matvec_only ? 4 : 6);
   NQS_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int multi_persist_grid(const MultiBatch& mb) {
+  static int cap = -1;
+  if (cap < 0) {
+    int dev = 0, per_cu = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mb_cg_persist, 256, 0) !=
+            hipSuccess) {
+      (void)hipGetLastError();
+      cap = 0;
+    } else {
+      // two blocks per CU: the grid barrier's cost grows with the block
+      // count (measured 768 us per CG iteration at 512 blocks, 926 at 768)
+      cap = (per_cu < 2 ? per_cu : 2) * prop.multiProcessorCount;
+    }
+  }
+  int need = mb.nblk_z;
+  if (mb.nblk_rows > need) need = mb.nblk_rows;
+  if (mb.nblk_g > need) need = mb.nblk_g;
+  if (mb.nblk_vec > need) need = mb.nblk_vec;
+  return need < cap ? need : cap;
+}
+
+int launch_multi_cg_persist(const MultiBatch& mb, int iters, hipStream_t st) {
+  if (mb.n <= 0 || iters <= 0) return 0;
+  const int grid = multi_persist_grid(mb);
+  if (grid <= 0) {
+    fprintf(stderr, "nqs_hip: no occupancy for the persistent CG launch\n");
+    return -1;
+  }
+  MultiBatch a = mb;
+  int it = iters;
+  void* args[] = {&a, &it};
+  KScope ks("multi_cg", st);
+  const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_mb_cg_persist),
+                                                  dim3((unsigned)grid), dim3(256), args, 0, st);
+  count_launch();
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    fprintf(stderr, "nqs_hip: persistent CG launch refused (%s)\n", hipGetErrorString(e));
+    return -1;
+  }
   return 0;
 }
 

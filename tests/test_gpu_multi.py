@@ -142,6 +142,44 @@ def test_multi_equals_single_path_ragged(hip, monkeypatch):
         hip.nqs_ansatz_free(p["a"])
 
 
+def test_multi_persistent_equals_launch_sequence(hip, monkeypatch):
+    """The CG of all problems in one cooperative launch (grid barriers
+    between the phases, the launch sequence's device functions) against one
+    launch per phase and iteration, on identical clones of a ragged batch
+    (N from 8 to 128, problems converging at different iterations): step
+    information and parameters bit-equal over four steps, graph replays of
+    the pre-CG part included."""
+    monkeypatch.setenv("NQS_HIP_MULTI_SPLIT", "64")
+    sizes = [(8, 0.6), (16, 1.4), (48, 0.9), (128, 1.1), (24, 0.5), (96, 1.3)]
+    print("multi persistent vs launch sequence:", sizes, flush=True)
+    r = scenarios.Runner(hip, "nqs_")
+    seq = [_tfim_problem(hip, r, N, h, 64, 512, 31 + N) for N, h in sizes]
+    per = [_tfim_problem(hip, r, N, h, 64, 512, 31 + N) for N, h in sizes]
+    for p in seq + per:
+        hip.nqs_sampler_thermalize(p["s"])
+    m_seq = _make_batch(hip, seq)
+    m_per = _make_batch(hip, per)
+    assert hip.nqs_hip_multi_use_persistent(m_seq, 0) == 0
+    assert hip.nqs_hip_multi_use_persistent(m_per, 1) == 0
+    iters = []
+    for step in range(4):
+        o_seq, o_per = _infos(len(sizes)), _infos(len(sizes))
+        assert hip.nqs_hip_multi_step(m_seq, o_seq) == 0
+        assert hip.nqs_hip_multi_step(m_per, o_per) == 0
+        assert hip.nqs_hip_multi_sync(m_seq) == 0
+        assert hip.nqs_hip_multi_sync(m_per) == 0
+        for i in range(len(sizes)):
+            assert _info_row(o_seq[i]) == _info_row(o_per[i]), (step, i)
+            assert np.array_equal(r.params(seq[i]["a"]), r.params(per[i]["a"])), (step, i)
+        iters.append([o_per[i].cg_iterations for i in range(len(sizes))])
+    print("CG iterations per step:", iters, flush=True)
+    hip.nqs_hip_multi_free(m_seq)
+    hip.nqs_hip_multi_free(m_per)
+    for p in seq + per:
+        hip.nqs_sampler_free(p["s"])
+        hip.nqs_ansatz_free(p["a"])
+
+
 def test_multi_rejects_unsupported(hip):
     r = scenarios.Runner(hip, "nqs_")
     cfg = config_defaults(ansatz=6, hamiltonian=0, num_samples=64)
