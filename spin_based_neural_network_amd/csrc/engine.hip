@@ -4370,7 +4370,7 @@ struct nqs_hip_multi {
     nqs_sampler* s = nullptr;
     nqs_config_t cfg;
     nqs::HamDev H;
-    DBuf ham, sublat, t, E, S1, S1t, T1t, Zt, TWt, Gp, vec, part, ticket, sc, stats;
+    DBuf ham, sublat, t, E, S1, S1t, T1t, U, Gp, vec, part, ticket, sc, stats;
     int B = 0, rounds = 0, steps = 1, max_split = 1;
   };
   std::vector<Prob> p;
@@ -4413,7 +4413,7 @@ struct nqs_hip_multi {
       if (e) (void)hipEventDestroy(e);
     if (h_out) (void)hipHostFree(h_out);
     for (Prob& q : p) {
-      DBuf* all[] = {&q.ham, &q.sublat, &q.t, &q.E, &q.S1, &q.S1t, &q.T1t, &q.Zt, &q.TWt,
+      DBuf* all[] = {&q.ham, &q.sublat, &q.t, &q.E, &q.S1, &q.S1t, &q.T1t, &q.U,
                      &q.Gp, &q.vec, &q.part, &q.ticket, &q.sc, &q.stats};
       for (DBuf* b : all) b->release();
     }
@@ -4738,8 +4738,8 @@ nqs_hip_multi* multi_new(int n, const nqs_config_t* cfgs, const int* size_x, con
     while (q.max_split * 2 <= 32 && q.max_split * 2 * 64 <= q.B) q.max_split *= 2;
     const size_t L = (size_t)M1 * N1, B = (size_t)q.B, d = sizeof(double);
     if (q.t.ensure(d * 2 * B * M) || q.E.ensure(d * 2 * B) || q.S1.ensure(d * B * N1) ||
-        q.S1t.ensure(d * B * N1) || q.T1t.ensure(d * B * M1) || q.Zt.ensure(d * B * M1) ||
-        q.TWt.ensure(d * B * M1) || q.Gp.ensure(d * 2 * q.max_split * L) ||
+        q.S1t.ensure(d * B * N1) || q.T1t.ensure(d * B * M1) || q.U.ensure(d * B * ((M1 + 63) / 64)) ||
+        q.Gp.ensure(d * 2 * q.max_split * L) ||
         q.vec.ensure(d * 6 * L) || q.part.ensure(d * nqs::kMultiMaxPart) ||
         q.ticket.ensure(64) || q.sc.ensure(sizeof(nqs::CGScalars)) || q.stats.ensure(d * 8) ||
         s->d_samples.ensure(sizeof(int) * B * N))
@@ -4771,8 +4771,7 @@ nqs_hip_multi* multi_new(int n, const nqs_config_t* cfgs, const int* size_x, con
     mp.S1 = q.S1.as<double>();
     mp.S1t = q.S1t.as<double>();
     mp.T1t = q.T1t.as<double>();
-    mp.Zt = q.Zt.as<double>();
-    mp.TWt = q.TWt.as<double>();
+    mp.u = q.U.as<double>();
     mp.Gp = q.Gp.as<double>();
     double* v = q.vec.as<double>();
     mp.x = v;

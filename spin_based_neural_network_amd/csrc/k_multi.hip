@@ -10,11 +10,13 @@
 // finds its problem from the per-launch block offsets.  The O matrix is never
 // formed: in the padded (M+1) x (N+1) parameter layout an O row is
 // [t_b, 1] (x) [s_b, 1], so
-//   (O v)_b     = sum_h [t_b, 1]_h (V [s_b, 1]^T)_h       (GEMM Zt = V S1^T)
+//   (O v)_b     = sum_h [t_b, 1]_h (V [s_b, 1]^T)_h       (Z = V S1^T, row weights u)
 //   (O^T w)_hi  = sum_b w_b [t_b, 1]_h [s_b, 1]_i         (GEMM (T1 w)^T S1)
 // on FP64 MFMA, and the last entry of O^T w is sum_b w_b, the mean that the
-// QGT needs.  Per CG iteration: Zt GEMM, row weights, split-K G GEMM,
-// finish (A p, p.Ap, alpha), update (x, r, r.r, beta), direction.
+// QGT needs.  Per CG iteration: Z product and row weights u in one pass
+// (Z never leaves the registers), split-K G GEMM with its operand [t_b, 1]
+// scaled by u_b as it is staged, finish (A p, p.Ap, alpha), update (x, r,
+// r.r, beta), direction.
 // Scalar reductions over a problem's blocks use one arrival counter: every
 // block stores its partial, the block that arrives last sums the partials in
 // block order (deterministic) and resets the counter.
@@ -29,13 +31,12 @@ namespace {
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int MT = 64, MKS = 16;     // GEMM tile and K step
 
-enum { F_PREP = 0, F_Z = 1, F_ROWS = 2, F_G = 3, F_G2 = 4, F_VEC = 5 };
+enum { F_PREP = 0, F_Z = 1, F_G = 3, F_G2 = 4, F_VEC = 5 };
 
 template <int FIELD>
 __device__ inline int first_blk(const MultiProb& q) {
   return FIELD == F_PREP ? q.blk_prep
        : FIELD == F_Z    ? q.blk_z
-       : FIELD == F_ROWS ? q.blk_rows
        : FIELD == F_G    ? q.blk_g
        : FIELD == F_G2   ? q.blk_g2
                          : q.blk_vec;
@@ -53,30 +54,45 @@ __device__ inline int find_prob(const MultiBatch& mb, int blk) {
   return lo;
 }
 
-// C[row0.., col0..] = sum_{k0 <= k < k1} A[r][k] Bm[c][k]  (64 x 64 tile,
-// 4 waves of 32 x 32, v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16]
-// and B[l%16][l/16]; result element j of lane l is D[l/16 + 4j][l%16])
-__device__ inline void tile_nt(int M, int N, int k0, int k1, const double* __restrict__ A,
-                               long lda, const double* __restrict__ Bm, long ldb, double* C,
-                               long ldc, int row0, int col0) {
+// acc = the 64 x 64 tile [row0.., col0..] of sum_{k0 <= k < k1} A[r][k] w_k
+// Bm[c][k] with w_k = sum_{t < nw} w[t ldw + k] added in t order (w null:
+// 1), 4 waves of 32 x 32, v_mfma_f64_16x16x4f64: lane l
+// supplies A[l%16][l/16] and B[l%16][l/16]; result element j of lane l is
+// D[l/16 + 4j][l%16] (wave rows wr = 32 (wave / 2), columns wc = 32 (wave % 2)).
+// A row times its weight is rounded once as it is staged.
+__device__ inline void tile_acc(int M, int N, int k0, int k1, const double* __restrict__ A,
+                                long lda, const double* __restrict__ w, int nw, long ldw,
+                                const double* __restrict__ Bm, long ldb, int row0, int col0,
+                                v4d (&acc)[2][2]) {
   __shared__ double As[MT][MKS + 1];
   __shared__ double Bs[MT][MKS + 1];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
-  v4d acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
   for (int kb = k0; kb < k1; kb += MKS) {
+    // a thread stages K index kb + tid % 16 in all four of its rows: its
+    // weight once per step
+    const int gk = kb + (tid & 15);
+    double wk = 1.0;
+    if (w && gk < k1) {
+      wk = w[gk];
+      for (int t = 1; t < nw; t++) wk += w[t * ldw + gk];
+    }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int e = tid + 256 * q;
       const int r = e >> 4, kk = e & 15;
-      const int gk = kb + kk;
       const int ga = row0 + r, gb = col0 + r;
-      As[r][kk] = (ga < M && gk < k1) ? A[(long)ga * lda + gk] : 0.0;
+      double a = 0.0;
+      if (ga < M && gk < k1) {
+        a = A[(long)ga * lda + gk];
+        if (w) a *= wk;
+      }
+      As[r][kk] = a;
       Bs[r][kk] = (gb < N && gk < k1) ? Bm[(long)gb * ldb + gk] : 0.0;
     }
     __syncthreads();
@@ -93,6 +109,18 @@ __device__ inline void tile_nt(int M, int N, int k0, int k1, const double* __res
     }
     __syncthreads();
   }
+}
+
+// C[row0.., col0..] = the tile of tile_acc
+__device__ inline void tile_nt(int M, int N, int k0, int k1, const double* __restrict__ A,
+                               long lda, const double* __restrict__ w, int nw, long ldw,
+                               const double* __restrict__ Bm, long ldb, double* C, long ldc,
+                               int row0, int col0) {
+  v4d acc[2][2];
+  tile_acc(M, N, k0, k1, A, lda, w, nw, ldw, Bm, ldb, row0, col0, acc);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -148,51 +176,60 @@ __global__ __launch_bounds__(256) void k_mb_prep(MultiBatch mb) {
     q.S1[(long)b * N1 + j] = v;
     q.S1t[(long)j * B + b] = v;
   }
-  const double e = q.E[b];
-  for (int h = 0; h <= M; h++) {
-    const double tv = h < M ? q.t[(long)b * M + h] : 1.0;
-    q.T1t[(long)h * B + b] = tv;
-    q.TWt[(long)h * B + b] = tv * e;
-  }
+  for (int h = 0; h <= M; h++) q.T1t[(long)h * B + b] = h < M ? q.t[(long)b * M + h] : 1.0;
 }
 
-// Zt = V S1^T  (M+1) x B, K = N+1
-__device__ inline void mb_gemm_z(const MultiBatch& mb, int blk) {
+// Row weights of the CG product, u_b = sum_h [t_b, 1]_h (V [s_b, 1]^T)_h, in
+// partial sums over 64-row tiles: work item (tile tm, samples 64 tb + [0,
+// 64)) forms its 64 x 64 tile of Z = V S1^T on MFMA (K = N+1), weights it by
+// [t_b, 1]_h as it leaves the accumulators (Z is not stored) and writes the
+// tile's column sums to u[tm][b].  A thread sums its 8 rows of the tile for
+// its 2 samples; the 8 threads that share a sample are added in a fixed
+// order.  The G product adds the tiles' partials in tile order.
+__device__ inline void mb_zu(const MultiBatch& mb, int blk) {
+  __shared__ double ured[8][MT];
   const int qi = find_prob<F_Z>(mb, blk);
   const MultiProb& q = mb.probs[qi];
   if (cg_done(q)) return;
   const int M1 = q.M + 1, N1 = q.N + 1;
-  const int tiles_n = (q.B + MT - 1) / MT;
-  const int t = blk - q.blk_z;
-  tile_nt(M1, q.B, 0, N1, q.p, N1, q.S1, N1, q.Zt, q.B, (t / tiles_n) * MT, (t % tiles_n) * MT);
-}
-
-// u_b = sum_h [t_b, 1]_h Zt[h][b];  TWt[h][b] = [t_b, 1]_h u_b.
-// Block: 64 samples x 4 groups of hidden rows (h = g, g + 4, ...); the four
-// group sums are added in group order, so every thread of a sample sees the
-// same u_b.
-__device__ inline void mb_rows(const MultiBatch& mb, int blk) {
-  __shared__ double red[4][64];
-  __syncthreads();   // a previous work item of this block has read red
-  const int qi = find_prob<F_ROWS>(mb, blk);
-  const MultiProb& q = mb.probs[qi];
-  if (cg_done(q)) return;
-  const int bl = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const int b = (blk - q.blk_rows) * 64 + bl;
   const long B = q.B;
-  double u = 0.0;
-  if (b < q.B)
-    for (int h = g; h <= q.M; h += 4) u += q.T1t[h * B + b] * q.Zt[h * B + b];
-  red[g][bl] = u;
+  const int tiles_n = (q.B + MT - 1) / MT;
+  const int t = blk - q.blk_z, tm = t / tiles_n;
+  const int row0 = tm * MT, col0 = (t % tiles_n) * MT;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  v4d acc[2][2];
+  tile_acc(M1, q.B, 0, N1, q.p, N1, nullptr, 0, 0, q.S1, N1, row0, col0, acc);
+  double part[2] = {0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const long gc = col0 + wc + 16 * j + lr;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gr = row0 + wr + 16 * i + lk + 4 * r;
+        if (gr < M1 && gc < B) part[j] += acc[i][j][r] * q.T1t[(long)gr * B + gc];
+      }
+  }
+  // slot (wave / 2) * 4 + lk of column wc + 16 j + lr: every (slot, column) once
+#pragma unroll
+  for (int j = 0; j < 2; j++) ured[(wave >> 1) * 4 + lk][wc + 16 * j + lr] = part[j];
   __syncthreads();
-  u = (red[0][bl] + red[1][bl]) + (red[2][bl] + red[3][bl]);
-  if (b < q.B)
-    for (int h = g; h <= q.M; h += 4) q.TWt[h * B + b] = q.T1t[h * B + b] * u;
+  if (tid < MT) {
+    double u = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) u += ured[k][tid];
+    if (col0 + tid < B) q.u[(long)tm * B + col0 + tid] = u;
+  }
+  __syncthreads();   // ured is read before a later work item of this block writes it
 }
 
-// Gp[slot][split] = A[:, Kchunk] S1t[:, Kchunk]^T, (M+1) x (N+1).  With two
-// slots (the force) slot 0 takes A = T1t (weights 1) and slot 1 A = TWt
-// (weights E); with one slot (CG) A = TWt (weights O p).
+// Gp[slot][split] = (T1t w)[:, Kchunk] S1t[:, Kchunk]^T, (M+1) x (N+1): the
+// operand [t_b, 1]_h times the sample weight w_b.  With two slots (the force)
+// slot 0 takes w = 1 and slot 1 w = E; with one slot (CG) w = u = O p, the
+// sum of the row-tile partials of mb_zu.
 __device__ inline void mb_gemm_g(const MultiBatch& mb, int blk, int force) {
   const int qi = force ? find_prob<F_G2>(mb, blk) : find_prob<F_G>(mb, blk);
   const MultiProb& q = mb.probs[qi];
@@ -207,7 +244,8 @@ __device__ inline void mb_gemm_g(const MultiBatch& mb, int blk, int force) {
   t -= split * tm * tn;
   const int chunk = ((q.B + q.nsplit - 1) / q.nsplit + MKS - 1) / MKS * MKS;
   const int k0 = split * chunk, k1 = min(q.B, k0 + chunk);
-  const double* A = (force && slot == 0) ? q.T1t : q.TWt;
+  const double* w = force ? (slot == 0 ? nullptr : q.E) : q.u;
+  const int nw = force ? 1 : tm;   // row tiles of mb_zu
   double* C = q.Gp + ((long)slot * q.nsplit + split) * M1 * N1;
   if (k0 >= k1) {
     // empty chunk: its partial is zero
@@ -217,7 +255,8 @@ __device__ inline void mb_gemm_g(const MultiBatch& mb, int blk, int force) {
     }
     return;
   }
-  tile_nt(M1, N1, k0, k1, A, q.B, q.S1t, q.B, C, N1, (t / tn) * MT, (t % tn) * MT);
+  tile_nt(M1, N1, k0, k1, q.T1t, q.B, w, nw, q.B, q.S1t, q.B, C, N1, (t / tn) * MT,
+          (t % tn) * MT);
 }
 
 // Om = (O^T 1)/B, F = (O^T E)/B - Om <E>; CG start x = 0, r = p = F,
@@ -340,8 +379,7 @@ __device__ inline void mb_dir(const MultiBatch& mb, int gblk) {
 }
 
 // the CG launches: one block per work item
-__global__ __launch_bounds__(256) void k_mb_gemm_z(MultiBatch mb) { mb_gemm_z(mb, blockIdx.x); }
-__global__ __launch_bounds__(256) void k_mb_rows(MultiBatch mb) { mb_rows(mb, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_mb_zu(MultiBatch mb) { mb_zu(mb, blockIdx.x); }
 __global__ __launch_bounds__(256) void k_mb_gemm_g(MultiBatch mb, int force) {
   mb_gemm_g(mb, blockIdx.x, force);
 }
@@ -353,7 +391,7 @@ __global__ __launch_bounds__(kMultiVecBlock) void k_mb_update(MultiBatch mb) {
 }
 __global__ __launch_bounds__(kMultiVecBlock) void k_mb_dir(MultiBatch mb) { mb_dir(mb, blockIdx.x); }
 
-// Every CG iteration of every problem in one cooperative launch: the six
+// Every CG iteration of every problem in one cooperative launch: the five
 // phases of an iteration walk their work items (the blocks of the launch
 // sequence above, same device functions, so the same results bit for bit)
 // grid-stride, with a grid barrier between phases.  Every block passes every
@@ -365,9 +403,7 @@ __global__ __launch_bounds__(256) void k_mb_cg_persist(MultiBatch mb, int iters)
   static_assert(kMultiVecBlock == 256, "one block size for every phase");
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   for (int it = 0; it < iters; it++) {
-    for (int b = blockIdx.x; b < mb.nblk_z; b += gridDim.x) mb_gemm_z(mb, b);
-    grid.sync();
-    for (int b = blockIdx.x; b < mb.nblk_rows; b += gridDim.x) mb_rows(mb, b);
+    for (int b = blockIdx.x; b < mb.nblk_z; b += gridDim.x) mb_zu(mb, b);
     grid.sync();
     for (int b = blockIdx.x; b < mb.nblk_g; b += gridDim.x) mb_gemm_g(mb, b, 0);
     grid.sync();
@@ -431,19 +467,17 @@ __global__ void k_mb_gather(MultiBatch mb, double* out) {
 }  // namespace
 
 void multi_layout(MultiProb* probs, int n, MultiBatch& mb) {
-  int prep = 0, z = 0, rows = 0, g = 0, g2 = 0, vec = 0;
+  int prep = 0, z = 0, g = 0, g2 = 0, vec = 0;
   for (int i = 0; i < n; i++) {
     MultiProb& q = probs[i];
     const int M1 = q.M + 1, N1 = q.N + 1;
     q.blk_prep = prep;
     q.blk_z = z;
-    q.blk_rows = rows;
     q.blk_g = g;
     q.blk_g2 = g2;
     q.blk_vec = vec;
     prep += (q.B + 255) / 256;
     z += ((M1 + MT - 1) / MT) * ((q.B + MT - 1) / MT);
-    rows += (q.B + 63) / 64;
     const int gt = ((M1 + MT - 1) / MT) * ((N1 + MT - 1) / MT) * q.nsplit;
     g += gt;
     g2 += 2 * gt;
@@ -452,7 +486,6 @@ void multi_layout(MultiProb* probs, int n, MultiBatch& mb) {
   mb.n = n;
   mb.nblk_prep = prep;
   mb.nblk_z = z;
-  mb.nblk_rows = rows;
   mb.nblk_g = g;
   mb.nblk_g2 = g2;
   mb.nblk_vec = vec;
@@ -483,11 +516,7 @@ int launch_multi_cg_iter(const MultiBatch& mb, int matvec_only, hipStream_t st) 
   if (mb.n <= 0) return 0;
   {
     KScope ks("multi_gemm_z", st);
-    hipLaunchKernelGGL(k_mb_gemm_z, dim3((unsigned)mb.nblk_z), dim3(256), 0, st, mb);
-  }
-  {
-    KScope ks("multi_rows", st);
-    hipLaunchKernelGGL(k_mb_rows, dim3((unsigned)mb.nblk_rows), dim3(256), 0, st, mb);
+    hipLaunchKernelGGL(k_mb_zu, dim3((unsigned)mb.nblk_z), dim3(256), 0, st, mb);
   }
   {
     KScope ks("multi_gemm_g", st);
@@ -502,7 +531,7 @@ int launch_multi_cg_iter(const MultiBatch& mb, int matvec_only, hipStream_t st) 
       hipLaunchKernelGGL(k_mb_dir, dim3((unsigned)mb.nblk_vec), dim3(kMultiVecBlock), 0, st, mb);
     }
   }
-  count_launch(matvec_only ? 4 : 6);
+  count_launch(matvec_only ? 3 : 5);
   NQS_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -524,7 +553,6 @@ int multi_persist_grid(const MultiBatch& mb) {
     }
   }
   int need = mb.nblk_z;
-  if (mb.nblk_rows > need) need = mb.nblk_rows;
   if (mb.nblk_g > need) need = mb.nblk_g;
   if (mb.nblk_vec > need) need = mb.nblk_vec;
   return need < cap ? need : cap;

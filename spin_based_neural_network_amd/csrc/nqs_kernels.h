@@ -458,8 +458,8 @@ struct MultiProb {
   double* S1 = nullptr;          // [B][N+1]   [s_b, 1]
   double* S1t = nullptr;         // [N+1][B]
   double* T1t = nullptr;         // [M+1][B]   [t_b, 1]^T
-  double* Zt = nullptr;          // [M+1][B]   V [s_b, 1]^T
-  double* TWt = nullptr;         // [2][M+1][B] weighted [t_b, 1]^T
+  double* u = nullptr;           // [(M+64)/64][B] row weights of the CG product,
+                                 // (O p)_b, in partial sums over 64-row tiles
   double* Gp = nullptr;          // [2][nsplit][M+1][N+1] split-K partials
   double *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr, *Om = nullptr,
          *F = nullptr;           // padded vectors
@@ -472,12 +472,12 @@ struct MultiProb {
   const int* range_flag = nullptr;  // the ansatz tables' range verdict (nonzero: refused)
   // first block of this problem in each batched launch
   // (blk_g: one G product per problem, CG; blk_g2: two, the force)
-  int blk_prep = 0, blk_z = 0, blk_rows = 0, blk_g = 0, blk_g2 = 0, blk_vec = 0;
+  int blk_prep = 0, blk_z = 0, blk_g = 0, blk_g2 = 0, blk_vec = 0;
 };
 struct MultiBatch {
   int n = 0;
   const MultiProb* probs = nullptr;   // device copy
-  int nblk_prep = 0, nblk_z = 0, nblk_rows = 0, nblk_g = 0, nblk_g2 = 0, nblk_vec = 0;
+  int nblk_prep = 0, nblk_z = 0, nblk_g = 0, nblk_g2 = 0, nblk_vec = 0;
 };
 // fills the problems' first-block offsets of every launch and the totals
 void multi_layout(MultiProb* probs, int n, MultiBatch& mb);
