@@ -136,6 +136,8 @@ def setup(lib, w, args, move=None):
             opt.ar_local = args.ar_local
         if getattr(args, "ar_epochs", None) is not None:
             opt.ar_epochs = args.ar_epochs
+        if getattr(args, "ar_batch", None) is not None:
+            opt.ar_batch = args.ar_batch
     s = lib.nqs_hip_sampler_create_ex(N, ctypes.byref(cfg), ctypes.byref(opt), fn, user)
     if not s:
         raise RuntimeError("nqs_hip_sampler_create_ex failed")
@@ -809,6 +811,8 @@ def main():
                     help="learned proposal: exchange moves before each learned move")
     ap.add_argument("--ar-epochs", type=int, default=None, metavar="E",
                     help="learned proposal: passes over the window per retraining")
+    ap.add_argument("--ar-batch", type=int, default=None, metavar="B",
+                    help="learned proposal: training minibatch size (default 2048)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of "
