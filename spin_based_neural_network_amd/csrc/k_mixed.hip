@@ -568,9 +568,11 @@ int launch_oct_factored(const int* spins, int B, int N, int M, const float* U1, 
   if (B <= 0 || P <= 0) return 0;
   const long Pr = (long)N + M + (long)M * N;
   if (P != (U2 ? 2 * Pr : Pr) || ldk < B || N + (U2 ? 2 : 1) * M > kOctRows) return -1;
-  // up to 8 hidden-unit slices per 64 samples (more blocks in flight); one
-  // block maximum each, fewer than x3_max_elems(B, P, ldk, 1)
-  const int gy = M < 8 ? (M > 0 ? M : 1) : 8;
+  // up to 8 hidden-unit slices per 64 samples (more blocks in flight), and
+  // at most (P + 63) / 64 of them: one block maximum each, within the
+  // x3_max_elems(B, P, ldk, 1) = ((P + 63) / 64) ((ldk + 63) / 64) + 1 floats
+  int gy = M < 8 ? (M > 0 ? M : 1) : 8;
+  if (gy > (P + 63) / 64) gy = (int)((P + 63) / 64);
   const dim3 grid((unsigned)((ldk + 63) / 64), (unsigned)gy);
   const long nb = (long)grid.x * grid.y;
   KScope ks("gen_O", st);

@@ -113,7 +113,7 @@ def test_mixed_parameter_space_update(hip, orc, shift):
     assert cos > 0.99999
 
 
-@pytest.mark.parametrize("ansatz,M,B", [(CRBM, 96, 4096), (5, 40, 1000)])
+@pytest.mark.parametrize("ansatz,M,B", [(CRBM, 96, 4096), (5, 40, 1000), (5, 3, 200)])
 def test_mixed_parameter_space_without_o_matrix(hip, orc, monkeypatch, ansatz, M, B):
     """The mixed parameter-space solve of an RBM takes O_c^T (FP32) from the
     spins and factors and the means and force from FP64 factor sums, without
@@ -121,7 +121,8 @@ def test_mixed_parameter_space_without_o_matrix(hip, orc, monkeypatch, ansatz, M
     samples (same FP32 entries up to the rounding of the column means, so
     the updates agree far inside the mixed mode's bound), and against the
     FP64 update.  Cases: the C5 complex RBM and a real RBM with B = 1000
-    (samples padded to 1024)."""
+    (samples padded to 1024), and a small real RBM (P = 99, fewer parameter
+    rows than the generator's hidden-unit slices)."""
     a, P, spins, e, Oc = _setup(hip, orc, 2, 4, M, B, 905, ansatz=ansatz)
     shift = 1e-2
     print(f"[mixed] parameter space without O: ansatz {ansatz} P={P} B={B}", flush=True)
