@@ -529,8 +529,9 @@ def gpu_run_multi(args, w):
     lib.nqs_hip_multi_step(m, infos)          # first step: autotune
     tg, us = (ctypes.c_int * 8)(), (ctypes.c_double * 8)()
     chosen = lib.nqs_hip_multi_tuning(m, tg, us, 8)
-    log("[bench] autotune (G tiles x splits target: CG product us): " +
-        ", ".join(f"{tg[i]}: {us[i]:.1f}" for i in range(6)) + f" -> {chosen}")
+    ntune = sum(1 for i in range(8) if us[i] > 0)
+    log("[bench] autotune (G tiles x splits target: CG product us; 0 = persistent launch): " +
+        ", ".join(f"{tg[i]}: {us[i]:.1f}" for i in range(ntune)) + f" -> {chosen}")
     # one step launch by launch (CG as one launch per phase and iteration,
     # the timed configuration) for the per-kernel split, then one with the
     # CG in the persistent cooperative launch for comparison
@@ -622,7 +623,7 @@ def gpu_run_multi(args, w):
                 ess_per_s_iteration=ess_total / t_total, sample_ms=1e3 * t_sample / args.steps,
                 ms_per_step=ms_step, taus=taus,
                 iters=iters, energies=energies, inst=inst, convergence=conv, chosen=chosen,
-                tuning={int(tg[i]): float(us[i]) for i in range(6)},
+                tuning={int(tg[i]): float(us[i]) for i in range(ntune)},
                 roofline=dict(kernel=dom, symbol=SYMBOL.get(dom), bound=bound, achieved=achieved,
                               peak=peak, unit=runit, frac=achieved / peak, traffic=None,
                               avg_ms=avg_ms),

@@ -187,16 +187,17 @@ int nqs_hip_sampler_learned_stats(const nqs_sampler_t *s, double *acceptance,
  * run: thermalizes each sampler once, then num_iterations steps;
  * out_traces [n][num_iterations] receives the mean energies (may be NULL).
  * tuning: the split-factor targets timed at the first step and their CG
- * product times (microseconds); returns the chosen target.
+ * product times (microseconds), then (target 0) the persistent launch's time
+ * for the same product at the chosen split; returns the chosen target.
  * use_graph(m, 0): later steps are issued launch by launch instead of as
  * the captured graph (same kernels and results; for per-kernel timing).
  * use_persistent(m, on): the CG iterations of all problems in one
- * cooperative launch with grid barriers between phases (on; or
- * NQS_HIP_MULTI_PERSIST=1 at creation), or one launch per phase and
- * iteration (off, the default: on the MI355X the six grid barriers per CG
- * iteration cost more than the captured graph's launch gaps); same results
- * bit for bit.  A device that refuses the cooperative launch gets the launch
- * sequence, with a message.
+ * cooperative launch with grid barriers between phases (on), or one launch
+ * per phase and iteration (off); same results bit for bit.  Without this
+ * call or NQS_HIP_MULTI_PERSIST=0/1 at creation the first step's autotuning
+ * times both and keeps the faster (on the MI355X the launch sequence: the
+ * grid barriers cost more than the captured graph's launch gaps).  A device
+ * that refuses the cooperative launch gets the launch sequence.
  * last_energies: the local energies of problem i's last batch, sample
  * r*C + c in slot r*C + c (chain c, round r); returns the count. */
 typedef struct nqs_hip_multi nqs_hip_multi_t;

@@ -4390,11 +4390,11 @@ struct nqs_hip_multi {
   double* h_out = nullptr;              // pinned, kMultiOut per problem
   long long steps_done = 0;
   int tuned = 0, use_graph = 1;
-  // the CG iterations in one cooperative launch (NQS_HIP_MULTI_PERSIST=1 at
-  // creation, or nqs_hip_multi_use_persistent), the captured graph then
-  // holding the step up to the CG start; off by default: its six grid
-  // barriers per CG iteration cost more than the graph's launch gaps
-  int persist = 0;
+  // the CG iterations in one cooperative launch, the captured graph then
+  // holding the step up to the CG start: chosen by the first step's
+  // autotuning against the launch sequence, unless NQS_HIP_MULTI_PERSIST at
+  // creation or nqs_hip_multi_use_persistent chose (persist_set)
+  int persist = 0, persist_set = 0;
   // Metropolis jobs of all problems, grouped by kernel variant:
   // (variant, first job, jobs, blocks)
   std::vector<nqs::MHJob> jobs;
@@ -4605,9 +4605,26 @@ int multi_autotune(nqs_hip_multi* m) {
       best_t = t;
     }
   }
+  CKR(multi_set_split(m, best_t));
+  // launch shape: the persistent cooperative launch against the launch
+  // sequence on the same product at the chosen split (bit-equal results, so
+  // the choice is timing only), unless the caller or NQS_HIP_MULTI_PERSIST
+  // chose; recorded in the tuning list as target 0
+  if (!m->persist_set && nqs::launch_multi_cg_persist(m->mb, 1, g_stream, 1) == 0) {
+    CK(hipEventRecord(e0, g_stream));
+    const int reps = 5;
+    for (int r = 0; r < reps; r++) CKR(nqs::launch_multi_cg_persist(m->mb, 1, g_stream, 1));
+    CK(hipEventRecord(e1, g_stream));
+    CK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CK(hipEventElapsedTime(&ms, e0, e1));
+    const double us = 1e3 * ms / 5;
+    m->tuning.push_back({0, us});
+    m->persist = us < best ? 1 : 0;
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  return multi_set_split(m, best_t);
+  return 0;
 }
 
 void multi_fill_info(nqs_hip_multi* m, nqs_sr_step_info_t* infos) {
@@ -4698,6 +4715,7 @@ nqs_hip_multi* multi_new(int n, const nqs_config_t* cfgs, const int* size_x, con
   {
     const char* e = getenv("NQS_HIP_MULTI_PERSIST");
     m->persist = (e && e[0] == '1') ? 1 : 0;
+    m->persist_set = (e && (e[0] == '0' || e[0] == '1')) ? 1 : 0;
   }
   m->p.resize(n);
   m->mp.resize(n);
@@ -4938,6 +4956,7 @@ NQS_EXPORT int nqs_hip_multi_use_persistent(nqs_hip_multi_t* m, int on) {
     m->graph = nullptr;
   }
   m->persist = on ? 1 : 0;
+  m->persist_set = 1;
   return 0;
 }
 

@@ -399,7 +399,9 @@ __global__ __launch_bounds__(kMultiVecBlock) void k_mb_dir(MultiBatch mb) { mb_d
 // together when, after an iteration's last barrier, every problem reads done
 // (its flags were written before that barrier).  The launch is cooperative,
 // so it either has all its blocks resident or does not start.
-__global__ __launch_bounds__(256) void k_mb_cg_persist(MultiBatch mb, int iters) {
+// matvec_only: one iteration's A p, p.Ap and alpha only (idempotent at the
+// CG start; the autotuner times it against the launch sequence's).
+__global__ __launch_bounds__(256) void k_mb_cg_persist(MultiBatch mb, int iters, int matvec_only) {
   static_assert(kMultiVecBlock == 256, "one block size for every phase");
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   for (int it = 0; it < iters; it++) {
@@ -408,6 +410,7 @@ __global__ __launch_bounds__(256) void k_mb_cg_persist(MultiBatch mb, int iters)
     for (int b = blockIdx.x; b < mb.nblk_g; b += gridDim.x) mb_gemm_g(mb, b, 0);
     grid.sync();
     for (int b = blockIdx.x; b < mb.nblk_vec; b += gridDim.x) mb_finish(mb, b);
+    if (matvec_only) break;   // uniform: a kernel argument
     grid.sync();
     for (int b = blockIdx.x; b < mb.nblk_vec; b += gridDim.x) mb_update(mb, b);
     grid.sync();
@@ -558,7 +561,7 @@ int multi_persist_grid(const MultiBatch& mb) {
   return need < cap ? need : cap;
 }
 
-int launch_multi_cg_persist(const MultiBatch& mb, int iters, hipStream_t st) {
+int launch_multi_cg_persist(const MultiBatch& mb, int iters, hipStream_t st, int matvec_only) {
   if (mb.n <= 0 || iters <= 0) return 0;
   const int grid = multi_persist_grid(mb);
   if (grid <= 0) {
@@ -566,8 +569,8 @@ int launch_multi_cg_persist(const MultiBatch& mb, int iters, hipStream_t st) {
     return -1;
   }
   MultiBatch a = mb;
-  int it = iters;
-  void* args[] = {&a, &it};
+  int it = matvec_only ? 1 : iters, mv = matvec_only ? 1 : 0;
+  void* args[] = {&a, &it, &mv};
   KScope ks("multi_cg", st);
   const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_mb_cg_persist),
                                                   dim3((unsigned)grid), dim3(256), args, 0, st);

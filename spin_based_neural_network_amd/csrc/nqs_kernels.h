@@ -490,9 +490,10 @@ int launch_multi_force(const MultiBatch& mb, hipStream_t st);
 // CG start, used to time the split factors)
 int launch_multi_cg_iter(const MultiBatch& mb, int matvec_only, hipStream_t st);
 // up to `iters` CG iterations of every problem in one cooperative launch
-// (same results as as many launch_multi_cg_iter calls); -1 when the device
+// (same results as as many launch_multi_cg_iter calls; matvec_only: one
+// iteration's A p and alpha, as launch_multi_cg_iter's); -1 when the device
 // refuses the cooperative launch
-int launch_multi_cg_persist(const MultiBatch& mb, int iters, hipStream_t st);
+int launch_multi_cg_persist(const MultiBatch& mb, int iters, hipStream_t st, int matvec_only = 0);
 // par -= lr * x (reference layout), |x|^2
 int launch_multi_apply(const MultiBatch& mb, hipStream_t st);
 // per problem kMultiOut doubles: <E>, <E^2>, |delta|^2, CG iterations,
