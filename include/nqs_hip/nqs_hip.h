@@ -101,7 +101,12 @@ typedef struct {
  * minibatch ar_batch, rate ar_lr): first as soon as the window holds
  * samples, then every ar_train_every batches.  A batch before the first
  * training uses exchange moves (num_decorrelate per round); afterwards each
- * round is ar_local exchange moves followed by one learned move. */
+ * round is ar_local exchange moves followed by one learned move.  A due
+ * retraining starts right after the batch before it, on a second stream,
+ * so it runs beside the caller's work between batches (the SR solve) with
+ * the same window and the same result; the next use of the network waits
+ * for it.  After a sampler's last batch it may thus hold one training more
+ * than batches used.  NQS_HIP_AR_OVERLAP=0 trains at the batch start. */
 nqs_hip_options_t nqs_hip_options_defaults(void);
 
 /* nqs_sampler_create with options.  NULL on bad args or unsupported

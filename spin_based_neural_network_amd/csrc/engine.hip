@@ -975,7 +975,15 @@ struct nqs_hip_ar {
   long t = 0;                                        // Adam steps taken
   uint64_t shuffle = 0x243F6A8885A308D3ULL;          // epoch offsets
   long loss_n = 0, loss_avail = 0;                   // last training's loss partials
+  // a training launched on the side stream after a sampler's batch (its
+  // end event; pending until the library's stream has waited for it)
+  hipEvent_t ev_trained = nullptr;
+  int pending = 0;
   ~nqs_hip_ar() {
+    if (ev_trained) {
+      (void)hipEventSynchronize(ev_trained);
+      (void)hipEventDestroy(ev_trained);
+    }
     DBuf* all[] = {&w, &m, &v, &op1f, &op2f, &op2b, &op3f, &op3b, &w1t, &b1, &b2, &b3,
                    &partial, &loss, &bits, &lq, &rng, &spins, &grad};
     for (DBuf* b : all) b->release();
@@ -1025,6 +1033,8 @@ struct nqs_sampler {
   void* solo_samples = nullptr;
   ~nqs_sampler() {
     multi_delete(solo);
+    // a training launched ahead reads the window: it ends first
+    if (ar && ar->pending && ar->ev_trained) (void)hipEventSynchronize(ar->ev_trained);
     DBuf* all[] = {&d_spins, &d_rng, &d_acc, &d_tot, &d_la, &d_cur, &d_samples,
                    &win, &lacc, &prop_bits, &prop_spins, &lq, &la_prop};
     for (DBuf* b : all) b->release();
@@ -1127,10 +1137,22 @@ int ar_last_loss(nqs_hip_ar* a, double* mean_loss) {
   return 0;
 }
 
-// Adam epochs over n_avail device bit rows; mean loss of the last epoch
-// into *mean_loss when it is given (a host synchronisation).
+// the library's stream waits for a training still running on the side
+// stream (every use of the network's device state comes after this)
+int ar_join(nqs_hip_ar* a) {
+  if (a && a->pending) {
+    CK(hipStreamWaitEvent(g_stream, a->ev_trained, 0));
+    a->pending = 0;
+  }
+  return 0;
+}
+
+// Adam epochs over n_avail device bit rows, on stream strm; mean loss of the
+// last epoch into *mean_loss when it is given (a host synchronisation).
 int ar_train(nqs_hip_ar* a, const uint64_t* rows, long n_avail, int constrained, int epochs,
-             int batch, double lr, double* mean_loss) {
+             int batch, double lr, double* mean_loss, hipStream_t strm = nullptr) {
+  if (!strm) strm = g_stream;
+  CKR(ar_join(a));
   if (n_avail <= 0 || epochs <= 0) return 0;
   if (batch < 1) batch = 1;
   const int nb = nqs::ar_grad_blocks(batch);
@@ -1138,7 +1160,7 @@ int ar_train(nqs_hip_ar* a, const uint64_t* rows, long n_avail, int constrained,
   CKR(a->partial.ensure(sizeof(float) * (size_t)nb * a->d.P));
   CKR(a->loss.ensure(sizeof(double) * (size_t)nb * steps));
   for (int e = 0; e < epochs; e++) {
-    if (e == epochs - 1) CKR(nqs::launch_fill(a->loss.as<double>(), 0.0, (long)nb * steps, g_stream));
+    if (e == epochs - 1) CKR(nqs::launch_fill(a->loss.as<double>(), 0.0, (long)nb * steps, strm));
     const long off = (long)(nqs::xs_next(a->shuffle) % (uint64_t)n_avail);
     for (long st = 0; st < steps; st++) {
       const int count = (int)std::min<long>(batch, n_avail - st * batch);
@@ -1151,7 +1173,7 @@ int ar_train(nqs_hip_ar* a, const uint64_t* rows, long n_avail, int constrained,
       g.scale = 1.0f / (float)count;
       g.partial = a->partial.as<float>();
       g.loss = a->loss.as<double>() + st * nb;
-      CKR(nqs::launch_ar_grad(a->dev, g, g_stream));
+      CKR(nqs::launch_ar_grad(a->dev, g, strm));
       a->t++;
       nqs::ArAdam ad;
       ad.partial = a->partial.as<float>();
@@ -1159,13 +1181,26 @@ int ar_train(nqs_hip_ar* a, const uint64_t* rows, long n_avail, int constrained,
       ad.lr = (float)lr;
       ad.bc1 = (float)(1.0 - pow(0.9, (double)a->t));
       ad.bc2 = (float)(1.0 - pow(0.999, (double)a->t));
-      CKR(nqs::launch_ar_adam(a->dev, ad, g_stream));
+      CKR(nqs::launch_ar_adam(a->dev, ad, strm));
     }
   }
   a->loss_n = (long)nb * steps;
   a->loss_avail = n_avail;
-  if (mean_loss) CKR(ar_last_loss(a, mean_loss));
+  if (mean_loss) {
+    if (strm != g_stream) return -1;
+    CKR(ar_last_loss(a, mean_loss));
+  }
   return 0;
+}
+
+// the side stream of ahead-of-time trainings (default priority)
+hipStream_t g_train_stream = nullptr;
+
+// NQS_HIP_AR_OVERLAP=0: a due retraining runs at the start of the batch
+// that needs it instead of beside the work that follows the previous batch
+bool ar_overlap_on() {
+  const char* e = getenv("NQS_HIP_AR_OVERLAP");
+  return !(e && e[0] == '0');
 }
 
 // retrain before a batch when due: first once the window holds samples,
@@ -1190,6 +1225,31 @@ int ar_window_append(nqs_sampler* s, int n_out, const int* d_out) {
   s->win_pos = (s->win_pos + k) % cap;
   s->win_n = std::min(cap, s->win_n + k);
   s->since_train++;
+  return 0;
+}
+
+// The retraining that the next batch would run at its start (same window,
+// same network, same Adam steps: the same result) starts now on the side
+// stream, beside whatever the caller does before that batch (the SR solve);
+// the next use of the network waits for it (ar_join).
+int ar_train_ahead(nqs_sampler* s) {
+  const int every = std::max(1, s->opt.ar_train_every);
+  const bool due = s->trained ? (s->since_train >= every) : (s->win_n > 0);
+  if (!due) return 0;
+  nqs_hip_ar* a = s->ar;
+  if (!g_train_stream) CK(hipStreamCreateWithFlags(&g_train_stream, hipStreamNonBlocking));
+  if (!a->ev_trained) CK(hipEventCreateWithFlags(&a->ev_trained, hipEventDisableTiming));
+  CKR(ar_join(a));
+  // the window as this batch left it
+  CK(hipEventRecord(a->ev_trained, g_stream));
+  CK(hipStreamWaitEvent(g_train_stream, a->ev_trained, 0));
+  CKR(ar_train(a, s->win.as<uint64_t>(), s->win_n, 1, s->opt.ar_epochs, s->opt.ar_batch,
+               s->opt.ar_lr, nullptr, g_train_stream));
+  CK(hipEventRecord(a->ev_trained, g_train_stream));
+  a->pending = 1;
+  s->trained = 1;
+  s->since_train = 0;
+  s->n_train++;
   return 0;
 }
 
@@ -1309,6 +1369,7 @@ int sampler_advance(nqs_sampler* s, int steps, int rounds, int n_out, int* d_out
     CKR(ensure_model_for_sampler(s));
     const bool learned_batch = s->ar && d_out;
     if (learned_batch) CKR(ar_maybe_train(s));
+    if (s->ar) CKR(ar_join(s->ar));
     if (learned_batch && s->trained) {
       if (low_prec(s->opt.precision)) CKR(learned_rounds<float>(s, rounds, n_out, d_out));
       else CKR(learned_rounds<double>(s, rounds, n_out, d_out));
@@ -1330,7 +1391,10 @@ int sampler_advance(nqs_sampler* s, int steps, int rounds, int n_out, int* d_out
         CKR(nqs::launch_mh<double>(v, s->cs, ml, g_stream));
       }
     }
-    if (learned_batch) CKR(ar_window_append(s, n_out, d_out));
+    if (learned_batch) {
+      CKR(ar_window_append(s, n_out, d_out));
+      if (ar_overlap_on()) CKR(ar_train_ahead(s));
+    }
     if (h_out && d_out) {
       CK(hipMemcpyAsync(h_out, d_out, sizeof(int) * (size_t)n_out * s->N, hipMemcpyDeviceToHost,
                         g_stream));
@@ -4197,6 +4261,7 @@ NQS_EXPORT long nqs_hip_ar_num_params(const nqs_hip_ar_t* a) { return a ? a->d.P
 NQS_EXPORT int nqs_hip_ar_get_params(nqs_hip_ar_t* a, float* out) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (!a || !out) return -1;
+  CKR(ar_join(a));
   CK(hipMemcpyAsync(out, a->w.p, sizeof(float) * a->d.P, hipMemcpyDeviceToHost, g_stream));
   CK(hipStreamSynchronize(g_stream));
   return 0;
@@ -4205,6 +4270,7 @@ NQS_EXPORT int nqs_hip_ar_get_params(nqs_hip_ar_t* a, float* out) {
 NQS_EXPORT int nqs_hip_ar_set_params(nqs_hip_ar_t* a, const float* in) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (!a || !in) return -1;
+  CKR(ar_join(a));
   CKR(ar_set(a, in));
   CK(hipStreamSynchronize(g_stream));
   return 0;
@@ -4214,6 +4280,7 @@ NQS_EXPORT int nqs_hip_ar_log_prob(nqs_hip_ar_t* a, const int* spins, int B, int
                                    double* out) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (!a || !spins || !out || B <= 0) return -1;
+  CKR(ar_join(a));
   uint64_t* bits = nullptr;
   CKR(ar_rows_to_device(a, spins, B, bits));
   CKR(a->lq.ensure(sizeof(double) * B));
@@ -4233,6 +4300,7 @@ NQS_EXPORT int nqs_hip_ar_sample(nqs_hip_ar_t* a, const int* ref_spins, int B, i
                                  const unsigned long long* seeds, int* out_spins, double* out_logq) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (!a || !ref_spins || !seeds || !out_spins || B <= 0) return -1;
+  CKR(ar_join(a));
   std::vector<int> ref((size_t)B * a->N);
   for (int b = 0; b < B; b++) memcpy(ref.data() + (size_t)b * a->N, ref_spins, sizeof(int) * a->N);
   uint64_t* bits = nullptr;
@@ -4264,6 +4332,7 @@ NQS_EXPORT int nqs_hip_ar_gradient(nqs_hip_ar_t* a, const int* spins, int B, int
                                    float* grad, double* mean_nll) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (!a || !spins || !grad || B <= 0) return -1;
+  CKR(ar_join(a));
   uint64_t* bits = nullptr;
   CKR(ar_rows_to_device(a, spins, B, bits));
   const int nb = nqs::ar_grad_blocks(B);

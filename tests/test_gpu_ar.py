@@ -206,6 +206,38 @@ def _exact_sector_probs(hip, a, N):
     return space, p / p.sum()
 
 
+def test_training_ahead_gives_the_same_chains(hip, monkeypatch):
+    """A due retraining launched right after a batch on the side stream
+    (beside the caller's work) instead of at the start of the next batch:
+    same window, same network, same Adam steps, so identical samples and
+    statistics batch after batch (three retrainings used, the first one as
+    soon as the window holds samples).  After the last batch the ahead
+    schedule has already run the next due retraining, which the other runs
+    only if another batch comes: one training more, and its network."""
+    N, C = 8, 512
+    runs = {}
+    for ov in ("0", "1"):
+        monkeypatch.setenv("NQS_HIP_AR_OVERLAP", ov)
+        cfg, a, s = _heis_chain_setup(hip, N, C, 91, 0.3, learned=True, train_every=3)
+        hip.nqs_sampler_thermalize(s)
+        outs = []
+        for _ in range(10):
+            out = np.zeros((C, N), dtype=np.int32)
+            assert hip.nqs_sampler_batch(s, C, ptr(out)) == 0
+            outs.append(out)
+        acc, props, ntr, nll = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_int(), ctypes.c_double()
+        assert hip.nqs_hip_sampler_learned_stats(s, ctypes.byref(acc), ctypes.byref(props),
+                                                 ctypes.byref(ntr), ctypes.byref(nll)) == 0
+        runs[ov] = (outs, acc.value, props.value, ntr.value)
+        hip.nqs_sampler_free(s)
+        hip.nqs_ansatz_free(a)
+    (o0, a0, p0, t0), (o1, a1, p1, t1) = runs["0"], runs["1"]
+    print(f"trainings {t0}, acceptance {a0:.4f}", flush=True)
+    assert t0 >= 3 and t1 == t0 + 1 and p0 == p1 and a0 == a1
+    for b0, b1 in zip(o0, o1):
+        np.testing.assert_array_equal(b0, b1)
+
+
 def test_learned_sampler_exact_stationary_distribution(hip):
     # independent chains: the chi-square uses one state per chain, taken
     # after many learned moves, so chain autocorrelation cannot inflate it
