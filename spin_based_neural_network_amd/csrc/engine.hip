@@ -1839,6 +1839,13 @@ bool factored_dual_on(const nqs_ansatz* a) {
   return a->N + (a->kind == NQS_ANSATZ_COMPLEX_RBM ? 2 : 1) * a->M <= 256;
 }
 
+// NQS_HIP_GRAM_SPLIT=0: the mixed parameter-space S is formed in one
+// product before the factorisation starts (read at each step)
+bool gram_split_on() {
+  const char* e = getenv("NQS_HIP_GRAM_SPLIT");
+  return !(e && e[0] == '0');
+}
+
 int minsr_core(const MinsrArgs& m, int& converged) {
   nqs_ansatz* a = m.a;
   const int B = m.B, N = a->N;
@@ -1903,7 +1910,38 @@ int minsr_core(const MinsrArgs& m, int& converged) {
                                 g_stream));
       return nqs::launch_add_diag(out, (int)P, P, eps, g_stream);
     };
-    CKR(gram_param(T));
+    // The Cholesky's first wide panel reads only columns [0, w) of S: those
+    // are formed first, and the trailing block of S on the factorisation's
+    // second stream, beside that panel (the same sums per element;
+    // NQS_HIP_GRAM_SPLIT=0: one product, then the factorisation)
+    const int w0 = nqs::potrf_first_width((int)P);
+    hipEvent_t gram_done = nullptr;
+    if (m.mixed && w0 < P && gram_split_on()) {
+      static hipEvent_t ev_cols = nullptr, ev_rest = nullptr;
+      if (!ev_cols) CK(hipEventCreateWithFlags(&ev_cols, hipEventDisableTiming));
+      if (!ev_rest) CK(hipEventCreateWithFlags(&ev_rest, hipEventDisableTiming));
+      hipStream_t side = nqs::potrf_side_stream();
+      if (!side) return -1;
+      const long Pr = P - w0;
+      float* part = g_ws.x3part.as<float>();
+      const float* mx = g_ws.x3max.as<float>();
+      const size_t p1 = nqs::x3_part_elems(w0, w0, Kp), p2 = nqs::x3_part_elems((int)Pr, w0, Kp);
+      CKR(nqs::launch_gemm_x3(w0, w0, (int)Kp, 1.0 / B, OT32, Kp, OT32, Kp, mx, mx, T, P, 1, part,
+                              g_stream));
+      CKR(nqs::launch_gemm_x3((int)Pr, w0, (int)Kp, 1.0 / B, OT32 + (size_t)w0 * Kp, Kp, OT32, Kp, mx,
+                              mx, T + (size_t)w0 * P, P, 0, part + p1, g_stream));
+      CKR(nqs::launch_add_diag(T, w0, P, eps, g_stream));
+      CK(hipEventRecord(ev_cols, g_stream));
+      CK(hipStreamWaitEvent(side, ev_cols, 0));
+      double* Tr = T + (size_t)w0 * P + w0;
+      CKR(nqs::launch_gemm_x3((int)Pr, (int)Pr, (int)Kp, 1.0 / B, OT32 + (size_t)w0 * Kp, Kp,
+                              OT32 + (size_t)w0 * Kp, Kp, mx, mx, Tr, P, 1, part + p1 + p2, side));
+      CKR(nqs::launch_add_diag(Tr, (int)Pr, P, eps, side));
+      CK(hipEventRecord(ev_rest, side));
+      gram_done = ev_rest;
+    } else {
+      CKR(gram_param(T));
+    }
     if (m.fdual) {
       // O_c^T eps / Ns = O^T (eps - <eps>) / Ns from the factors
       double* uc = g_ws.u2.as<double>();
@@ -1918,7 +1956,7 @@ int minsr_core(const MinsrArgs& m, int& converged) {
     CKR(g_ws.linv.ensure(sizeof(double) * nqs::potrf_linv_elems((int)P)));
     CKR(g_ws.y.ensure(sizeof(double) * 2 * (size_t)P));
     double* Linv = g_ws.linv.as<double>();
-    int f = nqs::launch_potrf(T, (int)P, Linv, g_ws.fail.as<int>(), g_stream);
+    int f = nqs::launch_potrf(T, (int)P, Linv, g_ws.fail.as<int>(), g_stream, gram_done);
     if (f < 0) return -1;
     if (f > 0) {
       // as in the sample-space solve: the partially factored matrix with

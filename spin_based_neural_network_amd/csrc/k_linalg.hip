@@ -1602,13 +1602,27 @@ static int panel_width(int rem) {
   return rules.back().first;
 }
 
-int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st) {
+int potrf_first_width(int n) {
+  const int w = panel_width(n);
+  return w > n ? n : w;
+}
+
+hipStream_t potrf_side_stream() {
+  if (!g_look && hipStreamCreateWithFlags(&g_look, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  return g_look;
+}
+
+int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st,
+                 hipEvent_t trailing) {
   if (!g_look) NQS_HIP_CHECK(hipStreamCreateWithFlags(&g_look, hipStreamNonBlocking));
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, fail_dev);
   count_launch();
   int K0 = 0, w = panel_width(n);
   if (w > n) w = n;
   if (wide_panel(A, n, 0, w, Linv, fail_dev, st, panel_rows_all() ? panel_rows() : 1)) return -1;
+  // everything after the first wide panel reads or updates the trailing
+  // part of A
+  if (trailing) NQS_HIP_CHECK(hipStreamWaitEvent(st, trailing, 0));
   // NQS_HIP_POTRF_LOOKAHEAD=0: the same schedule on one stream
   static const int lookahead = [] {
     const char* e = getenv("NQS_HIP_POTRF_LOOKAHEAD");

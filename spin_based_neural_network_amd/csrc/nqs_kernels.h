@@ -375,7 +375,15 @@ int launch_gemm_nt(int M, int N, int K, double alpha, const double* A, long lda,
 // inverses of the 64 x 64 diagonal blocks; it also holds launch_potrs's
 // super-block inverses (potrf_linv_elems(n) doubles in all).
 size_t potrf_linv_elems(int n);
-int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st);
+// trailing: an event after which rows and columns >= potrf_first_width(n)
+// of A hold their input (nullptr: all of A does on entry).  The first wide
+// panel, columns [0, potrf_first_width(n)), is factored before the wait.
+int launch_potrf(double* A, int n, double* Linv, int* fail_dev, hipStream_t st,
+                 hipEvent_t trailing = nullptr);
+int potrf_first_width(int n);
+// the factorisation's second stream (created on first use): work issued on
+// it ahead of launch_potrf runs before the lookahead updates
+hipStream_t potrf_side_stream();
 // After a failure at column jf: bring A to the state the unblocked
 // left-looking factorisation leaves (columns < jf hold L, columns >= jf
 // their input values, taken from the lower triangle of orig).

@@ -137,3 +137,27 @@ def test_mixed_parameter_space_without_o_matrix(hip, orc, monkeypatch, ansatz, M
     print(f"[mixed] |d(no O) - d(O)| / |d| = {rel:.3e}; against FP64 {rel64:.3e}", flush=True)
     assert rel < 1e-5
     assert rel64 < 1e-4
+
+
+@pytest.mark.parametrize("ansatz,M,B,split_dual", [(CRBM, 96, 4096, "1"), (5, 40, 1000, "1"),
+                                                   (5, 40, 1000, "0")])
+def test_mixed_gram_split_bit_equal(hip, orc, monkeypatch, ansatz, M, B, split_dual):
+    """S formed in two parts (the first wide panel's columns, then the
+    trailing block on the factorisation's second stream, beside that panel)
+    gives the update of the single product bit for bit.  Cases: the C5
+    complex RBM (P = 4848: ten wide panels), a real RBM with P = 1024 (two
+    wide panels), with and without the materialised O."""
+    a, P, spins, e, Oc = _setup(hip, orc, 2, 4, M, B, 907, ansatz=ansatz)
+    monkeypatch.setenv("NQS_HIP_FACTORED_DUAL", split_dual)
+    print(f"[mixed] Gram split: ansatz {ansatz} P={P} B={B} factored {split_dual}", flush=True)
+    out = {}
+    for sp in ("0", "1", "0"):
+        monkeypatch.setenv("NQS_HIP_GRAM_SPLIT", sp)
+        d, _ = _solve(hip, a, spins, e, 1e-3, MIXED, GRAM_PARAMETER, False)
+        if sp in out:
+            np.testing.assert_array_equal(d, out[sp])
+        out[sp] = d
+    hip.nqs_ansatz_free(a)
+    assert P > 512
+    np.testing.assert_array_equal(out["1"], out["0"])
+    assert np.all(np.isfinite(out["1"])) and np.linalg.norm(out["1"]) > 0
