@@ -485,15 +485,17 @@ long x3_max_elems(long R, long C, long ld, int transpose) {
   return (g > 16384 ? 16384 : g) + 1;
 }
 
-// Block (x, y): samples 64 x + [0, 64), weight rows of the hidden units in
-// slice y of gridDim.y (slice 0 also the bias rows).  The block's spins and
-// factors are staged in LDS sample-minor (row stride 65 floats:
-// conflict-free stores and reads), then wave w takes rows w, w + 4, ...
-// (the parameter decode is wave-uniform) and its 64 lanes the samples, so
-// each row segment is one 256-byte store.  An entry is formed in FP64 (an
-// FP32 factor times a spin, exact), centred there and rounded once;
-// samples past B give zeros.
-constexpr int kOctRows = 256;   // N + M (+ M) staged rows at most
+// Block (x, y): samples 64 x + [0, 64) and the hidden units [hlo, hhi) of
+// slice y of gridDim.y: their bias and weight rows (slice 0 also the
+// visible-bias rows).  The block stages only what those rows read, the
+// spins and its slice's factors, in LDS sample-minor (row stride 65 floats:
+// conflict-free stores and reads; a few KB, so that many blocks share a CU
+// and their stores overlap), then wave w takes rows w, w + 4, ... (the
+// parameter decode is wave-uniform) and its 64 lanes the samples, so each
+// row segment is one 256-byte store.  An entry is formed in FP64 (an FP32
+// factor times a spin, exact), centred there and rounded once; samples
+// past B give zeros.
+constexpr int kOctRows = 256;   // N + M (+ M) at most
 
 __global__ __launch_bounds__(256) void k_oct_factored(const int* __restrict__ spins, int B, int N,
                                                       int M, const float* __restrict__ U1,
@@ -501,27 +503,29 @@ __global__ __launch_bounds__(256) void k_oct_factored(const int* __restrict__ sp
                                                       const double* __restrict__ Om, long P,
                                                       float* __restrict__ out, long ldk,
                                                       float* __restrict__ bmax) {
-  __shared__ float stg[kOctRows][65];
+  // rows [0, N): spins; rows N + z ns + j: factor z of hidden unit hlo + j
+  extern __shared__ float stg[];
   __shared__ float red[256];
   const long b0 = (long)blockIdx.x * 64;
   const int nz = U2 ? 2 : 1;
   const int nvalid = (int)min(64L, max(0L, (long)B - b0));
+  const int hlo = (int)((long)blockIdx.y * M / gridDim.y);
+  const int hhi = (int)((long)(blockIdx.y + 1) * M / gridDim.y);
+  const int ns = hhi - hlo;
   for (int e = threadIdx.x; e < 64 * N; e += 256) {
     const int bl = e / N, i = e - bl * N;
-    stg[i][bl] = bl < nvalid ? (float)spins[(b0 + bl) * N + i] : 0.f;
+    stg[i * 65 + bl] = bl < nvalid ? (float)spins[(b0 + bl) * N + i] : 0.f;
   }
   for (int z = 0; z < nz; z++) {
     const float* U = z ? U2 : U1;
-    for (int e = threadIdx.x; e < 64 * M; e += 256) {
-      const int bl = e / M, h = e - bl * M;
-      stg[N + z * M + h][bl] = bl < nvalid ? U[(b0 + bl) * M + h] : 0.f;
+    for (int e = threadIdx.x; e < 64 * ns; e += 256) {
+      const int bl = e / ns, j = e - bl * ns;
+      stg[(N + z * ns + j) * 65 + bl] = bl < nvalid ? U[(b0 + bl) * M + hlo + j] : 0.f;
     }
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const bool live = lane < nvalid, inpad = b0 + lane < ldk;
-  const int hlo = (int)((long)blockIdx.y * M / gridDim.y);
-  const int hhi = (int)((long)(blockIdx.y + 1) * M / gridDim.y);
   const long Pr = (long)N + M + (long)M * N;
   float mx = 0.f;
   auto put = [&](long k, double o, double om) {
@@ -529,14 +533,20 @@ __global__ __launch_bounds__(256) void k_oct_factored(const int* __restrict__ sp
     if (inpad) out[k * ldk + b0 + lane] = v;
     mx = fmaxf(mx, fabsf(v));
   };
+  const int nvis = blockIdx.y ? 0 : N;
   for (int z = 0; z < nz; z++) {
     const double sg = z ? -1.0 : 1.0;
     const long kz = z * Pr;
-    // visible biases (zero in the second block), hidden biases
-    for (int r = blockIdx.y ? N + M : wave; r < N + M; r += 4) {
-      const double o = r < N ? (z ? 0.0 : (double)stg[r][lane])
-                             : sg * (double)stg[N + z * M + (r - N)][lane];
-      put(kz + r, o, Om[kz + r]);
+    const float* su = stg + (N + z * ns) * 65;
+    // visible biases (slice 0; zero in the second block), then the
+    // slice's hidden biases
+    for (int r = wave; r < nvis + ns; r += 4) {
+      if (r < nvis) {
+        put(kz + r, z ? 0.0 : (double)stg[r * 65 + lane], Om[kz + r]);
+      } else {
+        const int j = r - nvis;
+        put(kz + N + hlo + j, sg * (double)su[j * 65 + lane], Om[kz + N + hlo + j]);
+      }
     }
     // weights, row h*N + i of the block; with N <= 64 the means of the
     // next unit's rows are loaded (lane i: row i) while this unit's are
@@ -546,10 +556,10 @@ __global__ __launch_bounds__(256) void k_oct_factored(const int* __restrict__ sp
     double omv = (pre && hlo < hhi && lane < N) ? Om[kw + (long)hlo * N + lane] : 0.0;
     for (int h = hlo; h < hhi; h++) {
       const double omn = (pre && h + 1 < hhi && lane < N) ? Om[kw + (long)(h + 1) * N + lane] : 0.0;
-      const double u = sg * (double)stg[N + z * M + h][lane];
+      const double u = sg * (double)su[(h - hlo) * 65 + lane];
       const long kh = kw + (long)h * N;
       for (int i = wave; i < N; i += 4)
-        put(kh + i, u * (double)stg[i][lane], pre ? x3_readlane(omv, i) : Om[kh + i]);
+        put(kh + i, u * (double)stg[i * 65 + lane], pre ? x3_readlane(omv, i) : Om[kh + i]);
       omv = omn;
     }
   }
@@ -575,8 +585,14 @@ int launch_oct_factored(const int* spins, int B, int N, int M, const float* U1, 
   if (gy > (P + 63) / 64) gy = (int)((P + 63) / 64);
   const dim3 grid((unsigned)((ldk + 63) / 64), (unsigned)gy);
   const long nb = (long)grid.x * grid.y;
+  // staged rows: the spins and the largest slice's factors
+  const int ns_max = (M + gy - 1) / gy;
+  const size_t lds = sizeof(float) * 65 * (size_t)(N + (U2 ? 2 : 1) * ns_max);
+  if (lds > 65536)   // up to 66.5 KB (256 rows): above the default dynamic limit
+    NQS_HIP_CHECK(hipFuncSetAttribute((const void*)k_oct_factored,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   KScope ks("gen_O", st);
-  hipLaunchKernelGGL(k_oct_factored, grid, dim3(256), 0, st, spins, B, N, M, U1, U2, Om, P, out,
+  hipLaunchKernelGGL(k_oct_factored, grid, dim3(256), lds, st, spins, B, N, M, U1, U2, Om, P, out,
                      ldk, amax + 1);
   hipLaunchKernelGGL(k_absmax_finish, dim3(1), dim3(256), 0, st, amax + 1, nb, amax);
   count_launch();
