@@ -209,17 +209,33 @@ __global__ __launch_bounds__(256) void k_gemm_x3(int M, int N, int K, double alp
       }
 }
 
-// C = alpha sum_s part[s] in FP64 over the split-K partial products
+// C = alpha sum_s part[s] in FP64 over the split-K partial products.
+// Workgroups (p, y) take rows p and M - 1 - p (lower: p + 1 and M - p
+// elements, M + 1 together, so every pair has the same work) as one run of
+// coalesced columns, interleaved over the gridDim.y workgroups of the pair.
 __global__ __launch_bounds__(256) void k_x3_reduce(const float* __restrict__ part, int S, int M,
                                                    int N, double alpha, double* __restrict__ C,
                                                    long ldc, int lower) {
   const long n = (long)M * N;
-  for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
-    const long i = e / N, j = e - i * N;
-    if (lower && j > i) continue;
+  const int p = blockIdx.x, q = M - 1 - p;
+  const int len_p = lower ? p + 1 : N;
+  const int len = len_p + (q > p ? (lower ? q + 1 : N) : 0);
+  for (int t = blockIdx.y * 256 + threadIdx.x; t < len; t += 256 * gridDim.y) {
+    const bool first = t < len_p;
+    const long i = first ? p : q;
+    const long e = i * N + (first ? t : t - len_p);
+    // up to eight slices' loads in flight together; the sum runs over s in
+    // order
     double acc = 0.0;
-    for (int s = 0; s < S; s++) acc += (double)part[(long)s * n + e];
-    C[i * ldc + j] = alpha * acc;
+    for (int s0 = 0; s0 < S; s0 += 8) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = s0 + u < S ? part[(long)(s0 + u) * n + e] : 0.f;
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (s0 + u < S) acc += (double)v[u];
+    }
+    C[i * ldc + (e - i * N)] = alpha * acc;
   }
 }
 
@@ -469,10 +485,8 @@ int launch_gemm_x3(int M, int N, int K, double alpha, const float* A, long lda, 
                      tn, S > 1 ? part : nullptr);
   count_launch();
   if (S > 1) {
-    long g = ((long)M * N + 255) / 256;
-    if (g > 16384) g = 16384;
-    hipLaunchKernelGGL(k_x3_reduce, dim3((unsigned)g), dim3(256), 0, st, part, S, M, N, alpha, C,
-                       ldc, lower);
+    hipLaunchKernelGGL(k_x3_reduce, dim3((unsigned)((M + 1) / 2), 8), dim3(256), 0, st, part, S,
+                       M, N, alpha, C, ldc, lower);
     count_launch();
   }
   NQS_HIP_CHECK(hipGetLastError());
