@@ -165,7 +165,13 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-template <typename R, bool CPLX>
+// STAGED: the transposed weights and hidden biases are first copied into
+// LDS (when they fit, kFwdStage bytes), and the workgroup's waves take
+// several configurations each, so that the site loop reads LDS instead of
+// re-reading the weights through the caches for every configuration.
+constexpr size_t kFwdStage = 48 * 1024;
+
+template <typename R, bool CPLX, bool STAGED>
 __global__ __launch_bounds__(256) void k_forward(ModelView<R> m, const int* __restrict__ spins,
                                                  int B, R* __restrict__ t_re,
                                                  R* __restrict__ t_im, double* la,
@@ -174,18 +180,42 @@ __global__ __launch_bounds__(256) void k_forward(ModelView<R> m, const int* __re
   const int N = m.N, M = m.M;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  extern __shared__ char fwd_lds[];
+  // [N][M] re, [N][M] im, then b_re [M], b_im [M]
+  R* const sw = reinterpret_cast<R*>(fwd_lds);
+  const R* w_re = m.WT_re;
+  const R* w_im = m.WT_im;
+  const R* bb_re = m.b_re;
+  const R* bb_im = m.b_im;
+  if (STAGED) {
+    const int nw = N * M;
+    for (int e = threadIdx.x; e < nw; e += 256) {
+      sw[e] = m.WT_re[e];
+      if (CPLX) sw[nw + e] = m.WT_im[e];
+    }
+    R* const sb = sw + (CPLX ? 2 : 1) * nw;
+    for (int e = threadIdx.x; e < M; e += 256) {
+      sb[e] = m.b_re[e];
+      if (CPLX) sb[M + e] = m.b_im[e];
+    }
+    __syncthreads();
+    w_re = sw;
+    w_im = sw + nw;
+    bb_re = sb;
+    bb_im = sb + M;
+  }
   for (int b = blockIdx.x * 4 + wave; b < B; b += gridDim.x * 4) {
     const int* sp = spins + (size_t)b * N;
     double lsum = 0.0, asum = 0.0;
     for (int h = lane; h < M; h += 64) {
       // theta_h = b_h + sum_i W_hi s_i, accumulated in site order as in
       // nqs_ansatz.c:287-290 (s = +-1 makes every product exact)
-      R xr = m.b_re[h];
-      R xi = CPLX ? m.b_im[h] : (R)0;
+      R xr = bb_re[h];
+      R xi = CPLX ? bb_im[h] : (R)0;
       for (int i = 0; i < N; i++) {
         const R s = (R)sp[i];
-        xr += m.WT_re[(size_t)i * M + h] * s;
-        if (CPLX) xi += m.WT_im[(size_t)i * M + h] * s;
+        xr += w_re[(size_t)i * M + h] * s;
+        if (CPLX) xi += w_im[(size_t)i * M + h] * s;
       }
       if (t_re) {
         if (CPLX) {
@@ -249,14 +279,26 @@ int launch_forward(const ModelView<R>& m, const int* spins, int B, R* t_re, R* t
     fprintf(stderr, "nqs_hip: amplitudes support N <= %d sites (got %d)\n", kMaxSites, m.N);
     return -1;
   }
-  const int grid = (B + 3) / 4 < 65535 ? (B + 3) / 4 : 65535;
+  int grid = (B + 3) / 4 < 65535 ? (B + 3) / 4 : 65535;
   KScope ks("forward", st);
-  if (m.cplx)
-    hipLaunchKernelGGL((k_forward<R, true>), dim3(grid), dim3(256), 0, st, m, spins, B,
+  const size_t stage = sizeof(R) * ((size_t)(m.cplx ? 2 : 1) * ((size_t)m.N * m.M + m.M));
+  if (stage <= kFwdStage) {
+    // eight configurations per wave at most, at least 2048 workgroups
+    const int g2 = (B + 31) / 32 > 2048 ? (B + 31) / 32 : 2048;
+    if (g2 < grid) grid = g2;
+    if (m.cplx)
+      hipLaunchKernelGGL((k_forward<R, true, true>), dim3(grid), dim3(256), stage, st, m, spins,
+                         B, t_re, t_im, la, ar, q_re, q_im);
+    else
+      hipLaunchKernelGGL((k_forward<R, false, true>), dim3(grid), dim3(256), stage, st, m, spins,
+                         B, t_re, t_im, la, ar, q_re, q_im);
+  } else if (m.cplx) {
+    hipLaunchKernelGGL((k_forward<R, true, false>), dim3(grid), dim3(256), 0, st, m, spins, B,
                        t_re, t_im, la, ar, q_re, q_im);
-  else
-    hipLaunchKernelGGL((k_forward<R, false>), dim3(grid), dim3(256), 0, st, m, spins, B,
+  } else {
+    hipLaunchKernelGGL((k_forward<R, false, false>), dim3(grid), dim3(256), 0, st, m, spins, B,
                        t_re, t_im, la, ar, q_re, q_im);
+  }
   count_launch();
   NQS_HIP_CHECK(hipGetLastError());
   return 0;
