@@ -356,8 +356,12 @@ __global__ __launch_bounds__(64) void k_ar_propose(ArDev v, ChainState cs, ArBul
 // b.lq[R*C + c]); d = 2(log|psi(s')| - cached log|psi(s)|) + log q(s) -
 // log q(s'), accept when d >= 0 or log u < d (u from the chain's xorshift).
 // The chain's state after round r is sample r*C + c of b.out when that index
-// is below b.n_out.
-__global__ __launch_bounds__(256) void k_ar_accept_rounds(ChainState cs, ArBulk b) {
+// is below b.n_out.  The proposals of up to kAccPre rounds are loaded before
+// the first decision (the chain of decisions then waits on no load), and
+// the launch uses 64-thread workgroups so that the samples' stores spread
+// over C / 64 CUs.
+constexpr int kAccPre = 8;
+__global__ __launch_bounds__(64) void k_ar_accept_rounds(ChainState cs, ArBulk b) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= cs.C) return;
   const long C = cs.C;
@@ -365,21 +369,35 @@ __global__ __launch_bounds__(256) void k_ar_accept_rounds(ChainState cs, ArBulk 
   uint64_t bits = cs.spins[c], rng = cs.rng[c];
   double la = cs.la[c], lqc = b.lq[(long)b.rounds * C + c];
   unsigned long long nacc = 0;
-  for (int r = 0; r < b.rounds; r++) {
-    const long i = (long)r * C + c;
-    const double d = 2.0 * (b.lp[i] - la) + (lqc - b.lq[i]);
-    // one draw per round whatever d is, so that the chain's stream (which
-    // seeds the next batch's proposals) advances every batch
-    const double u = xs_uniform(rng);
-    const bool acc = d >= 0.0 ? true : (log(u) < d);
-    if (acc) {
-      bits = b.prop[i];
-      la = b.lp[i];
-      lqc = b.lq[i];
-      nacc++;
+  for (int r0 = 0; r0 < b.rounds; r0 += kAccPre) {
+    double plp[kAccPre], plq[kAccPre];
+    uint64_t pb[kAccPre];
+#pragma unroll
+    for (int u = 0; u < kAccPre; u++) {
+      const long i = (long)(r0 + u) * C + c;
+      const bool in = r0 + u < b.rounds;
+      plp[u] = in ? b.lp[i] : 0.0;
+      plq[u] = in ? b.lq[i] : 0.0;
+      pb[u] = in ? b.prop[i] : 0ULL;
     }
-    if (b.out && i < b.n_out)
-      for (int k = 0; k < N; k++) b.out[i * N + k] = ((bits >> k) & 1ULL) ? -1 : 1;
+#pragma unroll
+    for (int u = 0; u < kAccPre; u++) {
+      if (r0 + u >= b.rounds) break;
+      const long i = (long)(r0 + u) * C + c;
+      const double d = 2.0 * (plp[u] - la) + (lqc - plq[u]);
+      // one draw per round whatever d is, so that the chain's stream (which
+      // seeds the next batch's proposals) advances every batch
+      const double uu = xs_uniform(rng);
+      const bool acc = d >= 0.0 ? true : (log(uu) < d);
+      if (acc) {
+        bits = pb[u];
+        la = plp[u];
+        lqc = plq[u];
+        nacc++;
+      }
+      if (b.out && i < b.n_out)
+        for (int k = 0; k < N; k++) b.out[i * N + k] = ((bits >> k) & 1ULL) ? -1 : 1;
+    }
   }
   cs.spins[c] = bits;
   cs.rng[c] = rng;
@@ -762,7 +780,7 @@ int launch_ar_propose(const ArDev& v, ChainState& cs, const ArBulk& b, hipStream
 int launch_ar_accept_rounds(ChainState& cs, const ArBulk& b, hipStream_t st) {
   if (cs.C <= 0 || b.rounds <= 0) return 0;
   KScope ks("ar_accept", st);
-  hipLaunchKernelGGL(k_ar_accept_rounds, dim3((cs.C + 255) / 256), dim3(256), 0, st, cs, b);
+  hipLaunchKernelGGL(k_ar_accept_rounds, dim3((cs.C + 63) / 64), dim3(64), 0, st, cs, b);
   count_launch();
   NQS_HIP_CHECK(hipGetLastError());
   return 0;
