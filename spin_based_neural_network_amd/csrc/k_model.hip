@@ -156,14 +156,10 @@ void free_model(ModelBuffers<R>& mb) {
 
 // --------------------------------------------------------------------------
 // forward pass: one wave per configuration (4 per 256-thread block), lane h
-// owns hidden units h, h + 64, ...; the sums over hidden units are xor
+// owns hidden units h, h + 64, ... (or half a wave per configuration, lane
+// h of the half units h, h + 32, ...); the sums over hidden units are xor
 // butterflies inside the wave, so there is no block barrier.
 // --------------------------------------------------------------------------
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  return v;
-}
 
 // STAGED: the transposed weights and hidden biases are first copied into
 // LDS (when they fit, kFwdStage bytes), and the workgroup's waves take
@@ -171,7 +167,17 @@ __device__ inline double wave_sum(double v) {
 // re-reading the weights through the caches for every configuration.
 constexpr size_t kFwdStage = 48 * 1024;
 
-template <typename R, bool CPLX, bool STAGED>
+// LPC lanes per configuration (64, or 32 with two configurations per wave
+// when the hidden units fill 32-lane passes better); the sums over hidden
+// units are butterflies within the LPC lanes.
+template <int LPC>
+__device__ inline double group_sum(double v) {
+#pragma unroll
+  for (int m = LPC / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+template <typename R, bool CPLX, bool STAGED, int LPC>
 __global__ __launch_bounds__(256) void k_forward(ModelView<R> m, const int* __restrict__ spins,
                                                  int B, R* __restrict__ t_re,
                                                  R* __restrict__ t_im, double* la,
@@ -204,10 +210,12 @@ __global__ __launch_bounds__(256) void k_forward(ModelView<R> m, const int* __re
     bb_re = sb;
     bb_im = sb + M;
   }
-  for (int b = blockIdx.x * 4 + wave; b < B; b += gridDim.x * 4) {
+  constexpr int CPW = 64 / LPC;
+  const int sub = lane / LPC, ll = lane % LPC;
+  for (int b = (blockIdx.x * 4 + wave) * CPW + sub; b < B; b += gridDim.x * 4 * CPW) {
     const int* sp = spins + (size_t)b * N;
     double lsum = 0.0, asum = 0.0;
-    for (int h = lane; h < M; h += 64) {
+    for (int h = ll; h < M; h += LPC) {
       // theta_h = b_h + sum_i W_hi s_i, accumulated in site order as in
       // nqs_ansatz.c:287-290 (s = +-1 makes every product exact)
       R xr = bb_re[h];
@@ -257,13 +265,13 @@ __global__ __launch_bounds__(256) void k_forward(ModelView<R> m, const int* __re
     }
     if (la) {
       // visible term sum_i a_i s_i (nqs_ansatz.c:284-285, 333-336)
-      for (int i = lane; i < N; i += 64) {
+      for (int i = ll; i < N; i += LPC) {
         lsum += (double)m.a_re[i] * (double)sp[i];
         if (CPLX) asum += (double)m.a_im[i] * (double)sp[i];
       }
-      const double L = wave_sum(lsum);
-      const double A = CPLX ? wave_sum(asum) : 0.0;
-      if (lane == 0) {
+      const double L = group_sum<LPC>(lsum);
+      const double A = CPLX ? group_sum<LPC>(asum) : 0.0;
+      if (ll == 0) {
         la[b] = L;
         if (ar) ar[b] = A;
       }
@@ -282,22 +290,33 @@ int launch_forward(const ModelView<R>& m, const int* spins, int B, R* t_re, R* t
   int grid = (B + 3) / 4 < 65535 ? (B + 3) / 4 : 65535;
   KScope ks("forward", st);
   const size_t stage = sizeof(R) * ((size_t)(m.cplx ? 2 : 1) * ((size_t)m.N * m.M + m.M));
+  // FP32: 32 lanes per configuration where that takes fewer lane passes per
+  // configuration (M = 96: three full passes for two configurations instead
+  // of a full and a half-empty one for one).  FP64 keeps one configuration
+  // per wave and with it its order of the sums.
+  const bool half = sizeof(R) == 4 && (m.M + 31) / 32 < 2 * ((m.M + 63) / 64);
   if (stage <= kFwdStage) {
     // eight configurations per wave at most, at least 2048 workgroups
     const int g2 = (B + 31) / 32 > 2048 ? (B + 31) / 32 : 2048;
     if (g2 < grid) grid = g2;
-    if (m.cplx)
-      hipLaunchKernelGGL((k_forward<R, true, true>), dim3(grid), dim3(256), stage, st, m, spins,
-                         B, t_re, t_im, la, ar, q_re, q_im);
+    if (m.cplx && half)
+      hipLaunchKernelGGL((k_forward<R, true, true, 32>), dim3(grid), dim3(256), stage, st, m,
+                         spins, B, t_re, t_im, la, ar, q_re, q_im);
+    else if (m.cplx)
+      hipLaunchKernelGGL((k_forward<R, true, true, 64>), dim3(grid), dim3(256), stage, st, m,
+                         spins, B, t_re, t_im, la, ar, q_re, q_im);
+    else if (half)
+      hipLaunchKernelGGL((k_forward<R, false, true, 32>), dim3(grid), dim3(256), stage, st, m,
+                         spins, B, t_re, t_im, la, ar, q_re, q_im);
     else
-      hipLaunchKernelGGL((k_forward<R, false, true>), dim3(grid), dim3(256), stage, st, m, spins,
-                         B, t_re, t_im, la, ar, q_re, q_im);
+      hipLaunchKernelGGL((k_forward<R, false, true, 64>), dim3(grid), dim3(256), stage, st, m,
+                         spins, B, t_re, t_im, la, ar, q_re, q_im);
   } else if (m.cplx) {
-    hipLaunchKernelGGL((k_forward<R, true, false>), dim3(grid), dim3(256), 0, st, m, spins, B,
+    hipLaunchKernelGGL((k_forward<R, true, false, 64>), dim3(grid), dim3(256), 0, st, m, spins, B,
                        t_re, t_im, la, ar, q_re, q_im);
   } else {
-    hipLaunchKernelGGL((k_forward<R, false, false>), dim3(grid), dim3(256), 0, st, m, spins, B,
-                       t_re, t_im, la, ar, q_re, q_im);
+    hipLaunchKernelGGL((k_forward<R, false, false, 64>), dim3(grid), dim3(256), 0, st, m, spins,
+                       B, t_re, t_im, la, ar, q_re, q_im);
   }
   count_launch();
   NQS_HIP_CHECK(hipGetLastError());
