@@ -261,10 +261,13 @@ nqs::ModelBuffers<double>& model_bufs<double>(nqs_ansatz* a) { return a->m64; }
 template <>
 nqs::ModelBuffers<float>& model_bufs<float>(nqs_ansatz* a) { return a->m32; }
 
+// check = 0: tables prepared for amplitudes only, without waiting for the
+// ratio tables' range verdict (ratio_view reads it when a caller needs them)
 template <typename R>
-int model_view(nqs_ansatz* a, ModelView<R>& v) {
+int model_view(nqs_ansatz* a, ModelView<R>& v, int check = 1) {
   nqs::ModelBuffers<R>& mb = model_bufs<R>(a);
-  if (!mb.valid) CKR(nqs::launch_prep<R>(a->d_par.as<double>(), a->kind, a->N, a->M, mb, g_stream));
+  if (!mb.valid)
+    CKR(nqs::launch_prep<R>(a->d_par.as<double>(), a->kind, a->N, a->M, mb, g_stream, check));
   v = mb.view;
   return 0;
 }
@@ -276,6 +279,13 @@ template <typename R>
 int ratio_view(nqs_ansatz* a, ModelView<R>& v) {
   CKR(model_view<R>(a, v));
   nqs::ModelBuffers<R>& mb = model_bufs<R>(a);
+  if (mb.ratio_ok < 0) {
+    // prepared without the verdict (amplitude-only use): read it now
+    int flag = 0;
+    CK(hipMemcpyAsync(&flag, mb.range_flag, sizeof(int), hipMemcpyDeviceToHost, g_stream));
+    CK(hipStreamSynchronize(g_stream));
+    mb.ratio_ok = flag ? 0 : 1;
+  }
   if (mb.ratio_ok == 0) {
     (void)nqs::report_ratio_range<R>(a->d_par.as<double>(), a->N, a->M, mb.range_flag, g_stream);
     return -1;
@@ -1316,7 +1326,7 @@ int learned_rounds(nqs_sampler* s, int rounds, int n_out, int* d_out) {
   const int C = s->cs.C;
   ModelView<R> mv;
   if (s->opt.ar_local > 0) CKR(ratio_view<R>(s->src.a, mv));   // exchange moves between
-  else CKR(model_view<R>(s->src.a, mv));
+  else CKR(model_view<R>(s->src.a, mv, 0));   // amplitudes only: no wait for the range verdict
   nqs::ArRounds ar;
   ar.constrained = 1;
   ar.lacc = s->lacc.as<unsigned long long>();
