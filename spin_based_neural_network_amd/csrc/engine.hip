@@ -1031,6 +1031,7 @@ struct nqs_sampler {
   nqs_hip_ar* ar = nullptr;
   DBuf win, lacc, prop_bits, prop_spins, lq, la_prop;
   long win_pos = 0, win_n = 0;
+  bool win_written = false;   // the last learned batch wrote its window entries
   int since_train = 0, trained = 0, n_train = 0;
   long long ar_props = 0;
   // CG-SR steps of one real RBM run as a batch of one problem (one graph
@@ -1230,8 +1231,10 @@ int ar_maybe_train(nqs_sampler* s) {
 int ar_window_append(nqs_sampler* s, int n_out, const int* d_out) {
   const long cap = std::max(1, s->opt.ar_train_window);
   const long k = std::min<long>(n_out, cap);
-  CKR(nqs::launch_bits_append(d_out + (size_t)(n_out - k) * s->N, k, s->N, s->win.as<uint64_t>(),
-                              cap, s->win_pos, g_stream));
+  if (!s->win_written)
+    CKR(nqs::launch_bits_append(d_out + (size_t)(n_out - k) * s->N, k, s->N,
+                                s->win.as<uint64_t>(), cap, s->win_pos, g_stream));
+  s->win_written = false;
   s->win_pos = (s->win_pos + k) % cap;
   s->win_n = std::min(cap, s->win_n + k);
   s->since_train++;
@@ -1347,6 +1350,15 @@ int learned_rounds(nqs_sampler* s, int rounds, int n_out, int* d_out) {
     b.n_out = n_out;
     b.out = d_out;
     b.lacc = s->lacc.as<unsigned long long>();
+    // the accept rounds also write the training window's entries (what
+    // ar_window_append would pack from the samples)
+    if (s->win.p && n_out > 0) {
+      b.win = s->win.as<uint64_t>();
+      b.win_cap = std::max(1, s->opt.ar_train_window);
+      b.win_pos = s->win_pos;
+      b.win_first = n_out - std::min<long>(n_out, b.win_cap);
+      s->win_written = true;
+    }
     CKR(nqs::launch_ar_propose(s->ar->dev, s->cs, b, g_stream));
     // log|psi| only (no tanh theta)
     CKR(nqs::launch_forward<R>(mv, b.spins, (int)nprop, nullptr, nullptr, b.lp, nullptr, g_stream));

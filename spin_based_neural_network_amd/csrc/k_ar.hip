@@ -395,8 +395,11 @@ __global__ __launch_bounds__(64) void k_ar_accept_rounds(ChainState cs, ArBulk b
         lqc = plq[u];
         nacc++;
       }
-      if (b.out && i < b.n_out)
+      if (b.out && i < b.n_out) {
         for (int k = 0; k < N; k++) b.out[i * N + k] = ((bits >> k) & 1ULL) ? -1 : 1;
+        if (b.win && i >= b.win_first)
+          b.win[(b.win_pos + i - b.win_first) % b.win_cap] = bits & site_mask(N);
+      }
     }
   }
   cs.spins[c] = bits;

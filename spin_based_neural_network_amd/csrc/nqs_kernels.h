@@ -226,6 +226,10 @@ struct ArBulk {
   long n_out = 0;
   int* out = nullptr;
   unsigned long long* lacc = nullptr;
+  // training window (optional): samples i in [win_first, n_out) also go to
+  // win[(win_pos + i - win_first) % win_cap] as N-bit masks (bit 1: s = -1)
+  uint64_t* win = nullptr;
+  long win_cap = 1, win_pos = 0, win_first = 0;
 };
 int launch_ar_propose(const ArDev& v, ChainState& cs, const ArBulk& b, hipStream_t st);
 int launch_ar_accept_rounds(ChainState& cs, const ArBulk& b, hipStream_t st);
